@@ -1,0 +1,650 @@
+"""bfgpu -- host-side mirror of the ButterflyFactorizations.jl far-field API
+over the gfx950 C-ABI library libbfgpu.so.
+
+Names follow the reference (src/ButterflyFactorizations.jl:69-81):
+``assemble_BF``, ``ButterflyFactorization``, ``PetrovGalerkinBF``,
+``mul`` (``mul!``), ``adjoint``/``transpose``, ``nearandfar``,
+``isFarFunctor``, ``CenterDistanceAdmissibility``,
+``ButterflyRankEstimator``, ``GeometricRankEstimator``, ``tree_parameters``
+and the tree interface ``cluster_*`` of src/treeinterface.jl.  Indices are
+0-based.  Every numerical step runs in libbfgpu on the GPU; there is no CPU
+path.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import ArgumentError, BfgpuError, check, lib, ptr
+
+__all__ = [
+    "Context", "default_context", "ClusterTree", "BlockTree", "cluster_blktree",
+    "HelmholtzKernelMatrix", "ButterflyRankEstimator", "GeometricRankEstimator",
+    "LearnedRankEstimator", "TreeParameters", "tree_parameters", "assemble_BF",
+    "ButterflyFactorization", "PetrovGalerkinBF", "mul", "nearandfar", "isFarFunctor",
+    "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError",
+]
+
+
+# --------------------------------------------------------------------------
+# Context
+# --------------------------------------------------------------------------
+class Context:
+    """A device and the HIP stream all work of this context runs on."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().bfgpu_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr):
+        check(lib().bfgpu_ctx_set_stream(self._h, C.c_void_p(stream_ptr or None)))
+
+    def stream(self):
+        s = C.c_void_p()
+        check(lib().bfgpu_ctx_get_stream(self._h, C.byref(s)))
+        return s.value
+
+    def sync(self):
+        check(lib().bfgpu_ctx_sync(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().bfgpu_ctx_destroy(h)
+            self._h = None
+
+
+_DEFAULT = {}
+
+
+def default_context(device=0):
+    if device not in _DEFAULT:
+        _DEFAULT[device] = Context(device)
+    return _DEFAULT[device]
+
+
+# --------------------------------------------------------------------------
+# Trees (src/treeinterface.jl:7-164)
+# --------------------------------------------------------------------------
+class ClusterTree:
+    """Box tree over points (quadtree with dims=2, octree with dims=3).
+
+    ``mode='balanced'`` subdivides every node to depth
+    ceil(log_{2^dims}(N / leaf_size)); ``mode='maxpoints'`` subdivides while a
+    node holds more than ``leaf_size`` points (unbalanced trees).  Radii follow
+    the TwoNTree convention halfsize * sqrt(3) (ext/BFH2Trees/BFH2Trees.jl:15-16).
+    """
+
+    def __init__(self, points, dims=3, leaf_size=64, mode="balanced", _handle=None):
+        if _handle is None:
+            pts = np.ascontiguousarray(points, dtype=np.float64)
+            if pts.ndim != 2 or pts.shape[1] != 3:
+                raise ArgumentError(1, "points must have shape (N, 3)")
+            m = {"balanced": L.TREE_BALANCED, "maxpoints": L.TREE_MAXPOINTS}[mode]
+            h = C.c_void_p()
+            check(lib().bfgpu_tree_build(ptr(pts, C.c_double), pts.shape[0], int(dims),
+                                         int(leaf_size), m, C.byref(h)))
+            _handle = h
+        self._h = _handle
+        v = L.TreeView()
+        check(lib().bfgpu_tree_view_get(self._h, C.byref(v)))
+        nn, nv = v.n_nodes, v.n_values
+        cp = lambda p, n, dt: np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+        self.n_nodes, self.n_values = nn, nv
+        self.parent = cp(v.parent, nn, np.int32)
+        self.first_child = cp(v.first_child, nn, np.int32)
+        self.n_children = cp(v.n_children, nn, np.int32)
+        self.val_begin = cp(v.val_begin, nn, np.int32)
+        self.val_end = cp(v.val_end, nn, np.int32)
+        self.center = cp(v.center, 3 * nn, np.float64).reshape(nn, 3)
+        self.halfsize = cp(v.halfsize, nn, np.float64)
+        self.perm = cp(v.perm, nv, np.int32)
+        self.level = cp(v.level, nn, np.int32)
+        self.points = cp(v.points, 3 * nv, np.float64).reshape(nv, 3)
+
+    @classmethod
+    def from_arrays(cls, points, parent, first_child, n_children, val_begin, val_end,
+                    center, halfsize, perm):
+        """Adopt a tree built elsewhere (for example exported from H2Trees)."""
+        a = lambda x, dt: np.ascontiguousarray(x, dtype=dt)
+        pts = a(points, np.float64)
+        arrs = [a(parent, np.int32), a(first_child, np.int32), a(n_children, np.int32),
+                a(val_begin, np.int32), a(val_end, np.int32)]
+        cen, hs, pm = a(center, np.float64), a(halfsize, np.float64), a(perm, np.int32)
+        h = C.c_void_p()
+        check(lib().bfgpu_tree_create(ptr(pts, C.c_double), pts.shape[0], arrs[0].shape[0],
+                                      *[ptr(x, C.c_int32) for x in arrs], ptr(cen, C.c_double),
+                                      ptr(hs, C.c_double), ptr(pm, C.c_int32), C.byref(h)))
+        return cls(None, _handle=h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    # tree interface
+    def cluster_root(self):
+        return 0
+
+    def cluster_center(self, node):
+        return self.center[node]
+
+    def cluster_radius(self, node):
+        return float(self.halfsize[node] * np.sqrt(3.0))
+
+    def cluster_children(self, node):
+        f = int(self.first_child[node])
+        return list(range(f, f + int(self.n_children[node])))
+
+    def cluster_isleaf(self, node):
+        return int(self.n_children[node]) == 0
+
+    def cluster_parent(self, node):
+        return int(self.parent[node])
+
+    def cluster_values(self, node):
+        return self.perm[self.val_begin[node]:self.val_end[node]]
+
+    def cluster_levels(self):
+        levels = []
+        for lv in range(int(self.level.max()) + 1):
+            levels.append(np.nonzero(self.level == lv)[0].tolist())
+        return levels
+
+    def leaves(self):
+        return np.nonzero(self.n_children == 0)[0]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().bfgpu_tree_destroy(h)
+            self._h = None
+
+
+class BlockTree:
+    """Observer (test) tree and source (trial) tree of one operator."""
+
+    def __init__(self, testcluster, trialcluster):
+        self.testcluster = testcluster
+        self.trialcluster = trialcluster
+
+    def cluster_testtree(self):
+        return self.testcluster
+
+    def cluster_trialtree(self):
+        return self.trialcluster
+
+
+def cluster_blktree(testtree, trialtree):
+    return BlockTree(testtree, trialtree)
+
+
+# --------------------------------------------------------------------------
+# Kernel plug-in and rank estimators
+# --------------------------------------------------------------------------
+@dataclass
+class HelmholtzKernelMatrix:
+    """G(x, y) = exp(-i k |x - y|) / (4 pi |x - y|) between observer points
+    (test tree) and source points (trial tree); the AbstractKernelMatrix
+    plug-in (src/kernelmatrix/abstractkernelmatrix.jl:1-7), evaluated on the
+    GPU during assembly."""
+    k: float
+
+
+@dataclass
+class ButterflyRankEstimator:
+    """rankestimation.jl:197-205: n = max(ceil(Ceps * ln(1/eps)), Rmin)."""
+    Ceps: float = 4.0
+    Rmin: int = 10
+
+
+@dataclass
+class GeometricRankEstimator:
+    """rankestimation.jl:178-189 with estimate_rank_3d on trees."""
+    C: float = 1.5
+    Ceps: float = 1.5
+    Rmin: int = 3
+
+
+@dataclass
+class LearnedRankEstimator:
+    """Rank predicted by a trained RankPredictor plus `oversample` rows."""
+    predictor: "RankPredictor"
+    oversample: int = 10
+
+
+@dataclass
+class TreeParameters:
+    alpha: float
+    C: float
+    Ceps: float
+    beta: float
+    Ctau: float
+
+
+@dataclass
+class isFarFunctor:
+    """AABB-gap admissibility (src/Matrixassembly/farfieldcrit.jl:12-14)."""
+    alpha: float
+
+
+@dataclass
+class CenterDistanceAdmissibility:
+    """Centre-distance admissibility (farfieldcrit.jl:27-29)."""
+    beta: float
+
+
+def tree_parameters(tree=None, admissibility=None):
+    """Calibrated constants of the box tree (the TwoNTree values of
+    ext/BFH2Trees/h2parameters.jl:18-24, :98-102, :127-132)."""
+    if isinstance(admissibility, isFarFunctor):
+        return TreeParameters(1.0, 2.0, 2.0, 1.0, 2.9592)
+    if isinstance(admissibility, CenterDistanceAdmissibility):
+        return TreeParameters(1.0, 2.0, 2.0, 1.0, 2.9352)
+    return TreeParameters(1.0, 2.0, 2.0, 1.0, 4.0)
+
+
+def _opts(k, tol, rankestimator, adaptive, acctype, seed):
+    o = L.Opts()
+    o.k = float(k)
+    o.tol = float(tol)
+    o.adaptive = 1 if adaptive else 0
+    o.c_tau, o.rmin, o.c_geo, o.c_eps = 4.0, 10, 1.5, 1.5
+    o.oversample = 10
+    o.mlp = None
+    est = rankestimator or ButterflyRankEstimator()
+    if isinstance(est, ButterflyRankEstimator):
+        o.rank_mode, o.c_tau, o.rmin = L.RANK_BUTTERFLY, float(est.Ceps), int(est.Rmin)
+    elif isinstance(est, GeometricRankEstimator):
+        o.rank_mode, o.c_geo, o.c_eps, o.rmin = L.RANK_GEOMETRIC, float(est.C), float(est.Ceps), int(est.Rmin)
+    elif isinstance(est, LearnedRankEstimator):
+        o.rank_mode, o.mlp, o.oversample = L.RANK_LEARNED, est.predictor.handle, int(est.oversample)
+        o.rmin = 1
+    else:
+        raise ArgumentError(1, f"unknown rank estimator {type(est).__name__}")
+    dt = np.dtype(acctype)
+    if dt == np.complex128:
+        o.precision = L.BFGPU_C128
+    elif dt == np.complex64:
+        o.precision = L.BFGPU_C64
+    else:
+        raise ArgumentError(1, "acctype must be complex64 or complex128")
+    o.seed = int(seed)
+    return o
+
+
+# --------------------------------------------------------------------------
+# Factorizations
+# --------------------------------------------------------------------------
+_OPNAME = {L.OP_N: "N", L.OP_T: "T", L.OP_C: "C"}
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class _Handle:
+    """Owns one bfgpu_bf handle (shared by adjoint/transpose views)."""
+
+    def __init__(self, h, ctx, keep):
+        self.h, self.ctx, self.keep = h, ctx, keep
+
+    def __del__(self):
+        if self.h is not None and self.h.value:
+            lib().bfgpu_bf_destroy(self.h)
+            self.h = None
+
+
+class ButterflyFactorization:
+    """One (or a batch of) butterfly factorization(s) on the GPU
+    (src/ButterflyFactorization/bfstructs.jl:91-147).  ``adjoint()`` and
+    ``transpose()`` return views that apply the same device factors with
+    conjugated / transposed access (bfadjtr.jl:43-89) without copying."""
+
+    def __init__(self, handle, ctx, tree, k, tau, op=L.OP_N, _keep=None):
+        self._H = handle if isinstance(handle, _Handle) else _Handle(handle, ctx, _keep)
+        self.ctx = ctx
+        self.tree = tree
+        self.k = k
+        self.tau = tau
+        self.op = op
+
+    @property
+    def handle(self):
+        return self._H.h
+
+    def info(self):
+        i = L.BFInfo()
+        check(lib().bfgpu_bf_info_get(self.handle, C.byref(i)))
+        return i
+
+    @property
+    def eltype(self):
+        return np.complex128 if self.info().precision == L.BFGPU_C128 else np.complex64
+
+    @property
+    def shape(self):
+        i = self.info()
+        return (i.rows, i.cols) if self.op == L.OP_N else (i.cols, i.rows)
+
+    def size(self, dim=None):
+        s = self.shape
+        return s if dim is None else s[dim]
+
+    def __len__(self):
+        return self.info().max_stages
+
+    # ---- application ----
+    def mul(self, y, x, alpha=1.0, beta=0.0):
+        """y = alpha * op(A) x + beta * y  (mul!, bfvector.jl:200-296)."""
+        rows, cols = self.shape
+        prec = self.info().precision
+        want = np.complex128 if prec == L.BFGPU_C128 else np.complex64
+        a = (C.c_double * 2)(float(np.real(alpha)), float(np.imag(alpha)))
+        b = (C.c_double * 2)(float(np.real(beta)), float(np.imag(beta)))
+        if _is_torch(x) or _is_torch(y):
+            import torch
+            if not (_is_torch(x) and _is_torch(y)):
+                raise ArgumentError(1, "x and y must both be torch tensors or both numpy arrays")
+            tw = torch.complex128 if prec == L.BFGPU_C128 else torch.complex64
+            if x.dtype != tw or y.dtype != tw:
+                raise ArgumentError(1, f"x and y must be {tw}")
+            if not (x.is_cuda and y.is_cuda):
+                raise ArgumentError(1, "torch tensors must live on the GPU")
+            nrhs, ldx = self._layout(x.dim(), tuple(x.shape), tuple(x.stride()), 1, cols, "x")
+            nrhs2, ldy = self._layout(y.dim(), tuple(y.shape), tuple(y.stride()), 1, rows, "y")
+            if nrhs != nrhs2:
+                raise ArgumentError(1, "x and y have different numbers of columns")
+            self.ctx.set_stream(torch.cuda.current_stream(x.device).cuda_stream)
+            check(lib().bfgpu_matvec(self.handle, self.op, nrhs, C.c_void_p(x.data_ptr()), ldx,
+                                     C.c_void_p(y.data_ptr()), ldy, a, b, L.XY_DEVICE))
+            return y
+        if not (isinstance(x, np.ndarray) and isinstance(y, np.ndarray)):
+            raise ArgumentError(1, "x and y must be numpy arrays or torch tensors")
+        if x.dtype != want or y.dtype != want:
+            raise ArgumentError(1, f"x and y must be {np.dtype(want)}")
+        es = np.dtype(want).itemsize
+        nrhs, ldx = self._layout(x.ndim, x.shape, tuple(s // es for s in x.strides), 1, cols, "x")
+        nrhs2, ldy = self._layout(y.ndim, y.shape, tuple(s // es for s in y.strides), 1, rows, "y")
+        if nrhs != nrhs2:
+            raise ArgumentError(1, "x and y have different numbers of columns")
+        check(lib().bfgpu_matvec(self.handle, self.op, nrhs, C.c_void_p(x.ctypes.data), ldx,
+                                 C.c_void_p(y.ctypes.data), ldy, a, b, L.XY_HOST))
+        return y
+
+    @staticmethod
+    def _layout(ndim, shape, strides, unit, n, name):
+        if ndim == 1:
+            if shape[0] != n:
+                raise ArgumentError(1, f"DimensionMismatch: {name} has length {shape[0]}, expected {n}")
+            if strides[0] != unit:
+                raise ArgumentError(1, f"{name} must be contiguous")
+            return 1, n
+        if ndim == 2:
+            if shape[0] != n:
+                raise ArgumentError(1, f"DimensionMismatch: {name} has {shape[0]} rows, expected {n}")
+            if strides[0] != unit and shape[1] > 0:
+                raise ArgumentError(1, f"{name} must be column-major (Fortran order)")
+            return shape[1], max(strides[1], n) if shape[1] > 1 else n
+        raise ArgumentError(1, f"{name} must be 1-D or 2-D")
+
+    def __matmul__(self, x):
+        """Allocating product (Base.:*, bfvector.jl:308-314; bfmatrix.jl:59-75)."""
+        rows = self.shape[0]
+        if _is_torch(x):
+            import torch
+            shape = (rows,) if x.dim() == 1 else (x.shape[1], rows)
+            y = torch.zeros(shape, dtype=x.dtype, device=x.device)
+            if x.dim() == 2:
+                y = y.t()
+            return self.mul(y, x)
+        shape = (rows,) if x.ndim == 1 else (rows, x.shape[1])
+        y = np.zeros(shape, dtype=x.dtype, order="F")
+        return self.mul(y, x)
+
+    __mul__ = __matmul__
+
+    def adjoint(self):
+        if self.op == L.OP_T:
+            self._conj_view_error()
+        return ButterflyFactorization(self._H, self.ctx, self.tree, self.k, self.tau,
+                                      L.OP_C if self.op == L.OP_N else L.OP_N)
+
+    def transpose(self):
+        if self.op == L.OP_C:
+            self._conj_view_error()
+        return ButterflyFactorization(self._H, self.ctx, self.tree, self.k, self.tau,
+                                      L.OP_T if self.op == L.OP_N else L.OP_N)
+
+    @staticmethod
+    def _conj_view_error():
+        raise ArgumentError(1, "conj(A) views are not supported; use adjoint() or transpose() of A")
+
+    @property
+    def H(self):
+        return self.adjoint()
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    # ---- introspection / export ----
+    def n_factorizations(self):
+        return self.info().n_bf
+
+    def export(self, index=0):
+        """Blocks of factorization `index` in the reference layout."""
+        ns, nb, nd, nsk, nski, nc = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        roots = (C.c_int32 * 2)()
+        check(lib().bfgpu_bf_counts(self.handle, index, C.byref(ns), C.byref(nb), C.byref(nd),
+                                    C.byref(nsk), C.byref(nski), C.byref(nc), roots))
+        table = np.zeros((nb.value, 10), np.int32)
+        ofs = np.zeros(nb.value, np.int64)
+        data = np.zeros(max(nd.value, 1), np.complex128)
+        check(lib().bfgpu_bf_export(self.handle, index, ptr(table, C.c_int32), ptr(ofs, C.c_int64),
+                                    data.ctypes.data_as(L.P(C.c_double))))
+        skel = np.zeros((nsk.value, 5), np.int32)
+        skel_idx = np.zeros(max(nski.value, 1), np.int32)
+        comp = np.zeros((nc.value, 6), np.int32)
+        check(lib().bfgpu_bf_export_logs(self.handle, index, ptr(skel, C.c_int32),
+                                         ptr(skel_idx, C.c_int32), ptr(comp, C.c_int32)))
+        return {"n_stages": ns.value, "table": table, "data_ofs": ofs, "data": data[:nd.value],
+                "skel": skel, "skel_idx": skel_idx[:nski.value], "comp": comp,
+                "roots": (roots[0], roots[1])}
+
+    def stage_stats(self):
+        """Per launch stage: blocks, identity blocks, max rank, factor bytes
+        (the per-level table of src/showfcts.jl:81-124)."""
+        n = C.c_int32()
+        check(lib().bfgpu_bf_stage_stats(self.handle, C.byref(n), None))
+        st = np.zeros((n.value, 4), np.int64)
+        check(lib().bfgpu_bf_stage_stats(self.handle, C.byref(n), ptr(st, C.c_int64)))
+        return st
+
+    def convert(self, acctype):
+        """Copy with factor storage in another precision."""
+        prec = L.BFGPU_C128 if np.dtype(acctype) == np.complex128 else L.BFGPU_C64
+        h = C.c_void_p()
+        check(lib().bfgpu_bf_convert(self.handle, prec, C.byref(h)))
+        return ButterflyFactorization(h, self.ctx, self.tree, self.k, self.tau, self.op, _keep=self._H.keep)
+
+    def timed(self, x, y, iters=10, nrhs=1):
+        """Mean per-stage and total milliseconds of `iters` device matvecs
+        (HIP events on the context stream) and algorithmic bytes per stage."""
+        rows, cols = self.shape
+        nst = self.info().max_stages
+        ms = (C.c_double * nst)()
+        by = (C.c_int64 * nst)()
+        n = C.c_int32()
+        tot = C.c_double()
+        ldx = cols if self.op == L.OP_N else rows
+        ldy = rows if self.op == L.OP_N else cols
+        check(lib().bfgpu_matvec_timed(self.handle, self.op, nrhs, C.c_void_p(x.data_ptr()), ldx,
+                                       C.c_void_p(y.data_ptr()), ldy, int(iters), ms, by,
+                                       C.byref(n), C.byref(tot)))
+        return list(ms)[: n.value], list(by)[: n.value], tot.value
+
+    @classmethod
+    def from_blocks(cls, blktree, n_stages, table, data_ofs, data, acctype=np.complex128,
+                    ctx=None, k=0.0, tau=0.0):
+        """Upload a factorization given in the reference block layout."""
+        ctx = ctx or default_context()
+        t = np.ascontiguousarray(table, np.int32)
+        o = np.ascontiguousarray(data_ofs, np.int64)
+        d = np.ascontiguousarray(data, np.complex128)
+        prec = L.BFGPU_C128 if np.dtype(acctype) == np.complex128 else L.BFGPU_C64
+        h = C.c_void_p()
+        check(lib().bfgpu_bf_from_host(ctx.handle, blktree.testcluster.handle,
+                                       blktree.trialcluster.handle, int(n_stages), t.shape[0],
+                                       ptr(t, C.c_int32), ptr(o, C.c_int64),
+                                       d.ctypes.data_as(L.P(C.c_double)), prec, C.byref(h)))
+        return cls(h, ctx, blktree, k, tau, _keep=(blktree,))
+
+
+def assemble_BF(kernelmatrix, blktree, no, ns, k, tau, *, rankestimator=None, adaptive=True,
+                acctype=np.complex128, seed=0, ctx=None):
+    """Butterfly factorization of the block (observer node `no`, source node
+    `ns`) of `kernelmatrix` (bfassembly.jl:48-200), computed on the GPU."""
+    if not isinstance(kernelmatrix, HelmholtzKernelMatrix):
+        raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix")
+    if abs(kernelmatrix.k - k) > 1e-12 * abs(k):
+        raise ArgumentError(1, "wavenumber differs from the kernel's")
+    ctx = ctx or default_context()
+    if rankestimator is None:
+        rankestimator = ButterflyRankEstimator(tree_parameters(blktree).Ctau)
+    o = _opts(k, tau, rankestimator, adaptive, acctype, seed)
+    pairs = np.array([no, ns], np.int32)
+    h = C.c_void_p()
+    check(lib().bfgpu_factorize(ctx.handle, blktree.testcluster.handle, blktree.trialcluster.handle,
+                                1, ptr(pairs, C.c_int32), C.byref(o), C.byref(h)))
+    return ButterflyFactorization(h, ctx, blktree, k, tau, _keep=(blktree, rankestimator))
+
+
+def mul(y, A, x, alpha=1.0, beta=0.0):
+    """mul!(y, A, x, alpha, beta)."""
+    return A.mul(y, x, alpha, beta)
+
+
+def nearandfar(tree, admissibility, minbflvl=3, unbalancedints=False, leafcomp=True,
+               leafimbalance=True):
+    """(far, near) lists of (observer node, source node) pairs
+    (src/Matrixassembly/intlists.jl:30-59)."""
+    if isinstance(admissibility, (int, float)):
+        admissibility = isFarFunctor(float(admissibility))
+    if isinstance(admissibility, isFarFunctor):
+        kind, p = 0, admissibility.alpha
+    elif isinstance(admissibility, CenterDistanceAdmissibility):
+        kind, p = 1, admissibility.beta
+    else:
+        raise ArgumentError(1, "unknown admissibility")
+    te, tr = tree.testcluster.handle, tree.trialcluster.handle
+    nf, nn = C.c_int64(), C.c_int64()
+    args = (te, tr, kind, float(p), int(minbflvl), int(bool(unbalancedints)), int(bool(leafcomp)),
+            int(bool(leafimbalance)))
+    check(lib().bfgpu_nearandfar(*args, None, C.byref(nf), None, C.byref(nn)))
+    far = np.zeros((nf.value, 2), np.int32)
+    near = np.zeros((nn.value, 2), np.int32)
+    check(lib().bfgpu_nearandfar(*args, ptr(far, C.c_int32), C.byref(nf), ptr(near, C.c_int32),
+                                 C.byref(nn)))
+    return far, near
+
+
+class PetrovGalerkinBF(ButterflyFactorization):
+    """Far-field part of the global operator (petrovgalerkinbf.jl:33-161):
+    one butterfly factorization per admissible block, all factorized and
+    applied level by level in batched launches.  The near field is outside
+    this library (farfieldonly=True is required)."""
+
+    def __init__(self, kernelmatrix, tree, k, *, tol=1e-3, admissibility=None, rankestimator=None,
+                 acctype=np.complex128, minbflvl=3, adaptive=True, unbalancedints=False,
+                 leafcomp=True, leafimbalance=True, farfieldonly=True, seed=0, ctx=None):
+        if not farfieldonly:
+            raise NotImplementedError("near-field assembly is not part of this library; "
+                                      "pass farfieldonly=True")
+        ctx = ctx or default_context()
+        admissibility = admissibility or isFarFunctor(tree_parameters(tree.testcluster).alpha)
+        if rankestimator is None:
+            rankestimator = ButterflyRankEstimator(tree_parameters(tree, admissibility).Ctau)
+        far, near = nearandfar(tree, admissibility, minbflvl, unbalancedints, leafcomp, leafimbalance)
+        if far.shape[0] == 0:
+            raise ArgumentError(1, "no admissible (far) blocks")
+        self.farinteractions, self.nearinteractions = far, near
+        o = _opts(k, tol, rankestimator, adaptive, acctype, seed)
+        pairs = np.ascontiguousarray(far, np.int32)
+        h = C.c_void_p()
+        check(lib().bfgpu_factorize(ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,
+                                    pairs.shape[0], ptr(pairs, C.c_int32), C.byref(o), C.byref(h)))
+        super().__init__(h, ctx, tree, k, tol, _keep=(tree, rankestimator))
+
+
+class RankPredictor:
+    """MLP rank predictor (6 features -> hidden^(layers-1) -> 1, ReLU, FP32),
+    trained and evaluated by the library's own GPU kernels."""
+
+    def __init__(self, n_in=6, hidden=128, n_layers=4, seed=0, ctx=None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(lib().bfgpu_mlp_create(self.ctx.handle, n_in, hidden, n_layers, int(seed), C.byref(h)))
+        self._h = h
+        self.n_in = n_in
+
+    @property
+    def handle(self):
+        return self._h
+
+    def n_params(self):
+        n = C.c_int64()
+        check(lib().bfgpu_mlp_n_params(self._h, C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.zeros(self.n_params(), np.float32)
+        check(lib().bfgpu_mlp_get_params(self._h, ptr(p, C.c_float)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params():
+            raise ArgumentError(1, "wrong parameter count")
+        check(lib().bfgpu_mlp_set_params(self._h, ptr(p, C.c_float)))
+
+    def set_norm(self, mu, sigma, out_shift=0.0, out_scale=1.0):
+        mu = np.ascontiguousarray(mu, np.float32)
+        sigma = np.ascontiguousarray(sigma, np.float32)
+        check(lib().bfgpu_mlp_set_norm(self._h, ptr(mu, C.c_float), ptr(sigma, C.c_float),
+                                       float(out_shift), float(out_scale)))
+
+    def forward(self, features):
+        f = np.ascontiguousarray(features, np.float32)
+        out = np.zeros(f.shape[0], np.float32)
+        check(lib().bfgpu_mlp_forward(self._h, f.shape[0], ptr(f, C.c_float), ptr(out, C.c_float)))
+        return out
+
+    def loss_grad(self, features, target):
+        f = np.ascontiguousarray(features, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(self.n_params(), np.float32)
+        loss = C.c_float()
+        check(lib().bfgpu_mlp_loss_grad(self._h, f.shape[0], ptr(f, C.c_float), ptr(t, C.c_float),
+                                        C.byref(loss), ptr(g, C.c_float)))
+        return loss.value, g
+
+    def train(self, features, target, epochs=50, batch=512, lr=1e-3, seed=0):
+        f = np.ascontiguousarray(features, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_float()
+        check(lib().bfgpu_mlp_train(self._h, f.shape[0], ptr(f, C.c_float), ptr(t, C.c_float),
+                                    int(epochs), int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().bfgpu_mlp_destroy(h)
+            self._h = None

@@ -1,0 +1,434 @@
+// api.hip -- C ABI entry points (include/bfgpu.h): error strings, contexts,
+// factorization handles, export, conversion and matvec.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "internal.hpp"
+
+namespace bfg {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &m) { g_last_error = m; }
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+void DevBuf::alloc(size_t n) {
+    release();
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw Error(BFGPU_ERR_NOMEM, "hipMalloc(" + std::to_string(n) + "): " + hipGetErrorString(e));
+    }
+    bytes = n;
+}
+void DevBuf::reserve(size_t n) {
+    if (n <= bytes && p) return;
+    alloc(std::max(n, bytes + bytes / 2));
+}
+
+void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int32_t, int32_t>> &pairs);
+void matvec(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+            const double *alpha, const double *beta, int flags);
+void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
+                  int64_t ldy, int32_t iters, double *stage_ms, int64_t *stage_bytes,
+                  int32_t *n_stages, double *total_ms);
+}  // namespace bfg
+
+using namespace bfg;
+
+
+#define BFG_API_BEGIN try {
+#define BFG_API_END                                  \
+    }                                                \
+    catch (const Error &e) {                         \
+        set_last_error(e.what());                    \
+        return e.code;                               \
+    }                                                \
+    catch (const std::bad_alloc &) {                 \
+        set_last_error("host allocation failed");    \
+        return BFGPU_ERR_NOMEM;                      \
+    }                                                \
+    catch (const std::exception &e) {                \
+        set_last_error(e.what());                    \
+        return BFGPU_ERR_STATE;                      \
+    }                                                \
+    return BFGPU_OK;
+
+extern "C" const char *bfgpu_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char *bfgpu_version(void) { return "bfgpu 0.1 (gfx950)"; }
+
+extern "C" int bfgpu_ctx_create(int device, bfgpu_ctx **out) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(out, "ctx_create: null out");
+    int n = 0;
+    BFG_HIP(hipGetDeviceCount(&n));
+    BFG_REQUIRE(device >= 0 && device < n, "ctx_create: no such device");
+    BFG_HIP(hipSetDevice(device));
+    auto *h = new bfgpu_ctx{};
+    h->c.device = device;
+    hipError_t e = hipStreamCreateWithFlags(&h->c.own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        throw Error(BFGPU_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    h->c.stream = h->c.own;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->c.n_cu = prop.multiProcessorCount;
+    *out = h;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_ctx_destroy(bfgpu_ctx *ctx) {
+    if (!ctx) return BFGPU_OK;
+    if (ctx->c.own) (void)hipStreamDestroy(ctx->c.own);
+    delete ctx;
+    return BFGPU_OK;
+}
+
+extern "C" int bfgpu_ctx_set_stream(bfgpu_ctx *ctx, void *s) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(ctx, "ctx_set_stream: null ctx");
+    ctx->c.stream = s ? static_cast<hipStream_t>(s) : ctx->c.own;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_ctx_get_stream(bfgpu_ctx *ctx, void **s) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(ctx && s, "ctx_get_stream: null argument");
+    *s = ctx->c.stream;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_ctx_sync(bfgpu_ctx *ctx) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(ctx, "ctx_sync: null ctx");
+    BFG_HIP(hipStreamSynchronize(ctx->c.stream));
+    BFG_API_END
+}
+
+static void check_opts(const bfgpu_opts *o) {
+    BFG_REQUIRE(o, "factorize: null opts");
+    BFG_REQUIRE(o->precision == BFGPU_C64 || o->precision == BFGPU_C128, "factorize: precision must be 1 or 2");
+    BFG_REQUIRE(o->tol > 0.0 && o->tol < 1.0, "factorize: tol must lie in (0, 1)");
+    BFG_REQUIRE(o->k > 0.0, "factorize: wavenumber must be positive");
+    BFG_REQUIRE(o->rank_mode >= 0 && o->rank_mode <= 2, "factorize: unknown rank_mode");
+}
+
+extern "C" int bfgpu_factorize(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
+                               int32_t n_pairs, const int32_t *pairs, const bfgpu_opts *opts,
+                               bfgpu_bf **out) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(ctx && test && trial && pairs && out, "factorize: null argument");
+    BFG_REQUIRE(n_pairs >= 1, "factorize: need at least one block");
+    check_opts(opts);
+    BFG_HIP(hipSetDevice(ctx->c.device));
+    std::vector<std::pair<int32_t, int32_t>> pr;
+    for (int32_t i = 0; i < n_pairs; ++i) pr.emplace_back(pairs[2 * i], pairs[2 * i + 1]);
+    auto *h = new bfgpu_bf{};
+    h->b.ctx = &ctx->c;
+    h->b.test = &test->t;
+    h->b.trial = &trial->t;
+    h->b.precision = opts->precision;
+    try {
+        factorize_all(h->b, *opts, pr);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_from_host(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
+                                  int32_t n_stages, int64_t n_blocks, const int32_t *table,
+                                  const int64_t *data_ofs, const double *data, int32_t precision,
+                                  bfgpu_bf **out) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(ctx && test && trial && table && data_ofs && out, "bf_from_host: null argument");
+    BFG_REQUIRE(n_stages >= 2, "bf_from_host: need at least Q and P stages");
+    BFG_REQUIRE(precision == BFGPU_C64 || precision == BFGPU_C128, "bf_from_host: bad precision");
+    BFG_HIP(hipSetDevice(ctx->c.device));
+    auto *h = new bfgpu_bf{};
+    try {
+        BF &bf = h->b;
+        bf.ctx = &ctx->c;
+        bf.test = &test->t;
+        bf.trial = &trial->t;
+        bf.precision = precision;
+        bf.bfs.resize(1);
+        OneBF &ob = bf.bfs[0];
+        ob.st.resize(n_stages);
+        bf.n_stage_launch = n_stages;
+        std::vector<std::vector<double>> hd(n_stages);
+        std::vector<float> hf;
+        for (int64_t q = 0; q < n_blocks; ++q) {
+            const int32_t *r = table + 10 * q;
+            int32_t s = r[0];
+            BFG_REQUIRE(s >= 0 && s < n_stages, "bf_from_host: stage out of range");
+            BFG_REQUIRE(r[5] >= 0 && r[6] >= 0, "bf_from_host: negative block size");
+            Block b{r[1], r[2], r[3], r[4], r[5], r[6], r[7] ? 1 : 0, -1, -1, -1};
+            const Tree &ot = (s == n_stages - 1) ? test->t : test->t;
+            (void)ot;
+            if (!b.ident) {
+                BFG_REQUIRE(data && data_ofs[q] >= 0, "bf_from_host: missing block data");
+                b.fofs = int64_t(hd[s].size() / 2);
+                const double *src = data + 2 * data_ofs[q];
+                hd[s].insert(hd[s].end(), src, src + 2 * int64_t(b.m) * b.n);
+            } else {
+                BFG_REQUIRE(b.m == b.n, "bf_from_host: identity block must be square");
+            }
+            ob.st[s].b.push_back(b);
+        }
+        BFG_REQUIRE(!ob.st[0].b.empty() && !ob.st[n_stages - 1].b.empty(), "bf_from_host: empty Q or P stage");
+        ob.no = ob.st[0].b[0].obs_out;
+        ob.ns = ob.st[n_stages - 1].b[0].src_out;
+        for (const Block &b : ob.st[0].b)
+            BFG_REQUIRE(b.src_in >= 0 && b.src_in < trial->t.n_nodes, "bf_from_host: Q source node out of range");
+        for (const Block &b : ob.st[n_stages - 1].b) {
+            BFG_REQUIRE(b.obs_out >= 0 && b.obs_out < test->t.n_nodes, "bf_from_host: P observer node out of range");
+            BFG_REQUIRE(b.ident || b.m == test->t.nvals(b.obs_out), "bf_from_host: P rows differ from the leaf size");
+        }
+        for (const Block &b : ob.st[0].b)
+            BFG_REQUIRE(b.ident || b.n == trial->t.nvals(b.src_in), "bf_from_host: Q columns differ from the leaf size");
+        bf.factors.resize(n_stages);
+        bf.factor_elems.assign(n_stages, 0);
+        for (int32_t s = 0; s < n_stages; ++s) {
+            int64_t ne = int64_t(hd[s].size() / 2);
+            bf.factor_elems[s] = ne;
+            if (precision == BFGPU_C128) {
+                bf.factors[s].alloc(size_t(std::max<int64_t>(ne, 1)) * 16);
+                if (ne) BFG_HIP(hipMemcpy(bf.factors[s].p, hd[s].data(), size_t(ne) * 16, hipMemcpyHostToDevice));
+            } else {
+                hf.assign(hd[s].begin(), hd[s].end());
+                bf.factors[s].alloc(size_t(std::max<int64_t>(ne, 1)) * 8);
+                if (ne) BFG_HIP(hipMemcpy(bf.factors[s].p, hf.data(), size_t(ne) * 8, hipMemcpyHostToDevice));
+            }
+        }
+        ob.stage_data_base.assign(n_stages, 0);
+        pack_tables(bf);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_info_get(const bfgpu_bf *h, bfgpu_bf_info *o) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && o, "bf_info_get: null argument");
+    const BF &bf = h->b;
+    std::memset(o, 0, sizeof(*o));
+    o->n_bf = int32_t(bf.bfs.size());
+    o->precision = bf.precision;
+    o->rows = bf.rows;
+    o->cols = bf.cols;
+    o->max_stages = bf.n_stage_launch;
+    for (const OneBF &ob : bf.bfs)
+        for (const auto &st : ob.st) {
+            o->n_blocks += int64_t(st.b.size());
+            for (const Block &b : st.b)
+                if (!b.ident) o->n_data += int64_t(b.m) * b.n;
+        }
+    const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+    for (int64_t e : bf.factor_elems) o->factor_bytes += e * es;
+    o->slot_len = bf.slot_len;
+    o->n_comp = bf.n_comp;
+    o->n_retry = bf.n_retry;
+    o->t_factor_ms = bf.t_factor_ms;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_counts(const bfgpu_bf *h, int32_t index, int32_t *n_stages, int64_t *n_blocks,
+                               int64_t *n_data, int64_t *n_skel, int64_t *n_skel_idx, int64_t *n_comp,
+                               int32_t roots[2]) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "bf_counts: null handle");
+    BFG_REQUIRE(index >= 0 && index < int32_t(h->b.bfs.size()), "bf_counts: index out of range");
+    const OneBF &ob = h->b.bfs[index];
+    int64_t nb = 0, nd = 0;
+    for (const auto &st : ob.st) {
+        nb += int64_t(st.b.size());
+        for (const Block &b : st.b)
+            if (!b.ident) nd += int64_t(b.m) * b.n;
+    }
+    if (n_stages) *n_stages = int32_t(ob.st.size());
+    if (n_blocks) *n_blocks = nb;
+    if (n_data) *n_data = nd;
+    if (n_skel) *n_skel = int64_t(ob.skel.size());
+    if (n_skel_idx) *n_skel_idx = int64_t(ob.skel_idx.size());
+    if (n_comp) *n_comp = int64_t(ob.comp.size());
+    if (roots) { roots[0] = ob.no; roots[1] = ob.ns; }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_export(const bfgpu_bf *h, int32_t index, int32_t *table, int64_t *data_ofs,
+                               double *data) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && table && data_ofs, "bf_export: null argument");
+    const BF &bf = h->b;
+    BFG_REQUIRE(index >= 0 && index < int32_t(bf.bfs.size()), "bf_export: index out of range");
+    const OneBF &ob = bf.bfs[index];
+    const int32_t nst = int32_t(ob.st.size()), Lmax = bf.n_stage_launch - 1;
+    const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+    std::vector<std::vector<double>> host(bf.n_stage_launch);
+    if (data) {
+        BFG_HIP(hipStreamSynchronize(bf.ctx->stream));
+        for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
+            int64_t ne = bf.factor_elems[L];
+            host[L].resize(size_t(2 * ne));
+            if (!ne) continue;
+            if (es == 16) {
+                BFG_HIP(hipMemcpy(host[L].data(), bf.factors[L].p, size_t(ne) * 16, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> tmp(size_t(2 * ne));
+                BFG_HIP(hipMemcpy(tmp.data(), bf.factors[L].p, size_t(ne) * 8, hipMemcpyDeviceToHost));
+                std::copy(tmp.begin(), tmp.end(), host[L].begin());
+            }
+        }
+    }
+    int64_t q = 0, d = 0;
+    for (int32_t s = 0; s < nst; ++s) {
+        int32_t L = (s == nst - 1) ? Lmax : s;
+        for (const Block &b : ob.st[s].b) {
+            int32_t *r = table + 10 * q;
+            r[0] = s; r[1] = b.obs_out; r[2] = b.src_out; r[3] = b.obs_in; r[4] = b.src_in;
+            r[5] = b.m; r[6] = b.n; r[7] = b.ident; r[8] = b.out_ptr; r[9] = b.in_ptr;
+            if (b.ident) {
+                data_ofs[q] = -1;
+            } else {
+                data_ofs[q] = d;
+                if (data) {
+                    int64_t cnt = int64_t(b.m) * b.n;
+                    BFG_REQUIRE(b.fofs + cnt <= int64_t(host[L].size() / 2), "bf_export: block outside its stage buffer");
+                    std::memcpy(data + 2 * d, host[L].data() + 2 * b.fofs, sizeof(double) * 2 * size_t(cnt));
+                }
+                d += int64_t(b.m) * b.n;
+            }
+            ++q;
+        }
+    }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_export_logs(const bfgpu_bf *h, int32_t index, int32_t *skel_rec,
+                                    int32_t *skel_idx, int32_t *comp_rec) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "bf_export_logs: null handle");
+    BFG_REQUIRE(index >= 0 && index < int32_t(h->b.bfs.size()), "bf_export_logs: index out of range");
+    const OneBF &ob = h->b.bfs[index];
+    if (skel_rec)
+        for (size_t i = 0; i < ob.skel.size(); ++i) {
+            const SkelRec &r = ob.skel[i];
+            int32_t *o = skel_rec + 5 * i;
+            o[0] = r.stage; o[1] = r.s; o[2] = r.o; o[3] = r.ofs; o[4] = r.len;
+        }
+    if (skel_idx && !ob.skel_idx.empty())
+        std::memcpy(skel_idx, ob.skel_idx.data(), sizeof(int32_t) * ob.skel_idx.size());
+    if (comp_rec)
+        for (size_t i = 0; i < ob.comp.size(); ++i) {
+            const CompRec &r = ob.comp[i];
+            int32_t *o = comp_rec + 6 * i;
+            o[0] = r.stage; o[1] = r.s; o[2] = r.o; o[3] = r.guess; o[4] = r.rows; o[5] = r.rank;
+        }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_stage_stats(const bfgpu_bf *h, int32_t *n_stages, int64_t *stats) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && n_stages, "bf_stage_stats: null argument");
+    const BF &bf = h->b;
+    *n_stages = bf.n_stage_launch;
+    if (!stats) return BFGPU_OK;
+    const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+    const int32_t Lmax = bf.n_stage_launch - 1;
+    for (int32_t L = 0; L <= Lmax; ++L) {
+        int64_t nb = 0, ni = 0, mr = 0, by = 0;
+        for (const OneBF &ob : bf.bfs) {
+            int32_t nst = int32_t(ob.st.size());
+            for (int32_t s = 0; s < nst; ++s) {
+                if (((s == nst - 1) ? Lmax : s) != L) continue;
+                for (const Block &b : ob.st[s].b) {
+                    ++nb;
+                    if (b.ident) { ++ni; continue; }
+                    mr = std::max<int64_t>(mr, (s == nst - 1) ? b.n : b.m);
+                    by += int64_t(b.m) * b.n * es;
+                }
+            }
+        }
+        stats[4 * L + 0] = nb;
+        stats[4 * L + 1] = ni;
+        stats[4 * L + 2] = mr;
+        stats[4 * L + 3] = by;
+    }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_convert(bfgpu_bf *h, int32_t precision, bfgpu_bf **out) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && out, "bf_convert: null argument");
+    BFG_REQUIRE(precision == BFGPU_C64 || precision == BFGPU_C128, "bf_convert: bad precision");
+    const BF &src = h->b;
+    auto *n = new bfgpu_bf{};
+    try {
+        BF &bf = n->b;
+        bf.ctx = src.ctx;
+        bf.test = src.test;
+        bf.trial = src.trial;
+        bf.bfs = src.bfs;
+        bf.n_stage_launch = src.n_stage_launch;
+        bf.factor_elems = src.factor_elems;
+        bf.precision = precision;
+        bf.n_comp = src.n_comp;
+        bf.n_retry = src.n_retry;
+        bf.t_factor_ms = src.t_factor_ms;
+        bf.factors.resize(src.factors.size());
+        const int es = precision == BFGPU_C128 ? 16 : 8;
+        for (size_t L = 0; L < src.factors.size(); ++L) {
+            bf.factors[L].alloc(size_t(std::max<int64_t>(src.factor_elems[L], 1)) * es);
+            launch_convert(src.ctx->stream, src.factors[L].p, src.precision, bf.factors[L].p, precision,
+                           src.factor_elems[L]);
+        }
+        BFG_HIP(hipStreamSynchronize(src.ctx->stream));
+        pack_tables(bf);
+    } catch (...) {
+        delete n;
+        throw;
+    }
+    *out = n;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_matvec(bfgpu_bf *h, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
+                            void *y, int64_t ldy, const double *alpha, const double *beta,
+                            int32_t flags) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "matvec: null handle");
+    static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    matvec(h->b, op, nrhs, x, ldx, y, ldy, alpha ? alpha : one, beta ? beta : zero, flags);
+    BFG_API_END
+}
+
+extern "C" int bfgpu_matvec_timed(bfgpu_bf *h, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
+                                  void *y, int64_t ldy, int32_t iters, double *stage_ms,
+                                  int64_t *stage_bytes, int32_t *n_stages, double *total_ms) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && x && y, "matvec_timed: null argument");
+    BFG_REQUIRE(op == BFGPU_OP_N || op == BFGPU_OP_T || op == BFGPU_OP_C, "matvec_timed: bad op");
+    const bool fwd = op == BFGPU_OP_N;
+    BFG_REQUIRE(ldx >= (fwd ? h->b.cols : h->b.rows) && ldy >= (fwd ? h->b.rows : h->b.cols),
+                "matvec_timed: leading dimension too small");
+    matvec_timed(h->b, op, nrhs, x, ldx, y, ldy, iters, stage_ms, stage_bytes, n_stages, total_ms);
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_destroy(bfgpu_bf *h) {
+    if (h && h->b.ctx) (void)hipStreamSynchronize(h->b.ctx->stream);
+    delete h;
+    return BFGPU_OK;
+}

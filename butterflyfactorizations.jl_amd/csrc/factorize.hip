@@ -1,0 +1,531 @@
+// factorize.hip -- level-by-level butterfly factorization driven from the
+// host, with every kernel-matrix tile, pivoted QR and interpolation matrix
+// computed on the GPU in batches (all compressions of one level, over every
+// factorization of the batch, per launch).
+//
+// Restates assemble_BF (src/ButterflyFactorization/bfassembly.jl:48-200)
+// with build_U_skeletons (:206-230), build_nonfrozen_R_blocks (:232-349),
+// build_sourcefrozen_R_blocks (:351-412), build_observerfrozen_R_blocks
+// (:414-481) and the PartialQR functor (compressors.jl:81-165).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <unordered_map>
+
+#include "internal.hpp"
+
+namespace bfg {
+
+int32_t mlp_predict_ranks(const bfgpu_mlp *m, const std::vector<float> &feat, int64_t n,
+                          std::vector<int32_t> &out);
+
+// ---- row sample: sparse Fisher-Yates prefix over a SplitMix64 counter
+// stream (the documented replacement of compressors.jl:101 randperm) ----
+static inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+uint64_t sample_key(uint64_t seed, int32_t no, int32_t ns, int32_t stage, int32_t s, int32_t o) {
+    uint64_t h = mix64(seed + 0x9E3779B97F4A7C15ULL);
+    h = mix64(h ^ uint64_t(uint32_t(no)));
+    h = mix64(h ^ (uint64_t(uint32_t(ns)) << 1));
+    h = mix64(h ^ uint64_t(uint32_t(stage)));
+    h = mix64(h ^ uint64_t(uint32_t(s)));
+    h = mix64(h ^ uint64_t(uint32_t(o)));
+    return h;
+}
+
+void sample_prefix(uint64_t key, int32_t n, int32_t m, int32_t *out) {
+    if (m > n) m = n;
+    std::unordered_map<int32_t, int32_t> sw;
+    sw.reserve(size_t(2 * m + 8));
+    auto get = [&](int32_t i) {
+        auto it = sw.find(i);
+        return it == sw.end() ? i : it->second;
+    };
+    for (int32_t i = 0; i < m; ++i) {
+        uint64_t u = mix64(key + uint64_t(i + 1) * 0x9E3779B97F4A7C15ULL);
+        uint32_t span = uint32_t(n - i);
+        int32_t j = i + int32_t(((u >> 32) * uint64_t(span)) >> 32);
+        int32_t ai = get(i), aj = get(j);
+        sw[i] = aj;
+        sw[j] = ai;
+        out[i] = aj;
+    }
+}
+
+static inline uint64_t fastkey(int32_t s, int32_t o) {
+    return (uint64_t(uint32_t(s)) << 32) | uint64_t(uint32_t(o));
+}
+
+namespace {
+
+struct Comp {
+    int32_t bf, stage, S, O;
+    const int32_t *obs;  // candidate observer point ids
+    int32_t n_obs;
+    std::vector<int32_t> cols;  // source point ids
+    double eps;
+    uint64_t key;
+    int32_t guess = 10, rows = 0, rank = 0;
+    std::vector<int32_t> skel;
+    int64_t tofs = 0;
+    float feat[6];
+};
+
+// Per-factorization assembly state.
+struct Build {
+    int32_t no, ns, LS, LO, L;
+    double eps;
+    std::vector<std::vector<int32_t>> tS, tO;
+    std::unordered_map<uint64_t, std::vector<int32_t>> K;
+};
+
+template <class T> void upload(DevBuf &d, const std::vector<T> &h, hipStream_t s) {
+    d.reserve(sizeof(T) * std::max<size_t>(h.size(), 1));
+    if (!h.empty()) BFG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+struct Engine {
+    Ctx &ctx;
+    const Tree &test, &trial;
+    const bfgpu_opts &o;
+    int prec;
+    size_t es;
+    DevBuf d_test_pts, d_trial_pts;
+    DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
+    int64_t n_comp = 0, n_retry = 0;
+
+    Engine(Ctx &c, const Tree &te, const Tree &tr, const bfgpu_opts &op)
+        : ctx(c), test(te), trial(tr), o(op), prec(op.precision),
+          es(op.precision == BFGPU_C128 ? 16 : 8) {
+        upload(d_test_pts, test.points, ctx.stream);
+        upload(d_trial_pts, trial.points, ctx.stream);
+    }
+
+    void ensure_pool(DevBuf &pool, int64_t &cap_elems, int64_t need) {
+        if (need <= cap_elems) return;
+        int64_t nc = std::max<int64_t>(need, cap_elems * 3 / 2 + 1024);
+        DevBuf nb;
+        nb.alloc(size_t(nc) * es);
+        if (cap_elems > 0)
+            BFG_HIP(hipMemcpyAsync(nb.p, pool.p, size_t(cap_elems) * es, hipMemcpyDeviceToDevice, ctx.stream));
+        BFG_HIP(hipStreamSynchronize(ctx.stream));
+        pool = std::move(nb);
+        cap_elems = nc;
+    }
+
+    void guesses(std::vector<Comp> &cs) {
+        if (o.rank_mode == BFGPU_RANK_LEARNED) {
+            BFG_REQUIRE(o.mlp != nullptr, "factorize: learned ranks need an MLP");
+            std::vector<float> f(6 * cs.size());
+            for (size_t i = 0; i < cs.size(); ++i)
+                for (int q = 0; q < 6; ++q) f[6 * i + q] = cs[i].feat[q];
+            std::vector<int32_t> pr;
+            mlp_predict_ranks(o.mlp, f, int64_t(cs.size()), pr);
+            for (size_t i = 0; i < cs.size(); ++i) cs[i].guess = std::max(pr[i], 1) + o.oversample;
+            return;
+        }
+        for (auto &c : cs) {
+            if (o.rank_mode == BFGPU_RANK_GEOMETRIC) {
+                const double *csrc = &trial.center[3 * c.S], *cobs = &test.center[3 * c.O];
+                double d = std::sqrt((csrc[0] - cobs[0]) * (csrc[0] - cobs[0]) +
+                                     (csrc[1] - cobs[1]) * (csrc[1] - cobs[1]) +
+                                     (csrc[2] - cobs[2]) * (csrc[2] - cobs[2]));
+                double ds = std::max(d, 1e-4);
+                double a = o.k * (trial.radius(c.S) * test.radius(c.O)) / ds;
+                int32_t R = int32_t(std::ceil(o.c_geo * a * a + o.c_eps * std::log(1.0 / c.eps)));
+                c.guess = std::max(R, o.rmin);
+            } else {
+                int32_t R = int32_t(std::ceil(o.c_tau * std::log(1.0 / c.eps)));
+                c.guess = std::max(R, o.rmin);
+            }
+        }
+    }
+
+    // Run every compression of one level in batches: fill -> pivoted QR ->
+    // (adaptive doubling of the rows that saturated) -> interpolation
+    // matrices written into `pool` (the level's factor buffer).
+    void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used) {
+        guesses(cs);
+        std::vector<int32_t> pend;
+        for (size_t i = 0; i < cs.size(); ++i) {
+            Comp &c = cs[i];
+            ++n_comp;
+            if (c.n_obs == 0 || c.cols.empty()) {
+                c.rank = 0;
+                c.rows = 0;
+                c.tofs = pool_used;
+                continue;
+            }
+            c.rows = std::min(std::max(c.guess, 10), c.n_obs);
+            pend.push_back(int32_t(i));
+        }
+        std::vector<int32_t> hrows, hcols, perm, hrank, hpiv;
+        std::vector<FillTask> ft;
+        std::vector<int64_t> fe;
+        std::vector<QrTask> qt;
+        std::vector<InterpTask> it;
+        while (!pend.empty()) {
+            hrows.clear(); hcols.clear(); ft.clear(); fe.clear(); qt.clear();
+            int64_t zofs = 0;
+            int32_t pofs = 0, max_n = 1, max_m = 1;
+            for (int32_t ci : pend) {
+                Comp &c = cs[ci];
+                perm.resize(c.rows);
+                sample_prefix(c.key, c.n_obs, c.rows, perm.data());
+                FillTask f{zofs, c.rows, int32_t(c.cols.size()), int32_t(hrows.size()), int32_t(hcols.size())};
+                for (int32_t i = 0; i < c.rows; ++i) hrows.push_back(c.obs[perm[i]]);
+                hcols.insert(hcols.end(), c.cols.begin(), c.cols.end());
+                ft.push_back(f);
+                zofs += int64_t(f.m) * f.n;
+                fe.push_back(zofs);
+                qt.push_back(QrTask{f.zofs, f.m, f.n, pofs, 0});
+                pofs += f.n;
+                max_n = std::max(max_n, f.n);
+                max_m = std::max(max_m, f.m);
+            }
+            upload(d_rows, hrows, ctx.stream);
+            upload(d_cols, hcols, ctx.stream);
+            upload(d_fill, ft, ctx.stream);
+            upload(d_fill_end, fe, ctx.stream);
+            upload(d_qr, qt, ctx.stream);
+            d_Z.reserve(size_t(zofs) * es);
+            d_piv.reserve(sizeof(int32_t) * size_t(std::max(pofs, 1)));
+            d_rank.reserve(sizeof(int32_t) * pend.size());
+            launch_fill(ctx.stream, prec, d_test_pts.as<double>(), d_trial_pts.as<double>(),
+                        d_rows.as<int32_t>(), d_cols.as<int32_t>(), d_fill.as<FillTask>(),
+                        d_fill_end.as<int64_t>(), int32_t(ft.size()), zofs, d_Z.p, o.k);
+            // per-task tolerance: one launch per distinct eps (the batch may
+            // hold factorizations of different depth)
+            std::vector<double> eps_of;
+            for (int32_t ci : pend) eps_of.push_back(cs[ci].eps);
+            std::vector<double> uniq = eps_of;
+            std::sort(uniq.begin(), uniq.end());
+            uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+            hrank.assign(pend.size(), 0);
+            if (uniq.size() == 1) {
+                launch_cpqr(ctx.stream, prec, d_Z.p, d_qr.as<QrTask>(), int32_t(qt.size()),
+                            d_piv.as<int32_t>(), d_rank.as<int32_t>(), uniq[0], max_n, max_m);
+            } else {
+                // group tasks by tolerance
+                std::vector<QrTask> sub;
+                std::vector<int32_t> where;
+                DevBuf d_sub, d_subrank;
+                for (double e : uniq) {
+                    sub.clear(); where.clear();
+                    for (size_t q = 0; q < pend.size(); ++q)
+                        if (eps_of[q] == e) { sub.push_back(qt[q]); where.push_back(int32_t(q)); }
+                    upload(d_sub, sub, ctx.stream);
+                    d_subrank.reserve(sizeof(int32_t) * sub.size());
+                    launch_cpqr(ctx.stream, prec, d_Z.p, d_sub.as<QrTask>(), int32_t(sub.size()),
+                                d_piv.as<int32_t>(), d_subrank.as<int32_t>(), e, max_n, max_m);
+                    std::vector<int32_t> r(sub.size());
+                    BFG_HIP(hipMemcpyAsync(r.data(), d_subrank.p, sizeof(int32_t) * r.size(), hipMemcpyDeviceToHost, ctx.stream));
+                    BFG_HIP(hipStreamSynchronize(ctx.stream));
+                    for (size_t q = 0; q < where.size(); ++q) hrank[where[q]] = r[q];
+                }
+                BFG_HIP(hipMemcpyAsync(d_rank.p, hrank.data(), sizeof(int32_t) * hrank.size(), hipMemcpyHostToDevice, ctx.stream));
+            }
+            if (uniq.size() == 1) {
+                BFG_HIP(hipMemcpyAsync(hrank.data(), d_rank.p, sizeof(int32_t) * hrank.size(), hipMemcpyDeviceToHost, ctx.stream));
+            }
+            hpiv.resize(std::max(pofs, 1));
+            BFG_HIP(hipMemcpyAsync(hpiv.data(), d_piv.p, sizeof(int32_t) * size_t(pofs), hipMemcpyDeviceToHost, ctx.stream));
+            BFG_HIP(hipStreamSynchronize(ctx.stream));
+            std::vector<int32_t> retry;
+            it.clear();
+            int32_t max_rank = 0;
+            for (size_t q = 0; q < pend.size(); ++q) {
+                Comp &c = cs[pend[q]];
+                int32_t r = hrank[q];
+                if (o.adaptive && r > int32_t(std::floor(0.8 * c.rows)) && c.rows < c.n_obs) {
+                    c.rows = std::min(2 * c.rows, c.n_obs);
+                    retry.push_back(pend[q]);
+                    ++n_retry;
+                    continue;
+                }
+                c.rank = r;
+                c.skel.resize(r);
+                for (int32_t i = 0; i < r; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
+                c.tofs = pool_used;
+                pool_used += int64_t(r) * int64_t(c.cols.size());
+                it.push_back(InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, r, qt[q].pofs});
+                max_rank = std::max(max_rank, r);
+            }
+            if (!it.empty()) {
+                ensure_pool(pool, pool_cap, pool_used);
+                upload(d_interp, it, ctx.stream);
+                launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), int32_t(it.size()),
+                              d_piv.as<int32_t>(), pool.p, max_rank);
+                BFG_HIP(hipStreamSynchronize(ctx.stream));
+            }
+            pend.swap(retry);
+        }
+    }
+};
+
+// Geometric features of a compression (source node S, observer node O) for
+// the learned rank predictor: box radii in wavelengths, centre distance in
+// wavelengths, angle between the separation vector and the z axis, level,
+// and log10(1/eps).
+void features(const Tree &test, const Tree &trial, double k, int32_t S, int32_t O, int32_t stage,
+              double eps, float *f) {
+    double lam = 2.0 * M_PI / k;
+    const double *cs = &trial.center[3 * S], *co = &test.center[3 * O];
+    double d[3] = {co[0] - cs[0], co[1] - cs[1], co[2] - cs[2]};
+    double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    f[0] = float(trial.radius(S) / lam);
+    f[1] = float(test.radius(O) / lam);
+    f[2] = float(dist / lam);
+    f[3] = float(dist > 0 ? std::fabs(d[2]) / dist : 0.0);
+    f[4] = float(stage);
+    f[5] = float(std::log10(1.0 / eps));
+}
+
+}  // namespace
+
+void compression_features(const Tree &test, const Tree &trial, double k, int32_t S, int32_t O,
+                          int32_t stage, double eps, float *f) {
+    features(test, trial, k, S, O, stage, eps, f);
+}
+
+// Factorize n_pairs blocks; fills bf.bfs (reference layout) and bf.factors.
+void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int32_t, int32_t>> &pairs) {
+    auto t0 = std::chrono::steady_clock::now();
+    Ctx &ctx = *bf.ctx;
+    const Tree &test = *bf.test, &trial = *bf.trial;
+    Engine E(ctx, test, trial, o);
+    std::vector<Build> B(pairs.size());
+    int32_t Lmax = 1;
+    bf.bfs.assign(pairs.size(), OneBF{});
+    for (size_t b = 0; b < pairs.size(); ++b) {
+        Build &x = B[b];
+        x.no = pairs[b].first;
+        x.ns = pairs[b].second;
+        BFG_REQUIRE(x.no >= 0 && x.no < test.n_nodes && x.ns >= 0 && x.ns < trial.n_nodes,
+                    "factorize: root node out of range");
+        x.tS = traverse_and_pad(trial, x.ns);
+        x.tO = traverse_and_pad(test, x.no);
+        x.LS = int32_t(x.tS.size());
+        x.LO = int32_t(x.tO.size());
+        x.L = std::max(x.LS, x.LO);
+        x.eps = o.tol / x.L;
+        Lmax = std::max(Lmax, x.L);
+        bf.bfs[b].no = x.no;
+        bf.bfs[b].ns = x.ns;
+        bf.bfs[b].st.resize(x.L + 1);
+    }
+    bf.n_stage_launch = Lmax + 1;
+    bf.factors.clear();
+    bf.factors.resize(bf.n_stage_launch);
+    bf.factor_elems.assign(bf.n_stage_launch, 0);
+
+    auto mkcomp = [&](int32_t b, int32_t stage, int32_t S, int32_t O, std::vector<int32_t> cols) {
+        Comp c;
+        c.bf = b;
+        c.stage = stage;
+        c.S = S;
+        c.O = O;
+        c.obs = test.vals(O);
+        c.n_obs = test.nvals(O);
+        c.cols = std::move(cols);
+        c.eps = B[b].eps;
+        c.key = sample_key(o.seed, B[b].no, B[b].ns, stage, S, O);
+        features(test, trial, o.k, S, O, stage, c.eps, c.feat);
+        return c;
+    };
+    auto log_skel = [&](int32_t b, int32_t stage, int32_t s, int32_t on, const std::vector<int32_t> &v) {
+        OneBF &ob = bf.bfs[b];
+        ob.skel.push_back(SkelRec{stage, s, on, int32_t(ob.skel_idx.size()), int32_t(v.size())});
+        ob.skel_idx.insert(ob.skel_idx.end(), v.begin(), v.end());
+    };
+    auto log_comp = [&](const Comp &c) {
+        bf.bfs[c.bf].comp.push_back(CompRec{c.stage, c.S, c.O, c.guess, c.rows, c.rank});
+    };
+
+    // ---- stage 0: Q (bfassembly.jl:82-102) ----
+    {
+        std::vector<Comp> cs;
+        for (size_t b = 0; b < B.size(); ++b)
+            for (int32_t Sl : B[b].tS.back())
+                cs.push_back(mkcomp(int32_t(b), 0, Sl, B[b].no,
+                                    std::vector<int32_t>(trial.vals(Sl), trial.vals(Sl) + trial.nvals(Sl))));
+        int64_t cap = 0, used = 0;
+        E.run(cs, bf.factors[0], cap, used);
+        bf.factor_elems[0] = used;
+        for (auto &c : cs) {
+            log_comp(c);
+            bf.bfs[c.bf].st[0].b.push_back(Block{B[c.bf].no, c.S, B[c.bf].no, c.S, c.rank,
+                                                  int32_t(c.cols.size()), 0, c.tofs, -1, -1});
+            B[c.bf].K[fastkey(c.S, B[c.bf].no)] = c.skel;
+            log_skel(c.bf, 0, c.S, B[c.bf].no, c.skel);
+        }
+    }
+
+    // ---- R levels (bfassembly.jl:107-181) ----
+    // An action is either a compression (index into cs, with the emission
+    // rule) or an identity block.
+    struct Act {
+        int32_t b, kind;  // 0: compression, 1: identity
+        int32_t comp;
+        int32_t Oout, Sv, Ov;  // emission keys
+        int32_t len;           // identity length
+    };
+    for (int32_t l = 1; l <= Lmax - 1; ++l) {
+        std::vector<Comp> cs;
+        std::vector<Act> acts;
+        std::vector<std::vector<std::tuple<int32_t, int32_t, std::vector<int32_t>>>> upd(B.size());
+        auto U_of = [&](Build &x, int32_t Sv, int32_t Ov) {
+            std::vector<int32_t> u;
+            if (trial.isleaf(Sv)) return x.K.at(fastkey(Sv, Ov));
+            for (int32_t c = 0; c < trial.n_children[Sv]; ++c) {
+                const auto &k = x.K.at(fastkey(trial.first_child[Sv] + c, Ov));
+                u.insert(u.end(), k.begin(), k.end());
+            }
+            return u;
+        };
+        for (size_t b = 0; b < B.size(); ++b) {
+            Build &x = B[b];
+            if (l > x.L - 1) continue;
+            bool sf = l >= x.LS, of = l >= x.LO;
+            auto add_comp = [&](int32_t S, int32_t O, std::vector<int32_t> U, int32_t Oout, int32_t Sv, int32_t Ov) {
+                cs.push_back(mkcomp(int32_t(b), l, S, O, std::move(U)));
+                acts.push_back(Act{int32_t(b), 0, int32_t(cs.size() - 1), Oout, Sv, Ov, 0});
+            };
+            auto add_ident = [&](int32_t Ov, int32_t Sv) {
+                const auto &k = x.K.at(fastkey(Sv, Ov));
+                acts.push_back(Act{int32_t(b), 1, -1, Ov, Sv, Ov, int32_t(k.size())});
+                upd[b].emplace_back(Sv, Ov, k);
+            };
+            if (!sf && !of) {
+                const auto &Os = x.tO[l - 1];
+                const auto &Ss = x.tS[x.LS - l - 1];
+                for (int32_t Sv : Ss)
+                    for (int32_t Ov : Os) {
+                        if (!test.isleaf(Ov)) {
+                            std::vector<int32_t> U = U_of(x, Sv, Ov);
+                            for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
+                                int32_t Oc = test.first_child[Ov] + c;
+                                add_comp(Sv, Oc, U, Oc, Sv, Ov);
+                            }
+                        } else if (!trial.isleaf(Sv)) {
+                            add_comp(Sv, Ov, U_of(x, Sv, Ov), Ov, Sv, Ov);
+                        } else {
+                            add_ident(Ov, Sv);
+                        }
+                    }
+            } else if (sf && !of) {
+                const auto &Os = x.tO[l - 1];
+                int32_t Sv = x.ns;
+                for (int32_t Ov : Os) {
+                    if (!test.isleaf(Ov)) {
+                        for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
+                            int32_t Oc = test.first_child[Ov] + c;
+                            // source frozen: input is the single slot (Ov, ns)
+                            cs.push_back(mkcomp(int32_t(b), l, Sv, Oc, x.K.at(fastkey(Sv, Ov))));
+                            acts.push_back(Act{int32_t(b), 2, int32_t(cs.size() - 1), Oc, Sv, Ov, 0});
+                        }
+                    } else {
+                        add_ident(Ov, Sv);
+                    }
+                }
+            } else if (!sf && of) {
+                const auto &Os = x.tO[x.LO - 1];
+                const auto &Ss = x.tS[x.LS - l - 1];
+                for (int32_t Sv : Ss)
+                    for (int32_t Ov : Os) {
+                        if (!trial.isleaf(Sv)) add_comp(Sv, Ov, U_of(x, Sv, Ov), Ov, Sv, Ov);
+                        else add_ident(Ov, Sv);
+                    }
+            }
+        }
+        int64_t cap = 0, used = 0;
+        E.run(cs, bf.factors[l], cap, used);
+        bf.factor_elems[l] = used;
+        // emit blocks in action order; K updates in the same order
+        std::vector<std::vector<std::tuple<int32_t, int32_t, std::vector<int32_t>>>> upd2(B.size());
+        for (const Act &a : acts) {
+            Build &x = B[a.b];
+            auto &st = bf.bfs[a.b].st[l].b;
+            if (a.kind == 1) {
+                st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, a.len, a.len, 1, -1, -1, -1});
+                const auto &k = x.K.at(fastkey(a.Sv, a.Ov));
+                upd2[a.b].emplace_back(a.Sv, a.Ov, k);
+                continue;
+            }
+            Comp &c = cs[a.comp];
+            log_comp(c);
+            int32_t nU = int32_t(c.cols.size());
+            if (a.kind == 2 || trial.isleaf(a.Sv)) {
+                // whole T: source frozen (Oc, ns) <- (Ov, ns), or leaf Svert
+                st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, c.rank, nU, 0, c.tofs, -1, -1});
+            } else {
+                int32_t last = 0;
+                for (int32_t q = 0; q < trial.n_children[a.Sv]; ++q) {
+                    int32_t Sc = trial.first_child[a.Sv] + q;
+                    int32_t ks = int32_t(x.K.at(fastkey(Sc, a.Ov)).size());
+                    st.push_back(Block{a.Oout, a.Sv, a.Ov, Sc, c.rank, ks, 0,
+                                       c.tofs + int64_t(c.rank) * last, -1, -1});
+                    last += ks;
+                }
+            }
+            upd2[a.b].emplace_back(c.S, c.O, c.skel);
+        }
+        for (size_t b = 0; b < B.size(); ++b)
+            for (auto &u : upd2[b]) {
+                B[b].K[fastkey(std::get<0>(u), std::get<1>(u))] = std::get<2>(u);
+                log_skel(int32_t(b), l, std::get<0>(u), std::get<1>(u), std::get<2>(u));
+            }
+    }
+
+    // ---- P (bfassembly.jl:186-197): dense kernel blocks ----
+    {
+        int32_t sP = Lmax;
+        std::vector<int32_t> hrows, hcols;
+        std::vector<FillTask> ft;
+        std::vector<int64_t> fe;
+        int64_t zofs = 0;
+        for (size_t b = 0; b < B.size(); ++b) {
+            Build &x = B[b];
+            auto &st = bf.bfs[b].st[x.L].b;
+            for (int32_t Ol : x.tO.back()) {
+                const auto &col = x.K.at(fastkey(x.ns, Ol));
+                FillTask f{zofs, test.nvals(Ol), int32_t(col.size()), int32_t(hrows.size()), int32_t(hcols.size())};
+                hrows.insert(hrows.end(), test.vals(Ol), test.vals(Ol) + test.nvals(Ol));
+                hcols.insert(hcols.end(), col.begin(), col.end());
+                st.push_back(Block{Ol, x.ns, Ol, x.ns, f.m, f.n, 0, zofs, -1, -1});
+                if (int64_t(f.m) * f.n > 0) {
+                    ft.push_back(f);
+                    zofs += int64_t(f.m) * f.n;
+                    fe.push_back(zofs);
+                }
+            }
+        }
+        bf.factors[sP].alloc(size_t(std::max<int64_t>(zofs, 1)) * E.es);
+        bf.factor_elems[sP] = zofs;
+        upload(E.d_rows, hrows, ctx.stream);
+        upload(E.d_cols, hcols, ctx.stream);
+        upload(E.d_fill, ft, ctx.stream);
+        upload(E.d_fill_end, fe, ctx.stream);
+        launch_fill(ctx.stream, E.prec, E.d_test_pts.as<double>(), E.d_trial_pts.as<double>(),
+                    E.d_rows.as<int32_t>(), E.d_cols.as<int32_t>(), E.d_fill.as<FillTask>(),
+                    E.d_fill_end.as<int64_t>(), int32_t(ft.size()), zofs, bf.factors[sP].p, o.k);
+        BFG_HIP(hipStreamSynchronize(ctx.stream));
+    }
+    bf.n_comp = E.n_comp;
+    bf.n_retry = E.n_retry;
+    bf.precision = o.precision;
+    // stage index of each BF's blocks in launch numbering: Q = 0, R^l = l,
+    // P = Lmax; record the factor base per reference stage
+    for (size_t b = 0; b < B.size(); ++b) {
+        OneBF &ob = bf.bfs[b];
+        ob.stage_data_base.assign(ob.st.size(), 0);
+    }
+    pack_tables(bf);
+    bf.t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace bfg

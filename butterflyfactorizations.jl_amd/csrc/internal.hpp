@@ -1,0 +1,206 @@
+// internal.hpp -- shared declarations of the gfx950 butterfly library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bfgpu.h"
+
+namespace bfg {
+
+// Error carried from deep inside the library to the C ABI boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &m);
+
+#define BFG_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::bfg::Error(BFGPU_ERR_HIP, std::string(#call) + ": " +                 \
+                                                  hipGetErrorString(e_) + " at " __FILE__ \
+                                                  ":" + std::to_string(__LINE__));         \
+    } while (0)
+
+#define BFG_REQUIRE(cond, msg)                                                     \
+    do {                                                                           \
+        if (!(cond)) throw ::bfg::Error(BFGPU_ERR_ARG, std::string(msg));          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Cluster tree (host).  Node 0 is the root; children contiguous; the values
+// of a node are perm[val_begin, val_end).
+// ---------------------------------------------------------------------------
+struct Tree {
+    int32_t n_nodes = 0, n_values = 0;
+    std::vector<int32_t> parent, first_child, n_children, val_begin, val_end, perm, level;
+    std::vector<double> center, halfsize, points;
+    bool isleaf(int32_t v) const { return n_children[v] == 0; }
+    int32_t nvals(int32_t v) const { return val_end[v] - val_begin[v]; }
+    const int32_t *vals(int32_t v) const { return perm.data() + val_begin[v]; }
+    double radius(int32_t v) const { return halfsize[v] * 1.7320508075688772; }
+};
+
+// treelevels + traverseandpad (src/auxillaries.jl:263-281, :340-350).
+std::vector<std::vector<int32_t>> traverse_and_pad(const Tree &t, int32_t root);
+int32_t tree_height(const Tree &t);
+
+// ---------------------------------------------------------------------------
+// Device buffer (RAII).
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release();
+    void alloc(size_t n);              // exact size (frees old)
+    void reserve(size_t n);            // grow-only
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    int n_cu = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Factorization in the reference block layout + packed device tables.
+// ---------------------------------------------------------------------------
+struct Block {
+    int32_t obs_out, src_out, obs_in, src_in;
+    int32_t m, n, ident;
+    int64_t fofs;      // element offset into the stage's factor buffer (-1 identity)
+    int32_t out_ptr, in_ptr;  // reference slot ids (first use order per level)
+};
+
+struct SkelRec { int32_t stage, s, o, ofs, len; };
+struct CompRec { int32_t stage, s, o, guess, rows, rank; };
+
+// One stage of one factorization in reference layout.
+struct StageBlocks { std::vector<Block> b; };
+
+struct OneBF {
+    int32_t no = 0, ns = 0;
+    std::vector<StageBlocks> st;      // Q, R^1..R^{L-1}, P
+    std::vector<SkelRec> skel;
+    std::vector<int32_t> skel_idx;
+    std::vector<CompRec> comp;
+    std::vector<int64_t> stage_data_base;  // per stage: element base in that stage's buffer
+};
+
+// Work item of one stage launch: an output target with a list of
+// contributions (factor blocks) summed into it.
+struct MvItem {
+    int32_t out_ofs;   // slot offset, or value offset into the scatter perm
+    int32_t out_len;
+    int32_t c_begin, c_end;  // contributions [c_begin, c_end)
+};
+struct MvContrib {
+    int64_t fofs;      // element offset of the column-major block (-1: identity)
+    int32_t m, n;      // block rows / cols
+    int32_t in_ofs;    // slot offset, or value offset into the gather perm
+    int32_t pad;
+};
+
+enum InKind { IN_SLOT = 0, IN_GATHER = 1 };
+enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
+
+struct StageTable {
+    int32_t n_items = 0, n_contrib = 0;
+    int32_t in_kind = IN_SLOT, out_kind = OUT_SLOT;
+    int32_t max_in = 0, max_out = 0;   // longest input / output vector
+    int64_t alg_bytes_1 = 0;           // algorithmic bytes per RHS (factor bytes separate)
+    int64_t factor_elems = 0;          // factor entries read
+    DevBuf items, contribs;
+    std::vector<MvItem> h_items;
+    std::vector<MvContrib> h_contribs;
+};
+
+struct BF {
+    Ctx *ctx = nullptr;
+    int32_t precision = BFGPU_C128;
+    const Tree *test = nullptr;   // observer tree
+    const Tree *trial = nullptr;  // source tree
+    std::vector<OneBF> bfs;
+    int32_t n_stage_launch = 0;   // Q, R^1..R^{Lmax-1}, P
+    std::vector<DevBuf> factors;  // per launch stage: packed factor data
+    std::vector<int64_t> factor_elems;
+    int64_t slot_len = 0;
+    int64_t rows = 0, cols = 0;
+    bool y_covered = true;        // forward P covers every row of y
+    bool x_covered = true;        // transposed Q covers every column
+    std::vector<StageTable> fwd, tr;
+    DevBuf perm_test, perm_trial;
+    DevBuf slots;                 // slot buffer (slot_len * nrhs)
+    size_t slots_cap = 0;
+    DevBuf xy_host_stage;         // staging for host x / y
+    int64_t n_comp = 0, n_retry = 0;
+    double t_factor_ms = 0.0;
+};
+
+// Packing of reference-layout blocks into launch tables (matvec.hip).
+void pack_tables(BF &bf);
+
+// Kernel launchers (kernels.hip).
+struct FillTask {       // one kernel-matrix block
+    int64_t zofs;       // element offset of the column-major block
+    int32_t m, n;
+    int32_t rofs, cofs; // offsets into the row / column point-index lists
+};
+void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
+                 const int32_t *rows, const int32_t *cols, const FillTask *tasks,
+                 const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k);
+
+struct QrTask {
+    int64_t zofs;       // Z block (m x n, ld = m)
+    int32_t m, n;
+    int32_t pofs;       // pivot array offset (n entries)
+    int32_t pad;
+};
+void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
+                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m);
+
+struct InterpTask {
+    int64_t zofs;       // R (in Z workspace, ld = m)
+    int64_t tofs;       // T output (rank x n, column-major) in the factor buffer
+    int32_t m, n, rank, pofs;
+};
+void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tasks,
+                   int32_t n_tasks, const int32_t *piv, void *T, int32_t max_rank);
+
+void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const void *factors,
+                  const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
+                  int64_t slot_len, const int32_t *perm_in, const int32_t *perm_out,
+                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta);
+void launch_scale(hipStream_t s, int prec, void *y, int64_t n, int64_t ldy, int64_t nrhs,
+                  const double *beta);
+void launch_convert(hipStream_t s, const void *in, int in_prec, void *out, int out_prec,
+                    int64_t n);
+
+// features of one compression for the learned rank predictor
+void compression_features(const Tree &test, const Tree &trial, double k, int32_t S, int32_t O,
+                          int32_t stage, double eps, float *f);
+
+}  // namespace bfg
+
+// opaque handles of the C ABI
+struct bfgpu_ctx { bfg::Ctx c; };
+struct bfgpu_tree { bfg::Tree t; };
+struct bfgpu_bf { bfg::BF b; };

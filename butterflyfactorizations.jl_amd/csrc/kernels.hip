@@ -1,0 +1,563 @@
+// kernels.hip -- gfx950 kernels of the butterfly far-field path:
+//   * Helmholtz kernel-matrix tiles over index lists,
+//   * batched column-pivoted Householder QR with relative-tolerance stop,
+//   * interpolation matrices T = R11^{-1} [R11 R12] P^T (FP64 solve),
+//   * level stages of the butterfly matvec (forward and transposed).
+// All kernels use 256-thread workgroups = 4 wavefronts of 64 lanes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "internal.hpp"
+
+namespace bfg {
+
+template <typename R> struct C { R x, y; };
+
+template <typename R> __device__ __forceinline__ C<R> cmk(R a, R b) { return C<R>{a, b}; }
+template <typename R> __device__ __forceinline__ C<R> cadd(C<R> a, C<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R> __device__ __forceinline__ C<R> csub(C<R> a, C<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R> __device__ __forceinline__ C<R> cmul(C<R> a, C<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+template <typename R> __device__ __forceinline__ C<R> cmulc(C<R> a, C<R> b) {
+    return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+// acc += a * b
+template <typename R> __device__ __forceinline__ void cfma(C<R> &acc, C<R> a, C<R> b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+// acc += conj(a) * b
+template <typename R> __device__ __forceinline__ void cfmac(C<R> &acc, C<R> a, C<R> b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Helmholtz tiles.  Z[i + m j] = exp(-i k r) / (4 pi r), r = |obs[rows[i]] -
+// src[cols[j]]|, for every task; one thread per entry, task found by binary
+// search over the running entry counts.  The distance and the phase
+// reduction k r mod 2 pi are computed in FP64; FP32 tiles take sincos of the
+// reduced phase in FP32.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void fill_kernel(const double *__restrict__ obs,
+                                                   const double *__restrict__ src,
+                                                   const int32_t *__restrict__ rows,
+                                                   const int32_t *__restrict__ cols,
+                                                   const FillTask *__restrict__ tasks,
+                                                   const int64_t *__restrict__ task_end,
+                                                   int32_t n_tasks, int64_t total, C<R> *Z,
+                                                   double k) {
+    int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= total) return;
+    int32_t lo = 0, hi = n_tasks - 1;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if (task_end[mid] > e) hi = mid;
+        else lo = mid + 1;
+    }
+    const FillTask t = tasks[lo];
+    int64_t local = e - (task_end[lo] - int64_t(t.m) * t.n);
+    int32_t i = int32_t(local % t.m), j = int32_t(local / t.m);
+    const double *a = obs + 3 * int64_t(rows[t.rofs + i]);
+    const double *b = src + 3 * int64_t(cols[t.cofs + j]);
+    double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    double r = sqrt(dx * dx + dy * dy + dz * dz);
+    double g = 1.0 / (4.0 * M_PI * r);
+    C<R> v;
+    if constexpr (sizeof(R) == 8) {
+        double s, c;
+        sincos(k * r, &s, &c);
+        v = {c * g, -s * g};
+    } else {
+        double ph = k * r;
+        ph -= 6.283185307179586 * rint(ph * 0.15915494309189535);
+        float s, c;
+        sincosf(float(ph), &s, &c);
+        v = {float(c * g), float(-s * g)};
+    }
+    Z[t.zofs + local] = v;
+}
+
+void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
+                 const int32_t *rows, const int32_t *cols, const FillTask *tasks,
+                 const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k) {
+    if (total <= 0 || n_tasks <= 0) return;
+    dim3 grid(unsigned((total + 255) / 256));
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(fill_kernel<double>, grid, dim3(256), 0, s, obs_pts, src_pts, rows,
+                           cols, tasks, task_end, n_tasks, total, (C<double> *)Z, k);
+    else
+        hipLaunchKernelGGL(fill_kernel<float>, grid, dim3(256), 0, s, obs_pts, src_pts, rows,
+                           cols, tasks, task_end, n_tasks, total, (C<float> *)Z, k);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Batched column-pivoted Householder QR, one workgroup per block.
+// Same algorithm as LAPACK xGEQP3/xLAQP2 (unblocked): pivot = first column
+// of largest partial norm; reflector in xLARFG form; trailing update with
+// H^H; partial norms (FP64) downdated with the LAPACK Working Note 176 test
+// and recomputed when the downdate has cancelled.  Stops before step k >= 1
+// when the pivot column's remaining norm is <= rtol * |R_11|.
+// LDS: vn1[n], vn2[n] (FP64).  Z is updated in place in global memory.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__restrict__ tasks,
+                                                   int32_t *piv_all, int32_t *rank_out,
+                                                   double rtol) {
+    extern __shared__ double sm[];
+    const QrTask tk = tasks[blockIdx.x];
+    const int m = tk.m, n = tk.n;
+    C<R> *A = Z + tk.zofs;
+    int32_t *piv = piv_all + tk.pofs;
+    double *vn1 = sm, *vn2 = sm + n;
+    __shared__ double rv[4];
+    __shared__ int ri[4];
+    __shared__ double s_pn;
+    __shared__ double s_beta, s_tau_x, s_tau_y, s_scal_x, s_scal_y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    for (int j = wave; j < n; j += 4) {
+        double s = 0.0;
+        for (int i = lane; i < m; i += 64) {
+            C<R> a = A[i + int64_t(j) * m];
+            s += double(a.x) * a.x + double(a.y) * a.y;
+        }
+        s = wave_sum(s);
+        if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
+    }
+    for (int j = tid; j < n; j += 256) piv[j] = j;
+    __syncthreads();
+
+    const int mn = m < n ? m : n;
+    int rank = mn;
+    double r11 = 0.0;
+    const double tol3z = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
+    for (int k = 0; k < mn; ++k) {
+        // pivot: first maximal vn1 over [k, n)
+        double bv = -1.0;
+        int bi = n;
+        for (int j = k + tid; j < n; j += 256) {
+            double v = vn1[j];
+            if (v > bv || (v == bv && j < bi)) { bv = v; bi = j; }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            double ov = __shfl_xor(bv, o, 64);
+            int oi = __shfl_xor(bi, o, 64);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (lane == 0) { rv[wave] = bv; ri[wave] = bi; }
+        __syncthreads();
+        int p;
+        {
+            double b = rv[0];
+            p = ri[0];
+            for (int w = 1; w < 4; ++w)
+                if (rv[w] > b || (rv[w] == b && ri[w] < p)) { b = rv[w]; p = ri[w]; }
+        }
+        // exact remaining norm of the pivot column
+        if (wave == 0) {
+            double s = 0.0;
+            for (int i = k + lane; i < m; i += 64) {
+                C<R> a = A[i + int64_t(p) * m];
+                s += double(a.x) * a.x + double(a.y) * a.y;
+            }
+            s = wave_sum(s);
+            if (lane == 0) s_pn = sqrt(s);
+        }
+        __syncthreads();
+        double pn = s_pn;
+        if (k == 0) r11 = pn;
+        if (pn == 0.0 || (k > 0 && pn <= rtol * r11)) { rank = k; break; }
+        if (p != k) {
+            for (int i = tid; i < m; i += 256) {
+                C<R> t = A[i + int64_t(p) * m];
+                A[i + int64_t(p) * m] = A[i + int64_t(k) * m];
+                A[i + int64_t(k) * m] = t;
+            }
+            if (tid == 0) {
+                int32_t t = piv[p]; piv[p] = piv[k]; piv[k] = t;
+                vn1[p] = vn1[k];
+                vn2[p] = vn2[k];
+            }
+        }
+        __syncthreads();
+        // reflector from column k
+        C<R> *a = A + int64_t(k) * m;
+        if (wave == 0) {
+            double s = 0.0;
+            for (int i = k + 1 + lane; i < m; i += 64) s += double(a[i].x) * a[i].x + double(a[i].y) * a[i].y;
+            s = wave_sum(s);
+            if (lane == 0) {
+                double xnorm = sqrt(s);
+                double ar = a[k].x, ai = a[k].y;
+                if (xnorm == 0.0 && ai == 0.0) {
+                    s_beta = ar; s_tau_x = 0.0; s_tau_y = 0.0; s_scal_x = 1.0; s_scal_y = 0.0;
+                } else {
+                    double beta = hypot(hypot(ar, ai), xnorm);
+                    if (ar >= 0.0) beta = -beta;
+                    s_beta = beta;
+                    s_tau_x = (beta - ar) / beta;
+                    s_tau_y = -ai / beta;
+                    // 1 / (alpha - beta)
+                    double dr = ar - beta, di = ai, den = dr * dr + di * di;
+                    s_scal_x = dr / den;
+                    s_scal_y = -di / den;
+                }
+            }
+        }
+        __syncthreads();
+        const C<R> scal = {R(s_scal_x), R(s_scal_y)};
+        const C<R> ctau = {R(s_tau_x), R(-s_tau_y)};  // conj(tau)
+        const bool apply = !(s_tau_x == 0.0 && s_tau_y == 0.0);
+        if (apply)
+            for (int i = k + 1 + tid; i < m; i += 256) a[i] = cmul(a[i], scal);
+        __syncthreads();
+        if (tid == 0) a[k] = C<R>{R(s_beta), R(0)};
+        // trailing update, one wave per column
+        for (int j = k + 1 + wave; j < n; j += 4) {
+            C<R> *c = A + int64_t(j) * m;
+            if (apply) {
+                C<R> w = {R(0), R(0)};
+                for (int i = k + lane; i < m; i += 64) {
+                    C<R> v = (i == k) ? C<R>{R(1), R(0)} : a[i];
+                    cfmac(w, v, c[i]);
+                }
+                w.x = wave_sum(w.x);
+                w.y = wave_sum(w.y);
+                w = cmul(ctau, w);
+                for (int i = k + lane; i < m; i += 64) {
+                    C<R> v = (i == k) ? C<R>{R(1), R(0)} : a[i];
+                    c[i] = csub(c[i], cmul(v, w));
+                }
+            }
+            // norm downdate (lane 0 holds row k)
+            int recompute = 0;
+            double nv = 0.0;
+            if (lane == 0 && vn1[j] != 0.0) {
+                C<R> ck = c[k];
+                double t = sqrt(double(ck.x) * ck.x + double(ck.y) * ck.y) / vn1[j];
+                t = 1.0 - t * t;
+                if (t < 0.0) t = 0.0;
+                double r = vn1[j] / vn2[j];
+                double t2 = t * r * r;
+                if (t2 <= tol3z) recompute = 1;
+                else nv = vn1[j] * sqrt(t);
+            }
+            recompute = __shfl(recompute, 0, 64);
+            if (recompute) {
+                double s = 0.0;
+                // same lane -> row map as the update, so every lane reads its own writes
+                for (int i = k + lane; i < m; i += 64)
+                    if (i > k) s += double(c[i].x) * c[i].x + double(c[i].y) * c[i].y;
+                s = wave_sum(s);
+                if (lane == 0) { vn1[j] = sqrt(s); vn2[j] = vn1[j]; }
+            } else if (lane == 0 && vn1[j] != 0.0) {
+                vn1[j] = nv;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) rank_out[blockIdx.x] = rank;
+}
+
+void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
+                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t) {
+    if (n_tasks <= 0) return;
+    size_t lds = sizeof(double) * 2 * size_t(max_n > 0 ? max_n : 1);
+    BFG_REQUIRE(lds <= 64 * 1024, "cpqr: too many columns for the norm arrays");
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(cpqr_kernel<double>, dim3(n_tasks), dim3(256), lds, s,
+                           (C<double> *)Z, tasks, piv, rank, rtol);
+    else
+        hipLaunchKernelGGL(cpqr_kernel<float>, dim3(n_tasks), dim3(256), lds, s,
+                           (C<float> *)Z, tasks, piv, rank, rtol);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation matrices.  For each task: T[:, piv[j]] = e_j (j < rank) and
+// T[:, piv[j]] = R11^{-1} R[0:rank, j] (j >= rank), solved in FP64 by one
+// wave per column (lanes hold the right-hand side rows, column-oriented back
+// substitution).  T is column-major rank x n.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void interp_kernel(const C<R> *Z,
+                                                     const InterpTask *__restrict__ tasks,
+                                                     const int32_t *__restrict__ piv_all,
+                                                     C<R> *T) {
+    const InterpTask tk = tasks[blockIdx.x];
+    const int m = tk.m, n = tk.n, r = tk.rank;
+    const C<R> *A = Z + tk.zofs;
+    const int32_t *piv = piv_all + tk.pofs;
+    C<R> *out = T + tk.tofs;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int MAXR = 4;  // rank <= 256
+    for (int j = wave; j < n; j += 4) {
+        C<R> *t = out + int64_t(piv[j]) * r;
+        if (j < r) {
+            for (int i = lane; i < r; i += 64) t[i] = (i == j) ? C<R>{R(1), R(0)} : C<R>{R(0), R(0)};
+            continue;
+        }
+        C<double> b[MAXR];
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) {
+            int i = lane + 64 * q;
+            C<R> v = (i < r) ? A[i + int64_t(j) * m] : C<R>{R(0), R(0)};
+            b[q] = {double(v.x), double(v.y)};
+        }
+        for (int i = r - 1; i >= 0; --i) {
+            // x_i = b_i / R_ii, then b_q -= R_qi x_i for q < i
+            int owner = i & 63, slot = i >> 6;
+            C<double> bi = {0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q)
+                if (q == slot) bi = b[q];
+            bi.x = __shfl(bi.x, owner, 64);
+            bi.y = __shfl(bi.y, owner, 64);
+            C<R> d = A[i + int64_t(i) * m];
+            double dr = d.x, di = d.y, den = dr * dr + di * di;
+            C<double> xi = {(bi.x * dr + bi.y * di) / den, (bi.y * dr - bi.x * di) / den};
+#pragma unroll
+            for (int q = 0; q < MAXR; ++q) {
+                int row = lane + 64 * q;
+                if (row < i) {
+                    C<R> rv = A[row + int64_t(i) * m];
+                    C<double> rq = {double(rv.x), double(rv.y)};
+                    b[q] = csub(b[q], cmul(rq, xi));
+                } else if (row == i) {
+                    b[q] = xi;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXR; ++q) {
+            int i = lane + 64 * q;
+            if (i < r) t[i] = C<R>{R(b[q].x), R(b[q].y)};
+        }
+    }
+}
+
+void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tasks,
+                   int32_t n_tasks, const int32_t *piv, void *T, int32_t max_rank) {
+    if (n_tasks <= 0) return;
+    BFG_REQUIRE(max_rank <= 256, "interp: rank above 256 is not supported");
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(interp_kernel<double>, dim3(n_tasks), dim3(256), 0, s,
+                           (const C<double> *)Z, tasks, piv, (C<double> *)T);
+    else
+        hipLaunchKernelGGL(interp_kernel<float>, dim3(n_tasks), dim3(256), 0, s,
+                           (const C<float> *)Z, tasks, piv, (C<float> *)T);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Matvec stage.  One workgroup per (item, right-hand side).  An item is an
+// output vector (slot, or rows of y reached through a permutation) that
+// receives the sum of its contributions op(A_c) in_c; in_c is a slot or a
+// gather of x through a permutation.
+//   OP = 0 (N): out[i] = sum_c sum_j A_c[i,j] in_c[j].  Lanes walk rows
+//     (column-major blocks -> coalesced), the 4 waves split the columns,
+//     partial sums meet in LDS in fixed wave order (deterministic).
+//   OP = 1 (T) / 2 (C): out[j] = sum_c sum_i op(A_c[i,j]) in_c[i].  Wave w
+//     owns columns j = w mod 4, lanes walk rows, wave reduction per column.
+// Identity contributions (fofs < 0) add in_c.
+// ---------------------------------------------------------------------------
+template <typename R, int OP, int INK, int OUTK>
+__global__ __launch_bounds__(256) void stage_kernel(
+    const MvItem *__restrict__ items, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, const C<R> *__restrict__ x, int64_t ldx, C<R> *__restrict__ y,
+    int64_t ldy, C<R> *__restrict__ slots, int64_t slot_len, const int32_t *__restrict__ perm_in,
+    const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta, int fuse_beta, int max_in) {
+    extern __shared__ char smraw[];
+    C<R> *xin = reinterpret_cast<C<R> *>(smraw);               // max_in
+    C<R> *oacc = xin + max_in;                                 // OP != 0: out_len
+    __shared__ C<R> red[4][64];
+    const MvItem it = items[blockIdx.x];
+    const int64_t rhs = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const C<R> *xr = x + rhs * ldx;
+    C<R> *yr = y + rhs * ldy;
+    C<R> *sr = slots + rhs * slot_len;
+
+    auto load_in = [&](const MvContrib &c, int len) {
+        for (int j = tid; j < len; j += 256) {
+            if (INK == IN_GATHER) xin[j] = xr[perm_in[c.in_ofs + j]];
+            else xin[j] = sr[c.in_ofs + j];
+        }
+    };
+    auto store_out = [&](int i, C<R> v) {
+        if (OUTK == OUT_SLOT) {
+            sr[it.out_ofs + i] = v;
+        } else {
+            int64_t idx = perm_out[it.out_ofs + i];
+            C<R> old = yr[idx];
+            C<R> base = fuse_beta ? cmul(beta, old) : old;
+            yr[idx] = cadd(base, cmul(alpha, v));
+        }
+    };
+
+    if (OP == 0) {
+        for (int r0 = 0; r0 < it.out_len; r0 += 64) {
+            const int i = r0 + lane;
+            C<R> acc = {R(0), R(0)};
+            for (int ci = it.c_begin; ci < it.c_end; ++ci) {
+                const MvContrib c = contribs[ci];
+                __syncthreads();
+                load_in(c, c.n);
+                __syncthreads();
+                if (c.fofs < 0) {
+                    if (wave == 0 && i < c.m) acc = cadd(acc, xin[i]);
+                    continue;
+                }
+                const int jb = (c.n * wave) >> 2, je = (c.n * (wave + 1)) >> 2;
+                if (i < c.m) {
+                    const C<R> *Ac = F + c.fofs + i;
+                    for (int j = jb; j < je; ++j) cfma(acc, Ac[int64_t(j) * c.m], xin[j]);
+                }
+            }
+            red[wave][lane] = acc;
+            __syncthreads();
+            if (wave == 0 && i < it.out_len) {
+                C<R> s = red[0][lane];
+                s = cadd(s, red[1][lane]);
+                s = cadd(s, red[2][lane]);
+                s = cadd(s, red[3][lane]);
+                store_out(i, s);
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int j = tid; j < it.out_len; j += 256) oacc[j] = C<R>{R(0), R(0)};
+        for (int ci = it.c_begin; ci < it.c_end; ++ci) {
+            const MvContrib c = contribs[ci];
+            __syncthreads();
+            load_in(c, c.m);
+            __syncthreads();
+            if (c.fofs < 0) {
+                for (int j = tid; j < c.n; j += 256) oacc[j] = cadd(oacc[j], xin[j]);
+                continue;
+            }
+            for (int j = wave; j < c.n; j += 4) {
+                const C<R> *Ac = F + c.fofs + int64_t(j) * c.m;
+                C<R> s = {R(0), R(0)};
+                for (int i = lane; i < c.m; i += 64) {
+                    if (OP == 2) cfmac(s, Ac[i], xin[i]);
+                    else cfma(s, Ac[i], xin[i]);
+                }
+                s.x = wave_sum(s.x);
+                s.y = wave_sum(s.y);
+                if (lane == 0) oacc[j] = cadd(oacc[j], s);
+            }
+        }
+        __syncthreads();
+        for (int j = tid; j < it.out_len; j += 256) store_out(j, oacc[j]);
+    }
+}
+
+template <typename R, int OP>
+static void launch_stage_t(hipStream_t s, const StageTable &t, const void *factors,
+                           const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
+                           int64_t slot_len, const int32_t *perm_in, const int32_t *perm_out,
+                           int64_t nrhs, const double *alpha, const double *beta, bool fuse) {
+    C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
+    int max_in = t.max_in > 0 ? t.max_in : 1;
+    size_t lds = sizeof(C<R>) * (size_t(max_in) + (OP != 0 ? size_t(t.max_out) : 0));
+    BFG_REQUIRE(lds <= 64 * 1024, "matvec stage: vectors too long for LDS staging");
+    dim3 grid(unsigned(t.n_items), unsigned(nrhs));
+    auto F = (const C<R> *)factors;
+    auto X = (const C<R> *)x;
+    auto Y = (C<R> *)y;
+    auto S = (C<R> *)slots;
+    auto I = t.items.as<const MvItem>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_STAGE(INK, OUTK)                                                                \
+    hipLaunchKernelGGL((stage_kernel<R, OP, INK, OUTK>), grid, dim3(256), lds, s, I, P, F, X, \
+                       ldx, Y, ldy, S, slot_len, perm_in, perm_out, a, b, int(fuse), max_in)
+    if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_STAGE(IN_SLOT, OUT_SLOT);
+    else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_STAGE(IN_GATHER, OUT_SLOT);
+    else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_STAGE(IN_SLOT, OUT_SCATTER);
+    else BFG_STAGE(IN_GATHER, OUT_SCATTER);
+#undef BFG_STAGE
+    BFG_HIP(hipGetLastError());
+}
+
+void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const void *factors,
+                  const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
+                  int64_t slot_len, const int32_t *perm_in, const int32_t *perm_out,
+                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta) {
+    if (t.n_items == 0 || nrhs == 0) return;
+    BFG_REQUIRE(nrhs < 65536, "matvec: nrhs too large");
+    if (prec == BFGPU_C128) {
+        if (op == BFGPU_OP_N) launch_stage_t<double, 0>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+        else if (op == BFGPU_OP_T) launch_stage_t<double, 1>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+        else launch_stage_t<double, 2>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+    } else {
+        if (op == BFGPU_OP_N) launch_stage_t<float, 0>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+        else if (op == BFGPU_OP_T) launch_stage_t<float, 1>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+        else launch_stage_t<float, 2>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+    }
+}
+
+// y[:, r] = beta * y[:, r] over n rows (beta == 0 writes zeros).
+template <typename R>
+__global__ void scale_kernel(C<R> *y, int64_t n, int64_t ldy, C<R> beta) {
+    int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    C<R> *p = y + int64_t(blockIdx.y) * ldy + i;
+    *p = (beta.x == R(0) && beta.y == R(0)) ? C<R>{R(0), R(0)} : cmul(beta, *p);
+}
+
+void launch_scale(hipStream_t s, int prec, void *y, int64_t n, int64_t ldy, int64_t nrhs,
+                  const double *beta) {
+    if (n <= 0 || nrhs <= 0) return;
+    dim3 grid(unsigned((n + 255) / 256), unsigned(nrhs));
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(scale_kernel<double>, grid, dim3(256), 0, s, (C<double> *)y, n, ldy,
+                           C<double>{beta[0], beta[1]});
+    else
+        hipLaunchKernelGGL(scale_kernel<float>, grid, dim3(256), 0, s, (C<float> *)y, n, ldy,
+                           C<float>{float(beta[0]), float(beta[1])});
+    BFG_HIP(hipGetLastError());
+}
+
+template <typename A, typename B>
+__global__ void convert_kernel(const A *in, B *out, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) out[i] = B(in[i]);
+}
+
+void launch_convert(hipStream_t s, const void *in, int in_prec, void *out, int out_prec,
+                    int64_t n) {
+    // n complex entries -> 2 n reals
+    int64_t nr = 2 * n;
+    if (nr <= 0) return;
+    dim3 grid(unsigned((nr + 255) / 256));
+    if (in_prec == BFGPU_C128 && out_prec == BFGPU_C64)
+        hipLaunchKernelGGL((convert_kernel<double, float>), grid, dim3(256), 0, s,
+                           (const double *)in, (float *)out, nr);
+    else if (in_prec == BFGPU_C64 && out_prec == BFGPU_C128)
+        hipLaunchKernelGGL((convert_kernel<float, double>), grid, dim3(256), 0, s,
+                           (const float *)in, (double *)out, nr);
+    else
+        BFG_HIP(hipMemcpyAsync(out, in, size_t(n) * (in_prec == BFGPU_C128 ? 16 : 8),
+                               hipMemcpyDeviceToDevice, s));
+    BFG_HIP(hipGetLastError());
+}
+
+}  // namespace bfg

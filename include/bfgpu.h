@@ -1,0 +1,218 @@
+/*
+ * bfgpu.h -- C ABI of the MI355X (gfx950) butterfly far-field library.
+ *
+ * The boundary replaces the Julia call sites of ButterflyFactorizations.jl
+ * listed beside each entry point.  All entry points return an int status
+ * (BFGPU_OK = 0); on failure bfgpu_last_error() returns a message for the
+ * calling thread.  Indices are 0-based.  Complex data is interleaved
+ * (re, im) in either complex FP32 (precision 1) or complex FP64 (precision 2).
+ * Matrices are column-major.  Host buffers are borrowed for the duration of
+ * a call; device memory belongs to the handle that allocated it.  All GPU
+ * work of a context runs on one HIP stream; calls on one context must be
+ * serialised by the caller.
+ */
+#ifndef BFGPU_H
+#define BFGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BFGPU_OK = 0,
+    BFGPU_ERR_ARG = 1,      /* invalid argument (bad size, null handle, ...) */
+    BFGPU_ERR_HIP = 2,      /* a HIP runtime call failed */
+    BFGPU_ERR_NOMEM = 3,    /* device or host allocation failed */
+    BFGPU_ERR_STATE = 4,    /* object in the wrong state for the call */
+    BFGPU_ERR_NUMERIC = 5   /* numerical failure (e.g. singular R11) */
+};
+
+enum { BFGPU_C64 = 1, BFGPU_C128 = 2 };                        /* precision */
+enum { BFGPU_OP_N = 0, BFGPU_OP_T = 1, BFGPU_OP_C = 2 };       /* y = op(A) x */
+enum { BFGPU_TREE_BALANCED = 0, BFGPU_TREE_MAXPOINTS = 1 };    /* tree split rule */
+enum { BFGPU_RANK_BUTTERFLY = 0, BFGPU_RANK_GEOMETRIC = 1, BFGPU_RANK_LEARNED = 2 };
+enum { BFGPU_XY_HOST = 0, BFGPU_XY_DEVICE = 1 };               /* matvec flags */
+
+typedef struct bfgpu_ctx bfgpu_ctx;
+typedef struct bfgpu_tree bfgpu_tree;
+typedef struct bfgpu_bf bfgpu_bf;
+typedef struct bfgpu_mlp bfgpu_mlp;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *bfgpu_last_error(void);
+/* Library version string. */
+const char *bfgpu_version(void);
+
+/* ---- context (device + stream) ---- */
+int bfgpu_ctx_create(int device, bfgpu_ctx **out);
+int bfgpu_ctx_destroy(bfgpu_ctx *ctx);
+/* Use an external stream (e.g. torch's current stream); NULL restores the
+ * context's own stream. */
+int bfgpu_ctx_set_stream(bfgpu_ctx *ctx, void *hip_stream);
+int bfgpu_ctx_get_stream(bfgpu_ctx *ctx, void **hip_stream);
+int bfgpu_ctx_sync(bfgpu_ctx *ctx);
+
+/* ---- cluster trees ----
+ * The tree contract of src/treeinterface.jl:7-164 (cluster_center,
+ * cluster_radius = halfsize*sqrt(3) as ext/BFH2Trees/BFH2Trees.jl:15-16,
+ * cluster_children, cluster_isleaf, cluster_root = node 0, cluster_values,
+ * cluster_parent).  Node ids are breadth-first; children are contiguous;
+ * values(node) = perm[val_begin[node] .. val_end[node]). */
+typedef struct {
+    int32_t n_nodes;
+    int32_t n_values;
+    const int32_t *parent;
+    const int32_t *first_child;
+    const int32_t *n_children;
+    const int32_t *val_begin;
+    const int32_t *val_end;
+    const double *center;   /* 3 per node */
+    const double *halfsize; /* per node */
+    const int32_t *perm;    /* n_values */
+    const int32_t *level;   /* depth per node */
+    const double *points;   /* 3 per value, original point order */
+} bfgpu_tree_view;
+
+/* Build a 2^dims-ary box tree (dims 2: quadtree splitting x,y; dims 3:
+ * octree) over n points (xyz, 3 per point).  BALANCED subdivides every node
+ * to depth ceil(log_{2^dims}(n / leaf_size)); MAXPOINTS subdivides while a
+ * node holds more than leaf_size points.  Empty children are not created.
+ * Stands in for the H2Trees TwoNTree the reference's tests build. */
+int bfgpu_tree_build(const double *xyz, int64_t n, int32_t dims, int32_t leaf_size,
+                     int32_t mode, bfgpu_tree **out);
+/* Adopt a tree built elsewhere (e.g. exported from H2Trees); arrays copied. */
+int bfgpu_tree_create(const double *xyz, int32_t n_values, int32_t n_nodes,
+                      const int32_t *parent, const int32_t *first_child,
+                      const int32_t *n_children, const int32_t *val_begin,
+                      const int32_t *val_end, const double *center, const double *halfsize,
+                      const int32_t *perm, bfgpu_tree **out);
+int bfgpu_tree_view_get(const bfgpu_tree *t, bfgpu_tree_view *out);
+int bfgpu_tree_destroy(bfgpu_tree *t);
+
+/* ---- interaction lists ----
+ * nearandfar (src/Matrixassembly/intlists.jl:30-217) with isFarFunctor
+ * (kind 0, param alpha; ext/BFH2Trees/h2admissible.jl:11-40) or
+ * CenterDistanceAdmissibility (kind 1, param beta; :74-91).  far/near
+ * receive (observer node, source node) pairs; pass NULL to count only. */
+int bfgpu_nearandfar(const bfgpu_tree *test, const bfgpu_tree *trial, int32_t kind,
+                     double param, int32_t minbflvl, int32_t unbalancedints,
+                     int32_t leafcomp, int32_t leafimbalance, int32_t *far, int64_t *nfar,
+                     int32_t *near, int64_t *nnear);
+
+/* ---- factorization ---- */
+typedef struct {
+    double k;            /* wavenumber of G(x,y) = exp(-ik|x-y|)/(4 pi |x-y|) */
+    double tol;          /* tau; each compression uses tau / L (bfassembly.jl:74) */
+    int32_t adaptive;    /* adaptive row doubling (compressors.jl:134-151) */
+    int32_t rank_mode;   /* BFGPU_RANK_* */
+    double c_tau;        /* ButterflyRankEstimator constant (default 4.0) */
+    int32_t rmin;        /* estimator floor (default 10) */
+    double c_geo, c_eps; /* GeometricRankEstimator constants */
+    int32_t precision;   /* BFGPU_C64 or BFGPU_C128 */
+    int32_t oversample;  /* rows added to a learned rank prediction */
+    uint64_t seed;       /* row-sample seed */
+    const bfgpu_mlp *mlp;/* rank predictor for BFGPU_RANK_LEARNED */
+} bfgpu_opts;
+
+/* assemble_BF (src/ButterflyFactorization/bfassembly.jl:48-200) for n_pairs
+ * admissible blocks (observer root, source root) at once; n_pairs = 1 is
+ * the single-block constructor, several pairs form the far field of
+ * PetrovGalerkinBF (src/Matrixassembly/petrovgalerkinbf.jl:121-161).
+ * Kernel assembly, pivoted QR and interpolation matrices run on the GPU. */
+int bfgpu_factorize(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
+                    int32_t n_pairs, const int32_t *pairs, const bfgpu_opts *opts,
+                    bfgpu_bf **out);
+
+/* Build a device factorization from blocks in the reference layout
+ * (ButterflyFactorization, bfstructs.jl:21-147): table rows are (stage,
+ * obs_out, src_out, obs_in, src_in, m, n, identity, out_ptr, in_ptr) with
+ * stage 0 = Q, 1..n_stages-2 = R levels, n_stages-1 = P; data_ofs gives the
+ * first complex128 entry of each block's column-major data (-1 identity).
+ * Used to apply factorizations built elsewhere. */
+int bfgpu_bf_from_host(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
+                       int32_t n_stages, int64_t n_blocks, const int32_t *table,
+                       const int64_t *data_ofs, const double *data, int32_t precision,
+                       bfgpu_bf **out);
+
+typedef struct {
+    int32_t n_bf;          /* factorizations held */
+    int32_t precision;
+    int64_t rows, cols;    /* size(BF) (bfdims.jl:54-64); far field: tree sizes */
+    int32_t max_stages;
+    int64_t n_blocks;      /* reference-layout blocks over all factorizations */
+    int64_t n_data;        /* complex entries of factor data */
+    int64_t factor_bytes;  /* device bytes of packed factors */
+    int64_t slot_len;      /* skeleton-vector entries per right-hand side */
+    int64_t n_comp;        /* compressions performed */
+    int64_t n_retry;       /* adaptive doublings performed */
+    double t_factor_ms;    /* wall time of the factorization call */
+} bfgpu_bf_info;
+int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
+
+/* Export factorization `index` in the reference layout (complex128 data);
+ * sizes from bfgpu_bf_counts. */
+int bfgpu_bf_counts(const bfgpu_bf *bf, int32_t index, int32_t *n_stages, int64_t *n_blocks,
+                    int64_t *n_data, int64_t *n_skel, int64_t *n_skel_idx, int64_t *n_comp,
+                    int32_t roots[2]);
+int bfgpu_bf_export(const bfgpu_bf *bf, int32_t index, int32_t *table, int64_t *data_ofs,
+                    double *data);
+/* Skeleton records (stage, snode, onode, ofs, len) + point indices, and
+ * compression records (stage, snode, onode, guess, rows, rank). */
+int bfgpu_bf_export_logs(const bfgpu_bf *bf, int32_t index, int32_t *skel_rec,
+                         int32_t *skel_idx, int32_t *comp_rec);
+/* Per-stage statistics of the packed tables: blocks, identity blocks, max
+ * rank, factor bytes (show, src/showfcts.jl:81-124). 4 int64 per stage. */
+int bfgpu_bf_stage_stats(const bfgpu_bf *bf, int32_t *n_stages, int64_t *stats);
+
+/* Convert the factor storage to another precision (e.g. FP64 -> FP32). */
+int bfgpu_bf_convert(bfgpu_bf *bf, int32_t precision, bfgpu_bf **out);
+
+/* y = alpha * op(A) * x + beta * y for nrhs right-hand sides
+ * (mul!, src/Butterflyalgebra/bfvector.jl:200-296; bfmatrix.jl:10-50;
+ * adjoint/transpose of bfadjtr.jl:43-89 applied without materialising;
+ * far loop of src/matrixalgebra/matrixvector.jl:39-62).  x, y have the
+ * factorization's precision; flags BFGPU_XY_DEVICE means x and y are device
+ * pointers (work is enqueued on the context stream), otherwise host
+ * pointers (the call copies and synchronises).  alpha, beta: complex (2
+ * doubles). */
+int bfgpu_matvec(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, int64_t ldx, void *y,
+                 int64_t ldy, const double *alpha, const double *beta, int32_t flags);
+
+/* Same as bfgpu_matvec with device x/y, repeated `iters` times with HIP
+ * events around every stage launch on the context stream; stage_ms (length
+ * n_stages) receives the mean duration per stage, total_ms the mean per
+ * matvec; stage_bytes receives the algorithmic bytes per stage. */
+int bfgpu_matvec_timed(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
+                       void *y, int64_t ldy, int32_t iters, double *stage_ms,
+                       int64_t *stage_bytes, int32_t *n_stages, double *total_ms);
+
+int bfgpu_bf_destroy(bfgpu_bf *bf);
+
+/* ---- learned rank predictor (MLP 6 -> H -> H -> H -> 1, ReLU, FP32) ---- */
+int bfgpu_mlp_create(bfgpu_ctx *ctx, int32_t n_in, int32_t hidden, int32_t n_layers,
+                     uint64_t seed, bfgpu_mlp **out);
+int bfgpu_mlp_destroy(bfgpu_mlp *m);
+/* Parameters as one flat FP32 host array (W1,b1,...,WL,bL; W row-major
+ * out x in). */
+int bfgpu_mlp_n_params(const bfgpu_mlp *m, int64_t *n);
+int bfgpu_mlp_get_params(const bfgpu_mlp *m, float *params);
+int bfgpu_mlp_set_params(bfgpu_mlp *m, const float *params);
+/* Input standardisation x' = (x - mu) / sigma (n_in each) and output map
+ * rank = y * out_scale + out_shift; targets of loss_grad/train are ranks. */
+int bfgpu_mlp_set_norm(bfgpu_mlp *m, const float *mu, const float *sigma, float out_shift,
+                       float out_scale);
+/* Forward for n samples (features n x n_in row-major, host) -> out (n). */
+int bfgpu_mlp_forward(bfgpu_mlp *m, int64_t n, const float *features, float *out);
+/* Loss (mean squared error) and gradient w.r.t. the parameters for one
+ * batch (host arrays); grads has n_params entries. */
+int bfgpu_mlp_loss_grad(bfgpu_mlp *m, int64_t n, const float *features, const float *target,
+                        float *loss, float *grads);
+/* Train with Adam for `epochs` passes over the data in minibatches. */
+int bfgpu_mlp_train(bfgpu_mlp *m, int64_t n, const float *features, const float *target,
+                    int32_t epochs, int32_t batch, float lr, uint64_t seed, float *final_loss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

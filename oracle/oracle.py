@@ -1,0 +1,283 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the CPU-baseline leg of bench.py, always as the checker.  The product
+package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+i32, i64, u64, f64, vp = C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_void_p
+P = C.POINTER
+
+
+class Tree(C.Structure):
+    _fields_ = [("n_nodes", i32), ("n_values", i32), ("parent", P(i32)), ("first_child", P(i32)),
+                ("n_children", P(i32)), ("val_begin", P(i32)), ("val_end", P(i32)),
+                ("center", P(f64)), ("halfsize", P(f64)), ("perm", P(i32))]
+
+
+class Opts(C.Structure):
+    _fields_ = [("k", f64), ("tol", f64), ("adaptive", i32), ("rank_mode", i32), ("c_tau", f64),
+                ("rmin", i32), ("c_geo", f64), ("c_eps", f64), ("seed", u64),
+                ("fixed_guess", P(i32)), ("n_fixed", i32)]
+
+
+_SIGS = {
+    "bfo_helmholtz_fill": (None, [f64, P(f64), P(i32), i32, P(f64), P(i32), i32, P(f64)]),
+    "bfo_rank_butterfly": (i32, [f64, f64, i32, P(f64), P(f64)]),
+    "bfo_rank_geometric": (i32, [f64, P(f64), P(f64), f64, f64, f64, f64, f64, i32, P(f64), P(f64)]),
+    "bfo_mix64": (u64, [u64]),
+    "bfo_sample_key": (u64, [u64, i32, i32, i32, i32, i32]),
+    "bfo_sample_prefix": (None, [u64, i32, i32, P(i32)]),
+    "bfo_cpqr": (i32, [i32, i32, P(f64), f64, P(i32), P(f64)]),
+    "bfo_interp": (None, [i32, i32, P(f64), i32, P(i32), P(f64)]),
+    "bfo_compress": (i32, [f64, P(f64), P(i32), i32, P(f64), P(i32), i32, i32, f64, i32, u64,
+                           P(f64), P(i32), P(i32)]),
+    "bfo_assemble": (vp, [P(Tree), P(f64), P(Tree), P(f64), i32, i32, P(Opts)]),
+    "bfo_bf_free": (None, [vp]),
+    "bfo_bf_transform": (vp, [vp, i32]),
+    "bfo_bf_n_stages": (i32, [vp]),
+    "bfo_bf_n_blocks": (i64, [vp]),
+    "bfo_bf_n_data": (i64, [vp]),
+    "bfo_bf_export": (None, [vp, P(i32), P(i64), P(f64)]),
+    "bfo_bf_n_skel": (i64, [vp]),
+    "bfo_bf_n_skel_idx": (i64, [vp]),
+    "bfo_bf_export_skel": (None, [vp, P(i32), P(i32)]),
+    "bfo_bf_n_comp": (i64, [vp]),
+    "bfo_bf_export_comp": (None, [vp, P(i32)]),
+    "bfo_bf_dims": (None, [vp, P(i64), P(i64)]),
+    "bfo_bf_import": (vp, [P(Tree), P(Tree), i32, i64, P(i32), P(i64), P(f64)]),
+    "bfo_matvec": (None, [vp, i64, P(f64), i64, P(f64), i64, P(f64), P(f64)]),
+    "bfo_matvec_many": (None, [P(vp), i32, i64, P(f64), i64, P(f64), i64, P(f64), P(f64), i32]),
+    "bfo_isfar_aabb": (i32, [P(Tree), P(Tree), i32, i32, f64]),
+    "bfo_center_distance": (i32, [P(Tree), P(Tree), i32, i32, f64]),
+    "bfo_nearandfar": (None, [P(Tree), P(Tree), i32, f64, i32, i32, i32, i32, P(i32), P(i64),
+                              P(i32), P(i64)]),
+}
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        h = C.CDLL(LIB)
+        for n, (r, a) in _SIGS.items():
+            f = getattr(h, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = h
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(P(t))
+
+
+def _dp(a):
+    """double* view of a complex128 (or float64) array."""
+    assert a.dtype in (np.complex128, np.float64) and (a.flags.c_contiguous or a.flags.f_contiguous)
+    return a.ctypes.data_as(P(f64))
+
+
+class OTree:
+    """Oracle view of a cluster tree (arrays kept alive here)."""
+
+    def __init__(self, t):
+        self._arrs = [np.ascontiguousarray(getattr(t, n)) for n in
+                      ("parent", "first_child", "n_children", "val_begin", "val_end")]
+        self._center = np.ascontiguousarray(t.center, np.float64).reshape(-1)
+        self._half = np.ascontiguousarray(t.halfsize, np.float64)
+        self._perm = np.ascontiguousarray(t.perm, np.int32)
+        self.points = np.ascontiguousarray(t.points, np.float64)
+        self.s = Tree(t.n_nodes, t.n_values, *[_p(a, i32) for a in self._arrs],
+                      _p(self._center, f64), _p(self._half, f64), _p(self._perm, i32))
+        self.src = t
+
+
+def helmholtz(k, obs_pts, rows, src_pts, cols):
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    Z = np.zeros((len(rows), len(cols)), np.complex128, order="F")
+    lib().bfo_helmholtz_fill(k, _p(np.ascontiguousarray(obs_pts, np.float64), f64), _p(rows, i32),
+                             len(rows), _p(np.ascontiguousarray(src_pts, np.float64), f64),
+                             _p(cols, i32), len(cols), _dp(Z))
+    return Z
+
+
+def rank_butterfly(eps, c_tau=4.0, rmin=10):
+    x1, x2 = f64(), f64()
+    r = lib().bfo_rank_butterfly(eps, c_tau, rmin, C.byref(x1), C.byref(x2))
+    return r, x1.value, x2.value
+
+
+def rank_geometric(k, cs, co, a_s, a_o, eps, C_=1.5, Ceps=1.5, rmin=3):
+    cs = np.ascontiguousarray(cs, np.float64)
+    co = np.ascontiguousarray(co, np.float64)
+    x1, x2 = f64(), f64()
+    r = lib().bfo_rank_geometric(k, _p(cs, f64), _p(co, f64), a_s, a_o, eps, C_, Ceps, rmin,
+                                 C.byref(x1), C.byref(x2))
+    return r, x1.value, x2.value
+
+
+def sample_key(seed, no, ns, stage, s, o):
+    return lib().bfo_sample_key(seed, no, ns, stage, s, o)
+
+
+def sample_prefix(key, n, m):
+    out = np.zeros(min(m, n), np.int32)
+    lib().bfo_sample_prefix(key, n, m, _p(out, i32))
+    return out
+
+
+def cpqr(A, rtol=0.0):
+    """Pivoted QR of A (complex, m x n).  Returns (rank, piv, R-overwritten A, tau)."""
+    A = np.array(A, dtype=np.complex128, order="F", copy=True)
+    m, n = A.shape
+    piv = np.zeros(n, np.int32)
+    tau = np.zeros(max(min(m, n), 1), np.complex128)
+    r = lib().bfo_cpqr(m, n, _dp(A), rtol, _p(piv, i32), _dp(tau))
+    return r, piv, A, tau
+
+
+def interp(R, rank, piv):
+    R = np.asfortranarray(R, dtype=np.complex128)
+    m, n = R.shape
+    T = np.zeros((rank, n), np.complex128, order="F")
+    lib().bfo_interp(m, n, _dp(R), rank, _p(np.ascontiguousarray(piv, np.int32), i32), _dp(T))
+    return T
+
+
+def compress(k, obs_pts, obs_idx, src_pts, src_idx, guess, eps, adaptive, key):
+    obs_idx = np.ascontiguousarray(obs_idx, np.int32)
+    src_idx = np.ascontiguousarray(src_idx, np.int32)
+    ns = len(src_idx)
+    T = np.zeros(max(ns * ns, 1), np.complex128)
+    skel = np.zeros(max(ns, 1), np.int32)
+    rows = i32()
+    r = lib().bfo_compress(k, _p(np.ascontiguousarray(obs_pts, np.float64), f64), _p(obs_idx, i32),
+                           len(obs_idx), _p(np.ascontiguousarray(src_pts, np.float64), f64),
+                           _p(src_idx, i32), ns, guess, eps, int(adaptive), key, _dp(T),
+                           _p(skel, i32), C.byref(rows))
+    return T[: r * ns].reshape((r, ns), order="F"), skel[:r], r, rows.value
+
+
+class OBF:
+    """An oracle factorization (owns the C object)."""
+
+    def __init__(self, h, otest, otrial):
+        self.h = h
+        self.otest, self.otrial = otest, otrial
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bfo_bf_free(self.h)
+            self.h = None
+
+    def dims(self):
+        r, c = i64(), i64()
+        lib().bfo_bf_dims(self.h, C.byref(r), C.byref(c))
+        return r.value, c.value
+
+    def export(self):
+        nb = lib().bfo_bf_n_blocks(self.h)
+        nd = lib().bfo_bf_n_data(self.h)
+        table = np.zeros((nb, 10), np.int32)
+        ofs = np.zeros(nb, np.int64)
+        data = np.zeros(max(nd, 1), np.complex128)
+        lib().bfo_bf_export(self.h, _p(table, i32), _p(ofs, i64), _dp(data))
+        nsk = lib().bfo_bf_n_skel(self.h)
+        nski = lib().bfo_bf_n_skel_idx(self.h)
+        skel = np.zeros((nsk, 5), np.int32)
+        skel_idx = np.zeros(max(nski, 1), np.int32)
+        lib().bfo_bf_export_skel(self.h, _p(skel, i32), _p(skel_idx, i32))
+        nc = lib().bfo_bf_n_comp(self.h)
+        comp = np.zeros((nc, 6), np.int32)
+        lib().bfo_bf_export_comp(self.h, _p(comp, i32))
+        return {"n_stages": lib().bfo_bf_n_stages(self.h), "table": table, "data_ofs": ofs,
+                "data": data[:nd], "skel": skel, "skel_idx": skel_idx[:nski], "comp": comp}
+
+    def matvec(self, x, alpha=1.0, beta=0.0, y=None):
+        rows, cols = self.dims()
+        x = np.asfortranarray(x, np.complex128)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        if y is None:
+            y = np.zeros((rows,) if x.ndim == 1 else (rows, nrhs), np.complex128, order="F")
+        else:
+            y = np.array(y, np.complex128, order="F", copy=True)
+        a = (f64 * 2)(np.real(alpha), np.imag(alpha))
+        b = (f64 * 2)(np.real(beta), np.imag(beta))
+        lib().bfo_matvec(self.h, nrhs, _dp(x), x.shape[0], _dp(y), y.shape[0], a, b)
+        return y
+
+    def transform(self, conj):
+        return OBF(lib().bfo_bf_transform(self.h, int(conj)), self.otrial, self.otest)
+
+    def dense(self):
+        rows, cols = self.dims()
+        return self.matvec(np.eye(cols, dtype=np.complex128))
+
+
+def assemble(test, trial, no, ns, k, tol, adaptive=True, rank_mode=0, c_tau=4.0, rmin=10,
+             c_geo=1.5, c_eps=1.5, seed=0, fixed=None):
+    """Oracle assemble_BF over product trees (bfgpu.ClusterTree)."""
+    ot, os_ = OTree(test), OTree(trial)
+    fx = np.ascontiguousarray(fixed if fixed is not None else np.zeros(1), np.int32)
+    o = Opts(k, tol, int(adaptive), rank_mode, c_tau, rmin, c_geo, c_eps, seed, _p(fx, i32),
+             len(fixed) if fixed is not None else 0)
+    h = lib().bfo_assemble(C.byref(ot.s), _p(ot.points, f64), C.byref(os_.s), _p(os_.points, f64),
+                           no, ns, C.byref(o))
+    bf = OBF(h, ot, os_)
+    bf._fx = fx
+    return bf
+
+
+def import_bf(test, trial, ex):
+    ot, os_ = OTree(test), OTree(trial)
+    t = np.ascontiguousarray(ex["table"], np.int32)
+    o = np.ascontiguousarray(ex["data_ofs"], np.int64)
+    d = np.ascontiguousarray(ex["data"], np.complex128)
+    if d.size == 0:
+        d = np.zeros(1, np.complex128)
+    h = lib().bfo_bf_import(C.byref(ot.s), C.byref(os_.s), ex["n_stages"], t.shape[0], _p(t, i32),
+                            _p(o, i64), _dp(d))
+    return OBF(h, ot, os_)
+
+
+def matvec_many(bfs, x, nthreads=1, rows=None):
+    rows = max([b.dims()[0] for b in bfs] + [rows or 0])
+    x = np.asfortranarray(x, np.complex128)
+    nrhs = 1 if x.ndim == 1 else x.shape[1]
+    y = np.zeros((rows,) if x.ndim == 1 else (rows, nrhs), np.complex128, order="F")
+    arr = (vp * len(bfs))(*[b.h for b in bfs])
+    a = (f64 * 2)(1.0, 0.0)
+    b = (f64 * 2)(0.0, 0.0)
+    lib().bfo_matvec_many(arr, len(bfs), nrhs, _dp(x), x.shape[0], _dp(y), y.shape[0], a, b, nthreads)
+    return y
+
+
+def nearandfar(test, trial, kind, param, minbflvl=3, unbalancedints=False, leafcomp=True,
+               leafimbalance=True):
+    ot, os_ = OTree(test), OTree(trial)
+    nf, nn = i64(), i64()
+    args = (C.byref(os_.s), C.byref(ot.s), kind, param, minbflvl, int(unbalancedints),
+            int(leafcomp), int(leafimbalance))
+    lib().bfo_nearandfar(*args, None, C.byref(nf), None, C.byref(nn))
+    far = np.zeros((nf.value, 2), np.int32)
+    near = np.zeros((nn.value, 2), np.int32)
+    lib().bfo_nearandfar(*args, _p(far, i32), C.byref(nf), _p(near, i32), C.byref(nn))
+    return far, near

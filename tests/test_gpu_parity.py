@@ -1,0 +1,213 @@
+"""GPU parity of the HIP path against the CPU FP64 oracle.
+
+Bars (written in each test): FP64 matvecs agree with the oracle's matvec on
+the same factorization to 1e-12 relative; the FP64 GPU factorization
+reproduces the oracle's compressions (row counts, ranks, skeleton index
+sets) exactly and its factors to 1e-9; FP32 results stay within the
+compression tolerance of dense FP64 products.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import crandn, fibonacci_sphere, two_squares
+
+pytestmark = pytest.mark.gpu
+
+K = 2 * np.pi
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def ctx(bfgpu):
+    return bfgpu.default_context()
+
+
+def trees(bfgpu, obs, src, dims=2, lo=64, ls=64, mo="balanced", ms="balanced"):
+    To = bfgpu.ClusterTree(obs, dims=dims, leaf_size=lo, mode=mo)
+    Ts = bfgpu.ClusterTree(src, dims=dims, leaf_size=ls, mode=ms)
+    return bfgpu.BlockTree(To, Ts)
+
+
+def test_from_host_fp64_matvec_matches_oracle(bfgpu, oracle, ctx):
+    say("from_host fp64 N=1024")
+    obs, src = two_squares(1024, seed=1)
+    bt = trees(bfgpu, obs, src)
+    obf = oracle.assemble(bt.testcluster, bt.trialcluster, 0, 0, K, 1e-3)
+    ex = obf.export()
+    g = bfgpu.ButterflyFactorization.from_blocks(bt, ex["n_stages"], ex["table"], ex["data_ofs"],
+                                                  ex["data"], ctx=ctx)
+    rng = np.random.default_rng(0)
+    x = crandn(rng, 1024)
+    y = g @ x
+    assert rel(y, obf.matvec(x)) < 1e-12
+    # adjoint / transpose without materialisation vs the oracle's copies
+    assert rel(g.adjoint() @ x, obf.transform(1).matvec(x)) < 1e-12
+    assert rel(g.transpose() @ x, obf.transform(0).matvec(x)) < 1e-12
+    # alpha, beta and several right-hand sides
+    X = np.asfortranarray(crandn(rng, 1024, 3))
+    Y0 = np.asfortranarray(crandn(rng, 1024, 3))
+    Y = Y0.copy(order="F")
+    g.mul(Y, X, 2 - 1j, 0.5j)
+    ref = obf.matvec(X, alpha=2 - 1j, beta=0.5j, y=Y0)
+    assert rel(Y, ref) < 1e-12
+    # device tensors
+    import torch
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.zeros(1024, dtype=torch.complex128, device="cuda")
+    g.mul(yt, xt)
+    torch.cuda.synchronize()
+    assert rel(yt.cpu().numpy(), obf.matvec(x)) < 1e-12
+    # argument errors surface as exceptions
+    with pytest.raises(bfgpu.ArgumentError):
+        g.mul(np.zeros(10, np.complex128), x)
+    with pytest.raises(bfgpu.ArgumentError):
+        g.mul(np.zeros(1024, np.complex64), x)
+
+
+def _compare_factorizations(gex, oex, T_tol=1e-9):
+    np.testing.assert_array_equal(gex["comp"], oex["comp"])          # stage, S, O, guess, rows, rank
+    np.testing.assert_array_equal(gex["skel"], oex["skel"])
+    np.testing.assert_array_equal(gex["skel_idx"], oex["skel_idx"])  # skeleton index sets
+    np.testing.assert_array_equal(gex["table"], oex["table"])        # keys, sizes, slot ids
+    assert rel(gex["data"], oex["data"]) < T_tol
+
+
+@pytest.mark.parametrize("n,tol", [(1024, 1e-3), (4096, 1e-4)])
+def test_gpu_factorization_fp64_matches_oracle(bfgpu, oracle, ctx, n, tol):
+    say("factorize fp64 two squares", n, tol)
+    obs, src = two_squares(n, seed=0)
+    bt = trees(bfgpu, obs, src)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, tol, ctx=ctx)
+    obf = oracle.assemble(bt.testcluster, bt.trialcluster, 0, 0, K, tol)
+    gex, oex = g.export(), obf.export()
+    _compare_factorizations(gex, oex)
+    # matvec parity on the GPU's own factorization
+    imp = oracle.import_bf(bt.testcluster, bt.trialcluster, gex)
+    rng = np.random.default_rng(2)
+    x = crandn(rng, n)
+    y = g @ x
+    assert rel(y, imp.matvec(x)) < 1e-12
+    assert rel(g.adjoint() @ x, imp.transform(1).matvec(x)) < 1e-12
+    # accuracy against the dense kernel (test_subroutines.jl:174-177)
+    A = oracle.helmholtz(K, obs, np.arange(n), src, np.arange(n))
+    assert rel(y, A @ x) < 10 * tol
+
+
+@pytest.mark.parametrize("case", ["deep_obs", "shallow_obs", "identity"])
+def test_gpu_factorization_unbalanced(bfgpu, oracle, ctx, case):
+    say("factorize fp64 unbalanced", case)
+    if case == "identity":
+        rng = np.random.default_rng(11)
+        n, nd = 2000, 1700
+        obs = np.c_[np.r_[rng.uniform(0, 16, nd), rng.uniform(16, 24, n - nd)],
+                    np.r_[rng.uniform(0, 16, nd), rng.uniform(0, 8, n - nd)], np.zeros(n)]
+        src = np.c_[rng.uniform(0, 16, n), rng.uniform(0, 16, n), np.full(n, 32.0)]
+        bt = trees(bfgpu, obs, src, lo=32, ls=128, mo="maxpoints", ms="maxpoints")
+    else:
+        obs, src = two_squares(2048, seed=7)
+        lo, ls = (32, 512) if case == "deep_obs" else (512, 32)
+        bt = trees(bfgpu, obs, src, lo=lo, ls=ls)
+    tol = 1e-4
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, tol, ctx=ctx)
+    obf = oracle.assemble(bt.testcluster, bt.trialcluster, 0, 0, K, tol)
+    gex, oex = g.export(), obf.export()
+    _compare_factorizations(gex, oex)
+    if case == "identity":
+        assert gex["table"][:, 7].sum() > 0
+    x = crandn(np.random.default_rng(3), len(src))
+    imp = oracle.import_bf(bt.testcluster, bt.trialcluster, gex)
+    assert rel(g @ x, imp.matvec(x)) < 1e-12
+    assert rel(g.transpose() @ x[: len(obs)], imp.transform(0).matvec(x[: len(obs)])) < 1e-12
+
+
+def test_fp32_paths(bfgpu, oracle, ctx):
+    say("fp32 factorization and conversion N=4096")
+    n, tol = 4096, 1e-4
+    obs, src = two_squares(n, seed=0)
+    bt = trees(bfgpu, obs, src)
+    A = oracle.helmholtz(K, obs, np.arange(n), src, np.arange(n))
+    x = crandn(np.random.default_rng(4), n)
+    ye = A @ x
+    g64 = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, tol, ctx=ctx)
+    y64 = g64 @ x
+    # FP32 application of the FP64 factors: rounding-level difference
+    g32c = g64.convert(np.complex64)
+    y32c = (g32c @ x.astype(np.complex64)).astype(np.complex128)
+    assert rel(y32c, y64) < 1e-5
+    # FP32 factorization: within the compression tolerance of the dense product
+    g32 = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, tol, acctype=np.complex64,
+                            ctx=ctx)
+    y32 = (g32 @ x.astype(np.complex64)).astype(np.complex128)
+    assert rel(y32, ye) < 10 * tol
+    assert rel(y64, ye) < 10 * tol
+
+
+def test_farfield_operator_sphere(bfgpu, oracle, ctx):
+    say("far field sphere N=4000 fp64")
+    pts = fibonacci_sphere(4000, 3.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    bt = bfgpu.BlockTree(T, T)
+    tol = 1e-4
+    adm = bfgpu.CenterDistanceAdmissibility(2.0)
+    op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=tol, admissibility=adm,
+                                ctx=ctx)
+    far = op.farinteractions
+    assert op.n_factorizations() == far.shape[0] > 10
+    c_tau = bfgpu.tree_parameters(bt, adm).Ctau
+    obfs, imps = [], []
+    for i, (o, s) in enumerate(far):
+        gex = op.export(i)
+        ob = oracle.assemble(T, T, int(o), int(s), K, tol, c_tau=c_tau)
+        _compare_factorizations(gex, ob.export())
+        obfs.append(ob)
+        imps.append(oracle.import_bf(T, T, gex))
+    x = crandn(np.random.default_rng(5), 4000)
+    y = op @ x
+    assert rel(y, oracle.matvec_many(imps, x, rows=4000)) < 1e-12
+    # far-only dense product (farmatrix, matrixstructs.jl:50-61)
+    A = oracle.helmholtz(K, pts, np.arange(4000), pts, np.arange(4000))
+    mask = np.zeros_like(A, dtype=bool)
+    for o, s in far:
+        mask[np.ix_(T.cluster_values(o), T.cluster_values(s))] = True
+    Af = np.where(mask, A, 0)
+    assert rel(y, Af @ x) < 10 * tol
+    # adjoint of the far operator
+    assert rel(op.adjoint() @ x, Af.conj().T @ x) < 10 * tol
+
+
+def test_single_block_65536_properties(bfgpu, oracle, ctx):
+    """The two-squares block at 65,536 points per square: FP64 GPU matvec vs
+    the oracle's matvec of the same factorization (1e-12), error against 512
+    sampled dense rows (10 tol), the adjoint identity, FP32 1- and 64-RHS."""
+    say("two squares N=65536 fp64 + fp32")
+    n, tol = 65536, 1e-4
+    obs, src = two_squares(n, seed=0)
+    bt = trees(bfgpu, obs, src)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, tol, ctx=ctx)
+    rng = np.random.default_rng(6)
+    x = crandn(rng, n)
+    y = g @ x
+    imp = oracle.import_bf(bt.testcluster, bt.trialcluster, g.export())
+    assert rel(y, imp.matvec(x)) < 1e-12
+    rows = np.sort(rng.choice(n, 512, replace=False))
+    A = oracle.helmholtz(K, obs, rows, src, np.arange(n))
+    assert rel(y[rows], A @ x) < 10 * tol
+    yy = crandn(rng, n)
+    lhs, rhs = np.vdot(yy, y), np.vdot(g.adjoint() @ yy, x)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    g32 = g.convert(np.complex64)
+    X = np.asfortranarray(crandn(rng, n, 64)).astype(np.complex64)
+    Y = g32 @ X
+    Y64 = g @ X.astype(np.complex128)
+    assert rel(Y.astype(np.complex128), Y64) < 1e-5
+    assert rel(Y[:, 0].astype(np.complex128), (g32 @ X[:, 0]).astype(np.complex128)) < 1e-6

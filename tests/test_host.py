@@ -1,0 +1,109 @@
+"""Host-side checks of the C-ABI library that need no GPU: symbol exports,
+cluster trees (the tree contract of src/treeinterface.jl), interaction lists
+and argument errors."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, fibonacci_sphere, two_squares
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "bfgpu.h")).read()
+    return sorted(set(re.findall(r"\b(bfgpu_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(bfgpu):
+    from bfgpu import _lib
+    h = _lib.lib()
+    declared = header_symbols()
+    assert len(declared) > 30
+    for name in declared:
+        assert hasattr(h, name), name
+    assert set(declared) == set(_lib.EXPORTED)
+    assert b"gfx950" in h.bfgpu_version()
+
+
+def _tree_invariants(T, leaf, mode):
+    n = T.n_values
+    assert sorted(T.perm.tolist()) == list(range(n))
+    assert T.parent[0] == -1 and T.val_begin[0] == 0 and T.val_end[0] == n
+    for v in range(T.n_nodes):
+        ch = T.cluster_children(v)
+        if ch:
+            assert all(T.parent[c] == v for c in ch)
+            assert T.val_begin[ch[0]] == T.val_begin[v] and T.val_end[ch[-1]] == T.val_end[v]
+            for a, b in zip(ch[:-1], ch[1:]):
+                assert T.val_end[a] == T.val_begin[b]
+            assert all(T.val_end[c] > T.val_begin[c] for c in ch)
+        pts = T.points[T.cluster_values(v)]
+        assert np.all(np.abs(pts[:, :2] - T.center[v, :2]) <= T.halfsize[v] * (1 + 1e-9))
+    leaves = T.leaves()
+    if mode == "maxpoints":
+        assert all(T.val_end[v] - T.val_begin[v] <= leaf for v in leaves)
+    else:
+        assert len(set(T.level[leaves].tolist())) == 1
+
+
+@pytest.mark.parametrize("mode", ["balanced", "maxpoints"])
+def test_quadtree_and_octree_contract(bfgpu, mode):
+    obs, _ = two_squares(4096)
+    T = bfgpu.ClusterTree(obs, dims=2, leaf_size=64, mode=mode)
+    _tree_invariants(T, 64, mode)
+    if mode == "balanced":
+        assert len(T.cluster_levels()) == 4 and len(T.leaves()) == 64
+    S = bfgpu.ClusterTree(fibonacci_sphere(5000, 3.0), dims=3, leaf_size=64, mode=mode)
+    _tree_invariants(S, 64, mode)
+    assert S.cluster_radius(0) == pytest.approx(S.halfsize[0] * np.sqrt(3))
+
+
+def test_tree_from_arrays_round_trip(bfgpu):
+    obs, _ = two_squares(1000, seed=3)
+    T = bfgpu.ClusterTree(obs, dims=2, leaf_size=50, mode="maxpoints")
+    U = bfgpu.ClusterTree.from_arrays(T.points, T.parent, T.first_child, T.n_children,
+                                      T.val_begin, T.val_end, T.center, T.halfsize, T.perm)
+    for name in ("parent", "first_child", "n_children", "val_begin", "val_end", "perm", "level"):
+        np.testing.assert_array_equal(getattr(T, name), getattr(U, name))
+    bad = T.parent.copy()
+    bad[1] = 5
+    with pytest.raises(bfgpu.ArgumentError):
+        bfgpu.ClusterTree.from_arrays(T.points, bad, T.first_child, T.n_children, T.val_begin,
+                                      T.val_end, T.center, T.halfsize, T.perm)
+
+
+def test_nearandfar_matches_oracle(bfgpu, oracle):
+    T = bfgpu.ClusterTree(fibonacci_sphere(6000, 4.0), dims=3, leaf_size=64, mode="maxpoints")
+    bt = bfgpu.BlockTree(T, T)
+    for adm, kind, p in [(bfgpu.isFarFunctor(1.0), 0, 1.0),
+                         (bfgpu.CenterDistanceAdmissibility(2.0), 1, 2.0)]:
+        for unb in (False, True):
+            far, near = bfgpu.nearandfar(bt, adm, unbalancedints=unb)
+            ofar, onear = oracle.nearandfar(T, T, kind, p, unbalancedints=unb)
+            np.testing.assert_array_equal(far, ofar)
+            np.testing.assert_array_equal(near, onear)
+    far, near = bfgpu.nearandfar(bt, bfgpu.isFarFunctor(1.0), leafcomp=False)
+    ofar, onear = oracle.nearandfar(T, T, 0, 1.0, leafcomp=False)
+    np.testing.assert_array_equal(far, ofar)
+
+
+def test_argument_errors(bfgpu):
+    with pytest.raises(bfgpu.ArgumentError):
+        bfgpu.ClusterTree(np.zeros((10, 2)))
+    from bfgpu import _lib
+    h = C.c_void_p()
+    pts = np.zeros((4, 3))
+    st = _lib.lib().bfgpu_tree_build(_lib.ptr(pts, C.c_double), 4, 5, 64, 0, C.byref(h))
+    assert st == 1 and b"dims" in _lib.lib().bfgpu_last_error()
+    st = _lib.lib().bfgpu_matvec(None, 0, 1, None, 1, None, 1, None, None, 0)
+    assert st == 1
+    assert _lib.lib().bfgpu_factorize(None, None, None, 1, None, None, C.byref(h)) == 1
+
+
+def test_tree_parameters_and_estimator_defaults(bfgpu):
+    assert bfgpu.tree_parameters().Ctau == 4.0
+    assert bfgpu.tree_parameters(None, bfgpu.isFarFunctor(1.0)).Ctau == pytest.approx(2.9592)
+    assert bfgpu.tree_parameters(None, bfgpu.CenterDistanceAdmissibility(2.0)).Ctau == pytest.approx(2.9352)
+    assert bfgpu.ButterflyRankEstimator().Rmin == 10
