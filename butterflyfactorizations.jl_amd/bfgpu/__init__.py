@@ -458,6 +458,24 @@ class ButterflyFactorization:
                 "skel": skel, "skel_idx": skel_idx[:nski.value], "comp": comp,
                 "roots": (roots[0], roots[1])}
 
+    def compression_log(self, index=None):
+        """(records, features) of every compression: records are (stage, S,
+        O, guess, rows, rank) as the reference's RankLogger would log them
+        (compressors.jl:52-69); features feed the learned predictor."""
+        idx = range(self.n_factorizations()) if index is None else [index]
+        recs, feats = [], []
+        for i in idx:
+            nc = C.c_int64()
+            check(lib().bfgpu_bf_counts(self.handle, i, None, None, None, None, None,
+                                        C.byref(nc), None))
+            comp = np.zeros((nc.value, 6), np.int32)
+            check(lib().bfgpu_bf_export_logs(self.handle, i, None, None, ptr(comp, C.c_int32)))
+            f = np.zeros((max(nc.value, 1), 6), np.float32)
+            check(lib().bfgpu_bf_export_features(self.handle, i, ptr(f, C.c_float)))
+            recs.append(comp)
+            feats.append(f[: nc.value])
+        return np.concatenate(recs), np.concatenate(feats)
+
     def stage_stats(self):
         """Per launch stage: blocks, identity blocks, max rank, factor bytes
         (the per-level table of src/showfcts.jl:81-124)."""

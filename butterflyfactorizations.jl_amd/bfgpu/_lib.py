@@ -78,6 +78,7 @@ _SIGS = {
     "bfgpu_bf_counts": (i32, [vp, i32, P(i32), P(i64), P(i64), P(i64), P(i64), P(i64), P(i32)]),
     "bfgpu_bf_export": (i32, [vp, i32, P(i32), P(i64), P(f64)]),
     "bfgpu_bf_export_logs": (i32, [vp, i32, P(i32), P(i32), P(i32)]),
+    "bfgpu_bf_export_features": (i32, [vp, i32, P(f32)]),
     "bfgpu_bf_stage_stats": (i32, [vp, P(i32), P(i64)]),
     "bfgpu_bf_convert": (i32, [vp, i32, P(vp)]),
     "bfgpu_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
@@ -103,6 +104,13 @@ def lib():
     """Load libbfgpu.so once; raise if it has not been built."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process: torch ships its own libamdhip64 under
+        # a different file name, so load torch first (when present) and let
+        # libbfgpu bind to the runtime it brought in.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"libbfgpu.so not found at {LIB_PATH}; build it with "

@@ -276,8 +276,10 @@ extern "C" int bfgpu_bf_export(const bfgpu_bf *h, int32_t index, int32_t *table,
     const OneBF &ob = bf.bfs[index];
     const int32_t nst = int32_t(ob.st.size()), Lmax = bf.n_stage_launch - 1;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
-    std::vector<std::vector<double>> host(bf.n_stage_launch);
-    if (data) {
+    // factors never change after construction: download once, keep on host
+    std::vector<std::vector<double>> &host = bf.export_cache;
+    if (data && host.empty()) {
+        host.resize(bf.n_stage_launch);
         BFG_HIP(hipStreamSynchronize(bf.ctx->stream));
         for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
             int64_t ne = bf.factor_elems[L];
@@ -336,6 +338,16 @@ extern "C" int bfgpu_bf_export_logs(const bfgpu_bf *h, int32_t index, int32_t *s
             int32_t *o = comp_rec + 6 * i;
             o[0] = r.stage; o[1] = r.s; o[2] = r.o; o[3] = r.guess; o[4] = r.rows; o[5] = r.rank;
         }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_export_features(const bfgpu_bf *h, int32_t index, float *feat) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && feat, "bf_export_features: null argument");
+    BFG_REQUIRE(index >= 0 && index < int32_t(h->b.bfs.size()), "bf_export_features: index out of range");
+    const OneBF &ob = h->b.bfs[index];
+    for (size_t i = 0; i < ob.comp.size(); ++i)
+        for (int q = 0; q < 6; ++q) feat[6 * i + q] = ob.comp[i].feat[q];
     BFG_API_END
 }
 

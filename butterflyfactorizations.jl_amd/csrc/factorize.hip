@@ -343,7 +343,9 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         ob.skel_idx.insert(ob.skel_idx.end(), v.begin(), v.end());
     };
     auto log_comp = [&](const Comp &c) {
-        bf.bfs[c.bf].comp.push_back(CompRec{c.stage, c.S, c.O, c.guess, c.rows, c.rank});
+        CompRec r{c.stage, c.S, c.O, c.guess, c.rows, c.rank, {}};
+        for (int q = 0; q < 6; ++q) r.feat[q] = c.feat[q];
+        bf.bfs[c.bf].comp.push_back(r);
     };
 
     // ---- stage 0: Q (bfassembly.jl:82-102) ----

@@ -91,7 +91,7 @@ struct Block {
 };
 
 struct SkelRec { int32_t stage, s, o, ofs, len; };
-struct CompRec { int32_t stage, s, o, guess, rows, rank; };
+struct CompRec { int32_t stage, s, o, guess, rows, rank; float feat[6]; };
 
 // One stage of one factorization in reference layout.
 struct StageBlocks { std::vector<Block> b; };
@@ -151,6 +151,7 @@ struct BF {
     DevBuf slots;                 // slot buffer (slot_len * nrhs)
     size_t slots_cap = 0;
     DevBuf xy_host_stage;         // staging for host x / y
+    mutable std::vector<std::vector<double>> export_cache;  // host copy of factors (export)
     int64_t n_comp = 0, n_retry = 0;
     double t_factor_ms = 0.0;
 };

@@ -146,7 +146,10 @@ __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__rest
     const int mn = m < n ? m : n;
     int rank = mn;
     double r11 = 0.0;
-    const double tol3z = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON)
+    // recompute threshold sqrt(eps) of the precision the columns are stored
+    // in (LAPACK xLAMCH('E') = 2^-53 / 2^-24): FP32 columns lose accuracy in
+    // the downdate long before FP64 ones do
+    const double tol3z = sizeof(R) == 8 ? 1.0536712127723509e-08 : 2.44140625e-04;
     for (int k = 0; k < mn; ++k) {
         // pivot: first maximal vn1 over [k, n)
         double bv = -1.0;

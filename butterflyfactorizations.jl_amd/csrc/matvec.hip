@@ -31,13 +31,15 @@ template <class T> void upload_tab(DevBuf &d, const std::vector<T> &h) {
 
 // merge adjacent contributions that read consecutive inputs with
 // consecutive column slices of one column-major block
+// (only within one item: out[first..] holds this item's contributions)
 void merge_contribs(std::vector<MvContrib> &c, int32_t b, int32_t e, std::vector<MvContrib> &out) {
+    const size_t first = out.size();
     for (int32_t i = b; i < e; ++i) {
         const MvContrib &x = c[i];
-        if (!out.empty()) {
+        if (out.size() > first) {
             MvContrib &p = out.back();
             if (p.fofs >= 0 && x.fofs >= 0 && p.m == x.m && p.fofs + int64_t(p.m) * p.n == x.fofs &&
-                p.in_ofs + p.n == x.in_ofs && int64_t(out.size()) > 0) {
+                p.in_ofs + p.n == x.in_ofs) {
                 p.n += x.n;
                 continue;
             }

@@ -161,6 +161,11 @@ int bfgpu_bf_export(const bfgpu_bf *bf, int32_t index, int32_t *table, int64_t *
  * compression records (stage, snode, onode, guess, rows, rank). */
 int bfgpu_bf_export_logs(const bfgpu_bf *bf, int32_t index, int32_t *skel_rec,
                          int32_t *skel_idx, int32_t *comp_rec);
+/* Features of each compression record (6 floats, same order as comp_rec):
+ * source and observer radii in wavelengths, centre distance in
+ * wavelengths, |cos| of the separation angle to z, level, log10(1/eps).
+ * These are the inputs of the learned rank predictor. */
+int bfgpu_bf_export_features(const bfgpu_bf *bf, int32_t index, float *feat);
 /* Per-stage statistics of the packed tables: blocks, identity blocks, max
  * rank, factor bytes (show, src/showfcts.jl:81-124). 4 int64 per stage. */
 int bfgpu_bf_stage_stats(const bfgpu_bf *bf, int32_t *n_stages, int64_t *stats);

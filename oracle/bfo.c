@@ -208,7 +208,7 @@ int32_t bfo_cpqr(int32_t m, int32_t n, double *Ad, double rtol, int32_t *piv, do
     int32_t mn = m < n ? m : n;
     double *vn1 = (double *)xmalloc(sizeof(double) * (n > 0 ? n : 1));
     double *vn2 = (double *)xmalloc(sizeof(double) * (n > 0 ? n : 1));
-    const double tol3z = sqrt(DBL_EPSILON);
+    const double tol3z = sqrt(DBL_EPSILON / 2); /* sqrt(DLAMCH('E')) */
     for (int32_t j = 0; j < n; ++j) {
         piv[j] = j;
         vn1[j] = cnrm2(m, A + (int64_t)j * m);

@@ -82,7 +82,7 @@ def _compare_factorizations(gex, oex, T_tol=1e-9):
     assert rel(gex["data"], oex["data"]) < T_tol
 
 
-@pytest.mark.parametrize("n,tol", [(1024, 1e-3), (4096, 1e-4)])
+@pytest.mark.parametrize("n,tol", [(1024, 1e-4), (4096, 1e-4)])
 def test_gpu_factorization_fp64_matches_oracle(bfgpu, oracle, ctx, n, tol):
     say("factorize fp64 two squares", n, tol)
     obs, src = two_squares(n, seed=0)
@@ -165,11 +165,16 @@ def test_farfield_operator_sphere(bfgpu, oracle, ctx):
     assert op.n_factorizations() == far.shape[0] > 10
     c_tau = bfgpu.tree_parameters(bt, adm).Ctau
     obfs, imps = [], []
+    say("far pairs", len(far))
+    check = set(np.random.default_rng(0).choice(len(far), min(len(far), 1500), replace=False).tolist())
+    check |= set(range(0, len(far), max(1, len(far) // 50)))
     for i, (o, s) in enumerate(far):
+        if i % 5000 == 0:
+            say("comparing block", i)
         gex = op.export(i)
-        ob = oracle.assemble(T, T, int(o), int(s), K, tol, c_tau=c_tau)
-        _compare_factorizations(gex, ob.export())
-        obfs.append(ob)
+        if i in check:  # factorization parity on a sample of the blocks
+            ob = oracle.assemble(T, T, int(o), int(s), K, tol, c_tau=c_tau)
+            _compare_factorizations(gex, ob.export())
         imps.append(oracle.import_bf(T, T, gex))
     x = crandn(np.random.default_rng(5), 4000)
     y = op @ x
