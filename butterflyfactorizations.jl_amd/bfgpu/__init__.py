@@ -55,10 +55,8 @@ class Context:
         check(lib().bfgpu_ctx_sync(self._h))
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().bfgpu_ctx_destroy(h)
-            self._h = None
+        _destroy("bfgpu_ctx_destroy", getattr(self, "_h", None))
+        self._h = None
 
 
 _DEFAULT = {}
@@ -161,10 +159,8 @@ class ClusterTree:
         return np.nonzero(self.n_children == 0)[0]
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().bfgpu_tree_destroy(h)
-            self._h = None
+        _destroy("bfgpu_tree_destroy", getattr(self, "_h", None))
+        self._h = None
 
 
 class BlockTree:
@@ -285,6 +281,15 @@ def _opts(k, tol, rankestimator, adaptive, acctype, seed):
 _OPNAME = {L.OP_N: "N", L.OP_T: "T", L.OP_C: "C"}
 
 
+def _destroy(fn, h):
+    """Release a library handle; quiet during interpreter shutdown."""
+    try:
+        if h is not None and h.value:
+            getattr(lib(), fn)(h)
+    except Exception:
+        pass
+
+
 def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
@@ -296,9 +301,8 @@ class _Handle:
         self.h, self.ctx, self.keep = h, ctx, keep
 
     def __del__(self):
-        if self.h is not None and self.h.value:
-            lib().bfgpu_bf_destroy(self.h)
-            self.h = None
+        _destroy("bfgpu_bf_destroy", getattr(self, "h", None))
+        self.h = None
 
 
 class ButterflyFactorization:
@@ -475,6 +479,38 @@ class ButterflyFactorization:
             recs.append(comp)
             feats.append(f[: nc.value])
         return np.concatenate(recs), np.concatenate(feats)
+
+    def set_fused(self, enable=True):
+        """Fused single-launch forward sweep (default) or one launch per level."""
+        check(lib().bfgpu_bf_set_fused(self.handle, 1 if enable else 0))
+        return self
+
+    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3}
+
+    def set_option(self, key, value):
+        """Launch tuning: sweep (0 per-level, 1 fused dynamic, 2 fused
+        cooperative), wg_per_cu (fused sweep occupancy), graph (replay
+        per-level launches from a HIP graph)."""
+        check(lib().bfgpu_bf_set_option(self.handle, self.OPTIONS[key], int(value)))
+        return self
+
+    def get_option(self, key):
+        v = C.c_int32()
+        check(lib().bfgpu_bf_get_option(self.handle, self.OPTIONS[key], C.byref(v)))
+        return v.value
+
+    def autotune(self, nrhs=1, iters=10):
+        """Pick the fastest forward kernel variant of every level for `nrhs`
+        right-hand sides; returns (per-level ms, per-level variant)."""
+        n = self.info().max_stages
+        ms = (C.c_double * n)()
+        var = (C.c_int32 * n)()
+        check(lib().bfgpu_bf_autotune(self.handle, int(nrhs), int(iters), ms, var))
+        return list(ms), list(var)
+
+    def check(self):
+        """Synchronise and raise if the fused sweep flagged a fault."""
+        check(lib().bfgpu_bf_check(self.handle))
 
     def stage_stats(self):
         """Per launch stage: blocks, identity blocks, max rank, factor bytes
@@ -662,7 +698,5 @@ class RankPredictor:
         return loss.value
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().bfgpu_mlp_destroy(h)
-            self._h = None
+        _destroy("bfgpu_mlp_destroy", getattr(self, "_h", None))
+        self._h = None

@@ -83,6 +83,11 @@ _SIGS = {
     "bfgpu_bf_convert": (i32, [vp, i32, P(vp)]),
     "bfgpu_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
     "bfgpu_matvec_timed": (i32, [vp, i32, i64, vp, i64, vp, i64, i32, P(f64), P(i64), P(i32), P(f64)]),
+    "bfgpu_bf_set_fused": (i32, [vp, i32]),
+    "bfgpu_bf_set_option": (i32, [vp, i32, i32]),
+    "bfgpu_bf_get_option": (i32, [vp, i32, P(i32)]),
+    "bfgpu_bf_check": (i32, [vp]),
+    "bfgpu_bf_autotune": (i32, [vp, i64, i32, P(f64), P(i32)]),
     "bfgpu_bf_destroy": (i32, [vp]),
     "bfgpu_mlp_create": (i32, [vp, i32, i32, i32, u64, P(vp)]),
     "bfgpu_mlp_destroy": (i32, [vp]),

@@ -37,6 +37,9 @@ void matvec(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, i
 void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                   int64_t ldy, int32_t iters, double *stage_ms, int64_t *stage_bytes,
                   int32_t *n_stages, double *total_ms);
+void check_sweep_error(BF &bf);
+void release_graph(BF &bf);
+void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *variants);
 }  // namespace bfg
 
 using namespace bfg;
@@ -436,6 +439,65 @@ extern "C" int bfgpu_matvec_timed(bfgpu_bf *h, int32_t op, int64_t nrhs, const v
     BFG_REQUIRE(ldx >= (fwd ? h->b.cols : h->b.rows) && ldy >= (fwd ? h->b.rows : h->b.cols),
                 "matvec_timed: leading dimension too small");
     matvec_timed(h->b, op, nrhs, x, ldx, y, ldy, iters, stage_ms, stage_bytes, n_stages, total_ms);
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_set_fused(bfgpu_bf *h, int32_t enable) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "bf_set_fused: null handle");
+    h->b.use_sweep = enable != 0;
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "bf_set_option: null handle");
+    BF &bf = h->b;
+    switch (key) {
+    case BFGPU_OPT_SWEEP:
+        BFG_REQUIRE(value >= 0 && value <= 2, "bf_set_option: sweep mode 0..2");
+        bf.use_sweep = value != 0;
+        if (value) bf.sweep.mode = value == 1 ? 0 : 1;
+        break;
+    case BFGPU_OPT_WG_PER_CU:
+        BFG_REQUIRE(value >= 0 && value <= 32, "bf_set_option: workgroups per CU 0..32");
+        bf.sweep.wg_per_cu = value;
+        break;
+    case BFGPU_OPT_GRAPH:
+        bf.use_graph = value != 0;
+        release_graph(bf);
+        break;
+    default:
+        BFG_REQUIRE(false, "bf_set_option: unknown key");
+    }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *value) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && value, "bf_get_option: null argument");
+    const BF &bf = h->b;
+    switch (key) {
+    case BFGPU_OPT_SWEEP: *value = bf.use_sweep ? (bf.sweep.mode == 0 ? 1 : 2) : 0; break;
+    case BFGPU_OPT_WG_PER_CU: *value = bf.sweep.wg_per_cu; break;
+    case BFGPU_OPT_GRAPH: *value = bf.use_graph ? 1 : 0; break;
+    default: BFG_REQUIRE(false, "bf_get_option: unknown key");
+    }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_autotune(bfgpu_bf *h, int64_t nrhs, int32_t iters, double *stage_ms,
+                                 int32_t *variants) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "bf_autotune: null handle");
+    autotune(h->b, nrhs, iters, stage_ms, variants);
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_check(bfgpu_bf *h) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "bf_check: null handle");
+    check_sweep_error(h->b);
     BFG_API_END
 }
 

@@ -119,6 +119,17 @@ struct MvContrib {
     int32_t pad;
 };
 
+// Forward task: an item with its first contribution inline (one descriptor
+// load per workgroup); contributions [c_begin, c_end) follow in contribs.
+struct MvTask {
+    int64_t fofs;      // first contribution
+    int32_t m, n, in_ofs;
+    int32_t out_ofs, out_len;
+    int32_t c_begin, c_end;
+    int32_t wait_grp, wait_cnt, sig_grp;  // dependency counters (fused sweep)
+    int32_t pad;
+};
+
 enum InKind { IN_SLOT = 0, IN_GATHER = 1 };
 enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
 
@@ -128,9 +139,21 @@ struct StageTable {
     int32_t max_in = 0, max_out = 0;   // longest input / output vector
     int64_t alg_bytes_1 = 0;           // algorithmic bytes per RHS (factor bytes separate)
     int64_t factor_elems = 0;          // factor entries read
-    DevBuf items, contribs;
+    int32_t variant = 0;               // forward kernel: 0 workgroup / 1 wavefront per task
+    DevBuf items, contribs, tasks;
     std::vector<MvItem> h_items;
     std::vector<MvContrib> h_contribs;
+    std::vector<MvTask> h_tasks;
+};
+
+constexpr int SWEEP_MAX_STAGES = 48;
+struct SweepTables {
+    bool ok = false;
+    int32_t n_tasks = 0, n_groups = 0, max_wait = 1;
+    DevBuf tasks, contribs, counters, err;  // counters: n_groups + ticket pair
+    int32_t epoch = 0;
+    int32_t mode = 0;       // 0: dynamic tickets, ordinary launch; 1: cooperative, static
+    int32_t wg_per_cu = 0;  // workgroups per CU (0: occupancy limit)
 };
 
 struct BF {
@@ -147,6 +170,12 @@ struct BF {
     bool y_covered = true;        // forward P covers every row of y
     bool x_covered = true;        // transposed Q covers every column
     std::vector<StageTable> fwd, tr;
+    SweepTables sweep;            // fused forward sweep
+    bool use_sweep = false;       // measured slower than per-level launches; opt-in
+    bool use_graph = false;       // replay per-level launches from a HIP graph
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
+    ~BF();
     DevBuf perm_test, perm_trial;
     DevBuf slots;                 // slot buffer (slot_len * nrhs)
     size_t slots_cap = 0;
@@ -190,6 +219,14 @@ void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const vo
                   const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
                   int64_t slot_len, const int32_t *perm_in, const int32_t *perm_out,
                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta);
+// Fused forward sweep: every stage in one cooperative launch, tasks ordered
+// by stage, dependencies through per-group counters.
+void launch_sweep(hipStream_t s, int prec, const SweepTables &sw, const std::vector<DevBuf> &factors,
+                  const void *x, int64_t ldx, void *y, int64_t ldy, void *slots, int64_t slot_len,
+                  const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs, const double *alpha,
+                  const double *beta, bool fuse_beta, int n_cu);
+void launch_zero_i32(hipStream_t s, int32_t *p, int64_t n);
+
 void launch_scale(hipStream_t s, int prec, void *y, int64_t n, int64_t ldy, int64_t nrhs,
                   const double *beta);
 void launch_convert(hipStream_t s, const void *in, int in_prec, void *out, int out_prec,

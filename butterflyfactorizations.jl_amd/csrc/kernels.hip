@@ -370,6 +370,519 @@ void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tas
 }
 
 // ---------------------------------------------------------------------------
+// Forward matvec stage (y = A x), one workgroup per (task, group of NR
+// right-hand sides).  A task is an output vector (a slot, or rows of y
+// reached through a permutation) with its first contribution inline and
+// further contributions in `contribs`.  Thread t works on row i = t % rows
+// and column group g = t / rows (G = 256 / rows groups); group g owns
+// columns g, g + G, ...  For one column index q the active threads read G
+// consecutive columns of the column-major block -- one contiguous range --
+// and the factor loads of a pass are issued together with the input loads,
+// so a task costs one memory round trip per pass.  Partial sums meet in LDS
+// and are added in fixed group order (deterministic).
+// ---------------------------------------------------------------------------
+template <typename R, int INK, int OUTK, int NR>
+__global__ __launch_bounds__(256) void fwd_kernel(
+    const MvTask *__restrict__ tasks, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, const C<R> *__restrict__ x, int64_t ldx, C<R> *__restrict__ y,
+    int64_t ldy, C<R> *__restrict__ slots, int64_t slot_len, const int32_t *__restrict__ perm_in,
+    const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta, int fuse_beta, int nrhs) {
+    constexpr int QMAX = NR <= 2 ? 8 : 4;
+    __shared__ C<R> red[NR][256];
+    const MvTask tk = tasks[blockIdx.x];
+    const int t = threadIdx.x;
+    const int rb = blockIdx.y * NR;
+    const int nr = min(NR, nrhs - rb);
+
+    auto in_val = [&](int r, int idx) -> C<R> {
+        if (r >= nr) return C<R>{R(0), R(0)};
+        if (INK == IN_GATHER) return x[int64_t(rb + r) * ldx + perm_in[idx]];
+        return slots[int64_t(rb + r) * slot_len + idx];
+    };
+
+    for (int row0 = 0; row0 < tk.out_len; row0 += 256) {
+        const int rows = min(256, tk.out_len - row0);
+        const int G = 256 / rows;
+        const int i = t % rows, g = t / rows;
+        const bool act = g < G;
+        C<R> acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] = C<R>{R(0), R(0)};
+        for (int ci = tk.c_begin - 1; ci < tk.c_end; ++ci) {
+            MvContrib c;
+            if (ci < tk.c_begin) c = MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
+            else c = contribs[ci];
+            if (c.fofs < 0) {  // identity block
+                if (g == 0)
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) acc[r] = cadd(acc[r], in_val(r, c.in_ofs + row0 + i));
+                continue;
+            }
+            const C<R> *A = F + c.fofs + row0 + i;
+            for (int jb = g; jb < c.n; jb += G * QMAX) {
+                C<R> a[QMAX], xv[NR][QMAX];
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q) {
+                    const int j = jb + q * G;
+                    const bool ok = act && j < c.n;
+                    a[q] = ok ? A[int64_t(j) * c.m] : C<R>{R(0), R(0)};
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) xv[r][q] = ok ? in_val(r, c.in_ofs + j) : C<R>{R(0), R(0)};
+                }
+#pragma unroll
+                for (int q = 0; q < QMAX; ++q)
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) cfma(acc[r], a[q], xv[r][q]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r) red[r][t] = acc[r];
+        __syncthreads();
+        if (t < rows) {
+            for (int r = 0; r < nr; ++r) {
+                C<R> s = red[r][t];
+                for (int q = 1; q < G; ++q) s = cadd(s, red[r][q * rows + t]);
+                if (OUTK == OUT_SLOT) {
+                    slots[int64_t(rb + r) * slot_len + tk.out_ofs + row0 + t] = s;
+                } else {
+                    C<R> *yp = y + int64_t(rb + r) * ldy + perm_out[tk.out_ofs + row0 + t];
+                    C<R> old = *yp;
+                    *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, s));
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward stage, one wavefront per task (4 tasks per workgroup), for stages
+// of small blocks (leaf-level Q/P blocks of a few hundred entries).  Rows
+// are padded to rp = 2^ceil(log2 rows) <= 64 lanes; lane = g * rp + i works
+// on row i and column group g of G = 64 / rp groups, so one load
+// instruction of the wave reads G consecutive columns (contiguous).  Taller
+// blocks are walked in 64-row chunks with G = 1.  Groups are summed with
+// xor-shuffles (no LDS, no barriers).
+// ---------------------------------------------------------------------------
+template <typename R, int INK, int OUTK, int NR>
+__global__ __launch_bounds__(256) void fwdw_kernel(
+    const MvTask *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, const C<R> *__restrict__ x, int64_t ldx, C<R> *__restrict__ y,
+    int64_t ldy, C<R> *__restrict__ slots, int64_t slot_len, const int32_t *__restrict__ perm_in,
+    const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta, int fuse_beta, int nrhs) {
+    constexpr int QW = NR <= 2 ? 8 : 4;
+    const int lane = threadIdx.x & 63;
+    const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (task >= n_tasks) return;  // whole wave leaves; no barriers below
+    const MvTask tk = tasks[task];
+    const int rb = blockIdx.y * NR;
+    const int nr = min(NR, nrhs - rb);
+    auto in_val = [&](int r, int idx) -> C<R> {
+        if (r >= nr) return C<R>{R(0), R(0)};
+        if (INK == IN_GATHER) return x[int64_t(rb + r) * ldx + perm_in[idx]];
+        return slots[int64_t(rb + r) * slot_len + idx];
+    };
+    for (int row0 = 0; row0 < tk.out_len; row0 += 64) {
+        const int rows = min(64, tk.out_len - row0);
+        int rp = 1;
+        while (rp < rows) rp <<= 1;
+        const int G = 64 / rp;
+        const int i = lane & (rp - 1), g = lane / rp;
+        const bool rowok = i < rows;
+        C<R> acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) acc[r] = C<R>{R(0), R(0)};
+        for (int ci = tk.c_begin - 1; ci < tk.c_end; ++ci) {
+            MvContrib c;
+            if (ci < tk.c_begin) c = MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
+            else c = contribs[ci];
+            if (c.fofs < 0) {
+                if (g == 0 && rowok)
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) acc[r] = cadd(acc[r], in_val(r, c.in_ofs + row0 + i));
+                continue;
+            }
+            const C<R> *A = F + c.fofs + row0 + i;
+            for (int p0 = 0; p0 < c.n; p0 += G * QW) {
+                C<R> a[QW], xv[NR][QW];
+#pragma unroll
+                for (int q = 0; q < QW; ++q) {
+                    const int j = p0 + g + q * G;
+                    const bool ok = rowok && j < c.n;
+                    a[q] = ok ? A[int64_t(j) * c.m] : C<R>{R(0), R(0)};
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) xv[r][q] = ok ? in_val(r, c.in_ofs + j) : C<R>{R(0), R(0)};
+                }
+#pragma unroll
+                for (int q = 0; q < QW; ++q)
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) cfma(acc[r], a[q], xv[r][q]);
+            }
+        }
+        for (int s = rp; s < 64; s <<= 1)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                acc[r].x += __shfl_xor(acc[r].x, s, 64);
+                acc[r].y += __shfl_xor(acc[r].y, s, 64);
+            }
+        if (g == 0 && rowok) {
+            for (int r = 0; r < nr; ++r) {
+                if (OUTK == OUT_SLOT) {
+                    slots[int64_t(rb + r) * slot_len + tk.out_ofs + row0 + i] = acc[r];
+                } else {
+                    C<R> *yp = y + int64_t(rb + r) * ldy + perm_out[tk.out_ofs + row0 + i];
+                    C<R> old = *yp;
+                    *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc[r]));
+                }
+            }
+        }
+    }
+}
+
+template <typename R, int NR>
+static void launch_fwdw_t(hipStream_t s, const StageTable &t, const void *factors, const void *x,
+                          int64_t ldx, void *y, int64_t ldy, void *slots, int64_t slot_len,
+                          const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs,
+                          const double *alpha, const double *beta, bool fuse) {
+    C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
+    dim3 grid(unsigned((t.n_items + 3) / 4), unsigned((nrhs + NR - 1) / NR));
+    auto F = (const C<R> *)factors;
+    auto X = (const C<R> *)x;
+    auto Y = (C<R> *)y;
+    auto S = (C<R> *)slots;
+    auto T = t.tasks.as<const MvTask>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_FWDW(INK, OUTK)                                                                       \
+    hipLaunchKernelGGL((fwdw_kernel<R, INK, OUTK, NR>), grid, dim3(256), 0, s, T, t.n_items, P, F, X, \
+                       ldx, Y, ldy, S, slot_len, perm_in, perm_out, a, b, int(fuse), int(nrhs))
+    if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_FWDW(IN_SLOT, OUT_SLOT);
+    else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_FWDW(IN_GATHER, OUT_SLOT);
+    else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_FWDW(IN_SLOT, OUT_SCATTER);
+    else BFG_FWDW(IN_GATHER, OUT_SCATTER);
+#undef BFG_FWDW
+    BFG_HIP(hipGetLastError());
+}
+
+template <typename R, int NR>
+static void launch_fwd_t(hipStream_t s, const StageTable &t, const void *factors, const void *x,
+                         int64_t ldx, void *y, int64_t ldy, void *slots, int64_t slot_len,
+                         const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs,
+                         const double *alpha, const double *beta, bool fuse) {
+    C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
+    dim3 grid(unsigned(t.n_items), unsigned((nrhs + NR - 1) / NR));
+    auto F = (const C<R> *)factors;
+    auto X = (const C<R> *)x;
+    auto Y = (C<R> *)y;
+    auto S = (C<R> *)slots;
+    auto T = t.tasks.as<const MvTask>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_FWD(INK, OUTK)                                                                 \
+    hipLaunchKernelGGL((fwd_kernel<R, INK, OUTK, NR>), grid, dim3(256), 0, s, T, P, F, X, ldx, \
+                       Y, ldy, S, slot_len, perm_in, perm_out, a, b, int(fuse), int(nrhs))
+    if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_FWD(IN_SLOT, OUT_SLOT);
+    else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_FWD(IN_GATHER, OUT_SLOT);
+    else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_FWD(IN_SLOT, OUT_SCATTER);
+    else BFG_FWD(IN_GATHER, OUT_SCATTER);
+#undef BFG_FWD
+    BFG_HIP(hipGetLastError());
+}
+
+template <typename R>
+static void launch_fwd(hipStream_t s, const StageTable &t, const void *factors, const void *x,
+                       int64_t ldx, void *y, int64_t ldy, void *slots, int64_t slot_len,
+                       const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs,
+                       const double *alpha, const double *beta, bool fuse) {
+    if (t.variant == 1) {
+        if (nrhs == 1) launch_fwdw_t<R, 1>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+        else if (nrhs == 2) launch_fwdw_t<R, 2>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+        else if (nrhs <= 4) launch_fwdw_t<R, 4>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+        else launch_fwdw_t<R, 8>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+        return;
+    }
+    if (nrhs == 1) launch_fwd_t<R, 1>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+    else if (nrhs == 2) launch_fwd_t<R, 2>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+    else if (nrhs <= 4) launch_fwd_t<R, 4>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+    else launch_fwd_t<R, 8>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse);
+}
+
+// ---------------------------------------------------------------------------
+// Fused forward sweep.  All tasks of all stages (Q, R^1..R^{L-1}, P) in one
+// cooperative launch; workgroup w runs tasks w, w + W, w + 2W, ... (tasks are
+// ordered by stage, so every wait targets a task with a smaller index, and
+// all W workgroups are co-resident).  A task of stage s > 0 waits until the
+// counter of its input group has collected `wait_cnt` signals for this
+// launch's epoch; its factor loads are issued before the wait.  Skeleton
+// vectors written and read inside the launch use agent-scope (device-
+// coherent) atomic loads and stores, so producers and consumers on
+// different XCDs agree without cache flushes: a producer's stores complete
+// (s_waitcnt) before it increments the consumer's counter.  Waits are
+// bounded; an expired wait raises the error flag and the task proceeds, so
+// every wave reaches the end of the kernel.
+// ---------------------------------------------------------------------------
+struct SweepArgs {
+    const MvTask *tasks;
+    const MvContrib *contribs;
+    int32_t n_tasks, n_stages;
+    const void *F[SWEEP_MAX_STAGES];
+    const void *x;
+    int64_t ldx;
+    void *y;
+    int64_t ldy;
+    void *slots;
+    int64_t slot_len;
+    const int32_t *perm_in, *perm_out;
+    double alpha[2], beta[2];
+    int32_t fuse, nrhs;
+    int32_t *counters;
+    int32_t epoch;
+    int32_t *err;
+    int32_t *ticket;  // dynamic mode: [0] next task, [1] finished workgroups
+    int32_t dyn;
+};
+
+template <typename R> __device__ __forceinline__ C<R> ld_coh(const C<R> *p) {
+    const uint64_t *q = reinterpret_cast<const uint64_t *>(p);
+    if constexpr (sizeof(R) == 4) {
+        uint64_t v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        C<R> r;
+        __builtin_memcpy(&r, &v, 8);
+        return r;
+    } else {
+        uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        C<R> r;
+        __builtin_memcpy(&r.x, &a, 8);
+        __builtin_memcpy(&r.y, &b, 8);
+        return r;
+    }
+}
+
+template <typename R> __device__ __forceinline__ void st_coh(C<R> *p, C<R> v) {
+    uint64_t *q = reinterpret_cast<uint64_t *>(p);
+    if constexpr (sizeof(R) == 4) {
+        uint64_t u;
+        __builtin_memcpy(&u, &v, 8);
+        __hip_atomic_store(q, u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        uint64_t a, b;
+        __builtin_memcpy(&a, &v.x, 8);
+        __builtin_memcpy(&b, &v.y, 8);
+        __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <typename R, int NR>
+__global__ __launch_bounds__(256) void sweep_kernel(SweepArgs args) {
+    constexpr int QMAX = NR <= 2 ? 8 : 4;
+    constexpr int64_t SPIN_MAX = int64_t(1) << 26;
+    __shared__ C<R> red[NR][256];
+    const int t = threadIdx.x;
+    const int nr = min(NR, args.nrhs);
+    const C<R> *x = static_cast<const C<R> *>(args.x);
+    C<R> *y = static_cast<C<R> *>(args.y);
+    C<R> *slots = static_cast<C<R> *>(args.slots);
+    const C<R> alpha = {R(args.alpha[0]), R(args.alpha[1])}, beta = {R(args.beta[0]), R(args.beta[1])};
+
+    // Static mode (cooperative launch): tasks w, w + W, ...  Dynamic mode
+    // (ordinary launch): tasks are claimed in order from a ticket counter;
+    // a claimed task only waits on smaller tickets, which running
+    // workgroups hold, so no co-residency is needed.  The last workgroup to
+    // finish resets the counter for the next launch.
+    __shared__ int s_task;
+    auto claim = [&](int prev) -> int {
+        if (!args.dyn) return prev < 0 ? int(blockIdx.x) : prev + int(gridDim.x);
+        if (t == 0) s_task = __hip_atomic_fetch_add(args.ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        int v = s_task;
+        __syncthreads();
+        return v;
+    };
+    for (int task = claim(-1); task < args.n_tasks; task = claim(task)) {
+        const MvTask tk = args.tasks[task];
+        const int stage = tk.pad;
+        const bool gather = stage == 0, scatter = stage == args.n_stages - 1;
+        const C<R> *F = static_cast<const C<R> *>(args.F[stage]);
+        auto in_val = [&](int r, int idx) -> C<R> {
+            if (r >= nr) return C<R>{R(0), R(0)};
+            if (gather) return x[int64_t(r) * args.ldx + args.perm_in[idx]];
+            return ld_coh(slots + int64_t(r) * args.slot_len + idx);
+        };
+        // prefetch the first pass of the first contribution before waiting
+        const int rows0 = min(256, tk.out_len);
+        const bool pre = rows0 > 0 && tk.fofs >= 0;
+        C<R> a0[QMAX];
+        if (pre) {
+            const int G0 = 256 / rows0, i0 = t % rows0, g0 = t / rows0;
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q) {
+                const int j = g0 + q * G0;
+                a0[q] = (g0 < G0 && j < tk.n) ? F[tk.fofs + i0 + int64_t(j) * tk.m] : C<R>{R(0), R(0)};
+            }
+        }
+        if (tk.wait_grp >= 0) {  // uniform: every thread reaches the barrier
+            if (t == 0) {
+                const int32_t target = args.epoch * tk.wait_cnt;
+                int64_t spins = 0;
+                while (__hip_atomic_load(args.counters + tk.wait_grp, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > SPIN_MAX) {
+                        __hip_atomic_fetch_or(args.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        for (int row0 = 0; row0 < tk.out_len; row0 += 256) {
+            const int rows = min(256, tk.out_len - row0);
+            const int G = 256 / rows;
+            const int i = t % rows, g = t / rows;
+            const bool act = g < G;
+            C<R> acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] = C<R>{R(0), R(0)};
+            for (int ci = tk.c_begin - 1; ci < tk.c_end; ++ci) {
+                MvContrib c;
+                if (ci < tk.c_begin) c = MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
+                else c = args.contribs[ci];
+                if (c.fofs < 0) {  // identity block
+                    if (g == 0)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) acc[r] = cadd(acc[r], in_val(r, c.in_ofs + row0 + i));
+                    continue;
+                }
+                const C<R> *A = F + c.fofs + row0 + i;
+                for (int p0 = 0; p0 < c.n; p0 += G * QMAX) {
+                    C<R> a[QMAX], xv[NR][QMAX];
+                    const bool use_pre = pre && row0 == 0 && ci < tk.c_begin && p0 == 0;
+#pragma unroll
+                    for (int q = 0; q < QMAX; ++q) {
+                        const int j = p0 + g + q * G;
+                        const bool ok = act && j < c.n;
+                        a[q] = use_pre ? a0[q] : (ok ? A[int64_t(j) * c.m] : C<R>{R(0), R(0)});
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) xv[r][q] = ok ? in_val(r, c.in_ofs + j) : C<R>{R(0), R(0)};
+                    }
+#pragma unroll
+                    for (int q = 0; q < QMAX; ++q)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) cfma(acc[r], a[q], xv[r][q]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < NR; ++r) red[r][t] = acc[r];
+            __syncthreads();
+            if (t < rows) {
+                for (int r = 0; r < nr; ++r) {
+                    C<R> s = red[r][t];
+                    for (int q = 1; q < G; ++q) s = cadd(s, red[r][q * rows + t]);
+                    if (!scatter) {
+                        st_coh(slots + int64_t(r) * args.slot_len + tk.out_ofs + row0 + t, s);
+                    } else {
+                        C<R> *yp = y + int64_t(r) * args.ldy + args.perm_out[tk.out_ofs + row0 + t];
+                        C<R> old = *yp;
+                        *yp = cadd(args.fuse ? cmul(beta, old) : old, cmul(alpha, s));
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (tk.sig_grp >= 0) {
+            __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
+            __syncthreads();
+            if (t == 0)
+                __hip_atomic_fetch_add(args.counters + tk.sig_grp, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (args.dyn && t == 0) {
+        int d = __hip_atomic_fetch_add(args.ticket + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (d == int(gridDim.x) - 1) {  // every other workgroup has claimed its last ticket
+            __hip_atomic_store(args.ticket, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(args.ticket + 1, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+template <typename R, int NR>
+static void launch_sweep_t(hipStream_t s, const SweepTables &sw, const std::vector<DevBuf> &factors,
+                           const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
+                           int64_t slot_len, const int32_t *perm_in, const int32_t *perm_out,
+                           int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu) {
+    SweepArgs a{};
+    a.tasks = sw.tasks.as<const MvTask>();
+    a.contribs = sw.contribs.as<const MvContrib>();
+    a.n_tasks = sw.n_tasks;
+    a.n_stages = int32_t(factors.size());
+    BFG_REQUIRE(a.n_stages <= SWEEP_MAX_STAGES, "sweep: too many stages");
+    for (int i = 0; i < a.n_stages; ++i) a.F[i] = factors[i].p;
+    a.x = x;
+    a.ldx = ldx;
+    a.y = y;
+    a.ldy = ldy;
+    a.slots = slots;
+    a.slot_len = slot_len;
+    a.perm_in = perm_in;
+    a.perm_out = perm_out;
+    a.alpha[0] = alpha[0]; a.alpha[1] = alpha[1];
+    a.beta[0] = beta[0]; a.beta[1] = beta[1];
+    a.fuse = fuse;
+    a.nrhs = int32_t(nrhs);
+    a.counters = sw.counters.as<int32_t>();
+    a.epoch = sw.epoch;
+    a.err = sw.err.as<int32_t>();
+    a.ticket = sw.counters.as<int32_t>() + sw.n_groups;
+    a.dyn = sw.mode == 0;
+    int per_cu = 0;
+    BFG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel<R, NR>, 256, 0));
+    BFG_REQUIRE(per_cu >= 1, "sweep: kernel does not fit on a CU");
+    int grid = std::min<int64_t>(int64_t(std::min(per_cu, sw.wg_per_cu > 0 ? sw.wg_per_cu : per_cu)) * n_cu,
+                                 sw.n_tasks);
+    grid = std::max(grid, 1);
+    if (a.dyn) {
+        hipLaunchKernelGGL((sweep_kernel<R, NR>), dim3(grid), dim3(256), 0, s, a);
+        BFG_HIP(hipGetLastError());
+    } else {
+        void *kargs[] = {&a};
+        BFG_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(sweep_kernel<R, NR>), dim3(grid),
+                                           dim3(256), kargs, 0, s));
+    }
+}
+
+void launch_sweep(hipStream_t s, int prec, const SweepTables &sw, const std::vector<DevBuf> &factors,
+                  const void *x, int64_t ldx, void *y, int64_t ldy, void *slots, int64_t slot_len,
+                  const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs, const double *alpha,
+                  const double *beta, bool fuse, int n_cu) {
+    BFG_REQUIRE(nrhs >= 1 && nrhs <= 8, "sweep: 1..8 right-hand sides");
+#define BFG_SW(R, NR) launch_sweep_t<R, NR>(s, sw, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse, n_cu)
+    if (prec == BFGPU_C128) {
+        if (nrhs == 1) BFG_SW(double, 1);
+        else if (nrhs == 2) BFG_SW(double, 2);
+        else if (nrhs <= 4) BFG_SW(double, 4);
+        else BFG_SW(double, 8);
+    } else {
+        if (nrhs == 1) BFG_SW(float, 1);
+        else if (nrhs == 2) BFG_SW(float, 2);
+        else if (nrhs <= 4) BFG_SW(float, 4);
+        else BFG_SW(float, 8);
+    }
+#undef BFG_SW
+}
+
+__global__ void zero_i32_kernel(int32_t *p, int64_t n) {
+    int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_zero_i32(hipStream_t s, int32_t *p, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(zero_i32_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, p, n);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Matvec stage.  One workgroup per (item, right-hand side).  An item is an
 // output vector (slot, or rows of y reached through a permutation) that
 // receives the sum of its contributions op(A_c) in_c; in_c is a slot or a
@@ -507,11 +1020,11 @@ void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const vo
     if (t.n_items == 0 || nrhs == 0) return;
     BFG_REQUIRE(nrhs < 65536, "matvec: nrhs too large");
     if (prec == BFGPU_C128) {
-        if (op == BFGPU_OP_N) launch_stage_t<double, 0>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+        if (op == BFGPU_OP_N) launch_fwd<double>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
         else if (op == BFGPU_OP_T) launch_stage_t<double, 1>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
         else launch_stage_t<double, 2>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
     } else {
-        if (op == BFGPU_OP_N) launch_stage_t<float, 0>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
+        if (op == BFGPU_OP_N) launch_fwd<float>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
         else if (op == BFGPU_OP_T) launch_stage_t<float, 1>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
         else launch_stage_t<float, 2>(s, t, factors, x, ldx, y, ldy, slots, slot_len, perm_in, perm_out, nrhs, alpha, beta, fuse_beta);
     }

@@ -10,6 +10,7 @@
 // The adjoint and transpose (bfadjtr.jl:43-89) run the same tables backwards
 // with transposed (conjugated) block access instead of materialised copies.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <unordered_map>
 
@@ -48,6 +49,8 @@ void merge_contribs(std::vector<MvContrib> &c, int32_t b, int32_t e, std::vector
     }
 }
 }  // namespace
+
+void build_sweep(BF &bf, const std::vector<std::vector<std::vector<int32_t>>> &fin);
 
 void pack_tables(BF &bf) {
     const Tree &test = *bf.test, &trial = *bf.trial;
@@ -117,6 +120,7 @@ void pack_tables(BF &bf) {
     bf.tr.clear();
     bf.fwd.resize(nL);
     bf.tr.resize(nL);
+    std::vector<std::vector<std::vector<int32_t>>> fin(nL);
     int64_t covered_rows = 0, covered_cols = 0;
     for (int32_t L = 0; L < nL; ++L) {
         StageTable &F = bf.fwd[L], &T = bf.tr[L];
@@ -200,9 +204,26 @@ void pack_tables(BF &bf) {
             S.n_items = int32_t(S.h_items.size());
             S.n_contrib = int32_t(S.h_contribs.size());
             S.alg_bytes_1 = vec * es;
+            S.h_tasks.clear();
+            for (const MvItem &it : S.h_items) {
+                const MvContrib &c0 = S.h_contribs[it.c_begin];
+                S.h_tasks.push_back(MvTask{c0.fofs, c0.m, c0.n, c0.in_ofs, it.out_ofs, it.out_len,
+                                           it.c_begin + 1, it.c_end, -1, 0, -1, 0});
+            }
+            // default forward variant until the autotuner measures: a
+            // wavefront per task for small blocks, a workgroup otherwise
+            S.variant = (S.n_items > 0 && S.factor_elems / S.n_items < 4096) ? 1 : 0;
             upload_tab(S.items, S.h_items);
             upload_tab(S.contribs, S.h_contribs);
+            upload_tab(S.tasks, S.h_tasks);
         };
+        // input slots of every forward item (before merging), for the sweep
+        fin[L].resize(fc.size());
+        for (size_t i = 0; i < fc.size(); ++i) {
+            for (const MvContrib &c : fc[i]) fin[L][i].push_back(c.in_ofs);
+            std::sort(fin[L][i].begin(), fin[L][i].end());
+            fin[L][i].erase(std::unique(fin[L][i].begin(), fin[L][i].end()), fin[L][i].end());
+        }
         finish(F, fi, fc, true);
         finish(T, ti, tc, false);
         if (L == Lmax)
@@ -214,7 +235,96 @@ void pack_tables(BF &bf) {
     bf.x_covered = covered_cols == bf.cols;
     upload_tab(bf.perm_test, test.perm);
     upload_tab(bf.perm_trial, trial.perm);
+    build_sweep(bf, fin);
 }
+
+// Fused-sweep tables: every forward task of every stage in stage order.
+// Items that read the same set of skeleton vectors form one dependency
+// group with one counter; each produced vector is read by exactly one group
+// (butterfly data flow), so its producer signals that group's counter once.
+void build_sweep(BF &bf, const std::vector<std::vector<std::vector<int32_t>>> &fin) {
+    SweepTables &sw = bf.sweep;
+    sw.ok = false;
+    const int32_t nL = bf.n_stage_launch;
+    if (nL > SWEEP_MAX_STAGES) return;
+    std::map<std::vector<int32_t>, int32_t> gid;
+    std::unordered_map<int32_t, int32_t> consumer;  // slot offset -> group
+    std::vector<int32_t> gcnt;
+    std::vector<std::vector<int32_t>> wait(nL);
+    for (int32_t L = 1; L < nL; ++L) {
+        wait[L].resize(fin[L].size());
+        for (size_t i = 0; i < fin[L].size(); ++i) {
+            const auto &set = fin[L][i];
+            auto it = gid.find(set);
+            int32_t g;
+            if (it == gid.end()) {
+                g = int32_t(gcnt.size());
+                gid.emplace(set, g);
+                gcnt.push_back(int32_t(set.size()));
+                for (int32_t s : set) {
+                    auto c = consumer.find(s);
+                    if (c != consumer.end() && c->second != g) return;  // not a butterfly data flow
+                    consumer[s] = g;
+                }
+            } else {
+                g = it->second;
+            }
+            wait[L][i] = g;
+        }
+    }
+    std::vector<MvTask> tasks;
+    std::vector<MvContrib> contribs;
+    int32_t max_wait = 1;
+    for (int32_t L = 0; L < nL; ++L) {
+        const StageTable &S = bf.fwd[L];
+        const int32_t base = int32_t(contribs.size());
+        contribs.insert(contribs.end(), S.h_contribs.begin(), S.h_contribs.end());
+        for (size_t i = 0; i < S.h_tasks.size(); ++i) {
+            MvTask t = S.h_tasks[i];
+            t.c_begin += base;
+            t.c_end += base;
+            t.pad = L;
+            t.wait_grp = L > 0 ? wait[L][i] : -1;
+            t.wait_cnt = L > 0 ? gcnt[wait[L][i]] : 0;
+            max_wait = std::max(max_wait, t.wait_cnt);
+            t.sig_grp = -1;
+            if (L < nL - 1) {
+                auto c = consumer.find(t.out_ofs);
+                if (c != consumer.end()) t.sig_grp = c->second;
+            }
+            tasks.push_back(t);
+        }
+    }
+    sw.n_tasks = int32_t(tasks.size());
+    sw.n_groups = int32_t(gcnt.size());
+    sw.max_wait = max_wait;
+    upload_tab(sw.tasks, tasks);
+    upload_tab(sw.contribs, contribs);
+    sw.counters.alloc(sizeof(int32_t) * size_t(sw.n_groups + 2));
+    BFG_HIP(hipMemset(sw.counters.p, 0, sizeof(int32_t) * size_t(sw.n_groups + 2)));
+    sw.err.alloc(sizeof(int32_t));
+    BFG_HIP(hipMemset(sw.err.p, 0, sizeof(int32_t)));
+    sw.epoch = 0;
+    sw.ok = true;
+}
+
+void check_sweep_error(BF &bf);
+
+bool sweep_ok(const BF &bf, int64_t nrhs) {
+    return bf.use_sweep && bf.sweep.ok && nrhs >= 1 && nrhs <= 8 && bf.ctx->n_cu > 0;
+}
+
+void release_graph(BF &bf) {
+    if (bf.graph_exec) (void)hipGraphExecDestroy(bf.graph_exec);
+    bf.graph_exec = nullptr;
+    bf.graph_key.clear();
+}
+
+BF::~BF() { release_graph(*this); }
+
+static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
+                           int64_t ldy, const double *alpha, const double *beta,
+                           std::vector<hipEvent_t> *ev);
 
 static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                       int64_t ldy, const double *alpha, const double *beta,
@@ -223,9 +333,45 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
     size_t need = size_t(bf.slot_len) * size_t(nrhs) * es;
     if (need > bf.slots_cap) {
+        release_graph(bf);  // a captured graph holds the old slot buffer
         bf.slots.alloc(need);
         bf.slots_cap = need;
     }
+    const bool fused = op == BFGPU_OP_N && sweep_ok(bf, nrhs);
+    if (ev || !bf.use_graph || fused) {
+        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, ev);
+        return;
+    }
+    // per-level launches replayed from a captured graph while the call
+    // signature repeats (iterative solvers reuse their vectors)
+    auto bits = [](double v) { int64_t b; std::memcpy(&b, &v, 8); return b; };
+    std::vector<int64_t> key = {op, nrhs, int64_t(reinterpret_cast<intptr_t>(x)), ldx,
+                                int64_t(reinterpret_cast<intptr_t>(y)), ldy, bits(alpha[0]), bits(alpha[1]),
+                                bits(beta[0]), bits(beta[1]), int64_t(reinterpret_cast<intptr_t>(s))};
+    if (!bf.graph_exec || key != bf.graph_key) {
+        release_graph(bf);
+        BFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipGraph_t g = nullptr;
+        try {
+            stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        BFG_HIP(hipStreamEndCapture(s, &g));
+        hipError_t e = hipGraphInstantiate(&bf.graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BFG_HIP(e);
+        bf.graph_key = key;
+    }
+    BFG_HIP(hipGraphLaunch(bf.graph_exec, s));
+}
+
+static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
+                           int64_t ldy, const double *alpha, const double *beta,
+                           std::vector<hipEvent_t> *ev) {
+    hipStream_t s = bf.ctx->stream;
     const int32_t nL = bf.n_stage_launch;
     const bool fwd = (op == BFGPU_OP_N);
     const int64_t out_len = fwd ? bf.rows : bf.cols;
@@ -236,6 +382,18 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
     const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
     int q = 0;
     if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
+    if (fwd && sweep_ok(bf, nrhs)) {
+        SweepTables &sw = bf.sweep;
+        if (int64_t(sw.epoch + 1) * sw.max_wait >= (int64_t(1) << 30)) {
+            launch_zero_i32(s, sw.counters.as<int32_t>(), sw.n_groups);
+            sw.epoch = 0;
+        }
+        ++sw.epoch;
+        launch_sweep(s, bf.precision, sw, bf.factors, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pr, pt,
+                     nrhs, alpha, b_eff, fuse, bf.ctx->n_cu);
+        if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
+        return;
+    }
     for (int32_t i = 0; i < nL; ++i) {
         int32_t L = fwd ? i : nL - 1 - i;
         const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
@@ -273,6 +431,7 @@ void matvec(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, i
     BFG_HIP(hipMemcpy2DAsync(y, size_t(ldy) * es, dy, size_t(out_len) * es, size_t(out_len) * es,
                              size_t(nrhs), hipMemcpyDeviceToHost, s));
     BFG_HIP(hipStreamSynchronize(s));
+    check_sweep_error(bf);
 }
 
 void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
@@ -281,22 +440,25 @@ void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void
     BFG_REQUIRE(iters >= 1, "matvec_timed: iters >= 1");
     const int32_t nL = bf.n_stage_launch;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
-    std::vector<hipEvent_t> ev(nL + 1);
+    const bool fwd = (op == BFGPU_OP_N);
+    const bool fused = fwd && sweep_ok(bf, nrhs);
+    const int32_t nS = fused ? 1 : nL;  // timed launches per matvec
+    std::vector<hipEvent_t> ev(nS + 1);
     for (auto &e : ev) BFG_HIP(hipEventCreate(&e));
-    std::vector<double> acc(nL, 0.0);
+    std::vector<double> acc(nS, 0.0);
     double tot = 0.0;
     static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
     try {
         for (int32_t it = 0; it < iters; ++it) {
             stage_seq(bf, op, nrhs, x, ldx, y, ldy, one, zero, &ev);
-            BFG_HIP(hipEventSynchronize(ev[nL]));
-            for (int32_t i = 0; i < nL; ++i) {
+            BFG_HIP(hipEventSynchronize(ev[nS]));
+            for (int32_t i = 0; i < nS; ++i) {
                 float ms = 0.f;
                 BFG_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
                 acc[i] += ms;
             }
             float ms = 0.f;
-            BFG_HIP(hipEventElapsedTime(&ms, ev[0], ev[nL]));
+            BFG_HIP(hipEventElapsedTime(&ms, ev[0], ev[nS]));
             tot += ms;
         }
     } catch (...) {
@@ -304,15 +466,98 @@ void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void
         throw;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
-    const bool fwd = (op == BFGPU_OP_N);
-    for (int32_t i = 0; i < nL; ++i) {
-        int32_t L = fwd ? i : nL - 1 - i;
-        const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
+    check_sweep_error(bf);
+    for (int32_t i = 0; i < nS; ++i) {
         if (stage_ms) stage_ms[i] = acc[i] / iters;
-        if (stage_bytes) stage_bytes[i] = t.factor_elems * es + t.alg_bytes_1 * nrhs;
+        if (!stage_bytes) continue;
+        if (fused) {
+            int64_t b = 0;
+            for (const StageTable &t : bf.fwd) b += t.factor_elems * es + t.alg_bytes_1 * nrhs;
+            stage_bytes[i] = b;
+        } else {
+            int32_t L = fwd ? i : nL - 1 - i;
+            const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
+            stage_bytes[i] = t.factor_elems * es + t.alg_bytes_1 * nrhs;
+        }
     }
-    if (n_stages) *n_stages = nL;
+    if (n_stages) *n_stages = nS;
     if (total_ms) *total_ms = tot / iters;
+}
+
+// Raise (and clear) the fused sweep's error flag: a dependency wait that
+// expired means the results of the launch are not to be trusted.
+void check_sweep_error(BF &bf) {
+    if (!bf.sweep.ok) return;
+    int32_t e = 0;
+    BFG_HIP(hipStreamSynchronize(bf.ctx->stream));
+    BFG_HIP(hipMemcpy(&e, bf.sweep.err.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (e) {
+        BFG_HIP(hipMemset(bf.sweep.err.p, 0, sizeof(int32_t)));
+        throw Error(BFGPU_ERR_STATE, "fused sweep: a dependency wait timed out");
+    }
+}
+
+}  // namespace bfg
+
+namespace bfg {
+
+// Launch-shape autotuner: for every forward stage, time each kernel
+// variant on device buffers and keep the fastest (per-stage choice is
+// stored in the stage table and used by all later matvecs).
+void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *variants) {
+    BFG_REQUIRE(nrhs >= 1 && iters >= 1, "autotune: nrhs, iters >= 1");
+    hipStream_t s = bf.ctx->stream;
+    const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+    DevBuf dx, dy;
+    dx.alloc(size_t(bf.cols) * nrhs * es);
+    dy.alloc(size_t(bf.rows) * nrhs * es);
+    BFG_HIP(hipMemsetAsync(dx.p, 0, dx.bytes, s));
+    BFG_HIP(hipMemsetAsync(dy.p, 0, dy.bytes, s));
+    size_t need = size_t(bf.slot_len) * size_t(nrhs) * es;
+    if (need > bf.slots_cap) {
+        release_graph(bf);
+        bf.slots.alloc(need);
+        bf.slots_cap = need;
+    }
+    BFG_HIP(hipMemsetAsync(bf.slots.p, 0, need, s));
+    release_graph(bf);
+    const int32_t nL = bf.n_stage_launch;
+    const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
+    static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    hipEvent_t e0, e1;
+    BFG_HIP(hipEventCreate(&e0));
+    BFG_HIP(hipEventCreate(&e1));
+    try {
+        for (int32_t L = 0; L < nL; ++L) {
+            StageTable &t = bf.fwd[L];
+            const int32_t *pin = L == 0 ? pr : nullptr, *pout = L == nL - 1 ? pt : nullptr;
+            double best = 1e30;
+            int32_t bestv = t.variant;
+            for (int32_t v = 0; v <= 1; ++v) {
+                t.variant = v;
+                launch_stage(s, bf.precision, BFGPU_OP_N, t, bf.factors[L].p, dx.p, bf.cols, dy.p, bf.rows,
+                             bf.slots.p, bf.slot_len, pin, pout, nrhs, one, zero, true);
+                BFG_HIP(hipEventRecord(e0, s));
+                for (int32_t k = 0; k < iters; ++k)
+                    launch_stage(s, bf.precision, BFGPU_OP_N, t, bf.factors[L].p, dx.p, bf.cols, dy.p,
+                                 bf.rows, bf.slots.p, bf.slot_len, pin, pout, nrhs, one, zero, true);
+                BFG_HIP(hipEventRecord(e1, s));
+                BFG_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                BFG_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / iters < best) { best = ms / iters; bestv = v; }
+            }
+            t.variant = bestv;
+            if (stage_ms) stage_ms[L] = best;
+            if (variants) variants[L] = bestv;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
 }
 
 }  // namespace bfg

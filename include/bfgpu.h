@@ -192,6 +192,29 @@ int bfgpu_matvec_timed(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, in
                        void *y, int64_t ldy, int32_t iters, double *stage_ms,
                        int64_t *stage_bytes, int32_t *n_stages, double *total_ms);
 
+/* Forward matvecs with 1..8 right-hand sides run as one fused, dependency-
+ * driven launch over all levels (default on); 0 selects one launch per
+ * level.  Both give the same results to rounding. */
+int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
+/* Tuning options (used by the launch autotuner):
+ *   BFGPU_OPT_SWEEP      0: one launch per level, 1: fused sweep with dynamic
+ *                        task tickets, 2: fused sweep, cooperative launch
+ *   BFGPU_OPT_WG_PER_CU  workgroups per CU of the fused sweep (0 = occupancy)
+ *   BFGPU_OPT_GRAPH      1: replay per-level launches from a captured HIP
+ *                        graph when x, y, alpha, beta repeat */
+enum { BFGPU_OPT_SWEEP = 1, BFGPU_OPT_WG_PER_CU = 2, BFGPU_OPT_GRAPH = 3 };
+int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
+int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);
+/* Autotune the forward launch shape of every level for nrhs right-hand
+ * sides: times each kernel variant (workgroup or wavefront per block) over
+ * `iters` launches and keeps the fastest.  stage_ms / variants (length =
+ * number of launch stages, may be NULL) receive the choice. */
+int bfgpu_bf_autotune(bfgpu_bf *bf, int64_t nrhs, int32_t iters, double *stage_ms,
+                      int32_t *variants);
+/* Synchronise and report a fault flagged by the fused sweep since the last
+ * check (BFGPU_ERR_STATE), e.g. after device-pointer matvecs. */
+int bfgpu_bf_check(bfgpu_bf *bf);
+
 int bfgpu_bf_destroy(bfgpu_bf *bf);
 
 /* ---- learned rank predictor (MLP 6 -> H -> H -> H -> 1, ReLU, FP32) ---- */

@@ -50,6 +50,10 @@ def test_from_host_fp64_matvec_matches_oracle(bfgpu, oracle, ctx):
     x = crandn(rng, 1024)
     y = g @ x
     assert rel(y, obf.matvec(x)) < 1e-12
+    # one launch per level gives the same product as the fused sweep
+    y_levels = g.set_fused(False) @ x
+    g.set_fused(True)
+    assert rel(y_levels, obf.matvec(x)) < 1e-12
     # adjoint / transpose without materialisation vs the oracle's copies
     assert rel(g.adjoint() @ x, obf.transform(1).matvec(x)) < 1e-12
     assert rel(g.transpose() @ x, obf.transform(0).matvec(x)) < 1e-12
@@ -216,3 +220,12 @@ def test_single_block_65536_properties(bfgpu, oracle, ctx):
     Y64 = g @ X.astype(np.complex128)
     assert rel(Y.astype(np.complex128), Y64) < 1e-5
     assert rel(Y[:, 0].astype(np.complex128), (g32 @ X[:, 0]).astype(np.complex128)) < 1e-6
+    # fused sweep (1..8 RHS) and per-level launches agree; repeated sweeps too
+    for nr in (1, 3, 8):
+        Xs = np.asfortranarray(X[:, :nr])
+        Yf = [g32 @ Xs for _ in range(3)]
+        Yl = g32.set_fused(False) @ Xs
+        g32.set_fused(True)
+        for Yi in Yf:
+            assert rel(Yi.astype(np.complex128), Yl.astype(np.complex128)) < 1e-6
+    g32.check()
