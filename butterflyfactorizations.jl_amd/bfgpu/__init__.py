@@ -18,6 +18,17 @@ import numpy as np
 from . import _lib as L
 from ._lib import ArgumentError, BfgpuError, check, lib, ptr
 
+
+def _destroy(fn, h, _lib=lib):
+    """Release a library handle; quiet during interpreter shutdown (the
+    destructors bind this function as a default argument, since module
+    globals may already be cleared when they run)."""
+    try:
+        if h is not None and h.value:
+            getattr(_lib(), fn)(h)
+    except Exception:
+        pass
+
 __all__ = [
     "Context", "default_context", "ClusterTree", "BlockTree", "cluster_blktree",
     "HelmholtzKernelMatrix", "ButterflyRankEstimator", "GeometricRankEstimator",
@@ -54,7 +65,7 @@ class Context:
     def sync(self):
         check(lib().bfgpu_ctx_sync(self._h))
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):
         _destroy("bfgpu_ctx_destroy", getattr(self, "_h", None))
         self._h = None
 
@@ -158,7 +169,7 @@ class ClusterTree:
     def leaves(self):
         return np.nonzero(self.n_children == 0)[0]
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):
         _destroy("bfgpu_tree_destroy", getattr(self, "_h", None))
         self._h = None
 
@@ -281,15 +292,6 @@ def _opts(k, tol, rankestimator, adaptive, acctype, seed):
 _OPNAME = {L.OP_N: "N", L.OP_T: "T", L.OP_C: "C"}
 
 
-def _destroy(fn, h):
-    """Release a library handle; quiet during interpreter shutdown."""
-    try:
-        if h is not None and h.value:
-            getattr(lib(), fn)(h)
-    except Exception:
-        pass
-
-
 def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
@@ -300,7 +302,7 @@ class _Handle:
     def __init__(self, h, ctx, keep):
         self.h, self.ctx, self.keep = h, ctx, keep
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):
         _destroy("bfgpu_bf_destroy", getattr(self, "h", None))
         self.h = None
 
@@ -697,6 +699,6 @@ class RankPredictor:
                                     int(epochs), int(batch), float(lr), int(seed), C.byref(loss)))
         return loss.value
 
-    def __del__(self):
+    def __del__(self, _destroy=_destroy):
         _destroy("bfgpu_mlp_destroy", getattr(self, "_h", None))
         self._h = None

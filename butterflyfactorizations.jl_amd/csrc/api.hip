@@ -446,6 +446,7 @@ extern "C" int bfgpu_bf_set_fused(bfgpu_bf *h, int32_t enable) {
     BFG_API_BEGIN
     BFG_REQUIRE(h, "bf_set_fused: null handle");
     h->b.use_sweep = enable != 0;
+    if (h->b.use_sweep && !h->b.sweep.built) build_sweep(h->b);
     BFG_API_END
 }
 
@@ -458,6 +459,7 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         BFG_REQUIRE(value >= 0 && value <= 2, "bf_set_option: sweep mode 0..2");
         bf.use_sweep = value != 0;
         if (value) bf.sweep.mode = value == 1 ? 0 : 1;
+        if (bf.use_sweep && !bf.sweep.built) build_sweep(bf);
         break;
     case BFGPU_OPT_WG_PER_CU:
         BFG_REQUIRE(value >= 0 && value <= 32, "bf_set_option: workgroups per CU 0..32");

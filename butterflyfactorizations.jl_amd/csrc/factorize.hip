@@ -1,15 +1,21 @@
 // factorize.hip -- level-by-level butterfly factorization driven from the
 // host, with every kernel-matrix tile, pivoted QR and interpolation matrix
 // computed on the GPU in batches (all compressions of one level, over every
-// factorization of the batch, per launch).
+// factorization of the batch, per launch).  Host-side planning and
+// emission run in parallel over the factorizations of a batch.
 //
 // Restates assemble_BF (src/ButterflyFactorization/bfassembly.jl:48-200)
 // with build_U_skeletons (:206-230), build_nonfrozen_R_blocks (:232-349),
 // build_sourcefrozen_R_blocks (:351-412), build_observerfrozen_R_blocks
 // (:414-481) and the PartialQR functor (compressors.jl:81-165).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 
 #include "internal.hpp"
@@ -18,6 +24,18 @@ namespace bfg {
 
 int32_t mlp_predict_ranks(const bfgpu_mlp *m, const std::vector<float> &feat, int64_t n,
                           std::vector<int32_t> &out);
+
+int host_threads() {
+    static int n = [] {
+        int h = int(std::thread::hardware_concurrency());
+        if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+            int v = std::atoi(e);
+            if (v > 0) h = std::min(h > 0 ? h : v, v);
+        }
+        return std::max(1, std::min(h, 64));
+    }();
+    return n;
+}
 
 // ---- row sample: sparse Fisher-Yates prefix over a SplitMix64 counter
 // stream (the documented replacement of compressors.jl:101 randperm) ----
@@ -37,21 +55,49 @@ uint64_t sample_key(uint64_t seed, int32_t no, int32_t ns, int32_t stage, int32_
     return h;
 }
 
+// Same permutation prefix as a swap map over the identity; the map is an
+// open-addressing table reused per thread (generation-stamped, no clears).
 void sample_prefix(uint64_t key, int32_t n, int32_t m, int32_t *out) {
     if (m > n) m = n;
-    std::unordered_map<int32_t, int32_t> sw;
-    sw.reserve(size_t(2 * m + 8));
-    auto get = [&](int32_t i) {
-        auto it = sw.find(i);
-        return it == sw.end() ? i : it->second;
+    if (m <= 0) return;
+    thread_local std::vector<int32_t> tk, tv;
+    thread_local std::vector<uint32_t> tg;
+    thread_local uint32_t gen = 0;
+    size_t cap = 16;
+    while (cap < size_t(4) * size_t(m)) cap <<= 1;
+    if (tk.size() < cap) {
+        tk.assign(cap, 0);
+        tv.assign(cap, 0);
+        tg.assign(cap, 0);
+        gen = 0;
+    }
+    if (++gen == 0) {
+        std::fill(tg.begin(), tg.end(), 0u);
+        gen = 1;
+    }
+    const size_t mask = cap - 1;
+    auto slot = [&](int32_t k) {
+        size_t h = size_t((uint64_t(uint32_t(k)) * 0x9E3779B97F4A7C15ULL) >> 32) & mask;
+        while (tg[h] == gen && tk[h] != k) h = (h + 1) & mask;
+        return h;
+    };
+    auto get = [&](int32_t k) {
+        size_t h = slot(k);
+        return tg[h] == gen ? tv[h] : k;
+    };
+    auto set = [&](int32_t k, int32_t v) {
+        size_t h = slot(k);
+        tg[h] = gen;
+        tk[h] = k;
+        tv[h] = v;
     };
     for (int32_t i = 0; i < m; ++i) {
         uint64_t u = mix64(key + uint64_t(i + 1) * 0x9E3779B97F4A7C15ULL);
         uint32_t span = uint32_t(n - i);
         int32_t j = i + int32_t(((u >> 32) * uint64_t(span)) >> 32);
         int32_t ai = get(i), aj = get(j);
-        sw[i] = aj;
-        sw[j] = ai;
+        set(i, aj);
+        set(j, ai);
         out[i] = aj;
     }
 }
@@ -88,6 +134,9 @@ template <class T> void upload(DevBuf &d, const std::vector<T> &h, hipStream_t s
     if (!h.empty()) BFG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
 }
 
+using clk = std::chrono::steady_clock;
+inline double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
 struct Engine {
     Ctx &ctx;
     const Tree &test, &trial;
@@ -97,6 +146,7 @@ struct Engine {
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
     int64_t n_comp = 0, n_retry = 0;
+    double t_host = 0, t_qr = 0, t_interp = 0;  // seconds: sample prep, fill+QR, interpolation
 
     Engine(Ctx &c, const Tree &te, const Tree &tr, const bfgpu_opts &op)
         : ctx(c), test(te), trial(tr), o(op), prec(op.precision),
@@ -163,30 +213,47 @@ struct Engine {
             c.rows = std::min(std::max(c.guess, 10), c.n_obs);
             pend.push_back(int32_t(i));
         }
-        std::vector<int32_t> hrows, hcols, perm, hrank, hpiv;
+        std::vector<int32_t> hrows, hcols, hrank, hpiv;
         std::vector<FillTask> ft;
-        std::vector<int64_t> fe;
+        std::vector<int64_t> fe, rofs, cofs;
         std::vector<QrTask> qt;
         std::vector<InterpTask> it;
         while (!pend.empty()) {
-            hrows.clear(); hcols.clear(); ft.clear(); fe.clear(); qt.clear();
+            auto th = clk::now();
+            const size_t np = pend.size();
+            ft.resize(np);
+            fe.resize(np);
+            qt.resize(np);
+            rofs.assign(np + 1, 0);
+            cofs.assign(np + 1, 0);
             int64_t zofs = 0;
-            int32_t pofs = 0, max_n = 1, max_m = 1;
-            for (int32_t ci : pend) {
-                Comp &c = cs[ci];
-                perm.resize(c.rows);
-                sample_prefix(c.key, c.n_obs, c.rows, perm.data());
-                FillTask f{zofs, c.rows, int32_t(c.cols.size()), int32_t(hrows.size()), int32_t(hcols.size())};
-                for (int32_t i = 0; i < c.rows; ++i) hrows.push_back(c.obs[perm[i]]);
-                hcols.insert(hcols.end(), c.cols.begin(), c.cols.end());
-                ft.push_back(f);
+            int32_t max_n = 1, max_m = 1;
+            for (size_t q = 0; q < np; ++q) {
+                const Comp &c = cs[pend[q]];
+                rofs[q + 1] = rofs[q] + c.rows;
+                cofs[q + 1] = cofs[q] + int64_t(c.cols.size());
+                FillTask f{zofs, c.rows, int32_t(c.cols.size()), int32_t(rofs[q]), int32_t(cofs[q])};
+                ft[q] = f;
                 zofs += int64_t(f.m) * f.n;
-                fe.push_back(zofs);
-                qt.push_back(QrTask{f.zofs, f.m, f.n, pofs, 0});
-                pofs += f.n;
+                fe[q] = zofs;
+                qt[q] = QrTask{f.zofs, f.m, f.n, int32_t(cofs[q]), 0};
                 max_n = std::max(max_n, f.n);
                 max_m = std::max(max_m, f.m);
             }
+            BFG_REQUIRE(rofs[np] < (int64_t(1) << 31) && cofs[np] < (int64_t(1) << 31),
+                        "factorize: level too large for 32-bit index lists");
+            const int32_t pofs = int32_t(cofs[np]);
+            hrows.resize(size_t(rofs[np]));
+            hcols.resize(size_t(cofs[np]));
+            parallel_for(int64_t(np), [&](int64_t q) {
+                const Comp &c = cs[pend[q]];
+                int32_t *r = hrows.data() + rofs[q];
+                sample_prefix(c.key, c.n_obs, c.rows, r);
+                for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[r[i]];
+                std::copy(c.cols.begin(), c.cols.end(), hcols.begin() + cofs[q]);
+            });
+            auto tq = clk::now();
+            t_host += secs(th, tq);
             upload(d_rows, hrows, ctx.stream);
             upload(d_cols, hcols, ctx.stream);
             upload(d_fill, ft, ctx.stream);
@@ -194,29 +261,30 @@ struct Engine {
             upload(d_qr, qt, ctx.stream);
             d_Z.reserve(size_t(zofs) * es);
             d_piv.reserve(sizeof(int32_t) * size_t(std::max(pofs, 1)));
-            d_rank.reserve(sizeof(int32_t) * pend.size());
+            d_rank.reserve(sizeof(int32_t) * np);
             launch_fill(ctx.stream, prec, d_test_pts.as<double>(), d_trial_pts.as<double>(),
                         d_rows.as<int32_t>(), d_cols.as<int32_t>(), d_fill.as<FillTask>(),
                         d_fill_end.as<int64_t>(), int32_t(ft.size()), zofs, d_Z.p, o.k);
             // per-task tolerance: one launch per distinct eps (the batch may
             // hold factorizations of different depth)
-            std::vector<double> eps_of;
-            for (int32_t ci : pend) eps_of.push_back(cs[ci].eps);
+            std::vector<double> eps_of(np);
+            for (size_t q = 0; q < np; ++q) eps_of[q] = cs[pend[q]].eps;
             std::vector<double> uniq = eps_of;
             std::sort(uniq.begin(), uniq.end());
             uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-            hrank.assign(pend.size(), 0);
+            hrank.assign(np, 0);
             if (uniq.size() == 1) {
-                launch_cpqr(ctx.stream, prec, d_Z.p, d_qr.as<QrTask>(), int32_t(qt.size()),
+                launch_cpqr(ctx.stream, prec, d_Z.p, d_qr.as<QrTask>(), int32_t(np),
                             d_piv.as<int32_t>(), d_rank.as<int32_t>(), uniq[0], max_n, max_m);
+                BFG_HIP(hipMemcpyAsync(hrank.data(), d_rank.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx.stream));
             } else {
-                // group tasks by tolerance
                 std::vector<QrTask> sub;
                 std::vector<int32_t> where;
                 DevBuf d_sub, d_subrank;
                 for (double e : uniq) {
-                    sub.clear(); where.clear();
-                    for (size_t q = 0; q < pend.size(); ++q)
+                    sub.clear();
+                    where.clear();
+                    for (size_t q = 0; q < np; ++q)
                         if (eps_of[q] == e) { sub.push_back(qt[q]); where.push_back(int32_t(q)); }
                     upload(d_sub, sub, ctx.stream);
                     d_subrank.reserve(sizeof(int32_t) * sub.size());
@@ -227,18 +295,16 @@ struct Engine {
                     BFG_HIP(hipStreamSynchronize(ctx.stream));
                     for (size_t q = 0; q < where.size(); ++q) hrank[where[q]] = r[q];
                 }
-                BFG_HIP(hipMemcpyAsync(d_rank.p, hrank.data(), sizeof(int32_t) * hrank.size(), hipMemcpyHostToDevice, ctx.stream));
-            }
-            if (uniq.size() == 1) {
-                BFG_HIP(hipMemcpyAsync(hrank.data(), d_rank.p, sizeof(int32_t) * hrank.size(), hipMemcpyDeviceToHost, ctx.stream));
             }
             hpiv.resize(std::max(pofs, 1));
             BFG_HIP(hipMemcpyAsync(hpiv.data(), d_piv.p, sizeof(int32_t) * size_t(pofs), hipMemcpyDeviceToHost, ctx.stream));
             BFG_HIP(hipStreamSynchronize(ctx.stream));
+            auto ti = clk::now();
+            t_qr += secs(tq, ti);
             std::vector<int32_t> retry;
             it.clear();
             int32_t max_rank = 0;
-            for (size_t q = 0; q < pend.size(); ++q) {
+            for (size_t q = 0; q < np; ++q) {
                 Comp &c = cs[pend[q]];
                 int32_t r = hrank[q];
                 if (o.adaptive && r > int32_t(std::floor(0.8 * c.rows)) && c.rows < c.n_obs) {
@@ -248,13 +314,18 @@ struct Engine {
                     continue;
                 }
                 c.rank = r;
-                c.skel.resize(r);
-                for (int32_t i = 0; i < r; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
                 c.tofs = pool_used;
                 pool_used += int64_t(r) * int64_t(c.cols.size());
                 it.push_back(InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, r, qt[q].pofs});
                 max_rank = std::max(max_rank, r);
             }
+            // skeletons (pivot prefixes) in parallel
+            parallel_for(int64_t(np), [&](int64_t q) {
+                Comp &c = cs[pend[q]];
+                if (c.rows != ft[q].m) return;  // being retried with more rows
+                c.skel.resize(c.rank);
+                for (int32_t i = 0; i < c.rank; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
+            });
             if (!it.empty()) {
                 ensure_pool(pool, pool_cap, pool_used);
                 upload(d_interp, it, ctx.stream);
@@ -262,6 +333,7 @@ struct Engine {
                               d_piv.as<int32_t>(), pool.p, max_rank);
                 BFG_HIP(hipStreamSynchronize(ctx.stream));
             }
+            t_interp += secs(ti, clk::now());
             pend.swap(retry);
         }
     }
@@ -269,8 +341,8 @@ struct Engine {
 
 // Geometric features of a compression (source node S, observer node O) for
 // the learned rank predictor: box radii in wavelengths, centre distance in
-// wavelengths, angle between the separation vector and the z axis, level,
-// and log10(1/eps).
+// wavelengths, |cos| of the angle between the separation and the z axis,
+// level, and log10(1/eps).
 void features(const Tree &test, const Tree &trial, double k, int32_t S, int32_t O, int32_t stage,
               double eps, float *f) {
     double lam = 2.0 * M_PI / k;
@@ -285,6 +357,15 @@ void features(const Tree &test, const Tree &trial, double k, int32_t S, int32_t 
     f[5] = float(std::log10(1.0 / eps));
 }
 
+// An action of one level: a compression (rule `kind` 0 = column slices per
+// source child or the whole T for a leaf source, 2 = source frozen) or an
+// identity block (kind 1).
+struct Act {
+    int32_t kind, comp;      // comp: index into the factorization's own list
+    int32_t Oout, Sv, Ov;    // emission keys
+    int32_t len;             // identity length
+};
+
 }  // namespace
 
 void compression_features(const Tree &test, const Tree &trial, double k, int32_t S, int32_t O,
@@ -294,30 +375,33 @@ void compression_features(const Tree &test, const Tree &trial, double k, int32_t
 
 // Factorize n_pairs blocks; fills bf.bfs (reference layout) and bf.factors.
 void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int32_t, int32_t>> &pairs) {
-    auto t0 = std::chrono::steady_clock::now();
+    auto t0 = clk::now();
     Ctx &ctx = *bf.ctx;
     const Tree &test = *bf.test, &trial = *bf.trial;
     Engine E(ctx, test, trial, o);
-    std::vector<Build> B(pairs.size());
-    int32_t Lmax = 1;
-    bf.bfs.assign(pairs.size(), OneBF{});
-    for (size_t b = 0; b < pairs.size(); ++b) {
+    const int64_t nB = int64_t(pairs.size());
+    std::vector<Build> B(nB);
+    bf.bfs.assign(nB, OneBF{});
+    for (int64_t b = 0; b < nB; ++b)
+        BFG_REQUIRE(pairs[b].first >= 0 && pairs[b].first < test.n_nodes && pairs[b].second >= 0 &&
+                        pairs[b].second < trial.n_nodes,
+                    "factorize: root node out of range");
+    parallel_for(nB, [&](int64_t b) {
         Build &x = B[b];
         x.no = pairs[b].first;
         x.ns = pairs[b].second;
-        BFG_REQUIRE(x.no >= 0 && x.no < test.n_nodes && x.ns >= 0 && x.ns < trial.n_nodes,
-                    "factorize: root node out of range");
         x.tS = traverse_and_pad(trial, x.ns);
         x.tO = traverse_and_pad(test, x.no);
         x.LS = int32_t(x.tS.size());
         x.LO = int32_t(x.tO.size());
         x.L = std::max(x.LS, x.LO);
         x.eps = o.tol / x.L;
-        Lmax = std::max(Lmax, x.L);
         bf.bfs[b].no = x.no;
         bf.bfs[b].ns = x.ns;
         bf.bfs[b].st.resize(x.L + 1);
-    }
+    });
+    int32_t Lmax = 1;
+    for (const Build &x : B) Lmax = std::max(Lmax, x.L);
     bf.n_stage_launch = Lmax + 1;
     bf.factors.clear();
     bf.factors.resize(bf.n_stage_launch);
@@ -337,98 +421,105 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         features(test, trial, o.k, S, O, stage, c.eps, c.feat);
         return c;
     };
-    auto log_skel = [&](int32_t b, int32_t stage, int32_t s, int32_t on, const std::vector<int32_t> &v) {
-        OneBF &ob = bf.bfs[b];
+    auto log_skel = [&](OneBF &ob, int32_t stage, int32_t s, int32_t on, const std::vector<int32_t> &v) {
         ob.skel.push_back(SkelRec{stage, s, on, int32_t(ob.skel_idx.size()), int32_t(v.size())});
         ob.skel_idx.insert(ob.skel_idx.end(), v.begin(), v.end());
     };
-    auto log_comp = [&](const Comp &c) {
+    auto log_comp = [&](OneBF &ob, const Comp &c) {
         CompRec r{c.stage, c.S, c.O, c.guess, c.rows, c.rank, {}};
         for (int q = 0; q < 6; ++q) r.feat[q] = c.feat[q];
-        bf.bfs[c.bf].comp.push_back(r);
+        ob.comp.push_back(r);
     };
+    double t_plan = 0, t_emit = 0;
 
     // ---- stage 0: Q (bfassembly.jl:82-102) ----
     {
-        std::vector<Comp> cs;
-        for (size_t b = 0; b < B.size(); ++b)
-            for (int32_t Sl : B[b].tS.back())
-                cs.push_back(mkcomp(int32_t(b), 0, Sl, B[b].no,
-                                    std::vector<int32_t>(trial.vals(Sl), trial.vals(Sl) + trial.nvals(Sl))));
+        auto tp = clk::now();
+        std::vector<int64_t> off(nB + 1, 0);
+        for (int64_t b = 0; b < nB; ++b) off[b + 1] = off[b] + int64_t(B[b].tS.back().size());
+        std::vector<Comp> cs(static_cast<size_t>(off[nB]));
+        parallel_for(nB, [&](int64_t b) {
+            const auto &lv = B[b].tS.back();
+            for (size_t i = 0; i < lv.size(); ++i) {
+                int32_t Sl = lv[i];
+                cs[off[b] + int64_t(i)] =
+                    mkcomp(int32_t(b), 0, Sl, B[b].no,
+                           std::vector<int32_t>(trial.vals(Sl), trial.vals(Sl) + trial.nvals(Sl)));
+            }
+        });
+        t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
         E.run(cs, bf.factors[0], cap, used);
         bf.factor_elems[0] = used;
-        for (auto &c : cs) {
-            log_comp(c);
-            bf.bfs[c.bf].st[0].b.push_back(Block{B[c.bf].no, c.S, B[c.bf].no, c.S, c.rank,
-                                                  int32_t(c.cols.size()), 0, c.tofs, -1, -1});
-            B[c.bf].K[fastkey(c.S, B[c.bf].no)] = c.skel;
-            log_skel(c.bf, 0, c.S, B[c.bf].no, c.skel);
-        }
+        auto te = clk::now();
+        parallel_for(nB, [&](int64_t b) {
+            OneBF &ob = bf.bfs[b];
+            Build &x = B[b];
+            for (int64_t i = off[b]; i < off[b + 1]; ++i) {
+                Comp &c = cs[i];
+                log_comp(ob, c);
+                ob.st[0].b.push_back(Block{x.no, c.S, x.no, c.S, c.rank, int32_t(c.cols.size()), 0, c.tofs, -1, -1});
+                log_skel(ob, 0, c.S, x.no, c.skel);
+                x.K[fastkey(c.S, x.no)] = std::move(c.skel);
+            }
+        });
+        t_emit += secs(te, clk::now());
     }
 
     // ---- R levels (bfassembly.jl:107-181) ----
-    // An action is either a compression (index into cs, with the emission
-    // rule) or an identity block.
-    struct Act {
-        int32_t b, kind;  // 0: compression, 1: identity
-        int32_t comp;
-        int32_t Oout, Sv, Ov;  // emission keys
-        int32_t len;           // identity length
-    };
     for (int32_t l = 1; l <= Lmax - 1; ++l) {
-        std::vector<Comp> cs;
-        std::vector<Act> acts;
-        std::vector<std::vector<std::tuple<int32_t, int32_t, std::vector<int32_t>>>> upd(B.size());
-        auto U_of = [&](Build &x, int32_t Sv, int32_t Ov) {
-            std::vector<int32_t> u;
-            if (trial.isleaf(Sv)) return x.K.at(fastkey(Sv, Ov));
-            for (int32_t c = 0; c < trial.n_children[Sv]; ++c) {
-                const auto &k = x.K.at(fastkey(trial.first_child[Sv] + c, Ov));
-                u.insert(u.end(), k.begin(), k.end());
-            }
-            return u;
-        };
-        for (size_t b = 0; b < B.size(); ++b) {
+        auto tp = clk::now();
+        std::vector<std::vector<Comp>> csb(nB);
+        std::vector<std::vector<Act>> actb(nB);
+        parallel_for(nB, [&](int64_t b) {
             Build &x = B[b];
-            if (l > x.L - 1) continue;
-            bool sf = l >= x.LS, of = l >= x.LO;
-            auto add_comp = [&](int32_t S, int32_t O, std::vector<int32_t> U, int32_t Oout, int32_t Sv, int32_t Ov) {
+            if (l > x.L - 1) return;
+            auto &cs = csb[b];
+            auto &acts = actb[b];
+            auto U_of = [&](int32_t Sv, int32_t Ov) {
+                if (trial.isleaf(Sv)) return x.K.at(fastkey(Sv, Ov));
+                std::vector<int32_t> u;
+                for (int32_t c = 0; c < trial.n_children[Sv]; ++c) {
+                    const auto &k = x.K.at(fastkey(trial.first_child[Sv] + c, Ov));
+                    u.insert(u.end(), k.begin(), k.end());
+                }
+                return u;
+            };
+            auto add_comp = [&](int32_t S, int32_t O, std::vector<int32_t> U, int32_t kind, int32_t Oout,
+                                int32_t Sv, int32_t Ov) {
                 cs.push_back(mkcomp(int32_t(b), l, S, O, std::move(U)));
-                acts.push_back(Act{int32_t(b), 0, int32_t(cs.size() - 1), Oout, Sv, Ov, 0});
+                acts.push_back(Act{kind, int32_t(cs.size() - 1), Oout, Sv, Ov, 0});
             };
             auto add_ident = [&](int32_t Ov, int32_t Sv) {
-                const auto &k = x.K.at(fastkey(Sv, Ov));
-                acts.push_back(Act{int32_t(b), 1, -1, Ov, Sv, Ov, int32_t(k.size())});
-                upd[b].emplace_back(Sv, Ov, k);
+                acts.push_back(Act{1, -1, Ov, Sv, Ov, int32_t(x.K.at(fastkey(Sv, Ov)).size())});
             };
+            const bool sf = l >= x.LS, of = l >= x.LO;
             if (!sf && !of) {
                 const auto &Os = x.tO[l - 1];
                 const auto &Ss = x.tS[x.LS - l - 1];
                 for (int32_t Sv : Ss)
                     for (int32_t Ov : Os) {
                         if (!test.isleaf(Ov)) {
-                            std::vector<int32_t> U = U_of(x, Sv, Ov);
+                            std::vector<int32_t> U = U_of(Sv, Ov);
                             for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
                                 int32_t Oc = test.first_child[Ov] + c;
-                                add_comp(Sv, Oc, U, Oc, Sv, Ov);
+                                add_comp(Sv, Oc, U, 0, Oc, Sv, Ov);
                             }
                         } else if (!trial.isleaf(Sv)) {
-                            add_comp(Sv, Ov, U_of(x, Sv, Ov), Ov, Sv, Ov);
+                            add_comp(Sv, Ov, U_of(Sv, Ov), 0, Ov, Sv, Ov);
                         } else {
                             add_ident(Ov, Sv);
                         }
                     }
             } else if (sf && !of) {
                 const auto &Os = x.tO[l - 1];
-                int32_t Sv = x.ns;
+                const int32_t Sv = x.ns;
                 for (int32_t Ov : Os) {
                     if (!test.isleaf(Ov)) {
                         for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
                             int32_t Oc = test.first_child[Ov] + c;
-                            // source frozen: input is the single slot (Ov, ns)
-                            cs.push_back(mkcomp(int32_t(b), l, Sv, Oc, x.K.at(fastkey(Sv, Ov))));
-                            acts.push_back(Act{int32_t(b), 2, int32_t(cs.size() - 1), Oc, Sv, Ov, 0});
+                            // source frozen: the input is the single slot (Ov, ns)
+                            add_comp(Sv, Oc, x.K.at(fastkey(Sv, Ov)), 2, Oc, Sv, Ov);
                         }
                     } else {
                         add_ident(Ov, Sv);
@@ -439,73 +530,115 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
                 const auto &Ss = x.tS[x.LS - l - 1];
                 for (int32_t Sv : Ss)
                     for (int32_t Ov : Os) {
-                        if (!trial.isleaf(Sv)) add_comp(Sv, Ov, U_of(x, Sv, Ov), Ov, Sv, Ov);
+                        if (!trial.isleaf(Sv)) add_comp(Sv, Ov, U_of(Sv, Ov), 0, Ov, Sv, Ov);
                         else add_ident(Ov, Sv);
                     }
             }
-        }
+        });
+        std::vector<int64_t> off(nB + 1, 0);
+        for (int64_t b = 0; b < nB; ++b) off[b + 1] = off[b] + int64_t(csb[b].size());
+        std::vector<Comp> cs(static_cast<size_t>(off[nB]));
+        parallel_for(nB, [&](int64_t b) {
+            for (size_t i = 0; i < csb[b].size(); ++i) cs[off[b] + int64_t(i)] = std::move(csb[b][i]);
+            std::vector<Comp>().swap(csb[b]);
+        });
+        t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
         E.run(cs, bf.factors[l], cap, used);
         bf.factor_elems[l] = used;
-        // emit blocks in action order; K updates in the same order
-        std::vector<std::vector<std::tuple<int32_t, int32_t, std::vector<int32_t>>>> upd2(B.size());
-        for (const Act &a : acts) {
-            Build &x = B[a.b];
-            auto &st = bf.bfs[a.b].st[l].b;
-            if (a.kind == 1) {
-                st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, a.len, a.len, 1, -1, -1, -1});
-                const auto &k = x.K.at(fastkey(a.Sv, a.Ov));
-                upd2[a.b].emplace_back(a.Sv, a.Ov, k);
-                continue;
-            }
-            Comp &c = cs[a.comp];
-            log_comp(c);
-            int32_t nU = int32_t(c.cols.size());
-            if (a.kind == 2 || trial.isleaf(a.Sv)) {
-                // whole T: source frozen (Oc, ns) <- (Ov, ns), or leaf Svert
-                st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, c.rank, nU, 0, c.tofs, -1, -1});
-            } else {
-                int32_t last = 0;
-                for (int32_t q = 0; q < trial.n_children[a.Sv]; ++q) {
-                    int32_t Sc = trial.first_child[a.Sv] + q;
-                    int32_t ks = int32_t(x.K.at(fastkey(Sc, a.Ov)).size());
-                    st.push_back(Block{a.Oout, a.Sv, a.Ov, Sc, c.rank, ks, 0,
-                                       c.tofs + int64_t(c.rank) * last, -1, -1});
-                    last += ks;
+        // emit blocks in action order; K updates in the same order, applied
+        // after the level (the reference's sequential update, :177-180)
+        auto te = clk::now();
+        parallel_for(nB, [&](int64_t b) {
+            Build &x = B[b];
+            if (l > x.L - 1) return;
+            OneBF &ob = bf.bfs[b];
+            auto &st = ob.st[l].b;
+            std::vector<std::pair<uint64_t, std::vector<int32_t>>> upd;
+            for (const Act &a : actb[b]) {
+                if (a.kind == 1) {
+                    st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, a.len, a.len, 1, -1, -1, -1});
+                    upd.emplace_back(fastkey(a.Sv, a.Ov), x.K.at(fastkey(a.Sv, a.Ov)));
+                    continue;
                 }
+                Comp &c = cs[off[b] + a.comp];
+                log_comp(ob, c);
+                const int32_t nU = int32_t(c.cols.size());
+                if (a.kind == 2 || trial.isleaf(a.Sv)) {
+                    // whole T: source frozen (Oc, ns) <- (Ov, ns), or leaf Svert
+                    st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, c.rank, nU, 0, c.tofs, -1, -1});
+                } else {
+                    int32_t last = 0;
+                    for (int32_t q = 0; q < trial.n_children[a.Sv]; ++q) {
+                        int32_t Sc = trial.first_child[a.Sv] + q;
+                        int32_t ks = int32_t(x.K.at(fastkey(Sc, a.Ov)).size());
+                        st.push_back(Block{a.Oout, a.Sv, a.Ov, Sc, c.rank, ks, 0,
+                                           c.tofs + int64_t(c.rank) * last, -1, -1});
+                        last += ks;
+                    }
+                }
+                upd.emplace_back(fastkey(c.S, c.O), std::move(c.skel));
             }
-            upd2[a.b].emplace_back(c.S, c.O, c.skel);
-        }
-        for (size_t b = 0; b < B.size(); ++b)
-            for (auto &u : upd2[b]) {
-                B[b].K[fastkey(std::get<0>(u), std::get<1>(u))] = std::get<2>(u);
-                log_skel(int32_t(b), l, std::get<0>(u), std::get<1>(u), std::get<2>(u));
+            for (auto &u : upd) {
+                log_skel(ob, l, int32_t(u.first >> 32), int32_t(u.first & 0xffffffffu), u.second);
+                x.K[u.first] = std::move(u.second);
             }
+        });
+        t_emit += secs(te, clk::now());
     }
 
     // ---- P (bfassembly.jl:186-197): dense kernel blocks ----
     {
-        int32_t sP = Lmax;
-        std::vector<int32_t> hrows, hcols;
-        std::vector<FillTask> ft;
-        std::vector<int64_t> fe;
-        int64_t zofs = 0;
-        for (size_t b = 0; b < B.size(); ++b) {
+        auto tp = clk::now();
+        const int32_t sP = Lmax;
+        std::vector<int64_t> noff(nB + 1, 0);  // P blocks per factorization
+        for (int64_t b = 0; b < nB; ++b) noff[b + 1] = noff[b] + int64_t(B[b].tO.back().size());
+        const int64_t nP = noff[nB];
+        std::vector<int64_t> rofs(nP + 1, 0), cofs(nP + 1, 0), zo(nP + 1, 0);
+        parallel_for(nB, [&](int64_t b) {
+            const Build &x = B[b];
+            const auto &lv = x.tO.back();
+            for (size_t i = 0; i < lv.size(); ++i) {
+                int64_t q = noff[b] + int64_t(i);
+                rofs[q + 1] = test.nvals(lv[i]);
+                cofs[q + 1] = int64_t(x.K.at(fastkey(x.ns, lv[i])).size());
+            }
+        });
+        for (int64_t q = 0; q < nP; ++q) {
+            zo[q + 1] = zo[q] + rofs[q + 1] * cofs[q + 1];
+            rofs[q + 1] += rofs[q];
+            cofs[q + 1] += cofs[q];
+        }
+        BFG_REQUIRE(rofs[nP] < (int64_t(1) << 31) && cofs[nP] < (int64_t(1) << 31),
+                    "factorize: P stage too large for 32-bit index lists");
+        std::vector<int32_t> hrows(static_cast<size_t>(rofs[nP])), hcols(static_cast<size_t>(cofs[nP]));
+        std::vector<FillTask> ftall(static_cast<size_t>(nP));
+        parallel_for(nB, [&](int64_t b) {
             Build &x = B[b];
             auto &st = bf.bfs[b].st[x.L].b;
-            for (int32_t Ol : x.tO.back()) {
+            const auto &lv = x.tO.back();
+            for (size_t i = 0; i < lv.size(); ++i) {
+                int64_t q = noff[b] + int64_t(i);
+                int32_t Ol = lv[i];
                 const auto &col = x.K.at(fastkey(x.ns, Ol));
-                FillTask f{zofs, test.nvals(Ol), int32_t(col.size()), int32_t(hrows.size()), int32_t(hcols.size())};
-                hrows.insert(hrows.end(), test.vals(Ol), test.vals(Ol) + test.nvals(Ol));
-                hcols.insert(hcols.end(), col.begin(), col.end());
-                st.push_back(Block{Ol, x.ns, Ol, x.ns, f.m, f.n, 0, zofs, -1, -1});
-                if (int64_t(f.m) * f.n > 0) {
-                    ft.push_back(f);
-                    zofs += int64_t(f.m) * f.n;
-                    fe.push_back(zofs);
-                }
+                FillTask f{zo[q], test.nvals(Ol), int32_t(col.size()), int32_t(rofs[q]), int32_t(cofs[q])};
+                std::copy(test.vals(Ol), test.vals(Ol) + f.m, hrows.begin() + rofs[q]);
+                std::copy(col.begin(), col.end(), hcols.begin() + cofs[q]);
+                ftall[q] = f;
+                st.push_back(Block{Ol, x.ns, Ol, x.ns, f.m, f.n, 0, zo[q], -1, -1});
             }
-        }
+        });
+        std::vector<FillTask> ft;
+        std::vector<int64_t> fe;
+        ft.reserve(size_t(nP));
+        fe.reserve(size_t(nP));
+        for (const FillTask &f : ftall)
+            if (int64_t(f.m) * f.n > 0) {
+                ft.push_back(f);
+                fe.push_back(f.zofs + int64_t(f.m) * f.n);
+            }
+        const int64_t zofs = zo[nP];
+        t_plan += secs(tp, clk::now());
         bf.factors[sP].alloc(size_t(std::max<int64_t>(zofs, 1)) * E.es);
         bf.factor_elems[sP] = zofs;
         upload(E.d_rows, hrows, ctx.stream);
@@ -520,14 +653,18 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     bf.n_comp = E.n_comp;
     bf.n_retry = E.n_retry;
     bf.precision = o.precision;
-    // stage index of each BF's blocks in launch numbering: Q = 0, R^l = l,
-    // P = Lmax; record the factor base per reference stage
-    for (size_t b = 0; b < B.size(); ++b) {
-        OneBF &ob = bf.bfs[b];
-        ob.stage_data_base.assign(ob.st.size(), 0);
-    }
+    for (int64_t b = 0; b < nB; ++b) bf.bfs[b].stage_data_base.assign(bf.bfs[b].st.size(), 0);
+    std::vector<Build>().swap(B);
+    auto tpk = clk::now();
     pack_tables(bf);
-    bf.t_factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    auto t1 = clk::now();
+    bf.t_factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (std::getenv("BFGPU_TIMING"))
+        std::fprintf(stderr,
+                     "[bfgpu] factorize %.3fs: plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "
+                     "interp %.3fs, emit %.3fs, pack %.3fs (%lld compressions, %lld retries, %d threads)\n",
+                     bf.t_factor_ms * 1e-3, t_plan, E.t_host, E.t_qr, E.t_interp, t_emit, secs(tpk, t1),
+                     (long long)E.n_comp, (long long)E.n_retry, host_threads());
 }
 
 }  // namespace bfg

@@ -2,10 +2,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bfgpu.h"
@@ -148,7 +153,8 @@ struct StageTable {
 
 constexpr int SWEEP_MAX_STAGES = 48;
 struct SweepTables {
-    bool ok = false;
+    bool built = false;     // tables built (on first enable)
+    bool ok = false;        // built and the data flow admits the sweep
     int32_t n_tasks = 0, n_groups = 0, max_wait = 1;
     DevBuf tasks, contribs, counters, err;  // counters: n_groups + ticket pair
     int32_t epoch = 0;
@@ -187,6 +193,43 @@ struct BF {
 
 // Packing of reference-layout blocks into launch tables (matvec.hip).
 void pack_tables(BF &bf);
+// Fused-sweep tables, built on first use (matvec.hip).
+void build_sweep(BF &bf);
+
+// Host worker threads (OMP_NUM_THREADS caps the hardware count).
+int host_threads();
+
+// parallel_for over [0, n) in chunks of `chunk`; the first exception thrown
+// by any worker is rethrown on the caller.
+template <class F> void parallel_for(int64_t n, F &&f, int64_t chunk = 64) {
+    int nt = int(std::min<int64_t>(host_threads(), (n + chunk - 1) / chunk));
+    if (nt <= 1) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next{0};
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&]() {
+        try {
+            for (;;) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                int64_t e = std::min(n, b + chunk);
+                for (int64_t i = b; i < e; ++i) f(i);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> g(mu);
+            if (!err) err = std::current_exception();
+            next.store(n);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
 
 // Kernel launchers (kernels.hip).
 struct FillTask {       // one kernel-matrix block

@@ -21,8 +21,49 @@ namespace bfg {
 namespace {
 inline uint64_t key2(int32_t a, int32_t b) { return (uint64_t(uint32_t(a)) << 32) | uint32_t(b); }
 
-struct SlotMap {
-    std::unordered_map<uint64_t, std::pair<int64_t, int32_t>> m;  // key -> (offset, len)
+// Open-addressing map uint64 -> int32 for the small key sets of one
+// factorization; reused across calls (generation stamps, no clearing).
+struct KeyMap {
+    std::vector<uint64_t> k;
+    std::vector<int32_t> v;
+    std::vector<uint32_t> g;
+    uint32_t gen = 0;
+    size_t mask = 0;
+    void reset(size_t n) {
+        size_t cap = 16;
+        while (cap < 2 * n + 2) cap <<= 1;
+        if (k.size() < cap) {
+            k.assign(cap, 0);
+            v.assign(cap, 0);
+            g.assign(cap, 0);
+            gen = 0;
+        }
+        if (++gen == 0) {
+            std::fill(g.begin(), g.end(), 0u);
+            gen = 1;
+        }
+        mask = cap - 1;
+    }
+    size_t slot(uint64_t key) const {
+        uint64_t x = key * 0x9E3779B97F4A7C15ULL;
+        size_t h = size_t(x ^ (x >> 29)) & mask;
+        while (g[h] == gen && k[h] != key) h = (h + 1) & mask;
+        return h;
+    }
+    int32_t *get(uint64_t key) {
+        size_t h = slot(key);
+        return g[h] == gen ? &v[h] : nullptr;
+    }
+    // insert `val` unless present; returns the stored value and whether
+    // this call inserted it
+    std::pair<int32_t *, bool> emplace(uint64_t key, int32_t val) {
+        size_t h = slot(key);
+        if (g[h] == gen) return {&v[h], false};
+        g[h] = gen;
+        k[h] = key;
+        v[h] = val;
+        return {&v[h], true};
+    }
 };
 
 template <class T> void upload_tab(DevBuf &d, const std::vector<T> &h) {
@@ -30,75 +71,154 @@ template <class T> void upload_tab(DevBuf &d, const std::vector<T> &h) {
     if (!h.empty()) BFG_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
 }
 
-// merge adjacent contributions that read consecutive inputs with
-// consecutive column slices of one column-major block
-// (only within one item: out[first..] holds this item's contributions)
-void merge_contribs(std::vector<MvContrib> &c, int32_t b, int32_t e, std::vector<MvContrib> &out) {
-    const size_t first = out.size();
-    for (int32_t i = b; i < e; ++i) {
-        const MvContrib &x = c[i];
-        if (out.size() > first) {
-            MvContrib &p = out.back();
+// merge adjacent contributions of one item that read consecutive inputs
+// with consecutive column slices of one column-major block; returns the
+// merged count (the merged list is a prefix of c[0, n))
+int32_t merge_contribs(MvContrib *c, int32_t n) {
+    int32_t w = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const MvContrib x = c[i];
+        if (w > 0) {
+            MvContrib &p = c[w - 1];
             if (p.fofs >= 0 && x.fofs >= 0 && p.m == x.m && p.fofs + int64_t(p.m) * p.n == x.fofs &&
                 p.in_ofs + p.n == x.in_ofs) {
                 p.n += x.n;
                 continue;
             }
         }
-        out.push_back(x);
+        c[w++] = x;
     }
+    return w;
+}
+
+// Flat numbering of every block of every factorization: block i of stage s
+// of factorization b is block first[sf[b] + s] + i.
+struct BlockIndex {
+    std::vector<int64_t> sf;     // per factorization: offset into first
+    std::vector<int64_t> first;  // per (factorization, stage): first flat index; + one end entry
+    int64_t total = 0;
+};
+
+BlockIndex index_blocks(const BF &bf) {
+    BlockIndex ix;
+    const int64_t nb = int64_t(bf.bfs.size());
+    ix.sf.assign(nb + 1, 0);
+    for (int64_t b = 0; b < nb; ++b) ix.sf[b + 1] = ix.sf[b] + int64_t(bf.bfs[b].st.size()) + 1;
+    ix.first.assign(size_t(ix.sf[nb]), 0);
+    int64_t g = 0;
+    for (int64_t b = 0; b < nb; ++b) {
+        const OneBF &ob = bf.bfs[b];
+        int64_t *f = &ix.first[ix.sf[b]];
+        for (size_t s = 0; s < ob.st.size(); ++s) {
+            f[s] = g;
+            g += int64_t(ob.st[s].b.size());
+        }
+        f[ob.st.size()] = g;
+    }
+    ix.total = g;
+    return ix;
 }
 }  // namespace
 
-void build_sweep(BF &bf, const std::vector<std::vector<std::vector<int32_t>>> &fin);
+// Every range a launch table names lies inside the buffer it indexes (slot
+// buffer, permuted x / y, the launch's factor data); checked on the host so
+// that an inconsistent block set is an error, not a device fault.
+static void validate_tables(const BF &bf) {
+    const int32_t nL = bf.n_stage_launch, Lmax = nL - 1;
+    const int64_t nx = bf.trial->n_values, ny = bf.test->n_values;
+    for (int32_t L = 0; L < nL; ++L)
+        for (int dir = 0; dir < 2; ++dir) {
+            const bool fwd = dir == 0;
+            const StageTable &S = fwd ? bf.fwd[L] : bf.tr[L];
+            const int64_t out_lim = S.out_kind == OUT_SLOT ? bf.slot_len : (fwd ? ny : nx);
+            const int64_t in_lim = S.in_kind == IN_SLOT ? bf.slot_len : (fwd ? nx : ny);
+            const int64_t fe = L < int32_t(bf.factor_elems.size()) ? bf.factor_elems[L] : 0;
+            std::atomic<int> bad{0};
+            parallel_for(int64_t(S.h_items.size()), [&](int64_t i) {
+                const MvItem &it = S.h_items[i];
+                bool ok = it.out_ofs >= 0 && it.out_len >= 0 && int64_t(it.out_ofs) + it.out_len <= out_lim &&
+                          it.c_begin >= 0 && it.c_begin < it.c_end && it.c_end <= S.n_contrib;
+                for (int32_t c = ok ? it.c_begin : 0; ok && c < it.c_end; ++c) {
+                    const MvContrib &k = S.h_contribs[c];
+                    const int32_t rows = fwd ? k.m : k.n, cols = fwd ? k.n : k.m;
+                    ok = k.m >= 0 && k.n >= 0 && rows == it.out_len && k.in_ofs >= 0 &&
+                         int64_t(k.in_ofs) + cols <= in_lim &&
+                         (k.fofs < 0 ? k.m == k.n : k.fofs + int64_t(k.m) * k.n <= fe);
+                }
+                if (!ok) bad.store(1);
+            }, 4096);
+            BFG_REQUIRE(!bad.load(), "pack: launch table " + std::to_string(L) + (fwd ? " (forward)" : " (transposed)") +
+                                         " indexes outside its buffers");
+        }
+    (void)Lmax;
+}
 
+// Slot layout: every skeleton vector of every factorization in one buffer;
+// per factorization and level in the order its consumers first read them
+// (the column slices one compression reads are then adjacent, so their
+// contributions merge into one), then vectors written but never read.
+// Launch tables: forward items are output vectors (a slot, or an observer
+// leaf's rows of y shared by all factorizations), transposed items input
+// vectors; each item lists its contributions.
 void pack_tables(BF &bf) {
     const Tree &test = *bf.test, &trial = *bf.trial;
     const int32_t nL = bf.n_stage_launch, Lmax = nL - 1;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
-    const size_t nb = bf.bfs.size();
-    // ---- slot offsets, consumer order, all factorizations in one buffer ----
-    std::vector<std::vector<SlotMap>> slots(nb);
-    int64_t counter = 0;
-    for (size_t b = 0; b < nb; ++b) {
+    const int64_t nb = int64_t(bf.bfs.size());
+    const BlockIndex ix = index_blocks(bf);
+    // ---- slot offsets (per factorization, local), reference slot ids ----
+    std::vector<int32_t> sl_out(size_t(ix.total), -1), sl_in(size_t(ix.total), -1);
+    std::vector<int64_t> slot_base(nb + 1, 0);
+    parallel_for(nb, [&](int64_t b) {
+        thread_local KeyMap len, sm, ids;
         OneBF &ob = bf.bfs[b];
         const int32_t ns = int32_t(ob.st.size());
-        slots[b].resize(std::max(ns - 1, 1));
+        const int64_t *f = &ix.first[ix.sf[b]];
+        int64_t cnt = 0;
         for (int32_t s = 0; s + 1 < ns; ++s) {
-            std::unordered_map<uint64_t, int32_t> len;
-            for (const Block &k : ob.st[s].b) len[key2(k.obs_out, k.src_out)] = k.m;
-            SlotMap &sm = slots[b][s];
-            for (const Block &k : ob.st[s + 1].b) {
-                uint64_t kk = key2(k.obs_in, k.src_in);
-                if (sm.m.count(kk)) continue;
-                auto it = len.find(kk);
-                BFG_REQUIRE(it != len.end(), "pack: block reads a skeleton vector no block writes");
-                sm.m[kk] = {counter, it->second};
-                counter += it->second;
+            std::vector<Block> &P = ob.st[s].b, &Cn = ob.st[s + 1].b;
+            len.reset(P.size());
+            for (const Block &k : P) *len.emplace(key2(k.obs_out, k.src_out), k.m).first = k.m;
+            sm.reset(P.size() + Cn.size());
+            for (size_t i = 0; i < Cn.size(); ++i) {
+                const uint64_t kk = key2(Cn[i].obs_in, Cn[i].src_in);
+                auto r = sm.emplace(kk, int32_t(cnt));
+                if (r.second) {
+                    const int32_t *l = len.get(kk);
+                    BFG_REQUIRE(l, "pack: block reads a skeleton vector no block writes");
+                    cnt += *l;
+                }
+                sl_in[size_t(f[s + 1]) + i] = *r.first;
             }
-            for (const Block &k : ob.st[s].b) {  // written but never read
-                uint64_t kk = key2(k.obs_out, k.src_out);
-                if (!sm.m.count(kk)) { sm.m[kk] = {counter, k.m}; counter += k.m; }
+            for (size_t i = 0; i < P.size(); ++i) {
+                auto r = sm.emplace(key2(P[i].obs_out, P[i].src_out), int32_t(cnt));
+                if (r.second) cnt += P[i].m;
+                sl_out[size_t(f[s]) + i] = *r.first;
+            }
+            BFG_REQUIRE(cnt < (int64_t(1) << 31), "pack: skeleton vectors exceed 2^31 entries");
+            // reference slot ids of level s: first use in stage order, the
+            // writers of level s before its readers (bfstructs.jl:107-145)
+            ids.reset(P.size() + Cn.size());
+            int32_t nid = 0;
+            for (Block &k : P) {
+                auto r = ids.emplace(key2(k.obs_out, k.src_out), nid);
+                nid += r.second;
+                k.out_ptr = *r.first;
+            }
+            for (Block &k : Cn) {
+                auto r = ids.emplace(key2(k.obs_in, k.src_in), nid);
+                nid += r.second;
+                k.in_ptr = *r.first;
             }
         }
-        // reference slot ids: first-use order per level (bfstructs.jl:107-145)
-        std::vector<std::unordered_map<uint64_t, int32_t>> ids(std::max(ns - 1, 1));
-        auto idof = [&](int32_t lvl, uint64_t kk) {
-            auto &m = ids[lvl];
-            auto it = m.find(kk);
-            if (it != m.end()) return it->second;
-            int32_t v = int32_t(m.size());
-            m[kk] = v;
-            return v;
-        };
-        for (int32_t s = 0; s < ns; ++s)
-            for (Block &k : ob.st[s].b) {
-                k.in_ptr = s > 0 ? idof(s - 1, key2(k.obs_in, k.src_in)) : -1;
-                k.out_ptr = s < ns - 1 ? idof(s, key2(k.obs_out, k.src_out)) : -1;
-            }
-    }
-    bf.slot_len = std::max<int64_t>(counter, 1);
+        for (Block &k : ob.st[0].b) k.in_ptr = -1;
+        for (Block &k : ob.st[ns - 1].b) k.out_ptr = -1;
+        slot_base[b + 1] = cnt;
+    });
+    for (int64_t b = 0; b < nb; ++b) slot_base[b + 1] += slot_base[b];
+    const int64_t counter = slot_base[nb];
     BFG_REQUIRE(counter < (int64_t(1) << 31), "pack: skeleton vectors exceed 2^31 entries");
+    bf.slot_len = std::max<int64_t>(counter, 1);
 
     // ---- size (bfdims.jl:54-64) and coverage ----
     if (nb == 1) {
@@ -112,15 +232,181 @@ void pack_tables(BF &bf) {
         bf.cols = trial.n_values;
     }
 
-    auto launch_of = [&](const OneBF &ob, int32_t s) {
-        return s == int32_t(ob.st.size()) - 1 ? Lmax : s;
+    // stage of factorization b launched at L (Q = 0, R^l = l, P = Lmax)
+    auto stage_at = [&](int64_t b, int32_t L) -> int32_t {
+        const int32_t ns = int32_t(bf.bfs[b].st.size());
+        if (L == Lmax) return ns - 1;
+        return L < ns - 1 ? L : -1;
+    };
+    std::vector<int32_t> litem(size_t(std::max<int64_t>(ix.total, 1)));
+    // Group the blocks of launch L into items.  fwd: keyed by output,
+    // contributions read the input; transposed: the other way round.
+    auto group = [&](int32_t L, bool fwd, StageTable &S) {
+        const bool node_keyed = fwd ? (L == Lmax) : (L == 0);
+        std::vector<MvItem> items;
+        std::vector<MvContrib> con;
+        auto describe = [&](int64_t b, int32_t s, const Block &k, int64_t g, int64_t &out_ofs,
+                            int64_t &in_ofs) {
+            const bool isQ = s == 0, isP = s == int32_t(bf.bfs[b].st.size()) - 1;
+            const int64_t o = isP ? int64_t(test.val_begin[k.obs_out]) : slot_base[b] + sl_out[g];
+            const int64_t i = isQ ? int64_t(trial.val_begin[k.src_in]) : slot_base[b] + sl_in[g];
+            out_ofs = fwd ? o : i;
+            in_ofs = fwd ? i : o;
+        };
+        if (node_keyed) {
+            // items: the leaves of the shared side, in node order; each
+            // lists its blocks in factorization order
+            const Tree &t = fwd ? test : trial;
+            std::vector<int64_t> cnt(size_t(t.n_nodes) + 1, 0);
+            for (int64_t b = 0; b < nb; ++b) {
+                const int32_t s = stage_at(b, L);
+                if (s < 0) continue;
+                for (const Block &k : bf.bfs[b].st[s].b) ++cnt[size_t(fwd ? k.obs_out : k.src_in) + 1];
+            }
+            std::vector<int32_t> item_of(size_t(t.n_nodes), -1);
+            for (int32_t v = 0; v < t.n_nodes; ++v)
+                if (cnt[v + 1] > 0) {
+                    item_of[v] = int32_t(items.size());
+                    items.push_back(MvItem{0, 0, 0, 0});
+                }
+            for (int32_t v = 0; v < t.n_nodes; ++v) cnt[v + 1] += cnt[v];
+            BFG_REQUIRE(cnt[t.n_nodes] < (int64_t(1) << 31), "pack: too many blocks in one launch");
+            con.resize(size_t(cnt[t.n_nodes]));
+            for (int32_t v = 0; v < t.n_nodes; ++v)
+                if (item_of[v] >= 0) {
+                    items[item_of[v]].c_begin = int32_t(cnt[v]);
+                    items[item_of[v]].c_end = int32_t(cnt[v]);
+                }
+            for (int64_t b = 0; b < nb; ++b) {
+                const int32_t s = stage_at(b, L);
+                if (s < 0) continue;
+                const auto &blk = bf.bfs[b].st[s].b;
+                const int64_t g0 = ix.first[ix.sf[b] + s];
+                for (size_t q = 0; q < blk.size(); ++q) {
+                    const Block &k = blk[q];
+                    int64_t oo, io;
+                    describe(b, s, k, g0 + int64_t(q), oo, io);
+                    MvItem &it = items[item_of[fwd ? k.obs_out : k.src_in]];
+                    if (it.c_end == it.c_begin) {
+                        it.out_ofs = int32_t(oo);
+                        it.out_len = fwd ? k.m : k.n;
+                    }
+                    con[it.c_end++] = MvContrib{k.ident ? -1 : k.fofs, k.m, k.n, int32_t(io), 0};
+                }
+            }
+        } else {
+            // items: vectors of one factorization, first-use order within it
+            std::vector<int64_t> nit(nb + 1, 0), nct(nb + 1, 0);
+            parallel_for(nb, [&](int64_t b) {
+                const int32_t s = stage_at(b, L);
+                if (s < 0) return;
+                thread_local KeyMap km;
+                const auto &blk = bf.bfs[b].st[s].b;
+                const int64_t g0 = ix.first[ix.sf[b] + s];
+                km.reset(blk.size());
+                int32_t n = 0;
+                for (size_t q = 0; q < blk.size(); ++q) {
+                    const int64_t g = g0 + int64_t(q);
+                    auto r = km.emplace(uint64_t(uint32_t(fwd ? sl_out[g] : sl_in[g])), n);
+                    n += r.second;
+                    litem[g] = *r.first;
+                }
+                nit[b + 1] = n;
+                nct[b + 1] = int64_t(blk.size());
+            });
+            for (int64_t b = 0; b < nb; ++b) {
+                nit[b + 1] += nit[b];
+                nct[b + 1] += nct[b];
+            }
+            BFG_REQUIRE(nct[nb] < (int64_t(1) << 31), "pack: too many blocks in one launch");
+            items.resize(size_t(nit[nb]));
+            con.resize(size_t(nct[nb]));
+            parallel_for(nb, [&](int64_t b) {
+                const int32_t s = stage_at(b, L);
+                if (s < 0) return;
+                thread_local std::vector<int32_t> cur;
+                const auto &blk = bf.bfs[b].st[s].b;
+                const int64_t g0 = ix.first[ix.sf[b] + s];
+                const int32_t n = int32_t(nit[b + 1] - nit[b]);
+                cur.assign(size_t(n) + 1, 0);
+                for (size_t q = 0; q < blk.size(); ++q) ++cur[size_t(litem[g0 + int64_t(q)]) + 1];
+                for (int32_t j = 0; j < n; ++j) cur[j + 1] += cur[j];
+                for (int32_t j = 0; j < n; ++j) {
+                    MvItem &it = items[size_t(nit[b] + j)];
+                    it.c_begin = it.c_end = int32_t(nct[b] + cur[j]);
+                }
+                for (size_t q = 0; q < blk.size(); ++q) {
+                    const Block &k = blk[q];
+                    int64_t oo, io;
+                    describe(b, s, k, g0 + int64_t(q), oo, io);
+                    MvItem &it = items[size_t(nit[b] + litem[g0 + int64_t(q)])];
+                    if (it.c_end == it.c_begin) {
+                        it.out_ofs = int32_t(oo);
+                        it.out_len = fwd ? k.m : k.n;
+                    }
+                    con[size_t(it.c_end++)] = MvContrib{k.ident ? -1 : k.fofs, k.m, k.n, int32_t(io), 0};
+                }
+            });
+        }
+        int64_t fe = 0;
+        for (const MvContrib &c : con)
+            if (c.fofs >= 0) fe += int64_t(c.m) * c.n;
+        S.factor_elems = fe;
+        if (fwd) {
+            // merge within each item, then compact
+            std::vector<int32_t> kept(items.size() + 1, 0);
+            parallel_for(int64_t(items.size()), [&](int64_t i) {
+                MvItem &it = items[i];
+                kept[i + 1] = merge_contribs(con.data() + it.c_begin, it.c_end - it.c_begin);
+            }, 4096);
+            for (size_t i = 0; i < items.size(); ++i) kept[i + 1] += kept[i];
+            std::vector<MvContrib> out(static_cast<size_t>(kept[items.size()]));
+            parallel_for(int64_t(items.size()), [&](int64_t i) {
+                MvItem &it = items[i];
+                std::copy(con.begin() + it.c_begin, con.begin() + it.c_begin + (kept[i + 1] - kept[i]),
+                          out.begin() + kept[i]);
+                it.c_begin = kept[i];
+                it.c_end = kept[i + 1];
+            }, 4096);
+            con.swap(out);
+        }
+        S.h_items.swap(items);
+        S.h_contribs.swap(con);
+    };
+    auto finish = [&](StageTable &S, bool fwd) {
+        int64_t vec = 0;
+        S.max_out = S.max_in = 0;
+        for (const MvItem &it : S.h_items) {
+            S.max_out = std::max(S.max_out, it.out_len);
+            vec += int64_t(it.out_len) * ((S.out_kind == OUT_SCATTER) ? 2 : 1);
+        }
+        for (const MvContrib &c : S.h_contribs) {
+            const int32_t in_len = fwd ? c.n : c.m;
+            S.max_in = std::max(S.max_in, in_len);
+            vec += in_len;
+        }
+        S.n_items = int32_t(S.h_items.size());
+        S.n_contrib = int32_t(S.h_contribs.size());
+        S.alg_bytes_1 = vec * es;
+        S.h_tasks.resize(S.h_items.size());
+        parallel_for(int64_t(S.h_items.size()), [&](int64_t i) {
+            const MvItem &it = S.h_items[i];
+            const MvContrib &c0 = S.h_contribs[it.c_begin];
+            S.h_tasks[i] = MvTask{c0.fofs, c0.m, c0.n, c0.in_ofs, it.out_ofs, it.out_len,
+                                  it.c_begin + 1, it.c_end, -1, 0, -1, 0};
+        }, 4096);
+        // default forward variant until the autotuner measures: a
+        // wavefront per task for small blocks, a workgroup otherwise
+        S.variant = (S.n_items > 0 && S.factor_elems / S.n_items < 4096) ? 1 : 0;
+        upload_tab(S.items, S.h_items);
+        upload_tab(S.contribs, S.h_contribs);
+        upload_tab(S.tasks, S.h_tasks);
     };
 
     bf.fwd.clear();
     bf.tr.clear();
     bf.fwd.resize(nL);
     bf.tr.resize(nL);
-    std::vector<std::vector<std::vector<int32_t>>> fin(nL);
     int64_t covered_rows = 0, covered_cols = 0;
     for (int32_t L = 0; L < nL; ++L) {
         StageTable &F = bf.fwd[L], &T = bf.tr[L];
@@ -128,104 +414,10 @@ void pack_tables(BF &bf) {
         F.out_kind = (L == Lmax) ? OUT_SCATTER : OUT_SLOT;
         T.in_kind = (L == Lmax) ? IN_GATHER : IN_SLOT;
         T.out_kind = (L == 0) ? OUT_SCATTER : OUT_SLOT;
-        // forward items keyed by output target, transposed by input source
-        std::map<std::pair<int64_t, int64_t>, int32_t> fidx, tidx;
-        std::vector<std::vector<MvContrib>> fc, tc;
-        std::vector<MvItem> fi, ti;
-        for (size_t b = 0; b < nb; ++b) {
-            const OneBF &ob = bf.bfs[b];
-            for (int32_t s = 0; s < int32_t(ob.st.size()); ++s) {
-                if (launch_of(ob, s) != L) continue;
-                const bool isQ = (s == 0), isP = (s == int32_t(ob.st.size()) - 1);
-                for (const Block &k : ob.st[s].b) {
-                    // output side (forward target / transposed source)
-                    int64_t o_ofs;
-                    std::pair<int64_t, int64_t> okey;
-                    if (isP) {
-                        o_ofs = test.val_begin[k.obs_out];
-                        okey = {-1, k.obs_out};
-                    } else {
-                        o_ofs = slots[b][s].m.at(key2(k.obs_out, k.src_out)).first;
-                        okey = {int64_t(b), o_ofs};
-                    }
-                    // input side
-                    int64_t i_ofs;
-                    std::pair<int64_t, int64_t> ikey;
-                    if (isQ) {
-                        i_ofs = trial.val_begin[k.src_in];
-                        ikey = {-1, k.src_in};
-                    } else {
-                        i_ofs = slots[b][s - 1].m.at(key2(k.obs_in, k.src_in)).first;
-                        ikey = {int64_t(b), i_ofs};
-                    }
-                    int64_t fofs = k.ident ? -1 : k.fofs;
-                    // forward: out[m] += A (m x n) in[n]
-                    auto fit = fidx.find(okey);
-                    if (fit == fidx.end()) {
-                        fit = fidx.emplace(okey, int32_t(fi.size())).first;
-                        fi.push_back(MvItem{int32_t(o_ofs), k.m, 0, 0});
-                        fc.emplace_back();
-                    }
-                    fc[fit->second].push_back(MvContrib{fofs, k.m, k.n, int32_t(i_ofs), 0});
-                    // transposed: in[n] += op(A)^T out[m]
-                    auto tit = tidx.find(ikey);
-                    if (tit == tidx.end()) {
-                        tit = tidx.emplace(ikey, int32_t(ti.size())).first;
-                        ti.push_back(MvItem{int32_t(i_ofs), k.n, 0, 0});
-                        tc.emplace_back();
-                    }
-                    tc[tit->second].push_back(MvContrib{fofs, k.m, k.n, int32_t(o_ofs), 0});
-                    if (k.ident) F.factor_elems += 0;
-                    else F.factor_elems += int64_t(k.m) * k.n;
-                }
-            }
-        }
-        T.factor_elems = F.factor_elems;
-        auto finish = [&](StageTable &S, std::vector<MvItem> &items, std::vector<std::vector<MvContrib>> &cl,
-                          bool fwd) {
-            S.h_items.clear();
-            S.h_contribs.clear();
-            int64_t vec = 0;
-            for (size_t i = 0; i < items.size(); ++i) {
-                MvItem it = items[i];
-                it.c_begin = int32_t(S.h_contribs.size());
-                if (fwd) merge_contribs(cl[i], 0, int32_t(cl[i].size()), S.h_contribs);
-                else S.h_contribs.insert(S.h_contribs.end(), cl[i].begin(), cl[i].end());
-                it.c_end = int32_t(S.h_contribs.size());
-                S.h_items.push_back(it);
-                S.max_out = std::max(S.max_out, it.out_len);
-                vec += it.out_len * ((S.out_kind == OUT_SCATTER) ? 2 : 1);
-            }
-            for (const MvContrib &c : S.h_contribs) {
-                int32_t in_len = fwd ? c.n : c.m;
-                S.max_in = std::max(S.max_in, in_len);
-                vec += in_len;
-            }
-            S.n_items = int32_t(S.h_items.size());
-            S.n_contrib = int32_t(S.h_contribs.size());
-            S.alg_bytes_1 = vec * es;
-            S.h_tasks.clear();
-            for (const MvItem &it : S.h_items) {
-                const MvContrib &c0 = S.h_contribs[it.c_begin];
-                S.h_tasks.push_back(MvTask{c0.fofs, c0.m, c0.n, c0.in_ofs, it.out_ofs, it.out_len,
-                                           it.c_begin + 1, it.c_end, -1, 0, -1, 0});
-            }
-            // default forward variant until the autotuner measures: a
-            // wavefront per task for small blocks, a workgroup otherwise
-            S.variant = (S.n_items > 0 && S.factor_elems / S.n_items < 4096) ? 1 : 0;
-            upload_tab(S.items, S.h_items);
-            upload_tab(S.contribs, S.h_contribs);
-            upload_tab(S.tasks, S.h_tasks);
-        };
-        // input slots of every forward item (before merging), for the sweep
-        fin[L].resize(fc.size());
-        for (size_t i = 0; i < fc.size(); ++i) {
-            for (const MvContrib &c : fc[i]) fin[L][i].push_back(c.in_ofs);
-            std::sort(fin[L][i].begin(), fin[L][i].end());
-            fin[L][i].erase(std::unique(fin[L][i].begin(), fin[L][i].end()), fin[L][i].end());
-        }
-        finish(F, fi, fc, true);
-        finish(T, ti, tc, false);
+        group(L, true, F);
+        group(L, false, T);
+        finish(F, true);
+        finish(T, false);
         if (L == Lmax)
             for (const MvItem &it : F.h_items) covered_rows += it.out_len;
         if (L == 0)
@@ -233,31 +425,54 @@ void pack_tables(BF &bf) {
     }
     bf.y_covered = covered_rows == bf.rows;
     bf.x_covered = covered_cols == bf.cols;
+    validate_tables(bf);
     upload_tab(bf.perm_test, test.perm);
     upload_tab(bf.perm_trial, trial.perm);
-    build_sweep(bf, fin);
+    bf.sweep = SweepTables{};
+    if (bf.use_sweep) build_sweep(bf);
 }
 
 // Fused-sweep tables: every forward task of every stage in stage order.
 // Items that read the same set of skeleton vectors form one dependency
 // group with one counter; each produced vector is read by exactly one group
 // (butterfly data flow), so its producer signals that group's counter once.
-void build_sweep(BF &bf, const std::vector<std::vector<std::vector<int32_t>>> &fin) {
+// The vectors an item reads are the producer outputs inside the input
+// ranges of its (merged) contributions.
+void build_sweep(BF &bf) {
     SweepTables &sw = bf.sweep;
-    sw.ok = false;
+    const int32_t mode = sw.mode, wg = sw.wg_per_cu;
+    sw = SweepTables{};
+    sw.mode = mode;
+    sw.wg_per_cu = wg;
+    sw.built = true;
     const int32_t nL = bf.n_stage_launch;
     if (nL > SWEEP_MAX_STAGES) return;
+    std::vector<int32_t> starts;  // slot offsets of every produced vector
+    for (int32_t L = 0; L + 1 < nL; ++L)
+        for (const MvItem &it : bf.fwd[L].h_items) starts.push_back(it.out_ofs);
+    std::sort(starts.begin(), starts.end());
     std::map<std::vector<int32_t>, int32_t> gid;
     std::unordered_map<int32_t, int32_t> consumer;  // slot offset -> group
     std::vector<int32_t> gcnt;
     std::vector<std::vector<int32_t>> wait(nL);
+    std::vector<int32_t> set;
     for (int32_t L = 1; L < nL; ++L) {
-        wait[L].resize(fin[L].size());
-        for (size_t i = 0; i < fin[L].size(); ++i) {
-            const auto &set = fin[L][i];
-            auto it = gid.find(set);
+        const StageTable &S = bf.fwd[L];
+        wait[L].resize(S.h_items.size());
+        for (size_t i = 0; i < S.h_items.size(); ++i) {
+            const MvItem &it = S.h_items[i];
+            set.clear();
+            for (int32_t c = it.c_begin; c < it.c_end; ++c) {
+                const MvContrib &k = S.h_contribs[c];
+                auto lo = std::lower_bound(starts.begin(), starts.end(), k.in_ofs);
+                auto hi = std::lower_bound(starts.begin(), starts.end(), k.in_ofs + k.n);
+                set.insert(set.end(), lo, hi);
+            }
+            std::sort(set.begin(), set.end());
+            set.erase(std::unique(set.begin(), set.end()), set.end());
+            auto f = gid.find(set);
             int32_t g;
-            if (it == gid.end()) {
+            if (f == gid.end()) {
                 g = int32_t(gcnt.size());
                 gid.emplace(set, g);
                 gcnt.push_back(int32_t(set.size()));
@@ -267,7 +482,7 @@ void build_sweep(BF &bf, const std::vector<std::vector<std::vector<int32_t>>> &f
                     consumer[s] = g;
                 }
             } else {
-                g = it->second;
+                g = f->second;
             }
             wait[L][i] = g;
         }
