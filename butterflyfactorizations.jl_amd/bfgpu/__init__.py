@@ -487,12 +487,19 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_set_fused(self.handle, 1 if enable else 0))
         return self
 
-    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3}
+    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5}
+
+    @staticmethod
+    def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8):
+        """Packed value of the "tile" option."""
+        return int(rp_shift) | int(g_shift) << 4 | int(csplit) << 8 | int(wg_per_cu) << 12
 
     def set_option(self, key, value):
         """Launch tuning: sweep (0 per-level, 1 fused dynamic, 2 fused
         cooperative), wg_per_cu (fused sweep occupancy), graph (replay
-        per-level launches from a HIP graph)."""
+        per-level launches from a HIP graph), variant (0 workgroup per
+        task, 1 wavefront per task, 2 tiled), tile (tiled shape, see
+        tile_shape)."""
         check(lib().bfgpu_bf_set_option(self.handle, self.OPTIONS[key], int(value)))
         return self
 
@@ -513,6 +520,17 @@ class ButterflyFactorization:
     def check(self):
         """Synchronise and raise if the fused sweep flagged a fault."""
         check(lib().bfgpu_bf_check(self.handle))
+
+    def launch_stats(self):
+        """Per level in launch order of this operator's direction: items,
+        contributions, most contributions of an item, factor entries,
+        vector bytes per RHS, kernel variant, packed tiled shape,
+        partial-sum items."""
+        n = C.c_int32()
+        check(lib().bfgpu_bf_launch_stats(self.handle, self.op, C.byref(n), None))
+        st = np.zeros((n.value, 8), np.int64)
+        check(lib().bfgpu_bf_launch_stats(self.handle, self.op, C.byref(n), ptr(st, C.c_int64)))
+        return st
 
     def stage_stats(self):
         """Per launch stage: blocks, identity blocks, max rank, factor bytes

@@ -80,6 +80,7 @@ _SIGS = {
     "bfgpu_bf_export_logs": (i32, [vp, i32, P(i32), P(i32), P(i32)]),
     "bfgpu_bf_export_features": (i32, [vp, i32, P(f32)]),
     "bfgpu_bf_stage_stats": (i32, [vp, P(i32), P(i64)]),
+    "bfgpu_bf_launch_stats": (i32, [vp, i32, P(i32), P(i64)]),
     "bfgpu_bf_convert": (i32, [vp, i32, P(vp)]),
     "bfgpu_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
     "bfgpu_matvec_timed": (i32, [vp, i32, i64, vp, i64, vp, i64, i32, P(f64), P(i64), P(i32), P(f64)]),

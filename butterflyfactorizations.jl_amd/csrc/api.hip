@@ -384,6 +384,35 @@ extern "C" int bfgpu_bf_stage_stats(const bfgpu_bf *h, int32_t *n_stages, int64_
     BFG_API_END
 }
 
+static int64_t pack_tile(const TileShape &t) {
+    return int64_t(t.rp_shift) | int64_t(t.g_shift) << 4 | int64_t(t.csplit) << 8 | int64_t(t.wg_per_cu) << 12;
+}
+
+extern "C" int bfgpu_bf_launch_stats(const bfgpu_bf *h, int32_t op, int32_t *n_levels, int64_t *stats) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && n_levels, "bf_launch_stats: null argument");
+    BFG_REQUIRE(op == BFGPU_OP_N || op == BFGPU_OP_T || op == BFGPU_OP_C, "bf_launch_stats: bad op");
+    const BF &bf = h->b;
+    const int32_t nL = bf.n_stage_launch;
+    *n_levels = nL;
+    if (!stats) return BFGPU_OK;
+    const bool fwd = op == BFGPU_OP_N;
+    for (int32_t i = 0; i < nL; ++i) {
+        const int32_t L = fwd ? i : nL - 1 - i;
+        const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
+        int64_t *o = stats + 8 * i;
+        o[0] = t.n_items;
+        o[1] = t.n_contrib + (t.pre ? t.pre->n_contrib : 0);
+        o[2] = std::max(t.max_nc, t.pre ? t.pre->max_nc : 0);
+        o[3] = t.factor_elems + (t.pre ? t.pre->factor_elems : 0);
+        o[4] = t.alg_bytes_1 + (t.pre ? t.pre->alg_bytes_1 : 0);
+        o[5] = t.variant;
+        o[6] = pack_tile(t.tile);
+        o[7] = t.pre ? t.pre->n_items : 0;
+    }
+    BFG_API_END
+}
+
 extern "C" int bfgpu_bf_convert(bfgpu_bf *h, int32_t precision, bfgpu_bf **out) {
     BFG_API_BEGIN
     BFG_REQUIRE(h && out, "bf_convert: null argument");
@@ -469,6 +498,37 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         bf.use_graph = value != 0;
         release_graph(bf);
         break;
+    case BFGPU_OPT_VARIANT:
+        BFG_REQUIRE(value >= 0 && value <= 2, "bf_set_option: kernel variant 0..2");
+        for (int dir = 0; dir < 2; ++dir)
+            for (StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
+                const int32_t v = (dir == 1 && value == 1) ? 0 : value;
+                t.variant = v;
+                if (t.pre) t.pre->variant = v;
+            }
+        release_graph(bf);
+        break;
+    case BFGPU_OPT_TILE: {
+        TileShape sh;
+        sh.rp_shift = int8_t(value & 15);
+        sh.g_shift = int8_t((value >> 4) & 15);
+        sh.csplit = int8_t((value >> 8) & 15);
+        sh.wg_per_cu = int16_t((value >> 12) & 255);
+        BFG_REQUIRE(value >= 0 && sh.rp_shift + sh.g_shift <= 6 && sh.csplit <= 1 && sh.wg_per_cu >= 1 &&
+                        sh.wg_per_cu <= 64,
+                    "bf_set_option: bad tile shape");
+        for (int dir = 0; dir < 2; ++dir)
+            for (StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
+                t.variant = 2;
+                t.tile = sh;
+                if (t.pre) {
+                    t.pre->variant = 2;
+                    t.pre->tile = sh;
+                }
+            }
+        release_graph(bf);
+        break;
+    }
     default:
         BFG_REQUIRE(false, "bf_set_option: unknown key");
     }
@@ -483,6 +543,17 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_SWEEP: *value = bf.use_sweep ? (bf.sweep.mode == 0 ? 1 : 2) : 0; break;
     case BFGPU_OPT_WG_PER_CU: *value = bf.sweep.wg_per_cu; break;
     case BFGPU_OPT_GRAPH: *value = bf.use_graph ? 1 : 0; break;
+    case BFGPU_OPT_VARIANT: {
+        int32_t v = bf.fwd.empty() ? 0 : bf.fwd[0].variant;
+        for (int dir = 0; dir < 2; ++dir)
+            for (const StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
+                const int32_t w = (dir == 1 && v == 1) ? 0 : v;
+                if (t.variant != w || (t.pre && t.pre->variant != w)) v = -1;
+            }
+        *value = v;
+        break;
+    }
+    case BFGPU_OPT_TILE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }
     BFG_API_END

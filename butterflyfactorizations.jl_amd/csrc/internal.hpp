@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -138,17 +139,34 @@ struct MvTask {
 enum InKind { IN_SLOT = 0, IN_GATHER = 1 };
 enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
 
+// Launch shape of the tiled stage kernel: 2^rp_shift output entries times
+// 2^g_shift lane groups per task; groups split the inner index (csplit 0)
+// or the contributions (csplit 1); persistent grid of wg_per_cu
+// workgroups per CU.
+struct TileShape {
+    int8_t rp_shift = 3, g_shift = 0, csplit = 0;
+    int16_t wg_per_cu = 8;
+};
+
 struct StageTable {
     int32_t n_items = 0, n_contrib = 0;
     int32_t in_kind = IN_SLOT, out_kind = OUT_SLOT;
     int32_t max_in = 0, max_out = 0;   // longest input / output vector
     int64_t alg_bytes_1 = 0;           // algorithmic bytes per RHS (factor bytes separate)
     int64_t factor_elems = 0;          // factor entries read
-    int32_t variant = 0;               // forward kernel: 0 workgroup / 1 wavefront per task
+    // kernel: 0 workgroup per task, 1 wavefront per task (forward only),
+    // 2 tiled (several tasks per wavefront, persistent)
+    int32_t variant = 0;
+    TileShape tile;
+    int32_t in_lo = 0, in_hi = 0;      // IN_GATHER: value range read through the permutation
+    int32_t max_nc = 0;                // most contributions of one item
     DevBuf items, contribs, tasks;
     std::vector<MvItem> h_items;
     std::vector<MvContrib> h_contribs;
     std::vector<MvTask> h_tasks;
+    // items split into partial sums: `pre` writes the partials into slots,
+    // this table's combine items add them (identity contributions)
+    std::unique_ptr<StageTable> pre;
 };
 
 constexpr int SWEEP_MAX_STAGES = 48;
@@ -171,7 +189,9 @@ struct BF {
     int32_t n_stage_launch = 0;   // Q, R^1..R^{Lmax-1}, P
     std::vector<DevBuf> factors;  // per launch stage: packed factor data
     std::vector<int64_t> factor_elems;
-    int64_t slot_len = 0;
+    int64_t slot_len = 0;         // per RHS: skeleton vectors, partial sums, permuted inputs
+    int64_t part_ofs = 0;         // first partial-sum entry
+    int64_t gather_ofs = 0;       // permuted inputs: value v at gather_ofs + v
     int64_t rows = 0, cols = 0;
     bool y_covered = true;        // forward P covers every row of y
     bool x_covered = true;        // transposed Q covers every column
@@ -262,6 +282,11 @@ void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const vo
                   const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
                   int64_t slot_len, const int32_t *perm_in, const int32_t *perm_out,
                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta);
+void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
+                 int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                 int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
+void launch_gather(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm,
+                   int32_t lo, int32_t hi, void *dst, int64_t ld_dst, int64_t nrhs);
 // Fused forward sweep: every stage in one cooperative launch, tasks ordered
 // by stage, dependencies through per-group counters.
 void launch_sweep(hipStream_t s, int prec, const SweepTables &sw, const std::vector<DevBuf> &factors,

@@ -587,6 +587,222 @@ static void launch_fwd_t(hipStream_t s, const StageTable &t, const void *factors
     BFG_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Tiled stage kernel, forward (OP 0) and transposed (OP 1, conjugated when
+// `conj`).  A wavefront holds 64 >> (rp_shift + g_shift) tasks; the lanes of
+// one task are RP = 2^rp_shift output entries times G = 2^g_shift groups.
+//   CSPLIT = false: group g takes the inner indices k = g, g + G, ... of
+//     every contribution (one long block: leaf-level Q).
+//   CSPLIT = true: group g takes contributions g, g + G, ... whole (many
+//     short blocks: observer-leaf sums, transposed levels).
+// Each pass issues QW factor and QW input loads per lane, so RP x G x QW
+// entries of every task of the wave are in flight at once; groups meet by
+// xor-shuffles inside the task's lanes (fixed order, deterministic).
+// Waves are persistent: a wave walks task groups with a grid stride and
+// loads the next group's descriptors (and each group the descriptor after
+// the next contribution) before using the current ones.  Inputs come from
+// the slot buffer at in_shift (IN_GATHER inputs are permuted there first by
+// gather_kernel); outputs are slots, or rows of y through a permutation.
+// Element (i, k) of a forward block is F[fofs + i + k m]; the transposed
+// product reads F[fofs + k + i m] (column i of the block).
+// ---------------------------------------------------------------------------
+template <typename R, int OP, int OUTK, int NR, int QW, bool CSPLIT>
+__global__ __launch_bounds__(256) void tile_kernel(
+    const MvTask *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
+    C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
+    int fuse_beta, int nrhs, int rp_shift, int g_shift, int conj) {
+    const int lane = threadIdx.x & 63;
+    const int RP = 1 << rp_shift, G = 1 << g_shift;
+    const int lpt_shift = rp_shift + g_shift;
+    const int tpw = 64 >> lpt_shift;
+    const int sub = lane >> lpt_shift;
+    const int i = lane & (RP - 1);
+    const int g = (lane >> rp_shift) & (G - 1);
+    const int rb = blockIdx.y * NR;
+    const int nr = min(NR, nrhs - rb);
+    const R csign = conj ? R(-1) : R(1);
+    const C<R> *in = slots + in_shift + int64_t(rb) * slot_len;
+    const MvContrib none{-1, 0, 0, 0, 0};
+    const int64_t n_waves = int64_t(gridDim.x) * 4;
+    int64_t grp = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+
+    auto load_task = [&](int64_t gi) -> MvTask {
+        const int64_t t = gi * tpw + sub;
+        if (t < n_tasks) return tasks[t];
+        MvTask z{};
+        z.fofs = -1;
+        return z;
+    };
+    // contribution q of a task: q = 0 is the inline one
+    auto contrib = [&](const MvTask &tk, int q) -> MvContrib {
+        if (q == 0) return MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
+        const int ci = tk.c_begin + q - 1;
+        return ci < tk.c_end ? contribs[ci] : none;
+    };
+
+    MvTask tk = load_task(grp);
+    while (grp * tpw < n_tasks) {
+        const int64_t ngrp = grp + n_waves;
+        const MvTask nt = load_task(ngrp);
+        const int nc = tk.out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
+        for (int row0 = 0; row0 < tk.out_len; row0 += RP) {
+            const int row = row0 + i;
+            const bool rowok = row < tk.out_len;
+            C<R> acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) acc[r] = C<R>{R(0), R(0)};
+            auto apply = [&](const MvContrib &c, int kbeg, int kstep) {
+                if (c.fofs < 0) {  // identity (m == n) or none (m == 0)
+                    if (c.m > 0 && kbeg == 0 && rowok)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r)
+                            if (r < nr) acc[r] = cadd(acc[r], in[int64_t(r) * slot_len + c.in_ofs + row]);
+                    return;
+                }
+                const int K = OP == 0 ? c.n : c.m;
+                for (int k0 = kbeg; k0 < K; k0 += kstep * QW) {
+                    C<R> a[QW], xv[NR][QW];
+#pragma unroll
+                    for (int q = 0; q < QW; ++q) {
+                        const int k = k0 + q * kstep;
+                        const bool ok = rowok && k < K;
+                        const int64_t e = OP == 0 ? c.fofs + row + int64_t(k) * c.m : c.fofs + k + int64_t(row) * c.m;
+                        a[q] = ok ? F[e] : C<R>{R(0), R(0)};
+                        a[q].y *= csign;
+#pragma unroll
+                        for (int r = 0; r < NR; ++r)
+                            xv[r][q] = (ok && r < nr) ? in[int64_t(r) * slot_len + c.in_ofs + k] : C<R>{R(0), R(0)};
+                    }
+#pragma unroll
+                    for (int q = 0; q < QW; ++q)
+#pragma unroll
+                        for (int r = 0; r < NR; ++r) cfma(acc[r], a[q], xv[r][q]);
+                }
+            };
+            if (CSPLIT) {
+                // group g: contributions g, g + G, ...; descriptor of the next
+                // one loaded before the current one is applied
+                MvContrib cur = g < nc ? contrib(tk, g) : none;
+                for (int q = g; q < nc; q += G) {
+                    const MvContrib nx = q + G < nc ? contrib(tk, q + G) : none;
+                    apply(cur, 0, 1);
+                    cur = nx;
+                }
+            } else {
+                MvContrib cur = contrib(tk, 0);
+                for (int q = 0; q < nc; ++q) {
+                    const MvContrib nx = q + 1 < nc ? contrib(tk, q + 1) : none;
+                    apply(cur, g, G);
+                    cur = nx;
+                }
+            }
+            for (int s = RP; s < (RP << g_shift); s <<= 1)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    acc[r].x += __shfl_xor(acc[r].x, s, 64);
+                    acc[r].y += __shfl_xor(acc[r].y, s, 64);
+                }
+            if (g == 0 && rowok) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    if (r >= nr) break;
+                    if (OUTK == OUT_SLOT) {
+                        slots[int64_t(rb + r) * slot_len + tk.out_ofs + row] = acc[r];
+                    } else {
+                        C<R> *yp = y + int64_t(rb + r) * ldy + perm_out[tk.out_ofs + row];
+                        const C<R> old = *yp;
+                        *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc[r]));
+                    }
+                }
+            }
+        }
+        tk = nt;
+        grp = ngrp;
+    }
+}
+
+// dst[r][lo + v] = x[r][perm[lo + v]] for v < n: the inputs of an IN_GATHER
+// stage in tree order, so the tile kernel reads them contiguously.
+template <typename R>
+__global__ void gather_kernel(const C<R> *__restrict__ x, int64_t ldx, const int32_t *__restrict__ perm,
+                              int32_t lo, int32_t n, C<R> *__restrict__ dst, int64_t ld_dst) {
+    const int32_t v = int32_t(blockIdx.x) * 256 + int32_t(threadIdx.x);
+    if (v >= n) return;
+    const int64_t r = blockIdx.y;
+    dst[r * ld_dst + lo + v] = x[r * ldx + perm[lo + v]];
+}
+
+void launch_gather(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm,
+                   int32_t lo, int32_t hi, void *dst, int64_t ld_dst, int64_t nrhs) {
+    const int32_t n = hi - lo;
+    if (n <= 0 || nrhs <= 0) return;
+    dim3 grid(unsigned((n + 255) / 256), unsigned(nrhs));
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(gather_kernel<double>, grid, dim3(256), 0, s, (const C<double> *)x, ldx, perm, lo, n,
+                           (C<double> *)dst, ld_dst);
+    else
+        hipLaunchKernelGGL(gather_kernel<float>, grid, dim3(256), 0, s, (const C<float> *)x, ldx, perm, lo, n,
+                           (C<float> *)dst, ld_dst);
+    BFG_HIP(hipGetLastError());
+}
+
+template <typename R, int OP, int NR, int QW>
+static void launch_tile_t(hipStream_t s, const StageTable &t, const void *factors, void *slots,
+                          int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                          int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu, int conj) {
+    C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
+    const int lpt = t.tile.rp_shift + t.tile.g_shift;
+    BFG_REQUIRE(t.tile.rp_shift >= 0 && t.tile.g_shift >= 0 && lpt <= 6, "matvec: bad tile shape");
+    const int64_t tpw = 64 >> lpt;
+    const int64_t wgs = ((int64_t(t.n_items) + tpw - 1) / tpw + 3) / 4;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))), unsigned((nrhs + NR - 1) / NR));
+    auto F = (const C<R> *)factors;
+    auto S = (C<R> *)slots;
+    auto Y = (C<R> *)y;
+    auto T = t.tasks.as<const MvTask>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_TILE(OUTK, CS)                                                                              \
+    hipLaunchKernelGGL((tile_kernel<R, OP, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S, \
+                       slot_len, in_shift, Y, ldy, perm_out, a, b, int(fuse), int(nrhs), int(t.tile.rp_shift),  \
+                       int(t.tile.g_shift), conj)
+    if (t.out_kind == OUT_SLOT) {
+        if (t.tile.csplit) BFG_TILE(OUT_SLOT, true);
+        else BFG_TILE(OUT_SLOT, false);
+    } else {
+        if (t.tile.csplit) BFG_TILE(OUT_SCATTER, true);
+        else BFG_TILE(OUT_SCATTER, false);
+    }
+#undef BFG_TILE
+    BFG_HIP(hipGetLastError());
+}
+
+template <typename R, int OP>
+static void launch_tile_nr(hipStream_t s, const StageTable &t, const void *factors, void *slots,
+                           int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                           int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu, int conj) {
+    if (nrhs == 1) launch_tile_t<R, OP, 1, 8>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, conj);
+    else if (nrhs == 2) launch_tile_t<R, OP, 2, 8>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, conj);
+    else if (nrhs <= 4) launch_tile_t<R, OP, 4, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, conj);
+    else launch_tile_t<R, OP, 8, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, conj);
+}
+
+void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
+                 int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                 int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu) {
+    if (t.n_items == 0 || nrhs == 0) return;
+    BFG_REQUIRE(nrhs < 65536 * 8, "matvec: nrhs too large");
+    const int conj = op == BFGPU_OP_C ? 1 : 0;
+    if (prec == BFGPU_C128) {
+        if (op == BFGPU_OP_N) launch_tile_nr<double, 0>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu, 0);
+        else launch_tile_nr<double, 1>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu, conj);
+    } else {
+        if (op == BFGPU_OP_N) launch_tile_nr<float, 0>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu, 0);
+        else launch_tile_nr<float, 1>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu, conj);
+    }
+}
+
 template <typename R>
 static void launch_fwd(hipStream_t s, const StageTable &t, const void *factors, const void *x,
                        int64_t ldx, void *y, int64_t ldy, void *slots, int64_t slot_len,

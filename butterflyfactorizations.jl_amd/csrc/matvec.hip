@@ -11,6 +11,7 @@
 // with transposed (conjugated) block access instead of materialised copies.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <unordered_map>
 
@@ -127,9 +128,11 @@ static void validate_tables(const BF &bf) {
     const int32_t nL = bf.n_stage_launch, Lmax = nL - 1;
     const int64_t nx = bf.trial->n_values, ny = bf.test->n_values;
     for (int32_t L = 0; L < nL; ++L)
-        for (int dir = 0; dir < 2; ++dir) {
-            const bool fwd = dir == 0;
-            const StageTable &S = fwd ? bf.fwd[L] : bf.tr[L];
+        for (int dir = 0; dir < 4; ++dir) {
+            const bool fwd = (dir & 1) == 0;
+            const StageTable &M = fwd ? bf.fwd[L] : bf.tr[L];
+            if (dir >= 2 && !M.pre) continue;
+            const StageTable &S = dir >= 2 ? *M.pre : M;
             const int64_t out_lim = S.out_kind == OUT_SLOT ? bf.slot_len : (fwd ? ny : nx);
             const int64_t in_lim = S.in_kind == IN_SLOT ? bf.slot_len : (fwd ? nx : ny);
             const int64_t fe = L < int32_t(bf.factor_elems.size()) ? bf.factor_elems[L] : 0;
@@ -147,8 +150,17 @@ static void validate_tables(const BF &bf) {
                 }
                 if (!ok) bad.store(1);
             }, 4096);
-            BFG_REQUIRE(!bad.load(), "pack: launch table " + std::to_string(L) + (fwd ? " (forward)" : " (transposed)") +
-                                         " indexes outside its buffers");
+            BFG_REQUIRE(!bad.load(), "pack: launch table " + std::to_string(L) + (fwd ? " (forward" : " (transposed") +
+                                         (dir >= 2 ? ", partial sums)" : ")") + " indexes outside its buffers");
+            if (S.in_kind == IN_GATHER) {
+                // the permuted copy of the inputs covers [in_lo, in_hi)
+                const std::vector<int32_t> &perm = fwd ? bf.trial->perm : bf.test->perm;
+                const int64_t lim = fwd ? bf.cols : bf.rows;
+                BFG_REQUIRE(S.in_lo >= 0 && S.in_lo <= S.in_hi && S.in_hi <= int64_t(perm.size()),
+                            "pack: gather range outside the tree");
+                for (int32_t v = S.in_lo; v < S.in_hi; ++v)
+                    BFG_REQUIRE(perm[v] >= 0 && perm[v] < lim, "pack: gather range reads outside x");
+            }
         }
     (void)Lmax;
 }
@@ -218,7 +230,7 @@ void pack_tables(BF &bf) {
     for (int64_t b = 0; b < nb; ++b) slot_base[b + 1] += slot_base[b];
     const int64_t counter = slot_base[nb];
     BFG_REQUIRE(counter < (int64_t(1) << 31), "pack: skeleton vectors exceed 2^31 entries");
-    bf.slot_len = std::max<int64_t>(counter, 1);
+    bf.part_ofs = counter;
 
     // ---- size (bfdims.jl:54-64) and coverage ----
     if (nb == 1) {
@@ -373,18 +385,110 @@ void pack_tables(BF &bf) {
         S.h_items.swap(items);
         S.h_contribs.swap(con);
     };
-    auto finish = [&](StageTable &S, bool fwd) {
+    // Split items whose contribution lists are long (observer-leaf sums over
+    // many factorizations) into chunks of about CHUNK factor entries: a
+    // `pre` table writes one partial sum per chunk into slots after the
+    // skeleton vectors, the item then adds its partials (identity
+    // contributions).  Returns the partial-sum entries used.
+    auto split = [&](StageTable &S) -> int64_t {
+        constexpr int64_t CHUNK = 4096;
+        auto elems = [](const MvContrib &c) { return c.fofs < 0 ? int64_t(c.m) : int64_t(c.m) * c.n; };
+        std::vector<MvItem> mi, pi;
+        std::vector<MvContrib> mc, pc;
+        int64_t cur = 0;
+        for (const MvItem &it : S.h_items) {
+            int64_t total = 0;
+            for (int32_t c = it.c_begin; c < it.c_end; ++c) total += elems(S.h_contribs[c]);
+            MvItem m = it;
+            m.c_begin = int32_t(mc.size());
+            if (it.c_end - it.c_begin <= 2 || total <= 2 * CHUNK) {
+                mc.insert(mc.end(), S.h_contribs.begin() + it.c_begin, S.h_contribs.begin() + it.c_end);
+            } else {
+                int32_t start = it.c_begin;
+                int64_t acc = 0;
+                for (int32_t c = it.c_begin; c < it.c_end; ++c) {
+                    acc += elems(S.h_contribs[c]);
+                    if (acc < CHUNK && c + 1 < it.c_end) continue;
+                    const int64_t po = bf.part_ofs + cur;
+                    BFG_REQUIRE(po + it.out_len < (int64_t(1) << 31), "pack: partial sums exceed 2^31 entries");
+                    pi.push_back(MvItem{int32_t(po), it.out_len, int32_t(pc.size()), 0});
+                    pc.insert(pc.end(), S.h_contribs.begin() + start, S.h_contribs.begin() + c + 1);
+                    pi.back().c_end = int32_t(pc.size());
+                    mc.push_back(MvContrib{-1, it.out_len, it.out_len, int32_t(po), 0});
+                    cur += it.out_len;
+                    start = c + 1;
+                    acc = 0;
+                }
+            }
+            m.c_end = int32_t(mc.size());
+            mi.push_back(m);
+        }
+        if (pi.empty()) return 0;
+        S.pre = std::make_unique<StageTable>();
+        S.pre->in_kind = S.in_kind;
+        S.pre->out_kind = OUT_SLOT;
+        S.pre->h_items.swap(pi);
+        S.pre->h_contribs.swap(pc);
+        S.h_items.swap(mi);
+        S.h_contribs.swap(mc);
+        return cur;
+    };
+    // p-quantile of a histogram over power-of-two buckets: the smallest
+    // 2^b with at least q of the counted values <= 2^b
+    auto pow2_quantile = [](const std::vector<int64_t> &h, double q) {
+        int64_t tot = 0;
+        for (int64_t c : h) tot += c;
+        int64_t run = 0;
+        for (size_t b = 0; b < h.size(); ++b) {
+            run += h[b];
+            if (run >= q * double(tot)) return int(b);
+        }
+        return int(h.size()) - 1;
+    };
+    auto log2up = [](int64_t v) {
+        int b = 0;
+        while ((int64_t(1) << b) < v) ++b;
+        return b;
+    };
+    std::function<void(StageTable &, bool)> finish = [&](StageTable &S, bool fwd) {
         int64_t vec = 0;
         S.max_out = S.max_in = 0;
+        S.factor_elems = 0;
+        std::vector<int64_t> hrow(32, 0), hk(32, 0);
+        int64_t nc_sum = 0;
+        S.max_nc = 0;
         for (const MvItem &it : S.h_items) {
             S.max_out = std::max(S.max_out, it.out_len);
             vec += int64_t(it.out_len) * ((S.out_kind == OUT_SCATTER) ? 2 : 1);
+            ++hrow[size_t(log2up(std::max(it.out_len, 1)))];
+            S.max_nc = std::max(S.max_nc, it.c_end - it.c_begin);
+            nc_sum += it.c_end - it.c_begin;
         }
+        S.in_lo = INT32_MAX;
+        S.in_hi = 0;
         for (const MvContrib &c : S.h_contribs) {
             const int32_t in_len = fwd ? c.n : c.m;
             S.max_in = std::max(S.max_in, in_len);
             vec += in_len;
+            if (c.fofs >= 0) {
+                S.factor_elems += int64_t(c.m) * c.n;
+                ++hk[size_t(log2up(std::max(in_len, 1)))];
+            }
+            S.in_lo = std::min(S.in_lo, c.in_ofs);
+            S.in_hi = std::max(S.in_hi, c.in_ofs + in_len);
         }
+        if (S.in_hi <= S.in_lo) S.in_lo = S.in_hi = 0;
+        // tiled-kernel shape: rows cover 90% of the outputs; the lane groups
+        // split the contributions when items have several, else the inner
+        // index (QW = 8 loads per lane and pass)
+        const int rb = std::min(6, pow2_quantile(hrow, 0.9));
+        const double avg_nc = S.h_items.empty() ? 1.0 : double(nc_sum) / double(S.h_items.size());
+        S.tile.rp_shift = int8_t(rb);
+        S.tile.csplit = avg_nc >= 1.5 ? 1 : 0;
+        int gb = S.tile.csplit ? log2up(int64_t(std::min(avg_nc, 8.0) + 0.5))
+                               : std::max(0, pow2_quantile(hk, 0.9) - 3);
+        S.tile.g_shift = int8_t(std::max(0, std::min(gb, 6 - rb)));
+        S.tile.wg_per_cu = 8;
         S.n_items = int32_t(S.h_items.size());
         S.n_contrib = int32_t(S.h_contribs.size());
         S.alg_bytes_1 = vec * es;
@@ -397,17 +501,18 @@ void pack_tables(BF &bf) {
         }, 4096);
         // default forward variant until the autotuner measures: a
         // wavefront per task for small blocks, a workgroup otherwise
-        S.variant = (S.n_items > 0 && S.factor_elems / S.n_items < 4096) ? 1 : 0;
+        S.variant = fwd ? ((S.n_items > 0 && S.factor_elems / S.n_items < 4096) ? 1 : 0) : 0;
         upload_tab(S.items, S.h_items);
         upload_tab(S.contribs, S.h_contribs);
         upload_tab(S.tasks, S.h_tasks);
+        if (S.pre) finish(*S.pre, fwd);
     };
 
     bf.fwd.clear();
     bf.tr.clear();
     bf.fwd.resize(nL);
     bf.tr.resize(nL);
-    int64_t covered_rows = 0, covered_cols = 0;
+    int64_t covered_rows = 0, covered_cols = 0, part_used = 0;
     for (int32_t L = 0; L < nL; ++L) {
         StageTable &F = bf.fwd[L], &T = bf.tr[L];
         F.in_kind = (L == 0) ? IN_GATHER : IN_SLOT;
@@ -416,6 +521,8 @@ void pack_tables(BF &bf) {
         T.out_kind = (L == 0) ? OUT_SCATTER : OUT_SLOT;
         group(L, true, F);
         group(L, false, T);
+        if (L == Lmax) part_used = std::max(part_used, split(F));
+        if (L == 0) part_used = std::max(part_used, split(T));
         finish(F, true);
         finish(T, false);
         if (L == Lmax)
@@ -425,6 +532,9 @@ void pack_tables(BF &bf) {
     }
     bf.y_covered = covered_rows == bf.rows;
     bf.x_covered = covered_cols == bf.cols;
+    bf.gather_ofs = bf.part_ofs + part_used;
+    bf.slot_len = bf.gather_ofs + std::max(test.n_values, trial.n_values);
+    BFG_REQUIRE(bf.slot_len < (int64_t(1) << 31), "pack: slot buffer exceeds 2^31 entries");
     validate_tables(bf);
     upload_tab(bf.perm_test, test.perm);
     upload_tab(bf.perm_trial, trial.perm);
@@ -447,6 +557,8 @@ void build_sweep(BF &bf) {
     sw.built = true;
     const int32_t nL = bf.n_stage_launch;
     if (nL > SWEEP_MAX_STAGES) return;
+    for (const StageTable &S : bf.fwd)
+        if (S.pre) return;  // partial sums need their own launch
     std::vector<int32_t> starts;  // slot offsets of every produced vector
     for (int32_t L = 0; L + 1 < nL; ++L)
         for (const MvItem &it : bf.fwd[L].h_items) starts.push_back(it.out_ofs);
@@ -541,6 +653,32 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
                            int64_t ldy, const double *alpha, const double *beta,
                            std::vector<hipEvent_t> *ev);
 
+// One launch table of level L: the tiled kernel (after permuting an
+// IN_GATHER input into the slot buffer) or the per-task kernels.
+static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nrhs, const void *x,
+                      int64_t ldx, void *y, int64_t ldy, const double *alpha, const double *beta,
+                      bool fuse) {
+    hipStream_t s = bf.ctx->stream;
+    const bool fwd = op == BFGPU_OP_N;
+    const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
+    const int32_t *pin = t.in_kind == IN_GATHER ? (fwd ? pr : pt) : nullptr;
+    const int32_t *pout = t.out_kind == OUT_SCATTER ? (fwd ? pt : pr) : nullptr;
+    if (t.variant == 2) {
+        int64_t shift = 0;
+        if (t.in_kind == IN_GATHER) {
+            const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+            launch_gather(s, bf.precision, x, ldx, pin, t.in_lo, t.in_hi,
+                          static_cast<char *>(bf.slots.p) + size_t(bf.gather_ofs) * es, bf.slot_len, nrhs);
+            shift = bf.gather_ofs;
+        }
+        launch_tile(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
+                    nrhs, alpha, beta, fuse, bf.ctx->n_cu);
+    } else {
+        launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pin,
+                     pout, nrhs, alpha, beta, fuse);
+    }
+}
+
 static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                       int64_t ldy, const double *alpha, const double *beta,
                       std::vector<hipEvent_t> *ev) {
@@ -609,13 +747,13 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
         return;
     }
+    (void)pt;
+    (void)pr;
     for (int32_t i = 0; i < nL; ++i) {
         int32_t L = fwd ? i : nL - 1 - i;
         const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
-        const int32_t *pin = fwd ? (L == 0 ? pr : nullptr) : (L == nL - 1 ? pt : nullptr);
-        const int32_t *pout = fwd ? (L == nL - 1 ? pt : nullptr) : (L == 0 ? pr : nullptr);
-        launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, bf.slots.p,
-                     bf.slot_len, pin, pout, nrhs, alpha, b_eff, fuse);
+        if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse);
+        run_table(bf, op, t, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse);
         if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
     }
 }
@@ -693,6 +831,7 @@ void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void
             int32_t L = fwd ? i : nL - 1 - i;
             const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
             stage_bytes[i] = t.factor_elems * es + t.alg_bytes_1 * nrhs;
+            if (t.pre) stage_bytes[i] += t.pre->factor_elems * es + t.pre->alg_bytes_1 * nrhs;
         }
     }
     if (n_stages) *n_stages = nS;
@@ -716,16 +855,19 @@ void check_sweep_error(BF &bf) {
 
 namespace bfg {
 
-// Launch-shape autotuner: for every forward stage, time each kernel
-// variant on device buffers and keep the fastest (per-stage choice is
-// stored in the stage table and used by all later matvecs).
+// Launch-shape autotuner: for every launch table of both directions
+// (forward, and the adjoint whose tables the transpose shares), time each
+// kernel variant and tiled shape on device buffers and keep the fastest;
+// the choice is stored in the table and used by all later matvecs.
+// stage_ms / variants report the forward levels.
 void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *variants) {
     BFG_REQUIRE(nrhs >= 1 && iters >= 1, "autotune: nrhs, iters >= 1");
     hipStream_t s = bf.ctx->stream;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+    const int64_t nv = std::max(bf.rows, bf.cols);
     DevBuf dx, dy;
-    dx.alloc(size_t(bf.cols) * nrhs * es);
-    dy.alloc(size_t(bf.rows) * nrhs * es);
+    dx.alloc(size_t(nv) * nrhs * es);
+    dy.alloc(size_t(nv) * nrhs * es);
     BFG_HIP(hipMemsetAsync(dx.p, 0, dx.bytes, s));
     BFG_HIP(hipMemsetAsync(dy.p, 0, dy.bytes, s));
     size_t need = size_t(bf.slot_len) * size_t(nrhs) * es;
@@ -737,34 +879,62 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
     BFG_HIP(hipMemsetAsync(bf.slots.p, 0, need, s));
     release_graph(bf);
     const int32_t nL = bf.n_stage_launch;
-    const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
     static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
     hipEvent_t e0, e1;
     BFG_HIP(hipEventCreate(&e0));
     BFG_HIP(hipEventCreate(&e1));
+    auto time_table = [&](int op, StageTable &t, int32_t L) {
+        run_table(bf, op, t, L, nrhs, dx.p, nv, dy.p, nv, one, zero, true);
+        BFG_HIP(hipEventRecord(e0, s));
+        for (int32_t k = 0; k < iters; ++k) run_table(bf, op, t, L, nrhs, dx.p, nv, dy.p, nv, one, zero, true);
+        BFG_HIP(hipEventRecord(e1, s));
+        BFG_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        BFG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return double(ms) / iters;
+    };
+    auto tune = [&](int op, StageTable &t, int32_t L) {
+        if (t.n_items == 0) return 0.0;
+        const int32_t v0 = t.variant;
+        const TileShape t0 = t.tile;
+        double best = 1e30;
+        int32_t bv = v0;
+        TileShape bs = t0;
+        auto consider = [&](int32_t v, TileShape sh) {
+            t.variant = v;
+            t.tile = sh;
+            const double ms = time_table(op, t, L);
+            if (ms < best) { best = ms; bv = v; bs = sh; }
+        };
+        consider(0, t0);
+        if (op == BFGPU_OP_N) consider(1, t0);
+        for (int rp = std::max(0, t0.rp_shift - 1); rp <= std::min(6, t0.rp_shift + 1); ++rp)
+            for (int gs = 0; gs <= std::min(3, 6 - rp); ++gs)
+                for (int cs = 0; cs <= (t.max_nc >= 2 ? 1 : 0); ++cs) {
+                    if (cs == 1 && gs == 0) continue;  // same as cs == 0 with one group
+                    TileShape sh = t0;
+                    sh.rp_shift = int8_t(rp);
+                    sh.g_shift = int8_t(gs);
+                    sh.csplit = int8_t(cs);
+                    consider(2, sh);
+                }
+        t.variant = bv;
+        t.tile = bs;
+        return best;
+    };
     try {
-        for (int32_t L = 0; L < nL; ++L) {
-            StageTable &t = bf.fwd[L];
-            const int32_t *pin = L == 0 ? pr : nullptr, *pout = L == nL - 1 ? pt : nullptr;
-            double best = 1e30;
-            int32_t bestv = t.variant;
-            for (int32_t v = 0; v <= 1; ++v) {
-                t.variant = v;
-                launch_stage(s, bf.precision, BFGPU_OP_N, t, bf.factors[L].p, dx.p, bf.cols, dy.p, bf.rows,
-                             bf.slots.p, bf.slot_len, pin, pout, nrhs, one, zero, true);
-                BFG_HIP(hipEventRecord(e0, s));
-                for (int32_t k = 0; k < iters; ++k)
-                    launch_stage(s, bf.precision, BFGPU_OP_N, t, bf.factors[L].p, dx.p, bf.cols, dy.p,
-                                 bf.rows, bf.slots.p, bf.slot_len, pin, pout, nrhs, one, zero, true);
-                BFG_HIP(hipEventRecord(e1, s));
-                BFG_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                BFG_HIP(hipEventElapsedTime(&ms, e0, e1));
-                if (ms / iters < best) { best = ms / iters; bestv = v; }
+        for (int dir = 0; dir < 2; ++dir) {
+            const int op = dir == 0 ? BFGPU_OP_N : BFGPU_OP_C;
+            for (int32_t L = 0; L < nL; ++L) {
+                StageTable &t = dir == 0 ? bf.fwd[L] : bf.tr[L];
+                double ms = 0.0;
+                if (t.pre) ms += tune(op, *t.pre, L);
+                ms += tune(op, t, L);
+                if (dir == 0) {
+                    if (stage_ms) stage_ms[L] = ms;
+                    if (variants) variants[L] = t.variant;
+                }
             }
-            t.variant = bestv;
-            if (stage_ms) stage_ms[L] = best;
-            if (variants) variants[L] = bestv;
         }
     } catch (...) {
         (void)hipEventDestroy(e0);
@@ -773,6 +943,7 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    BFG_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace bfg

@@ -169,6 +169,12 @@ int bfgpu_bf_export_features(const bfgpu_bf *bf, int32_t index, float *feat);
 /* Per-stage statistics of the packed tables: blocks, identity blocks, max
  * rank, factor bytes (show, src/showfcts.jl:81-124). 4 int64 per stage. */
 int bfgpu_bf_stage_stats(const bfgpu_bf *bf, int32_t *n_stages, int64_t *stats);
+/* Launch tables of one direction (op N, or T/C which share theirs), per
+ * level in launch order: items, contributions, most contributions of an
+ * item, factor entries, algorithmic vector bytes per RHS, kernel variant,
+ * tiled shape (packed as for BFGPU_OPT_TILE), partial-sum items (0 when the
+ * level has no split items).  8 int64 per level. */
+int bfgpu_bf_launch_stats(const bfgpu_bf *bf, int32_t op, int32_t *n_levels, int64_t *stats);
 
 /* Convert the factor storage to another precision (e.g. FP64 -> FP32). */
 int bfgpu_bf_convert(bfgpu_bf *bf, int32_t precision, bfgpu_bf **out);
@@ -192,23 +198,40 @@ int bfgpu_matvec_timed(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, in
                        void *y, int64_t ldy, int32_t iters, double *stage_ms,
                        int64_t *stage_bytes, int32_t *n_stages, double *total_ms);
 
-/* Forward matvecs with 1..8 right-hand sides run as one fused, dependency-
- * driven launch over all levels (default on); 0 selects one launch per
- * level.  Both give the same results to rounding. */
+/* enable = 1: forward matvecs with 1..8 right-hand sides run as one fused,
+ * dependency-driven launch over all levels; 0 (default) selects one launch
+ * per level.  Both give the same results to rounding. */
 int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
 /* Tuning options (used by the launch autotuner):
  *   BFGPU_OPT_SWEEP      0: one launch per level, 1: fused sweep with dynamic
  *                        task tickets, 2: fused sweep, cooperative launch
  *   BFGPU_OPT_WG_PER_CU  workgroups per CU of the fused sweep (0 = occupancy)
  *   BFGPU_OPT_GRAPH      1: replay per-level launches from a captured HIP
- *                        graph when x, y, alpha, beta repeat */
-enum { BFGPU_OPT_SWEEP = 1, BFGPU_OPT_WG_PER_CU = 2, BFGPU_OPT_GRAPH = 3 };
+ *                        graph when x, y, alpha, beta repeat
+ *   BFGPU_OPT_VARIANT    kernel of every launch table: 0 workgroup per task,
+ *                        1 wavefront per task (forward; transposed tables
+ *                        take 0), 2 tiled (several tasks per wavefront);
+ *                        get: the common value, or -1 when levels differ
+ *   BFGPU_OPT_TILE       tiled shape of every table, packed as rp_shift |
+ *                        g_shift << 4 | csplit << 8 | wg_per_cu << 12
+ *                        (2^rp_shift outputs x 2^g_shift lane groups per
+ *                        task, rp_shift + g_shift <= 6; groups split the
+ *                        inner index (csplit 0) or the contributions (1);
+ *                        wg_per_cu 1..64); also selects variant 2 */
+enum {
+    BFGPU_OPT_SWEEP = 1,
+    BFGPU_OPT_WG_PER_CU = 2,
+    BFGPU_OPT_GRAPH = 3,
+    BFGPU_OPT_VARIANT = 4,
+    BFGPU_OPT_TILE = 5
+};
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);
-/* Autotune the forward launch shape of every level for nrhs right-hand
- * sides: times each kernel variant (workgroup or wavefront per block) over
- * `iters` launches and keeps the fastest.  stage_ms / variants (length =
- * number of launch stages, may be NULL) receive the choice. */
+/* Autotune the launch shape of every level for nrhs right-hand sides, in
+ * both directions (forward, and adjoint/transpose): times each kernel
+ * variant and tiled shape over `iters` launches and keeps the fastest.
+ * stage_ms / variants (length = number of launch stages, may be NULL)
+ * receive the forward choice. */
 int bfgpu_bf_autotune(bfgpu_bf *bf, int64_t nrhs, int32_t iters, double *stage_ms,
                       int32_t *variants);
 /* Synchronise and report a fault flagged by the fused sweep since the last

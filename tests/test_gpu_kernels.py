@@ -1,0 +1,132 @@
+"""Kernel variants of the matvec levels against the oracle.
+
+Every launch table can run on the workgroup-per-task kernel (variant 0), the
+wavefront-per-task kernel (variant 1, forward), or the tiled kernel
+(variant 2) in any legal shape; observer-leaf sums over many
+factorizations are split into partial sums plus a combine pass.  Bar
+(written in each test): every variant and shape gives the oracle's FP64
+matvec of the same factorization to 1e-12 relative, in all three
+directions and for 1, 2, 3 and 8 right-hand sides.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import crandn, fibonacci_sphere, two_squares
+
+pytestmark = pytest.mark.gpu
+
+K = 2 * np.pi
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def ctx(bfgpu):
+    return bfgpu.default_context()
+
+
+def _shapes(bfgpu):
+    """A spread of legal tiled shapes (rows x groups, both group modes)."""
+    out = []
+    for rp in range(0, 7):
+        for gs in sorted({0, min(2, 6 - rp), 6 - rp}):
+            for cs in (0, 1):
+                if cs and gs == 0:
+                    continue
+                out.append(bfgpu.ButterflyFactorization.tile_shape(rp, gs, cs, 8))
+    out.append(bfgpu.ButterflyFactorization.tile_shape(3, 1, 0, 1))  # one workgroup per CU
+    return out
+
+
+def _check_all_directions(g, refs, X):
+    """refs: op name -> the oracle's product of X (columns independent)."""
+    for name, want in refs.items():
+        op = {"N": g, "C": g.adjoint(), "T": g.transpose()}[name]
+        for nr in (1, 2, 3, 8):
+            Xs = np.asfortranarray(X[:, :nr])
+            got = op @ Xs
+            if nr == 1 and got.ndim == 1:
+                got = got[:, None]
+            assert rel(got, want[:, :nr]) < 1e-12, (name, nr)
+
+
+def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
+    say("two squares N=4096 fp64: kernel variants and tiled shapes")
+    n = 4096
+    obs, src = two_squares(n, seed=2)
+    To = bfgpu.ClusterTree(obs, dims=2, leaf_size=64)
+    Ts = bfgpu.ClusterTree(src, dims=2, leaf_size=64)
+    bt = bfgpu.BlockTree(To, Ts)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+    imp = oracle.import_bf(To, Ts, g.export())
+    adj, tra = imp.transform(1), imp.transform(0)
+
+    def cols(f):
+        return lambda X: np.stack([f(X[:, j]) for j in range(X.shape[1])], axis=1)
+
+    X = crandn(np.random.default_rng(7), n, 8)
+    refs = {"N": cols(imp.matvec)(X), "C": cols(adj.matvec)(X), "T": cols(tra.matvec)(X)}
+    for v in (0, 1, 2):
+        say("variant", v)
+        g.set_option("variant", v)
+        assert g.get_option("variant") == v
+        _check_all_directions(g, refs, X)
+    for sh in _shapes(bfgpu):
+        say("tile", hex(sh))
+        g.set_option("tile", sh)
+        assert g.get_option("tile") == sh and g.get_option("variant") == 2
+        _check_all_directions(g, refs, X)
+    # the autotuner's choice (both directions) keeps the product
+    ms, var = g.autotune(nrhs=1, iters=2)
+    assert len(ms) == len(var) == len(g)
+    _check_all_directions(g, refs, X)
+    with pytest.raises(bfgpu.ArgumentError):
+        g.set_option("tile", bfgpu.ButterflyFactorization.tile_shape(4, 3))
+    with pytest.raises(bfgpu.ArgumentError):
+        g.set_option("variant", 3)
+
+
+def test_variants_sphere_partial_sums_match_oracle(bfgpu, oracle, ctx):
+    say("far field sphere N=6000 fp64: partial sums and kernel variants")
+    n = 6000
+    pts = fibonacci_sphere(n, 4.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    bt = bfgpu.BlockTree(T, T)
+    op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=1e-4,
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), ctx=ctx)
+    fw = op.launch_stats()
+    bw = op.adjoint().launch_stats()
+    say("far pairs", len(op.farinteractions), "partial items fwd", fw[-1, 7], "adj", bw[-1, 7])
+    # observer-leaf sums of the last forward level and source-leaf sums of
+    # the last adjoint level are split into partial sums at this size
+    assert fw[-1, 7] > 0 and bw[-1, 7] > 0
+    say("importing", len(op.farinteractions), "factorizations into the oracle")
+    imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
+    adj = [b.transform(1) for b in imps]
+    tra = [b.transform(0) for b in imps]
+
+    def cols(bfs):
+        return lambda X: np.stack([oracle.matvec_many(bfs, X[:, j], nthreads=8, rows=n)
+                                   for j in range(X.shape[1])], axis=1)
+
+    X = crandn(np.random.default_rng(8), n, 8)
+    refs = {"N": cols(imps)(X), "C": cols(adj)(X), "T": cols(tra)(X)}
+    for v in (0, 1, 2):
+        say("variant", v)
+        op.set_option("variant", v)
+        _check_all_directions(op, refs, X)
+    for sh in _shapes(bfgpu)[::3]:
+        say("tile", hex(sh))
+        op.set_option("tile", sh)
+        _check_all_directions(op, refs, X)
+    op.autotune(nrhs=1, iters=2)
+    _check_all_directions(op, refs, X)
