@@ -301,6 +301,10 @@ class _Handle:
 
     def __init__(self, h, ctx, keep):
         self.h, self.ctx, self.keep = h, ctx, keep
+        i = L.BFInfo()
+        check(lib().bfgpu_bf_info_get(h, C.byref(i)))
+        # fixed after construction: read once
+        self.rows, self.cols, self.precision = int(i.rows), int(i.cols), int(i.precision)
 
     def __del__(self, _destroy=_destroy):
         _destroy("bfgpu_bf_destroy", getattr(self, "h", None))
@@ -332,12 +336,12 @@ class ButterflyFactorization:
 
     @property
     def eltype(self):
-        return np.complex128 if self.info().precision == L.BFGPU_C128 else np.complex64
+        return np.complex128 if self._H.precision == L.BFGPU_C128 else np.complex64
 
     @property
     def shape(self):
-        i = self.info()
-        return (i.rows, i.cols) if self.op == L.OP_N else (i.cols, i.rows)
+        H = self._H
+        return (H.rows, H.cols) if self.op == L.OP_N else (H.cols, H.rows)
 
     def size(self, dim=None):
         s = self.shape
@@ -350,7 +354,7 @@ class ButterflyFactorization:
     def mul(self, y, x, alpha=1.0, beta=0.0):
         """y = alpha * op(A) x + beta * y  (mul!, bfvector.jl:200-296)."""
         rows, cols = self.shape
-        prec = self.info().precision
+        prec = self._H.precision
         want = np.complex128 if prec == L.BFGPU_C128 else np.complex64
         a = (C.c_double * 2)(float(np.real(alpha)), float(np.imag(alpha)))
         b = (C.c_double * 2)(float(np.real(beta)), float(np.imag(beta)))
@@ -487,19 +491,21 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_set_fused(self.handle, 1 if enable else 0))
         return self
 
-    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5}
+    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6}
 
     @staticmethod
-    def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8):
-        """Packed value of the "tile" option."""
-        return int(rp_shift) | int(g_shift) << 4 | int(csplit) << 8 | int(wg_per_cu) << 12
+    def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0):
+        """Packed value of the "tile" option (and of "dense" with g_shift 0,
+        rp_shift = log2 of the lanes per task)."""
+        return (int(rp_shift) | int(g_shift) << 4 | int(csplit) << 8 | int(wg_per_cu) << 12
+                | int(xcd) << 20 | int(qw4) << 24)
 
     def set_option(self, key, value):
         """Launch tuning: sweep (0 per-level, 1 fused dynamic, 2 fused
         cooperative), wg_per_cu (fused sweep occupancy), graph (replay
         per-level launches from a HIP graph), variant (0 workgroup per
-        task, 1 wavefront per task, 2 tiled), tile (tiled shape, see
-        tile_shape)."""
+        task, 1 wavefront per task, 2 tiled, 3 dense), tile (tiled shape,
+        see tile_shape), dense (dense shape of the forward tables)."""
         check(lib().bfgpu_bf_set_option(self.handle, self.OPTIONS[key], int(value)))
         return self
 
@@ -525,10 +531,11 @@ class ButterflyFactorization:
         """Per level in launch order of this operator's direction: items,
         contributions, most contributions of an item, factor entries,
         vector bytes per RHS, kernel variant, packed tiled shape,
-        partial-sum items."""
+        partial-sum items, distinct input entries, output entries (y
+        counted twice: read and written)."""
         n = C.c_int32()
         check(lib().bfgpu_bf_launch_stats(self.handle, self.op, C.byref(n), None))
-        st = np.zeros((n.value, 8), np.int64)
+        st = np.zeros((n.value, 10), np.int64)
         check(lib().bfgpu_bf_launch_stats(self.handle, self.op, C.byref(n), ptr(st, C.c_int64)))
         return st
 
@@ -563,6 +570,37 @@ class ButterflyFactorization:
                                        C.c_void_p(y.data_ptr()), ldy, int(iters), ms, by,
                                        C.byref(n), C.byref(tot)))
         return list(ms)[: n.value], list(by)[: n.value], tot.value
+
+    def launch_times(self, x, y, iters=10, nrhs=1):
+        """Mean milliseconds of every kernel launch of one device matvec (HIP
+        events on the context stream) and a description per launch:
+        (level, table, variant, output kind, contribution split, RHS per
+        lane); table 0 level, 1 partial sums, 2 input permutation, 3 sweep."""
+        rows, cols = self.shape
+        ldx = cols if self.op == L.OP_N else rows
+        ldy = rows if self.op == L.OP_N else cols
+        args = (self.handle, self.op, nrhs, C.c_void_p(x.data_ptr()), ldx, C.c_void_p(y.data_ptr()), ldy,
+                int(iters))
+        n = C.c_int32()
+        check(lib().bfgpu_matvec_launch_times(*args, 0, C.byref(n), None, None))
+        ms = np.zeros(n.value)
+        desc = np.zeros((n.value, 6), np.int32)
+        check(lib().bfgpu_matvec_launch_times(*args, n.value, C.byref(n), ptr(ms, C.c_double),
+                                              ptr(desc, C.c_int32)))
+        return ms, desc
+
+    def tuning(self):
+        """Launch tuning of every table (to save and restore)."""
+        n = C.c_int32()
+        check(lib().bfgpu_bf_get_tuning(self.handle, C.byref(n), None))
+        v = np.zeros(n.value, np.int32)
+        check(lib().bfgpu_bf_get_tuning(self.handle, C.byref(n), ptr(v, C.c_int32)))
+        return v
+
+    def set_tuning(self, vals):
+        v = np.ascontiguousarray(vals, np.int32)
+        check(lib().bfgpu_bf_set_tuning(self.handle, v.size, ptr(v, C.c_int32)))
+        return self
 
     @classmethod
     def from_blocks(cls, blktree, n_stages, table, data_ofs, data, acctype=np.complex128,

@@ -37,6 +37,8 @@ void matvec(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, i
 void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                   int64_t ldy, int32_t iters, double *stage_ms, int64_t *stage_bytes,
                   int32_t *n_stages, double *total_ms);
+void matvec_launch_times(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                         int32_t iters, int32_t cap, int32_t *n_launch, double *launch_ms, int32_t *desc);
 void check_sweep_error(BF &bf);
 void release_graph(BF &bf);
 void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *variants);
@@ -203,12 +205,13 @@ extern "C" int bfgpu_bf_from_host(bfgpu_ctx *ctx, const bfgpu_tree *test, const 
         for (int32_t s = 0; s < n_stages; ++s) {
             int64_t ne = int64_t(hd[s].size() / 2);
             bf.factor_elems[s] = ne;
+            const size_t es = precision == BFGPU_C128 ? 16 : 8;
+            bf.factors[s].alloc(size_t(ne + FACTOR_PAD) * es);
+            BFG_HIP(hipMemset(static_cast<char *>(bf.factors[s].p) + size_t(ne) * es, 0, size_t(FACTOR_PAD) * es));
             if (precision == BFGPU_C128) {
-                bf.factors[s].alloc(size_t(std::max<int64_t>(ne, 1)) * 16);
                 if (ne) BFG_HIP(hipMemcpy(bf.factors[s].p, hd[s].data(), size_t(ne) * 16, hipMemcpyHostToDevice));
             } else {
                 hf.assign(hd[s].begin(), hd[s].end());
-                bf.factors[s].alloc(size_t(std::max<int64_t>(ne, 1)) * 8);
                 if (ne) BFG_HIP(hipMemcpy(bf.factors[s].p, hf.data(), size_t(ne) * 8, hipMemcpyHostToDevice));
             }
         }
@@ -232,12 +235,8 @@ extern "C" int bfgpu_bf_info_get(const bfgpu_bf *h, bfgpu_bf_info *o) {
     o->rows = bf.rows;
     o->cols = bf.cols;
     o->max_stages = bf.n_stage_launch;
-    for (const OneBF &ob : bf.bfs)
-        for (const auto &st : ob.st) {
-            o->n_blocks += int64_t(st.b.size());
-            for (const Block &b : st.b)
-                if (!b.ident) o->n_data += int64_t(b.m) * b.n;
-        }
+    o->n_blocks = bf.n_blocks;
+    o->n_data = bf.n_data;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
     for (int64_t e : bf.factor_elems) o->factor_bytes += e * es;
     o->slot_len = bf.slot_len;
@@ -279,9 +278,13 @@ extern "C" int bfgpu_bf_export(const bfgpu_bf *h, int32_t index, int32_t *table,
     const OneBF &ob = bf.bfs[index];
     const int32_t nst = int32_t(ob.st.size()), Lmax = bf.n_stage_launch - 1;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
-    // factors never change after construction: download once, keep on host
+    // factors never change after construction: download once and keep a
+    // host copy when it is small; large factorizations copy block by block
+    int64_t total_bytes = 0;
+    for (int64_t e : bf.factor_elems) total_bytes += e * es;
+    const bool cached = total_bytes <= (int64_t(2) << 30);
     std::vector<std::vector<double>> &host = bf.export_cache;
-    if (data && host.empty()) {
+    if (data && cached && host.empty()) {
         host.resize(bf.n_stage_launch);
         BFG_HIP(hipStreamSynchronize(bf.ctx->stream));
         for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
@@ -297,6 +300,7 @@ extern "C" int bfgpu_bf_export(const bfgpu_bf *h, int32_t index, int32_t *table,
             }
         }
     }
+    if (data && !cached) BFG_HIP(hipStreamSynchronize(bf.ctx->stream));
     int64_t q = 0, d = 0;
     for (int32_t s = 0; s < nst; ++s) {
         int32_t L = (s == nst - 1) ? Lmax : s;
@@ -309,9 +313,19 @@ extern "C" int bfgpu_bf_export(const bfgpu_bf *h, int32_t index, int32_t *table,
             } else {
                 data_ofs[q] = d;
                 if (data) {
-                    int64_t cnt = int64_t(b.m) * b.n;
-                    BFG_REQUIRE(b.fofs + cnt <= int64_t(host[L].size() / 2), "bf_export: block outside its stage buffer");
-                    std::memcpy(data + 2 * d, host[L].data() + 2 * b.fofs, sizeof(double) * 2 * size_t(cnt));
+                    const int64_t cnt = int64_t(b.m) * b.n;
+                    BFG_REQUIRE(b.fofs + cnt <= bf.factor_elems[L], "bf_export: block outside its stage buffer");
+                    if (cached) {
+                        std::memcpy(data + 2 * d, host[L].data() + 2 * b.fofs, sizeof(double) * 2 * size_t(cnt));
+                    } else if (es == 16) {
+                        BFG_HIP(hipMemcpy(data + 2 * d, static_cast<const char *>(bf.factors[L].p) + size_t(b.fofs) * 16,
+                                          size_t(cnt) * 16, hipMemcpyDeviceToHost));
+                    } else {
+                        std::vector<float> tmp(size_t(2 * cnt));
+                        BFG_HIP(hipMemcpy(tmp.data(), static_cast<const char *>(bf.factors[L].p) + size_t(b.fofs) * 8,
+                                          size_t(cnt) * 8, hipMemcpyDeviceToHost));
+                        std::copy(tmp.begin(), tmp.end(), data + 2 * d);
+                    }
                 }
                 d += int64_t(b.m) * b.n;
             }
@@ -384,10 +398,6 @@ extern "C" int bfgpu_bf_stage_stats(const bfgpu_bf *h, int32_t *n_stages, int64_
     BFG_API_END
 }
 
-static int64_t pack_tile(const TileShape &t) {
-    return int64_t(t.rp_shift) | int64_t(t.g_shift) << 4 | int64_t(t.csplit) << 8 | int64_t(t.wg_per_cu) << 12;
-}
-
 extern "C" int bfgpu_bf_launch_stats(const bfgpu_bf *h, int32_t op, int32_t *n_levels, int64_t *stats) {
     BFG_API_BEGIN
     BFG_REQUIRE(h && n_levels, "bf_launch_stats: null argument");
@@ -397,10 +407,31 @@ extern "C" int bfgpu_bf_launch_stats(const bfgpu_bf *h, int32_t op, int32_t *n_l
     *n_levels = nL;
     if (!stats) return BFGPU_OK;
     const bool fwd = op == BFGPU_OP_N;
+    std::vector<uint8_t> mark;
     for (int32_t i = 0; i < nL; ++i) {
         const int32_t L = fwd ? i : nL - 1 - i;
         const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
-        int64_t *o = stats + 8 * i;
+        int64_t *o = stats + 10 * i;
+        // distinct input entries the level reads (partial sums excluded)
+        // and output entries it writes (read and written for y)
+        const int64_t span = t.in_kind == IN_SLOT ? bf.slot_len : std::max(bf.test->n_values, bf.trial->n_values);
+        mark.assign(size_t(span), 0);
+        auto mark_tab = [&](const StageTable &u) {
+            parallel_for(int64_t(u.h_contribs.size()), [&](int64_t c) {
+                const MvContrib &k = u.h_contribs[c];
+                const int32_t len = fwd ? k.n : k.m;
+                if (u.in_kind == IN_SLOT && k.in_ofs >= bf.part_ofs && k.in_ofs < bf.gather_ofs) return;
+                std::fill(mark.begin() + k.in_ofs, mark.begin() + k.in_ofs + len, uint8_t(1));
+            }, 4096);
+        };
+        mark_tab(t);
+        if (t.pre) mark_tab(*t.pre);
+        int64_t uniq = 0;
+        for (uint8_t m : mark) uniq += m;
+        int64_t outs = 0;
+        for (const MvItem &it : t.h_items) outs += it.out_len;
+        o[8] = uniq;
+        o[9] = outs * (t.out_kind == OUT_SCATTER ? 2 : 1);
         o[0] = t.n_items;
         o[1] = t.n_contrib + (t.pre ? t.pre->n_contrib : 0);
         o[2] = std::max(t.max_nc, t.pre ? t.pre->max_nc : 0);
@@ -434,7 +465,9 @@ extern "C" int bfgpu_bf_convert(bfgpu_bf *h, int32_t precision, bfgpu_bf **out) 
         bf.factors.resize(src.factors.size());
         const int es = precision == BFGPU_C128 ? 16 : 8;
         for (size_t L = 0; L < src.factors.size(); ++L) {
-            bf.factors[L].alloc(size_t(std::max<int64_t>(src.factor_elems[L], 1)) * es);
+            bf.factors[L].alloc(size_t(src.factor_elems[L] + FACTOR_PAD) * es);
+            BFG_HIP(hipMemset(static_cast<char *>(bf.factors[L].p) + size_t(src.factor_elems[L]) * es, 0,
+                              size_t(FACTOR_PAD) * es));
             launch_convert(src.ctx->stream, src.factors[L].p, src.precision, bf.factors[L].p, precision,
                            src.factor_elems[L]);
         }
@@ -471,6 +504,68 @@ extern "C" int bfgpu_matvec_timed(bfgpu_bf *h, int32_t op, int64_t nrhs, const v
     BFG_API_END
 }
 
+extern "C" int bfgpu_matvec_launch_times(bfgpu_bf *h, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
+                                         void *y, int64_t ldy, int32_t iters, int32_t cap, int32_t *n_launch,
+                                         double *launch_ms, int32_t *desc) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && x && y && n_launch, "matvec_launch_times: null argument");
+    BFG_REQUIRE(op == BFGPU_OP_N || op == BFGPU_OP_T || op == BFGPU_OP_C, "matvec_launch_times: bad op");
+    BFG_REQUIRE(nrhs >= 1 && cap >= 0, "matvec_launch_times: nrhs >= 1, cap >= 0");
+    const bool fwd = op == BFGPU_OP_N;
+    BFG_REQUIRE(ldx >= (fwd ? h->b.cols : h->b.rows) && ldy >= (fwd ? h->b.rows : h->b.cols),
+                "matvec_launch_times: leading dimension too small");
+    matvec_launch_times(h->b, op, nrhs, x, ldx, y, ldy, iters, cap, n_launch, launch_ms, desc);
+    BFG_API_END
+}
+
+// Tuning state: per level, the forward table, its partial-sum table, the
+// transposed table and its partial-sum table, each (variant, packed shape);
+// (-1, -1) where a level has no partial-sum table.
+extern "C" int bfgpu_bf_get_tuning(const bfgpu_bf *h, int32_t *n, int32_t *vals) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && n, "bf_get_tuning: null argument");
+    const BF &bf = h->b;
+    *n = 8 * bf.n_stage_launch;
+    if (!vals) return BFGPU_OK;
+    for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
+        const StageTable *t[4] = {&bf.fwd[L], bf.fwd[L].pre.get(), &bf.tr[L], bf.tr[L].pre.get()};
+        for (int q = 0; q < 4; ++q) {
+            vals[8 * L + 2 * q] = t[q] ? t[q]->variant : -1;
+            vals[8 * L + 2 * q + 1] = t[q] ? int32_t(pack_tile(t[q]->tile)) : -1;
+        }
+    }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && vals, "bf_set_tuning: null argument");
+    BF &bf = h->b;
+    BFG_REQUIRE(n == 8 * bf.n_stage_launch, "bf_set_tuning: wrong length (levels differ)");
+    // validate everything before changing anything
+    for (int pass = 0; pass < 2; ++pass)
+        for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
+            StageTable *t[4] = {&bf.fwd[L], bf.fwd[L].pre.get(), &bf.tr[L], bf.tr[L].pre.get()};
+            for (int q = 0; q < 4; ++q) {
+                const int32_t v = vals[8 * L + 2 * q], p = vals[8 * L + 2 * q + 1];
+                if (!t[q]) {
+                    BFG_REQUIRE(v == -1 && p == -1, "bf_set_tuning: partial-sum tables differ");
+                    continue;
+                }
+                const TileShape sh = unpack_tile(p);
+                BFG_REQUIRE(v >= 0 && v <= 3 && !(q >= 2 && (v == 1 || v == 3)) && p >= 0 && p < (1 << 28) &&
+                                tile_ok(sh) && !(v == 3 && sh.rp_shift > 6),
+                            "bf_set_tuning: bad variant or shape");
+                if (pass == 1) {
+                    t[q]->variant = v;
+                    t[q]->tile = sh;
+                }
+            }
+        }
+    release_graph(bf);
+    BFG_API_END
+}
+
 extern "C" int bfgpu_bf_set_fused(bfgpu_bf *h, int32_t enable) {
     BFG_API_BEGIN
     BFG_REQUIRE(h, "bf_set_fused: null handle");
@@ -499,24 +594,25 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         release_graph(bf);
         break;
     case BFGPU_OPT_VARIANT:
-        BFG_REQUIRE(value >= 0 && value <= 2, "bf_set_option: kernel variant 0..2");
+        BFG_REQUIRE(value >= 0 && value <= 3, "bf_set_option: kernel variant 0..3");
         for (int dir = 0; dir < 2; ++dir)
             for (StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
-                const int32_t v = (dir == 1 && value == 1) ? 0 : value;
+                const int32_t v = dir == 1 ? (value == 1 ? 0 : value == 3 ? 2 : value) : value;
+                if (value == 3 && dir == 0) {  // dense: as many lanes per task as the tiled shape, >= 16
+                    for (StageTable *u : {&t, t.pre.get()}) {
+                        if (!u) continue;
+                        u->tile.rp_shift = int8_t(std::max<int>(4, u->tile.rp_shift + u->tile.g_shift));
+                        u->tile.g_shift = 0;
+                    }
+                }
                 t.variant = v;
                 if (t.pre) t.pre->variant = v;
             }
         release_graph(bf);
         break;
     case BFGPU_OPT_TILE: {
-        TileShape sh;
-        sh.rp_shift = int8_t(value & 15);
-        sh.g_shift = int8_t((value >> 4) & 15);
-        sh.csplit = int8_t((value >> 8) & 15);
-        sh.wg_per_cu = int16_t((value >> 12) & 255);
-        BFG_REQUIRE(value >= 0 && sh.rp_shift + sh.g_shift <= 6 && sh.csplit <= 1 && sh.wg_per_cu >= 1 &&
-                        sh.wg_per_cu <= 64,
-                    "bf_set_option: bad tile shape");
+        const TileShape sh = unpack_tile(value);
+        BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh), "bf_set_option: bad tile shape");
         for (int dir = 0; dir < 2; ++dir)
             for (StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
                 t.variant = 2;
@@ -526,6 +622,19 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
                     t.pre->tile = sh;
                 }
             }
+        release_graph(bf);
+        break;
+    }
+    case BFGPU_OPT_DENSE: {
+        const TileShape sh = unpack_tile(value);
+        BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
+                    "bf_set_option: bad dense shape");
+        for (StageTable &t : bf.fwd)
+            for (StageTable *u : {&t, t.pre.get()})
+                if (u) {
+                    u->variant = 3;
+                    u->tile = sh;
+                }
         release_graph(bf);
         break;
     }
@@ -547,13 +656,14 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
         int32_t v = bf.fwd.empty() ? 0 : bf.fwd[0].variant;
         for (int dir = 0; dir < 2; ++dir)
             for (const StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
-                const int32_t w = (dir == 1 && v == 1) ? 0 : v;
+                const int32_t w = dir == 1 ? (v == 1 ? 0 : v == 3 ? 2 : v) : v;
                 if (t.variant != w || (t.pre && t.pre->variant != w)) v = -1;
             }
         *value = v;
         break;
     }
-    case BFGPU_OPT_TILE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
+    case BFGPU_OPT_TILE:
+    case BFGPU_OPT_DENSE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }
     BFG_API_END

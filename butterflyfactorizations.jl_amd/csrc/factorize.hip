@@ -327,7 +327,7 @@ struct Engine {
                 for (int32_t i = 0; i < c.rank; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
             });
             if (!it.empty()) {
-                ensure_pool(pool, pool_cap, pool_used);
+                ensure_pool(pool, pool_cap, pool_used + FACTOR_PAD);
                 upload(d_interp, it, ctx.stream);
                 launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), int32_t(it.size()),
                               d_piv.as<int32_t>(), pool.p, max_rank);
@@ -639,7 +639,9 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
             }
         const int64_t zofs = zo[nP];
         t_plan += secs(tp, clk::now());
-        bf.factors[sP].alloc(size_t(std::max<int64_t>(zofs, 1)) * E.es);
+        bf.factors[sP].alloc(size_t(zofs + FACTOR_PAD) * E.es);
+        BFG_HIP(hipMemsetAsync(static_cast<char *>(bf.factors[sP].p) + size_t(zofs) * E.es, 0,
+                               size_t(FACTOR_PAD) * E.es, ctx.stream));
         bf.factor_elems[sP] = zofs;
         upload(E.d_rows, hrows, ctx.stream);
         upload(E.d_cols, hcols, ctx.stream);

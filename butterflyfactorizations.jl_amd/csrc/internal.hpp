@@ -146,7 +146,27 @@ enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
 struct TileShape {
     int8_t rp_shift = 3, g_shift = 0, csplit = 0;
     int16_t wg_per_cu = 8;
+    int8_t xcd = 0;  // 1: each XCD walks one contiguous range of tasks
+    int8_t qw4 = 0;  // dense kernel, one RHS: 4 loads per lane and pass (else 8)
 };
+inline int64_t pack_tile(const TileShape &t) {
+    return int64_t(t.rp_shift) | int64_t(t.g_shift) << 4 | int64_t(t.csplit) << 8 | int64_t(t.wg_per_cu) << 12 |
+           int64_t(t.xcd) << 20 | int64_t(t.qw4) << 24;
+}
+inline TileShape unpack_tile(int64_t v) {
+    TileShape t;
+    t.rp_shift = int8_t(v & 15);
+    t.g_shift = int8_t((v >> 4) & 15);
+    t.csplit = int8_t((v >> 8) & 15);
+    t.wg_per_cu = int16_t((v >> 12) & 255);
+    t.xcd = int8_t((v >> 20) & 15);
+    t.qw4 = int8_t((v >> 24) & 15);
+    return t;
+}
+inline bool tile_ok(const TileShape &t) {
+    return t.rp_shift >= 0 && t.g_shift >= 0 && t.rp_shift + t.g_shift <= 6 && t.csplit <= 1 &&
+           t.wg_per_cu >= 1 && t.wg_per_cu <= 64 && t.xcd <= 1 && t.qw4 <= 1;
+}
 
 struct StageTable {
     int32_t n_items = 0, n_contrib = 0;
@@ -155,7 +175,9 @@ struct StageTable {
     int64_t alg_bytes_1 = 0;           // algorithmic bytes per RHS (factor bytes separate)
     int64_t factor_elems = 0;          // factor entries read
     // kernel: 0 workgroup per task, 1 wavefront per task (forward only),
-    // 2 tiled (several tasks per wavefront, persistent)
+    // 2 tiled (several tasks per wavefront, persistent), 3 dense (forward
+    // only: lanes packed to each block's row count; tile.rp_shift is the
+    // log2 of the lanes per task)
     int32_t variant = 0;
     TileShape tile;
     int32_t in_lo = 0, in_hi = 0;      // IN_GATHER: value range read through the permutation
@@ -208,8 +230,13 @@ struct BF {
     DevBuf xy_host_stage;         // staging for host x / y
     mutable std::vector<std::vector<double>> export_cache;  // host copy of factors (export)
     int64_t n_comp = 0, n_retry = 0;
+    int64_t n_blocks = 0, n_data = 0;  // all blocks, factor entries (set by pack_tables)
     double t_factor_ms = 0.0;
 };
+
+// Entries allocated past the end of every factor buffer: the dense kernel
+// reads two rows with one load, one entry beyond an odd-height block.
+constexpr int64_t FACTOR_PAD = 16;
 
 // Packing of reference-layout blocks into launch tables (matvec.hip).
 void pack_tables(BF &bf);
@@ -285,6 +312,9 @@ void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const vo
 void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
                  int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
+void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
+                  int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
 void launch_gather(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm,
                    int32_t lo, int32_t hi, void *dst, int64_t ld_dst, int64_t nrhs);
 // Fused forward sweep: every stage in one cooperative launch, tasks ordered

@@ -611,7 +611,7 @@ __global__ __launch_bounds__(256) void tile_kernel(
     const MvTask *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
     const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
     C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
-    int fuse_beta, int nrhs, int rp_shift, int g_shift, int conj) {
+    int fuse_beta, int nrhs, int rp_shift, int g_shift, int conj, int xcd_map) {
     const int lane = threadIdx.x & 63;
     const int RP = 1 << rp_shift, G = 1 << g_shift;
     const int lpt_shift = rp_shift + g_shift;
@@ -624,8 +624,22 @@ __global__ __launch_bounds__(256) void tile_kernel(
     const R csign = conj ? R(-1) : R(1);
     const C<R> *in = slots + in_shift + int64_t(rb) * slot_len;
     const MvContrib none{-1, 0, 0, 0, 0};
-    const int64_t n_waves = int64_t(gridDim.x) * 4;
-    int64_t grp = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    // task groups [grp_lo, grp_hi) walked with stride n_waves.  xcd_map:
+    // workgroups are dispatched round-robin over the 8 XCDs, so workgroup b
+    // runs on XCD b % 8; each XCD takes one contiguous eighth of the groups
+    // (neighbouring blocks and the inputs they share stay in one L2)
+    const int64_t n_groups = (int64_t(n_tasks) + tpw - 1) / tpw;
+    int64_t grp, grp_hi, n_waves;
+    if (xcd_map && gridDim.x >= 8) {  // every XCD holds a workgroup
+        const int64_t nwg = gridDim.x, x = blockIdx.x % 8;
+        n_waves = ((nwg - x + 7) / 8) * 4;
+        grp = n_groups * x / 8 + (int64_t(blockIdx.x) / 8) * 4 + (threadIdx.x >> 6);
+        grp_hi = n_groups * (x + 1) / 8;
+    } else {
+        n_waves = int64_t(gridDim.x) * 4;
+        grp = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+        grp_hi = n_groups;
+    }
 
     auto load_task = [&](int64_t gi) -> MvTask {
         const int64_t t = gi * tpw + sub;
@@ -642,9 +656,9 @@ __global__ __launch_bounds__(256) void tile_kernel(
     };
 
     MvTask tk = load_task(grp);
-    while (grp * tpw < n_tasks) {
+    while (grp < grp_hi) {
         const int64_t ngrp = grp + n_waves;
-        const MvTask nt = load_task(ngrp);
+        const MvTask nt = ngrp < grp_hi ? load_task(ngrp) : load_task(n_groups);
         const int nc = tk.out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
         for (int row0 = 0; row0 < tk.out_len; row0 += RP) {
             const int row = row0 + i;
@@ -652,6 +666,9 @@ __global__ __launch_bounds__(256) void tile_kernel(
             C<R> acc[NR];
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] = C<R>{R(0), R(0)};
+            // Regular contribution: loads carry no predicate (row and inner
+            // index clamped into the block); inputs of clamped entries are
+            // zeroed by a select.
             auto apply = [&](const MvContrib &c, int kbeg, int kstep) {
                 if (c.fofs < 0) {  // identity (m == n) or none (m == 0)
                     if (c.m > 0 && kbeg == 0 && rowok)
@@ -660,19 +677,25 @@ __global__ __launch_bounds__(256) void tile_kernel(
                             if (r < nr) acc[r] = cadd(acc[r], in[int64_t(r) * slot_len + c.in_ofs + row]);
                     return;
                 }
-                const int K = OP == 0 ? c.n : c.m;
+                const int K = OP == 0 ? c.n : c.m, last = K - 1;
+                const int rowc = min(row, tk.out_len - 1);
+                const C<R> *pa = F + c.fofs + (OP == 0 ? rowc : rowc * c.m);
+                const int ks = OP == 0 ? c.m : 1;
+                const C<R> *px = in + c.in_ofs;
                 for (int k0 = kbeg; k0 < K; k0 += kstep * QW) {
                     C<R> a[QW], xv[NR][QW];
 #pragma unroll
                     for (int q = 0; q < QW; ++q) {
                         const int k = k0 + q * kstep;
+                        const int kk = min(k, last);
                         const bool ok = rowok && k < K;
-                        const int64_t e = OP == 0 ? c.fofs + row + int64_t(k) * c.m : c.fofs + k + int64_t(row) * c.m;
-                        a[q] = ok ? F[e] : C<R>{R(0), R(0)};
+                        a[q] = pa[kk * ks];
                         a[q].y *= csign;
 #pragma unroll
-                        for (int r = 0; r < NR; ++r)
-                            xv[r][q] = (ok && r < nr) ? in[int64_t(r) * slot_len + c.in_ofs + k] : C<R>{R(0), R(0)};
+                        for (int r = 0; r < NR; ++r) {
+                            const C<R> v = px[int64_t(r < nr ? r : nr - 1) * slot_len + kk];
+                            xv[r][q] = C<R>{ok && r < nr ? v.x : R(0), ok && r < nr ? v.y : R(0)};
+                        }
                     }
 #pragma unroll
                     for (int q = 0; q < QW; ++q)
@@ -713,6 +736,204 @@ __global__ __launch_bounds__(256) void tile_kernel(
                         C<R> *yp = y + int64_t(rb + r) * ldy + perm_out[tk.out_ofs + row];
                         const C<R> old = *yp;
                         *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc[r]));
+                    }
+                }
+            }
+        }
+        tk = nt;
+        grp = ngrp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense forward kernel.  As the tiled kernel, but the lanes of a task are
+// packed to the block's own row count, two rows per lane: with LPT =
+// 2^lpt_shift lanes per task and r = min(2 LPT, rows left) rows, r2 =
+// ceil(r / 2) lanes form one group and lane l holds rows 2 i, 2 i + 1 with
+// i = l mod r2 of group g = l / r2 (G = LPT / r2 groups; lanes l >= G r2
+// idle).  One 16-byte load per lane reads both rows of a column, so the
+// active lanes of one load read G r consecutive entries of a column-major
+// block.  Groups split the inner index (CSPLIT false) or the contributions
+// (CSPLIT true).  Factor loads carry no predicate: columns are clamped into
+// the block (the second row of an odd block reads one entry past it: the
+// factor buffers are allocated with a tail pad, and that row is dropped).
+// The inputs of one pass are loaded once, one lane per column, and handed
+// to the row lanes by cross-lane permutes; inputs of clamped columns and
+// idle lanes are zeroed by a select.  Groups are summed by a shuffle tree
+// over the stride r2, in fixed order.
+// ---------------------------------------------------------------------------
+template <typename R, int OUTK, int NR, int QW, bool CSPLIT>
+__global__ __launch_bounds__(256) void dense_kernel(
+    const MvTask *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
+    C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
+    int fuse_beta, int nrhs, int lpt_shift, int xcd_map) {
+    const int lane = threadIdx.x & 63;
+    const int LPT = 1 << lpt_shift;
+    const int tpw = 64 >> lpt_shift;
+    const int sub = lane >> lpt_shift;
+    const int l = lane & (LPT - 1);
+    const int base = sub << lpt_shift;
+    const int rb = blockIdx.y * NR;
+    const int nr = min(NR, nrhs - rb);
+    const C<R> *in = slots + in_shift + int64_t(rb) * slot_len;
+    const MvContrib none{-1, 0, 0, 0, 0};
+    const int64_t n_groups = (int64_t(n_tasks) + tpw - 1) / tpw;
+    int64_t grp, grp_hi, n_waves;
+    if (xcd_map && gridDim.x >= 8) {  // see tile_kernel
+        const int64_t nwg = gridDim.x, x = blockIdx.x % 8;
+        n_waves = ((nwg - x + 7) / 8) * 4;
+        grp = n_groups * x / 8 + (int64_t(blockIdx.x) / 8) * 4 + (threadIdx.x >> 6);
+        grp_hi = n_groups * (x + 1) / 8;
+    } else {
+        n_waves = int64_t(gridDim.x) * 4;
+        grp = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+        grp_hi = n_groups;
+    }
+    auto load_task = [&](int64_t gi) -> MvTask {
+        const int64_t t = gi * tpw + sub;
+        if (t < n_tasks) return tasks[t];
+        MvTask z{};
+        z.fofs = -1;
+        return z;
+    };
+    auto contrib = [&](const MvTask &tk, int q) -> MvContrib {
+        if (q == 0) return MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
+        const int ci = tk.c_begin + q - 1;
+        return ci < tk.c_end ? contribs[ci] : none;
+    };
+    struct Pair { C<R> a, b; };
+
+    MvTask tk = load_task(grp);
+    while (grp < grp_hi) {
+        const int64_t ngrp = grp + n_waves;
+        const MvTask nt = ngrp < grp_hi ? load_task(ngrp) : load_task(n_groups);
+        const int nc = tk.out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
+        for (int row0 = 0; row0 < tk.out_len; row0 += 2 * LPT) {
+            const int r = min(2 * LPT, tk.out_len - row0);
+            const int r2 = (r + 1) >> 1;
+            // G = LPT / r2 and g = l / r2 through a reciprocal: exact for
+            // operands below 2^7 (the quotient is off by less than 1/r2^2)
+            const float inv = 1.0f / float(r2);
+            const int G = int((float(LPT) + 0.5f) * inv);
+            const int g = int((float(l) + 0.5f) * inv);
+            const int i2 = l - g * r2;
+            const bool act = g < G;
+            const int row = row0 + 2 * i2;
+            const bool two = 2 * i2 + 1 < r;
+            C<R> acc0[NR], acc1[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc0[q] = acc1[q] = C<R>{R(0), R(0)};
+            auto ident = [&](const MvContrib &c) {
+#pragma unroll
+                for (int rr = 0; rr < NR; ++rr)
+                    if (rr < nr) {
+                        const C<R> *p = in + int64_t(rr) * slot_len + c.in_ofs + row;
+                        acc0[rr] = cadd(acc0[rr], p[0]);
+                        if (two) acc1[rr] = cadd(acc1[rr], p[1]);
+                    }
+            };
+            // pass over columns k0 + kc, kc = kc0 + q kcs < w; the inputs
+            // are held by lanes xbase + j (column k0 + j, j < w)
+            auto pass = [&](const C<R> *pa, const C<R> *px, int m, int K, int k0, int w, int xbase, int xl,
+                            int kc0, int kcs) {
+                const int last = K - 1;
+                const int kx = k0 + xl;
+                const bool okx = xl < w && kx < K;
+                C<R> xs[NR];
+#pragma unroll
+                for (int rr = 0; rr < NR; ++rr) {
+                    const C<R> v = px[int64_t(rr < nr ? rr : nr - 1) * slot_len + min(kx, last)];
+                    xs[rr] = C<R>{okx ? v.x : R(0), okx ? v.y : R(0)};
+                }
+                Pair a[QW];
+                C<R> xv[NR][QW];
+#pragma unroll
+                for (int q = 0; q < QW; ++q) {
+                    const int kc = kc0 + q * kcs;
+                    const int k = k0 + kc;
+                    const bool ok = act && kc < w && k < K;
+                    __builtin_memcpy(&a[q], __builtin_assume_aligned(pa + min(k, last) * m, sizeof(C<R>)),
+                                     sizeof(Pair));
+                    const int src = xbase + min(kc, w - 1);
+#pragma unroll
+                    for (int rr = 0; rr < NR; ++rr) {
+                        const R vx = __shfl(xs[rr].x, src, 64), vy = __shfl(xs[rr].y, src, 64);
+                        xv[rr][q] = C<R>{ok && rr < nr ? vx : R(0), ok && rr < nr ? vy : R(0)};
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < QW; ++q)
+#pragma unroll
+                    for (int rr = 0; rr < NR; ++rr) {
+                        cfma(acc0[rr], a[q].a, xv[rr][q]);
+                        cfma(acc1[rr], a[q].b, xv[rr][q]);
+                    }
+            };
+            if (CSPLIT) {
+                // group g takes contributions g, g + G, ...; its r2 lanes hold
+                // the inputs of min(QW, r2) columns per pass
+                const int w = min(QW, r2);
+                MvContrib cur = g < nc ? contrib(tk, g) : none;
+                for (int q = g; q < nc; q += G) {
+                    const MvContrib nx = q + G < nc ? contrib(tk, q + G) : none;
+                    if (cur.fofs < 0) {
+                        if (cur.m > 0 && act) ident(cur);
+                    } else {
+                        for (int k0 = 0; k0 < cur.n; k0 += w)
+                            pass(F + cur.fofs + row, in + cur.in_ofs, cur.m, cur.n, k0, w, base + g * r2, i2, 0, 1);
+                    }
+                    cur = nx;
+                }
+            } else {
+                // groups split the columns: lane (g, i) takes columns
+                // k0 + g + G q of a window of min(G QW, LPT) columns
+                const int w = min(G * QW, LPT);
+                MvContrib cur = contrib(tk, 0);
+                for (int q = 0; q < nc; ++q) {
+                    const MvContrib nx = q + 1 < nc ? contrib(tk, q + 1) : none;
+                    if (cur.fofs < 0) {
+                        if (cur.m > 0 && g == 0 && act) ident(cur);
+                    } else {
+                        for (int k0 = 0; k0 < cur.n; k0 += w)
+                            pass(F + cur.fofs + row, in + cur.in_ofs, cur.m, cur.n, k0, w, base, l, g, G);
+                    }
+                    cur = nx;
+                }
+            }
+            for (int s = 1; s < G; s <<= 1) {
+                const int src = base + min(l + s * r2, LPT - 1);
+                const bool take = g % (2 * s) == 0 && g + s < G;
+#pragma unroll
+                for (int rr = 0; rr < NR; ++rr) {
+                    const R o0x = __shfl(acc0[rr].x, src, 64), o0y = __shfl(acc0[rr].y, src, 64);
+                    const R o1x = __shfl(acc1[rr].x, src, 64), o1y = __shfl(acc1[rr].y, src, 64);
+                    if (take) {
+                        acc0[rr].x += o0x;
+                        acc0[rr].y += o0y;
+                        acc1[rr].x += o1x;
+                        acc1[rr].y += o1y;
+                    }
+                }
+            }
+            if (g == 0) {
+#pragma unroll
+                for (int rr = 0; rr < NR; ++rr) {
+                    if (rr >= nr) break;
+                    if (OUTK == OUT_SLOT) {
+                        C<R> *o = slots + int64_t(rb + rr) * slot_len + tk.out_ofs + row;
+                        o[0] = acc0[rr];
+                        if (two) o[1] = acc1[rr];
+                    } else {
+                        C<R> *yr = y + int64_t(rb + rr) * ldy;
+                        C<R> *yp = yr + perm_out[tk.out_ofs + row];
+                        C<R> old = *yp;
+                        *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc0[rr]));
+                        if (two) {
+                            yp = yr + perm_out[tk.out_ofs + row + 1];
+                            old = *yp;
+                            *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc1[rr]));
+                        }
                     }
                 }
             }
@@ -766,7 +987,7 @@ static void launch_tile_t(hipStream_t s, const StageTable &t, const void *factor
 #define BFG_TILE(OUTK, CS)                                                                              \
     hipLaunchKernelGGL((tile_kernel<R, OP, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S, \
                        slot_len, in_shift, Y, ldy, perm_out, a, b, int(fuse), int(nrhs), int(t.tile.rp_shift),  \
-                       int(t.tile.g_shift), conj)
+                       int(t.tile.g_shift), conj, int(t.tile.xcd))
     if (t.out_kind == OUT_SLOT) {
         if (t.tile.csplit) BFG_TILE(OUT_SLOT, true);
         else BFG_TILE(OUT_SLOT, false);
@@ -776,6 +997,63 @@ static void launch_tile_t(hipStream_t s, const StageTable &t, const void *factor
     }
 #undef BFG_TILE
     BFG_HIP(hipGetLastError());
+}
+
+template <typename R, int NR, int QW>
+static void launch_dense_t(hipStream_t s, const StageTable &t, const void *factors, void *slots,
+                           int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                           int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu) {
+    C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
+    const int lpt = t.tile.rp_shift;
+    BFG_REQUIRE(lpt >= 0 && lpt <= 6, "matvec: bad dense shape");
+    const int64_t tpw = 64 >> lpt;
+    const int64_t wgs = ((int64_t(t.n_items) + tpw - 1) / tpw + 3) / 4;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))), unsigned((nrhs + NR - 1) / NR));
+    auto F = (const C<R> *)factors;
+    auto S = (C<R> *)slots;
+    auto Y = (C<R> *)y;
+    auto T = t.tasks.as<const MvTask>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_DENSE(OUTK, CS)                                                                              \
+    hipLaunchKernelGGL((dense_kernel<R, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S,  \
+                       slot_len, in_shift, Y, ldy, perm_out, a, b, int(fuse), int(nrhs), lpt, int(t.tile.xcd))
+    if (t.out_kind == OUT_SLOT) {
+        if (t.tile.csplit) BFG_DENSE(OUT_SLOT, true);
+        else BFG_DENSE(OUT_SLOT, false);
+    } else {
+        if (t.tile.csplit) BFG_DENSE(OUT_SCATTER, true);
+        else BFG_DENSE(OUT_SCATTER, false);
+    }
+#undef BFG_DENSE
+    BFG_HIP(hipGetLastError());
+}
+
+template <typename R>
+static void launch_dense_nr(hipStream_t s, const StageTable &t, const void *factors, void *slots,
+                            int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                            int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu) {
+    if (nrhs == 1) {
+        if (t.tile.qw4) launch_dense_t<R, 1, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+        else launch_dense_t<R, 1, 8>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+    } else if (nrhs == 2) {
+        launch_dense_t<R, 2, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+    } else if (nrhs <= 4) {
+        launch_dense_t<R, 4, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+    } else {
+        launch_dense_t<R, 8, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+    }
+}
+
+void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
+                  int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu) {
+    if (t.n_items == 0 || nrhs == 0) return;
+    BFG_REQUIRE(nrhs < 65536 * 8, "matvec: nrhs too large");
+    if (prec == BFGPU_C128)
+        launch_dense_nr<double>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu);
+    else
+        launch_dense_nr<float>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu);
 }
 
 template <typename R, int OP>

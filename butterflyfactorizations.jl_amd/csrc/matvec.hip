@@ -10,6 +10,7 @@
 // The adjoint and transpose (bfadjtr.jl:43-89) run the same tables backwards
 // with transposed (conjugated) block access instead of materialised copies.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -136,6 +137,10 @@ static void validate_tables(const BF &bf) {
             const int64_t out_lim = S.out_kind == OUT_SLOT ? bf.slot_len : (fwd ? ny : nx);
             const int64_t in_lim = S.in_kind == IN_SLOT ? bf.slot_len : (fwd ? nx : ny);
             const int64_t fe = L < int32_t(bf.factor_elems.size()) ? bf.factor_elems[L] : 0;
+            BFG_REQUIRE(fe == 0 || (L < int32_t(bf.factors.size()) &&
+                                    bf.factors[L].bytes >= size_t(fe + FACTOR_PAD) *
+                                                               (bf.precision == BFGPU_C128 ? 16 : 8)),
+                        "pack: factor buffer without its tail pad");
             std::atomic<int> bad{0};
             parallel_for(int64_t(S.h_items.size()), [&](int64_t i) {
                 const MvItem &it = S.h_items[i];
@@ -229,6 +234,17 @@ void pack_tables(BF &bf) {
     });
     for (int64_t b = 0; b < nb; ++b) slot_base[b + 1] += slot_base[b];
     const int64_t counter = slot_base[nb];
+    {
+        std::vector<int64_t> nd(nb, 0);
+        parallel_for(nb, [&](int64_t b) {
+            for (const auto &st : bf.bfs[b].st)
+                for (const Block &k : st.b)
+                    if (!k.ident) nd[b] += int64_t(k.m) * k.n;
+        });
+        bf.n_blocks = ix.total;
+        bf.n_data = 0;
+        for (int64_t v : nd) bf.n_data += v;
+    }
     BFG_REQUIRE(counter < (int64_t(1) << 31), "pack: skeleton vectors exceed 2^31 entries");
     bf.part_ofs = counter;
 
@@ -649,39 +665,78 @@ void release_graph(BF &bf) {
 
 BF::~BF() { release_graph(*this); }
 
+// HIP events around every kernel launch of a matvec (timing runs only):
+// ev[0] before the first launch, ev[i + 1] after launch i, described by
+// rec[i] = {level, table (0 level table, 1 partial sums, 2 input
+// permutation, 3 fused sweep), variant, output kind, packed launch shape,
+// right-hand sides per lane}.
+struct Timeline {
+    std::vector<hipEvent_t> ev;
+    std::vector<std::array<int32_t, 6>> rec;
+    size_t n = 0;
+    void start(hipStream_t s) {
+        n = 0;
+        rec.clear();
+        mark(s);
+    }
+    void mark(hipStream_t s) {
+        if (ev.size() <= n) {
+            hipEvent_t e;
+            BFG_HIP(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        BFG_HIP(hipEventRecord(ev[n++], s));
+    }
+    void launched(hipStream_t s, std::array<int32_t, 6> r) {
+        rec.push_back(r);
+        mark(s);
+    }
+    ~Timeline() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+static int32_t nr_per_lane(int64_t nrhs) { return nrhs <= 1 ? 1 : nrhs == 2 ? 2 : nrhs <= 4 ? 4 : 8; }
+
 static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                           int64_t ldy, const double *alpha, const double *beta,
-                           std::vector<hipEvent_t> *ev);
+                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl);
 
 // One launch table of level L: the tiled kernel (after permuting an
 // IN_GATHER input into the slot buffer) or the per-task kernels.
 static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nrhs, const void *x,
                       int64_t ldx, void *y, int64_t ldy, const double *alpha, const double *beta,
-                      bool fuse) {
+                      bool fuse, Timeline *tl = nullptr, int32_t table = 0) {
     hipStream_t s = bf.ctx->stream;
     const bool fwd = op == BFGPU_OP_N;
     const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
     const int32_t *pin = t.in_kind == IN_GATHER ? (fwd ? pr : pt) : nullptr;
     const int32_t *pout = t.out_kind == OUT_SCATTER ? (fwd ? pt : pr) : nullptr;
-    if (t.variant == 2) {
+    if (t.variant >= 2) {
         int64_t shift = 0;
         if (t.in_kind == IN_GATHER) {
             const int es = bf.precision == BFGPU_C128 ? 16 : 8;
             launch_gather(s, bf.precision, x, ldx, pin, t.in_lo, t.in_hi,
                           static_cast<char *>(bf.slots.p) + size_t(bf.gather_ofs) * es, bf.slot_len, nrhs);
             shift = bf.gather_ofs;
+            if (tl) tl->launched(s, {L, 2, 0, 0, 0, 1});
         }
-        launch_tile(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
-                    nrhs, alpha, beta, fuse, bf.ctx->n_cu);
+        if (t.variant == 3) {
+            BFG_REQUIRE(fwd, "matvec: the dense kernel is forward only");
+            launch_dense(s, bf.precision, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout, nrhs,
+                         alpha, beta, fuse, bf.ctx->n_cu);
+        } else {
+            launch_tile(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
+                        nrhs, alpha, beta, fuse, bf.ctx->n_cu);
+        }
     } else {
         launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pin,
                      pout, nrhs, alpha, beta, fuse);
     }
+    if (tl) tl->launched(s, {L, table, t.variant, t.out_kind, int32_t(pack_tile(t.tile)), nr_per_lane(nrhs)});
 }
 
 static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                      int64_t ldy, const double *alpha, const double *beta,
-                      std::vector<hipEvent_t> *ev) {
+                      int64_t ldy, const double *alpha, const double *beta, Timeline *tl) {
     hipStream_t s = bf.ctx->stream;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
     size_t need = size_t(bf.slot_len) * size_t(nrhs) * es;
@@ -691,8 +746,8 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
         bf.slots_cap = need;
     }
     const bool fused = op == BFGPU_OP_N && sweep_ok(bf, nrhs);
-    if (ev || !bf.use_graph || fused) {
-        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, ev);
+    if (tl || !bf.use_graph || fused) {
+        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl);
         return;
     }
     // per-level launches replayed from a captured graph while the call
@@ -722,8 +777,7 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
 }
 
 static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                           int64_t ldy, const double *alpha, const double *beta,
-                           std::vector<hipEvent_t> *ev) {
+                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl) {
     hipStream_t s = bf.ctx->stream;
     const int32_t nL = bf.n_stage_launch;
     const bool fwd = (op == BFGPU_OP_N);
@@ -733,8 +787,7 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
     if (!fuse) launch_scale(s, bf.precision, y, out_len, ldy, nrhs, beta);
     const double *b_eff = fuse ? beta : one;
     const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
-    int q = 0;
-    if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
+    if (tl) tl->start(s);
     if (fwd && sweep_ok(bf, nrhs)) {
         SweepTables &sw = bf.sweep;
         if (int64_t(sw.epoch + 1) * sw.max_wait >= (int64_t(1) << 30)) {
@@ -744,17 +797,14 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         ++sw.epoch;
         launch_sweep(s, bf.precision, sw, bf.factors, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pr, pt,
                      nrhs, alpha, b_eff, fuse, bf.ctx->n_cu);
-        if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
+        if (tl) tl->launched(s, {-1, 3, 0, 0, 0, nr_per_lane(nrhs)});
         return;
     }
-    (void)pt;
-    (void)pr;
     for (int32_t i = 0; i < nL; ++i) {
         int32_t L = fwd ? i : nL - 1 - i;
         const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
-        if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse);
-        run_table(bf, op, t, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse);
-        if (ev) BFG_HIP(hipEventRecord((*ev)[q++], s));
+        if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 1);
+        run_table(bf, op, t, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 0);
     }
 }
 
@@ -787,41 +837,57 @@ void matvec(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, i
     check_sweep_error(bf);
 }
 
+// Mean duration of every launch over `iters` device matvecs (alpha 1,
+// beta 0); launches are the same sequence in every iteration.
+static void time_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                          int32_t iters, std::vector<double> &ms, std::vector<std::array<int32_t, 6>> &rec,
+                          double &total_ms) {
+    BFG_REQUIRE(iters >= 1, "matvec_timed: iters >= 1");
+    static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    Timeline tl;
+    total_ms = 0.0;
+    for (int32_t it = 0; it < iters; ++it) {
+        stage_seq(bf, op, nrhs, x, ldx, y, ldy, one, zero, &tl);
+        const size_t nl = tl.rec.size();
+        BFG_HIP(hipEventSynchronize(tl.ev[nl]));
+        if (it == 0) {
+            ms.assign(nl, 0.0);
+            rec = tl.rec;
+        }
+        BFG_REQUIRE(nl == ms.size(), "matvec_timed: launch sequence changed between iterations");
+        for (size_t i = 0; i < nl; ++i) {
+            float e = 0.f;
+            BFG_HIP(hipEventElapsedTime(&e, tl.ev[i], tl.ev[i + 1]));
+            ms[i] += e;
+        }
+        float e = 0.f;
+        BFG_HIP(hipEventElapsedTime(&e, tl.ev[0], tl.ev[nl]));
+        total_ms += e;
+    }
+    for (double &v : ms) v /= iters;
+    total_ms /= iters;
+    check_sweep_error(bf);
+}
+
 void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                   int64_t ldy, int32_t iters, double *stage_ms, int64_t *stage_bytes,
                   int32_t *n_stages, double *total_ms) {
-    BFG_REQUIRE(iters >= 1, "matvec_timed: iters >= 1");
     const int32_t nL = bf.n_stage_launch;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
     const bool fwd = (op == BFGPU_OP_N);
-    const bool fused = fwd && sweep_ok(bf, nrhs);
-    const int32_t nS = fused ? 1 : nL;  // timed launches per matvec
-    std::vector<hipEvent_t> ev(nS + 1);
-    for (auto &e : ev) BFG_HIP(hipEventCreate(&e));
-    std::vector<double> acc(nS, 0.0);
+    std::vector<double> ms;
+    std::vector<std::array<int32_t, 6>> rec;
     double tot = 0.0;
-    static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
-    try {
-        for (int32_t it = 0; it < iters; ++it) {
-            stage_seq(bf, op, nrhs, x, ldx, y, ldy, one, zero, &ev);
-            BFG_HIP(hipEventSynchronize(ev[nS]));
-            for (int32_t i = 0; i < nS; ++i) {
-                float ms = 0.f;
-                BFG_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-                acc[i] += ms;
-            }
-            float ms = 0.f;
-            BFG_HIP(hipEventElapsedTime(&ms, ev[0], ev[nS]));
-            tot += ms;
-        }
-    } catch (...) {
-        for (auto &e : ev) (void)hipEventDestroy(e);
-        throw;
+    time_launches(bf, op, nrhs, x, ldx, y, ldy, iters, ms, rec, tot);
+    const bool fused = !rec.empty() && rec[0][1] == 3;
+    const int32_t nS = fused ? 1 : nL;  // reported entries: levels in launch order
+    std::vector<double> acc(nS, 0.0);
+    for (size_t i = 0; i < rec.size(); ++i) {
+        const int32_t L = rec[i][0];
+        acc[fused ? 0 : (fwd ? L : nL - 1 - L)] += ms[i];
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    check_sweep_error(bf);
     for (int32_t i = 0; i < nS; ++i) {
-        if (stage_ms) stage_ms[i] = acc[i] / iters;
+        if (stage_ms) stage_ms[i] = acc[i];
         if (!stage_bytes) continue;
         if (fused) {
             int64_t b = 0;
@@ -835,7 +901,21 @@ void matvec_timed(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void
         }
     }
     if (n_stages) *n_stages = nS;
-    if (total_ms) *total_ms = tot / iters;
+    if (total_ms) *total_ms = tot;
+}
+
+void matvec_launch_times(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                         int32_t iters, int32_t cap, int32_t *n_launch, double *launch_ms, int32_t *desc) {
+    std::vector<double> ms;
+    std::vector<std::array<int32_t, 6>> rec;
+    double tot = 0.0;
+    time_launches(bf, op, nrhs, x, ldx, y, ldy, iters, ms, rec, tot);
+    *n_launch = int32_t(ms.size());
+    for (int32_t i = 0; i < std::min<int32_t>(cap, int32_t(ms.size())); ++i) {
+        if (launch_ms) launch_ms[i] = ms[i];
+        if (desc)
+            for (int q = 0; q < 6; ++q) desc[6 * i + q] = rec[i][q];
+    }
 }
 
 // Raise (and clear) the fused sweep's error flag: a dependency wait that
@@ -918,6 +998,22 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                     sh.csplit = int8_t(cs);
                     consider(2, sh);
                 }
+        if (op == BFGPU_OP_N)
+            for (int lp = 2; lp <= 6; ++lp)
+                for (int cs = 0; cs <= (t.max_nc >= 2 ? 1 : 0); ++cs)
+                    for (int q4 = 0; q4 <= 1; ++q4) {
+                        TileShape sh = t0;
+                        sh.rp_shift = int8_t(lp);
+                        sh.g_shift = 0;
+                        sh.csplit = int8_t(cs);
+                        sh.qw4 = int8_t(q4);
+                        consider(3, sh);
+                    }
+        if (bv >= 2) {  // the other task-to-XCD mapping for the best shape
+            TileShape sh = bs;
+            sh.xcd = int8_t(1 - sh.xcd);
+            consider(bv, sh);
+        }
         t.variant = bv;
         t.tile = bs;
         return best;

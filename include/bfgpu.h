@@ -173,7 +173,9 @@ int bfgpu_bf_stage_stats(const bfgpu_bf *bf, int32_t *n_stages, int64_t *stats);
  * level in launch order: items, contributions, most contributions of an
  * item, factor entries, algorithmic vector bytes per RHS, kernel variant,
  * tiled shape (packed as for BFGPU_OPT_TILE), partial-sum items (0 when the
- * level has no split items).  8 int64 per level. */
+ * level has no split items), distinct input entries read, output entries
+ * written (twice for y, which is read and written).  10 int64 per level;
+ * the last two give the level's algorithmic vector traffic. */
 int bfgpu_bf_launch_stats(const bfgpu_bf *bf, int32_t op, int32_t *n_levels, int64_t *stats);
 
 /* Convert the factor storage to another precision (e.g. FP64 -> FP32). */
@@ -198,6 +200,23 @@ int bfgpu_matvec_timed(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, in
                        void *y, int64_t ldy, int32_t iters, double *stage_ms,
                        int64_t *stage_bytes, int32_t *n_stages, double *total_ms);
 
+/* Per-launch form of bfgpu_matvec_timed: mean duration (HIP events on the
+ * context stream) of every kernel launch of one matvec, in launch order;
+ * n_launch receives the count, the first `cap` launches are written.
+ * desc (6 int32 per launch): level, table (0 level table, 1 partial sums,
+ * 2 input permutation, 3 fused sweep), kernel variant, output kind (0
+ * slot, 1 rows of y), contribution split, right-hand sides per lane. */
+int bfgpu_matvec_launch_times(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
+                              void *y, int64_t ldy, int32_t iters, int32_t cap, int32_t *n_launch,
+                              double *launch_ms, int32_t *desc);
+/* Launch tuning of every table (what the autotuner chose), to save and
+ * restore: 8 int32 per level = (variant, packed tile shape) of the forward
+ * table, its partial-sum table, the transposed table and its partial-sum
+ * table, (-1, -1) where a level has none.  set_tuning validates all
+ * entries before applying any. */
+int bfgpu_bf_get_tuning(const bfgpu_bf *bf, int32_t *n, int32_t *vals);
+int bfgpu_bf_set_tuning(bfgpu_bf *bf, int32_t n, const int32_t *vals);
+
 /* enable = 1: forward matvecs with 1..8 right-hand sides run as one fused,
  * dependency-driven launch over all levels; 0 (default) selects one launch
  * per level.  Both give the same results to rounding. */
@@ -210,20 +229,30 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        graph when x, y, alpha, beta repeat
  *   BFGPU_OPT_VARIANT    kernel of every launch table: 0 workgroup per task,
  *                        1 wavefront per task (forward; transposed tables
- *                        take 0), 2 tiled (several tasks per wavefront);
- *                        get: the common value, or -1 when levels differ
+ *                        take 0), 2 tiled (several tasks per wavefront),
+ *                        3 dense (forward; lanes packed to each block's row
+ *                        count; transposed tables take 2); get: the common
+ *                        value, or -1 when levels differ
  *   BFGPU_OPT_TILE       tiled shape of every table, packed as rp_shift |
- *                        g_shift << 4 | csplit << 8 | wg_per_cu << 12
- *                        (2^rp_shift outputs x 2^g_shift lane groups per
- *                        task, rp_shift + g_shift <= 6; groups split the
- *                        inner index (csplit 0) or the contributions (1);
- *                        wg_per_cu 1..64); also selects variant 2 */
+ *                        g_shift << 4 | csplit << 8 | wg_per_cu << 12 |
+ *                        xcd << 20 (2^rp_shift outputs x 2^g_shift lane
+ *                        groups per task, rp_shift + g_shift <= 6; groups
+ *                        split the inner index (csplit 0) or the
+ *                        contributions (1); wg_per_cu 1..64; xcd 1: each
+ *                        XCD walks one contiguous range of tasks; qw4 <<
+ *                        24: 4 instead of 8 loads per lane and pass for
+ *                        the dense kernel at one RHS); also selects
+ *                        variant 2
+ *   BFGPU_OPT_DENSE      dense shape of every forward table, packed as for
+ *                        BFGPU_OPT_TILE with rp_shift = log2(lanes per
+ *                        task) and g_shift = 0; selects variant 3 */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
     BFGPU_OPT_GRAPH = 3,
     BFGPU_OPT_VARIANT = 4,
-    BFGPU_OPT_TILE = 5
+    BFGPU_OPT_TILE = 5,
+    BFGPU_OPT_DENSE = 6
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

@@ -15,20 +15,7 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
-def two_squares(n, seed=0, side=16.0, gap=32.0):
-    """Observers uniform on [0,side]^2 x {0}, sources on [0,side]^2 x {gap}
-    (the single-block geometry of the two-squares workloads)."""
-    rng = np.random.default_rng(seed)
-    obs = np.c_[rng.uniform(0, side, n), rng.uniform(0, side, n), np.zeros(n)]
-    src = np.c_[rng.uniform(0, side, n), rng.uniform(0, side, n), np.full(n, gap)]
-    return obs, src
-
-
-def fibonacci_sphere(n, radius):
-    i = np.arange(n) + 0.5
-    phi = np.arccos(1 - 2 * i / n)
-    theta = np.pi * (1 + 5 ** 0.5) * i
-    return radius * np.c_[np.cos(theta) * np.sin(phi), np.sin(theta) * np.sin(phi), np.cos(phi)]
+from bfgpu.geometry import ellipsoid_surface, fibonacci_sphere, plane_waves, two_squares  # noqa: E402,F401
 
 
 def crandn(rng, *shape):

@@ -1,8 +1,8 @@
 """Kernel variants of the matvec levels against the oracle.
 
 Every launch table can run on the workgroup-per-task kernel (variant 0), the
-wavefront-per-task kernel (variant 1, forward), or the tiled kernel
-(variant 2) in any legal shape; observer-leaf sums over many
+wavefront-per-task kernel (variant 1, forward), the tiled kernel (variant 2)
+or the dense kernel (variant 3, forward) in any legal shape; observer-leaf sums over many
 factorizations are split into partial sums plus a combine pass.  Bar
 (written in each test): every variant and shape gives the oracle's FP64
 matvec of the same factorization to 1e-12 relative, in all three
@@ -44,6 +44,19 @@ def _shapes(bfgpu):
                     continue
                 out.append(bfgpu.ButterflyFactorization.tile_shape(rp, gs, cs, 8))
     out.append(bfgpu.ButterflyFactorization.tile_shape(3, 1, 0, 1))  # one workgroup per CU
+    # contiguous task range per XCD (small levels launch fewer than 8
+    # workgroups and fall back to the plain mapping)
+    for rp, gs, cs, wg in ((3, 1, 0, 8), (2, 2, 1, 8), (4, 0, 0, 1), (6, 0, 0, 2)):
+        out.append(bfgpu.ButterflyFactorization.tile_shape(rp, gs, cs, wg, 1))
+    return out
+
+
+def _dense_shapes(bfgpu):
+    """Dense forward shapes: lanes per task 1..64, both group modes, 4 or
+    8 loads per pass, both task-to-XCD mappings."""
+    sh = bfgpu.ButterflyFactorization.tile_shape
+    out = [sh(lp, 0, cs, 8, 0, q4) for lp in range(0, 7) for cs in (0, 1) for q4 in (0, 1)]
+    out += [sh(4, 0, 0, 8, 1, 0), sh(5, 0, 1, 2, 1, 1), sh(6, 0, 0, 1, 1, 0)]
     return out
 
 
@@ -75,7 +88,12 @@ def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
 
     X = crandn(np.random.default_rng(7), n, 8)
     refs = {"N": cols(imp.matvec)(X), "C": cols(adj.matvec)(X), "T": cols(tra.matvec)(X)}
-    for v in (0, 1, 2):
+    # traffic columns: the first level reads every source value once, the
+    # last one reads and writes every row of y
+    st = g.launch_stats()
+    assert st[0, 8] == n and st[-1, 9] == 2 * n
+    assert (st[:, 3] > 0).all() and st[:, 3].sum() * 16 == g.info().factor_bytes
+    for v in (0, 1, 2, 3):
         say("variant", v)
         g.set_option("variant", v)
         assert g.get_option("variant") == v
@@ -85,6 +103,11 @@ def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
         g.set_option("tile", sh)
         assert g.get_option("tile") == sh and g.get_option("variant") == 2
         _check_all_directions(g, refs, X)
+    for sh in _dense_shapes(bfgpu):
+        say("dense", hex(sh))
+        g.set_option("dense", sh)
+        assert g.get_option("dense") == sh
+        _check_all_directions(g, {"N": refs["N"]}, X)
     # the autotuner's choice (both directions) keeps the product
     ms, var = g.autotune(nrhs=1, iters=2)
     assert len(ms) == len(var) == len(g)
@@ -92,7 +115,7 @@ def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
     with pytest.raises(bfgpu.ArgumentError):
         g.set_option("tile", bfgpu.ButterflyFactorization.tile_shape(4, 3))
     with pytest.raises(bfgpu.ArgumentError):
-        g.set_option("variant", 3)
+        g.set_option("variant", 4)
 
 
 def test_variants_sphere_partial_sums_match_oracle(bfgpu, oracle, ctx):
@@ -120,13 +143,17 @@ def test_variants_sphere_partial_sums_match_oracle(bfgpu, oracle, ctx):
 
     X = crandn(np.random.default_rng(8), n, 8)
     refs = {"N": cols(imps)(X), "C": cols(adj)(X), "T": cols(tra)(X)}
-    for v in (0, 1, 2):
+    for v in (0, 1, 2, 3):
         say("variant", v)
         op.set_option("variant", v)
         _check_all_directions(op, refs, X)
-    for sh in _shapes(bfgpu)[::3]:
+    for sh in _shapes(bfgpu)[::3] + _shapes(bfgpu)[-4:]:
         say("tile", hex(sh))
         op.set_option("tile", sh)
         _check_all_directions(op, refs, X)
+    for sh in _dense_shapes(bfgpu)[::2]:
+        say("dense", hex(sh))
+        op.set_option("dense", sh)
+        _check_all_directions(op, {"N": refs["N"]}, X)
     op.autotune(nrhs=1, iters=2)
     _check_all_directions(op, refs, X)
