@@ -136,6 +136,17 @@ struct MvTask {
     int32_t pad;
 };
 
+// Compact task of the tiled and dense kernels (32 bytes, read by a wave as
+// one coalesced load of its task group): first contribution inline,
+// further ones in contribs[c_begin, c_end).
+struct MvTaskC {
+    int64_t fofs;
+    int32_t in_ofs, out_ofs;
+    int32_t m, n;
+    int32_t c_begin, c_end;
+};
+static_assert(sizeof(MvTaskC) == 32, "MvTaskC is 8 dwords");
+
 enum InKind { IN_SLOT = 0, IN_GATHER = 1 };
 enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
 
@@ -163,8 +174,10 @@ inline TileShape unpack_tile(int64_t v) {
     t.qw4 = int8_t((v >> 24) & 15);
     return t;
 }
+// at least 4 lanes per task (a wave's task group is at most 16 descriptors)
 inline bool tile_ok(const TileShape &t) {
-    return t.rp_shift >= 0 && t.g_shift >= 0 && t.rp_shift + t.g_shift <= 6 && t.csplit <= 1 &&
+    return t.rp_shift >= 0 && t.g_shift >= 0 && t.rp_shift + t.g_shift >= 2 && t.rp_shift + t.g_shift <= 6 &&
+           t.csplit <= 1 &&
            t.wg_per_cu >= 1 && t.wg_per_cu <= 64 && t.xcd <= 1 && t.qw4 <= 1;
 }
 
@@ -182,7 +195,7 @@ struct StageTable {
     TileShape tile;
     int32_t in_lo = 0, in_hi = 0;      // IN_GATHER: value range read through the permutation
     int32_t max_nc = 0;                // most contributions of one item
-    DevBuf items, contribs, tasks;
+    DevBuf items, contribs, tasks, tasks_c;
     std::vector<MvItem> h_items;
     std::vector<MvContrib> h_contribs;
     std::vector<MvTask> h_tasks;

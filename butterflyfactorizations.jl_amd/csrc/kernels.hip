@@ -587,6 +587,36 @@ static void launch_fwd_t(hipStream_t s, const StageTable &t, const void *factors
     BFG_HIP(hipGetLastError());
 }
 
+// Task descriptors of one wave's task group (tpw consecutive MvTaskC, at
+// most 16 = 128 dwords): lane j holds dwords 2 j and 2 j + 1, read with one
+// coalesced load; zero past the last task (an empty task, m = n = 0).
+__device__ __forceinline__ uint2 load_task_group(const MvTaskC *__restrict__ tasks, int32_t n_tasks, int64_t gi,
+                                                 int tpw, int lane) {
+    const int64_t d = gi * tpw * 8 + 2 * lane;
+    if (2 * lane < tpw * 8 && d < int64_t(n_tasks) * 8) return reinterpret_cast<const uint2 *>(tasks)[d >> 1];
+    return make_uint2(0u, 0u);
+}
+
+struct TaskView {
+    int64_t fofs;
+    int in_ofs, out_ofs, m, n, c_begin, c_end;
+};
+
+// Fields of task `sub` of the group held in v; every lane of the wave must
+// take part (the fields are cross-lane permutes).
+__device__ __forceinline__ TaskView unpack_task(uint2 v, int sub) {
+    auto f = [&](int k) { return __shfl(int((k & 1) ? v.y : v.x), sub * 4 + (k >> 1), 64); };
+    TaskView t;
+    t.fofs = int64_t((uint64_t(uint32_t(f(1))) << 32) | uint64_t(uint32_t(f(0))));
+    t.in_ofs = f(2);
+    t.out_ofs = f(3);
+    t.m = f(4);
+    t.n = f(5);
+    t.c_begin = f(6);
+    t.c_end = f(7);
+    return t;
+}
+
 // ---------------------------------------------------------------------------
 // Tiled stage kernel, forward (OP 0) and transposed (OP 1, conjugated when
 // `conj`).  A wavefront holds 64 >> (rp_shift + g_shift) tasks; the lanes of
@@ -608,7 +638,7 @@ static void launch_fwd_t(hipStream_t s, const StageTable &t, const void *factors
 // ---------------------------------------------------------------------------
 template <typename R, int OP, int OUTK, int NR, int QW, bool CSPLIT>
 __global__ __launch_bounds__(256) void tile_kernel(
-    const MvTask *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const MvTaskC *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
     const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
     C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
     int fuse_beta, int nrhs, int rp_shift, int g_shift, int conj, int xcd_map) {
@@ -641,28 +671,23 @@ __global__ __launch_bounds__(256) void tile_kernel(
         grp_hi = n_groups;
     }
 
-    auto load_task = [&](int64_t gi) -> MvTask {
-        const int64_t t = gi * tpw + sub;
-        if (t < n_tasks) return tasks[t];
-        MvTask z{};
-        z.fofs = -1;
-        return z;
-    };
     // contribution q of a task: q = 0 is the inline one
-    auto contrib = [&](const MvTask &tk, int q) -> MvContrib {
+    auto contrib = [&](const TaskView &tk, int q) -> MvContrib {
         if (q == 0) return MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
         const int ci = tk.c_begin + q - 1;
         return ci < tk.c_end ? contribs[ci] : none;
     };
 
-    MvTask tk = load_task(grp);
+    uint2 tv = load_task_group(tasks, n_tasks, grp, tpw, lane);
     while (grp < grp_hi) {
         const int64_t ngrp = grp + n_waves;
-        const MvTask nt = ngrp < grp_hi ? load_task(ngrp) : load_task(n_groups);
-        const int nc = tk.out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
-        for (int row0 = 0; row0 < tk.out_len; row0 += RP) {
+        const uint2 nv = ngrp < grp_hi ? load_task_group(tasks, n_tasks, ngrp, tpw, lane) : make_uint2(0u, 0u);
+        TaskView tk = unpack_task(tv, sub);
+        const int out_len = OP == 0 ? tk.m : tk.n;
+        const int nc = out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
+        for (int row0 = 0; row0 < out_len; row0 += RP) {
             const int row = row0 + i;
-            const bool rowok = row < tk.out_len;
+            const bool rowok = row < out_len;
             C<R> acc[NR];
 #pragma unroll
             for (int r = 0; r < NR; ++r) acc[r] = C<R>{R(0), R(0)};
@@ -678,7 +703,7 @@ __global__ __launch_bounds__(256) void tile_kernel(
                     return;
                 }
                 const int K = OP == 0 ? c.n : c.m, last = K - 1;
-                const int rowc = min(row, tk.out_len - 1);
+                const int rowc = min(row, out_len - 1);
                 const C<R> *pa = F + c.fofs + (OP == 0 ? rowc : rowc * c.m);
                 const int ks = OP == 0 ? c.m : 1;
                 const C<R> *px = in + c.in_ofs;
@@ -740,7 +765,7 @@ __global__ __launch_bounds__(256) void tile_kernel(
                 }
             }
         }
-        tk = nt;
+        tv = nv;
         grp = ngrp;
     }
 }
@@ -764,7 +789,7 @@ __global__ __launch_bounds__(256) void tile_kernel(
 // ---------------------------------------------------------------------------
 template <typename R, int OUTK, int NR, int QW, bool CSPLIT>
 __global__ __launch_bounds__(256) void dense_kernel(
-    const MvTask *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const MvTaskC *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
     const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
     C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
     int fuse_beta, int nrhs, int lpt_shift, int xcd_map) {
@@ -790,27 +815,22 @@ __global__ __launch_bounds__(256) void dense_kernel(
         grp = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
         grp_hi = n_groups;
     }
-    auto load_task = [&](int64_t gi) -> MvTask {
-        const int64_t t = gi * tpw + sub;
-        if (t < n_tasks) return tasks[t];
-        MvTask z{};
-        z.fofs = -1;
-        return z;
-    };
-    auto contrib = [&](const MvTask &tk, int q) -> MvContrib {
+    auto contrib = [&](const TaskView &tk, int q) -> MvContrib {
         if (q == 0) return MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
         const int ci = tk.c_begin + q - 1;
         return ci < tk.c_end ? contribs[ci] : none;
     };
     struct Pair { C<R> a, b; };
 
-    MvTask tk = load_task(grp);
+    uint2 tv = load_task_group(tasks, n_tasks, grp, tpw, lane);
     while (grp < grp_hi) {
         const int64_t ngrp = grp + n_waves;
-        const MvTask nt = ngrp < grp_hi ? load_task(ngrp) : load_task(n_groups);
-        const int nc = tk.out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
-        for (int row0 = 0; row0 < tk.out_len; row0 += 2 * LPT) {
-            const int r = min(2 * LPT, tk.out_len - row0);
+        const uint2 nv = ngrp < grp_hi ? load_task_group(tasks, n_tasks, ngrp, tpw, lane) : make_uint2(0u, 0u);
+        const TaskView tk = unpack_task(tv, sub);
+        const int out_len = tk.m;
+        const int nc = out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
+        for (int row0 = 0; row0 < out_len; row0 += 2 * LPT) {
+            const int r = min(2 * LPT, out_len - row0);
             const int r2 = (r + 1) >> 1;
             // G = LPT / r2 and g = l / r2 through a reciprocal: exact for
             // operands below 2^7 (the quotient is off by less than 1/r2^2)
@@ -938,7 +958,7 @@ __global__ __launch_bounds__(256) void dense_kernel(
                 }
             }
         }
-        tk = nt;
+        tv = nv;
         grp = ngrp;
     }
 }
@@ -974,7 +994,7 @@ static void launch_tile_t(hipStream_t s, const StageTable &t, const void *factor
                           int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu, int conj) {
     C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
     const int lpt = t.tile.rp_shift + t.tile.g_shift;
-    BFG_REQUIRE(t.tile.rp_shift >= 0 && t.tile.g_shift >= 0 && lpt <= 6, "matvec: bad tile shape");
+    BFG_REQUIRE(tile_ok(t.tile), "matvec: bad tile shape");
     const int64_t tpw = 64 >> lpt;
     const int64_t wgs = ((int64_t(t.n_items) + tpw - 1) / tpw + 3) / 4;
     const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
@@ -982,7 +1002,7 @@ static void launch_tile_t(hipStream_t s, const StageTable &t, const void *factor
     auto F = (const C<R> *)factors;
     auto S = (C<R> *)slots;
     auto Y = (C<R> *)y;
-    auto T = t.tasks.as<const MvTask>();
+    auto T = t.tasks_c.as<const MvTaskC>();
     auto P = t.contribs.as<const MvContrib>();
 #define BFG_TILE(OUTK, CS)                                                                              \
     hipLaunchKernelGGL((tile_kernel<R, OP, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S, \
@@ -1005,7 +1025,7 @@ static void launch_dense_t(hipStream_t s, const StageTable &t, const void *facto
                            int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu) {
     C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
     const int lpt = t.tile.rp_shift;
-    BFG_REQUIRE(lpt >= 0 && lpt <= 6, "matvec: bad dense shape");
+    BFG_REQUIRE(tile_ok(t.tile) && t.tile.g_shift == 0, "matvec: bad dense shape");
     const int64_t tpw = 64 >> lpt;
     const int64_t wgs = ((int64_t(t.n_items) + tpw - 1) / tpw + 3) / 4;
     const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
@@ -1013,7 +1033,7 @@ static void launch_dense_t(hipStream_t s, const StageTable &t, const void *facto
     auto F = (const C<R> *)factors;
     auto S = (C<R> *)slots;
     auto Y = (C<R> *)y;
-    auto T = t.tasks.as<const MvTask>();
+    auto T = t.tasks_c.as<const MvTaskC>();
     auto P = t.contribs.as<const MvContrib>();
 #define BFG_DENSE(OUTK, CS)                                                                              \
     hipLaunchKernelGGL((dense_kernel<R, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S,  \

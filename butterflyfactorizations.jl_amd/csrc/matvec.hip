@@ -503,7 +503,7 @@ void pack_tables(BF &bf) {
         S.tile.csplit = avg_nc >= 1.5 ? 1 : 0;
         int gb = S.tile.csplit ? log2up(int64_t(std::min(avg_nc, 8.0) + 0.5))
                                : std::max(0, pow2_quantile(hk, 0.9) - 3);
-        S.tile.g_shift = int8_t(std::max(0, std::min(gb, 6 - rb)));
+        S.tile.g_shift = int8_t(std::max(2 - rb, std::min(gb, 6 - rb)));
         S.tile.wg_per_cu = 8;
         S.n_items = int32_t(S.h_items.size());
         S.n_contrib = int32_t(S.h_contribs.size());
@@ -521,6 +521,14 @@ void pack_tables(BF &bf) {
         upload_tab(S.items, S.h_items);
         upload_tab(S.contribs, S.h_contribs);
         upload_tab(S.tasks, S.h_tasks);
+        {
+            std::vector<MvTaskC> tc(S.h_tasks.size());
+            parallel_for(int64_t(tc.size()), [&](int64_t i) {
+                const MvTask &t = S.h_tasks[i];
+                tc[i] = MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.m, t.n, t.c_begin, t.c_end};
+            }, 4096);
+            upload_tab(S.tasks_c, tc);
+        }
         if (S.pre) finish(*S.pre, fwd);
     };
 
@@ -992,6 +1000,7 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
             for (int gs = 0; gs <= std::min(3, 6 - rp); ++gs)
                 for (int cs = 0; cs <= (t.max_nc >= 2 ? 1 : 0); ++cs) {
                     if (cs == 1 && gs == 0) continue;  // same as cs == 0 with one group
+                    if (rp + gs < 2) continue;          // at least 4 lanes per task
                     TileShape sh = t0;
                     sh.rp_shift = int8_t(rp);
                     sh.g_shift = int8_t(gs);

@@ -236,7 +236,7 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *   BFGPU_OPT_TILE       tiled shape of every table, packed as rp_shift |
  *                        g_shift << 4 | csplit << 8 | wg_per_cu << 12 |
  *                        xcd << 20 (2^rp_shift outputs x 2^g_shift lane
- *                        groups per task, rp_shift + g_shift <= 6; groups
+ *                        groups per task, 2 <= rp_shift + g_shift <= 6; groups
  *                        split the inner index (csplit 0) or the
  *                        contributions (1); wg_per_cu 1..64; xcd 1: each
  *                        XCD walks one contiguous range of tasks; qw4 <<

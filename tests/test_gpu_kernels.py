@@ -40,7 +40,7 @@ def _shapes(bfgpu):
     for rp in range(0, 7):
         for gs in sorted({0, min(2, 6 - rp), 6 - rp}):
             for cs in (0, 1):
-                if cs and gs == 0:
+                if (cs and gs == 0) or rp + gs < 2:
                     continue
                 out.append(bfgpu.ButterflyFactorization.tile_shape(rp, gs, cs, 8))
     out.append(bfgpu.ButterflyFactorization.tile_shape(3, 1, 0, 1))  # one workgroup per CU
@@ -52,10 +52,10 @@ def _shapes(bfgpu):
 
 
 def _dense_shapes(bfgpu):
-    """Dense forward shapes: lanes per task 1..64, both group modes, 4 or
+    """Dense forward shapes: lanes per task 4..64, both group modes, 4 or
     8 loads per pass, both task-to-XCD mappings."""
     sh = bfgpu.ButterflyFactorization.tile_shape
-    out = [sh(lp, 0, cs, 8, 0, q4) for lp in range(0, 7) for cs in (0, 1) for q4 in (0, 1)]
+    out = [sh(lp, 0, cs, 8, 0, q4) for lp in range(2, 7) for cs in (0, 1) for q4 in (0, 1)]
     out += [sh(4, 0, 0, 8, 1, 0), sh(5, 0, 1, 2, 1, 1), sh(6, 0, 0, 1, 1, 0)]
     return out
 
@@ -114,6 +114,8 @@ def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
     _check_all_directions(g, refs, X)
     with pytest.raises(bfgpu.ArgumentError):
         g.set_option("tile", bfgpu.ButterflyFactorization.tile_shape(4, 3))
+    with pytest.raises(bfgpu.ArgumentError):  # fewer than 4 lanes per task
+        g.set_option("dense", bfgpu.ButterflyFactorization.tile_shape(1, 0))
     with pytest.raises(bfgpu.ArgumentError):
         g.set_option("variant", 4)
 
