@@ -91,16 +91,19 @@ def alg_bytes(stats, es):
 
 def kernel_name(d, op, real):
     """Template name (as rocprofv3 prints it) of one launch described by
-    (level, table, variant, output kind, contribution split, RHS per lane)."""
+    (level, table, variant, output kind, packed launch shape, RHS per lane)."""
     level, table, variant, outk, cs, nr = (int(v) for v in d)
     if table == 2:
         return f"gather_kernel<{real}>"
     if table == 3:
         return f"sweep_kernel<{real}, {nr}>"
     ink = 0 if (table == 1 or level > 0) else 1
+    if variant == 3:
+        qw = (4 if (cs >> 24) & 1 else 8) if nr == 1 else (4 if nr == 2 else 2)
+        return f"dense_kernel<{real}, {outk}, {nr}, {qw}, {'true' if (cs >> 8) & 1 else 'false'}>"
     if variant == 2:
         qw = 8 if nr <= 2 else 4
-        return f"tile_kernel<{real}, {0 if op == 0 else 1}, {outk}, {nr}, {qw}, {'true' if cs else 'false'}>"
+        return f"tile_kernel<{real}, {0 if op == 0 else 1}, {outk}, {nr}, {qw}, {'true' if (cs >> 8) & 1 else 'false'}>"
     if op != 0:
         return f"stage_kernel<{real}, {op}, {ink}, {outk}>"
     return f"{'fwdw_kernel' if variant == 1 else 'fwd_kernel'}<{real}, {ink}, {outk}, {nr}>"
@@ -202,10 +205,10 @@ def main():
         a = torch.empty(1 << 27, dtype=torch.complex64, device="cuda")  # 1 GiB
         b = torch.empty_like(a)
         a.fill_(1.0)
-        b.copy_(a)
+        torch.mul(a, 2.0, out=b)  # an elementwise kernel: 1 GiB read, 1 GiB written
         torch.cuda.synchronize()
         del a, b
-        log("calibration copy: 1 GiB read + 1 GiB written")
+        log("calibration kernel: 1 GiB read + 1 GiB written")
 
     gen = torch.Generator(device="cpu").manual_seed(0)
     xr = torch.randn(cols_n, generator=gen, dtype=torch.float64) / np.sqrt(2)

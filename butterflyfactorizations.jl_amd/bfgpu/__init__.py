@@ -574,7 +574,7 @@ class ButterflyFactorization:
     def launch_times(self, x, y, iters=10, nrhs=1):
         """Mean milliseconds of every kernel launch of one device matvec (HIP
         events on the context stream) and a description per launch:
-        (level, table, variant, output kind, contribution split, RHS per
+        (level, table, variant, output kind, packed launch shape, RHS per
         lane); table 0 level, 1 partial sums, 2 input permutation, 3 sweep."""
         rows, cols = self.shape
         ldx = cols if self.op == L.OP_N else rows
