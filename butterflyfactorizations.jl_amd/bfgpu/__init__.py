@@ -527,6 +527,17 @@ class ButterflyFactorization:
         """Synchronise and raise if the fused sweep flagged a fault."""
         check(lib().bfgpu_bf_check(self.handle))
 
+    def compressions(self):
+        """All compression records of the operator: (rec, feat) with rec
+        (n, 7) int32 = (factorization, stage, snode, onode, guess, rows,
+        rank) and feat (n, 6) float32, the rank predictor's features."""
+        n = C.c_int64()
+        check(lib().bfgpu_bf_compressions(self.handle, C.byref(n), None, None))
+        rec = np.zeros((n.value, 7), np.int32)
+        feat = np.zeros((n.value, 6), np.float32)
+        check(lib().bfgpu_bf_compressions(self.handle, C.byref(n), ptr(rec, C.c_int32), ptr(feat, C.c_float)))
+        return rec, feat
+
     def launch_stats(self):
         """Per level in launch order of this operator's direction: items,
         contributions, most contributions of an item, factor entries,
@@ -704,7 +715,8 @@ class RankPredictor:
         h = C.c_void_p()
         check(lib().bfgpu_mlp_create(self.ctx.handle, n_in, hidden, n_layers, int(seed), C.byref(h)))
         self._h = h
-        self.n_in = n_in
+        self.n_in, self.hidden, self.n_layers = n_in, hidden, n_layers
+        self.norm = (np.zeros(n_in, np.float32), np.ones(n_in, np.float32), 0.0, 1.0)
 
     @property
     def handle(self):
@@ -729,6 +741,7 @@ class RankPredictor:
     def set_norm(self, mu, sigma, out_shift=0.0, out_scale=1.0):
         mu = np.ascontiguousarray(mu, np.float32)
         sigma = np.ascontiguousarray(sigma, np.float32)
+        self.norm = (mu.copy(), sigma.copy(), float(out_shift), float(out_scale))
         check(lib().bfgpu_mlp_set_norm(self._h, ptr(mu, C.c_float), ptr(sigma, C.c_float),
                                        float(out_shift), float(out_scale)))
 

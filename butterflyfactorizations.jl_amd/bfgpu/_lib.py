@@ -43,6 +43,7 @@ class BFInfo(C.Structure):
         ("max_stages", i32), ("n_blocks", i64), ("n_data", i64),
         ("factor_bytes", i64), ("slot_len", i64), ("n_comp", i64),
         ("n_retry", i64), ("t_factor_ms", f64),
+        ("t_plan_ms", f64), ("t_sample_ms", f64), ("t_qr_ms", f64), ("t_interp_ms", f64), ("t_pack_ms", f64),
     ]
 
 
@@ -79,6 +80,7 @@ _SIGS = {
     "bfgpu_bf_export": (i32, [vp, i32, P(i32), P(i64), P(f64)]),
     "bfgpu_bf_export_logs": (i32, [vp, i32, P(i32), P(i32), P(i32)]),
     "bfgpu_bf_export_features": (i32, [vp, i32, P(f32)]),
+    "bfgpu_bf_compressions": (i32, [vp, P(i64), P(i32), P(f32)]),
     "bfgpu_bf_stage_stats": (i32, [vp, P(i32), P(i64)]),
     "bfgpu_bf_launch_stats": (i32, [vp, i32, P(i32), P(i64)]),
     "bfgpu_bf_convert": (i32, [vp, i32, P(vp)]),

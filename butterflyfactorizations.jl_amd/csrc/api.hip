@@ -243,6 +243,11 @@ extern "C" int bfgpu_bf_info_get(const bfgpu_bf *h, bfgpu_bf_info *o) {
     o->n_comp = bf.n_comp;
     o->n_retry = bf.n_retry;
     o->t_factor_ms = bf.t_factor_ms;
+    o->t_plan_ms = bf.t_plan_ms;
+    o->t_sample_ms = bf.t_sample_ms;
+    o->t_qr_ms = bf.t_qr_ms;
+    o->t_interp_ms = bf.t_interp_ms;
+    o->t_pack_ms = bf.t_pack_ms;
     BFG_API_END
 }
 
@@ -365,6 +370,37 @@ extern "C" int bfgpu_bf_export_features(const bfgpu_bf *h, int32_t index, float 
     const OneBF &ob = h->b.bfs[index];
     for (size_t i = 0; i < ob.comp.size(); ++i)
         for (int q = 0; q < 6; ++q) feat[6 * i + q] = ob.comp[i].feat[q];
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_compressions(const bfgpu_bf *h, int64_t *n, int32_t *rec, float *feat) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && n, "bf_compressions: null argument");
+    const BF &bf = h->b;
+    const int64_t nb = int64_t(bf.bfs.size());
+    std::vector<int64_t> ofs(nb + 1, 0);
+    for (int64_t b = 0; b < nb; ++b) ofs[b + 1] = ofs[b] + int64_t(bf.bfs[b].comp.size());
+    *n = ofs[nb];
+    if (!rec && !feat) return BFGPU_OK;
+    parallel_for(nb, [&](int64_t b) {
+        const auto &cs = bf.bfs[b].comp;
+        for (size_t i = 0; i < cs.size(); ++i) {
+            const int64_t q = ofs[b] + int64_t(i);
+            const CompRec &c = cs[i];
+            if (rec) {
+                int32_t *r = rec + 7 * q;
+                r[0] = int32_t(b);
+                r[1] = c.stage;
+                r[2] = c.s;
+                r[3] = c.o;
+                r[4] = c.guess;
+                r[5] = c.rows;
+                r[6] = c.rank;
+            }
+            if (feat)
+                for (int k = 0; k < 6; ++k) feat[6 * q + k] = c.feat[k];
+        }
+    }, 1024);
     BFG_API_END
 }
 

@@ -661,6 +661,11 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     pack_tables(bf);
     auto t1 = clk::now();
     bf.t_factor_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    bf.t_plan_ms = 1e3 * (t_plan + t_emit);
+    bf.t_sample_ms = 1e3 * E.t_host;
+    bf.t_qr_ms = 1e3 * E.t_qr;
+    bf.t_interp_ms = 1e3 * E.t_interp;
+    bf.t_pack_ms = 1e3 * secs(tpk, t1);
     if (std::getenv("BFGPU_TIMING"))
         std::fprintf(stderr,
                      "[bfgpu] factorize %.3fs: plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "

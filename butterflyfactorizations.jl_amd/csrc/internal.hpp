@@ -245,6 +245,7 @@ struct BF {
     int64_t n_comp = 0, n_retry = 0;
     int64_t n_blocks = 0, n_data = 0;  // all blocks, factor entries (set by pack_tables)
     double t_factor_ms = 0.0;
+    double t_plan_ms = 0.0, t_sample_ms = 0.0, t_qr_ms = 0.0, t_interp_ms = 0.0, t_pack_ms = 0.0;
 };
 
 // Entries allocated past the end of every factor buffer: the dense kernel

@@ -147,6 +147,10 @@ typedef struct {
     int64_t n_comp;        /* compressions performed */
     int64_t n_retry;       /* adaptive doublings performed */
     double t_factor_ms;    /* wall time of the factorization call */
+    /* its parts: host planning and block emission, row-sample preparation,
+     * kernel fill + pivoted QR (GPU, synchronised), interpolation matrices
+     * (GPU), launch-table packing */
+    double t_plan_ms, t_sample_ms, t_qr_ms, t_interp_ms, t_pack_ms;
 } bfgpu_bf_info;
 int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
 
@@ -166,6 +170,12 @@ int bfgpu_bf_export_logs(const bfgpu_bf *bf, int32_t index, int32_t *skel_rec,
  * wavelengths, |cos| of the separation angle to z, level, log10(1/eps).
  * These are the inputs of the learned rank predictor. */
 int bfgpu_bf_export_features(const bfgpu_bf *bf, int32_t index, float *feat);
+/* Every compression record of every factorization, in factorization order
+ * (the rank logs of compressors.jl:52-69 over a whole operator): 7 int32
+ * per record (factorization index, stage, snode, onode, guess, rows, rank)
+ * and its 6 features (as bfgpu_bf_export_features).  With rec and feat
+ * NULL only the count is returned. */
+int bfgpu_bf_compressions(const bfgpu_bf *bf, int64_t *n, int32_t *rec, float *feat);
 /* Per-stage statistics of the packed tables: blocks, identity blocks, max
  * rank, factor bytes (show, src/showfcts.jl:81-124). 4 int64 per stage. */
 int bfgpu_bf_stage_stats(const bfgpu_bf *bf, int32_t *n_stages, int64_t *stats);

@@ -107,3 +107,24 @@ def test_tree_parameters_and_estimator_defaults(bfgpu):
     assert bfgpu.tree_parameters(None, bfgpu.isFarFunctor(1.0)).Ctau == pytest.approx(2.9592)
     assert bfgpu.tree_parameters(None, bfgpu.CenterDistanceAdmissibility(2.0)).Ctau == pytest.approx(2.9352)
     assert bfgpu.ButterflyRankEstimator().Rmin == 10
+
+
+def test_ellipsoid_points_are_uniform_in_area():
+    """Area-uniform sampling on a prolate spheroid: the fraction of points in
+    the band |z| < c/2 matches the band's share of the surface area
+    (surface of revolution, numerical quadrature); points lie on the
+    surface.  A sampler without the area correction fails the band test."""
+    from scipy.integrate import quad
+    from conftest import ellipsoid_surface
+    a, c = 1.0, 3.0
+    p = ellipsoid_surface(200000, (a, a, c), seed=5)
+    assert np.allclose((p[:, 0] / a) ** 2 + (p[:, 1] / a) ** 2 + (p[:, 2] / c) ** 2, 1.0, atol=1e-9)
+    # surface of revolution rho(z) = a sqrt(1 - z^2/c^2); dA = 2 pi rho sqrt(1 + rho'^2) dz
+    dA = lambda z: 2 * np.pi * np.sqrt(a * a * (1 - z * z / (c * c)) + (a * a * z / (c * c)) ** 2)
+    band = quad(dA, -c / 2, c / 2)[0] / quad(dA, -c, c)[0]
+    frac = np.mean(np.abs(p[:, 2]) < c / 2)
+    assert abs(frac - band) < 0.005
+    # the same map without the rejection step puts too few points in the band
+    u = np.random.default_rng(5).standard_normal((200000, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    assert abs(np.mean(np.abs(u[:, 2] * c) < c / 2) - band) > 0.05
