@@ -661,6 +661,11 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         release_graph(bf);
         break;
     }
+    case BFGPU_OPT_RHS_LANES:
+        BFG_REQUIRE(value >= 0, "bf_set_option: rhs_lanes is a minimum number of right-hand sides (0: off)");
+        bf.rhs_lanes_min = value;
+        release_graph(bf);
+        break;
     case BFGPU_OPT_DENSE: {
         const TileShape sh = unpack_tile(value);
         BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
@@ -700,6 +705,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     }
     case BFGPU_OPT_TILE:
     case BFGPU_OPT_DENSE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
+    case BFGPU_OPT_RHS_LANES: *value = bf.rhs_lanes_min; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }
     BFG_API_END

@@ -127,6 +127,13 @@ struct Build {
     double eps;
     std::vector<std::vector<int32_t>> tS, tO;
     std::unordered_map<uint64_t, std::vector<int32_t>> K;
+    // release o's containers here (called from worker threads, so the
+    // many small frees of a large batch run in parallel)
+    void swap_out(Build &o) {
+        tS.swap(o.tS);
+        tO.swap(o.tO);
+        K.swap(o.K);
+    }
 };
 
 template <class T> void upload(DevBuf &d, const std::vector<T> &h, hipStream_t s) {
@@ -400,6 +407,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         bf.bfs[b].ns = x.ns;
         bf.bfs[b].st.resize(x.L + 1);
     });
+    const double t_setup = secs(t0, clk::now());
     int32_t Lmax = 1;
     for (const Build &x : B) Lmax = std::max(Lmax, x.L);
     bf.n_stage_launch = Lmax + 1;
@@ -656,7 +664,10 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     bf.n_retry = E.n_retry;
     bf.precision = o.precision;
     for (int64_t b = 0; b < nB; ++b) bf.bfs[b].stage_data_base.assign(bf.bfs[b].st.size(), 0);
+    auto tfree = clk::now();
+    parallel_for(nB, [&](int64_t b) { Build().swap_out(B[b]); });
     std::vector<Build>().swap(B);
+    const double t_free = secs(tfree, clk::now());
     auto tpk = clk::now();
     pack_tables(bf);
     auto t1 = clk::now();
@@ -668,9 +679,11 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     bf.t_pack_ms = 1e3 * secs(tpk, t1);
     if (std::getenv("BFGPU_TIMING"))
         std::fprintf(stderr,
-                     "[bfgpu] factorize %.3fs: plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "
-                     "interp %.3fs, emit %.3fs, pack %.3fs (%lld compressions, %lld retries, %d threads)\n",
-                     bf.t_factor_ms * 1e-3, t_plan, E.t_host, E.t_qr, E.t_interp, t_emit, secs(tpk, t1),
+                     "[bfgpu] factorize %.3fs: setup %.3fs, plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "
+                     "interp %.3fs, emit %.3fs, free %.3fs, pack %.3fs (%lld compressions, %lld retries, "
+                     "%d threads)\n",
+                     bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_host, E.t_qr, E.t_interp, t_emit, t_free,
+                     secs(tpk, t1),
                      (long long)E.n_comp, (long long)E.n_retry, host_threads());
 }
 

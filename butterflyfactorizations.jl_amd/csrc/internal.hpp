@@ -234,6 +234,7 @@ struct BF {
     SweepTables sweep;            // fused forward sweep
     bool use_sweep = false;       // measured slower than per-level launches; opt-in
     bool use_graph = false;       // replay per-level launches from a HIP graph
+    int32_t rhs_lanes_min = 32;   // forward matvecs with >= this many RHS: RHS across lanes (0: never)
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
     ~BF();
@@ -329,6 +330,11 @@ void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const voi
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
+void launch_rhs(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots, int32_t ns,
+                int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
+                const double *beta, bool fuse, int n_cu);
+void launch_gather_rhs(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm, int32_t lo,
+                       int32_t hi, void *dst, int32_t ns, int nc);
 void launch_gather(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm,
                    int32_t lo, int32_t hi, void *dst, int64_t ld_dst, int64_t nrhs);
 // Fused forward sweep: every stage in one cooperative launch, tasks ordered

@@ -963,6 +963,163 @@ __global__ __launch_bounds__(256) void dense_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-RHS forward kernel: right-hand sides across lanes.  Lane r < nc of
+// a wavefront is right-hand side r; a wavefront takes one task at a time
+// (persistent, grid stride; the task index is wave-uniform, so task and
+// contribution descriptors are scalar loads).  Skeleton vectors are
+// interleaved: entry e of right-hand side r at slots[e * ns + r], so one
+// input row of a block is one coalesced load across the lanes.  The block's
+// entries are loaded spread over the lanes (64 entries per coalesced load)
+// and broadcast to every lane with readlane as they are used; input rows
+// are loaded KC at a time so their latencies overlap.  Output rows are
+// accumulated in chunks of RM in registers; each factor entry feeds nc
+// complex FMAs.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ C<R> lane_bcast(C<R> v, int lane) {
+    if constexpr (sizeof(R) == 4) {
+        return C<R>{__builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.x), lane)),
+                    __builtin_bit_cast(R, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v.y), lane))};
+    } else {
+        const int64_t bx = __builtin_bit_cast(int64_t, v.x), by = __builtin_bit_cast(int64_t, v.y);
+        const int64_t rx = (int64_t(uint32_t(__builtin_amdgcn_readlane(int(bx), lane)))) |
+                           (int64_t(__builtin_amdgcn_readlane(int(bx >> 32), lane)) << 32);
+        const int64_t ry = (int64_t(uint32_t(__builtin_amdgcn_readlane(int(by), lane)))) |
+                           (int64_t(__builtin_amdgcn_readlane(int(by >> 32), lane)) << 32);
+        return C<R>{__builtin_bit_cast(R, rx), __builtin_bit_cast(R, ry)};
+    }
+}
+
+template <typename R, int OUTK, int RM>
+__global__ __launch_bounds__(256) void rhs_kernel(
+    const MvTaskC *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, C<R> *__restrict__ slots, int32_t ns, int64_t in_shift,
+    C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
+    int fuse_beta, int nc) {
+    constexpr int KC = 8;
+    const int r = threadIdx.x & 63;
+    const bool lane_ok = r < nc;
+    const int rr = lane_ok ? r : 0;  // a valid column of the slot rows for idle lanes
+    const int64_t n_waves = int64_t(gridDim.x) * 4;
+    const C<R> *in = slots + in_shift * ns;
+    // the wave index is uniform across the wavefront: say so, so that the
+    // descriptors it selects are scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    for (int64_t t = int64_t(blockIdx.x) * 4 + wave; t < n_tasks; t += n_waves) {
+        const MvTaskC tk = tasks[t];
+        const int out_len = tk.m;
+        for (int i0 = 0; i0 < out_len; i0 += RM) {
+            const int rows = min(RM, out_len - i0);
+            C<R> acc[RM];
+#pragma unroll
+            for (int i = 0; i < RM; ++i) acc[i] = C<R>{R(0), R(0)};
+            for (int q = tk.c_begin - 1; q < tk.c_end; ++q) {
+                const MvContrib c = q < tk.c_begin ? MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0} : contribs[q];
+                if (c.fofs < 0) {  // identity
+#pragma unroll
+                    for (int i = 0; i < RM; ++i)
+                        if (i < rows) acc[i] = cadd(acc[i], in[int64_t(c.in_ofs + i0 + i) * ns + rr]);
+                    continue;
+                }
+                // entries (i0 + i, k), i < rows, column-major with stride m:
+                // a window of 64 consecutive entries per lane-distributed load
+                const C<R> *a = F + c.fofs;
+                int64_t win = -1;
+                C<R> aw = C<R>{R(0), R(0)};
+                for (int k0 = 0; k0 < c.n; k0 += KC) {
+                    C<R> xv[KC];
+#pragma unroll
+                    for (int kk = 0; kk < KC; ++kk) {
+                        const int k = min(k0 + kk, c.n - 1);
+                        xv[kk] = in[int64_t(c.in_ofs + k) * ns + rr];
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < KC; ++kk) {
+                        const int k = k0 + kk;
+                        if (k >= c.n) break;
+#pragma unroll
+                        for (int i = 0; i < RM; ++i) {
+                            if (i >= rows) break;
+                            const int64_t e = int64_t(k) * c.m + i0 + i;
+                            if ((e >> 6) != win) {  // next 64-entry window of the block
+                                win = e >> 6;
+                                const int64_t ew = (win << 6) + r;
+                                aw = F[c.fofs + min<int64_t>(ew, int64_t(c.m) * c.n - 1)];
+                            }
+                            cfma(acc[i], lane_bcast(aw, int(e & 63)), xv[kk]);
+                        }
+                    }
+                }
+                (void)a;
+            }
+            if (lane_ok) {
+#pragma unroll
+                for (int i = 0; i < RM; ++i) {
+                    if (i >= rows) break;
+                    if (OUTK == OUT_SLOT) {
+                        slots[int64_t(tk.out_ofs + i0 + i) * ns + r] = acc[i];
+                    } else {
+                        C<R> *yp = y + int64_t(r) * ldy + perm_out[tk.out_ofs + i0 + i];
+                        const C<R> old = *yp;
+                        *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc[i]));
+                    }
+                }
+            }
+        }
+    }
+}
+
+// dst[v * ns + r] = x[r][perm[v]] for v in [lo, hi), r < nc: inputs of the
+// first level in tree order, right-hand sides interleaved.
+template <typename R>
+__global__ void gather_rhs_kernel(const C<R> *__restrict__ x, int64_t ldx, const int32_t *__restrict__ perm,
+                                  int32_t lo, int32_t n, C<R> *__restrict__ dst, int32_t ns, int nc) {
+    const int r = threadIdx.x & 63;
+    const int64_t v = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (v >= n || r >= nc) return;
+    dst[(lo + v) * ns + r] = x[int64_t(r) * ldx + perm[lo + v]];
+}
+
+void launch_rhs(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots, int32_t ns,
+                int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
+                const double *beta, bool fuse, int n_cu) {
+    if (t.n_items == 0) return;
+    BFG_REQUIRE(nc >= 1 && nc <= 64 && nc <= ns, "matvec: right-hand-side chunk of 1..64");
+    const int64_t wgs = (int64_t(t.n_items) + 3) / 4;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * 16;
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
+    auto T = t.tasks_c.as<const MvTaskC>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_RHS(RT, OUTK)                                                                                     \
+    hipLaunchKernelGGL((rhs_kernel<RT, OUTK, 8>), grid, dim3(256), 0, s, T, t.n_items, P, (const C<RT> *)factors, \
+                       (C<RT> *)slots, ns, in_shift, (C<RT> *)y, ldy, perm_out, C<RT>{RT(alpha[0]), RT(alpha[1])}, \
+                       C<RT>{RT(beta[0]), RT(beta[1])}, int(fuse), nc)
+    if (prec == BFGPU_C128) {
+        if (t.out_kind == OUT_SLOT) BFG_RHS(double, OUT_SLOT);
+        else BFG_RHS(double, OUT_SCATTER);
+    } else {
+        if (t.out_kind == OUT_SLOT) BFG_RHS(float, OUT_SLOT);
+        else BFG_RHS(float, OUT_SCATTER);
+    }
+#undef BFG_RHS
+    BFG_HIP(hipGetLastError());
+}
+
+void launch_gather_rhs(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm, int32_t lo,
+                       int32_t hi, void *dst, int32_t ns, int nc) {
+    const int32_t n = hi - lo;
+    if (n <= 0 || nc <= 0) return;
+    dim3 grid(unsigned((n + 3) / 4));
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(gather_rhs_kernel<double>, grid, dim3(256), 0, s, (const C<double> *)x, ldx, perm, lo, n,
+                           (C<double> *)dst, ns, nc);
+    else
+        hipLaunchKernelGGL(gather_rhs_kernel<float>, grid, dim3(256), 0, s, (const C<float> *)x, ldx, perm, lo, n,
+                           (C<float> *)dst, ns, nc);
+    BFG_HIP(hipGetLastError());
+}
+
 // dst[r][lo + v] = x[r][perm[lo + v]] for v < n: the inputs of an IN_GATHER
 // stage in tree order, so the tile kernel reads them contiguously.
 template <typename R>

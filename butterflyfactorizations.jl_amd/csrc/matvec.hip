@@ -11,6 +11,8 @@
 // with transposed (conjugated) block access instead of materialised copies.
 #include <algorithm>
 #include <array>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -182,6 +184,11 @@ void pack_tables(BF &bf) {
     const int32_t nL = bf.n_stage_launch, Lmax = nL - 1;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
     const int64_t nb = int64_t(bf.bfs.size());
+    using clk = std::chrono::steady_clock;
+    const bool timing = std::getenv("BFGPU_TIMING") != nullptr;
+    double t_group = 0, t_split = 0, t_finish = 0;
+    auto tp0 = clk::now();
+    auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     const BlockIndex ix = index_blocks(bf);
     // ---- slot offsets (per factorization, local), reference slot ids ----
     std::vector<int32_t> sl_out(size_t(ix.total), -1), sl_in(size_t(ix.total), -1);
@@ -543,12 +550,18 @@ void pack_tables(BF &bf) {
         F.out_kind = (L == Lmax) ? OUT_SCATTER : OUT_SLOT;
         T.in_kind = (L == Lmax) ? IN_GATHER : IN_SLOT;
         T.out_kind = (L == 0) ? OUT_SCATTER : OUT_SLOT;
+        auto ta = clk::now();
         group(L, true, F);
         group(L, false, T);
+        t_group += since(ta);
+        ta = clk::now();
         if (L == Lmax) part_used = std::max(part_used, split(F));
         if (L == 0) part_used = std::max(part_used, split(T));
+        t_split += since(ta);
+        ta = clk::now();
         finish(F, true);
         finish(T, false);
+        t_finish += since(ta);
         if (L == Lmax)
             for (const MvItem &it : F.h_items) covered_rows += it.out_len;
         if (L == 0)
@@ -559,11 +572,18 @@ void pack_tables(BF &bf) {
     bf.gather_ofs = bf.part_ofs + part_used;
     bf.slot_len = bf.gather_ofs + std::max(test.n_values, trial.n_values);
     BFG_REQUIRE(bf.slot_len < (int64_t(1) << 31), "pack: slot buffer exceeds 2^31 entries");
+    auto tv = clk::now();
     validate_tables(bf);
+    const double t_valid = since(tv);
     upload_tab(bf.perm_test, test.perm);
     upload_tab(bf.perm_trial, trial.perm);
     bf.sweep = SweepTables{};
     if (bf.use_sweep) build_sweep(bf);
+    if (timing)
+        std::fprintf(stderr, "[bfgpu] pack %.3fs: slots %.3fs, group %.3fs, split %.3fs, finish+upload %.3fs, "
+                             "validate %.3fs\n",
+                     since(tp0), since(tp0) - t_group - t_split - t_finish - t_valid, t_group, t_split, t_finish,
+                     t_valid);
 }
 
 // Fused-sweep tables: every forward task of every stage in stage order.
@@ -747,7 +767,11 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
                       int64_t ldy, const double *alpha, const double *beta, Timeline *tl) {
     hipStream_t s = bf.ctx->stream;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
-    size_t need = size_t(bf.slot_len) * size_t(nrhs) * es;
+    // slot buffer: one column per RHS, or (right-hand sides across lanes)
+    // up to 64 interleaved RHS per sweep
+    const bool lanes = op == BFGPU_OP_N && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min;
+    const size_t cols = lanes ? size_t((std::min<int64_t>(64, nrhs) + 15) / 16 * 16) : size_t(nrhs);
+    const size_t need = size_t(bf.slot_len) * cols * es;
     if (need > bf.slots_cap) {
         release_graph(bf);  // a captured graph holds the old slot buffer
         bf.slots.alloc(need);
@@ -806,6 +830,33 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         launch_sweep(s, bf.precision, sw, bf.factors, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pr, pt,
                      nrhs, alpha, b_eff, fuse, bf.ctx->n_cu);
         if (tl) tl->launched(s, {-1, 3, 0, 0, 0, nr_per_lane(nrhs)});
+        return;
+    }
+    if (fwd && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min) {
+        // right-hand sides across lanes, up to 64 per sweep, skeleton
+        // vectors interleaved (entry-major) in the slot buffer
+        const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+        for (int64_t c0 = 0; c0 < nrhs; c0 += 64) {
+            const int nc = int(std::min<int64_t>(64, nrhs - c0));
+            const int32_t ns = (nc + 15) / 16 * 16;
+            const char *xc = static_cast<const char *>(x) + size_t(c0) * size_t(ldx) * es;
+            char *yc = static_cast<char *>(y) + size_t(c0) * size_t(ldy) * es;
+            const StageTable &t0 = bf.fwd[0];
+            launch_gather_rhs(s, bf.precision, xc, ldx, pr, t0.in_lo, t0.in_hi,
+                              static_cast<char *>(bf.slots.p) + size_t(bf.gather_ofs) * ns * es, ns, nc);
+            if (tl) tl->launched(s, {0, 2, 4, 0, 0, nc});
+            for (int32_t L = 0; L < nL; ++L) {
+                const StageTable &t = bf.fwd[L];
+                const StageTable *pre = t.pre.get();
+                for (const StageTable *u : {pre, &t}) {
+                    if (!u) continue;
+                    launch_rhs(s, bf.precision, *u, bf.factors[L].p, bf.slots.p, ns,
+                               u->in_kind == IN_GATHER ? bf.gather_ofs : 0, yc, ldy, pt, nc, alpha, b_eff, fuse,
+                               bf.ctx->n_cu);
+                    if (tl) tl->launched(s, {L, u == &t ? 0 : 1, 4, u->out_kind, 0, nc});
+                }
+            }
+        }
         return;
     }
     for (int32_t i = 0; i < nL; ++i) {

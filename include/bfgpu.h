@@ -256,14 +256,19 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        variant 2
  *   BFGPU_OPT_DENSE      dense shape of every forward table, packed as for
  *                        BFGPU_OPT_TILE with rp_shift = log2(lanes per
- *                        task) and g_shift = 0; selects variant 3 */
+ *                        task) and g_shift = 0; selects variant 3
+ *   BFGPU_OPT_RHS_LANES  forward matvecs with at least this many right-hand
+ *                        sides (default 32; 0 = never) put the right-hand
+ *                        sides across lanes, 64 per sweep, with the factor
+ *                        entries read once per sweep */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
     BFGPU_OPT_GRAPH = 3,
     BFGPU_OPT_VARIANT = 4,
     BFGPU_OPT_TILE = 5,
-    BFGPU_OPT_DENSE = 6
+    BFGPU_OPT_DENSE = 6,
+    BFGPU_OPT_RHS_LANES = 7
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);
