@@ -589,7 +589,7 @@ extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) 
                     continue;
                 }
                 const TileShape sh = unpack_tile(p);
-                BFG_REQUIRE(v >= 0 && v <= 3 && !(q >= 2 && (v == 1 || v == 3)) && p >= 0 && p < (1 << 28) &&
+                BFG_REQUIRE(v >= 0 && v <= 3 && !(q >= 2 && v == 1) && p >= 0 && p < (1 << 28) &&
                                 tile_ok(sh) && !(v == 3 && sh.rp_shift > 6),
                             "bf_set_tuning: bad variant or shape");
                 if (pass == 1) {
@@ -633,8 +633,8 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         BFG_REQUIRE(value >= 0 && value <= 3, "bf_set_option: kernel variant 0..3");
         for (int dir = 0; dir < 2; ++dir)
             for (StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
-                const int32_t v = dir == 1 ? (value == 1 ? 0 : value == 3 ? 2 : value) : value;
-                if (value == 3 && dir == 0) {  // dense: as many lanes per task as the tiled shape, >= 16
+                const int32_t v = dir == 1 ? (value == 1 ? 0 : value) : value;
+                if (value == 3) {  // dense: as many lanes per task as the tiled shape, >= 16
                     for (StageTable *u : {&t, t.pre.get()}) {
                         if (!u) continue;
                         u->tile.rp_shift = int8_t(std::max<int>(4, u->tile.rp_shift + u->tile.g_shift));
@@ -670,12 +670,13 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         const TileShape sh = unpack_tile(value);
         BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
                     "bf_set_option: bad dense shape");
-        for (StageTable &t : bf.fwd)
-            for (StageTable *u : {&t, t.pre.get()})
-                if (u) {
-                    u->variant = 3;
-                    u->tile = sh;
-                }
+        for (int dir = 0; dir < 2; ++dir)
+            for (StageTable &t : dir == 0 ? bf.fwd : bf.tr)
+                for (StageTable *u : {&t, t.pre.get()})
+                    if (u) {
+                        u->variant = 3;
+                        u->tile = sh;
+                    }
         release_graph(bf);
         break;
     }
@@ -697,7 +698,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
         int32_t v = bf.fwd.empty() ? 0 : bf.fwd[0].variant;
         for (int dir = 0; dir < 2; ++dir)
             for (const StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
-                const int32_t w = dir == 1 ? (v == 1 ? 0 : v == 3 ? 2 : v) : v;
+                const int32_t w = dir == 1 ? (v == 1 ? 0 : v) : v;
                 if (t.variant != w || (t.pre && t.pre->variant != w)) v = -1;
             }
         *value = v;

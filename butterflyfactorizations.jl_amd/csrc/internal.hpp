@@ -188,9 +188,9 @@ struct StageTable {
     int64_t alg_bytes_1 = 0;           // algorithmic bytes per RHS (factor bytes separate)
     int64_t factor_elems = 0;          // factor entries read
     // kernel: 0 workgroup per task, 1 wavefront per task (forward only),
-    // 2 tiled (several tasks per wavefront, persistent), 3 dense (forward
-    // only: lanes packed to each block's row count; tile.rp_shift is the
-    // log2 of the lanes per task)
+    // 2 tiled (several tasks per wavefront, persistent), 3 dense (forward:
+    // lanes packed to each block's row count; transposed: a lane per output
+    // column; tile.rp_shift is the log2 of the lanes per task)
     int32_t variant = 0;
     TileShape tile;
     int32_t in_lo = 0, in_hi = 0;      // IN_GATHER: value range read through the permutation
@@ -330,6 +330,9 @@ void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const voi
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
+void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
+                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
 void launch_rhs(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots, int32_t ns,
                 int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
                 const double *beta, bool fuse, int n_cu);

@@ -964,6 +964,134 @@ __global__ __launch_bounds__(256) void dense_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Dense transposed kernel (adjoint / transpose): out[j] = sum_i op(A[i, j])
+// in[i].  Lane l of a task owns output column j = j0 + l (LPT outputs per
+// pass), so its accumulator persists over all contributions and no
+// cross-lane reduction is needed; the lane walks its block column, which
+// is contiguous, with 16-byte loads of two entries.  The inputs of an inner
+// chunk of LPT entries are loaded once, one lane per entry, and handed to
+// every lane of the task by cross-lane permutes.  Loads carry no predicate
+// (column clamped into the block; the entry after the last row of a column
+// reads the next column or the buffer's tail pad and meets a zeroed input).
+// ---------------------------------------------------------------------------
+template <typename R, int OUTK, int NR>
+__global__ __launch_bounds__(256) void denset_kernel(
+    const MvTaskC *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
+    const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
+    C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
+    int fuse_beta, int nrhs, int lpt_shift, int xcd_map, int conj) {
+    constexpr int UP = 4;  // pairs per unrolled step (8 inner entries)
+    const int lane = threadIdx.x & 63;
+    const int LPT = 1 << lpt_shift;
+    const int tpw = 64 >> lpt_shift;
+    const int sub = lane >> lpt_shift;
+    const int l = lane & (LPT - 1);
+    const int base = sub << lpt_shift;
+    const int rb = blockIdx.y * NR;
+    const int nr = min(NR, nrhs - rb);
+    const R csign = conj ? R(-1) : R(1);
+    const C<R> *in = slots + in_shift + int64_t(rb) * slot_len;
+    const MvContrib none{-1, 0, 0, 0, 0};
+    const int64_t n_groups = (int64_t(n_tasks) + tpw - 1) / tpw;
+    int64_t grp, grp_hi, n_waves;
+    if (xcd_map && gridDim.x >= 8) {  // see tile_kernel
+        const int64_t nwg = gridDim.x, x = blockIdx.x % 8;
+        n_waves = ((nwg - x + 7) / 8) * 4;
+        grp = n_groups * x / 8 + (int64_t(blockIdx.x) / 8) * 4 + (threadIdx.x >> 6);
+        grp_hi = n_groups * (x + 1) / 8;
+    } else {
+        n_waves = int64_t(gridDim.x) * 4;
+        grp = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+        grp_hi = n_groups;
+    }
+    auto contrib = [&](const TaskView &tk, int q) -> MvContrib {
+        if (q == 0) return MvContrib{tk.fofs, tk.m, tk.n, tk.in_ofs, 0};
+        const int ci = tk.c_begin + q - 1;
+        return ci < tk.c_end ? contribs[ci] : none;
+    };
+    struct Pair { C<R> a, b; };
+
+    uint2 tv = load_task_group(tasks, n_tasks, grp, tpw, lane);
+    while (grp < grp_hi) {
+        const int64_t ngrp = grp + n_waves;
+        const uint2 nv = ngrp < grp_hi ? load_task_group(tasks, n_tasks, ngrp, tpw, lane) : make_uint2(0u, 0u);
+        const TaskView tk = unpack_task(tv, sub);
+        const int out_len = tk.n;
+        const int nc = out_len > 0 ? 1 + tk.c_end - tk.c_begin : 0;
+        for (int j0 = 0; j0 < out_len; j0 += LPT) {
+            const int j = j0 + l;
+            const bool jok = j < out_len;
+            const int jc = min(j, out_len - 1);
+            C<R> acc[NR];
+#pragma unroll
+            for (int q = 0; q < NR; ++q) acc[q] = C<R>{R(0), R(0)};
+            MvContrib cur = contrib(tk, 0);
+            for (int q = 0; q < nc; ++q) {
+                const MvContrib nx = q + 1 < nc ? contrib(tk, q + 1) : none;
+                if (cur.fofs < 0) {  // identity: out[j] += in[j]
+                    if (cur.m > 0 && jok)
+#pragma unroll
+                        for (int rr = 0; rr < NR; ++rr)
+                            if (rr < nr) acc[rr] = cadd(acc[rr], in[int64_t(rr) * slot_len + cur.in_ofs + j]);
+                } else {
+                    const C<R> *col = F + cur.fofs + int64_t(jc) * cur.m;
+                    const C<R> *px = in + cur.in_ofs;
+                    for (int i0 = 0; i0 < cur.m; i0 += LPT) {
+                        const int iw = min(LPT, cur.m - i0);
+                        C<R> xs[NR];
+#pragma unroll
+                        for (int rr = 0; rr < NR; ++rr) {
+                            const C<R> v = px[int64_t(rr < nr ? rr : nr - 1) * slot_len + i0 + min(l, iw - 1)];
+                            xs[rr] = C<R>{l < iw ? v.x : R(0), l < iw ? v.y : R(0)};
+                        }
+                        for (int ii = 0; ii < iw; ii += 2 * UP) {
+                            Pair p[UP];
+#pragma unroll
+                            for (int u = 0; u < UP; ++u)
+                                __builtin_memcpy(&p[u],
+                                                 __builtin_assume_aligned(col + i0 + min(ii + 2 * u, iw - 1),
+                                                                          sizeof(C<R>)),
+                                                 sizeof(Pair));
+#pragma unroll
+                            for (int u = 0; u < UP; ++u) {
+                                const int e0 = ii + 2 * u, e1 = e0 + 1;
+                                const int s0 = base + min(e0, iw - 1), s1 = base + min(e1, iw - 1);
+#pragma unroll
+                                for (int rr = 0; rr < NR; ++rr) {
+                                    const R x0r = __shfl(xs[rr].x, s0, 64), x0i = __shfl(xs[rr].y, s0, 64);
+                                    const R x1r = __shfl(xs[rr].x, s1, 64), x1i = __shfl(xs[rr].y, s1, 64);
+                                    const bool ok0 = e0 < iw, ok1 = e1 < iw;
+                                    const C<R> x0{ok0 ? x0r : R(0), ok0 ? x0i : R(0)};
+                                    const C<R> x1{ok1 ? x1r : R(0), ok1 ? x1i : R(0)};
+                                    cfma(acc[rr], C<R>{p[u].a.x, p[u].a.y * csign}, x0);
+                                    cfma(acc[rr], C<R>{p[u].b.x, p[u].b.y * csign}, x1);
+                                }
+                            }
+                        }
+                    }
+                }
+                cur = nx;
+            }
+            if (jok) {
+#pragma unroll
+                for (int rr = 0; rr < NR; ++rr) {
+                    if (rr >= nr) break;
+                    if (OUTK == OUT_SLOT) {
+                        slots[int64_t(rb + rr) * slot_len + tk.out_ofs + j] = acc[rr];
+                    } else {
+                        C<R> *yp = y + int64_t(rb + rr) * ldy + perm_out[tk.out_ofs + j];
+                        const C<R> old = *yp;
+                        *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, acc[rr]));
+                    }
+                }
+            }
+        }
+        tv = nv;
+        grp = ngrp;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Multi-RHS forward kernel: right-hand sides across lanes.  Lane r < nc of
 // a wavefront is right-hand side r; a wavefront takes one task at a time
 // (persistent, grid stride; the task index is wave-uniform, so task and
@@ -1220,6 +1348,51 @@ static void launch_dense_nr(hipStream_t s, const StageTable &t, const void *fact
     } else {
         launch_dense_t<R, 8, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
     }
+}
+
+template <typename R, int NR>
+static void launch_denset_t(hipStream_t s, const StageTable &t, const void *factors, void *slots,
+                            int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                            int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu, int conj) {
+    C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
+    const int lpt = t.tile.rp_shift;
+    BFG_REQUIRE(tile_ok(t.tile) && t.tile.g_shift == 0, "matvec: bad dense shape");
+    const int64_t tpw = 64 >> lpt;
+    const int64_t wgs = ((int64_t(t.n_items) + tpw - 1) / tpw + 3) / 4;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))), unsigned((nrhs + NR - 1) / NR));
+    auto T = t.tasks_c.as<const MvTaskC>();
+    auto P = t.contribs.as<const MvContrib>();
+#define BFG_DENSET(OUTK)                                                                                    \
+    hipLaunchKernelGGL((denset_kernel<R, OUTK, NR>), grid, dim3(256), 0, s, T, t.n_items, P, (const C<R> *)factors, \
+                       (C<R> *)slots, slot_len, in_shift, (C<R> *)y, ldy, perm_out, a, b, int(fuse), int(nrhs), lpt,  \
+                       int(t.tile.xcd), conj)
+    if (t.out_kind == OUT_SLOT) BFG_DENSET(OUT_SLOT);
+    else BFG_DENSET(OUT_SCATTER);
+#undef BFG_DENSET
+    BFG_HIP(hipGetLastError());
+}
+
+void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
+                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
+                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu) {
+    if (t.n_items == 0 || nrhs == 0) return;
+    BFG_REQUIRE(nrhs < 65536 * 8, "matvec: nrhs too large");
+    const int conj = op == BFGPU_OP_C ? 1 : 0;
+#define BFG_DT(RT, NRv) \
+    launch_denset_t<RT, NRv>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu, conj)
+    if (prec == BFGPU_C128) {
+        if (nrhs == 1) BFG_DT(double, 1);
+        else if (nrhs == 2) BFG_DT(double, 2);
+        else if (nrhs <= 4) BFG_DT(double, 4);
+        else BFG_DT(double, 8);
+    } else {
+        if (nrhs == 1) BFG_DT(float, 1);
+        else if (nrhs == 2) BFG_DT(float, 2);
+        else if (nrhs <= 4) BFG_DT(float, 4);
+        else BFG_DT(float, 8);
+    }
+#undef BFG_DT
 }
 
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,

@@ -748,10 +748,12 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
             shift = bf.gather_ofs;
             if (tl) tl->launched(s, {L, 2, 0, 0, 0, 1});
         }
-        if (t.variant == 3) {
-            BFG_REQUIRE(fwd, "matvec: the dense kernel is forward only");
+        if (t.variant == 3 && fwd) {
             launch_dense(s, bf.precision, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout, nrhs,
                          alpha, beta, fuse, bf.ctx->n_cu);
+        } else if (t.variant == 3) {
+            launch_denset(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
+                          nrhs, alpha, beta, fuse, bf.ctx->n_cu);
         } else {
             launch_tile(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
                         nrhs, alpha, beta, fuse, bf.ctx->n_cu);
@@ -1058,6 +1060,15 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                     sh.csplit = int8_t(cs);
                     consider(2, sh);
                 }
+        if (op != BFGPU_OP_N)  // transposed dense: a lane per output column
+            for (int lp = 2; lp <= 6; ++lp) {
+                TileShape sh = t0;
+                sh.rp_shift = int8_t(lp);
+                sh.g_shift = 0;
+                sh.csplit = 0;
+                sh.qw4 = 0;
+                consider(3, sh);
+            }
         if (op == BFGPU_OP_N)
             for (int lp = 2; lp <= 6; ++lp)
                 for (int cs = 0; cs <= (t.max_nc >= 2 ? 1 : 0); ++cs)

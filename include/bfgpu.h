@@ -241,9 +241,9 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *   BFGPU_OPT_VARIANT    kernel of every launch table: 0 workgroup per task,
  *                        1 wavefront per task (forward; transposed tables
  *                        take 0), 2 tiled (several tasks per wavefront),
- *                        3 dense (forward; lanes packed to each block's row
- *                        count; transposed tables take 2); get: the common
- *                        value, or -1 when levels differ
+ *                        3 dense (forward: lanes packed to each block's row
+ *                        count; transposed: a lane per output column); get:
+ *                        the common value, or -1 when levels differ
  *   BFGPU_OPT_TILE       tiled shape of every table, packed as rp_shift |
  *                        g_shift << 4 | csplit << 8 | wg_per_cu << 12 |
  *                        xcd << 20 (2^rp_shift outputs x 2^g_shift lane
@@ -254,9 +254,11 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        24: 4 instead of 8 loads per lane and pass for
  *                        the dense kernel at one RHS); also selects
  *                        variant 2
- *   BFGPU_OPT_DENSE      dense shape of every forward table, packed as for
+ *   BFGPU_OPT_DENSE      dense shape of every table, packed as for
  *                        BFGPU_OPT_TILE with rp_shift = log2(lanes per
- *                        task) and g_shift = 0; selects variant 3
+ *                        task) and g_shift = 0 (the transposed tables use
+ *                        only the lanes per task and the XCD mapping);
+ *                        selects variant 3
  *   BFGPU_OPT_RHS_LANES  forward matvecs with at least this many right-hand
  *                        sides (default 32; 0 = never) put the right-hand
  *                        sides across lanes, 64 per sweep, with the factor

@@ -2,7 +2,8 @@
 
 Every launch table can run on the workgroup-per-task kernel (variant 0), the
 wavefront-per-task kernel (variant 1, forward), the tiled kernel (variant 2)
-or the dense kernel (variant 3, forward) in any legal shape; observer-leaf sums over many
+or the dense kernel (variant 3: forward packed rows, transposed a lane per
+output column) in any legal shape; observer-leaf sums over many
 factorizations are split into partial sums plus a combine pass.  Bar
 (written in each test): every variant and shape gives the oracle's FP64
 matvec of the same factorization to 1e-12 relative, in all three
@@ -107,7 +108,7 @@ def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
         say("dense", hex(sh))
         g.set_option("dense", sh)
         assert g.get_option("dense") == sh
-        _check_all_directions(g, {"N": refs["N"]}, X)
+        _check_all_directions(g, refs, X)
     # the autotuner's choice (both directions) keeps the product
     ms, var = g.autotune(nrhs=1, iters=2)
     assert len(ms) == len(var) == len(g)
@@ -156,7 +157,7 @@ def test_variants_sphere_partial_sums_match_oracle(bfgpu, oracle, ctx):
     for sh in _dense_shapes(bfgpu)[::2]:
         say("dense", hex(sh))
         op.set_option("dense", sh)
-        _check_all_directions(op, {"N": refs["N"]}, X)
+        _check_all_directions(op, refs, X)
     op.autotune(nrhs=1, iters=2)
     _check_all_directions(op, refs, X)
 
