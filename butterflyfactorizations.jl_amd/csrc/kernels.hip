@@ -279,9 +279,169 @@ __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__rest
     if (tid == 0) rank_out[blockIdx.x] = rank;
 }
 
+// ---------------------------------------------------------------------------
+// The same pivoted QR for blocks of at most 64 columns that fit in LDS: one
+// wavefront per block, `wpb` blocks per workgroup, the block copied into the
+// wavefront's LDS region and factored there, then written back.  Lane j owns
+// column j (partial norms in FP64 registers, trailing update and downdate of
+// its column); row sums are wave reductions.  No workgroup barriers.  The
+// steps, the pivot rule (first maximal partial norm), the reflector, the
+// downdate test and the stopping rule are those of cpqr_kernel.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void cpqr_wave_kernel(C<R> *Z, const QrTask *__restrict__ tasks, int32_t n_tasks,
+                                                        int32_t *piv_all, int32_t *rank_out, double rtol,
+                                                        int32_t region) {
+    extern __shared__ char smraw[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t task = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave;
+    if (task >= n_tasks) return;  // whole wavefront; no barriers below
+    const QrTask tk = tasks[task];
+    const int m = tk.m, n = tk.n;
+    C<R> *G = Z + tk.zofs;
+    // LDS copy with an odd column stride (lanes walking a row of the block
+    // then spread over the banks instead of hitting one)
+    const int ld = m | 1;
+    C<R> *A = reinterpret_cast<C<R> *>(smraw) + int64_t(wave) * region;
+    for (int e = lane; e < m * n; e += 64) {
+        const int j = e / m;
+        A[j * ld + (e - j * m)] = G[e];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const bool colok = lane < n;
+    C<R> *cj = A + int64_t(colok ? lane : 0) * ld;  // this lane's column
+    double vn1 = 0.0, vn2 = 0.0;
+    if (colok) {
+        double s = 0.0;
+        for (int i = 0; i < m; ++i) s += double(cj[i].x) * cj[i].x + double(cj[i].y) * cj[i].y;
+        vn1 = vn2 = sqrt(s);
+    }
+    int pv = lane;
+    const int mn = m < n ? m : n;
+    int rank = mn;
+    double r11 = 0.0;
+    const double tol3z = sizeof(R) == 8 ? 1.0536712127723509e-08 : 2.44140625e-04;
+    auto sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    for (int k = 0; k < mn; ++k) {
+        // pivot: first maximal partial norm over columns [k, n)
+        double bv = (colok && lane >= k) ? vn1 : -1.0;
+        int bi = lane;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(bv, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        const int p = __shfl(bi, 0, 64);
+        // exact remaining norm of the pivot column
+        const C<R> *ap = A + int64_t(p) * ld;
+        double s = 0.0;
+        for (int i = k + lane; i < m; i += 64) s += double(ap[i].x) * ap[i].x + double(ap[i].y) * ap[i].y;
+        const double pn = sqrt(wave_sum(s));
+        if (k == 0) r11 = pn;
+        if (pn == 0.0 || (k > 0 && pn <= rtol * r11)) { rank = k; break; }
+        if (p != k) {
+            sync();  // the norm's reads of column p precede the swap's writes
+            C<R> *ak = A + int64_t(k) * ld;
+            C<R> *aq = A + int64_t(p) * ld;
+            for (int i = lane; i < m; i += 64) {
+                const C<R> t = aq[i];
+                aq[i] = ak[i];
+                ak[i] = t;
+            }
+            const int pk = __shfl(pv, k, 64), pp = __shfl(pv, p, 64);
+            const double v1k = __shfl(vn1, k, 64), v1p = __shfl(vn1, p, 64);
+            const double v2k = __shfl(vn2, k, 64), v2p = __shfl(vn2, p, 64);
+            if (lane == p) { pv = pk; vn1 = v1k; vn2 = v2k; }
+            if (lane == k) { pv = pp; vn1 = v1p; vn2 = v2p; }
+            sync();
+        }
+        // reflector from column k (scalars in FP64, every lane the same)
+        C<R> *a = A + int64_t(k) * ld;
+        double xs = 0.0;
+        for (int i = k + 1 + lane; i < m; i += 64) xs += double(a[i].x) * a[i].x + double(a[i].y) * a[i].y;
+        const double xnorm = sqrt(wave_sum(xs));
+        const double ar = a[k].x, ai = a[k].y;
+        double beta, tx, ty, sx, sy;
+        if (xnorm == 0.0 && ai == 0.0) {
+            beta = ar; tx = 0.0; ty = 0.0; sx = 1.0; sy = 0.0;
+        } else {
+            beta = hypot(hypot(ar, ai), xnorm);
+            if (ar >= 0.0) beta = -beta;
+            tx = (beta - ar) / beta;
+            ty = -ai / beta;
+            const double dr = ar - beta, di = ai, den = dr * dr + di * di;
+            sx = dr / den;
+            sy = -di / den;
+        }
+        const bool apply = !(tx == 0.0 && ty == 0.0);
+        const C<R> scal = {R(sx), R(sy)};
+        const C<R> ctau = {R(tx), R(-ty)};  // conj(tau)
+        sync();  // every lane has read a[k] before it is overwritten
+        if (apply)
+            for (int i = k + 1 + lane; i < m; i += 64) a[i] = cmul(a[i], scal);
+        if (lane == 0) a[k] = C<R>{R(beta), R(0)};
+        sync();
+        // trailing update and norm downdate of this lane's column
+        if (colok && lane > k) {
+            if (apply) {
+                C<R> w = cj[k];  // v_k = 1
+                for (int i = k + 1; i < m; ++i) cfmac(w, a[i], cj[i]);
+                w = cmul(ctau, w);
+                cj[k] = csub(cj[k], w);
+                for (int i = k + 1; i < m; ++i) cj[i] = csub(cj[i], cmul(a[i], w));
+            }
+            if (vn1 != 0.0) {
+                const C<R> ck = cj[k];
+                double t = sqrt(double(ck.x) * ck.x + double(ck.y) * ck.y) / vn1;
+                t = 1.0 - t * t;
+                if (t < 0.0) t = 0.0;
+                const double rr = vn1 / vn2;
+                if (t * rr * rr <= tol3z) {
+                    double s2 = 0.0;
+                    for (int i = k + 1; i < m; ++i) s2 += double(cj[i].x) * cj[i].x + double(cj[i].y) * cj[i].y;
+                    vn1 = vn2 = sqrt(s2);
+                } else {
+                    vn1 = vn1 * sqrt(t);
+                }
+            }
+        }
+        sync();
+    }
+    for (int e = lane; e < m * n; e += 64) {
+        const int j = e / m;
+        G[e] = A[j * ld + (e - j * m)];
+    }
+    if (colok) piv_all[tk.pofs + lane] = pv;
+    if (lane == 0) rank_out[task] = rank;
+}
+
 void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
-                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t) {
+                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m) {
     if (n_tasks <= 0) return;
+    const size_t es = prec == BFGPU_C128 ? 16 : 8;
+    const size_t region = size_t(std::max(max_m, 1) | 1) * size_t(std::max(max_n, 1));  // entries per block
+    const size_t per_wave = region * es;
+    if (max_n <= 64 && per_wave <= 32 * 1024) {
+        // blocks per workgroup: as many as 64 KiB of LDS hold, at most 4
+        const int wpb = int(std::min<size_t>(4, (64 * 1024) / per_wave));
+        const size_t lds = per_wave * size_t(wpb);
+        dim3 grid(unsigned((int64_t(n_tasks) + wpb - 1) / wpb));
+        if (prec == BFGPU_C128)
+            hipLaunchKernelGGL(cpqr_wave_kernel<double>, grid, dim3(64 * wpb), lds, s, (C<double> *)Z, tasks,
+                               n_tasks, piv, rank, rtol, int32_t(region));
+        else
+            hipLaunchKernelGGL(cpqr_wave_kernel<float>, grid, dim3(64 * wpb), lds, s, (C<float> *)Z, tasks,
+                               n_tasks, piv, rank, rtol, int32_t(region));
+        BFG_HIP(hipGetLastError());
+        return;
+    }
     size_t lds = sizeof(double) * 2 * size_t(max_n > 0 ? max_n : 1);
     BFG_REQUIRE(lds <= 64 * 1024, "cpqr: too many columns for the norm arrays");
     if (prec == BFGPU_C128)
