@@ -94,10 +94,14 @@ def kernel_name(d, op, real):
     (level, table, variant, output kind, packed launch shape, RHS per lane)."""
     level, table, variant, outk, cs, nr = (int(v) for v in d)
     if table == 2:
-        return f"gather_kernel<{real}>"
+        return f"gather_rhs_kernel<{real}>" if variant == 4 else f"gather_kernel<{real}>"
     if table == 3:
         return f"sweep_kernel<{real}, {nr}>"
     ink = 0 if (table == 1 or level > 0) else 1
+    if variant == 5:
+        return f"mfma_rhs_kernel<{outk}, {(nr + 15) // 16}>"
+    if variant == 4:
+        return f"rhs_kernel<{real}, {outk}, 8>"
     if variant == 3:
         qw = (4 if (cs >> 24) & 1 else 8) if nr == 1 else (4 if nr == 2 else 2)
         return f"dense_kernel<{real}, {outk}, {nr}, {qw}, {'true' if (cs >> 8) & 1 else 'false'}>"

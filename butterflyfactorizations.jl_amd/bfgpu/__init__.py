@@ -491,7 +491,8 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_set_fused(self.handle, 1 if enable else 0))
         return self
 
-    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7}
+    OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7,
+               "mfma": 8}
 
     @staticmethod
     def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0):
@@ -505,7 +506,9 @@ class ButterflyFactorization:
         cooperative), wg_per_cu (fused sweep occupancy), graph (replay
         per-level launches from a HIP graph), variant (0 workgroup per
         task, 1 wavefront per task, 2 tiled, 3 dense), tile (tiled shape,
-        see tile_shape), dense (dense shape of the forward tables)."""
+        see tile_shape), dense (dense shape of the forward tables),
+        rhs_lanes (fewest right-hand sides put across lanes), mfma (1:
+        FP32 right-hand sides across lanes on the matrix cores)."""
         check(lib().bfgpu_bf_set_option(self.handle, self.OPTIONS[key], int(value)))
         return self
 

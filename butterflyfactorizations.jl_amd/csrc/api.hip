@@ -666,6 +666,11 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         bf.rhs_lanes_min = value;
         release_graph(bf);
         break;
+    case BFGPU_OPT_MFMA:
+        BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: mfma is 0 or 1");
+        bf.use_mfma = value == 1;
+        release_graph(bf);
+        break;
     case BFGPU_OPT_DENSE: {
         const TileShape sh = unpack_tile(value);
         BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
@@ -707,6 +712,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_TILE:
     case BFGPU_OPT_DENSE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
     case BFGPU_OPT_RHS_LANES: *value = bf.rhs_lanes_min; break;
+    case BFGPU_OPT_MFMA: *value = bf.use_mfma ? 1 : 0; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }
     BFG_API_END

@@ -202,6 +202,12 @@ struct StageTable {
     // items split into partial sums: `pre` writes the partials into slots,
     // this table's combine items add them (identity contributions)
     std::unique_ptr<StageTable> pre;
+    // matrix-core tiles of the multi-RHS kernel (built on first use):
+    // int4 {first, count, row0, rows} over mtasks, the tasks in stacked
+    // order
+    bool mfma_built = false;
+    int32_t n_mtiles = 0;
+    DevBuf mtiles, mtasks;
 };
 
 constexpr int SWEEP_MAX_STAGES = 48;
@@ -235,6 +241,7 @@ struct BF {
     bool use_sweep = false;       // measured slower than per-level launches; opt-in
     bool use_graph = false;       // replay per-level launches from a HIP graph
     int32_t rhs_lanes_min = 32;   // forward matvecs with >= this many RHS: RHS across lanes (0: never)
+    bool use_mfma = true;         // RHS across lanes in complex FP32: matrix-core kernel
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
     ~BF();
@@ -336,6 +343,11 @@ void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const v
 void launch_rhs(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots, int32_t ns,
                 int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
                 const double *beta, bool fuse, int n_cu);
+void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, void *slots, int32_t ns,
+                     int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
+                     const double *beta, bool fuse, int n_cu);
+// Matrix-core tiles of a forward table (matvec.hip).
+void build_mfma(StageTable &t);
 void launch_gather_rhs(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm, int32_t lo,
                        int32_t hi, void *dst, int32_t ns, int nc);
 void launch_gather(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm,

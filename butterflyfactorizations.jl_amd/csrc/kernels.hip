@@ -1358,6 +1358,191 @@ __global__ __launch_bounds__(256) void rhs_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-RHS forward kernel on the matrix cores (complex FP32).  Skeleton
+// vectors are interleaved as for rhs_kernel (entry e of right-hand side r at
+// slots[e * ns + r], ns = 16 NT).  One wavefront per tile of at most 16
+// output rows: either rows [row0, row0 + rows) of one item, summed over its
+// contributions, or whole items stacked on top of each other that read the
+// same input entries (one contribution each, same in_ofs and n), so small
+// blocks fill the 16 rows of the product.  Per step of 4 input entries the
+// lanes load A[i][k] (lane i + 16 k: row i of the tile, entry k) and, per
+// group of 16 right-hand sides, B[k][j] (lane j + 16 k), and issue
+// v_mfma_f32_16x16x4_f32 four times per group (re*re, -im*im into the real
+// part, re*im, im*re into the imaginary part); lane j + 16 q then holds
+// rows 4q..4q+3 of column j.  Padding rows and entries load zeros; lanes of
+// right-hand sides >= nc compute on whatever the slot buffer holds there and
+// are never stored (a column of the product depends on its own column of
+// B only).
+// ---------------------------------------------------------------------------
+typedef float bfg_v4f __attribute__((ext_vector_type(4)));
+
+template <int OUTK, int NT>
+__global__ __launch_bounds__(256) void mfma_rhs_kernel(
+    const int4 *__restrict__ tiles, int32_t n_tiles, const MvTaskC *__restrict__ mtasks,
+    const MvContrib *__restrict__ contribs, const C<float> *__restrict__ F, C<float> *__restrict__ slots,
+    int32_t ns, int64_t in_shift, C<float> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out,
+    C<float> alpha, C<float> beta, int fuse_beta, int nc) {
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int64_t n_waves = int64_t(gridDim.x) * 4;
+    const C<float> *in = slots + in_shift * ns;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    int64_t t = int64_t(blockIdx.x) * 4 + wave;
+    int4 tl_next = t < n_tiles ? tiles[t] : int4{0, 0, 0, 0};
+    for (; t < n_tiles; t += n_waves) {
+        const int4 tl = tl_next;  // first stacked task, task count, first row, rows
+        if (t + n_waves < n_tiles) tl_next = tiles[t + n_waves];
+        // the tile's tasks, one per lane (at most 16)
+        int64_t qf = 0;
+        int32_t qm = 0, qo = 0;
+        if (l < tl.y) {
+            qf = mtasks[tl.x + l].fofs;
+            qm = mtasks[tl.x + l].m;
+            qo = mtasks[tl.x + l].out_ofs;
+        }
+        const MvTaskC tk0 = mtasks[tl.x];
+        // row li of the tile: its block and output entry
+        int64_t rbase = 0;
+        int32_t rld = 1, rout = tk0.out_ofs + tl.z + li;
+        bool rok = li < tl.w;
+        if (tl.y > 1) {
+            rok = false;
+            int start = 0;
+            for (int q = 0; q < tl.y; ++q) {
+                const int32_t mq = __builtin_amdgcn_readlane(qm, q);
+                if (li >= start && li < start + mq) {
+                    rok = true;
+                    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint64_t(qf)), q));
+                    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint64_t(qf) >> 32), q));
+                    rbase = int64_t((uint64_t(hi) << 32) | lo) + (li - start);
+                    rld = mq;
+                    rout = __builtin_amdgcn_readlane(qo, q) + (li - start);
+                }
+                start += mq;
+            }
+        }
+        bfg_v4f accr[NT], acci[NT];
+#pragma unroll
+        for (int g = 0; g < NT; ++g) {
+            accr[g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+            acci[g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+        }
+        // contributions; the first is inline in the task, the next one's
+        // descriptor is loaded while the current one is multiplied
+        int64_t cf = tk0.fofs;
+        int32_t cm = tk0.m, cn = tk0.n, ci = tk0.in_ofs;
+        for (int q = tk0.c_begin - 1; q < tk0.c_end; ++q) {
+            int64_t nf = 0;
+            int32_t nm = 0, nn = 0, ni = 0;
+            if (q + 1 < tk0.c_end) {
+                nf = contribs[q + 1].fofs;
+                nm = contribs[q + 1].m;
+                nn = contribs[q + 1].n;
+                ni = contribs[q + 1].in_ofs;
+            }
+            if (cf < 0) {  // identity (single-task tiles only): add the input rows
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 4 * lk + v;
+                    if (r < tl.w) {
+#pragma unroll
+                        for (int g = 0; g < NT; ++g) {
+                            const C<float> xv = in[int64_t(ci + tl.z + r) * ns + 16 * g + li];
+                            accr[g][v] += xv.x;
+                            acci[g][v] += xv.y;
+                        }
+                    }
+                }
+            } else {
+                const C<float> *ap = F + (tl.y == 1 ? cf + tl.z + li : rbase);
+                const int64_t ald = tl.y == 1 ? cm : rld;
+                const C<float> *xin = in + int64_t(ci) * ns + li;
+                // step k0 multiplies entries k0..k0+3 while k0+4..k0+7 load
+                C<float> a = C<float>{0.f, 0.f}, b[NT];
+                if (rok && lk < cn) a = ap[int64_t(lk) * ald];
+#pragma unroll
+                for (int g = 0; g < NT; ++g) b[g] = lk < cn ? xin[int64_t(lk) * ns + 16 * g] : C<float>{0.f, 0.f};
+                for (int k0 = 0; k0 < cn; k0 += 4) {
+                    const int kn = k0 + 4 + lk;
+                    C<float> an = C<float>{0.f, 0.f}, bn[NT];
+                    if (rok && kn < cn) an = ap[int64_t(kn) * ald];
+#pragma unroll
+                    for (int g = 0; g < NT; ++g)
+                        bn[g] = kn < cn ? xin[int64_t(kn) * ns + 16 * g] : C<float>{0.f, 0.f};
+                    const float nai = -a.y;
+#pragma unroll
+                    for (int g = 0; g < NT; ++g) accr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[g].x, accr[g], 0, 0, 0);
+#pragma unroll
+                    for (int g = 0; g < NT; ++g) acci[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[g].y, acci[g], 0, 0, 0);
+#pragma unroll
+                    for (int g = 0; g < NT; ++g) accr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai, b[g].y, accr[g], 0, 0, 0);
+#pragma unroll
+                    for (int g = 0; g < NT; ++g) acci[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b[g].x, acci[g], 0, 0, 0);
+                    a = an;
+#pragma unroll
+                    for (int g = 0; g < NT; ++g) b[g] = bn[g];
+                }
+            }
+            cf = nf;
+            cm = nm;
+            cn = nn;
+            ci = ni;
+        }
+        // lane li + 16 lk holds rows 4 lk + v of column li (+ 16 g)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 4 * lk + v;
+            const int orow = __shfl(rout, r, 64);
+            if (__shfl(int(rok), r, 64) == 0) continue;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) {
+                const int j = 16 * g + li;
+                if (j >= nc) continue;
+                const C<float> val = C<float>{accr[g][v], acci[g][v]};
+                if (OUTK == OUT_SLOT) {
+                    slots[int64_t(orow) * ns + j] = val;
+                } else {
+                    C<float> *yp = y + int64_t(j) * ldy + perm_out[orow];
+                    const C<float> old = *yp;
+                    *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, val));
+                }
+            }
+        }
+    }
+}
+
+void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, void *slots, int32_t ns,
+                     int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
+                     const double *beta, bool fuse, int n_cu) {
+    if (t.n_mtiles == 0) return;
+    BFG_REQUIRE(ns % 16 == 0 && ns >= 16 && ns <= 64 && nc >= 1 && nc <= ns,
+                "matvec: matrix-core chunk of 16, 32, 48 or 64 right-hand sides");
+    const int64_t wgs = (int64_t(t.n_mtiles) + 3) / 4;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * 16;
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
+    const C<float> al{float(alpha[0]), float(alpha[1])}, be{float(beta[0]), float(beta[1])};
+#define BFG_MF(OUTK, NT)                                                                                        \
+    hipLaunchKernelGGL((mfma_rhs_kernel<OUTK, NT>), grid, dim3(256), 0, s, t.mtiles.as<const int4>(), t.n_mtiles, \
+                       t.mtasks.as<const MvTaskC>(), t.contribs.as<const MvContrib>(), (const C<float> *)factors, (C<float> *)slots, ns, in_shift, (C<float> *)y, ldy, perm_out, al, \
+                       be, int(fuse), nc)
+#define BFG_MF_NT(OUTK)                     \
+    switch (ns / 16) {                      \
+    case 1: BFG_MF(OUTK, 1); break;         \
+    case 2: BFG_MF(OUTK, 2); break;         \
+    case 3: BFG_MF(OUTK, 3); break;         \
+    default: BFG_MF(OUTK, 4); break;        \
+    }
+    if (t.out_kind == OUT_SLOT) {
+        BFG_MF_NT(OUT_SLOT);
+    } else {
+        BFG_MF_NT(OUT_SCATTER);
+    }
+#undef BFG_MF_NT
+#undef BFG_MF
+    BFG_HIP(hipGetLastError());
+}
+
 // dst[v * ns + r] = x[r][perm[v]] for v in [lo, hi), r < nc: inputs of the
 // first level in tree order, right-hand sides interleaved.
 template <typename R>

@@ -592,6 +592,65 @@ void pack_tables(BF &bf) {
 // (butterfly data flow), so its producer signals that group's counter once.
 // The vectors an item reads are the producer outputs inside the input
 // ranges of its (merged) contributions.
+// Matrix-core tiles of a forward table.  Items with one factor-block
+// contribution and at most 16 rows that read the same input entries (same
+// in_ofs and n) are stacked, whole, into tiles of at most 16 rows; every
+// other item (several contributions, an identity block, more than 16 rows)
+// gets tiles of 16 rows of its own.  Items are taken in order of their
+// first input entry, so consecutive tiles read neighbouring inputs; the
+// tasks are copied in that order.
+void build_mfma(StageTable &S) {
+    if (S.mfma_built) return;
+    const int32_t n = int32_t(S.h_items.size());
+    struct Key { int32_t in_ofs, n, idx; };
+    std::vector<Key> keys(size_t(std::max(n, 0)));
+    std::vector<uint8_t> stack(size_t(std::max(n, 0)));
+    parallel_for(int64_t(n), [&](int64_t i) {
+        const MvItem &it = S.h_items[i];
+        const MvContrib &c = S.h_contribs[it.c_begin];
+        keys[i] = Key{c.in_ofs, c.n, int32_t(i)};
+        stack[i] = it.c_end - it.c_begin == 1 && c.fofs >= 0 && it.out_len <= 16;
+    }, 4096);
+    std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+        return a.in_ofs != b.in_ofs ? a.in_ofs < b.in_ofs : a.n != b.n ? a.n < b.n : a.idx < b.idx;
+    });
+    std::vector<MvTaskC> items(keys.size());
+    std::vector<int4> tiles;
+    tiles.reserve(keys.size() / 2 + 16);
+    int4 open{0, 0, 0, 0};  // open stacked tile (count 0: none)
+    int32_t open_in = 0, open_n = 0;
+    auto close = [&] {
+        if (open.y > 0) tiles.push_back(open);
+        open = int4{0, 0, 0, 0};
+    };
+    for (int32_t p = 0; p < int32_t(keys.size()); ++p) {
+        const int32_t i = keys[p].idx;
+        const MvTask &h = S.h_tasks[i];
+        items[p] = MvTaskC{h.fofs, h.in_ofs, h.out_ofs, h.m, h.n, h.c_begin, h.c_end};
+        const int32_t m = S.h_items[i].out_len;
+        if (m <= 0) continue;
+        if (stack[i]) {
+            if (open.y > 0 && (keys[p].in_ofs != open_in || keys[p].n != open_n || open.w + m > 16)) close();
+            if (open.y == 0) {
+                open = int4{p, 0, 0, 0};
+                open_in = keys[p].in_ofs;
+                open_n = keys[p].n;
+            }
+            ++open.y;
+            open.w += m;
+        } else {
+            close();
+            for (int32_t r0 = 0; r0 < m; r0 += 16) tiles.push_back(int4{p, 1, r0, std::min(16, m - r0)});
+        }
+    }
+    close();
+    BFG_REQUIRE(tiles.size() < size_t(INT32_MAX), "pack: too many matrix-core tiles");
+    upload_tab(S.mtasks, items);
+    upload_tab(S.mtiles, tiles);
+    S.n_mtiles = int32_t(tiles.size());
+    S.mfma_built = true;
+}
+
 void build_sweep(BF &bf) {
     SweepTables &sw = bf.sweep;
     const int32_t mode = sw.mode, wg = sw.wg_per_cu;
@@ -779,6 +838,11 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
         bf.slots.alloc(need);
         bf.slots_cap = need;
     }
+    if (lanes && bf.use_mfma && bf.precision == BFGPU_C64)  // before any capture: the build uploads
+        for (StageTable &t : bf.fwd) {
+            if (t.pre) build_mfma(*t.pre);
+            build_mfma(t);
+        }
     const bool fused = op == BFGPU_OP_N && sweep_ok(bf, nrhs);
     if (tl || !bf.use_graph || fused) {
         stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl);
@@ -838,6 +902,10 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         // right-hand sides across lanes, up to 64 per sweep, skeleton
         // vectors interleaved (entry-major) in the slot buffer
         const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+        const bool mfma = bf.use_mfma && bf.precision == BFGPU_C64;
+        if (mfma)
+            for (const StageTable &t : bf.fwd)
+                BFG_REQUIRE(t.mfma_built && (!t.pre || t.pre->mfma_built), "matvec: matrix-core tiles not built");
         for (int64_t c0 = 0; c0 < nrhs; c0 += 64) {
             const int nc = int(std::min<int64_t>(64, nrhs - c0));
             const int32_t ns = (nc + 15) / 16 * 16;
@@ -852,10 +920,15 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
                 const StageTable *pre = t.pre.get();
                 for (const StageTable *u : {pre, &t}) {
                     if (!u) continue;
-                    launch_rhs(s, bf.precision, *u, bf.factors[L].p, bf.slots.p, ns,
-                               u->in_kind == IN_GATHER ? bf.gather_ofs : 0, yc, ldy, pt, nc, alpha, b_eff, fuse,
-                               bf.ctx->n_cu);
-                    if (tl) tl->launched(s, {L, u == &t ? 0 : 1, 4, u->out_kind, 0, nc});
+                    const int64_t shift = u->in_kind == IN_GATHER ? bf.gather_ofs : 0;
+                    if (mfma) {
+                        launch_mfma_rhs(s, *u, bf.factors[L].p, bf.slots.p, ns, shift, yc, ldy, pt, nc, alpha,
+                                        b_eff, fuse, bf.ctx->n_cu);
+                    } else {
+                        launch_rhs(s, bf.precision, *u, bf.factors[L].p, bf.slots.p, ns, shift, yc, ldy, pt, nc,
+                                   alpha, b_eff, fuse, bf.ctx->n_cu);
+                    }
+                    if (tl) tl->launched(s, {L, u == &t ? 0 : 1, mfma ? 5 : 4, u->out_kind, 0, nc});
                 }
             }
         }

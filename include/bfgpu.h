@@ -262,7 +262,12 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *   BFGPU_OPT_RHS_LANES  forward matvecs with at least this many right-hand
  *                        sides (default 32; 0 = never) put the right-hand
  *                        sides across lanes, 64 per sweep, with the factor
- *                        entries read once per sweep */
+ *                        entries read once per sweep
+ *   BFGPU_OPT_MFMA       1 (default): with right-hand sides across lanes,
+ *                        complex FP32 factorizations multiply on the matrix
+ *                        cores (v_mfma_f32_16x16x4_f32, small blocks that
+ *                        read the same inputs stacked into 16-row tiles);
+ *                        0: vector-ALU kernel */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -270,7 +275,8 @@ enum {
     BFGPU_OPT_VARIANT = 4,
     BFGPU_OPT_TILE = 5,
     BFGPU_OPT_DENSE = 6,
-    BFGPU_OPT_RHS_LANES = 7
+    BFGPU_OPT_RHS_LANES = 7,
+    BFGPU_OPT_MFMA = 8
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

@@ -203,3 +203,55 @@ def test_rhs_across_lanes_matches_oracle(bfgpu, oracle, ctx):
     want = np.stack([oracle.matvec_many(imps, Xs[:, j], nthreads=8, rows=6000) for j in range(20)], axis=1)
     op.set_option("rhs_lanes", 16)
     assert rel(op @ Xs, want) < 1e-12
+
+
+def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
+    """Complex FP32 right-hand sides across lanes on the matrix cores
+    (stacked 16-row tiles, items of more than 16 rows, partial sums and
+    identity combines) against the oracle's FP64 product of the same FP32
+    factors and against the vector-ALU kernel.  Bar: 2e-6 relative, the
+    FP32 rounding of a few hundred products per entry along the level
+    chain (the two FP32 kernels differ in summation order only)."""
+    say("two squares N=4096 fp32: matrix-core right-hand sides")
+    n = 4096
+    obs, src = two_squares(n, seed=4)
+    To = bfgpu.ClusterTree(obs, dims=2, leaf_size=64)
+    Ts = bfgpu.ClusterTree(src, dims=2, leaf_size=64)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(To, Ts), 0, 0, K, 1e-4,
+                          acctype=np.complex64, ctx=ctx)
+    # the P level's items have 64 rows (a leaf): several row tiles each
+    imp = oracle.import_bf(To, Ts, g.export())
+    rng = np.random.default_rng(12)
+    X = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
+    want = np.stack([imp.matvec(X[:, j].astype(np.complex128)) for j in range(70)], axis=1)
+    assert g.get_option("mfma") == 1
+    g.set_option("rhs_lanes", 16)
+    for nr in (16, 17, 33, 64, 70):
+        got = (g @ np.asfortranarray(X[:, :nr])).astype(np.complex128)
+        assert rel(got, want[:, :nr]) < 2e-6, nr
+    Y0 = np.asfortranarray(crandn(rng, n, 17)).astype(np.complex64)
+    a, b = 0.5 - 0.25j, -1.5 + 0.5j
+    got = g.mul(Y0.copy(order="F"), np.asfortranarray(X[:, :17]), a, b).astype(np.complex128)
+    assert rel(got, a * want[:, :17] + b * Y0.astype(np.complex128)) < 2e-6
+    g.set_option("mfma", 0)
+    valu = (g @ np.asfortranarray(X[:, :64])).astype(np.complex128)
+    g.set_option("mfma", 1)
+    assert rel(valu, want[:, :64]) < 2e-6
+
+    say("far field sphere N=6000 fp32: matrix-core right-hand sides, partial sums")
+    pts = fibonacci_sphere(6000, 4.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
+                                ctx=ctx)
+    assert op.launch_stats()[-1, 7] > 0
+    imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
+    Xs = np.asfortranarray(crandn(rng, 6000, 40)).astype(np.complex64)
+    want = np.stack([oracle.matvec_many(imps, Xs[:, j].astype(np.complex128), nthreads=8, rows=6000)
+                     for j in range(40)], axis=1)
+    for nr in (32, 40):
+        got = (op @ np.asfortranarray(Xs[:, :nr])).astype(np.complex128)
+        assert rel(got, want[:, :nr]) < 2e-6, nr
+    op.set_option("mfma", 0)
+    valu = (op @ Xs).astype(np.complex128)
+    assert rel(valu, want) < 2e-6
