@@ -55,7 +55,13 @@ class Context:
         return self._h
 
     def set_stream(self, stream_ptr):
+        """Launch on this HIP stream (0 / None: the null stream, torch's
+        default stream)."""
         check(lib().bfgpu_ctx_set_stream(self._h, C.c_void_p(stream_ptr or None)))
+
+    def reset_stream(self):
+        """Launch on the context's own stream again."""
+        check(lib().bfgpu_ctx_reset_stream(self._h))
 
     def stream(self):
         s = C.c_void_p()
@@ -492,7 +498,7 @@ class ButterflyFactorization:
         return self
 
     OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7,
-               "mfma": 8}
+               "mfma": 8, "rhs_chunk": 9}
 
     @staticmethod
     def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0):
@@ -508,7 +514,8 @@ class ButterflyFactorization:
         task, 1 wavefront per task, 2 tiled, 3 dense), tile (tiled shape,
         see tile_shape), dense (dense shape of the forward tables),
         rhs_lanes (fewest right-hand sides put across lanes), mfma (1:
-        FP32 right-hand sides across lanes on the matrix cores)."""
+        FP32 right-hand sides across lanes on the matrix cores), rhs_chunk
+        (most right-hand sides per pass; 0: as device memory allows)."""
         check(lib().bfgpu_bf_set_option(self.handle, self.OPTIONS[key], int(value)))
         return self
 

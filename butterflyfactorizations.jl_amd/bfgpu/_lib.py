@@ -44,6 +44,7 @@ class BFInfo(C.Structure):
         ("factor_bytes", i64), ("slot_len", i64), ("n_comp", i64),
         ("n_retry", i64), ("t_factor_ms", f64),
         ("t_plan_ms", f64), ("t_sample_ms", f64), ("t_qr_ms", f64), ("t_interp_ms", f64), ("t_pack_ms", f64),
+        ("device_bytes", i64),
     ]
 
 
@@ -65,6 +66,7 @@ _SIGS = {
     "bfgpu_ctx_create": (i32, [C.c_int, P(vp)]),
     "bfgpu_ctx_destroy": (i32, [vp]),
     "bfgpu_ctx_set_stream": (i32, [vp, vp]),
+    "bfgpu_ctx_reset_stream": (i32, [vp]),
     "bfgpu_ctx_get_stream": (i32, [vp, P(vp)]),
     "bfgpu_ctx_sync": (i32, [vp]),
     "bfgpu_tree_build": (i32, [P(f64), i64, i32, i32, i32, P(vp)]),

@@ -98,7 +98,14 @@ extern "C" int bfgpu_ctx_destroy(bfgpu_ctx *ctx) {
 extern "C" int bfgpu_ctx_set_stream(bfgpu_ctx *ctx, void *s) {
     BFG_API_BEGIN
     BFG_REQUIRE(ctx, "ctx_set_stream: null ctx");
-    ctx->c.stream = s ? static_cast<hipStream_t>(s) : ctx->c.own;
+    ctx->c.stream = static_cast<hipStream_t>(s);
+    BFG_API_END
+}
+
+extern "C" int bfgpu_ctx_reset_stream(bfgpu_ctx *ctx) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(ctx, "ctx_reset_stream: null ctx");
+    ctx->c.stream = ctx->c.own;
     BFG_API_END
 }
 
@@ -248,6 +255,21 @@ extern "C" int bfgpu_bf_info_get(const bfgpu_bf *h, bfgpu_bf_info *o) {
     o->t_qr_ms = bf.t_qr_ms;
     o->t_interp_ms = bf.t_interp_ms;
     o->t_pack_ms = bf.t_pack_ms;
+    int64_t dev = 0;
+    for (const DevBuf &d : bf.factors) dev += int64_t(d.bytes);
+    auto tables = [&](const StageTable &t) {
+        for (const DevBuf *d : {&t.items, &t.contribs, &t.tasks, &t.tasks_c, &t.mtiles, &t.mtasks})
+            dev += int64_t(d->bytes);
+    };
+    for (const std::vector<StageTable> *v : {&bf.fwd, &bf.tr})
+        for (const StageTable &t : *v) {
+            tables(t);
+            if (t.pre) tables(*t.pre);
+        }
+    for (const DevBuf *d : {&bf.perm_test, &bf.perm_trial, &bf.slots, &bf.xy_host_stage, &bf.sweep.tasks,
+                            &bf.sweep.contribs, &bf.sweep.counters, &bf.sweep.err})
+        dev += int64_t(d->bytes);
+    o->device_bytes = dev;
     BFG_API_END
 }
 
@@ -671,6 +693,11 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         bf.use_mfma = value == 1;
         release_graph(bf);
         break;
+    case BFGPU_OPT_RHS_CHUNK:
+        BFG_REQUIRE(value >= 0, "bf_set_option: rhs_chunk is a number of right-hand sides (0: automatic)");
+        bf.rhs_chunk = value;
+        release_graph(bf);
+        break;
     case BFGPU_OPT_DENSE: {
         const TileShape sh = unpack_tile(value);
         BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
@@ -713,6 +740,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_DENSE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
     case BFGPU_OPT_RHS_LANES: *value = bf.rhs_lanes_min; break;
     case BFGPU_OPT_MFMA: *value = bf.use_mfma ? 1 : 0; break;
+    case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }
     BFG_API_END

@@ -152,7 +152,7 @@ struct Engine {
     size_t es;
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
-    int64_t n_comp = 0, n_retry = 0;
+    int64_t n_comp = 0, n_retry = 0, n_rank0 = 0;
     double t_host = 0, t_qr = 0, t_interp = 0;  // seconds: sample prep, fill+QR, interpolation
 
     Engine(Ctx &c, const Tree &te, const Tree &tr, const bfgpu_opts &op)
@@ -160,6 +160,24 @@ struct Engine {
           es(op.precision == BFGPU_C128 ? 16 : 8) {
         upload(d_test_pts, test.points, ctx.stream);
         upload(d_trial_pts, trial.points, ctx.stream);
+    }
+
+    // Shrink a level's factor buffer to what it holds (the growth policy
+    // below leaves up to a third of it unused).
+    void trim_pool(DevBuf &pool, int64_t &cap_elems, int64_t used) {
+        const int64_t need = used + FACTOR_PAD;
+        if (cap_elems <= need + need / 32) return;
+        DevBuf nb;
+        nb.alloc(size_t(need) * es);
+        BFG_HIP(hipMemcpyAsync(nb.p, pool.p, size_t(need) * es, hipMemcpyDeviceToDevice, ctx.stream));
+        BFG_HIP(hipStreamSynchronize(ctx.stream));
+        pool = std::move(nb);
+        cap_elems = need;
+    }
+
+    void release() {
+        for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z})
+            *d = DevBuf();
     }
 
     void ensure_pool(DevBuf &pool, int64_t &cap_elems, int64_t need) {
@@ -225,8 +243,27 @@ struct Engine {
         std::vector<int64_t> fe, rofs, cofs;
         std::vector<QrTask> qt;
         std::vector<InterpTask> it;
+        // kernel-matrix tiles of one batch: at most 32 GiB or 30 % of the
+        // free device memory (besides what the tile buffer already holds)
+        size_t free_b = 0, total_b = 0;
+        BFG_HIP(hipMemGetInfo(&free_b, &total_b));
+        const int64_t z_budget = std::max<int64_t>(
+            int64_t(1) << 30, std::min<int64_t>(int64_t(32) << 30, int64_t(0.3 * double(free_b + d_Z.bytes))));
+        std::vector<int32_t> later;
         while (!pend.empty()) {
             auto th = clk::now();
+            {
+                size_t nb = 0;
+                int64_t zb = 0;
+                for (; nb < pend.size(); ++nb) {
+                    const Comp &c = cs[pend[nb]];
+                    const int64_t z = int64_t(c.rows) * int64_t(c.cols.size()) * es;
+                    if (nb > 0 && zb + z > z_budget) break;
+                    zb += z;
+                }
+                later.assign(pend.begin() + nb, pend.end());
+                pend.resize(nb);
+            }
             const size_t np = pend.size();
             ft.resize(np);
             fe.resize(np);
@@ -321,6 +358,9 @@ struct Engine {
                     continue;
                 }
                 c.rank = r;
+                if (r == 0 && n_rank0++ < 4 && std::getenv("BFGPU_TIMING"))
+                    std::fprintf(stderr, "[bfgpu] rank-0 compression: %d sampled rows of %d observers x %d columns\n",
+                                 c.rows, c.n_obs, int(c.cols.size()));
                 c.tofs = pool_used;
                 pool_used += int64_t(r) * int64_t(c.cols.size());
                 it.push_back(InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, r, qt[q].pofs});
@@ -341,6 +381,7 @@ struct Engine {
                 BFG_HIP(hipStreamSynchronize(ctx.stream));
             }
             t_interp += secs(ti, clk::now());
+            retry.insert(retry.end(), later.begin(), later.end());
             pend.swap(retry);
         }
     }
@@ -458,6 +499,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
         E.run(cs, bf.factors[0], cap, used);
+        E.trim_pool(bf.factors[0], cap, used);
         bf.factor_elems[0] = used;
         auto te = clk::now();
         parallel_for(nB, [&](int64_t b) {
@@ -553,6 +595,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
         E.run(cs, bf.factors[l], cap, used);
+        E.trim_pool(bf.factors[l], cap, used);
         bf.factor_elems[l] = used;
         // emit blocks in action order; K updates in the same order, applied
         // after the level (the reference's sequential update, :177-180)
@@ -665,6 +708,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     bf.precision = o.precision;
     for (int64_t b = 0; b < nB; ++b) bf.bfs[b].stage_data_base.assign(bf.bfs[b].st.size(), 0);
     auto tfree = clk::now();
+    E.release();  // tile workspace and index lists: not needed for packing
     parallel_for(nB, [&](int64_t b) { Build().swap_out(B[b]); });
     std::vector<Build>().swap(B);
     const double t_free = secs(tfree, clk::now());
@@ -681,10 +725,10 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         std::fprintf(stderr,
                      "[bfgpu] factorize %.3fs: setup %.3fs, plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "
                      "interp %.3fs, emit %.3fs, free %.3fs, pack %.3fs (%lld compressions, %lld retries, "
-                     "%d threads)\n",
+                     "%lld of rank 0, %d threads)\n",
                      bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_host, E.t_qr, E.t_interp, t_emit, t_free,
                      secs(tpk, t1),
-                     (long long)E.n_comp, (long long)E.n_retry, host_threads());
+                     (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, host_threads());
 }
 
 }  // namespace bfg

@@ -242,6 +242,7 @@ struct BF {
     bool use_graph = false;       // replay per-level launches from a HIP graph
     int32_t rhs_lanes_min = 32;   // forward matvecs with >= this many RHS: RHS across lanes (0: never)
     bool use_mfma = true;         // RHS across lanes in complex FP32: matrix-core kernel
+    int32_t rhs_chunk = 0;        // most right-hand sides per pass (0: as device memory allows)
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
     ~BF();
@@ -255,6 +256,10 @@ struct BF {
     double t_factor_ms = 0.0;
     double t_plan_ms = 0.0, t_sample_ms = 0.0, t_qr_ms = 0.0, t_interp_ms = 0.0, t_pack_ms = 0.0;
 };
+
+// Most tasks per dispatch of the kernels that give every task a workgroup
+// (a dispatch holds fewer than 2^32 work-items in x).
+constexpr int32_t TASK_CHUNK = 1 << 23;
 
 // Entries allocated past the end of every factor buffer: the dense kernel
 // reads two rows with one load, one entry beyond an odd-height block.

@@ -61,9 +61,11 @@ __global__ __launch_bounds__(256) void fill_kernel(const double *__restrict__ ob
                                                    const int64_t *__restrict__ task_end,
                                                    int32_t n_tasks, int64_t total, C<R> *Z,
                                                    double k) {
-    int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (e >= total) return;
-    int32_t lo = 0, hi = n_tasks - 1;
+    // grid-stride: a dispatch holds fewer than 2^32 work-items, the tiles of
+    // one level can hold more entries
+    int32_t lo = 0;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+    int32_t hi = n_tasks - 1;  // entries only grow: search from the last task
     while (lo < hi) {
         int32_t mid = (lo + hi) >> 1;
         if (task_end[mid] > e) hi = mid;
@@ -90,13 +92,14 @@ __global__ __launch_bounds__(256) void fill_kernel(const double *__restrict__ ob
         v = {float(c * g), float(-s * g)};
     }
     Z[t.zofs + local] = v;
+    }
 }
 
 void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
                  const int32_t *rows, const int32_t *cols, const FillTask *tasks,
                  const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k) {
     if (total <= 0 || n_tasks <= 0) return;
-    dim3 grid(unsigned((total + 255) / 256));
+    dim3 grid(unsigned(std::min<int64_t>((total + 255) / 256, int64_t(1) << 22)));
     if (prec == BFGPU_C128)
         hipLaunchKernelGGL(fill_kernel<double>, grid, dim3(256), 0, s, obs_pts, src_pts, rows,
                            cols, tasks, task_end, n_tasks, total, (C<double> *)Z, k);
@@ -173,6 +176,8 @@ __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__rest
             for (int w = 1; w < 4; ++w)
                 if (rv[w] > b || (rv[w] == b && ri[w] < p)) { b = rv[w]; p = ri[w]; }
         }
+        // no pivot among the columns (only when norms are NaN): stop here
+        if (p >= n) { rank = k; break; }
         // exact remaining norm of the pivot column
         if (wave == 0) {
             double s = 0.0;
@@ -186,7 +191,7 @@ __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__rest
         __syncthreads();
         double pn = s_pn;
         if (k == 0) r11 = pn;
-        if (pn == 0.0 || (k > 0 && pn <= rtol * r11)) { rank = k; break; }
+        if (!(pn > 0.0) || (k > 0 && pn <= rtol * r11)) { rank = k; break; }
         if (p != k) {
             for (int i = tid; i < m; i += 256) {
                 C<R> t = A[i + int64_t(p) * m];
@@ -339,13 +344,15 @@ __global__ __launch_bounds__(256) void cpqr_wave_kernel(C<R> *Z, const QrTask *_
             if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
         }
         const int p = __shfl(bi, 0, 64);
+        // no pivot among the columns (only when norms are NaN): stop here
+        if (p >= n || !(__shfl(bv, 0, 64) >= 0.0)) { rank = k; break; }
         // exact remaining norm of the pivot column
         const C<R> *ap = A + int64_t(p) * ld;
         double s = 0.0;
         for (int i = k + lane; i < m; i += 64) s += double(ap[i].x) * ap[i].x + double(ap[i].y) * ap[i].y;
         const double pn = sqrt(wave_sum(s));
         if (k == 0) r11 = pn;
-        if (pn == 0.0 || (k > 0 && pn <= rtol * r11)) { rank = k; break; }
+        if (!(pn > 0.0) || (k > 0 && pn <= rtol * r11)) { rank = k; break; }
         if (p != k) {
             sync();  // the norm's reads of column p precede the swap's writes
             C<R> *ak = A + int64_t(k) * ld;
@@ -432,25 +439,32 @@ void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t 
         // blocks per workgroup: as many as 64 KiB of LDS hold, at most 4
         const int wpb = int(std::min<size_t>(4, (64 * 1024) / per_wave));
         const size_t lds = per_wave * size_t(wpb);
-        dim3 grid(unsigned((int64_t(n_tasks) + wpb - 1) / wpb));
-        if (prec == BFGPU_C128)
-            hipLaunchKernelGGL(cpqr_wave_kernel<double>, grid, dim3(64 * wpb), lds, s, (C<double> *)Z, tasks,
-                               n_tasks, piv, rank, rtol, int32_t(region));
-        else
-            hipLaunchKernelGGL(cpqr_wave_kernel<float>, grid, dim3(64 * wpb), lds, s, (C<float> *)Z, tasks,
-                               n_tasks, piv, rank, rtol, int32_t(region));
-        BFG_HIP(hipGetLastError());
+        // chunks of tasks: a dispatch holds fewer than 2^32 work-items
+        for (int32_t t0 = 0; t0 < n_tasks; t0 += TASK_CHUNK) {
+            const int32_t nt = std::min(TASK_CHUNK, n_tasks - t0);
+            dim3 grid(unsigned((int64_t(nt) + wpb - 1) / wpb));
+            if (prec == BFGPU_C128)
+                hipLaunchKernelGGL(cpqr_wave_kernel<double>, grid, dim3(64 * wpb), lds, s, (C<double> *)Z, tasks + t0,
+                                   nt, piv, rank + t0, rtol, int32_t(region));
+            else
+                hipLaunchKernelGGL(cpqr_wave_kernel<float>, grid, dim3(64 * wpb), lds, s, (C<float> *)Z, tasks + t0,
+                                   nt, piv, rank + t0, rtol, int32_t(region));
+            BFG_HIP(hipGetLastError());
+        }
         return;
     }
     size_t lds = sizeof(double) * 2 * size_t(max_n > 0 ? max_n : 1);
     BFG_REQUIRE(lds <= 64 * 1024, "cpqr: too many columns for the norm arrays");
-    if (prec == BFGPU_C128)
-        hipLaunchKernelGGL(cpqr_kernel<double>, dim3(n_tasks), dim3(256), lds, s,
-                           (C<double> *)Z, tasks, piv, rank, rtol);
-    else
-        hipLaunchKernelGGL(cpqr_kernel<float>, dim3(n_tasks), dim3(256), lds, s,
-                           (C<float> *)Z, tasks, piv, rank, rtol);
-    BFG_HIP(hipGetLastError());
+    for (int32_t t0 = 0; t0 < n_tasks; t0 += TASK_CHUNK) {
+        const int32_t nt = std::min(TASK_CHUNK, n_tasks - t0);
+        if (prec == BFGPU_C128)
+            hipLaunchKernelGGL(cpqr_kernel<double>, dim3(nt), dim3(256), lds, s,
+                               (C<double> *)Z, tasks + t0, piv, rank + t0, rtol);
+        else
+            hipLaunchKernelGGL(cpqr_kernel<float>, dim3(nt), dim3(256), lds, s,
+                               (C<float> *)Z, tasks + t0, piv, rank + t0, rtol);
+        BFG_HIP(hipGetLastError());
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -520,13 +534,16 @@ void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tas
                    int32_t n_tasks, const int32_t *piv, void *T, int32_t max_rank) {
     if (n_tasks <= 0) return;
     BFG_REQUIRE(max_rank <= 256, "interp: rank above 256 is not supported");
-    if (prec == BFGPU_C128)
-        hipLaunchKernelGGL(interp_kernel<double>, dim3(n_tasks), dim3(256), 0, s,
-                           (const C<double> *)Z, tasks, piv, (C<double> *)T);
-    else
-        hipLaunchKernelGGL(interp_kernel<float>, dim3(n_tasks), dim3(256), 0, s,
-                           (const C<float> *)Z, tasks, piv, (C<float> *)T);
-    BFG_HIP(hipGetLastError());
+    for (int32_t t0 = 0; t0 < n_tasks; t0 += TASK_CHUNK) {
+        const int32_t nt = std::min(TASK_CHUNK, n_tasks - t0);
+        if (prec == BFGPU_C128)
+            hipLaunchKernelGGL(interp_kernel<double>, dim3(nt), dim3(256), 0, s,
+                               (const C<double> *)Z, tasks + t0, piv, (C<double> *)T);
+        else
+            hipLaunchKernelGGL(interp_kernel<float>, dim3(nt), dim3(256), 0, s,
+                               (const C<float> *)Z, tasks + t0, piv, (C<float> *)T);
+        BFG_HIP(hipGetLastError());
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -705,22 +722,25 @@ static void launch_fwdw_t(hipStream_t s, const StageTable &t, const void *factor
                           const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs,
                           const double *alpha, const double *beta, bool fuse) {
     C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
-    dim3 grid(unsigned((t.n_items + 3) / 4), unsigned((nrhs + NR - 1) / NR));
     auto F = (const C<R> *)factors;
     auto X = (const C<R> *)x;
     auto Y = (C<R> *)y;
     auto S = (C<R> *)slots;
-    auto T = t.tasks.as<const MvTask>();
     auto P = t.contribs.as<const MvContrib>();
-#define BFG_FWDW(INK, OUTK)                                                                       \
-    hipLaunchKernelGGL((fwdw_kernel<R, INK, OUTK, NR>), grid, dim3(256), 0, s, T, t.n_items, P, F, X, \
+    for (int32_t t0 = 0; t0 < t.n_items; t0 += TASK_CHUNK) {
+        const int32_t nt = std::min(TASK_CHUNK, t.n_items - t0);
+        dim3 grid(unsigned((nt + 3) / 4), unsigned((nrhs + NR - 1) / NR));
+        auto T = t.tasks.as<const MvTask>() + t0;
+#define BFG_FWDW(INK, OUTK)                                                                \
+    hipLaunchKernelGGL((fwdw_kernel<R, INK, OUTK, NR>), grid, dim3(256), 0, s, T, nt, P, F, X, \
                        ldx, Y, ldy, S, slot_len, perm_in, perm_out, a, b, int(fuse), int(nrhs))
-    if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_FWDW(IN_SLOT, OUT_SLOT);
-    else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_FWDW(IN_GATHER, OUT_SLOT);
-    else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_FWDW(IN_SLOT, OUT_SCATTER);
-    else BFG_FWDW(IN_GATHER, OUT_SCATTER);
+        if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_FWDW(IN_SLOT, OUT_SLOT);
+        else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_FWDW(IN_GATHER, OUT_SLOT);
+        else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_FWDW(IN_SLOT, OUT_SCATTER);
+        else BFG_FWDW(IN_GATHER, OUT_SCATTER);
 #undef BFG_FWDW
-    BFG_HIP(hipGetLastError());
+        BFG_HIP(hipGetLastError());
+    }
 }
 
 template <typename R, int NR>
@@ -729,22 +749,25 @@ static void launch_fwd_t(hipStream_t s, const StageTable &t, const void *factors
                          const int32_t *perm_in, const int32_t *perm_out, int64_t nrhs,
                          const double *alpha, const double *beta, bool fuse) {
     C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
-    dim3 grid(unsigned(t.n_items), unsigned((nrhs + NR - 1) / NR));
     auto F = (const C<R> *)factors;
     auto X = (const C<R> *)x;
     auto Y = (C<R> *)y;
     auto S = (C<R> *)slots;
-    auto T = t.tasks.as<const MvTask>();
     auto P = t.contribs.as<const MvContrib>();
+    for (int32_t t0 = 0; t0 < t.n_items; t0 += TASK_CHUNK) {
+        const int32_t nt = std::min(TASK_CHUNK, t.n_items - t0);
+        dim3 grid(static_cast<unsigned>(nt), static_cast<unsigned>((nrhs + NR - 1) / NR));
+        auto T = t.tasks.as<const MvTask>() + t0;
 #define BFG_FWD(INK, OUTK)                                                                 \
     hipLaunchKernelGGL((fwd_kernel<R, INK, OUTK, NR>), grid, dim3(256), 0, s, T, P, F, X, ldx, \
                        Y, ldy, S, slot_len, perm_in, perm_out, a, b, int(fuse), int(nrhs))
-    if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_FWD(IN_SLOT, OUT_SLOT);
-    else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_FWD(IN_GATHER, OUT_SLOT);
-    else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_FWD(IN_SLOT, OUT_SCATTER);
-    else BFG_FWD(IN_GATHER, OUT_SCATTER);
+        if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_FWD(IN_SLOT, OUT_SLOT);
+        else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_FWD(IN_GATHER, OUT_SLOT);
+        else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_FWD(IN_SLOT, OUT_SCATTER);
+        else BFG_FWD(IN_GATHER, OUT_SCATTER);
 #undef BFG_FWD
-    BFG_HIP(hipGetLastError());
+        BFG_HIP(hipGetLastError());
+    }
 }
 
 // Task descriptors of one wave's task group (tpw consecutive MvTaskC, at
@@ -2184,22 +2207,25 @@ static void launch_stage_t(hipStream_t s, const StageTable &t, const void *facto
     int max_in = t.max_in > 0 ? t.max_in : 1;
     size_t lds = sizeof(C<R>) * (size_t(max_in) + (OP != 0 ? size_t(t.max_out) : 0));
     BFG_REQUIRE(lds <= 64 * 1024, "matvec stage: vectors too long for LDS staging");
-    dim3 grid(unsigned(t.n_items), unsigned(nrhs));
     auto F = (const C<R> *)factors;
     auto X = (const C<R> *)x;
     auto Y = (C<R> *)y;
     auto S = (C<R> *)slots;
-    auto I = t.items.as<const MvItem>();
     auto P = t.contribs.as<const MvContrib>();
+    for (int32_t t0 = 0; t0 < t.n_items; t0 += TASK_CHUNK) {
+        const int32_t nt = std::min(TASK_CHUNK, t.n_items - t0);
+        dim3 grid(static_cast<unsigned>(nt), static_cast<unsigned>(nrhs));
+        auto I = t.items.as<const MvItem>() + t0;
 #define BFG_STAGE(INK, OUTK)                                                                \
     hipLaunchKernelGGL((stage_kernel<R, OP, INK, OUTK>), grid, dim3(256), lds, s, I, P, F, X, \
                        ldx, Y, ldy, S, slot_len, perm_in, perm_out, a, b, int(fuse), max_in)
-    if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_STAGE(IN_SLOT, OUT_SLOT);
-    else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_STAGE(IN_GATHER, OUT_SLOT);
-    else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_STAGE(IN_SLOT, OUT_SCATTER);
-    else BFG_STAGE(IN_GATHER, OUT_SCATTER);
+        if (t.in_kind == IN_SLOT && t.out_kind == OUT_SLOT) BFG_STAGE(IN_SLOT, OUT_SLOT);
+        else if (t.in_kind == IN_GATHER && t.out_kind == OUT_SLOT) BFG_STAGE(IN_GATHER, OUT_SLOT);
+        else if (t.in_kind == IN_SLOT && t.out_kind == OUT_SCATTER) BFG_STAGE(IN_SLOT, OUT_SCATTER);
+        else BFG_STAGE(IN_GATHER, OUT_SCATTER);
 #undef BFG_STAGE
-    BFG_HIP(hipGetLastError());
+        BFG_HIP(hipGetLastError());
+    }
 }
 
 void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const void *factors,
@@ -2243,8 +2269,8 @@ void launch_scale(hipStream_t s, int prec, void *y, int64_t n, int64_t ldy, int6
 
 template <typename A, typename B>
 __global__ void convert_kernel(const A *in, B *out, int64_t n) {
-    int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < n) out[i] = B(in[i]);
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+        out[i] = B(in[i]);
 }
 
 void launch_convert(hipStream_t s, const void *in, int in_prec, void *out, int out_prec,
@@ -2252,7 +2278,7 @@ void launch_convert(hipStream_t s, const void *in, int in_prec, void *out, int o
     // n complex entries -> 2 n reals
     int64_t nr = 2 * n;
     if (nr <= 0) return;
-    dim3 grid(unsigned((nr + 255) / 256));
+    dim3 grid(unsigned(std::min<int64_t>((nr + 255) / 256, int64_t(1) << 22)));
     if (in_prec == BFGPU_C128 && out_prec == BFGPU_C64)
         hipLaunchKernelGGL((convert_kernel<double, float>), grid, dim3(256), 0, s,
                            (const double *)in, (float *)out, nr);

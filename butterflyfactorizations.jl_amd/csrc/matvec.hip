@@ -143,7 +143,7 @@ static void validate_tables(const BF &bf) {
                                     bf.factors[L].bytes >= size_t(fe + FACTOR_PAD) *
                                                                (bf.precision == BFGPU_C128 ? 16 : 8)),
                         "pack: factor buffer without its tail pad");
-            std::atomic<int> bad{0};
+            std::atomic<int64_t> bad{INT64_MAX};  // first item out of bounds
             parallel_for(int64_t(S.h_items.size()), [&](int64_t i) {
                 const MvItem &it = S.h_items[i];
                 bool ok = it.out_ofs >= 0 && it.out_len >= 0 && int64_t(it.out_ofs) + it.out_len <= out_lim &&
@@ -155,10 +155,27 @@ static void validate_tables(const BF &bf) {
                          int64_t(k.in_ofs) + cols <= in_lim &&
                          (k.fofs < 0 ? k.m == k.n : k.fofs + int64_t(k.m) * k.n <= fe);
                 }
-                if (!ok) bad.store(1);
+                if (!ok) {
+                    int64_t cur = bad.load();
+                    while (i < cur && !bad.compare_exchange_weak(cur, i)) {
+                    }
+                }
             }, 4096);
-            BFG_REQUIRE(!bad.load(), "pack: launch table " + std::to_string(L) + (fwd ? " (forward" : " (transposed") +
-                                         (dir >= 2 ? ", partial sums)" : ")") + " indexes outside its buffers");
+            if (bad.load() != INT64_MAX) {
+                const MvItem &it = S.h_items[size_t(bad.load())];
+                std::string d = "item " + std::to_string(bad.load()) + " out " + std::to_string(it.out_ofs) + "+" +
+                                std::to_string(it.out_len) + " of " + std::to_string(out_lim) + ", contributions [" +
+                                std::to_string(it.c_begin) + ", " + std::to_string(it.c_end) + ") of " +
+                                std::to_string(S.n_contrib);
+                for (int32_t c = std::max(it.c_begin, 0); c < it.c_end && c < S.n_contrib && c < it.c_begin + 4; ++c) {
+                    const MvContrib &k = S.h_contribs[c];
+                    d += "; block " + std::to_string(k.m) + "x" + std::to_string(k.n) + " at " + std::to_string(k.fofs) +
+                         " of " + std::to_string(fe) + ", input " + std::to_string(k.in_ofs) + " of " +
+                         std::to_string(in_lim);
+                }
+                BFG_REQUIRE(false, "pack: launch table " + std::to_string(L) + (fwd ? " (forward" : " (transposed") +
+                                       (dir >= 2 ? ", partial sums)" : ")") + " indexes outside its buffers: " + d);
+            }
             if (S.in_kind == IN_GATHER) {
                 // the permuted copy of the inputs covers [in_lo, in_hi)
                 const std::vector<int32_t> &perm = fwd ? bf.trial->perm : bf.test->perm;
@@ -339,15 +356,23 @@ void pack_tables(BF &bf) {
                 const auto &blk = bf.bfs[b].st[s].b;
                 const int64_t g0 = ix.first[ix.sf[b] + s];
                 km.reset(blk.size());
-                int32_t n = 0;
+                int32_t n = 0, kept = 0;
                 for (size_t q = 0; q < blk.size(); ++q) {
                     const int64_t g = g0 + int64_t(q);
+                    // a block that writes an empty vector (rank 0) writes
+                    // nothing; its vector shares its offset with the next
+                    // one, so it must not key an item
+                    if ((fwd ? blk[q].m : blk[q].n) == 0) {
+                        litem[g] = -1;
+                        continue;
+                    }
                     auto r = km.emplace(uint64_t(uint32_t(fwd ? sl_out[g] : sl_in[g])), n);
                     n += r.second;
                     litem[g] = *r.first;
+                    ++kept;
                 }
                 nit[b + 1] = n;
-                nct[b + 1] = int64_t(blk.size());
+                nct[b + 1] = kept;
             });
             for (int64_t b = 0; b < nb; ++b) {
                 nit[b + 1] += nit[b];
@@ -364,7 +389,8 @@ void pack_tables(BF &bf) {
                 const int64_t g0 = ix.first[ix.sf[b] + s];
                 const int32_t n = int32_t(nit[b + 1] - nit[b]);
                 cur.assign(size_t(n) + 1, 0);
-                for (size_t q = 0; q < blk.size(); ++q) ++cur[size_t(litem[g0 + int64_t(q)]) + 1];
+                for (size_t q = 0; q < blk.size(); ++q)
+                    if (litem[g0 + int64_t(q)] >= 0) ++cur[size_t(litem[g0 + int64_t(q)]) + 1];
                 for (int32_t j = 0; j < n; ++j) cur[j + 1] += cur[j];
                 for (int32_t j = 0; j < n; ++j) {
                     MvItem &it = items[size_t(nit[b] + j)];
@@ -372,6 +398,7 @@ void pack_tables(BF &bf) {
                 }
                 for (size_t q = 0; q < blk.size(); ++q) {
                     const Block &k = blk[q];
+                    if (litem[g0 + int64_t(q)] < 0) continue;
                     int64_t oo, io;
                     describe(b, s, k, g0 + int64_t(q), oo, io);
                     MvItem &it = items[size_t(nit[b] + litem[g0 + int64_t(q)])];
@@ -786,7 +813,7 @@ struct Timeline {
 static int32_t nr_per_lane(int64_t nrhs) { return nrhs <= 1 ? 1 : nrhs == 2 ? 2 : nrhs <= 4 ? 4 : 8; }
 
 static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl);
+                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl, bool lanes);
 
 // One launch table of level L: the tiled kernel (after permuting an
 // IN_GATHER input into the slot buffer) or the per-task kernels.
@@ -824,13 +851,48 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
     if (tl) tl->launched(s, {L, table, t.variant, t.out_kind, int32_t(pack_tile(t.tile)), nr_per_lane(nrhs)});
 }
 
+static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                           const double *alpha, const double *beta, Timeline *tl, bool lanes, bool graph_ok);
+
 static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                       int64_t ldy, const double *alpha, const double *beta, Timeline *tl) {
+    const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+    // right-hand sides per pass: all of them (at most 64 per sweep with
+    // right-hand sides across lanes), fewer when the caller caps them or
+    // when the slot buffer of one pass would not fit in 90 % of the device
+    // memory it may take
+    const bool lanes = op == BFGPU_OP_N && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min;
+    const int64_t full = lanes ? std::min<int64_t>(64, nrhs) : nrhs;
+    int64_t per = full;
+    if (bf.rhs_chunk > 0) per = std::min<int64_t>(per, bf.rhs_chunk);
+    auto cols_of = [&](int64_t c) { return size_t(lanes ? (c + 15) / 16 * 16 : c); };
+    const size_t col_bytes = size_t(std::max<int64_t>(bf.slot_len, 1)) * es;
+    if (cols_of(per) * col_bytes > bf.slots_cap) {
+        size_t free_b = 0, total_b = 0;
+        BFG_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t avail = size_t(0.9 * double(free_b + bf.slots_cap));
+        while (per > 1 && cols_of(per) * col_bytes > avail) per = lanes && per > 16 ? per - 16 : (per + 1) / 2;
+        BFG_REQUIRE(cols_of(per) * col_bytes <= avail, "matvec: not enough device memory for the skeleton vectors");
+    }
+    if (per >= full) {
+        stage_seq_pass(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl, lanes, true);
+        return;
+    }
+    for (int64_t c0 = 0; c0 < nrhs; c0 += per)
+        stage_seq_pass(bf, op, std::min(per, nrhs - c0), static_cast<const char *>(x) + size_t(c0) * size_t(ldx) * es,
+                       ldx, static_cast<char *>(y) + size_t(c0) * size_t(ldy) * es, ldy, alpha, beta, tl, lanes,
+                       false);
+}
+
+// One pass over nrhs columns (lanes: right-hand sides across lanes, up to
+// 64 per sweep); graph_ok: the per-level launches may be replayed from a
+// captured graph.
+static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                           const double *alpha, const double *beta, Timeline *tl, bool lanes, bool graph_ok) {
     hipStream_t s = bf.ctx->stream;
     const int es = bf.precision == BFGPU_C128 ? 16 : 8;
     // slot buffer: one column per RHS, or (right-hand sides across lanes)
     // up to 64 interleaved RHS per sweep
-    const bool lanes = op == BFGPU_OP_N && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min;
     const size_t cols = lanes ? size_t((std::min<int64_t>(64, nrhs) + 15) / 16 * 16) : size_t(nrhs);
     const size_t need = size_t(bf.slot_len) * cols * es;
     if (need > bf.slots_cap) {
@@ -843,9 +905,9 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
             if (t.pre) build_mfma(*t.pre);
             build_mfma(t);
         }
-    const bool fused = op == BFGPU_OP_N && sweep_ok(bf, nrhs);
-    if (tl || !bf.use_graph || fused) {
-        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl);
+    const bool fused = op == BFGPU_OP_N && !lanes && sweep_ok(bf, nrhs);
+    if (tl || !bf.use_graph || fused || !graph_ok) {
+        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl, lanes);
         return;
     }
     // per-level launches replayed from a captured graph while the call
@@ -859,7 +921,7 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
         BFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipGraph_t g = nullptr;
         try {
-            stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr);
+            stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr, lanes);
         } catch (...) {
             (void)hipStreamEndCapture(s, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -875,7 +937,7 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
 }
 
 static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl) {
+                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl, bool lanes) {
     hipStream_t s = bf.ctx->stream;
     const int32_t nL = bf.n_stage_launch;
     const bool fwd = (op == BFGPU_OP_N);
@@ -886,7 +948,7 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
     const double *b_eff = fuse ? beta : one;
     const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
     if (tl) tl->start(s);
-    if (fwd && sweep_ok(bf, nrhs)) {
+    if (fwd && !lanes && sweep_ok(bf, nrhs)) {
         SweepTables &sw = bf.sweep;
         if (int64_t(sw.epoch + 1) * sw.max_wait >= (int64_t(1) << 30)) {
             launch_zero_i32(s, sw.counters.as<int32_t>(), sw.n_groups);
@@ -898,7 +960,7 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         if (tl) tl->launched(s, {-1, 3, 0, 0, 0, nr_per_lane(nrhs)});
         return;
     }
-    if (fwd && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min) {
+    if (fwd && lanes) {
         // right-hand sides across lanes, up to 64 per sweep, skeleton
         // vectors interleaved (entry-major) in the slot buffer
         const int es = bf.precision == BFGPU_C128 ? 16 : 8;

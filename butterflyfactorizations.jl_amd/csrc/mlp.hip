@@ -214,20 +214,33 @@ void mlp_upload_norm(bfgpu_mlp &m) {
     BFG_HIP(hipMemcpy(m.norm.p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
 }
 
+// Forward pass over n samples in chunks of at most INFER_CHUNK, so that the
+// activation workspace stays bounded (n can be every compression of a
+// level); raw network outputs into y (host).
+constexpr int64_t INFER_CHUNK = int64_t(1) << 20;
+static void forward_chunked(bfgpu_mlp &m, int64_t n, const float *features, float *y) {
+    hipStream_t s = m.ctx->stream;
+    DevBuf dx;
+    const int64_t ch = std::min(n, INFER_CHUNK);
+    dx.alloc(sizeof(float) * size_t(ch) * m.n_in);
+    for (int64_t i0 = 0; i0 < n; i0 += ch) {
+        const int64_t nc = std::min(ch, n - i0);
+        BFG_HIP(hipMemcpyAsync(dx.p, features + i0 * m.n_in, sizeof(float) * size_t(nc) * m.n_in,
+                               hipMemcpyHostToDevice, s));
+        MlpRun r(m, nc);
+        r.forward(s, dx.as<float>());
+        BFG_HIP(hipMemcpyAsync(y + i0, r.y, sizeof(float) * size_t(nc), hipMemcpyDeviceToHost, s));
+        BFG_HIP(hipStreamSynchronize(s));
+    }
+}
+
 int32_t mlp_predict_ranks(const bfgpu_mlp *mc, const std::vector<float> &feat, int64_t n,
                           std::vector<int32_t> &out) {
     bfgpu_mlp &m = const_cast<bfgpu_mlp &>(*mc);
     out.assign(size_t(n), 1);
     if (n == 0) return 0;
-    hipStream_t s = m.ctx->stream;
-    DevBuf dx;
-    dx.alloc(sizeof(float) * size_t(n) * m.n_in);
-    BFG_HIP(hipMemcpyAsync(dx.p, feat.data(), sizeof(float) * size_t(n) * m.n_in, hipMemcpyHostToDevice, s));
-    MlpRun r(m, n);
-    r.forward(s, dx.as<float>());
     std::vector<float> y(static_cast<size_t>(n));
-    BFG_HIP(hipMemcpyAsync(y.data(), r.y, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, s));
-    BFG_HIP(hipStreamSynchronize(s));
+    forward_chunked(m, n, feat.data(), y.data());
     for (int64_t i = 0; i < n; ++i) {
         float rk = y[i] * m.out_scale + m.out_shift;
         out[i] = std::max(1, int32_t(std::ceil(rk)));
@@ -347,14 +360,7 @@ extern "C" int bfgpu_mlp_forward(bfgpu_mlp *m, int64_t n, const float *features,
     BFG_REQUIRE(m && features && out && n >= 0, "mlp_forward: bad argument");
     BFG_REQUIRE(n < (int64_t(1) << 30), "mlp_forward: too many samples");
     if (n == 0) return BFGPU_OK;
-    hipStream_t s = m->ctx->stream;
-    DevBuf dx;
-    dx.alloc(sizeof(float) * size_t(n) * m->n_in);
-    BFG_HIP(hipMemcpyAsync(dx.p, features, sizeof(float) * size_t(n) * m->n_in, hipMemcpyHostToDevice, s));
-    MlpRun r(*m, n);
-    r.forward(s, dx.as<float>());
-    BFG_HIP(hipMemcpyAsync(out, r.y, sizeof(float) * size_t(n), hipMemcpyDeviceToHost, s));
-    BFG_HIP(hipStreamSynchronize(s));
+    forward_chunked(*m, n, features, out);
     for (int64_t i = 0; i < n; ++i) out[i] = out[i] * m->out_scale + m->out_shift;
     MLP_API_END
 }

@@ -47,9 +47,11 @@ const char *bfgpu_version(void);
 /* ---- context (device + stream) ---- */
 int bfgpu_ctx_create(int device, bfgpu_ctx **out);
 int bfgpu_ctx_destroy(bfgpu_ctx *ctx);
-/* Use an external stream (e.g. torch's current stream); NULL restores the
- * context's own stream. */
+/* Launch on an external stream (e.g. torch's current stream); NULL is the
+ * null (default) stream, which torch's default stream is.  reset_stream
+ * returns to the context's own (non-blocking) stream. */
 int bfgpu_ctx_set_stream(bfgpu_ctx *ctx, void *hip_stream);
+int bfgpu_ctx_reset_stream(bfgpu_ctx *ctx);
 int bfgpu_ctx_get_stream(bfgpu_ctx *ctx, void **hip_stream);
 int bfgpu_ctx_sync(bfgpu_ctx *ctx);
 
@@ -151,6 +153,8 @@ typedef struct {
      * kernel fill + pivoted QR (GPU, synchronised), interpolation matrices
      * (GPU), launch-table packing */
     double t_plan_ms, t_sample_ms, t_qr_ms, t_interp_ms, t_pack_ms;
+    int64_t device_bytes;  /* device memory the handle holds: factors, launch
+                              tables, skeleton-vector and staging buffers */
 } bfgpu_bf_info;
 int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
 
@@ -267,7 +271,12 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        complex FP32 factorizations multiply on the matrix
  *                        cores (v_mfma_f32_16x16x4_f32, small blocks that
  *                        read the same inputs stacked into 16-row tiles);
- *                        0: vector-ALU kernel */
+ *                        0: vector-ALU kernel
+ *   BFGPU_OPT_RHS_CHUNK  most right-hand sides per pass over the levels;
+ *                        0 (default): as many as the skeleton-vector buffer
+ *                        of one pass allows in 90 % of the free device
+ *                        memory (at most 64 with right-hand sides across
+ *                        lanes) */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -276,7 +285,8 @@ enum {
     BFGPU_OPT_TILE = 5,
     BFGPU_OPT_DENSE = 6,
     BFGPU_OPT_RHS_LANES = 7,
-    BFGPU_OPT_MFMA = 8
+    BFGPU_OPT_MFMA = 8,
+    BFGPU_OPT_RHS_CHUNK = 9
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

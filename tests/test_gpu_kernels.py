@@ -237,6 +237,19 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     valu = (g @ np.asfortranarray(X[:, :64])).astype(np.complex128)
     g.set_option("mfma", 1)
     assert rel(valu, want[:, :64]) < 2e-6
+    # passes over column blocks: 70 = 16 + 16 + 16 + 16 + 6 across lanes,
+    # and 20 = 7 + 7 + 6 with one column per RHS
+    Yc = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
+    g.set_option("rhs_chunk", 16)
+    got = g.mul(Yc.copy(order="F"), np.asfortranarray(X), a, b).astype(np.complex128)
+    g.set_option("rhs_chunk", 0)
+    assert rel(got, a * want + b * Yc.astype(np.complex128)) < 2e-6
+    g.set_option("rhs_lanes", 0)
+    g.set_option("rhs_chunk", 7)
+    got = (g @ np.asfortranarray(X[:, :20])).astype(np.complex128)
+    g.set_option("rhs_chunk", 0)
+    g.set_option("rhs_lanes", 16)
+    assert rel(got, want[:, :20]) < 2e-6
 
     say("far field sphere N=6000 fp32: matrix-core right-hand sides, partial sums")
     pts = fibonacci_sphere(6000, 4.0)

@@ -71,6 +71,12 @@ def test_from_host_fp64_matvec_matches_oracle(bfgpu, oracle, ctx):
     g.mul(yt, xt)
     torch.cuda.synchronize()
     assert rel(yt.cpu().numpy(), obf.matvec(x)) < 1e-12
+    # several columns on torch's default (null) stream, read back with no
+    # explicit synchronisation: the product is ordered before the copy
+    Xt = torch.from_numpy(np.ascontiguousarray(X.T)).cuda().t()
+    Yt = torch.zeros(3, 1024, dtype=torch.complex128, device="cuda").t()
+    g.mul(Yt, Xt)
+    assert rel(Yt.cpu().numpy(), obf.matvec(X)) < 1e-12
     # argument errors surface as exceptions
     with pytest.raises(bfgpu.ArgumentError):
         g.mul(np.zeros(10, np.complex128), x)
@@ -217,6 +223,12 @@ def test_single_block_65536_properties(bfgpu, oracle, ctx):
     g32 = g.convert(np.complex64)
     X = np.asfortranarray(crandn(rng, n, 64)).astype(np.complex64)
     Y = g32 @ X
+    # device tensors on torch's default stream, read back unsynchronised
+    import torch
+    Xt = torch.from_numpy(np.ascontiguousarray(X[:, :4].T)).cuda().t()
+    Yt = torch.zeros(4, n, dtype=torch.complex64, device="cuda").t()
+    g32.mul(Yt, Xt)
+    assert rel(Yt.cpu().numpy().astype(np.complex128), Y[:, :4].astype(np.complex128)) < 1e-6
     Y64 = g @ X.astype(np.complex128)
     assert rel(Y.astype(np.complex128), Y64) < 1e-5
     assert rel(Y[:, 0].astype(np.complex128), (g32 @ X[:, 0]).astype(np.complex128)) < 1e-6
