@@ -28,6 +28,7 @@ for p in (ROOT, os.path.join(ROOT, "butterflyfactorizations.jl_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 METRIC = "butterfly matvec ms & %HBM roofline, FP32-batched vs FP64 plain at 1e-4 err"
 
 
@@ -111,6 +112,28 @@ def kernel_name(d, op, real):
     if op != 0:
         return f"stage_kernel<{real}, {op}, {ink}, {outk}>"
     return f"{'fwdw_kernel' if variant == 1 else 'fwd_kernel'}<{real}, {ink}, {outk}, {nr}>"
+
+
+def multi_rhs(stats, nr, ms, launch_ms, desc):
+    """The 32-RHS FP32 sweep against both of its bounds: complex FMA work
+    (8 flops per factor entry and RHS) on the matrix cores (dense FP32
+    peak) and algorithmic bytes (factors once, skeleton vectors and x/y per
+    RHS) against HBM; per kernel template the time of one sweep."""
+    entries = sum(int(r[3]) for r in stats)
+    flops = 8.0 * nr * entries
+    by = sum(int(r[3]) * 8 + (int(r[8]) + int(r[9])) * 8 * nr for r in stats)
+    per = {}
+    for m, d in zip(launch_ms, desc):
+        name = kernel_name(d, 0, "float")
+        per[name] = per.get(name, 0.0) + float(m)
+    tf = flops / (ms * 1e-3) / 1e12
+    gbs = by / (ms * 1e-3) / 1e9
+    return {"rhs": nr, "ms": round(ms, 4), "flops": flops, "alg_bytes": by,
+            "mfma": {"achieved": round(tf, 2), "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / MFMA_FP32_PEAK_TFLOPS, 4)},
+            "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            "kernels_ms": {k: round(v, 4) for k, v in per.items()}}
 
 
 def dominant_kernel(launch_ms, desc, stats, es, real):
@@ -246,6 +269,7 @@ def main():
     Y = torch.zeros(nr, rows_n, dtype=torch.complex64, device="cuda").t()
     ms32_32 = timed(g32, X, Y, max(args.steps // 4, 3), 1)
     log(f"adjoint fp32 {ms32_adj:.4f} ms, 32 RHS {ms32_32:.4f} ms")
+    rhs_ms, rhs_desc = g32.launch_times(X, Y, iters=3, nrhs=nr)
 
     # ---- per-launch HIP-event timings on the context stream; roofline ----
     st_ms, _, tot_ms = g32.timed(x32, y32, iters=max(args.steps, 10))
@@ -330,6 +354,7 @@ def main():
                      "whole_matvec_gbs": round(whole, 1), "whole_matvec_frac": round(whole / HBM_PEAK_GBS, 4),
                      "kernels_ms_per_matvec": {k: round(v["ms"], 4) for k, v in per_kernel.items()}},
         "cpu_baseline": cpu,
+        "multi_rhs": multi_rhs(stats, nr, ms32_32, rhs_ms, rhs_desc),
         "fp64_ms": None if ms64 is None else round(ms64, 4),
         "fp32_adjoint_ms": round(ms32_adj, 4), "fp32_32rhs_ms": round(ms32_32, 4),
         "factorize_s": {"fp32": round(t_f32, 2), "fp64": None if t_f64 is None else round(t_f64, 2),
