@@ -263,8 +263,9 @@ def tree_parameters(tree=None, admissibility=None):
     return TreeParameters(1.0, 2.0, 2.0, 1.0, 4.0)
 
 
-def _opts(k, tol, rankestimator, adaptive, acctype, seed):
+def _opts(k, tol, rankestimator, adaptive, acctype, seed, workspace_bytes=0):
     o = L.Opts()
+    o.workspace_bytes = int(workspace_bytes)
     o.k = float(k)
     o.tol = float(tol)
     o.adaptive = 1 if adaptive else 0
@@ -641,9 +642,11 @@ class ButterflyFactorization:
 
 
 def assemble_BF(kernelmatrix, blktree, no, ns, k, tau, *, rankestimator=None, adaptive=True,
-                acctype=np.complex128, seed=0, ctx=None):
+                acctype=np.complex128, seed=0, workspace_bytes=0, ctx=None):
     """Butterfly factorization of the block (observer node `no`, source node
-    `ns`) of `kernelmatrix` (bfassembly.jl:48-200), computed on the GPU."""
+    `ns`) of `kernelmatrix` (bfassembly.jl:48-200), computed on the GPU.
+    `workspace_bytes` caps the device memory of kernel-matrix tiles per
+    batch of compressions (0: automatic)."""
     if not isinstance(kernelmatrix, HelmholtzKernelMatrix):
         raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix")
     if abs(kernelmatrix.k - k) > 1e-12 * abs(k):
@@ -651,7 +654,7 @@ def assemble_BF(kernelmatrix, blktree, no, ns, k, tau, *, rankestimator=None, ad
     ctx = ctx or default_context()
     if rankestimator is None:
         rankestimator = ButterflyRankEstimator(tree_parameters(blktree).Ctau)
-    o = _opts(k, tau, rankestimator, adaptive, acctype, seed)
+    o = _opts(k, tau, rankestimator, adaptive, acctype, seed, workspace_bytes)
     pairs = np.array([no, ns], np.int32)
     h = C.c_void_p()
     check(lib().bfgpu_factorize(ctx.handle, blktree.testcluster.handle, blktree.trialcluster.handle,
@@ -696,7 +699,7 @@ class PetrovGalerkinBF(ButterflyFactorization):
 
     def __init__(self, kernelmatrix, tree, k, *, tol=1e-3, admissibility=None, rankestimator=None,
                  acctype=np.complex128, minbflvl=3, adaptive=True, unbalancedints=False,
-                 leafcomp=True, leafimbalance=True, farfieldonly=True, seed=0, ctx=None):
+                 leafcomp=True, leafimbalance=True, farfieldonly=True, seed=0, workspace_bytes=0, ctx=None):
         if not farfieldonly:
             raise NotImplementedError("near-field assembly is not part of this library; "
                                       "pass farfieldonly=True")
@@ -708,7 +711,7 @@ class PetrovGalerkinBF(ButterflyFactorization):
         if far.shape[0] == 0:
             raise ArgumentError(1, "no admissible (far) blocks")
         self.farinteractions, self.nearinteractions = far, near
-        o = _opts(k, tol, rankestimator, adaptive, acctype, seed)
+        o = _opts(k, tol, rankestimator, adaptive, acctype, seed, workspace_bytes)
         pairs = np.ascontiguousarray(far, np.int32)
         h = C.c_void_p()
         check(lib().bfgpu_factorize(ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,

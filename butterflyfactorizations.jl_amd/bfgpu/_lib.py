@@ -33,7 +33,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("k", f64), ("tol", f64), ("adaptive", i32), ("rank_mode", i32),
         ("c_tau", f64), ("rmin", i32), ("c_geo", f64), ("c_eps", f64),
-        ("precision", i32), ("oversample", i32), ("seed", u64), ("mlp", vp),
+        ("precision", i32), ("oversample", i32), ("seed", u64), ("mlp", vp), ("workspace_bytes", i64),
     ]
 
 

@@ -129,6 +129,7 @@ static void check_opts(const bfgpu_opts *o) {
     BFG_REQUIRE(o->tol > 0.0 && o->tol < 1.0, "factorize: tol must lie in (0, 1)");
     BFG_REQUIRE(o->k > 0.0, "factorize: wavenumber must be positive");
     BFG_REQUIRE(o->rank_mode >= 0 && o->rank_mode <= 2, "factorize: unknown rank_mode");
+    BFG_REQUIRE(o->workspace_bytes >= 0, "factorize: workspace_bytes must be >= 0 (0: automatic)");
 }
 
 extern "C" int bfgpu_factorize(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,

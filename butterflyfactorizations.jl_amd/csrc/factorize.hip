@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <exception>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -245,10 +246,13 @@ struct Engine {
         std::vector<InterpTask> it;
         // kernel-matrix tiles of one batch: at most 32 GiB or 30 % of the
         // free device memory (besides what the tile buffer already holds)
-        size_t free_b = 0, total_b = 0;
-        BFG_HIP(hipMemGetInfo(&free_b, &total_b));
-        const int64_t z_budget = std::max<int64_t>(
-            int64_t(1) << 30, std::min<int64_t>(int64_t(32) << 30, int64_t(0.3 * double(free_b + d_Z.bytes))));
+        int64_t z_budget = o.workspace_bytes;
+        if (z_budget <= 0) {
+            size_t free_b = 0, total_b = 0;
+            BFG_HIP(hipMemGetInfo(&free_b, &total_b));
+            z_budget = std::max<int64_t>(int64_t(1) << 30,
+                                         std::min<int64_t>(int64_t(32) << 30, int64_t(0.3 * double(free_b + d_Z.bytes))));
+        }
         std::vector<int32_t> later;
         while (!pend.empty()) {
             auto th = clk::now();
@@ -309,31 +313,34 @@ struct Engine {
             launch_fill(ctx.stream, prec, d_test_pts.as<double>(), d_trial_pts.as<double>(),
                         d_rows.as<int32_t>(), d_cols.as<int32_t>(), d_fill.as<FillTask>(),
                         d_fill_end.as<int64_t>(), int32_t(ft.size()), zofs, d_Z.p, o.k);
-            // per-task tolerance: one launch per distinct eps (the batch may
-            // hold factorizations of different depth)
-            std::vector<double> eps_of(np);
-            for (size_t q = 0; q < np; ++q) eps_of[q] = cs[pend[q]].eps;
-            std::vector<double> uniq = eps_of;
-            std::sort(uniq.begin(), uniq.end());
-            uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+            // QR launches: one per (tolerance, kernel) group -- the batch may
+            // hold factorizations of different depth, and the blocks that
+            // fit a wavefront's LDS take the wavefront kernel while larger
+            // ones take the workgroup kernel, whatever else is in the batch
+            std::map<std::pair<double, int>, std::vector<int32_t>> groups;
+            for (size_t q = 0; q < np; ++q)
+                groups[{cs[pend[q]].eps, cpqr_wave_ok(prec, qt[q].m, qt[q].n) ? 1 : 0}].push_back(int32_t(q));
             hrank.assign(np, 0);
-            if (uniq.size() == 1) {
+            if (groups.size() == 1) {
                 launch_cpqr(ctx.stream, prec, d_Z.p, d_qr.as<QrTask>(), int32_t(np),
-                            d_piv.as<int32_t>(), d_rank.as<int32_t>(), uniq[0], max_n, max_m);
+                            d_piv.as<int32_t>(), d_rank.as<int32_t>(), groups.begin()->first.first, max_n, max_m);
                 BFG_HIP(hipMemcpyAsync(hrank.data(), d_rank.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx.stream));
             } else {
                 std::vector<QrTask> sub;
-                std::vector<int32_t> where;
                 DevBuf d_sub, d_subrank;
-                for (double e : uniq) {
-                    sub.clear();
-                    where.clear();
-                    for (size_t q = 0; q < np; ++q)
-                        if (eps_of[q] == e) { sub.push_back(qt[q]); where.push_back(int32_t(q)); }
+                for (const auto &gr : groups) {
+                    const std::vector<int32_t> &where = gr.second;
+                    sub.resize(where.size());
+                    int32_t gn = 1, gm = 1;
+                    for (size_t i = 0; i < where.size(); ++i) {
+                        sub[i] = qt[where[i]];
+                        gn = std::max(gn, sub[i].n);
+                        gm = std::max(gm, sub[i].m);
+                    }
                     upload(d_sub, sub, ctx.stream);
                     d_subrank.reserve(sizeof(int32_t) * sub.size());
                     launch_cpqr(ctx.stream, prec, d_Z.p, d_sub.as<QrTask>(), int32_t(sub.size()),
-                                d_piv.as<int32_t>(), d_subrank.as<int32_t>(), e, max_n, max_m);
+                                d_piv.as<int32_t>(), d_subrank.as<int32_t>(), gr.first.first, gn, gm);
                     std::vector<int32_t> r(sub.size());
                     BFG_HIP(hipMemcpyAsync(r.data(), d_subrank.p, sizeof(int32_t) * r.size(), hipMemcpyDeviceToHost, ctx.stream));
                     BFG_HIP(hipStreamSynchronize(ctx.stream));

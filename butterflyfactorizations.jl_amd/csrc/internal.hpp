@@ -321,6 +321,8 @@ struct QrTask {
     int32_t pofs;       // pivot array offset (n entries)
     int32_t pad;
 };
+// Blocks the wavefront QR kernel takes (the rest: a workgroup per block).
+bool cpqr_wave_ok(int prec, int32_t m, int32_t n);
 void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
                  int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m);
 

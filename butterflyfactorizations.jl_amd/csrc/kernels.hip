@@ -429,15 +429,19 @@ __global__ __launch_bounds__(256) void cpqr_wave_kernel(C<R> *Z, const QrTask *_
     if (lane == 0) rank_out[task] = rank;
 }
 
+bool cpqr_wave_ok(int prec, int32_t m, int32_t n) {
+    return n <= 64 && m <= (prec == BFGPU_C128 ? 32 : 64);
+}
+
 void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
                  int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m) {
     if (n_tasks <= 0) return;
     const size_t es = prec == BFGPU_C128 ? 16 : 8;
     const size_t region = size_t(std::max(max_m, 1) | 1) * size_t(std::max(max_n, 1));  // entries per block
-    const size_t per_wave = region * es;
-    if (max_n <= 64 && per_wave <= 32 * 1024) {
+    const size_t per_wave = region * es;  // at most 65 x 64 x 8 or 33 x 64 x 16 bytes
+    if (cpqr_wave_ok(prec, max_m, max_n)) {
         // blocks per workgroup: as many as 64 KiB of LDS hold, at most 4
-        const int wpb = int(std::min<size_t>(4, (64 * 1024) / per_wave));
+        const int wpb = int(std::max<size_t>(1, std::min<size_t>(4, (64 * 1024) / per_wave)));
         const size_t lds = per_wave * size_t(wpb);
         // chunks of tasks: a dispatch holds fewer than 2^32 work-items
         for (int32_t t0 = 0; t0 < n_tasks; t0 += TASK_CHUNK) {

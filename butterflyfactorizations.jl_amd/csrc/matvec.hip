@@ -1215,10 +1215,17 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                         sh.qw4 = int8_t(q4);
                         consider(3, sh);
                     }
-        if (bv >= 2) {  // the other task-to-XCD mapping for the best shape
+        if (bv >= 2) {  // the other task-to-XCD mapping, then other occupancies, for the best shape
             TileShape sh = bs;
             sh.xcd = int8_t(1 - sh.xcd);
             consider(bv, sh);
+            const TileShape b0 = bs;
+            for (int wg : {2, 3, 4, 5, 6, 12, 16}) {
+                if (wg == b0.wg_per_cu) continue;
+                sh = b0;
+                sh.wg_per_cu = int16_t(wg);
+                consider(bv, sh);
+            }
         }
         t.variant = bv;
         t.tile = bs;

@@ -115,6 +115,9 @@ typedef struct {
     int32_t oversample;  /* rows added to a learned rank prediction */
     uint64_t seed;       /* row-sample seed */
     const bfgpu_mlp *mlp;/* rank predictor for BFGPU_RANK_LEARNED */
+    int64_t workspace_bytes; /* most device bytes of kernel-matrix tiles per
+                                batch of compressions (0: min(32 GiB, 30 % of
+                                the free device memory)) */
 } bfgpu_opts;
 
 /* assemble_BF (src/ButterflyFactorization/bfassembly.jl:48-200) for n_pairs
