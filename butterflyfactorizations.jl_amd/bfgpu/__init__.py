@@ -11,6 +11,7 @@ and the tree interface ``cluster_*`` of src/treeinterface.jl.  Indices are
 path.
 """
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -34,7 +35,7 @@ __all__ = [
     "HelmholtzKernelMatrix", "ButterflyRankEstimator", "GeometricRankEstimator",
     "LearnedRankEstimator", "TreeParameters", "tree_parameters", "assemble_BF",
     "ButterflyFactorization", "PetrovGalerkinBF", "mul", "nearandfar", "isFarFunctor",
-    "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError",
+    "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError", "load",
 ]
 
 
@@ -570,6 +571,11 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_stage_stats(self.handle, C.byref(n), ptr(st, C.c_int64)))
         return st
 
+    def save(self, path):
+        """Write the operator (blocks, logs and factors in their stored
+        precision) to `path`; `bfgpu.load` reads it back."""
+        check(lib().bfgpu_bf_save(self.handle, os.fsencode(path)))
+
     def convert(self, acctype):
         """Copy with factor storage in another precision."""
         prec = L.BFGPU_C128 if np.dtype(acctype) == np.complex128 else L.BFGPU_C64
@@ -660,6 +666,16 @@ def assemble_BF(kernelmatrix, blktree, no, ns, k, tau, *, rankestimator=None, ad
     check(lib().bfgpu_factorize(ctx.handle, blktree.testcluster.handle, blktree.trialcluster.handle,
                                 1, ptr(pairs, C.c_int32), C.byref(o), C.byref(h)))
     return ButterflyFactorization(h, ctx, blktree, k, tau, _keep=(blktree, rankestimator))
+
+
+def load(path, tree, ctx=None):
+    """An operator written by `ButterflyFactorization.save`, read onto the
+    device for the same block tree (observer and source trees; checked)."""
+    ctx = ctx or default_context()
+    h = C.c_void_p()
+    check(lib().bfgpu_bf_load(ctx.handle, tree.testcluster.handle, tree.trialcluster.handle, os.fsencode(path),
+                              C.byref(h)))
+    return ButterflyFactorization(h, ctx, tree, None, None, _keep=(tree,))
 
 
 def mul(y, A, x, alpha=1.0, beta=0.0):

@@ -67,6 +67,8 @@ _SIGS = {
     "bfgpu_ctx_destroy": (i32, [vp]),
     "bfgpu_ctx_set_stream": (i32, [vp, vp]),
     "bfgpu_ctx_reset_stream": (i32, [vp]),
+    "bfgpu_bf_save": (i32, [vp, C.c_char_p]),
+    "bfgpu_bf_load": (i32, [vp, vp, vp, C.c_char_p, P(vp)]),
     "bfgpu_ctx_get_stream": (i32, [vp, P(vp)]),
     "bfgpu_ctx_sync": (i32, [vp]),
     "bfgpu_tree_build": (i32, [P(f64), i64, i32, i32, i32, P(vp)]),

@@ -161,6 +161,17 @@ typedef struct {
 } bfgpu_bf_info;
 int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
 
+/* Serialized factorizations (no single counterpart in the reference: the
+ * reference keeps factorizations in memory only; this lets a factorization
+ * be cached between runs).  save writes every factorization's blocks in the
+ * reference layout (bfstructs.jl:21-147) with the skeleton and compression
+ * logs and the factor buffers in their stored precision; load reads it
+ * onto the device for the same observer / source trees (checked by a
+ * fingerprint of their numbering) and rebuilds the launch tables. */
+int bfgpu_bf_save(const bfgpu_bf *bf, const char *path);
+int bfgpu_bf_load(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial, const char *path,
+                  bfgpu_bf **out);
+
 /* Export factorization `index` in the reference layout (complex128 data);
  * sizes from bfgpu_bf_counts. */
 int bfgpu_bf_counts(const bfgpu_bf *bf, int32_t index, int32_t *n_stages, int64_t *n_blocks,

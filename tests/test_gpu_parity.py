@@ -258,3 +258,52 @@ def test_factorization_in_small_batches_is_identical(bfgpu, oracle, ctx):
     _compare_factorizations(e2, e1, T_tol=1e-12)
     x = crandn(np.random.default_rng(4), 4096)
     assert rel(g2 @ x, g1 @ x) < 1e-12
+
+
+def test_saved_factorization_applies_identically(bfgpu, oracle, ctx, tmp_path):
+    """A factorization written to a file and read back (blocks in the
+    reference layout, logs, factors in their stored precision) exports the
+    same blocks and applies bit for bit as the one that was saved: FP64
+    single block and FP32 far field; a file read against other trees, a
+    truncated file and a file that is not one are rejected."""
+    say("save / load fp64 two squares 4096")
+    obs, src = two_squares(4096, seed=7)
+    bt = trees(bfgpu, obs, src)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+    path = str(tmp_path / "squares.bfgpu")
+    g.save(path)
+    h = bfgpu.load(path, bt, ctx=ctx)
+    _compare_factorizations(h.export(), g.export(), T_tol=1e-15)
+    x = crandn(np.random.default_rng(9), 4096)
+    assert np.array_equal(h @ x, g @ x)
+    assert np.array_equal(h.adjoint() @ x, g.adjoint() @ x)
+
+    say("save / load fp32 sphere 6000 far field")
+    pts = fibonacci_sphere(6000, 4.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    B = bfgpu.BlockTree(T, T)
+    op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), B, K, tol=1e-4,
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
+                                ctx=ctx)
+    p2 = str(tmp_path / "sphere.bfgpu")
+    op.save(p2)
+    op2 = bfgpu.load(p2, B, ctx=ctx)
+    assert op2.eltype == np.complex64 and op2.info().n_bf == len(op.farinteractions)
+    xs = crandn(np.random.default_rng(10), 6000).astype(np.complex64)
+    assert np.array_equal(op2 @ xs, op @ xs)
+    Xs = np.asfortranarray(crandn(np.random.default_rng(11), 6000, 40)).astype(np.complex64)
+    assert np.array_equal(op2 @ Xs, op @ Xs)
+
+    with pytest.raises(bfgpu.ArgumentError):  # other trees
+        bfgpu.load(path, B, ctx=ctx)
+    with open(p2, "rb") as f:
+        head = f.read(4096)
+    short = str(tmp_path / "short.bfgpu")
+    with open(short, "wb") as f:
+        f.write(head)
+    with pytest.raises(bfgpu.ArgumentError):
+        bfgpu.load(short, B, ctx=ctx)
+    with open(short, "wb") as f:
+        f.write(b"not a factorization")
+    with pytest.raises(bfgpu.ArgumentError):
+        bfgpu.load(short, B, ctx=ctx)
