@@ -35,7 +35,7 @@ __all__ = [
     "HelmholtzKernelMatrix", "ButterflyRankEstimator", "GeometricRankEstimator",
     "LearnedRankEstimator", "TreeParameters", "tree_parameters", "assemble_BF",
     "ButterflyFactorization", "PetrovGalerkinBF", "mul", "nearandfar", "isFarFunctor",
-    "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError", "load",
+    "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError", "load", "recompress_BF",
 ]
 
 
@@ -571,6 +571,13 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_stage_stats(self.handle, C.byref(n), ptr(st, C.c_int64)))
         return st
 
+    def recompress(self, tau):
+        """recompress_BF (algrecomp.jl:19-148): a new operator with its R
+        levels recompressed at relative tolerance `tau`, on the GPU."""
+        h = C.c_void_p()
+        check(lib().bfgpu_bf_recompress(self.handle, float(tau), C.byref(h)))
+        return ButterflyFactorization(h, self.ctx, self.tree, self.k, tau, self.op, _keep=self._H.keep)
+
     def save(self, path):
         """Write the operator (blocks, logs and factors in their stored
         precision) to `path`; `bfgpu.load` reads it back."""
@@ -666,6 +673,11 @@ def assemble_BF(kernelmatrix, blktree, no, ns, k, tau, *, rankestimator=None, ad
     check(lib().bfgpu_factorize(ctx.handle, blktree.testcluster.handle, blktree.trialcluster.handle,
                                 1, ptr(pairs, C.c_int32), C.byref(o), C.byref(h)))
     return ButterflyFactorization(h, ctx, blktree, k, tau, _keep=(blktree, rankestimator))
+
+
+def recompress_BF(bf, tau):
+    """recompress_BF(Butterfly, tau) (algrecomp.jl:19-33)."""
+    return bf.recompress(tau)
 
 
 def load(path, tree, ctx=None):

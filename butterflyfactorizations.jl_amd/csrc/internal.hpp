@@ -324,7 +324,8 @@ struct QrTask {
 // Blocks the wavefront QR kernel takes (the rest: a workgroup per block).
 bool cpqr_wave_ok(int prec, int32_t m, int32_t n);
 void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
-                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m);
+                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m,
+                 double *tau = nullptr);
 
 struct InterpTask {
     int64_t zofs;       // R (in Z workspace, ld = m)

@@ -8,42 +8,11 @@
 
 #include <cmath>
 
+#include "device.hpp"
 #include "internal.hpp"
 
 namespace bfg {
 
-template <typename R> struct C { R x, y; };
-
-template <typename R> __device__ __forceinline__ C<R> cmk(R a, R b) { return C<R>{a, b}; }
-template <typename R> __device__ __forceinline__ C<R> cadd(C<R> a, C<R> b) { return {a.x + b.x, a.y + b.y}; }
-template <typename R> __device__ __forceinline__ C<R> csub(C<R> a, C<R> b) { return {a.x - b.x, a.y - b.y}; }
-template <typename R> __device__ __forceinline__ C<R> cmul(C<R> a, C<R> b) {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-// conj(a) * b
-template <typename R> __device__ __forceinline__ C<R> cmulc(C<R> a, C<R> b) {
-    return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
-}
-// acc += a * b
-template <typename R> __device__ __forceinline__ void cfma(C<R> &acc, C<R> a, C<R> b) {
-    acc.x = fma(a.x, b.x, acc.x);
-    acc.x = fma(-a.y, b.y, acc.x);
-    acc.y = fma(a.x, b.y, acc.y);
-    acc.y = fma(a.y, b.x, acc.y);
-}
-// acc += conj(a) * b
-template <typename R> __device__ __forceinline__ void cfmac(C<R> &acc, C<R> a, C<R> b) {
-    acc.x = fma(a.x, b.x, acc.x);
-    acc.x = fma(a.y, b.y, acc.x);
-    acc.y = fma(a.x, b.y, acc.y);
-    acc.y = fma(-a.y, b.x, acc.y);
-}
-
-template <typename T> __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // Helmholtz tiles.  Z[i + m j] = exp(-i k r) / (4 pi r), r = |obs[rows[i]] -
@@ -121,7 +90,7 @@ void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *s
 template <typename R>
 __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__restrict__ tasks,
                                                    int32_t *piv_all, int32_t *rank_out,
-                                                   double rtol) {
+                                                   double rtol, double *tau_out) {
     extern __shared__ double sm[];
     const QrTask tk = tasks[blockIdx.x];
     const int m = tk.m, n = tk.n;
@@ -230,6 +199,10 @@ __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__rest
             }
         }
         __syncthreads();
+        if (tau_out && tid == 0) {
+            tau_out[2 * (int64_t(tk.pofs) + k)] = s_tau_x;
+            tau_out[2 * (int64_t(tk.pofs) + k) + 1] = s_tau_y;
+        }
         const C<R> scal = {R(s_scal_x), R(s_scal_y)};
         const C<R> ctau = {R(s_tau_x), R(-s_tau_y)};  // conj(tau)
         const bool apply = !(s_tau_x == 0.0 && s_tau_y == 0.0);
@@ -296,7 +269,7 @@ __global__ __launch_bounds__(256) void cpqr_kernel(C<R> *Z, const QrTask *__rest
 template <typename R>
 __global__ __launch_bounds__(256) void cpqr_wave_kernel(C<R> *Z, const QrTask *__restrict__ tasks, int32_t n_tasks,
                                                         int32_t *piv_all, int32_t *rank_out, double rtol,
-                                                        int32_t region) {
+                                                        int32_t region, double *tau_out) {
     extern __shared__ char smraw[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t task = int64_t(blockIdx.x) * (blockDim.x >> 6) + wave;
@@ -388,6 +361,10 @@ __global__ __launch_bounds__(256) void cpqr_wave_kernel(C<R> *Z, const QrTask *_
             sy = -di / den;
         }
         const bool apply = !(tx == 0.0 && ty == 0.0);
+        if (tau_out && lane == 0) {
+            tau_out[2 * (int64_t(tk.pofs) + k)] = tx;
+            tau_out[2 * (int64_t(tk.pofs) + k) + 1] = ty;
+        }
         const C<R> scal = {R(sx), R(sy)};
         const C<R> ctau = {R(tx), R(-ty)};  // conj(tau)
         sync();  // every lane has read a[k] before it is overwritten
@@ -434,7 +411,7 @@ bool cpqr_wave_ok(int prec, int32_t m, int32_t n) {
 }
 
 void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,
-                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m) {
+                 int32_t *piv, int32_t *rank, double rtol, int32_t max_n, int32_t max_m, double *tau) {
     if (n_tasks <= 0) return;
     const size_t es = prec == BFGPU_C128 ? 16 : 8;
     const size_t region = size_t(std::max(max_m, 1) | 1) * size_t(std::max(max_n, 1));  // entries per block
@@ -449,10 +426,10 @@ void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t 
             dim3 grid(unsigned((int64_t(nt) + wpb - 1) / wpb));
             if (prec == BFGPU_C128)
                 hipLaunchKernelGGL(cpqr_wave_kernel<double>, grid, dim3(64 * wpb), lds, s, (C<double> *)Z, tasks + t0,
-                                   nt, piv, rank + t0, rtol, int32_t(region));
+                                   nt, piv, rank + t0, rtol, int32_t(region), tau);
             else
                 hipLaunchKernelGGL(cpqr_wave_kernel<float>, grid, dim3(64 * wpb), lds, s, (C<float> *)Z, tasks + t0,
-                                   nt, piv, rank + t0, rtol, int32_t(region));
+                                   nt, piv, rank + t0, rtol, int32_t(region), tau);
             BFG_HIP(hipGetLastError());
         }
         return;
@@ -463,10 +440,10 @@ void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t 
         const int32_t nt = std::min(TASK_CHUNK, n_tasks - t0);
         if (prec == BFGPU_C128)
             hipLaunchKernelGGL(cpqr_kernel<double>, dim3(nt), dim3(256), lds, s,
-                               (C<double> *)Z, tasks + t0, piv, rank + t0, rtol);
+                               (C<double> *)Z, tasks + t0, piv, rank + t0, rtol, tau);
         else
             hipLaunchKernelGGL(cpqr_kernel<float>, dim3(nt), dim3(256), lds, s,
-                               (C<float> *)Z, tasks + t0, piv, rank + t0, rtol);
+                               (C<float> *)Z, tasks + t0, piv, rank + t0, rtol, tau);
         BFG_HIP(hipGetLastError());
     }
 }

@@ -161,6 +161,12 @@ typedef struct {
 } bfgpu_bf_info;
 int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
 
+/* recompress_BF (src/Butterflyalgebra/algrecomp.jl:19-148): a new operator
+ * whose R levels are recompressed algebraically at relative tolerance tau
+ * (right recompression, then left recompression through the adjoint);
+ * every factorization of the operator, on the GPU.  Q and P are kept. */
+int bfgpu_bf_recompress(bfgpu_bf *bf, double tau, bfgpu_bf **out);
+
 /* Serialized factorizations (no single counterpart in the reference: the
  * reference keeps factorizations in memory only; this lets a factorization
  * be cached between runs).  save writes every factorization's blocks in the

@@ -307,3 +307,52 @@ def test_saved_factorization_applies_identically(bfgpu, oracle, ctx, tmp_path):
         f.write(b"not a factorization")
     with pytest.raises(bfgpu.ArgumentError):
         bfgpu.load(short, B, ctx=ctx)
+
+
+def test_gpu_recompression_matches_oracle(bfgpu, oracle, ctx):
+    """recompress_BF (algrecomp.jl:19-148) on the GPU against the oracle's
+    restatement applied to the same factorization: identical block tables
+    (keys and the new sizes) and FP64 data to 1e-9; the reference's bars
+    (test_algrecomp.jl: assembled at 1e-4, recompressed at 1e-2, error
+    < 10 x 1e-2 against the dense kernel, storage not larger); tau = 0
+    keeps the operator to 1e-12; the adjoint of the result is its own
+    transformation; the FP32 far field of a sphere recompresses as well."""
+    from oracle import recompress as orc
+    say("recompress fp64 two squares 4096")
+    obs, src = two_squares(4096, seed=2)
+    bt = trees(bfgpu, obs, src)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+    ex = g.export()
+    rng = np.random.default_rng(13)
+    x = crandn(rng, 4096)
+    rows = np.sort(rng.choice(4096, 256, replace=False))
+    ye = oracle.helmholtz(K, obs, rows, src, np.arange(4096)) @ x
+    for tau in (1e-2, 1e-5):
+        h = bfgpu.recompress_BF(g, tau)
+        gex, oex = h.export(), orc.recompress(ex, tau)
+        np.testing.assert_array_equal(gex["table"][:, :8], oex["table"][:, :8])
+        assert rel(gex["data"], oex["data"]) < 1e-9
+        assert h.info().factor_bytes <= g.info().factor_bytes
+        y = h @ x
+        assert rel(y[rows], ye) < 10 * max(tau, 1e-4)
+        imp = oracle.import_bf(bt.testcluster, bt.trialcluster, gex)
+        assert rel(y, imp.matvec(x)) < 1e-12
+        assert rel(h.adjoint() @ x, imp.transform(1).matvec(x)) < 1e-12
+    h0 = g.recompress(0.0)
+    assert rel(h0 @ x, g @ x) < 1e-12
+
+    say("recompress fp32 sphere 6000 far field")
+    pts = fibonacci_sphere(6000, 4.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
+                                ctx=ctx)
+    rop = op.recompress(1e-3)
+    assert rop.info().factor_bytes <= op.info().factor_bytes and rop.info().n_bf == len(op.farinteractions)
+    xs = crandn(rng, 6000).astype(np.complex64)
+    y0 = (op @ xs).astype(np.complex128)
+    assert rel((rop @ xs).astype(np.complex128), y0) < 1e-2
+    # FP32 factors: the block keys match the oracle's (ranks may differ at
+    # the truncation threshold between FP32 and FP64 arithmetic)
+    oex = orc.recompress(op.export(0), 1e-3)
+    np.testing.assert_array_equal(rop.export(0)["table"][:, :5], oex["table"][:, :5])
