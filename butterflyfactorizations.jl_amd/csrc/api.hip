@@ -690,8 +690,10 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         release_graph(bf);
         break;
     case BFGPU_OPT_MFMA:
-        BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: mfma is 0 or 1");
-        bf.use_mfma = value == 1;
+        BFG_REQUIRE(value == 0 || value == 1 || (value >= 16 && value <= MTILE_ROWS && value % 16 == 0),
+                    "bf_set_option: mfma is 0, 1 or a tile height of 16, 32, 48 or 64 rows");
+        bf.use_mfma = value != 0;
+        if (value >= 16) bf.mfma_rows = value;
         release_graph(bf);
         break;
     case BFGPU_OPT_RHS_CHUNK:
@@ -740,7 +742,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_TILE:
     case BFGPU_OPT_DENSE: *value = bf.fwd.empty() ? 0 : int32_t(pack_tile(bf.fwd[0].tile)); break;
     case BFGPU_OPT_RHS_LANES: *value = bf.rhs_lanes_min; break;
-    case BFGPU_OPT_MFMA: *value = bf.use_mfma ? 1 : 0; break;
+    case BFGPU_OPT_MFMA: *value = bf.use_mfma ? bf.mfma_rows : 0; break;
     case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }

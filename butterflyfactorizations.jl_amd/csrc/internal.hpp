@@ -204,9 +204,11 @@ struct StageTable {
     std::unique_ptr<StageTable> pre;
     // matrix-core tiles of the multi-RHS kernel (built on first use):
     // int4 {first, count, row0, rows} over mtasks, the tasks in stacked
-    // order
+    // order; tiles sorted by their number of 16-row blocks
     bool mfma_built = false;
+    int32_t mfma_rows = 0;  // tile height the tiles were built for
     int32_t n_mtiles = 0;
+    int32_t mtile_ofs[5] = {0, 0, 0, 0, 0};  // tiles of 1, 2, 3, 4 row blocks: [ofs[rb-1], ofs[rb])
     DevBuf mtiles, mtasks;
 };
 
@@ -242,6 +244,7 @@ struct BF {
     bool use_graph = false;       // replay per-level launches from a HIP graph
     int32_t rhs_lanes_min = 32;   // forward matvecs with >= this many RHS: RHS across lanes (0: never)
     bool use_mfma = true;         // RHS across lanes in complex FP32: matrix-core kernel
+    int32_t mfma_rows = 16;       // rows of its tiles (16, 32, 48 or 64)
     int32_t rhs_chunk = 0;        // most right-hand sides per pass (0: as device memory allows)
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
@@ -256,6 +259,9 @@ struct BF {
     double t_factor_ms = 0.0;
     double t_plan_ms = 0.0, t_sample_ms = 0.0, t_qr_ms = 0.0, t_interp_ms = 0.0, t_pack_ms = 0.0;
 };
+
+// Most rows of a matrix-core tile (4 blocks of 16).
+constexpr int32_t MTILE_ROWS = 64;
 
 // Most tasks per dispatch of the kernels that give every task a workgroup
 // (a dispatch holds fewer than 2^32 work-items in x).
@@ -354,8 +360,8 @@ void launch_rhs(hipStream_t s, int prec, const StageTable &t, const void *factor
 void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, void *slots, int32_t ns,
                      int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
                      const double *beta, bool fuse, int n_cu);
-// Matrix-core tiles of a forward table (matvec.hip).
-void build_mfma(StageTable &t);
+// Matrix-core tiles of a forward table, at most `rows` rows each (matvec.hip).
+void build_mfma(StageTable &t, int32_t rows);
 void launch_gather_rhs(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm, int32_t lo,
                        int32_t hi, void *dst, int32_t ns, int nc);
 void launch_gather(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm,

@@ -1365,11 +1365,12 @@ __global__ __launch_bounds__(256) void rhs_kernel(
 // ---------------------------------------------------------------------------
 // Multi-RHS forward kernel on the matrix cores (complex FP32).  Skeleton
 // vectors are interleaved as for rhs_kernel (entry e of right-hand side r at
-// slots[e * ns + r], ns = 16 NT).  One wavefront per tile of at most 16
+// slots[e * ns + r], ns = 16 NT).  One wavefront per tile of at most 64
 // output rows: either rows [row0, row0 + rows) of one item, summed over its
 // contributions, or whole items stacked on top of each other that read the
 // same input entries (one contribution each, same in_ofs and n), so small
-// blocks fill the 16 rows of the product.  Per step of 4 input entries the
+// blocks fill the rows of the product; a tile holds RB row blocks of 16
+// (RB = 1..4, up to 64 rows), which share every input load.  Per step of 4 input entries the
 // lanes load A[i][k] (lane i + 16 k: row i of the tile, entry k) and, per
 // group of 16 right-hand sides, B[k][j] (lane j + 16 k), and issue
 // v_mfma_f32_16x16x4_f32 four times per group (re*re, -im*im into the real
@@ -1381,7 +1382,7 @@ __global__ __launch_bounds__(256) void rhs_kernel(
 // ---------------------------------------------------------------------------
 typedef float bfg_v4f __attribute__((ext_vector_type(4)));
 
-template <int OUTK, int NT>
+template <int OUTK, int NT, int RB>
 __global__ __launch_bounds__(256) void mfma_rhs_kernel(
     const int4 *__restrict__ tiles, int32_t n_tiles, const MvTaskC *__restrict__ mtasks,
     const MvContrib *__restrict__ contribs, const C<float> *__restrict__ F, C<float> *__restrict__ slots,
@@ -1395,9 +1396,9 @@ __global__ __launch_bounds__(256) void mfma_rhs_kernel(
     int64_t t = int64_t(blockIdx.x) * 4 + wave;
     int4 tl_next = t < n_tiles ? tiles[t] : int4{0, 0, 0, 0};
     for (; t < n_tiles; t += n_waves) {
-        const int4 tl = tl_next;  // first stacked task, task count, first row, rows
+        const int4 tl = tl_next;  // first stacked task, task count, first row, rows (<= 16 RB)
         if (t + n_waves < n_tiles) tl_next = tiles[t + n_waves];
-        // the tile's tasks, one per lane (at most 16)
+        // the tile's tasks, one per lane (at most 64)
         int64_t qf = 0;
         int32_t qm = 0, qo = 0;
         if (l < tl.y) {
@@ -1406,32 +1407,48 @@ __global__ __launch_bounds__(256) void mfma_rhs_kernel(
             qo = mtasks[tl.x + l].out_ofs;
         }
         const MvTaskC tk0 = mtasks[tl.x];
-        // row li of the tile: its block and output entry
-        int64_t rbase = 0;
-        int32_t rld = 1, rout = tk0.out_ofs + tl.z + li;
-        bool rok = li < tl.w;
+        // row 16 rb + li of the tile: its block and output entry
+        int64_t rbase[RB];
+        int32_t rld[RB], rout[RB];
+        bool rok[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) {
+            rbase[rb] = 0;
+            rld[rb] = 1;
+            rout[rb] = tk0.out_ofs + tl.z + 16 * rb + li;
+            rok[rb] = 16 * rb + li < tl.w;
+        }
         if (tl.y > 1) {
-            rok = false;
+#pragma unroll
+            for (int rb = 0; rb < RB; ++rb) rok[rb] = false;
             int start = 0;
             for (int q = 0; q < tl.y; ++q) {
                 const int32_t mq = __builtin_amdgcn_readlane(qm, q);
-                if (li >= start && li < start + mq) {
-                    rok = true;
-                    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint64_t(qf)), q));
-                    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint64_t(qf) >> 32), q));
-                    rbase = int64_t((uint64_t(hi) << 32) | lo) + (li - start);
-                    rld = mq;
-                    rout = __builtin_amdgcn_readlane(qo, q) + (li - start);
+                const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint64_t(qf)), q));
+                const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint64_t(qf) >> 32), q));
+                const int64_t fq = int64_t((uint64_t(hi) << 32) | lo);
+                const int32_t oq = __builtin_amdgcn_readlane(qo, q);
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    const int ri = 16 * rb + li;
+                    if (ri >= start && ri < start + mq) {
+                        rok[rb] = true;
+                        rbase[rb] = fq + (ri - start);
+                        rld[rb] = mq;
+                        rout[rb] = oq + (ri - start);
+                    }
                 }
                 start += mq;
             }
         }
-        bfg_v4f accr[NT], acci[NT];
+        bfg_v4f accr[RB][NT], acci[RB][NT];
 #pragma unroll
-        for (int g = 0; g < NT; ++g) {
-            accr[g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
-            acci[g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
-        }
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+            for (int g = 0; g < NT; ++g) {
+                accr[rb][g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+                acci[rb][g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+            }
         // contributions; the first is inline in the task, the next one's
         // descriptor is loaded while the current one is multiplied
         int64_t cf = tk0.fofs;
@@ -1447,43 +1464,62 @@ __global__ __launch_bounds__(256) void mfma_rhs_kernel(
             }
             if (cf < 0) {  // identity (single-task tiles only): add the input rows
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = 4 * lk + v;
-                    if (r < tl.w) {
+                for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                        for (int g = 0; g < NT; ++g) {
-                            const C<float> xv = in[int64_t(ci + tl.z + r) * ns + 16 * g + li];
-                            accr[g][v] += xv.x;
-                            acci[g][v] += xv.y;
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = 16 * rb + 4 * lk + v;
+                        if (r < tl.w) {
+#pragma unroll
+                            for (int g = 0; g < NT; ++g) {
+                                const C<float> xv = in[int64_t(ci + tl.z + r) * ns + 16 * g + li];
+                                accr[rb][g][v] += xv.x;
+                                acci[rb][g][v] += xv.y;
+                            }
                         }
                     }
-                }
             } else {
-                const C<float> *ap = F + (tl.y == 1 ? cf + tl.z + li : rbase);
-                const int64_t ald = tl.y == 1 ? cm : rld;
+                const C<float> *ap[RB];
+                int64_t ald[RB];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) {
+                    ap[rb] = F + (tl.y == 1 ? cf + tl.z + 16 * rb + li : rbase[rb]);
+                    ald[rb] = tl.y == 1 ? cm : rld[rb];
+                }
                 const C<float> *xin = in + int64_t(ci) * ns + li;
                 // step k0 multiplies entries k0..k0+3 while k0+4..k0+7 load
-                C<float> a = C<float>{0.f, 0.f}, b[NT];
-                if (rok && lk < cn) a = ap[int64_t(lk) * ald];
+                C<float> a[RB], b[NT];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb)
+                    a[rb] = (rok[rb] && lk < cn) ? ap[rb][int64_t(lk) * ald[rb]] : C<float>{0.f, 0.f};
 #pragma unroll
                 for (int g = 0; g < NT; ++g) b[g] = lk < cn ? xin[int64_t(lk) * ns + 16 * g] : C<float>{0.f, 0.f};
                 for (int k0 = 0; k0 < cn; k0 += 4) {
                     const int kn = k0 + 4 + lk;
-                    C<float> an = C<float>{0.f, 0.f}, bn[NT];
-                    if (rok && kn < cn) an = ap[int64_t(kn) * ald];
+                    C<float> an[RB], bn[NT];
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb)
+                        an[rb] = (rok[rb] && kn < cn) ? ap[rb][int64_t(kn) * ald[rb]] : C<float>{0.f, 0.f};
 #pragma unroll
                     for (int g = 0; g < NT; ++g)
                         bn[g] = kn < cn ? xin[int64_t(kn) * ns + 16 * g] : C<float>{0.f, 0.f};
-                    const float nai = -a.y;
 #pragma unroll
-                    for (int g = 0; g < NT; ++g) accr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[g].x, accr[g], 0, 0, 0);
+                    for (int rb = 0; rb < RB; ++rb) {
+                        const float nai = -a[rb].y;
 #pragma unroll
-                    for (int g = 0; g < NT; ++g) acci[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[g].y, acci[g], 0, 0, 0);
+                        for (int g = 0; g < NT; ++g)
+                            accr[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb].x, b[g].x, accr[rb][g], 0, 0, 0);
 #pragma unroll
-                    for (int g = 0; g < NT; ++g) accr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai, b[g].y, accr[g], 0, 0, 0);
+                        for (int g = 0; g < NT; ++g)
+                            acci[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb].x, b[g].y, acci[rb][g], 0, 0, 0);
 #pragma unroll
-                    for (int g = 0; g < NT; ++g) acci[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b[g].x, acci[g], 0, 0, 0);
-                    a = an;
+                        for (int g = 0; g < NT; ++g)
+                            accr[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai, b[g].y, accr[rb][g], 0, 0, 0);
+#pragma unroll
+                        for (int g = 0; g < NT; ++g)
+                            acci[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb].y, b[g].x, acci[rb][g], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int rb = 0; rb < RB; ++rb) a[rb] = an[rb];
 #pragma unroll
                     for (int g = 0; g < NT; ++g) b[g] = bn[g];
                 }
@@ -1493,26 +1529,28 @@ __global__ __launch_bounds__(256) void mfma_rhs_kernel(
             cn = nn;
             ci = ni;
         }
-        // lane li + 16 lk holds rows 4 lk + v of column li (+ 16 g)
+        // lane li + 16 lk holds rows 16 rb + 4 lk + v of column li (+ 16 g)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = 4 * lk + v;
-            const int orow = __shfl(rout, r, 64);
-            if (__shfl(int(rok), r, 64) == 0) continue;
+        for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-            for (int g = 0; g < NT; ++g) {
-                const int j = 16 * g + li;
-                if (j >= nc) continue;
-                const C<float> val = C<float>{accr[g][v], acci[g][v]};
-                if (OUTK == OUT_SLOT) {
-                    slots[int64_t(orow) * ns + j] = val;
-                } else {
-                    C<float> *yp = y + int64_t(j) * ldy + perm_out[orow];
-                    const C<float> old = *yp;
-                    *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, val));
+            for (int v = 0; v < 4; ++v) {
+                const int r = 4 * lk + v;
+                const int orow = __shfl(rout[rb], r, 64);
+                if (__shfl(int(rok[rb]), r, 64) == 0) continue;
+#pragma unroll
+                for (int g = 0; g < NT; ++g) {
+                    const int j = 16 * g + li;
+                    if (j >= nc) continue;
+                    const C<float> val = C<float>{accr[rb][g][v], acci[rb][g][v]};
+                    if (OUTK == OUT_SLOT) {
+                        slots[int64_t(orow) * ns + j] = val;
+                    } else {
+                        C<float> *yp = y + int64_t(j) * ldy + perm_out[orow];
+                        const C<float> old = *yp;
+                        *yp = cadd(fuse_beta ? cmul(beta, old) : old, cmul(alpha, val));
+                    }
                 }
             }
-        }
     }
 }
 
@@ -1522,29 +1560,43 @@ void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, vo
     if (t.n_mtiles == 0) return;
     BFG_REQUIRE(ns % 16 == 0 && ns >= 16 && ns <= 64 && nc >= 1 && nc <= ns,
                 "matvec: matrix-core chunk of 16, 32, 48 or 64 right-hand sides");
-    const int64_t wgs = (int64_t(t.n_mtiles) + 3) / 4;
-    const int64_t cap = int64_t(std::max(n_cu, 1)) * 16;
-    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
     const C<float> al{float(alpha[0]), float(alpha[1])}, be{float(beta[0]), float(beta[1])};
-#define BFG_MF(OUTK, NT)                                                                                        \
-    hipLaunchKernelGGL((mfma_rhs_kernel<OUTK, NT>), grid, dim3(256), 0, s, t.mtiles.as<const int4>(), t.n_mtiles, \
-                       t.mtasks.as<const MvTaskC>(), t.contribs.as<const MvContrib>(), (const C<float> *)factors, (C<float> *)slots, ns, in_shift, (C<float> *)y, ldy, perm_out, al, \
-                       be, int(fuse), nc)
-#define BFG_MF_NT(OUTK)                     \
-    switch (ns / 16) {                      \
-    case 1: BFG_MF(OUTK, 1); break;         \
-    case 2: BFG_MF(OUTK, 2); break;         \
-    case 3: BFG_MF(OUTK, 3); break;         \
-    default: BFG_MF(OUTK, 4); break;        \
+    // tiles are stored by row blocks (1..4 of 16 rows): one launch per kind
+    for (int rb = 1; rb <= 4; ++rb) {
+        const int32_t t0 = t.mtile_ofs[rb - 1], nt = t.mtile_ofs[rb] - t0;
+        if (nt <= 0) continue;
+        const int64_t wgs = (int64_t(nt) + 3) / 4;
+        const int64_t cap = int64_t(std::max(n_cu, 1)) * 16;
+        dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
+        const int4 *T = t.mtiles.as<const int4>() + t0;
+#define BFG_MF(OUTK, NT, RBV)                                                                                  \
+    hipLaunchKernelGGL((mfma_rhs_kernel<OUTK, NT, RBV>), grid, dim3(256), 0, s, T, nt, t.mtasks.as<const MvTaskC>(), \
+                       t.contribs.as<const MvContrib>(), (const C<float> *)factors, (C<float> *)slots, ns, in_shift, \
+                       (C<float> *)y, ldy, perm_out, al, be, int(fuse), nc)
+#define BFG_MF_RB(OUTK, NT)            \
+    switch (rb) {                      \
+    case 1: BFG_MF(OUTK, NT, 1); break; \
+    case 2: BFG_MF(OUTK, NT, 2); break; \
+    case 3: BFG_MF(OUTK, NT, 3); break; \
+    default: BFG_MF(OUTK, NT, 4); break; \
     }
-    if (t.out_kind == OUT_SLOT) {
-        BFG_MF_NT(OUT_SLOT);
-    } else {
-        BFG_MF_NT(OUT_SCATTER);
+#define BFG_MF_NT(OUTK)                 \
+    switch (ns / 16) {                  \
+    case 1: BFG_MF_RB(OUTK, 1); break;  \
+    case 2: BFG_MF_RB(OUTK, 2); break;  \
+    case 3: BFG_MF_RB(OUTK, 3); break;  \
+    default: BFG_MF_RB(OUTK, 4); break; \
     }
+        if (t.out_kind == OUT_SLOT) {
+            BFG_MF_NT(OUT_SLOT);
+        } else {
+            BFG_MF_NT(OUT_SCATTER);
+        }
 #undef BFG_MF_NT
+#undef BFG_MF_RB
 #undef BFG_MF
-    BFG_HIP(hipGetLastError());
+        BFG_HIP(hipGetLastError());
+    }
 }
 
 // dst[v * ns + r] = x[r][perm[v]] for v in [lo, hi), r < nc: inputs of the

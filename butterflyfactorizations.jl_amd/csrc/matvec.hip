@@ -620,14 +620,15 @@ void pack_tables(BF &bf) {
 // The vectors an item reads are the producer outputs inside the input
 // ranges of its (merged) contributions.
 // Matrix-core tiles of a forward table.  Items with one factor-block
-// contribution and at most 16 rows that read the same input entries (same
-// in_ofs and n) are stacked, whole, into tiles of at most 16 rows; every
-// other item (several contributions, an identity block, more than 16 rows)
-// gets tiles of 16 rows of its own.  Items are taken in order of their
-// first input entry, so consecutive tiles read neighbouring inputs; the
-// tasks are copied in that order.
-void build_mfma(StageTable &S) {
-    if (S.mfma_built) return;
+// contribution and at most `tile_rows` rows that read the same input
+// entries (same in_ofs and n) are stacked, whole, into tiles of at most
+// `tile_rows` rows; every other item (several contributions, an identity
+// block, more rows) gets tiles of `tile_rows` rows of its own.  Items are taken in order
+// of their first input entry, so consecutive tiles read neighbouring
+// inputs; the tasks are copied in that order.
+void build_mfma(StageTable &S, int32_t tile_rows) {
+    if (S.mfma_built && S.mfma_rows == tile_rows) return;
+    BFG_REQUIRE(tile_rows >= 16 && tile_rows <= MTILE_ROWS && tile_rows % 16 == 0, "pack: tile rows 16..64");
     const int32_t n = int32_t(S.h_items.size());
     struct Key { int32_t in_ofs, n, idx; };
     std::vector<Key> keys(size_t(std::max(n, 0)));
@@ -636,7 +637,7 @@ void build_mfma(StageTable &S) {
         const MvItem &it = S.h_items[i];
         const MvContrib &c = S.h_contribs[it.c_begin];
         keys[i] = Key{c.in_ofs, c.n, int32_t(i)};
-        stack[i] = it.c_end - it.c_begin == 1 && c.fofs >= 0 && it.out_len <= 16;
+        stack[i] = it.c_end - it.c_begin == 1 && c.fofs >= 0 && it.out_len <= tile_rows;
     }, 4096);
     std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
         return a.in_ofs != b.in_ofs ? a.in_ofs < b.in_ofs : a.n != b.n ? a.n < b.n : a.idx < b.idx;
@@ -657,7 +658,7 @@ void build_mfma(StageTable &S) {
         const int32_t m = S.h_items[i].out_len;
         if (m <= 0) continue;
         if (stack[i]) {
-            if (open.y > 0 && (keys[p].in_ofs != open_in || keys[p].n != open_n || open.w + m > 16)) close();
+            if (open.y > 0 && (keys[p].in_ofs != open_in || keys[p].n != open_n || open.w + m > tile_rows)) close();
             if (open.y == 0) {
                 open = int4{p, 0, 0, 0};
                 open_in = keys[p].in_ofs;
@@ -667,15 +668,22 @@ void build_mfma(StageTable &S) {
             open.w += m;
         } else {
             close();
-            for (int32_t r0 = 0; r0 < m; r0 += 16) tiles.push_back(int4{p, 1, r0, std::min(16, m - r0)});
+            for (int32_t r0 = 0; r0 < m; r0 += tile_rows) tiles.push_back(int4{p, 1, r0, std::min(tile_rows, m - r0)});
         }
     }
     close();
     BFG_REQUIRE(tiles.size() < size_t(INT32_MAX), "pack: too many matrix-core tiles");
+    // by number of 16-row blocks (one launch per kind), stable within a kind
+    auto blocks = [](const int4 &t) { return (t.w + 15) / 16; };
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const int4 &a, const int4 &b) { return blocks(a) < blocks(b); });
+    for (int rb = 0; rb <= 4; ++rb) S.mtile_ofs[rb] = 0;
+    for (const int4 &t : tiles) ++S.mtile_ofs[blocks(t)];
+    for (int rb = 1; rb <= 4; ++rb) S.mtile_ofs[rb] += S.mtile_ofs[rb - 1];
     upload_tab(S.mtasks, items);
     upload_tab(S.mtiles, tiles);
     S.n_mtiles = int32_t(tiles.size());
     S.mfma_built = true;
+    S.mfma_rows = tile_rows;
 }
 
 void build_sweep(BF &bf) {
@@ -902,8 +910,8 @@ static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
     }
     if (lanes && bf.use_mfma && bf.precision == BFGPU_C64)  // before any capture: the build uploads
         for (StageTable &t : bf.fwd) {
-            if (t.pre) build_mfma(*t.pre);
-            build_mfma(t);
+            if (t.pre) build_mfma(*t.pre, bf.mfma_rows);
+            build_mfma(t, bf.mfma_rows);
         }
     const bool fused = op == BFGPU_OP_N && !lanes && sweep_ok(bf, nrhs);
     if (tl || !bf.use_graph || fused || !graph_ok) {
@@ -967,7 +975,8 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         const bool mfma = bf.use_mfma && bf.precision == BFGPU_C64;
         if (mfma)
             for (const StageTable &t : bf.fwd)
-                BFG_REQUIRE(t.mfma_built && (!t.pre || t.pre->mfma_built), "matvec: matrix-core tiles not built");
+                BFG_REQUIRE(t.mfma_built && t.mfma_rows == bf.mfma_rows && (!t.pre || t.pre->mfma_built),
+                            "matvec: matrix-core tiles not built");
         for (int64_t c0 = 0; c0 < nrhs; c0 += 64) {
             const int nc = int(std::min<int64_t>(64, nrhs - c0));
             const int32_t ns = (nc + 15) / 16 * 16;

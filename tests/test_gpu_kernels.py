@@ -224,11 +224,16 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     rng = np.random.default_rng(12)
     X = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
     want = np.stack([imp.matvec(X[:, j].astype(np.complex128)) for j in range(70)], axis=1)
-    assert g.get_option("mfma") == 1
+    assert g.get_option("mfma") == 16
     g.set_option("rhs_lanes", 16)
     for nr in (16, 17, 33, 64, 70):
         got = (g @ np.asfortranarray(X[:, :nr])).astype(np.complex128)
         assert rel(got, want[:, :nr]) < 2e-6, nr
+    for rows in (32, 48, 64, 16):  # tiles of 1..4 row blocks
+        g.set_option("mfma", rows)
+        assert g.get_option("mfma") == rows
+        got = (g @ np.asfortranarray(X[:, :33])).astype(np.complex128)
+        assert rel(got, want[:, :33]) < 2e-6, rows
     Y0 = np.asfortranarray(crandn(rng, n, 17)).astype(np.complex64)
     a, b = 0.5 - 0.25j, -1.5 + 0.5j
     got = g.mul(Y0.copy(order="F"), np.asfortranarray(X[:, :17]), a, b).astype(np.complex128)
@@ -265,6 +270,10 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     for nr in (32, 40):
         got = (op @ np.asfortranarray(Xs[:, :nr])).astype(np.complex128)
         assert rel(got, want[:, :nr]) < 2e-6, nr
+    for rows in (64, 32, 16):
+        op.set_option("mfma", rows)
+        got = (op @ np.asfortranarray(Xs[:, :40])).astype(np.complex128)
+        assert rel(got, want[:, :40]) < 2e-6, rows
     op.set_option("mfma", 0)
     valu = (op @ Xs).astype(np.complex128)
     assert rel(valu, want) < 2e-6
