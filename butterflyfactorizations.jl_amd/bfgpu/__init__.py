@@ -398,18 +398,24 @@ class ButterflyFactorization:
 
     @staticmethod
     def _layout(ndim, shape, strides, unit, n, name):
+        # the stride of a dimension of length 1 is never used (numpy and
+        # torch report C-order strides for n x 1 and 1 x k arrays)
         if ndim == 1:
             if shape[0] != n:
                 raise ArgumentError(1, f"DimensionMismatch: {name} has length {shape[0]}, expected {n}")
-            if strides[0] != unit:
+            if strides[0] != unit and shape[0] > 1:
                 raise ArgumentError(1, f"{name} must be contiguous")
             return 1, n
         if ndim == 2:
             if shape[0] != n:
                 raise ArgumentError(1, f"DimensionMismatch: {name} has {shape[0]} rows, expected {n}")
-            if strides[0] != unit and shape[1] > 0:
+            if strides[0] != unit and shape[0] > 1 and shape[1] > 0:
                 raise ArgumentError(1, f"{name} must be column-major (Fortran order)")
-            return shape[1], max(strides[1], n) if shape[1] > 1 else n
+            if shape[1] > 1:
+                if strides[1] < n:
+                    raise ArgumentError(1, f"{name}: columns overlap (column stride {strides[1]} < {n} rows)")
+                return shape[1], strides[1]
+            return shape[1], n
         raise ArgumentError(1, f"{name} must be 1-D or 2-D")
 
     def __matmul__(self, x):

@@ -356,3 +356,23 @@ def test_gpu_recompression_matches_oracle(bfgpu, oracle, ctx):
     # the truncation threshold between FP32 and FP64 arithmetic)
     oex = orc.recompress(op.export(0), 1e-3)
     np.testing.assert_array_equal(rop.export(0)["table"][:, :5], oex["table"][:, :5])
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 300])
+def test_small_and_ragged_blocks_match_oracle(bfgpu, oracle, ctx, n):
+    """Blocks of one point, of fewer points than a leaf, of exactly one leaf
+    and of a leaf plus one point (trees of one or two levels, Q and P only
+    or one ragged level): FP64 factorization identical to the oracle's and
+    products to 1e-12, forward and adjoint, one and three RHS."""
+    say("small two squares fp64", n)
+    obs, src = two_squares(n, seed=n)
+    bt = trees(bfgpu, obs, src)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+    obf = oracle.assemble(bt.testcluster, bt.trialcluster, 0, 0, K, 1e-4)
+    _compare_factorizations(g.export(), obf.export())
+    rng = np.random.default_rng(n)
+    x = crandn(rng, n)
+    assert rel(g @ x, obf.matvec(x)) < 1e-12
+    assert rel(g.adjoint() @ x, obf.transform(1).matvec(x)) < 1e-12
+    X = np.asfortranarray(crandn(rng, n, 3))
+    assert rel(g @ X, obf.matvec(X)) < 1e-12

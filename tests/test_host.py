@@ -128,3 +128,30 @@ def test_ellipsoid_points_are_uniform_in_area():
     u = np.random.default_rng(5).standard_normal((200000, 3))
     u /= np.linalg.norm(u, axis=1)[:, None]
     assert abs(np.mean(np.abs(u[:, 2] * c) < c / 2) - band) > 0.05
+
+
+def test_vector_layouts(bfgpu):
+    """Column-major blocks of right-hand sides, with the strides numpy gives
+    length-1 dimensions (never used); row-major or overlapping blocks are
+    rejected before any call."""
+    lay = bfgpu.ButterflyFactorization._layout
+    es = 16
+    for shape in [(1, 3), (4, 1), (1, 1), (5, 4)]:
+        a = np.asfortranarray(np.zeros(shape, np.complex128))
+        b = np.ascontiguousarray(np.zeros(shape, np.complex128))
+        for arr in (a, b):
+            if arr is b and shape[0] > 1 and shape[1] > 1:
+                continue
+            nr, ld = lay(2, arr.shape, tuple(s // es for s in arr.strides), 1, shape[0], "x")
+            assert nr == shape[1] and ld >= shape[0]
+            if shape[1] > 1:  # column j starts at j * ld
+                assert ld == arr.strides[1] // es
+    with pytest.raises(bfgpu.ArgumentError):
+        lay(2, (5, 4), (4, 1), 1, 5, "x")  # row-major
+    with pytest.raises(bfgpu.ArgumentError):
+        lay(2, (5, 4), (1, 2), 1, 5, "x")  # overlapping columns
+    with pytest.raises(bfgpu.ArgumentError):
+        lay(2, (4, 4), (1, 4), 1, 5, "x")  # wrong row count
+    assert lay(1, (1,), (7,), 1, 1, "x") == (1, 1)
+    with pytest.raises(bfgpu.ArgumentError):
+        lay(1, (3,), (2,), 1, 3, "x")
