@@ -123,6 +123,23 @@ struct Comp {
 };
 
 // Per-factorization assembly state.
+struct Build;
+
+// The planning state of a finished factorization (millions of small
+// containers over all its factorizations) is released by a background
+// thread, so the call returns without waiting for the frees; the next
+// release, and the end of the process, wait for the previous one.
+class Reaper {
+    std::mutex mu_;
+    std::thread th_;
+
+public:
+    void post(std::vector<Build> &&b);
+    ~Reaper() {
+        if (th_.joinable()) th_.join();
+    }
+};
+
 struct Build {
     int32_t no, ns, LS, LO, L;
     double eps;
@@ -136,6 +153,22 @@ struct Build {
         K.swap(o.K);
     }
 };
+
+void Reaper::post(std::vector<Build> &&b) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (th_.joinable()) th_.join();
+    th_ = std::thread([v = std::move(b)]() mutable {
+        try {
+            parallel_for(int64_t(v.size()), [&](int64_t i) { Build().swap_out(v[i]); }, 4096);
+        } catch (...) {  // the remaining frees happen when v goes out of scope
+        }
+    });
+}
+
+Reaper &reaper() {
+    static Reaper r;
+    return r;
+}
 
 template <class T> void upload(DevBuf &d, const std::vector<T> &h, hipStream_t s) {
     d.reserve(sizeof(T) * std::max<size_t>(h.size(), 1));
@@ -716,8 +749,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     for (int64_t b = 0; b < nB; ++b) bf.bfs[b].stage_data_base.assign(bf.bfs[b].st.size(), 0);
     auto tfree = clk::now();
     E.release();  // tile workspace and index lists: not needed for packing
-    parallel_for(nB, [&](int64_t b) { Build().swap_out(B[b]); });
-    std::vector<Build>().swap(B);
+    reaper().post(std::move(B));
     const double t_free = secs(tfree, clk::now());
     auto tpk = clk::now();
     pack_tables(bf);

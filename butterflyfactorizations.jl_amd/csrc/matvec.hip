@@ -501,31 +501,65 @@ void pack_tables(BF &bf) {
         return b;
     };
     std::function<void(StageTable &, bool)> finish = [&](StageTable &S, bool fwd) {
+        // statistics of the table, in parallel over chunks of items and of
+        // contributions, partials merged in chunk order
+        struct Part {
+            int64_t vec = 0, fe = 0, nc_sum = 0;
+            int32_t max_out = 0, max_in = 0, max_nc = 0, in_lo = INT32_MAX, in_hi = 0;
+            int64_t hrow[32] = {}, hk[32] = {};
+        };
+        constexpr int64_t PCH = 1 << 16;
+        const int64_t ni = int64_t(S.h_items.size()), nk = int64_t(S.h_contribs.size());
+        const int64_t nci = (ni + PCH - 1) / PCH, nck = (nk + PCH - 1) / PCH;
+        std::vector<Part> parts(size_t(nci + nck));
+        parallel_for(nci + nck, [&](int64_t q) {
+            Part &p = parts[size_t(q)];
+            if (q < nci) {
+                for (int64_t i = q * PCH; i < std::min(ni, (q + 1) * PCH); ++i) {
+                    const MvItem &it = S.h_items[i];
+                    p.max_out = std::max(p.max_out, it.out_len);
+                    p.vec += int64_t(it.out_len) * ((S.out_kind == OUT_SCATTER) ? 2 : 1);
+                    ++p.hrow[size_t(log2up(std::max(it.out_len, 1)))];
+                    p.max_nc = std::max(p.max_nc, it.c_end - it.c_begin);
+                    p.nc_sum += it.c_end - it.c_begin;
+                }
+            } else {
+                const int64_t c0 = (q - nci) * PCH;
+                for (int64_t i = c0; i < std::min(nk, c0 + PCH); ++i) {
+                    const MvContrib &c = S.h_contribs[i];
+                    const int32_t in_len = fwd ? c.n : c.m;
+                    p.max_in = std::max(p.max_in, in_len);
+                    p.vec += in_len;
+                    if (c.fofs >= 0) {
+                        p.fe += int64_t(c.m) * c.n;
+                        ++p.hk[size_t(log2up(std::max(in_len, 1)))];
+                    }
+                    p.in_lo = std::min(p.in_lo, c.in_ofs);
+                    p.in_hi = std::max(p.in_hi, c.in_ofs + in_len);
+                }
+            }
+        }, 1);
         int64_t vec = 0;
         S.max_out = S.max_in = 0;
         S.factor_elems = 0;
         std::vector<int64_t> hrow(32, 0), hk(32, 0);
         int64_t nc_sum = 0;
         S.max_nc = 0;
-        for (const MvItem &it : S.h_items) {
-            S.max_out = std::max(S.max_out, it.out_len);
-            vec += int64_t(it.out_len) * ((S.out_kind == OUT_SCATTER) ? 2 : 1);
-            ++hrow[size_t(log2up(std::max(it.out_len, 1)))];
-            S.max_nc = std::max(S.max_nc, it.c_end - it.c_begin);
-            nc_sum += it.c_end - it.c_begin;
-        }
         S.in_lo = INT32_MAX;
         S.in_hi = 0;
-        for (const MvContrib &c : S.h_contribs) {
-            const int32_t in_len = fwd ? c.n : c.m;
-            S.max_in = std::max(S.max_in, in_len);
-            vec += in_len;
-            if (c.fofs >= 0) {
-                S.factor_elems += int64_t(c.m) * c.n;
-                ++hk[size_t(log2up(std::max(in_len, 1)))];
+        for (const Part &p : parts) {
+            vec += p.vec;
+            S.factor_elems += p.fe;
+            nc_sum += p.nc_sum;
+            S.max_out = std::max(S.max_out, p.max_out);
+            S.max_in = std::max(S.max_in, p.max_in);
+            S.max_nc = std::max(S.max_nc, p.max_nc);
+            S.in_lo = std::min(S.in_lo, p.in_lo);
+            S.in_hi = std::max(S.in_hi, p.in_hi);
+            for (int b = 0; b < 32; ++b) {
+                hrow[b] += p.hrow[b];
+                hk[b] += p.hk[b];
             }
-            S.in_lo = std::min(S.in_lo, c.in_ofs);
-            S.in_hi = std::max(S.in_hi, c.in_ofs + in_len);
         }
         if (S.in_hi <= S.in_lo) S.in_lo = S.in_hi = 0;
         // tiled-kernel shape: rows cover 90% of the outputs; the lane groups
