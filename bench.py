@@ -44,7 +44,7 @@ def build_sphere(bfgpu, n, k, tol, acctype, ctx):
     t0 = time.perf_counter()
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(k), bt, k, tol=tol,
                                 admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=acctype,
-                                ctx=ctx)
+                                farfieldonly=True, ctx=ctx)
     return pts, pts, bt, op, time.perf_counter() - t0
 
 

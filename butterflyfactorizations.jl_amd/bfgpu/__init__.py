@@ -34,7 +34,7 @@ __all__ = [
     "Context", "default_context", "ClusterTree", "BlockTree", "cluster_blktree",
     "HelmholtzKernelMatrix", "ButterflyRankEstimator", "GeometricRankEstimator",
     "LearnedRankEstimator", "TreeParameters", "tree_parameters", "assemble_BF",
-    "ButterflyFactorization", "PetrovGalerkinBF", "mul", "nearandfar", "isFarFunctor",
+    "ButterflyFactorization", "PetrovGalerkinBF", "BlockSparseNear", "mul", "nearandfar", "isFarFunctor",
     "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError", "load", "recompress_BF",
 ]
 
@@ -304,6 +304,84 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
+def _layout(ndim, shape, strides, unit, n, name):
+    """(nrhs, leading dimension) of a 1-D vector or a column-major matrix
+    with n rows.  The stride of a dimension of length 1 is never used (numpy
+    and torch report C-order strides for n x 1 and 1 x k arrays)."""
+    if ndim == 1:
+        if shape[0] != n:
+            raise ArgumentError(1, f"DimensionMismatch: {name} has length {shape[0]}, expected {n}")
+        if strides[0] != unit and shape[0] > 1:
+            raise ArgumentError(1, f"{name} must be contiguous")
+        return 1, n
+    if ndim == 2:
+        if shape[0] != n:
+            raise ArgumentError(1, f"DimensionMismatch: {name} has {shape[0]} rows, expected {n}")
+        if strides[0] != unit and shape[0] > 1 and shape[1] > 0:
+            raise ArgumentError(1, f"{name} must be column-major (Fortran order)")
+        if shape[1] > 1:
+            if strides[1] < n:
+                raise ArgumentError(1, f"{name}: columns overlap (column stride {strides[1]} < {n} rows)")
+            return shape[1], strides[1]
+        return shape[1], n
+    raise ArgumentError(1, f"{name} must be 1-D or 2-D")
+
+
+def _apply(cfun, handle, op, shape, prec, ctx, y, x, alpha, beta):
+    """y = alpha * op(A) x + beta * y through the C entry point `cfun`
+    (bfgpu_matvec or bfgpu_near_matvec): numpy arrays go through host
+    staging, torch GPU tensors are used in place on torch's current stream."""
+    rows, cols = shape
+    want = np.complex128 if prec == L.BFGPU_C128 else np.complex64
+    a = (C.c_double * 2)(float(np.real(alpha)), float(np.imag(alpha)))
+    b = (C.c_double * 2)(float(np.real(beta)), float(np.imag(beta)))
+    fn = getattr(lib(), cfun)
+    if _is_torch(x) or _is_torch(y):
+        import torch
+        if not (_is_torch(x) and _is_torch(y)):
+            raise ArgumentError(1, "x and y must both be torch tensors or both numpy arrays")
+        tw = torch.complex128 if prec == L.BFGPU_C128 else torch.complex64
+        if x.dtype != tw or y.dtype != tw:
+            raise ArgumentError(1, f"x and y must be {tw}")
+        if not (x.is_cuda and y.is_cuda):
+            raise ArgumentError(1, "torch tensors must live on the GPU")
+        nrhs, ldx = _layout(x.dim(), tuple(x.shape), tuple(x.stride()), 1, cols, "x")
+        nrhs2, ldy = _layout(y.dim(), tuple(y.shape), tuple(y.stride()), 1, rows, "y")
+        if nrhs != nrhs2:
+            raise ArgumentError(1, "x and y have different numbers of columns")
+        ctx.set_stream(torch.cuda.current_stream(x.device).cuda_stream)
+        check(fn(handle, op, nrhs, C.c_void_p(x.data_ptr()), ldx, C.c_void_p(y.data_ptr()), ldy, a, b,
+                 L.XY_DEVICE))
+        return y
+    if not (isinstance(x, np.ndarray) and isinstance(y, np.ndarray)):
+        raise ArgumentError(1, "x and y must be numpy arrays or torch tensors")
+    if x.dtype != want or y.dtype != want:
+        raise ArgumentError(1, f"x and y must be {np.dtype(want)}")
+    es = np.dtype(want).itemsize
+    nrhs, ldx = _layout(x.ndim, x.shape, tuple(s // es for s in x.strides), 1, cols, "x")
+    nrhs2, ldy = _layout(y.ndim, y.shape, tuple(s // es for s in y.strides), 1, rows, "y")
+    if nrhs != nrhs2:
+        raise ArgumentError(1, "x and y have different numbers of columns")
+    check(fn(handle, op, nrhs, C.c_void_p(x.ctypes.data), ldx, C.c_void_p(y.ctypes.data), ldy, a, b,
+             L.XY_HOST))
+    return y
+
+
+def _alloc_product(A, x):
+    """y = A x into a new zeroed array of x's kind (column-major for matrices)."""
+    rows = A.shape[0]
+    if _is_torch(x):
+        import torch
+        shape = (rows,) if x.dim() == 1 else (x.shape[1], rows)
+        y = torch.zeros(shape, dtype=x.dtype, device=x.device)
+        if x.dim() == 2:
+            y = y.t()
+        return A.mul(y, x)
+    shape = (rows,) if x.ndim == 1 else (rows, x.shape[1])
+    y = np.zeros(shape, dtype=x.dtype, order="F")
+    return A.mul(y, x)
+
+
 class _Handle:
     """Owns one bfgpu_bf handle (shared by adjoint/transpose views)."""
 
@@ -361,76 +439,14 @@ class ButterflyFactorization:
     # ---- application ----
     def mul(self, y, x, alpha=1.0, beta=0.0):
         """y = alpha * op(A) x + beta * y  (mul!, bfvector.jl:200-296)."""
-        rows, cols = self.shape
-        prec = self._H.precision
-        want = np.complex128 if prec == L.BFGPU_C128 else np.complex64
-        a = (C.c_double * 2)(float(np.real(alpha)), float(np.imag(alpha)))
-        b = (C.c_double * 2)(float(np.real(beta)), float(np.imag(beta)))
-        if _is_torch(x) or _is_torch(y):
-            import torch
-            if not (_is_torch(x) and _is_torch(y)):
-                raise ArgumentError(1, "x and y must both be torch tensors or both numpy arrays")
-            tw = torch.complex128 if prec == L.BFGPU_C128 else torch.complex64
-            if x.dtype != tw or y.dtype != tw:
-                raise ArgumentError(1, f"x and y must be {tw}")
-            if not (x.is_cuda and y.is_cuda):
-                raise ArgumentError(1, "torch tensors must live on the GPU")
-            nrhs, ldx = self._layout(x.dim(), tuple(x.shape), tuple(x.stride()), 1, cols, "x")
-            nrhs2, ldy = self._layout(y.dim(), tuple(y.shape), tuple(y.stride()), 1, rows, "y")
-            if nrhs != nrhs2:
-                raise ArgumentError(1, "x and y have different numbers of columns")
-            self.ctx.set_stream(torch.cuda.current_stream(x.device).cuda_stream)
-            check(lib().bfgpu_matvec(self.handle, self.op, nrhs, C.c_void_p(x.data_ptr()), ldx,
-                                     C.c_void_p(y.data_ptr()), ldy, a, b, L.XY_DEVICE))
-            return y
-        if not (isinstance(x, np.ndarray) and isinstance(y, np.ndarray)):
-            raise ArgumentError(1, "x and y must be numpy arrays or torch tensors")
-        if x.dtype != want or y.dtype != want:
-            raise ArgumentError(1, f"x and y must be {np.dtype(want)}")
-        es = np.dtype(want).itemsize
-        nrhs, ldx = self._layout(x.ndim, x.shape, tuple(s // es for s in x.strides), 1, cols, "x")
-        nrhs2, ldy = self._layout(y.ndim, y.shape, tuple(s // es for s in y.strides), 1, rows, "y")
-        if nrhs != nrhs2:
-            raise ArgumentError(1, "x and y have different numbers of columns")
-        check(lib().bfgpu_matvec(self.handle, self.op, nrhs, C.c_void_p(x.ctypes.data), ldx,
-                                 C.c_void_p(y.ctypes.data), ldy, a, b, L.XY_HOST))
-        return y
+        return _apply("bfgpu_matvec", self.handle, self.op, self.shape, self._H.precision, self.ctx,
+                      y, x, alpha, beta)
 
-    @staticmethod
-    def _layout(ndim, shape, strides, unit, n, name):
-        # the stride of a dimension of length 1 is never used (numpy and
-        # torch report C-order strides for n x 1 and 1 x k arrays)
-        if ndim == 1:
-            if shape[0] != n:
-                raise ArgumentError(1, f"DimensionMismatch: {name} has length {shape[0]}, expected {n}")
-            if strides[0] != unit and shape[0] > 1:
-                raise ArgumentError(1, f"{name} must be contiguous")
-            return 1, n
-        if ndim == 2:
-            if shape[0] != n:
-                raise ArgumentError(1, f"DimensionMismatch: {name} has {shape[0]} rows, expected {n}")
-            if strides[0] != unit and shape[0] > 1 and shape[1] > 0:
-                raise ArgumentError(1, f"{name} must be column-major (Fortran order)")
-            if shape[1] > 1:
-                if strides[1] < n:
-                    raise ArgumentError(1, f"{name}: columns overlap (column stride {strides[1]} < {n} rows)")
-                return shape[1], strides[1]
-            return shape[1], n
-        raise ArgumentError(1, f"{name} must be 1-D or 2-D")
+    _layout = staticmethod(lambda *a: _layout(*a))
 
     def __matmul__(self, x):
         """Allocating product (Base.:*, bfvector.jl:308-314; bfmatrix.jl:59-75)."""
-        rows = self.shape[0]
-        if _is_torch(x):
-            import torch
-            shape = (rows,) if x.dim() == 1 else (x.shape[1], rows)
-            y = torch.zeros(shape, dtype=x.dtype, device=x.device)
-            if x.dim() == 2:
-                y = y.t()
-            return self.mul(y, x)
-        shape = (rows,) if x.ndim == 1 else (rows, x.shape[1])
-        y = np.zeros(shape, dtype=x.dtype, order="F")
-        return self.mul(y, x)
+        return _alloc_product(self, x)
 
     __mul__ = __matmul__
 
@@ -725,32 +741,185 @@ def nearandfar(tree, admissibility, minbflvl=3, unbalancedints=False, leafcomp=T
     return far, near
 
 
+class BlockSparseNear:
+    """Near-field blocks of the global operator on the GPU: for every near
+    pair (observer node, source node) the dense kernel block, as the
+    BlockSparseMatrix of petrovgalerkinbf.jl:55-119 holds them, applied by
+    ``mul`` (the near ``mul!`` of matrixvector.jl:31-37) and its adjoint /
+    transpose views.  ``selfterm`` is the value on coincident points, where
+    the point kernel exp(-ik r)/(4 pi r) is singular (default 0)."""
+
+    def __init__(self, kernelmatrix, tree, pairs, *, acctype=np.complex128, selfterm=0.0, ctx=None):
+        if not isinstance(kernelmatrix, HelmholtzKernelMatrix):
+            raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix")
+        self.ctx = ctx or default_context()
+        self.tree = tree
+        self.pairs = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        dt = np.dtype(acctype)
+        if dt not in (np.complex64, np.complex128):
+            raise ArgumentError(1, "acctype must be complex64 or complex128")
+        prec = L.BFGPU_C128 if dt == np.complex128 else L.BFGPU_C64
+        sv = (C.c_double * 2)(float(np.real(selfterm)), float(np.imag(selfterm)))
+        h = C.c_void_p()
+        check(lib().bfgpu_near_assemble(self.ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,
+                                        float(kernelmatrix.k), prec, sv, self.pairs.shape[0],
+                                        ptr(self.pairs, C.c_int32), C.byref(h)))
+        self._h = h
+        self._owner = None
+        self.op = L.OP_N
+        inf = self.info()
+        self._rows, self._cols, self._prec = inf["rows"], inf["cols"], inf["precision"]
+
+    def _view(self, op):
+        v = object.__new__(BlockSparseNear)
+        v.__dict__.update(self.__dict__)
+        v._owner = self._owner or self  # keeps the handle alive
+        v.op = op
+        return v
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def eltype(self):
+        return np.complex128 if self._prec == L.BFGPU_C128 else np.complex64
+
+    @property
+    def shape(self):
+        return (self._rows, self._cols) if self.op == L.OP_N else (self._cols, self._rows)
+
+    def info(self):
+        v = (C.c_int64 * 8)()
+        check(lib().bfgpu_near_info(self._h, v))
+        keys = ("rows", "cols", "pairs", "items", "entries", "device_bytes", "precision", "direct")
+        return dict(zip(keys, (int(a) for a in v)))
+
+    def block(self, i):
+        """Dense block of near pair i (input order), m x n."""
+        mn = (C.c_int32 * 2)()
+        check(lib().bfgpu_near_block(self._h, int(i), mn, None))
+        out = np.zeros((mn[0], mn[1]), dtype=self.eltype, order="F")
+        check(lib().bfgpu_near_block(self._h, int(i), mn, C.c_void_p(out.ctypes.data)))
+        return out
+
+    def mul(self, y, x, alpha=1.0, beta=0.0):
+        """y = alpha * op(N) x + beta * y."""
+        return _apply("bfgpu_near_matvec", self._h, self.op, self.shape, self._prec, self.ctx, y, x, alpha, beta)
+
+    def __matmul__(self, x):
+        return _alloc_product(self, x)
+
+    __mul__ = __matmul__
+
+    def adjoint(self):
+        if self.op == L.OP_T:
+            ButterflyFactorization._conj_view_error()
+        return self._view(L.OP_C if self.op == L.OP_N else L.OP_N)
+
+    def transpose(self):
+        if self.op == L.OP_C:
+            ButterflyFactorization._conj_view_error()
+        return self._view(L.OP_T if self.op == L.OP_N else L.OP_N)
+
+    @property
+    def H(self):
+        return self.adjoint()
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    def __del__(self, _destroy=_destroy):
+        if getattr(self, "_owner", None) is None:
+            _destroy("bfgpu_near_destroy", getattr(self, "_h", None))
+        self._h = None
+
+
 class PetrovGalerkinBF(ButterflyFactorization):
-    """Far-field part of the global operator (petrovgalerkinbf.jl:33-161):
-    one butterfly factorization per admissible block, all factorized and
-    applied level by level in batched launches.  The near field is outside
-    this library (farfieldonly=True is required)."""
+    """The global operator (petrovgalerkinbf.jl:33-161): one butterfly
+    factorization per admissible block, all factorized and applied level by
+    level in batched launches, plus the near-field blocks
+    (``nearinteractions``, a BlockSparseNear) unless ``farfieldonly``.
+    ``mul`` computes y = alpha (N + F) x + beta y as matrixvector.jl:8-62;
+    ``selfterm`` is the near field's value on coincident points."""
 
     def __init__(self, kernelmatrix, tree, k, *, tol=1e-3, admissibility=None, rankestimator=None,
                  acctype=np.complex128, minbflvl=3, adaptive=True, unbalancedints=False,
-                 leafcomp=True, leafimbalance=True, farfieldonly=True, seed=0, workspace_bytes=0, ctx=None):
-        if not farfieldonly:
-            raise NotImplementedError("near-field assembly is not part of this library; "
-                                      "pass farfieldonly=True")
+                 leafcomp=True, leafimbalance=True, farfieldonly=False, selfterm=0.0, seed=0,
+                 workspace_bytes=0, ctx=None):
         ctx = ctx or default_context()
         admissibility = admissibility or isFarFunctor(tree_parameters(tree.testcluster).alpha)
         if rankestimator is None:
             rankestimator = ButterflyRankEstimator(tree_parameters(tree, admissibility).Ctau)
         far, near = nearandfar(tree, admissibility, minbflvl, unbalancedints, leafcomp, leafimbalance)
+        self.farinteractions, self.nearpairs = far, near
+        self.nearinteractions = None
+        if not farfieldonly:
+            self.nearinteractions = BlockSparseNear(kernelmatrix, tree, near, acctype=acctype,
+                                                    selfterm=selfterm, ctx=ctx)
         if far.shape[0] == 0:
-            raise ArgumentError(1, "no admissible (far) blocks")
-        self.farinteractions, self.nearinteractions = far, near
+            if farfieldonly:
+                raise ArgumentError(1, "no admissible (far) blocks")
+            # near field only: no factorization handle
+            self._H = None
+            self.ctx, self.tree, self.k, self.tau, self.op = ctx, tree, k, tol, L.OP_N
+            return
         o = _opts(k, tol, rankestimator, adaptive, acctype, seed, workspace_bytes)
         pairs = np.ascontiguousarray(far, np.int32)
         h = C.c_void_p()
         check(lib().bfgpu_factorize(ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,
                                     pairs.shape[0], ptr(pairs, C.c_int32), C.byref(o), C.byref(h)))
         super().__init__(h, ctx, tree, k, tol, _keep=(tree, rankestimator))
+
+    @property
+    def shape(self):
+        if self._H is None:
+            return self.nearinteractions.shape if self.op == L.OP_N else self.nearinteractions.shape[::-1]
+        return ButterflyFactorization.shape.fget(self)
+
+    @property
+    def eltype(self):
+        return self.nearinteractions.eltype if self._H is None else ButterflyFactorization.eltype.fget(self)
+
+    def _near_op(self):
+        N = self.nearinteractions
+        if N is None:
+            return None
+        return N if self.op == L.OP_N else N._view(self.op)
+
+    def mul(self, y, x, alpha=1.0, beta=0.0):
+        """y = alpha * op(N + F) x + beta * y: the far field scales y by beta
+        and adds alpha F x, the near field adds alpha N x."""
+        N = self._near_op()
+        if self._H is None:
+            return N.mul(y, x, alpha, beta)
+        ButterflyFactorization.mul(self, y, x, alpha, beta)
+        if N is not None:
+            N.mul(y, x, alpha, 1.0)
+        return y
+
+    def farfield(self):
+        """The far-field part alone, as a ButterflyFactorization view."""
+        if self._H is None:
+            raise ArgumentError(1, "the operator has no far field")
+        return ButterflyFactorization(self._H, self.ctx, self.tree, self.k, self.tau, self.op)
+
+    def _view(self, op):
+        v = object.__new__(PetrovGalerkinBF)
+        v.__dict__.update(self.__dict__)
+        v.op = op
+        return v
+
+    def adjoint(self):
+        if self.op == L.OP_T:
+            self._conj_view_error()
+        return self._view(L.OP_C if self.op == L.OP_N else L.OP_N)
+
+    def transpose(self):
+        if self.op == L.OP_C:
+            self._conj_view_error()
+        return self._view(L.OP_T if self.op == L.OP_N else L.OP_N)
 
 
 class RankPredictor:

@@ -39,7 +39,7 @@ def far_operator(points, k=K, tol=1e-4, acctype=np.complex64, rankestimator=None
     t0 = time.perf_counter()
     op = PetrovGalerkinBF(HelmholtzKernelMatrix(k), bt, k, tol=tol,
                           admissibility=CenterDistanceAdmissibility(separation), acctype=acctype,
-                          rankestimator=rankestimator, ctx=ctx)
+                          rankestimator=rankestimator, farfieldonly=True, ctx=ctx)
     return op, time.perf_counter() - t0
 
 

@@ -319,7 +319,8 @@ struct FillTask {       // one kernel-matrix block
 };
 void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
                  const int32_t *rows, const int32_t *cols, const FillTask *tasks,
-                 const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k);
+                 const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k,
+                 double self_re = 0.0, double self_im = 0.0);
 
 struct QrTask {
     int64_t zofs;       // Z block (m x n, ld = m)

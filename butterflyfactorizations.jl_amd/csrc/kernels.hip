@@ -19,7 +19,8 @@ namespace bfg {
 // src[cols[j]]|, for every task; one thread per entry, task found by binary
 // search over the running entry counts.  The distance and the phase
 // reduction k r mod 2 pi are computed in FP64; FP32 tiles take sincos of the
-// reduced phase in FP32.
+// reduced phase in FP32.  Coincident points (r = 0, only in near-field
+// blocks) take the given self value.
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void fill_kernel(const double *__restrict__ obs,
@@ -29,7 +30,7 @@ __global__ __launch_bounds__(256) void fill_kernel(const double *__restrict__ ob
                                                    const FillTask *__restrict__ tasks,
                                                    const int64_t *__restrict__ task_end,
                                                    int32_t n_tasks, int64_t total, C<R> *Z,
-                                                   double k) {
+                                                   double k, double self_re, double self_im) {
     // grid-stride: a dispatch holds fewer than 2^32 work-items, the tiles of
     // one level can hold more entries
     int32_t lo = 0;
@@ -60,21 +61,23 @@ __global__ __launch_bounds__(256) void fill_kernel(const double *__restrict__ ob
         sincosf(float(ph), &s, &c);
         v = {float(c * g), float(-s * g)};
     }
+    if (r == 0.0) v = {R(self_re), R(self_im)};  // coincident points
     Z[t.zofs + local] = v;
     }
 }
 
 void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
                  const int32_t *rows, const int32_t *cols, const FillTask *tasks,
-                 const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k) {
+                 const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k,
+                 double self_re, double self_im) {
     if (total <= 0 || n_tasks <= 0) return;
     dim3 grid(unsigned(std::min<int64_t>((total + 255) / 256, int64_t(1) << 22)));
     if (prec == BFGPU_C128)
         hipLaunchKernelGGL(fill_kernel<double>, grid, dim3(256), 0, s, obs_pts, src_pts, rows,
-                           cols, tasks, task_end, n_tasks, total, (C<double> *)Z, k);
+                           cols, tasks, task_end, n_tasks, total, (C<double> *)Z, k, self_re, self_im);
     else
         hipLaunchKernelGGL(fill_kernel<float>, grid, dim3(256), 0, s, obs_pts, src_pts, rows,
-                           cols, tasks, task_end, n_tasks, total, (C<float> *)Z, k);
+                           cols, tasks, task_end, n_tasks, total, (C<float> *)Z, k, self_re, self_im);
     BFG_HIP(hipGetLastError());
 }
 

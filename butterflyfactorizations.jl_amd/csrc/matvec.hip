@@ -307,45 +307,60 @@ void pack_tables(BF &bf) {
         };
         if (node_keyed) {
             // items: the leaves of the shared side, in node order; each
-            // lists its blocks in factorization order
+            // lists its blocks in factorization order.  Two passes over
+            // ranges of factorizations: count per (range, node), then each
+            // range writes from its own start position in every item.
             const Tree &t = fwd ? test : trial;
-            std::vector<int64_t> cnt(size_t(t.n_nodes) + 1, 0);
-            for (int64_t b = 0; b < nb; ++b) {
-                const int32_t s = stage_at(b, L);
-                if (s < 0) continue;
-                for (const Block &k : bf.bfs[b].st[s].b) ++cnt[size_t(fwd ? k.obs_out : k.src_in) + 1];
-            }
-            std::vector<int32_t> item_of(size_t(t.n_nodes), -1);
-            for (int32_t v = 0; v < t.n_nodes; ++v)
-                if (cnt[v + 1] > 0) {
+            const int64_t NV = t.n_nodes;
+            const int64_t nr = std::min<int64_t>(nb, 4 * int64_t(host_threads()));
+            auto rbeg = [&](int64_t r) { return nb * r / nr; };
+            std::vector<int64_t> pos(size_t(nr * NV), 0);
+            parallel_for(nr, [&](int64_t r) {
+                int64_t *c = &pos[size_t(r * NV)];
+                for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b) {
+                    const int32_t s = stage_at(b, L);
+                    if (s < 0) continue;
+                    for (const Block &k : bf.bfs[b].st[s].b) ++c[fwd ? k.obs_out : k.src_in];
+                }
+            }, 1);
+            std::vector<int32_t> item_of(size_t(NV), -1);
+            int64_t total = 0;
+            for (int64_t v = 0; v < NV; ++v) {
+                const int64_t start = total;
+                for (int64_t r = 0; r < nr; ++r) {
+                    const int64_t n = pos[size_t(r * NV + v)];
+                    pos[size_t(r * NV + v)] = total;
+                    total += n;
+                }
+                if (total > start) {
+                    BFG_REQUIRE(total < (int64_t(1) << 31), "pack: too many blocks in one launch");
                     item_of[v] = int32_t(items.size());
-                    items.push_back(MvItem{0, 0, 0, 0});
-                }
-            for (int32_t v = 0; v < t.n_nodes; ++v) cnt[v + 1] += cnt[v];
-            BFG_REQUIRE(cnt[t.n_nodes] < (int64_t(1) << 31), "pack: too many blocks in one launch");
-            con.resize(size_t(cnt[t.n_nodes]));
-            for (int32_t v = 0; v < t.n_nodes; ++v)
-                if (item_of[v] >= 0) {
-                    items[item_of[v]].c_begin = int32_t(cnt[v]);
-                    items[item_of[v]].c_end = int32_t(cnt[v]);
-                }
-            for (int64_t b = 0; b < nb; ++b) {
-                const int32_t s = stage_at(b, L);
-                if (s < 0) continue;
-                const auto &blk = bf.bfs[b].st[s].b;
-                const int64_t g0 = ix.first[ix.sf[b] + s];
-                for (size_t q = 0; q < blk.size(); ++q) {
-                    const Block &k = blk[q];
-                    int64_t oo, io;
-                    describe(b, s, k, g0 + int64_t(q), oo, io);
-                    MvItem &it = items[item_of[fwd ? k.obs_out : k.src_in]];
-                    if (it.c_end == it.c_begin) {
-                        it.out_ofs = int32_t(oo);
-                        it.out_len = fwd ? k.m : k.n;
-                    }
-                    con[it.c_end++] = MvContrib{k.ident ? -1 : k.fofs, k.m, k.n, int32_t(io), 0};
+                    items.push_back(MvItem{0, 0, int32_t(start), int32_t(total)});
                 }
             }
+            con.resize(size_t(total));
+            parallel_for(nr, [&](int64_t r) {
+                int64_t *c = &pos[size_t(r * NV)];
+                for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b) {
+                    const int32_t s = stage_at(b, L);
+                    if (s < 0) continue;
+                    const auto &blk = bf.bfs[b].st[s].b;
+                    const int64_t g0 = ix.first[ix.sf[b] + s];
+                    for (size_t q = 0; q < blk.size(); ++q) {
+                        const Block &k = blk[q];
+                        const int32_t v = fwd ? k.obs_out : k.src_in;
+                        int64_t oo, io;
+                        describe(b, s, k, g0 + int64_t(q), oo, io);
+                        const int64_t p = c[v]++;
+                        MvItem &it = items[size_t(item_of[size_t(v)])];
+                        if (p == it.c_begin) {  // the item's first block sets its output
+                            it.out_ofs = int32_t(oo);
+                            it.out_len = fwd ? k.m : k.n;
+                        }
+                        con[size_t(p)] = MvContrib{k.ident ? -1 : k.fofs, k.m, k.n, int32_t(io), 0};
+                    }
+                }
+            }, 1);
         } else {
             // items: vectors of one factorization, first-use order within it
             std::vector<int64_t> nit(nb + 1, 0), nct(nb + 1, 0);

@@ -38,6 +38,7 @@ typedef struct bfgpu_ctx bfgpu_ctx;
 typedef struct bfgpu_tree bfgpu_tree;
 typedef struct bfgpu_bf bfgpu_bf;
 typedef struct bfgpu_mlp bfgpu_mlp;
+typedef struct bfgpu_near bfgpu_near;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *bfgpu_last_error(void);
@@ -101,6 +102,31 @@ int bfgpu_nearandfar(const bfgpu_tree *test, const bfgpu_tree *trial, int32_t ki
                      double param, int32_t minbflvl, int32_t unbalancedints,
                      int32_t leafcomp, int32_t leafimbalance, int32_t *far, int64_t *nfar,
                      int32_t *near, int64_t *nnear);
+
+/* ---- near field ----
+ * Near-field blocks of the global operator: for every (observer node,
+ * source node) pair, the dense block G(vals(O), vals(S)) assembled on the
+ * device in `precision` (replaces the near assembly of
+ * src/Matrixassembly/petrovgalerkinbf.jl:55-119, a BlockSparseMatrix of
+ * blocks filled by the :near kernel matrix).  Coincident points (r = 0)
+ * take *self_value (complex, 2 doubles; NULL: 0), the point kernel's
+ * value on the diagonal.  Both trees must carry points.  The operator is
+ * rows(test) x rows(trial), like the far field. */
+int bfgpu_near_assemble(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial, double k,
+                        int32_t precision, const double *self_value, int64_t n_pairs,
+                        const int32_t *pairs, bfgpu_near **out);
+/* y = alpha * op(N) * x + beta * y (the near mul! of
+ * src/matrixalgebra/matrixvector.jl:31-37 and its adjoint/transpose,
+ * src/matrixalgebra/matrix_adj_tr.jl); arguments as bfgpu_matvec. */
+int bfgpu_near_matvec(bfgpu_near *n, int32_t op, int64_t nrhs, const void *x, int64_t ldx, void *y,
+                      int64_t ldy, const double *alpha, const double *beta, int32_t flags);
+/* 8 int64: rows, cols, pairs, items (observer nodes), stored entries,
+ * device bytes, precision, 1 when the product writes y in one pass. */
+int bfgpu_near_info(const bfgpu_near *n, int64_t *info);
+/* Block of pair `pair` (input order): mn receives (m, n); data (may be
+ * NULL) receives the m x n column-major block in the stored precision. */
+int bfgpu_near_block(const bfgpu_near *n, int64_t pair, int32_t *mn, void *data);
+int bfgpu_near_destroy(bfgpu_near *n);
 
 /* ---- factorization ---- */
 typedef struct {

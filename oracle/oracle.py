@@ -119,6 +119,32 @@ def helmholtz(k, obs_pts, rows, src_pts, cols):
     return Z
 
 
+def near_block(k, obs_pts, rows, src_pts, cols, selfterm=0.0):
+    """Near-field block G(rows, cols) (the :near kernel matrix of
+    petrovgalerkinbf.jl:56, for the point kernel): the oracle's Helmholtz
+    fill, with coincident points (identical coordinates, r = 0) set to
+    selfterm."""
+    Z = helmholtz(k, obs_pts, rows, src_pts, cols)
+    P = np.asarray(obs_pts, np.float64).reshape(-1, 3)[np.asarray(rows)]
+    Q = np.asarray(src_pts, np.float64).reshape(-1, 3)[np.asarray(cols)]
+    same = (P[:, None, :] == Q[None, :, :]).all(axis=2)
+    Z[same] = selfterm
+    return Z
+
+
+def near_dense(k, test, trial, pairs, selfterm=0.0):
+    """Dense n_test x n_trial matrix of the near field
+    (petrovgalerkinbf.jl:55-119): each pair (O, S) adds its block at
+    (vals(O), vals(S)); blocks of overlapping pairs add, as the
+    BlockSparseMatrix product (matrixvector.jl:31-37) does."""
+    N = np.zeros((test.n_values, trial.n_values), np.complex128)
+    for O, S in np.asarray(pairs).reshape(-1, 2):
+        rows = test.perm[test.val_begin[O]:test.val_end[O]]
+        cols = trial.perm[trial.val_begin[S]:trial.val_end[S]]
+        N[np.ix_(rows, cols)] += near_block(k, test.points, rows, trial.points, cols, selfterm)
+    return N
+
+
 def rank_butterfly(eps, c_tau=4.0, rmin=10):
     x1, x2 = f64(), f64()
     r = lib().bfo_rank_butterfly(eps, c_tau, rmin, C.byref(x1), C.byref(x2))

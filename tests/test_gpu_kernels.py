@@ -128,7 +128,8 @@ def test_variants_sphere_partial_sums_match_oracle(bfgpu, oracle, ctx):
     T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
     bt = bfgpu.BlockTree(T, T)
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=1e-4,
-                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), ctx=ctx)
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), farfieldonly=True,
+                                ctx=ctx)
     fw = op.launch_stats()
     bw = op.adjoint().launch_stats()
     say("far pairs", len(op.farinteractions), "partial items fwd", fw[-1, 7], "adj", bw[-1, 7])
@@ -196,7 +197,8 @@ def test_rhs_across_lanes_matches_oracle(bfgpu, oracle, ctx):
     pts = fibonacci_sphere(6000, 4.0)
     T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
-                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), ctx=ctx)
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), farfieldonly=True,
+                                ctx=ctx)
     assert op.launch_stats()[-1, 7] > 0
     imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
     Xs = np.asfortranarray(crandn(rng, 6000, 20))
@@ -261,7 +263,7 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
                                 admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
-                                ctx=ctx)
+                                farfieldonly=True, ctx=ctx)
     assert op.launch_stats()[-1, 7] > 0
     imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
     Xs = np.asfortranarray(crandn(rng, 6000, 40)).astype(np.complex64)

@@ -170,7 +170,7 @@ def test_farfield_operator_sphere(bfgpu, oracle, ctx):
     tol = 1e-4
     adm = bfgpu.CenterDistanceAdmissibility(2.0)
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=tol, admissibility=adm,
-                                ctx=ctx)
+                                farfieldonly=True, ctx=ctx)
     far = op.farinteractions
     assert op.n_factorizations() == far.shape[0] > 10
     c_tau = bfgpu.tree_parameters(bt, adm).Ctau
@@ -284,7 +284,7 @@ def test_saved_factorization_applies_identically(bfgpu, oracle, ctx, tmp_path):
     B = bfgpu.BlockTree(T, T)
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), B, K, tol=1e-4,
                                 admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
-                                ctx=ctx)
+                                farfieldonly=True, ctx=ctx)
     p2 = str(tmp_path / "sphere.bfgpu")
     op.save(p2)
     op2 = bfgpu.load(p2, B, ctx=ctx)
@@ -346,7 +346,7 @@ def test_gpu_recompression_matches_oracle(bfgpu, oracle, ctx):
     T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
                                 admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
-                                ctx=ctx)
+                                farfieldonly=True, ctx=ctx)
     rop = op.recompress(1e-3)
     assert rop.info().factor_bytes <= op.info().factor_bytes and rop.info().n_bf == len(op.farinteractions)
     xs = crandn(rng, 6000).astype(np.complex64)

@@ -249,3 +249,28 @@ def test_nearandfar_tiles_the_matrix(oracle, bfgpu):
             else:
                 gap = np.maximum(np.abs(T.center[o] - T.center[s]) - (ro + rs), 0)
                 assert np.linalg.norm(gap) > param * max(ro, rs)
+
+
+def test_near_dense_places_every_block(oracle, bfgpu):
+    """The near-field oracle: coincident points take selfterm; near blocks
+    plus the far blocks written densely rebuild the whole kernel matrix
+    (each entry once); a symmetric pair list gives a symmetric matrix
+    (reciprocity), so a transposed placement fails."""
+    from conftest import fibonacci_sphere
+    k = 2 * np.pi
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 3.0]])
+    y = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 3.25]])
+    Z = oracle.near_block(k, x, [0, 1], y, [0, 1], selfterm=0.5 + 2j)
+    assert Z[0, 0] == 0.5 + 2j
+    assert abs(Z[1, 1] - (-1j / np.pi)) < 1e-15  # quarter wavelength
+    pts = fibonacci_sphere(1500, 2.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    far, near = oracle.nearandfar(T, T, 1, 2.0)
+    N = oracle.near_dense(k, T, T, near, selfterm=0.3)
+    F = oracle.near_dense(k, T, T, far)
+    idx = np.arange(1500, dtype=np.int32)
+    G = oracle.near_block(k, pts, idx, pts, idx, selfterm=0.3)
+    np.testing.assert_allclose(N + F, G, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(N, N.T, rtol=1e-15, atol=0)
+    assert np.count_nonzero(F) > 0 and np.count_nonzero(N) + np.count_nonzero(F) == N.size
+    assert np.all(np.diag(N) == 0.3)
