@@ -803,6 +803,13 @@ class BlockSparseNear:
         check(lib().bfgpu_near_block(self._h, int(i), mn, C.c_void_p(out.ctypes.data)))
         return out
 
+    def set_transposed_copy(self, on=True):
+        """Adjoint / transpose products through a transposed copy of the
+        blocks (default; assembled on first use, twice the memory) or the
+        two-pass product over the stored blocks (off)."""
+        check(lib().bfgpu_near_set_transposed_copy(self._h, 1 if on else 0))
+        return self
+
     def mul(self, y, x, alpha=1.0, beta=0.0):
         """y = alpha * op(N) x + beta * y."""
         return _apply("bfgpu_near_matvec", self._h, self.op, self.shape, self._prec, self.ctx, y, x, alpha, beta)

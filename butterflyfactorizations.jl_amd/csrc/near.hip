@@ -66,13 +66,14 @@ template <typename R> __device__ __forceinline__ C<R> shfl_xor_c(C<R> v, int s) 
 enum { NEAR_PART = 0, NEAR_Y_SET = 1, NEAR_Y_ACC = 2 };
 
 // mode NEAR_PART: part[r ldo + rflat + i] = acc; NEAR_Y_SET: y = alpha acc;
-// NEAR_Y_ACC: y = beta y + alpha acc (y rows through rowx).  A chunk of
+// NEAR_Y_ACC: y = beta y + alpha acc (y rows through rowx); CONJ multiplies
+// by the conjugate entries (the adjoint through a transposed copy).  A chunk of
 // `rows` <= 32 rows spans whole columns (rows = m): P = 64 / rows columns are
 // loaded per instruction, lane = (column sub, row), so one load reads P
 // consecutive columns as one contiguous run; the sub-columns are summed with
 // the wavefronts at the end.  With P = 1 a lane is a row and the inputs are
 // broadcast by readlane; with P > 1 by cross-lane permute.
-template <typename R, int NR>
+template <typename R, int NR, bool CONJ>
 __global__ __launch_bounds__(256) void near_fwd_kernel(const NearTask *__restrict__ tasks,
                                                        const C<R> *__restrict__ A,
                                                        const int32_t *__restrict__ colx,
@@ -112,7 +113,11 @@ __global__ __launch_bounds__(256) void near_fwd_kernel(const NearTask *__restric
                 for (int u = 0; u < 8; ++u)
 #pragma unroll
                     for (int r = 0; r < NR; ++r)
-                        cfma(acc[r], av[u], C<R>{bcast(xs[r].x, cc + u), bcast(xs[r].y, cc + u)});
+                    {
+                        const C<R> xv{bcast(xs[r].x, cc + u), bcast(xs[r].y, cc + u)};
+                        if (CONJ) cfmac(acc[r], av[u], xv);
+                        else cfma(acc[r], av[u], xv);
+                    }
             }
         } else {
             for (int cc = 0; cc < cn; cc += 8 * P) {
@@ -130,7 +135,9 @@ __global__ __launch_bounds__(256) void near_fwd_kernel(const NearTask *__restric
 #pragma unroll
                     for (int r = 0; r < NR; ++r) {
                         const R vx = __shfl(xs[r].x, src, 64), vy = __shfl(xs[r].y, src, 64);
-                        cfma(acc[r], av[u], C<R>{ok ? vx : R(0), ok ? vy : R(0)});
+                        const C<R> xv{ok ? vx : R(0), ok ? vy : R(0)};
+                        if (CONJ) cfmac(acc[r], av[u], xv);
+                        else cfma(acc[r], av[u], xv);
                     }
                 }
             }
@@ -314,6 +321,13 @@ struct Near {
     // per pair: item, first column within the item, m, n (block export)
     std::vector<int64_t> pair_fofs;
     std::vector<int32_t> pair_m, pair_n;
+    // transposed copy for op T / C: the blocks of the swapped pairs over the
+    // swapped trees, G(y, x) = G(x, y); built on the first T / C product,
+    // skipped (two-pass product instead) when device memory is short
+    double k = 0, self_re = 0, self_im = 0;
+    std::vector<int32_t> pairs;
+    std::unique_ptr<Near> tcopy;
+    bool tcopy_failed = false, use_tcopy = true;
 };
 
 namespace {
@@ -351,13 +365,13 @@ void run_sum(hipStream_t s, const Csr &c, const void *part, int64_t ldp, void *y
     BFG_HIP(hipGetLastError());
 }
 
-template <typename R, int NR>
+template <typename R, int NR, bool CONJ>
 void launch_fwd(hipStream_t s, const Near &nf, int32_t ntask, const void *x, int64_t ldx, void *out, int64_t ldo,
                 int64_t nrhs, const double *alpha, const double *beta, int mode) {
     for (int64_t r0 = 0; r0 < nrhs; r0 += int64_t(NR) * 65535) {
         const int64_t nr = std::min<int64_t>(nrhs - r0, int64_t(NR) * 65535);
         dim3 grid(unsigned(ntask), unsigned((nr + NR - 1) / NR));
-        hipLaunchKernelGGL((near_fwd_kernel<R, NR>), grid, dim3(256), 0, s, nf.fwd_tasks.as<NearTask>(),
+        hipLaunchKernelGGL((near_fwd_kernel<R, NR, CONJ>), grid, dim3(256), 0, s, nf.fwd_tasks.as<NearTask>(),
                            nf.data.as<C<R>>(), nf.colx.as<int32_t>(), nf.rowx.as<int32_t>(),
                            static_cast<const C<R> *>(x) + r0 * ldx, ldx, static_cast<C<R> *>(out) + r0 * ldo,
                            ldo, int(nr), C<R>{R(alpha[0]), R(alpha[1])}, C<R>{R(beta[0]), R(beta[1])}, mode);
@@ -377,32 +391,78 @@ void launch_tr(hipStream_t s, const Near &nf, const void *x, int64_t ldx, void *
     }
 }
 
+template <typename R, bool CONJ>
+void fwd_nr(hipStream_t s, const Near &nf, const void *x, int64_t ldx, void *out, int64_t ldo, int64_t nrhs,
+            const double *alpha, const double *beta, int mode) {
+    if (nrhs == 1) launch_fwd<R, 1, CONJ>(s, nf, nf.n_fwd, x, ldx, out, ldo, nrhs, alpha, beta, mode);
+    else if (nrhs == 2) launch_fwd<R, 2, CONJ>(s, nf, nf.n_fwd, x, ldx, out, ldo, nrhs, alpha, beta, mode);
+    else if (nrhs <= 4) launch_fwd<R, 4, CONJ>(s, nf, nf.n_fwd, x, ldx, out, ldo, nrhs, alpha, beta, mode);
+    else launch_fwd<R, 8, CONJ>(s, nf, nf.n_fwd, x, ldx, out, ldo, nrhs, alpha, beta, mode);
+}
+
+// y = alpha N x + beta y (conj(N) with CONJ), device x and y
+template <typename R, bool CONJ>
+void apply_forward(Near &nf, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy, const double *alpha,
+                   const double *beta) {
+    hipStream_t s = nf.ctx->stream;
+    const bool bz = beta[0] == 0.0 && beta[1] == 0.0;
+    if (nf.direct && nf.n_fwd > 0) {
+        fwd_nr<R, CONJ>(s, nf, x, ldx, y, ldy, nrhs, alpha, beta, bz ? NEAR_Y_SET : NEAR_Y_ACC);
+        return;
+    }
+    const int64_t ldp = std::max<int64_t>(nf.n_rflat, 1);
+    nf.part.reserve(size_t(ldp) * size_t(nrhs) * sizeof(C<R>));
+    if (nf.n_fwd > 0) {
+        const double one[2] = {1, 0}, zero[2] = {0, 0};
+        fwd_nr<R, CONJ>(s, nf, x, ldx, nf.part.p, ldp, nrhs, one, zero, NEAR_PART);
+    }
+    run_sum<R>(s, nf.rsum, nf.part.p, ldp, y, ldy, nrhs, alpha, beta);
+}
+
+void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n_pairs, const int32_t *pairs);
+
+// the transposed copy, built on first use; null when it does not fit
+Near *transposed_copy(Near &nf) {
+    if (!nf.use_tcopy || nf.tcopy_failed) return nullptr;
+    if (nf.tcopy) return nf.tcopy.get();
+    size_t fr = 0, tot = 0;
+    BFG_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t es = nf.precision == BFGPU_C128 ? 16 : 8;
+    // the copy holds as many entries as the original; keep a tenth of the
+    // device free for the caller
+    if (size_t(nf.entries) * es + (size_t(64) << 20) > fr / 10 * 9) {
+        nf.tcopy_failed = true;
+        return nullptr;
+    }
+    auto t = std::make_unique<Near>();
+    t->ctx = nf.ctx;
+    t->test = nf.trial;
+    t->trial = nf.test;
+    t->precision = nf.precision;
+    t->use_tcopy = false;
+    std::vector<int32_t> sw(nf.pairs.size());
+    for (size_t p = 0; p + 1 < nf.pairs.size(); p += 2) {
+        sw[p] = nf.pairs[p + 1];
+        sw[p + 1] = nf.pairs[p];
+    }
+    near_assemble(*t, nf.k, nf.self_re, nf.self_im, int64_t(sw.size() / 2), sw.data());
+    nf.tcopy = std::move(t);
+    return nf.tcopy.get();
+}
+
 // y = alpha op(N) x + beta y, device x and y
 template <typename R>
 void near_apply(Near &nf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
                 const double *alpha, const double *beta) {
     hipStream_t s = nf.ctx->stream;
     const size_t es = sizeof(C<R>);
-    const bool bz = beta[0] == 0.0 && beta[1] == 0.0;
     if (op == BFGPU_OP_N) {
-        if (nf.direct && nf.n_fwd > 0) {
-            const int mode = bz ? NEAR_Y_SET : NEAR_Y_ACC;
-            if (nrhs == 1) launch_fwd<R, 1>(s, nf, nf.n_fwd, x, ldx, y, ldy, nrhs, alpha, beta, mode);
-            else if (nrhs == 2) launch_fwd<R, 2>(s, nf, nf.n_fwd, x, ldx, y, ldy, nrhs, alpha, beta, mode);
-            else if (nrhs <= 4) launch_fwd<R, 4>(s, nf, nf.n_fwd, x, ldx, y, ldy, nrhs, alpha, beta, mode);
-            else launch_fwd<R, 8>(s, nf, nf.n_fwd, x, ldx, y, ldy, nrhs, alpha, beta, mode);
-            return;
-        }
-        const int64_t ldp = std::max<int64_t>(nf.n_rflat, 1);
-        nf.part.reserve(size_t(ldp) * size_t(nrhs) * es);
-        if (nf.n_fwd > 0) {
-            const double one[2] = {1, 0}, zero[2] = {0, 0};
-            if (nrhs == 1) launch_fwd<R, 1>(s, nf, nf.n_fwd, x, ldx, nf.part.p, ldp, nrhs, one, zero, NEAR_PART);
-            else if (nrhs == 2) launch_fwd<R, 2>(s, nf, nf.n_fwd, x, ldx, nf.part.p, ldp, nrhs, one, zero, NEAR_PART);
-            else if (nrhs <= 4) launch_fwd<R, 4>(s, nf, nf.n_fwd, x, ldx, nf.part.p, ldp, nrhs, one, zero, NEAR_PART);
-            else launch_fwd<R, 8>(s, nf, nf.n_fwd, x, ldx, nf.part.p, ldp, nrhs, one, zero, NEAR_PART);
-        }
-        run_sum<R>(s, nf.rsum, nf.part.p, ldp, y, ldy, nrhs, alpha, beta);
+        apply_forward<R, false>(nf, nrhs, x, ldx, y, ldy, alpha, beta);
+        return;
+    }
+    if (Near *t = transposed_copy(nf)) {
+        if (op == BFGPU_OP_C) apply_forward<R, true>(*t, nrhs, x, ldx, y, ldy, alpha, beta);
+        else apply_forward<R, false>(*t, nrhs, x, ldx, y, ldy, alpha, beta);
         return;
     }
     const int64_t ldp = std::max<int64_t>(nf.n_cflat, 1);
@@ -461,6 +521,9 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
     nf.rows = te.n_values;
     nf.cols = tr.n_values;
     nf.n_pairs = n_pairs;
+    nf.k = k;
+    nf.self_re = self_re;
+    nf.self_im = self_im;
     for (int64_t p = 0; p < n_pairs; ++p)
         BFG_REQUIRE(pairs[2 * p] >= 0 && pairs[2 * p] < te.n_nodes && pairs[2 * p + 1] >= 0 &&
                         pairs[2 * p + 1] < tr.n_nodes,
@@ -515,6 +578,7 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
         a = b;
     }
     BFG_REQUIRE(fwd.size() < (size_t(1) << 31) && trt.size() < (size_t(1) << 31), "near_assemble: too many items");
+    if (nf.use_tcopy) nf.pairs.assign(pairs, pairs + 2 * n_pairs);
     nf.entries = fofs;
     nf.n_rflat = int64_t(rowx.size());
     nf.n_cflat = int64_t(colx.size());
@@ -612,8 +676,11 @@ extern "C" int bfgpu_near_info(const bfgpu_near *h, int64_t *info) {
     info[2] = n.n_pairs;
     info[3] = n.n_items;
     info[4] = n.entries;
-    info[5] = int64_t(n.data.bytes + n.fwd_tasks.bytes + n.tr_tasks.bytes + n.colx.bytes + n.rowx.bytes +
-                      n.rsum.ptr.bytes + n.rsum.idx.bytes + n.csum.ptr.bytes + n.csum.idx.bytes + n.part.bytes);
+    auto bytes = [](const Near &q) {
+        return int64_t(q.data.bytes + q.fwd_tasks.bytes + q.tr_tasks.bytes + q.colx.bytes + q.rowx.bytes +
+                       q.rsum.ptr.bytes + q.rsum.idx.bytes + q.csum.ptr.bytes + q.csum.idx.bytes + q.part.bytes);
+    };
+    info[5] = bytes(n) + (n.tcopy ? bytes(*n.tcopy) : 0);
     info[6] = n.precision;
     info[7] = n.direct ? 1 : 0;
     NF_API_END
@@ -634,6 +701,17 @@ extern "C" int bfgpu_near_block(const bfgpu_near *h, int64_t pair, int32_t *mn, 
     BFG_HIP(hipStreamSynchronize(n.ctx->stream));
     BFG_HIP(hipMemcpy(data, static_cast<const char *>(n.data.p) + size_t(n.pair_fofs[size_t(pair)]) * es, cnt * es,
                       hipMemcpyDeviceToHost));
+    NF_API_END
+}
+
+extern "C" int bfgpu_near_set_transposed_copy(bfgpu_near *h, int32_t on) {
+    NF_API_BEGIN
+    BFG_REQUIRE(h, "near_set_transposed_copy: null handle");
+    BFG_HIP(hipSetDevice(h->n.ctx->device));
+    BFG_HIP(hipStreamSynchronize(h->n.ctx->stream));
+    h->n.use_tcopy = on != 0;
+    h->n.tcopy_failed = false;
+    if (!on) h->n.tcopy.reset();
     NF_API_END
 }
 

@@ -126,6 +126,13 @@ int bfgpu_near_info(const bfgpu_near *n, int64_t *info);
 /* Block of pair `pair` (input order): mn receives (m, n); data (may be
  * NULL) receives the m x n column-major block in the stored precision. */
 int bfgpu_near_block(const bfgpu_near *n, int64_t pair, int32_t *mn, void *data);
+/* Transposed / adjoint products: on (default) applies them through a
+ * transposed copy of the blocks (the kernel is symmetric, G(y,x) = G(x,y)),
+ * assembled on the first such product when device memory allows (the copy
+ * doubles the near field's memory); off frees the copy and uses the
+ * two-pass product over the stored blocks.  No counterpart in the reference,
+ * whose adjoint views re-read the same blocks (matrix_adj_tr.jl). */
+int bfgpu_near_set_transposed_copy(bfgpu_near *n, int32_t on);
 int bfgpu_near_destroy(bfgpu_near *n);
 
 /* ---- factorization ---- */

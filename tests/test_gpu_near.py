@@ -74,17 +74,22 @@ def test_near_products_match_oracle(bfgpu, oracle, ctx, sphere, prec):
     D = oracle.near_dense(K, T, T, near, selfterm=0.5 - 0.25j)
     rng = np.random.default_rng(3)
     a, b = 0.7 - 0.2j, -0.3 + 1.1j
-    for nrhs in (1, 2, 3, 9):
-        X = np.asfortranarray(crandn(rng, 3000, nrhs).astype(dt))
-        Y0 = np.asfortranarray(crandn(rng, 3000, nrhs).astype(dt))
-        Xd = X.astype(np.complex128)
-        for name, op, M in (("N", Nf, D), ("T", Nf.transpose(), D.T), ("C", Nf.adjoint(), D.conj().T)):
-            say("  nrhs", nrhs, "op", name)
-            y = op.mul(Y0.copy(order="F"), X, a, b)
-            want = a * (M @ Xd) + b * Y0.astype(np.complex128)
-            assert rel(y, want) < bar, (nrhs, name)
-            y0 = op @ X
-            assert rel(y0, M @ Xd) < bar, (nrhs, name)
+    # adjoint / transpose through the transposed copy, then the two-pass
+    # product over the stored blocks
+    for copy in (True, False):
+        Nf.set_transposed_copy(copy)
+        for nrhs in (1, 2, 3, 9):
+            X = np.asfortranarray(crandn(rng, 3000, nrhs).astype(dt))
+            Y0 = np.asfortranarray(crandn(rng, 3000, nrhs).astype(dt))
+            Xd = X.astype(np.complex128)
+            for name, op, M in (("N", Nf, D), ("T", Nf.transpose(), D.T), ("C", Nf.adjoint(), D.conj().T)):
+                say("  copy", copy, "nrhs", nrhs, "op", name)
+                y = op.mul(Y0.copy(order="F"), X, a, b)
+                want = a * (M @ Xd) + b * Y0.astype(np.complex128)
+                assert rel(y, want) < bar, (copy, nrhs, name)
+                y0 = op @ X
+                assert rel(y0, M @ Xd) < bar, (copy, nrhs, name)
+    Nf.set_transposed_copy(True)
     # one vector, beta = 0 ignores what y held (NaN included)
     x = crandn(rng, 3000).astype(dt)
     y = np.full(3000, np.nan, dt)
@@ -110,9 +115,11 @@ def test_near_overlapping_pairs_two_pass(bfgpu, oracle, ctx, sphere):
         rng = np.random.default_rng(4)
         X = np.asfortranarray(crandn(rng, 3000, 5).astype(dt))
         Y0 = np.asfortranarray(crandn(rng, 3000, 5).astype(dt))
-        for op, M in ((Nf, D), (Nf.adjoint(), D.conj().T), (Nf.transpose(), D.T)):
-            y = op.mul(Y0.copy(order="F"), X, 2.0, -1.0)
-            assert rel(y, 2.0 * (M @ X.astype(np.complex128)) - Y0) < bar
+        for copy in (True, False):
+            Nf.set_transposed_copy(copy)
+            for op, M in ((Nf, D), (Nf.adjoint(), D.conj().T), (Nf.transpose(), D.T)):
+                y = op.mul(Y0.copy(order="F"), X, 2.0, -1.0)
+                assert rel(y, 2.0 * (M @ X.astype(np.complex128)) - Y0) < bar, copy
 
     say("near field of two different trees")
     rng = np.random.default_rng(5)
