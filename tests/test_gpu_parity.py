@@ -260,6 +260,20 @@ def test_factorization_in_small_batches_is_identical(bfgpu, oracle, ctx):
     assert rel(g2 @ x, g1 @ x) < 1e-12
 
 
+def test_sparse_level_matrices_of_a_gpu_factorization(bfgpu, oracle, ctx):
+    """to_mat (ButterflyFactorization_Mat form) of a GPU factorization applies
+    as the GPU matvec does, FP64 to 1e-12."""
+    say("to_mat fp64 two squares 4096")
+    obs, src = two_squares(4096, seed=9)
+    bt = trees(bfgpu, obs, src)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+    M = g.to_mat()
+    rng = np.random.default_rng(9)
+    X = np.asfortranarray(crandn(rng, 4096, 2))
+    assert rel(M @ X, g @ X) < 1e-12
+    assert len(M.R) == g.export()["n_stages"] - 2
+
+
 def test_saved_factorization_applies_identically(bfgpu, oracle, ctx, tmp_path):
     """A factorization written to a file and read back (blocks in the
     reference layout, logs, factors in their stored precision) exports the

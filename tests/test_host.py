@@ -155,3 +155,35 @@ def test_vector_layouts(bfgpu):
     assert lay(1, (1,), (7,), 1, 1, "x") == (1, 1)
     with pytest.raises(bfgpu.ArgumentError):
         lay(1, (3,), (2,), 1, 3, "x")
+
+
+def test_sparse_level_matrices_apply_the_factorization(bfgpu, oracle):
+    """bf_to_mat (ButterflyFactorization_Mat, bfstructs.jl:177-199) of an
+    exported factorization: P R ... R Q on x[permQ], scattered by permP,
+    equals the factorization's matvec; the permutations are permutations of
+    the points; balanced (two squares) and padded (max-points) trees."""
+    from conftest import crandn, fibonacci_sphere, two_squares
+    rng = np.random.default_rng(0)
+    obs, src = two_squares(1024, seed=2)
+    To = bfgpu.ClusterTree(obs, dims=2, leaf_size=64)
+    Ts = bfgpu.ClusterTree(src, dims=2, leaf_size=64)
+    cases = [(To, Ts, 0, 0)]
+    pts = fibonacci_sphere(3000, 3.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    far, _ = oracle.nearandfar(T, T, 1, 2.0)
+    big = max(far, key=lambda p: len(T.cluster_values(p[0])) * len(T.cluster_values(p[1])))
+    cases.append((T, T, int(big[0]), int(big[1])))
+    for te, tr, no, ns in cases:
+        obf = oracle.assemble(te, tr, no, ns, 2 * np.pi, 1e-4)
+        M = bfgpu.bf_to_mat(obf.export(), te, tr)
+        assert sorted(M.permQ) == sorted(tr.cluster_values(ns))
+        assert sorted(M.permP) == sorted(te.cluster_values(no))
+        assert M.Q.shape[1] == len(M.permQ) and M.P.shape[0] == len(M.permP)
+        for a, b in zip([M.Q] + M.R, M.R + [M.P]):
+            assert a.shape[0] == b.shape[1]
+        x = crandn(rng, tr.n_values)
+        want = obf.matvec(x)
+        got = M @ x
+        assert np.linalg.norm(got[:len(want)] - want) <= 1e-13 * np.linalg.norm(want)
+        X = crandn(rng, tr.n_values, 3)
+        assert np.allclose((M @ X)[:, 1], M @ X[:, 1], rtol=0, atol=1e-14 * np.abs(want).max())
