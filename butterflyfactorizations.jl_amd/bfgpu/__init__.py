@@ -34,7 +34,8 @@ __all__ = [
     "Context", "default_context", "ClusterTree", "BlockTree", "cluster_blktree",
     "HelmholtzKernelMatrix", "ButterflyRankEstimator", "GeometricRankEstimator",
     "LearnedRankEstimator", "TreeParameters", "tree_parameters", "assemble_BF",
-    "ButterflyFactorization", "PetrovGalerkinBF", "BlockSparseNear", "mul", "nearandfar", "isFarFunctor",
+    "ButterflyFactorization", "ButterflyFactorizationMat", "bf_to_mat", "PetrovGalerkinBF",
+    "BlockSparseNear", "mul", "nearandfar", "isFarFunctor",
     "CenterDistanceAdmissibility", "RankPredictor", "ArgumentError", "BfgpuError", "load", "recompress_BF",
 ]
 
@@ -498,6 +499,11 @@ class ButterflyFactorization:
                 "skel": skel, "skel_idx": skel_idx[:nski.value], "comp": comp,
                 "roots": (roots[0], roots[1])}
 
+    def to_mat(self, index=0):
+        """Factorization `index` as sparse level matrices with the point
+        permutations (ButterflyFactorization_Mat, bfstructs.jl:177-199)."""
+        return bf_to_mat(self.export(index), self.tree.testcluster, self.tree.trialcluster)
+
     def compression_log(self, index=None):
         """(records, features) of every compression: records are (stage, S,
         O, guess, rows, rank) as the reference's RankLogger would log them
@@ -741,6 +747,87 @@ def nearandfar(tree, admissibility, minbflvl=3, unbalancedints=False, leafcomp=T
     return far, near
 
 
+class ButterflyFactorizationMat:
+    """Sparse-matrix form of one factorization (ButterflyFactorization_Mat,
+    bfstructs.jl:177-199): A[permP, permQ] = P R[-1] ... R[0] Q, with Q, each
+    R and P scipy.sparse CSR matrices and permP / permQ the observer / source
+    point indices of the leaves in first-use order.  The skeleton vectors
+    between levels are ordered by the reference's slot ids (first use per
+    level, bfstructs.jl:107-145)."""
+
+    def __init__(self, Q, R, P, permP, permQ, shape):
+        self.Q, self.R, self.P = Q, R, P
+        self.permP, self.permQ = permP, permQ
+        self.shape = shape
+
+    def __matmul__(self, x):
+        x = np.asarray(x)
+        v = self.Q @ x[self.permQ]
+        for Rl in self.R:
+            v = Rl @ v
+        y = np.zeros((self.shape[0],) + x.shape[1:], np.result_type(v, x))
+        np.add.at(y, self.permP, self.P @ v)
+        return y
+
+
+def bf_to_mat(ex, test, trial):
+    """Build the sparse level matrices of an exported factorization (the
+    reference block layout: rows stage, obs_out, src_out, obs_in, src_in, m,
+    n, identity, out_ptr, in_ptr; column-major block data)."""
+    import scipy.sparse as sp
+    table, ofs, data = np.asarray(ex["table"]), np.asarray(ex["data_ofs"]), np.asarray(ex["data"])
+    ns = int(ex["n_stages"])
+    stage = table[:, 0]
+    mats, permP, permQ = [], [], []
+    in_off, in_len = {}, 0   # input offsets of the stage being built
+    for s_ in range(ns):
+        rows_idx = np.nonzero(stage == s_)[0]
+        out_off, out_len = {}, 0
+        ri, ci, vals = [], [], []
+        last = s_ == ns - 1
+        if s_ == 0:  # Q reads x at the source leaves, in first-use order
+            in_off, in_len = {}, 0
+            for b in rows_idx:
+                leaf = int(table[b, 4])
+                if leaf not in in_off:
+                    in_off[leaf] = in_len
+                    v = trial.cluster_values(leaf)
+                    permQ.extend(int(t) for t in v)
+                    in_len += len(v)
+        for b in rows_idx:
+            _, oo, so, oi, si, m, n, ident, op_, ip_ = (int(t) for t in table[b])
+            if last:  # P writes y at the observer leaves
+                key = oo
+                if key not in out_off:
+                    out_off[key] = out_len
+                    v = test.cluster_values(oo)
+                    permP.extend(int(t) for t in v)
+                    out_len += len(v)
+            else:
+                key = op_
+                if key not in out_off:
+                    out_off[key] = out_len
+                    out_len += m
+            r0 = out_off[key]
+            c0 = in_off[si] if s_ == 0 else in_off[ip_]
+            if ident:
+                blk = np.eye(m, dtype=np.complex128)
+            else:
+                blk = data[ofs[b]:ofs[b] + m * n].reshape((m, n), order="F")
+            rr, cc = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+            ri.append((r0 + rr).ravel())
+            ci.append((c0 + cc).ravel())
+            vals.append(blk.ravel())
+        M = sp.coo_matrix((np.concatenate(vals) if vals else np.zeros(0, np.complex128),
+                           (np.concatenate(ri) if ri else np.zeros(0, int),
+                            np.concatenate(ci) if ci else np.zeros(0, int))),
+                          shape=(out_len, in_len)).tocsr()  # duplicates add
+        mats.append(M)
+        in_off, in_len = out_off, out_len
+    return ButterflyFactorizationMat(mats[0], mats[1:-1], mats[-1], np.array(permP, np.int64),
+                                     np.array(permQ, np.int64), (test.n_values, trial.n_values))
+
+
 class BlockSparseNear:
     """Near-field blocks of the global operator on the GPU: for every near
     pair (observer node, source node) the dense kernel block, as the
@@ -808,6 +895,12 @@ class BlockSparseNear:
         blocks (default; assembled on first use, twice the memory) or the
         two-pass product over the stored blocks (off)."""
         check(lib().bfgpu_near_set_transposed_copy(self._h, 1 if on else 0))
+        return self
+
+    def set_mfma(self, min_rhs=8):
+        """Complex FP32 products of at least `min_rhs` right-hand sides on
+        the matrix cores (0: vector ALU only)."""
+        check(lib().bfgpu_near_set_mfma(self._h, int(min_rhs)))
         return self
 
     def mul(self, y, x, alpha=1.0, beta=0.0):

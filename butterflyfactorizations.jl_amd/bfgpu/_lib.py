@@ -84,6 +84,7 @@ _SIGS = {
     "bfgpu_near_info": (i32, [vp, P(i64)]),
     "bfgpu_near_block": (i32, [vp, i64, P(i32), vp]),
     "bfgpu_near_set_transposed_copy": (i32, [vp, i32]),
+    "bfgpu_near_set_mfma": (i32, [vp, i32]),
     "bfgpu_near_destroy": (i32, [vp]),
     "bfgpu_bf_from_host": (i32, [vp, vp, vp, i32, i64, P(i32), P(i64), P(f64), i32, P(vp)]),
     "bfgpu_bf_info_get": (i32, [vp, P(BFInfo)]),

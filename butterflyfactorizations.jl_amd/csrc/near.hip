@@ -282,6 +282,117 @@ __global__ __launch_bounds__(256) void near_tr_kernel(const NearTask *__restrict
     }
 }
 
+// Many right-hand sides, complex FP32 on the matrix cores.  The inputs of
+// a pass (at most 64 RHS) are gathered RHS-interleaved in source-tree order
+// (xg[p ns + j], ns = 16 NT), so the columns of one source leaf are
+// consecutive rows of xg.  One workgroup per 16-row chunk of an item, its
+// four wavefronts on quarters of the columns (summed through LDS in fixed
+// order at the end); per 4 columns: one A load (row li, column lk at lane
+// li + 16 lk), one B load per 16-RHS group (column lk, RHS li), four
+// v_mfma_f32_16x16x4_f32 per group into real and imaginary accumulators
+// (conjugated entries for CONJ); the next 4 columns load while the current
+// ones multiply.  Lane
+// li + 16 lk then holds rows 4 lk .. 4 lk + 3 of RHS li; rows are disjoint
+// across chunks (direct items), so each result is written once.
+typedef float near_v4f __attribute__((ext_vector_type(4)));
+
+template <int NT, bool CONJ>
+__global__ __launch_bounds__(256) void near_mfma_kernel(const NearTask *__restrict__ tasks, int32_t n_tasks,
+                                                        const C<float> *__restrict__ A,
+                                                        const int32_t *__restrict__ ctree,
+                                                        const int32_t *__restrict__ rowx,
+                                                        const C<float> *__restrict__ xg, int32_t ns,
+                                                        C<float> *__restrict__ y, int64_t ldy, int nc,
+                                                        C<float> alpha, C<float> beta, int beta_zero) {
+    __shared__ near_v4f red[3][2 * NT][64];
+    const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+    const int w = threadIdx.x >> 6;
+    const NearTask tk = tasks[blockIdx.x];
+    const bool rok = li < tk.rows;
+    const C<float> *ap = A + tk.fofs + min(li, tk.rows - 1);
+    const int32_t *cp = ctree + tk.col0;
+    // wavefront w takes columns [cb, ce), a quarter rounded to 4
+    const int q = (((tk.ncol + 3) >> 2) + 3) & ~3;
+    const int cb = w * q, ce = min(tk.ncol, cb + q);
+    near_v4f accr[NT], acci[NT];
+#pragma unroll
+    for (int g = 0; g < NT; ++g) accr[g] = acci[g] = near_v4f{0.f, 0.f, 0.f, 0.f};
+    if (cb < ce) {
+        auto load_a = [&](int k) -> C<float> {
+            const C<float> v = ap[int64_t(min(k, ce - 1)) * tk.m];
+            return (rok && k < ce) ? v : C<float>{0.f, 0.f};
+        };
+        C<float> a = load_a(cb + lk), b[NT];
+        {
+            const C<float> *xp = xg + int64_t(cp[min(cb + lk, ce - 1)]) * ns + li;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) b[g] = cb + lk < ce ? xp[16 * g] : C<float>{0.f, 0.f};
+        }
+        for (int k0 = cb; k0 < ce; k0 += 4) {
+            const int kn = k0 + 4 + lk;
+            const C<float> an = load_a(kn);
+            C<float> bn[NT];
+            const C<float> *xp = xg + int64_t(cp[min(kn, ce - 1)]) * ns + li;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) bn[g] = kn < ce ? xp[16 * g] : C<float>{0.f, 0.f};
+            const float nai = -a.y;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) {
+                accr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[g].x, accr[g], 0, 0, 0);
+                acci[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[g].y, acci[g], 0, 0, 0);
+                accr[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(CONJ ? a.y : nai, b[g].y, accr[g], 0, 0, 0);
+                acci[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(CONJ ? nai : a.y, b[g].x, acci[g], 0, 0, 0);
+            }
+            a = an;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) b[g] = bn[g];
+        }
+    }
+    // the four wavefronts' tiles summed in fixed order
+    if (w > 0) {
+#pragma unroll
+        for (int g = 0; g < NT; ++g) {
+            red[w - 1][2 * g][l] = accr[g];
+            red[w - 1][2 * g + 1][l] = acci[g];
+        }
+    }
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int g = 0; g < NT; ++g)
+#pragma unroll
+        for (int ww = 0; ww < 3; ++ww) {
+            accr[g] += red[ww][2 * g][l];
+            acci[g] += red[ww][2 * g + 1][l];
+        }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = 4 * lk + v;
+        if (r >= tk.rows) continue;
+        const int64_t yrow = rowx[tk.rflat + r];
+#pragma unroll
+        for (int g = 0; g < NT; ++g) {
+            const int j = 16 * g + li;
+            if (j >= nc) continue;
+            const C<float> val{accr[g][v], acci[g][v]};
+            C<float> *yp = y + int64_t(j) * ldy + yrow;
+            *yp = beta_zero ? cmul(alpha, val) : cadd(cmul(beta, *yp), cmul(alpha, val));
+        }
+    }
+}
+
+// xg[p ns + j] = x[perm[p] + j ldx] for j < nc, 0 for nc <= j < ns
+__global__ __launch_bounds__(256) void near_gather_kernel(const C<float> *__restrict__ x, int64_t ldx,
+                                                          const int32_t *__restrict__ perm, int32_t n,
+                                                          C<float> *__restrict__ xg, int32_t ns, int nc) {
+    const int64_t total = int64_t(n) * ns;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        const int64_t p = e / ns;
+        const int j = int(e - p * ns);
+        xg[e] = j < nc ? x[perm[p] + int64_t(j) * ldx] : C<float>{0.f, 0.f};
+    }
+}
+
 // y[r ldy + v] = beta y + alpha sum_{j in [ptr[v], ptr[v+1])} part[r ldp + idx[j]]
 // (beta exactly zero: y is not read)
 template <typename R>
@@ -326,6 +437,12 @@ struct Near {
     // skipped (two-pass product instead) when device memory is short
     double k = 0, self_re = 0, self_im = 0;
     std::vector<int32_t> pairs;
+    // matrix-core products (complex FP32, direct items): 16-row chunk tasks,
+    // source-tree position of every flat column, the source tree's
+    // permutation, the RHS-interleaved inputs of a pass
+    DevBuf mtasks, ctree, perm_src, xg;
+    int32_t n_mtasks = 0;
+    int32_t mfma_min_rhs = 8;        // 0: vector ALU only
     std::unique_ptr<Near> tcopy;
     bool tcopy_failed = false, use_tcopy = true;
 };
@@ -401,11 +518,53 @@ void fwd_nr(hipStream_t s, const Near &nf, const void *x, int64_t ldx, void *out
 }
 
 // y = alpha N x + beta y (conj(N) with CONJ), device x and y
+template <int NT, bool CONJ>
+void launch_mfma(hipStream_t s, const Near &nf, int32_t ns, void *y, int64_t ldy, int nc, const double *alpha,
+                 const double *beta, bool bz) {
+    hipLaunchKernelGGL((near_mfma_kernel<NT, CONJ>), dim3(unsigned(nf.n_mtasks)), dim3(256), 0, s,
+                       nf.mtasks.as<NearTask>(), nf.n_mtasks, nf.data.as<C<float>>(), nf.ctree.as<int32_t>(),
+                       nf.rowx.as<int32_t>(), nf.xg.as<C<float>>(), ns, static_cast<C<float> *>(y), ldy, nc,
+                       C<float>{float(alpha[0]), float(alpha[1])}, C<float>{float(beta[0]), float(beta[1])},
+                       bz ? 1 : 0);
+    BFG_HIP(hipGetLastError());
+}
+
+// complex FP32 on the matrix cores, passes of at most 64 RHS
+template <bool CONJ>
+void apply_mfma(Near &nf, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy, const double *alpha,
+                const double *beta) {
+    hipStream_t s = nf.ctx->stream;
+    const bool bz = beta[0] == 0.0 && beta[1] == 0.0;
+    const int32_t nsrc = nf.trial->n_values;
+    nf.xg.reserve(size_t(std::max(nsrc, 1)) * 64 * sizeof(C<float>));
+    for (int64_t r0 = 0; r0 < nrhs; r0 += 64) {
+        const int nc = int(std::min<int64_t>(64, nrhs - r0));
+        const int nt = (nc + 15) / 16;
+        const int32_t ns = 16 * nt;
+        const int64_t total = int64_t(nsrc) * ns;
+        if (total > 0) {
+            dim3 grid(unsigned(std::min<int64_t>((total + 255) / 256, int64_t(1) << 20)));
+            hipLaunchKernelGGL(near_gather_kernel, grid, dim3(256), 0, s, static_cast<const C<float> *>(x) + r0 * ldx,
+                               ldx, nf.perm_src.as<int32_t>(), nsrc, nf.xg.as<C<float>>(), ns, nc);
+            BFG_HIP(hipGetLastError());
+        }
+        C<float> *yp = static_cast<C<float> *>(y) + r0 * ldy;
+        if (nt == 1) launch_mfma<1, CONJ>(s, nf, ns, yp, ldy, nc, alpha, beta, bz);
+        else if (nt == 2) launch_mfma<2, CONJ>(s, nf, ns, yp, ldy, nc, alpha, beta, bz);
+        else if (nt == 3) launch_mfma<3, CONJ>(s, nf, ns, yp, ldy, nc, alpha, beta, bz);
+        else launch_mfma<4, CONJ>(s, nf, ns, yp, ldy, nc, alpha, beta, bz);
+    }
+}
+
 template <typename R, bool CONJ>
 void apply_forward(Near &nf, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy, const double *alpha,
                    const double *beta) {
     hipStream_t s = nf.ctx->stream;
     const bool bz = beta[0] == 0.0 && beta[1] == 0.0;
+    if (sizeof(R) == 4 && nf.direct && nf.n_mtasks > 0 && nf.mfma_min_rhs > 0 && nrhs >= nf.mfma_min_rhs) {
+        apply_mfma<CONJ>(nf, nrhs, x, ldx, y, ldy, alpha, beta);
+        return;
+    }
     if (nf.direct && nf.n_fwd > 0) {
         fwd_nr<R, CONJ>(s, nf, x, ldx, y, ldy, nrhs, alpha, beta, bz ? NEAR_Y_SET : NEAR_Y_ACC);
         return;
@@ -440,6 +599,7 @@ Near *transposed_copy(Near &nf) {
     t->trial = nf.test;
     t->precision = nf.precision;
     t->use_tcopy = false;
+    t->mfma_min_rhs = nf.mfma_min_rhs;
     std::vector<int32_t> sw(nf.pairs.size());
     for (size_t p = 0; p + 1 < nf.pairs.size(); p += 2) {
         sw[p] = nf.pairs[p + 1];
@@ -537,7 +697,8 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
     std::vector<NearTask> fwd, trt;
     std::vector<FillTask> fill;
     std::vector<int64_t> fill_end;
-    std::vector<int32_t> rowx, colx;
+    std::vector<int32_t> rowx, colx, ctree;
+    std::vector<NearTask> mt;
     nf.pair_fofs.assign(size_t(n_pairs), 0);
     nf.pair_m.assign(size_t(n_pairs), 0);
     nf.pair_n.assign(size_t(n_pairs), 0);
@@ -555,6 +716,7 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
             nf.pair_m[size_t(p)] = m;
             nf.pair_n[size_t(p)] = tr.nvals(S);
             colx.insert(colx.end(), tr.vals(S), tr.vals(S) + tr.nvals(S));
+            for (int32_t j = tr.val_begin[S]; j < tr.val_end[S]; ++j) ctree.push_back(j);
         }
         const int64_t ncol = int64_t(colx.size()) - col0;
         BFG_REQUIRE(ncol < (int64_t(1) << 31) && int64_t(colx.size()) < (int64_t(1) << 31),
@@ -563,12 +725,16 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
             // an empty block contributes nothing: no flat rows or columns
             // (the second pass would read partials no task writes)
             colx.resize(size_t(col0));
+            ctree.resize(size_t(col0));
         } else {
             rowx.insert(rowx.end(), te.vals(O), te.vals(O) + m);
             BFG_REQUIRE(int64_t(rowx.size()) < (int64_t(1) << 31), "near_assemble: too many near rows");
             for (int32_t r0 = 0; r0 < m; r0 += 64)
                 fwd.push_back(NearTask{fofs + r0, m, std::min(64, m - r0), int32_t(ncol), int32_t(col0),
                                        int32_t(rflat + r0), 0});
+            for (int32_t r0 = 0; r0 < m; r0 += 16)
+                mt.push_back(NearTask{fofs + r0, m, std::min(16, m - r0), int32_t(ncol), int32_t(col0),
+                                      int32_t(rflat + r0), 0});
             trt.push_back(NearTask{fofs, m, m, int32_t(ncol), int32_t(col0), int32_t(rflat), 0});
             fill.push_back(FillTask{fofs, m, int32_t(ncol), int32_t(rflat), int32_t(col0)});
             fofs += int64_t(m) * ncol;
@@ -577,13 +743,14 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
         ++nf.n_items;
         a = b;
     }
-    BFG_REQUIRE(fwd.size() < (size_t(1) << 31) && trt.size() < (size_t(1) << 31), "near_assemble: too many items");
+    BFG_REQUIRE(mt.size() < (size_t(1) << 31) && trt.size() < (size_t(1) << 31), "near_assemble: too many items");
     if (nf.use_tcopy) nf.pairs.assign(pairs, pairs + 2 * n_pairs);
     nf.entries = fofs;
     nf.n_rflat = int64_t(rowx.size());
     nf.n_cflat = int64_t(colx.size());
     nf.n_fwd = int32_t(fwd.size());
     nf.n_tr = int32_t(trt.size());
+    nf.n_mtasks = int32_t(mt.size());
     // forward writes y directly when every row of y is produced by exactly
     // one item row
     {
@@ -597,6 +764,9 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
     put(nf.tr_tasks, trt, s);
     put(nf.rowx, rowx, s);
     put(nf.colx, colx, s);
+    put(nf.mtasks, mt, s);
+    put(nf.ctree, ctree, s);
+    put(nf.perm_src, tr.perm, s);
     build_csr(nf.rsum, rowx, te.n_values, s);
     build_csr(nf.csum, colx, tr.n_values, s);
     if (!fill.empty()) {
@@ -678,7 +848,8 @@ extern "C" int bfgpu_near_info(const bfgpu_near *h, int64_t *info) {
     info[4] = n.entries;
     auto bytes = [](const Near &q) {
         return int64_t(q.data.bytes + q.fwd_tasks.bytes + q.tr_tasks.bytes + q.colx.bytes + q.rowx.bytes +
-                       q.rsum.ptr.bytes + q.rsum.idx.bytes + q.csum.ptr.bytes + q.csum.idx.bytes + q.part.bytes);
+                       q.rsum.ptr.bytes + q.rsum.idx.bytes + q.csum.ptr.bytes + q.csum.idx.bytes + q.part.bytes +
+                       q.mtasks.bytes + q.ctree.bytes + q.perm_src.bytes + q.xg.bytes);
     };
     info[5] = bytes(n) + (n.tcopy ? bytes(*n.tcopy) : 0);
     info[6] = n.precision;
@@ -712,6 +883,14 @@ extern "C" int bfgpu_near_set_transposed_copy(bfgpu_near *h, int32_t on) {
     h->n.use_tcopy = on != 0;
     h->n.tcopy_failed = false;
     if (!on) h->n.tcopy.reset();
+    NF_API_END
+}
+
+extern "C" int bfgpu_near_set_mfma(bfgpu_near *h, int32_t min_rhs) {
+    NF_API_BEGIN
+    BFG_REQUIRE(h && min_rhs >= 0, "near_set_mfma: null handle or negative count");
+    h->n.mfma_min_rhs = min_rhs;
+    if (h->n.tcopy) h->n.tcopy->mfma_min_rhs = min_rhs;
     NF_API_END
 }
 

@@ -133,6 +133,10 @@ int bfgpu_near_block(const bfgpu_near *n, int64_t pair, int32_t *mn, void *data)
  * two-pass product over the stored blocks.  No counterpart in the reference,
  * whose adjoint views re-read the same blocks (matrix_adj_tr.jl). */
 int bfgpu_near_set_transposed_copy(bfgpu_near *n, int32_t on);
+/* Complex FP32 products of at least min_rhs right-hand sides run on the
+ * matrix cores (default 8; 0: vector ALU only).  Applies when every row is
+ * produced by one item (the leaf case).  No counterpart in the reference. */
+int bfgpu_near_set_mfma(bfgpu_near *n, int32_t min_rhs);
 int bfgpu_near_destroy(bfgpu_near *n);
 
 /* ---- factorization ---- */

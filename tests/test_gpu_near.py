@@ -76,9 +76,10 @@ def test_near_products_match_oracle(bfgpu, oracle, ctx, sphere, prec):
     a, b = 0.7 - 0.2j, -0.3 + 1.1j
     # adjoint / transpose through the transposed copy, then the two-pass
     # product over the stored blocks
+    # complex FP32 from 8 RHS runs on the matrix cores (passes of <= 64 RHS)
     for copy in (True, False):
         Nf.set_transposed_copy(copy)
-        for nrhs in (1, 2, 3, 9):
+        for nrhs in (1, 2, 3, 9, 40, 70):
             X = np.asfortranarray(crandn(rng, 3000, nrhs).astype(dt))
             Y0 = np.asfortranarray(crandn(rng, 3000, nrhs).astype(dt))
             Xd = X.astype(np.complex128)
@@ -90,6 +91,13 @@ def test_near_products_match_oracle(bfgpu, oracle, ctx, sphere, prec):
                 y0 = op @ X
                 assert rel(y0, M @ Xd) < bar, (copy, nrhs, name)
     Nf.set_transposed_copy(True)
+    # the vector-ALU path for many RHS agrees with the matrix cores
+    X = np.asfortranarray(crandn(rng, 3000, 24).astype(dt))
+    on = Nf @ X
+    Nf.set_mfma(0)
+    off = Nf @ X
+    Nf.set_mfma(8)
+    assert rel(on, off) < bar
     # one vector, beta = 0 ignores what y held (NaN included)
     x = crandn(rng, 3000).astype(dt)
     y = np.full(3000, np.nan, dt)
