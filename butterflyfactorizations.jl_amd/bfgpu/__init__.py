@@ -606,6 +606,20 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_recompress(self.handle, float(tau), C.byref(h)))
         return ButterflyFactorization(h, self.ctx, self.tree, self.k, tau, self.op, _keep=self._H.keep)
 
+    def materialize(self):
+        """The operator this view applies, constructed explicitly: for an
+        adjoint or transpose view, adjoint(BF) / transpose(BF) of
+        bfadjtr.jl:43-89 (stages reversed, keys swapped, blocks conjugate-
+        transposed or transposed) over the swapped trees, applied by the
+        forward kernels.  A forward operator is returned as it is."""
+        if self.op == L.OP_N:
+            return self
+        h = C.c_void_p()
+        check(lib().bfgpu_bf_materialize(self.handle, self.op, C.byref(h)))
+        rt = BlockTree(self.tree.trialcluster, self.tree.testcluster)
+        keep = (rt,) + tuple(self._H.keep or ())
+        return ButterflyFactorization(h, self.ctx, rt, self.k, self.tau, L.OP_N, _keep=keep)
+
     def save(self, path):
         """Write the operator (blocks, logs and factors in their stored
         precision) to `path`; `bfgpu.load` reads it back."""
@@ -984,9 +998,26 @@ class PetrovGalerkinBF(ButterflyFactorization):
 
     def _near_op(self):
         N = self.nearinteractions
-        if N is None:
-            return None
-        return N if self.op == L.OP_N else N._view(self.op)
+        if N is None or self.op == L.OP_N:
+            return N
+        return N.adjoint() if self.op == L.OP_C else N.transpose()
+
+    def materialize(self):
+        """The adjoint / transpose view as an operator of its own: the far
+        field constructed explicitly (ButterflyFactorization.materialize),
+        the near field applied through its transposed blocks."""
+        if self.op == L.OP_N:
+            return self
+        v = object.__new__(PetrovGalerkinBF)
+        v.__dict__.update(self.__dict__)
+        v.nearinteractions = self._near_op()
+        v.op = L.OP_N
+        v.tree = BlockTree(self.tree.trialcluster, self.tree.testcluster)
+        v.farinteractions = self.farinteractions[:, ::-1].copy()
+        if self._H is not None:
+            far = ButterflyFactorization.materialize(self)
+            v._H = far._H
+        return v
 
     def mul(self, y, x, alpha=1.0, beta=0.0):
         """y = alpha * op(N + F) x + beta * y: the far field scales y by beta

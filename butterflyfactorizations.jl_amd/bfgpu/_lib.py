@@ -69,6 +69,7 @@ _SIGS = {
     "bfgpu_ctx_reset_stream": (i32, [vp]),
     "bfgpu_bf_save": (i32, [vp, C.c_char_p]),
     "bfgpu_bf_recompress": (i32, [vp, f64, P(vp)]),
+    "bfgpu_bf_materialize": (i32, [vp, i32, P(vp)]),
     "bfgpu_bf_load": (i32, [vp, vp, vp, C.c_char_p, P(vp)]),
     "bfgpu_ctx_get_stream": (i32, [vp, P(vp)]),
     "bfgpu_ctx_sync": (i32, [vp]),

@@ -26,7 +26,8 @@ void pack_tables(BF &bf);
 namespace {
 
 // copy of one m x n block (column-major, leading dimension ld_src) to dst
-// (leading dimension ld_dst), or of its conjugate transpose (n x m)
+// (leading dimension ld_dst), or of its conjugate transpose (conj_t 1) or
+// transpose (conj_t 2), n x m
 struct CopyTask {
     const void *src;
     void *dst;
@@ -50,7 +51,8 @@ __global__ __launch_bounds__(256) void copy_blocks_kernel(const CopyTask *__rest
         const int64_t local = e - (t.end - int64_t(t.m) * t.n);
         const int32_t i = int32_t(local % t.m), j = int32_t(local / t.m);
         const C<R> v = static_cast<const C<R> *>(t.src)[i + int64_t(j) * t.ld_src];
-        if (t.conj_t) static_cast<C<R> *>(t.dst)[j + int64_t(i) * t.ld_dst] = C<R>{v.x, -v.y};
+        if (t.conj_t == 1) static_cast<C<R> *>(t.dst)[j + int64_t(i) * t.ld_dst] = C<R>{v.x, -v.y};
+        else if (t.conj_t == 2) static_cast<C<R> *>(t.dst)[j + int64_t(i) * t.ld_dst] = v;
         else static_cast<C<R> *>(t.dst)[i + int64_t(j) * t.ld_dst] = v;
     }
 }
@@ -513,9 +515,105 @@ void recompress(const BF &src, double tau, BF &out) {
     pack_tables(out);
 }
 
+// The adjoint (conj) or transpose of every factorization as a factorization
+// of its own over the swapped trees (bfadjtr.jl:43-89): stages reversed,
+// keys swapped (transform_block, :13-24), blocks conjugate-transposed or
+// transposed, identity blocks kept.  Applying it forward equals applying the
+// source through its transposed launch tables.
+void materialize_transpose(const BF &src, bool conj, BF &out) {
+    const size_t es = src.precision == BFGPU_C128 ? 16 : 8;
+    Recompressor rc{src.ctx->stream, src.precision, es, {}};
+    const int32_t nL = src.n_stage_launch;
+    auto level_of = [&](size_t ns, size_t s) { return s + 1 == ns ? nL - 1 : int32_t(s); };
+    out.ctx = src.ctx;
+    out.test = src.trial;
+    out.trial = src.test;
+    out.precision = src.precision;
+    out.n_stage_launch = nL;
+    out.n_comp = src.n_comp;
+    out.n_retry = src.n_retry;
+    out.bfs.resize(src.bfs.size());
+    out.factor_elems.assign(size_t(nL), 0);
+    struct Put {
+        const void *from;
+        int32_t L, m, n;  // source block m x n
+        int64_t fofs;     // destination offset in level L
+    };
+    std::vector<Put> puts;
+    for (size_t b = 0; b < src.bfs.size(); ++b) {
+        const OneBF &ob = src.bfs[b];
+        OneBF &a = out.bfs[b];
+        const size_t ns = ob.st.size();
+        a.no = ob.ns;
+        a.ns = ob.no;
+        a.st.resize(ns);
+        a.stage_data_base.assign(ns, 0);
+        for (size_t sp = 0; sp < ns; ++sp) {
+            const size_t s = ns - 1 - sp;
+            const int32_t Ls = level_of(ns, s), Ld = level_of(ns, sp);
+            for (const Block &k : ob.st[s].b) {
+                Block t = k;
+                t.obs_out = k.src_in;
+                t.src_out = k.obs_in;
+                t.obs_in = k.src_out;
+                t.src_in = k.obs_out;
+                t.m = k.n;
+                t.n = k.m;
+                t.out_ptr = t.in_ptr = -1;
+                if (k.ident) {
+                    t.fofs = -1;
+                } else {
+                    t.fofs = out.factor_elems[Ld];
+                    out.factor_elems[Ld] += int64_t(k.m) * k.n;
+                    puts.push_back(Put{static_cast<const char *>(src.factors[Ls].p) + size_t(k.fofs) * es, Ld,
+                                       k.m, k.n, t.fofs});
+                }
+                a.st[sp].b.push_back(t);
+            }
+        }
+    }
+    out.factors.resize(size_t(nL));
+    for (int32_t L = 0; L < nL; ++L) {
+        out.factors[L].alloc(size_t(out.factor_elems[L] + FACTOR_PAD) * es);
+        BFG_HIP(hipMemsetAsync(static_cast<char *>(out.factors[L].p) + size_t(out.factor_elems[L]) * es, 0,
+                               size_t(FACTOR_PAD) * es, rc.s));
+    }
+    // copies in bounded batches (the task list of a large operator is big)
+    std::vector<CopyTask> ct;
+    constexpr size_t BATCH = size_t(1) << 22;
+    for (size_t i = 0; i < puts.size(); ++i) {
+        const Put &p = puts[i];
+        ct.push_back(CopyTask{p.from, static_cast<char *>(out.factors[p.L].p) + size_t(p.fofs) * es, p.m, p.n, p.m,
+                              p.n, conj ? 1 : 2, 0, 0});
+        if (ct.size() == BATCH || i + 1 == puts.size()) {
+            rc.copy(ct);
+            ct.clear();
+        }
+    }
+    pack_tables(out);
+}
+
 }  // namespace bfg
 
 using namespace bfg;
+
+extern "C" int bfgpu_bf_materialize(bfgpu_bf *h, int32_t op, bfgpu_bf **out) {
+    try {
+        BFG_REQUIRE(h && out, "bf_materialize: null argument");
+        BFG_REQUIRE(op == BFGPU_OP_T || op == BFGPU_OP_C, "bf_materialize: op must be BFGPU_OP_T or BFGPU_OP_C");
+        BFG_HIP(hipSetDevice(h->b.ctx->device));
+        auto n = std::make_unique<bfgpu_bf>();
+        materialize_transpose(h->b, op == BFGPU_OP_C, n->b);
+        *out = n.release();
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return BFGPU_ERR_STATE;
+    }
+    return BFGPU_OK;
+}
 
 extern "C" int bfgpu_bf_recompress(bfgpu_bf *h, double tau, bfgpu_bf **out) {
     try {

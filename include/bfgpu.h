@@ -204,6 +204,14 @@ int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
  * every factorization of the operator, on the GPU.  Q and P are kept. */
 int bfgpu_bf_recompress(bfgpu_bf *bf, double tau, bfgpu_bf **out);
 
+/* adjoint(BF) / transpose(BF) constructed explicitly
+ * (src/Butterflyalgebra/bfadjtr.jl:43-89): a new operator over the swapped
+ * trees whose factorizations have their stages reversed, block keys swapped
+ * and blocks conjugate-transposed (op BFGPU_OP_C) or transposed
+ * (BFGPU_OP_T).  Its forward product equals op(A) x through the lazy
+ * views of bfgpu_matvec, and runs on the forward kernels. */
+int bfgpu_bf_materialize(bfgpu_bf *bf, int32_t op, bfgpu_bf **out);
+
 /* Serialized factorizations (no single counterpart in the reference: the
  * reference keeps factorizations in memory only; this lets a factorization
  * be cached between runs).  save writes every factorization's blocks in the

@@ -193,6 +193,10 @@ def test_full_operator_matches_dense_matrix(bfgpu, oracle, ctx):
     assert rel(op.transpose() @ X[:, 0], G.T @ X[:, 0]) < 1e-5
     # the far field alone misses the near blocks
     assert rel(op.farfield() @ X, G @ X) > 1e-2
+    # the adjoint constructed explicitly applies as the adjoint view
+    Ah = op.adjoint().materialize()
+    assert rel(Ah @ X, op.adjoint() @ X) < 1e-12
+    assert rel(Ah.adjoint() @ X, Y) < 1e-12
     # alpha / beta through both parts
     Y0 = np.asfortranarray(crandn(rng, 4096, 3))
     got = op.mul(Y0.copy(order="F"), X, 0.5, 2.0 - 1.0j)

@@ -260,6 +260,34 @@ def test_factorization_in_small_batches_is_identical(bfgpu, oracle, ctx):
     assert rel(g2 @ x, g1 @ x) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["balanced", "unbalanced"])
+def test_materialized_adjoint_and_transpose_match_oracle(bfgpu, oracle, ctx, case):
+    """adjoint(BF) / transpose(BF) constructed on the GPU (bfadjtr.jl:43-89)
+    export the same blocks as the oracle's transform (keys, sizes, slot ids,
+    data exactly) and apply as the lazy views do, FP64 to 1e-12."""
+    say("materialize adjoint/transpose", case)
+    if case == "balanced":
+        obs, src = two_squares(2048, seed=11)
+        bt = trees(bfgpu, obs, src)
+    else:
+        obs, src = two_squares(2048, seed=12)
+        bt = trees(bfgpu, obs, src, lo=16, ls=64)
+    g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+    imp = oracle.import_bf(bt.testcluster, bt.trialcluster, g.export())
+    rng = np.random.default_rng(13)
+    X = np.asfortranarray(crandn(rng, 2048, 3))
+    for conj, view in ((1, g.adjoint()), (0, g.transpose())):
+        M = view.materialize()
+        assert M.shape == view.shape
+        mex, oex = M.export(), imp.transform(conj).export()
+        np.testing.assert_array_equal(mex["table"], oex["table"])
+        np.testing.assert_array_equal(mex["data_ofs"] >= 0, oex["data_ofs"] >= 0)
+        assert np.array_equal(mex["data"], oex["data"])
+        assert rel(M @ X, view @ X) < 1e-12
+        back = M.adjoint() if conj else M.transpose()  # the original operator again
+        assert rel(back @ X, g @ X) < 1e-12
+
+
 def test_sparse_level_matrices_of_a_gpu_factorization(bfgpu, oracle, ctx):
     """to_mat (ButterflyFactorization_Mat form) of a GPU factorization applies
     as the GPU matvec does, FP64 to 1e-12."""
