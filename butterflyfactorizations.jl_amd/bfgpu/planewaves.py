@@ -11,7 +11,10 @@ room for it after the FP32 operator is freed, the plain FP64 GPU
 factorization applied to the first few right-hand sides.
 
     python -m bfgpu.planewaves --points 1048576 --axes 16 12 8 --rhs 128 \\
-        --model bfgpu/data/rank_mlp.npz
+        --model bfgpu/data/rank_mlp.npz [--near]
+
+With --near the operator's near field is assembled too (FP32) and applied to
+the same plane waves.
 """
 import argparse
 import json
@@ -58,8 +61,37 @@ def far_rows_many(op, points, k, X, rows):
     return out
 
 
+def near_field(op, n_rhs, Xd, reps, ctx, log):
+    """The operator's near field (BlockSparseNear, FP32) assembled beside
+    the far field and applied to the same right-hand sides."""
+    import torch
+    from . import BlockSparseNear, HelmholtzKernelMatrix
+    t0 = time.perf_counter()
+    N = BlockSparseNear(HelmholtzKernelMatrix(K), op.tree, op.nearpairs, acctype=np.complex64, ctx=ctx)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
+    inf = N.info()
+    Yd = torch.zeros(n_rhs, Xd.shape[0], dtype=torch.complex64, device="cuda").t()
+    N.mul(Yd, Xd)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        N.mul(Yd, Xd)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = 1e3 * min(times)
+    log(f"near field: {inf['pairs']} pairs, {inf['entries'] * 8 / 1e9:.1f} GB, assembled in {t_asm:.2f} s, "
+        f"{n_rhs}-RHS product {ms:.1f} ms")
+    out = {"pairs": inf["pairs"], "entries": inf["entries"], "device_bytes": inf["device_bytes"],
+           "assemble_s": round(t_asm, 2), "matvec_ms": round(ms, 2)}
+    del N, Yd
+    torch.cuda.empty_cache()
+    return out
+
+
 def run(points=1048576, axes=(16.0, 12.0, 8.0), n_rhs=128, model=None, err_rows=16, fp64_rhs=8, reps=3,
-        seed=0, ctx=None, log=print):
+        seed=0, ctx=None, log=print, near=False):
     import torch
     ctx = ctx or default_context()
     rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -107,6 +139,9 @@ def run(points=1048576, axes=(16.0, 12.0, 8.0), n_rhs=128, model=None, err_rows=
            "device_in_use_after_matvec_gb": round((free0 - free2) / 1e9, 2), "hbm_total_gb": round(total / 1e9, 1),
            "slot_bytes_per_rhs": int(inf.slot_len) * 8, "host_rss_gb_after_factorization": round(rss, 1),
            "err_vs_dense_rows": rel(Y[rows].astype(complex), ye), "dense_rows": int(err_rows)}
+    if near:
+        out["near_field"] = near_field(op, n_rhs, Xd, reps, ctx, log)
+        out["whole_operator_matvec_ms"] = round(ms + out["near_field"]["matvec_ms"], 2)
     del op, Xd, Yd
     torch.cuda.empty_cache()
     free3, _ = torch.cuda.mem_get_info()
@@ -140,9 +175,11 @@ def main(argv=None):
     ap.add_argument("--model", default=None, help="rank predictor (.npz); without it the ranks are plain")
     ap.add_argument("--err-rows", type=int, default=16)
     ap.add_argument("--fp64-rhs", type=int, default=8, help="right-hand sides of the FP64 GPU check (0: none)")
+    ap.add_argument("--near", action="store_true", help="also assemble and apply the near field (FP32)")
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, file=sys.stderr, flush=True)
-    print(json.dumps(run(a.points, tuple(a.axes), a.rhs, a.model, a.err_rows, a.fp64_rhs, log=log)), flush=True)
+    print(json.dumps(run(a.points, tuple(a.axes), a.rhs, a.model, a.err_rows, a.fp64_rhs, log=log, near=a.near)),
+          flush=True)
 
 
 if __name__ == "__main__":
