@@ -414,6 +414,8 @@ class ButterflyFactorization:
 
     @property
     def handle(self):
+        if self._H is None:
+            raise ArgumentError(1, "the operator has no far field (no admissible blocks)")
         return self._H.h
 
     def info(self):
