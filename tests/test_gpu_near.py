@@ -224,3 +224,45 @@ def test_near_only_operator(bfgpu, oracle, ctx):
     if far.shape[0] == 0:
         assert op.shape == (200, 200)
     assert rel(op @ x, G @ x) < 1e-4
+
+
+def _power_norm(apply, apply_h, n, rng, tol=1e-4, itmax=100):
+    """sqrt of the largest eigenvalue of M^H M by power iteration (the
+    spectral-norm estimate of test_pgalerkin.jl)."""
+    v = crandn(rng, n)
+    v /= np.linalg.norm(v)
+    s_old, s_new = 1.0, 1.0
+    for i in range(itmax):
+        if i >= 3 and abs(np.sqrt(s_old) - np.sqrt(s_new)) / np.sqrt(s_old) <= tol:
+            break
+        s_old = s_new
+        x = apply_h(apply(v))
+        s_new = np.linalg.norm(x)
+        if s_new < 1e-14:
+            return 0.0
+        v = x / s_new
+    return np.sqrt(s_new)
+
+
+def test_operator_spectral_error_within_ten_tolerances(bfgpu, oracle, ctx):
+    """The global operator's check of test_pgalerkin.jl: relative spectral
+    difference ||H - A||_2 / ||A||_2 of the whole operator (near + far) to
+    the dense matrix, by power iteration on products with H and H', below
+    10 tol at tol = 1e-3."""
+    pts = fibonacci_sphere(4096, 2.5)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    say("spectral error of the whole operator, sphere N=4096, tol 1e-3, fp64 and fp32")
+    idx = np.arange(4096, dtype=np.int32)
+    A = oracle.near_block(K, pts, idx, pts, idx)
+    Ah = A.conj().T.copy()
+    rng = np.random.default_rng(21)
+    nA = _power_norm(lambda v: A @ v, lambda w: Ah @ w, 4096, rng)
+    for dt in (np.complex128, np.complex64):
+        H = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-3,
+                                   admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=dt, ctx=ctx)
+        Hh = H.adjoint()
+        cast = lambda v: v.astype(dt)
+        err = _power_norm(lambda v: (H @ cast(v)).astype(np.complex128) - A @ v,
+                          lambda w: (Hh @ cast(w)).astype(np.complex128) - Ah @ w, 4096, rng) / nA
+        say("  ", np.dtype(dt).name, "relative spectral error", err)
+        assert err < 10 * 1e-3
