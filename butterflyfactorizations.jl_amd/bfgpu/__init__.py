@@ -274,6 +274,7 @@ def _opts(k, tol, rankestimator, adaptive, acctype, seed, workspace_bytes=0):
     o.c_tau, o.rmin, o.c_geo, o.c_eps = 4.0, 10, 1.5, 1.5
     o.oversample = 10
     o.mlp = None
+    o.kernel = None
     est = rankestimator or ButterflyRankEstimator()
     if isinstance(est, ButterflyRankEstimator):
         o.rank_mode, o.c_tau, o.rmin = L.RANK_BUTTERFLY, float(est.Ceps), int(est.Rmin)
@@ -698,20 +699,32 @@ class ButterflyFactorization:
         return cls(h, ctx, blktree, k, tau, _keep=(blktree,))
 
 
+def _check_kernel(kernelmatrix, k):
+    from .rwg import EFIEKernelMatrix
+    if not isinstance(kernelmatrix, (HelmholtzKernelMatrix, EFIEKernelMatrix)):
+        raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix or an EFIEKernelMatrix")
+    if abs(kernelmatrix.k - k) > 1e-12 * abs(k):
+        raise ArgumentError(1, "wavenumber differs from the kernel's")
+
+
+def _plugin(kernelmatrix, ctx):
+    """The kernel plug-in handle for bfgpu_opts.kernel (None: point kernel)."""
+    from .rwg import EFIEKernelMatrix
+    return kernelmatrix.handle(ctx) if isinstance(kernelmatrix, EFIEKernelMatrix) else None
+
+
 def assemble_BF(kernelmatrix, blktree, no, ns, k, tau, *, rankestimator=None, adaptive=True,
                 acctype=np.complex128, seed=0, workspace_bytes=0, ctx=None):
     """Butterfly factorization of the block (observer node `no`, source node
     `ns`) of `kernelmatrix` (bfassembly.jl:48-200), computed on the GPU.
     `workspace_bytes` caps the device memory of kernel-matrix tiles per
     batch of compressions (0: automatic)."""
-    if not isinstance(kernelmatrix, HelmholtzKernelMatrix):
-        raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix")
-    if abs(kernelmatrix.k - k) > 1e-12 * abs(k):
-        raise ArgumentError(1, "wavenumber differs from the kernel's")
+    _check_kernel(kernelmatrix, k)
     ctx = ctx or default_context()
     if rankestimator is None:
         rankestimator = ButterflyRankEstimator(tree_parameters(blktree).Ctau)
     o = _opts(k, tau, rankestimator, adaptive, acctype, seed, workspace_bytes)
+    o.kernel = _plugin(kernelmatrix, ctx)
     pairs = np.array([no, ns], np.int32)
     h = C.c_void_p()
     check(lib().bfgpu_factorize(ctx.handle, blktree.testcluster.handle, blktree.trialcluster.handle,
@@ -965,6 +978,10 @@ class PetrovGalerkinBF(ButterflyFactorization):
                  leafcomp=True, leafimbalance=True, farfieldonly=False, selfterm=0.0, seed=0,
                  workspace_bytes=0, ctx=None):
         ctx = ctx or default_context()
+        _check_kernel(kernelmatrix, k)
+        if not farfieldonly and not isinstance(kernelmatrix, HelmholtzKernelMatrix):
+            raise ArgumentError(1, "the EFIE near field needs singular quadrature, which is not part of this "
+                                   "library: pass farfieldonly=True")
         admissibility = admissibility or isFarFunctor(tree_parameters(tree.testcluster).alpha)
         if rankestimator is None:
             rankestimator = ButterflyRankEstimator(tree_parameters(tree, admissibility).Ctau)
@@ -982,6 +999,7 @@ class PetrovGalerkinBF(ButterflyFactorization):
             self.ctx, self.tree, self.k, self.tau, self.op = ctx, tree, k, tol, L.OP_N
             return
         o = _opts(k, tol, rankestimator, adaptive, acctype, seed, workspace_bytes)
+        o.kernel = _plugin(kernelmatrix, ctx)
         pairs = np.ascontiguousarray(far, np.int32)
         h = C.c_void_p()
         check(lib().bfgpu_factorize(ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,

@@ -34,6 +34,7 @@ class Opts(C.Structure):
         ("k", f64), ("tol", f64), ("adaptive", i32), ("rank_mode", i32),
         ("c_tau", f64), ("rmin", i32), ("c_geo", f64), ("c_eps", f64),
         ("precision", i32), ("oversample", i32), ("seed", u64), ("mlp", vp), ("workspace_bytes", i64),
+        ("kernel", vp),
     ]
 
 
@@ -80,6 +81,9 @@ _SIGS = {
     "bfgpu_tree_destroy": (i32, [vp]),
     "bfgpu_nearandfar": (i32, [vp, vp, i32, f64, i32, i32, i32, i32, P(i32), P(i64), P(i32), P(i64)]),
     "bfgpu_factorize": (i32, [vp, vp, vp, i32, P(i32), P(Opts), P(vp)]),
+    "bfgpu_kernel_rwg_create": (i32, [vp, P(f64), P(f64), i64, i32, P(f64), i64, i32, P(f64), P(vp)]),
+    "bfgpu_kernel_fill": (i32, [vp, f64, i32, i32, P(i32), i32, P(i32), vp]),
+    "bfgpu_kernel_destroy": (i32, [vp]),
     "bfgpu_near_assemble": (i32, [vp, vp, vp, f64, i32, P(f64), i64, P(i32), P(vp)]),
     "bfgpu_near_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
     "bfgpu_near_info": (i32, [vp, P(i64)]),

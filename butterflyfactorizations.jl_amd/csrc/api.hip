@@ -139,6 +139,9 @@ extern "C" int bfgpu_factorize(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfg
     BFG_REQUIRE(ctx && test && trial && pairs && out, "factorize: null argument");
     BFG_REQUIRE(n_pairs >= 1, "factorize: need at least one block");
     check_opts(opts);
+    if (const KernelRWG *kr = kernel_of(opts->kernel))
+        BFG_REQUIRE(kernel_functions(*kr, true) >= test->t.n_values && kernel_functions(*kr, false) >= trial->t.n_values,
+                    "factorize: the kernel plug-in has fewer functions than the trees have values");
     BFG_HIP(hipSetDevice(ctx->c.device));
     std::vector<std::pair<int32_t, int32_t>> pr;
     for (int32_t i = 0; i < n_pairs; ++i) pr.emplace_back(pairs[2 * i], pairs[2 * i + 1]);

@@ -196,6 +196,17 @@ struct Engine {
         upload(d_trial_pts, trial.points, ctx.stream);
     }
 
+    // Kernel-matrix entries of the listed blocks: the plug-in's fill when
+    // one is given, else the Helmholtz point kernel.
+    void fill(const int32_t *rows, const int32_t *cols, const FillTask *tasks, const int64_t *task_end,
+              int32_t n_tasks, int64_t total, void *Z) {
+        if (const KernelRWG *kr = kernel_of(o.kernel))
+            launch_fill_rwg(ctx.stream, prec, *kr, rows, cols, tasks, task_end, n_tasks, total, Z, o.k);
+        else
+            launch_fill(ctx.stream, prec, d_test_pts.as<double>(), d_trial_pts.as<double>(), rows, cols, tasks,
+                        task_end, n_tasks, total, Z, o.k);
+    }
+
     // Shrink a level's factor buffer to what it holds (the growth policy
     // below leaves up to a third of it unused).
     void trim_pool(DevBuf &pool, int64_t &cap_elems, int64_t used) {
@@ -343,9 +354,8 @@ struct Engine {
             d_Z.reserve(size_t(zofs) * es);
             d_piv.reserve(sizeof(int32_t) * size_t(std::max(pofs, 1)));
             d_rank.reserve(sizeof(int32_t) * np);
-            launch_fill(ctx.stream, prec, d_test_pts.as<double>(), d_trial_pts.as<double>(),
-                        d_rows.as<int32_t>(), d_cols.as<int32_t>(), d_fill.as<FillTask>(),
-                        d_fill_end.as<int64_t>(), int32_t(ft.size()), zofs, d_Z.p, o.k);
+            fill(d_rows.as<int32_t>(), d_cols.as<int32_t>(), d_fill.as<FillTask>(), d_fill_end.as<int64_t>(),
+                 int32_t(ft.size()), zofs, d_Z.p);
             // QR launches: one per (tolerance, kernel) group -- the batch may
             // hold factorizations of different depth, and the blocks that
             // fit a wavefront's LDS take the wavefront kernel while larger
@@ -738,9 +748,8 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         upload(E.d_cols, hcols, ctx.stream);
         upload(E.d_fill, ft, ctx.stream);
         upload(E.d_fill_end, fe, ctx.stream);
-        launch_fill(ctx.stream, E.prec, E.d_test_pts.as<double>(), E.d_trial_pts.as<double>(),
-                    E.d_rows.as<int32_t>(), E.d_cols.as<int32_t>(), E.d_fill.as<FillTask>(),
-                    E.d_fill_end.as<int64_t>(), int32_t(ft.size()), zofs, bf.factors[sP].p, o.k);
+        E.fill(E.d_rows.as<int32_t>(), E.d_cols.as<int32_t>(), E.d_fill.as<FillTask>(), E.d_fill_end.as<int64_t>(),
+               int32_t(ft.size()), zofs, bf.factors[sP].p);
         BFG_HIP(hipStreamSynchronize(ctx.stream));
     }
     bf.n_comp = E.n_comp;

@@ -317,6 +317,13 @@ struct FillTask {       // one kernel-matrix block
     int32_t m, n;
     int32_t rofs, cofs; // offsets into the row / column point-index lists
 };
+// EFIE / RWG plug-in (efie.hip)
+struct KernelRWG;
+const KernelRWG *kernel_of(const bfgpu_kernel *h);
+int64_t kernel_functions(const KernelRWG &k, bool test);
+void launch_fill_rwg(hipStream_t s, int prec, const KernelRWG &kr, const int32_t *rows, const int32_t *cols,
+                     const FillTask *tasks, const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z,
+                     double k);
 void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
                  const int32_t *rows, const int32_t *cols, const FillTask *tasks,
                  const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k,

@@ -39,6 +39,7 @@ typedef struct bfgpu_tree bfgpu_tree;
 typedef struct bfgpu_bf bfgpu_bf;
 typedef struct bfgpu_mlp bfgpu_mlp;
 typedef struct bfgpu_near bfgpu_near;
+typedef struct bfgpu_kernel bfgpu_kernel;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *bfgpu_last_error(void);
@@ -139,6 +140,26 @@ int bfgpu_near_set_transposed_copy(bfgpu_near *n, int32_t on);
 int bfgpu_near_set_mfma(bfgpu_near *n, int32_t min_rhs);
 int bfgpu_near_destroy(bfgpu_near *n);
 
+/* ---- kernel-matrix plug-in: EFIE on RWG functions ----
+ * The Maxwell single layer between RWG test and trial functions, far
+ * interactions by double numerical quadrature (the BEAST plug-in of
+ * ext/BFBEAST/kernelmatrix.jl:22-82 with DoubleNumQStrat):
+ *   Z[m,n] = sum_a sum_b G(x_a, y_b) (alpha f_a . g_b + beta d_a e_b),
+ * G = exp(-ik|x-y|)/(4 pi |x-y|), k from bfgpu_opts.k.  test_samples holds,
+ * per test function, s_test samples of 7 doubles (x, y, z, w A f_m(x) (3),
+ * w A div f_m): the quadrature points of both triangles with weights w and
+ * areas A folded in; trial_samples likewise.  alpha, beta: complex (2
+ * doubles).  Pass the handle as bfgpu_opts.kernel; the trees' values are
+ * function indices. */
+int bfgpu_kernel_rwg_create(bfgpu_ctx *ctx, const double *alpha, const double *beta, int64_t n_test,
+                            int32_t s_test, const double *test_samples, int64_t n_trial, int32_t s_trial,
+                            const double *trial_samples, bfgpu_kernel **out);
+/* One m x n block Z[rows, cols] (column-major, `precision`) computed on the
+ * device by the plug-in's fill kernel; for checks. */
+int bfgpu_kernel_fill(const bfgpu_kernel *kern, double k, int32_t precision, int32_t m, const int32_t *rows,
+                      int32_t n, const int32_t *cols, void *Z);
+int bfgpu_kernel_destroy(bfgpu_kernel *kern);
+
 /* ---- factorization ---- */
 typedef struct {
     double k;            /* wavenumber of G(x,y) = exp(-ik|x-y|)/(4 pi |x-y|) */
@@ -155,6 +176,9 @@ typedef struct {
     int64_t workspace_bytes; /* most device bytes of kernel-matrix tiles per
                                 batch of compressions (0: min(32 GiB, 30 % of
                                 the free device memory)) */
+    const bfgpu_kernel *kernel; /* kernel-matrix plug-in (NULL: the Helmholtz
+                                   point kernel above); tree values index its
+                                   test / trial functions */
 } bfgpu_opts;
 
 /* assemble_BF (src/ButterflyFactorization/bfassembly.jl:48-200) for n_pairs
