@@ -113,6 +113,20 @@ struct OneBF {
 
 // Work item of one stage launch: an output target with a list of
 // contributions (factor blocks) summed into it.
+// Host arrays of plain records whose elements are all written after
+// allocation: resize() leaves new elements uninitialized (no serial
+// zero-fill of hundreds of MB before a parallel fill).
+template <class T> struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U> void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
+    template <class U, class... A> void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T> using hvec = std::vector<T, NoInitAlloc<T>>;
+
 struct MvItem {
     int32_t out_ofs;   // slot offset, or value offset into the scatter perm
     int32_t out_len;
@@ -196,9 +210,9 @@ struct StageTable {
     int32_t in_lo = 0, in_hi = 0;      // IN_GATHER: value range read through the permutation
     int32_t max_nc = 0;                // most contributions of one item
     DevBuf items, contribs, tasks, tasks_c;
-    std::vector<MvItem> h_items;
-    std::vector<MvContrib> h_contribs;
-    std::vector<MvTask> h_tasks;
+    hvec<MvItem> h_items;
+    hvec<MvContrib> h_contribs;
+    hvec<MvTask> h_tasks;
     // items split into partial sums: `pre` writes the partials into slots,
     // this table's combine items add them (identity contributions)
     std::unique_ptr<StageTable> pre;

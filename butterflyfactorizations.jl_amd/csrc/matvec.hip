@@ -70,7 +70,7 @@ struct KeyMap {
     }
 };
 
-template <class T> void upload_tab(DevBuf &d, const std::vector<T> &h) {
+template <class T, class A> void upload_tab(DevBuf &d, const std::vector<T, A> &h) {
     d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
     if (!h.empty()) BFG_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
 }
@@ -290,13 +290,13 @@ void pack_tables(BF &bf) {
         if (L == Lmax) return ns - 1;
         return L < ns - 1 ? L : -1;
     };
-    std::vector<int32_t> litem(size_t(std::max<int64_t>(ix.total, 1)));
+    hvec<int32_t> litem(size_t(std::max<int64_t>(ix.total, 1)));  // written before it is read
     // Group the blocks of launch L into items.  fwd: keyed by output,
     // contributions read the input; transposed: the other way round.
     auto group = [&](int32_t L, bool fwd, StageTable &S) {
         const bool node_keyed = fwd ? (L == Lmax) : (L == 0);
-        std::vector<MvItem> items;
-        std::vector<MvContrib> con;
+        hvec<MvItem> items;
+        hvec<MvContrib> con;
         auto describe = [&](int64_t b, int32_t s, const Block &k, int64_t g, int64_t &out_ofs,
                             int64_t &in_ofs) {
             const bool isQ = s == 0, isP = s == int32_t(bf.bfs[b].st.size()) - 1;
@@ -437,7 +437,7 @@ void pack_tables(BF &bf) {
                 kept[i + 1] = merge_contribs(con.data() + it.c_begin, it.c_end - it.c_begin);
             }, 4096);
             for (size_t i = 0; i < items.size(); ++i) kept[i + 1] += kept[i];
-            std::vector<MvContrib> out(static_cast<size_t>(kept[items.size()]));
+            hvec<MvContrib> out(static_cast<size_t>(kept[items.size()]));
             parallel_for(int64_t(items.size()), [&](int64_t i) {
                 MvItem &it = items[i];
                 std::copy(con.begin() + it.c_begin, con.begin() + it.c_begin + (kept[i + 1] - kept[i]),
@@ -458,8 +458,8 @@ void pack_tables(BF &bf) {
     auto split = [&](StageTable &S) -> int64_t {
         constexpr int64_t CHUNK = 4096;
         auto elems = [](const MvContrib &c) { return c.fofs < 0 ? int64_t(c.m) : int64_t(c.m) * c.n; };
-        std::vector<MvItem> mi, pi;
-        std::vector<MvContrib> mc, pc;
+        hvec<MvItem> mi, pi;
+        hvec<MvContrib> mc, pc;
         int64_t cur = 0;
         for (const MvItem &it : S.h_items) {
             int64_t total = 0;
@@ -605,7 +605,7 @@ void pack_tables(BF &bf) {
         upload_tab(S.contribs, S.h_contribs);
         upload_tab(S.tasks, S.h_tasks);
         {
-            std::vector<MvTaskC> tc(S.h_tasks.size());
+            hvec<MvTaskC> tc(S.h_tasks.size());
             parallel_for(int64_t(tc.size()), [&](int64_t i) {
                 const MvTask &t = S.h_tasks[i];
                 tc[i] = MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.m, t.n, t.c_begin, t.c_end};
