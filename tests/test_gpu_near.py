@@ -121,8 +121,10 @@ def test_near_overlapping_pairs_two_pass(bfgpu, oracle, ctx, sphere):
         assert Nf.info()["direct"] == 0
         D = oracle.near_dense(K, T, T, pairs)
         rng = np.random.default_rng(4)
-        X = np.asfortranarray(crandn(rng, 3000, 5).astype(dt))
-        Y0 = np.asfortranarray(crandn(rng, 3000, 5).astype(dt))
+        # 9 RHS: the matrix cores take only direct fields, so this is the
+        # vector two-pass product even in FP32
+        X = np.asfortranarray(crandn(rng, 3000, 9).astype(dt))
+        Y0 = np.asfortranarray(crandn(rng, 3000, 9).astype(dt))
         for copy in (True, False):
             Nf.set_transposed_copy(copy)
             for op, M in ((Nf, D), (Nf.adjoint(), D.conj().T), (Nf.transpose(), D.T)):
