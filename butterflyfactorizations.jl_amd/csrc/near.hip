@@ -605,7 +605,14 @@ Near *transposed_copy(Near &nf) {
         sw[p] = nf.pairs[p + 1];
         sw[p + 1] = nf.pairs[p];
     }
-    near_assemble(*t, nf.k, nf.self_re, nf.self_im, int64_t(sw.size() / 2), sw.data());
+    try {
+        near_assemble(*t, nf.k, nf.self_re, nf.self_im, int64_t(sw.size() / 2), sw.data());
+    } catch (const Error &e) {
+        // device memory ran short after all: keep the two-pass product
+        if (e.code != BFGPU_ERR_NOMEM) throw;
+        nf.tcopy_failed = true;
+        return nullptr;
+    }
     nf.tcopy = std::move(t);
     return nf.tcopy.get();
 }
