@@ -187,3 +187,19 @@ def test_sparse_level_matrices_apply_the_factorization(bfgpu, oracle):
         assert np.linalg.norm(got[:len(want)] - want) <= 1e-13 * np.linalg.norm(want)
         X = crandn(rng, tr.n_values, 3)
         assert np.allclose((M @ X)[:, 1], M @ X[:, 1], rtol=0, atol=1e-14 * np.abs(want).max())
+
+
+def test_public_header_is_plain_c(tmp_path):
+    """include/bfgpu.h is a C header (the boundary a Julia ccall, a cgo or
+    a ctypes binding reads): it compiles as C99 with warnings as errors."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "h.c"
+    src.write_text('#include "bfgpu.h"\nint main(void) { bfgpu_opts o; bfgpu_bf_info i; (void)o; (void)i; return 0; }\n')
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", inc, "-c", str(src), "-o",
+                        str(tmp_path / "h.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
