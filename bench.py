@@ -99,6 +99,8 @@ def kernel_name(d, op, real):
     if table == 3:
         return f"sweep_kernel<{real}, {nr}>"
     ink = 0 if (table == 1 or level > 0) else 1
+    if variant == 6:
+        return "pfree_kernel"
     if variant == 5:
         return f"mfma_rhs_kernel<{outk}, {(nr + 15) // 16}>"
     if variant == 4:
@@ -276,6 +278,9 @@ def main():
     launch_ms, desc = g32.launch_times(x32, y32, iters=max(args.steps, 10))
     stats = g32.launch_stats()
     ab = alg_bytes(stats, 8)
+    pfree = hasattr(g32, "get_option") and g32.get_option("pfree") == 1
+    if pfree:  # the last level computes its P entries: only its vectors move
+        ab[-1] = int((stats[-1][8] + stats[-1][9]) * 8)
     kname, kd, per_kernel = dominant_kernel(launch_ms, desc, stats, 8, "float")
     avg_launch_ms = kd["ms"] / kd["n"]
     bytes_per_launch = kd["bytes"] / kd["n"]
@@ -366,6 +371,7 @@ def main():
                      "avg_launch_ms": round(avg_launch_ms, 4), "alg_bytes_per_launch": int(bytes_per_launch),
                      "level_ms": [round(v, 4) for v in st_ms], "level_alg_bytes": ab,
                      "whole_matvec_gbs": round(whole, 1), "whole_matvec_frac": round(whole / HBM_PEAK_GBS, 4),
+                     "last_level_entries_computed": bool(pfree),
                      "kernels_ms_per_matvec": {k: round(v["ms"], 4) for k, v in per_kernel.items()}},
         "cpu_baseline": cpu,
         "multi_rhs": multi_rhs(stats, nr, ms32_32, rhs_ms, rhs_desc),

@@ -524,6 +524,8 @@ extern "C" int bfgpu_bf_convert(bfgpu_bf *h, int32_t precision, bfgpu_bf **out) 
         bf.n_comp = src.n_comp;
         bf.n_retry = src.n_retry;
         bf.t_factor_ms = src.t_factor_ms;
+        bf.pf_k = src.pf_k;
+        bf.use_pfree = src.use_pfree;
         bf.factors.resize(src.factors.size());
         const int es = precision == BFGPU_C128 ? 16 : 8;
         for (size_t L = 0; L < src.factors.size(); ++L) {
@@ -704,6 +706,11 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         bf.rhs_chunk = value;
         release_graph(bf);
         break;
+    case BFGPU_OPT_PFREE:
+        BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: pfree is 0 or 1");
+        bf.use_pfree = value != 0;
+        release_graph(bf);
+        break;
     case BFGPU_OPT_DENSE: {
         const TileShape sh = unpack_tile(value);
         BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
@@ -747,6 +754,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_RHS_LANES: *value = bf.rhs_lanes_min; break;
     case BFGPU_OPT_MFMA: *value = bf.use_mfma ? bf.mfma_rows : 0; break;
     case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
+    case BFGPU_OPT_PFREE: *value = bf.use_pfree && bf.pfree_ok ? 1 : 0; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }
     BFG_API_END

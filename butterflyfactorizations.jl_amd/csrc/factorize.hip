@@ -474,6 +474,7 @@ void compression_features(const Tree &test, const Tree &trial, double k, int32_t
 // Factorize n_pairs blocks; fills bf.bfs (reference layout) and bf.factors.
 void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int32_t, int32_t>> &pairs) {
     auto t0 = clk::now();
+    bf.pf_k = o.kernel ? 0.0 : o.k;  // P blocks of the point kernel can be recomputed
     Ctx &ctx = *bf.ctx;
     const Tree &test = *bf.test, &trial = *bf.trial;
     Engine E(ctx, test, trial, o);
@@ -714,10 +715,13 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
                     "factorize: P stage too large for 32-bit index lists");
         std::vector<int32_t> hrows(static_cast<size_t>(rofs[nP])), hcols(static_cast<size_t>(cofs[nP]));
         std::vector<FillTask> ftall(static_cast<size_t>(nP));
+        const bool kernel_entries = o.kernel == nullptr;  // point kernel: P is recomputable
         parallel_for(nB, [&](int64_t b) {
             Build &x = B[b];
             auto &st = bf.bfs[b].st[x.L].b;
             const auto &lv = x.tO.back();
+            if (kernel_entries)
+                bf.bfs[b].pcols.reserve(size_t(cofs[noff[b + 1]] - cofs[noff[b]]));
             for (size_t i = 0; i < lv.size(); ++i) {
                 int64_t q = noff[b] + int64_t(i);
                 int32_t Ol = lv[i];
@@ -727,6 +731,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
                 std::copy(col.begin(), col.end(), hcols.begin() + cofs[q]);
                 ftall[q] = f;
                 st.push_back(Block{Ol, x.ns, Ol, x.ns, f.m, f.n, 0, zo[q], -1, -1});
+                if (kernel_entries) bf.bfs[b].pcols.insert(bf.bfs[b].pcols.end(), col.begin(), col.end());
             }
         });
         std::vector<FillTask> ft;

@@ -109,6 +109,10 @@ struct OneBF {
     std::vector<int32_t> skel_idx;
     std::vector<CompRec> comp;
     std::vector<int64_t> stage_data_base;  // per stage: element base in that stage's buffer
+    // source point ids of the columns of the P blocks, block after block
+    // (P blocks are kernel entries G(observer leaf, these points)); empty
+    // when the P blocks are not known to be kernel entries
+    std::vector<int32_t> pcols;
 };
 
 // Work item of one stage launch: an output target with a list of
@@ -260,6 +264,13 @@ struct BF {
     bool use_mfma = true;         // RHS across lanes in complex FP32: matrix-core kernel
     int32_t mfma_rows = 16;       // rows of its tiles (16, 32, 48 or 64)
     int32_t rhs_chunk = 0;        // most right-hand sides per pass (0: as device memory allows)
+    // matrix-free last level (forward, one RHS, complex FP32): per observer
+    // leaf, its rows and the flat list of P columns (source point, input
+    // slot); the entries G(x, y) are computed from the points
+    bool pfree_ok = false, use_pfree = true;
+    int32_t pf_n_items = 0;
+    double pf_k = 0.0;            // wavenumber of the point kernel
+    DevBuf pf_items, pf_col, pf_in, pf_row;  // matrix-free last level tables
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
     ~BF();
@@ -382,6 +393,10 @@ void launch_rhs(hipStream_t s, int prec, const StageTable &t, const void *factor
 void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, void *slots, int32_t ns,
                      int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out, int nc, const double *alpha,
                      const double *beta, bool fuse, int n_cu);
+// Matrix-free last level (kernels.hip): y rows of every observer leaf from
+// the P columns, entries computed from the points.
+void launch_pfree(hipStream_t s, const BF &bf, const void *slots, void *y, int64_t ldy, const double *alpha,
+                  const double *beta, bool fuse);
 // Matrix-core tiles of a forward table, at most `rows` rows each (matvec.hip).
 void build_mfma(StageTable &t, int32_t rows);
 void launch_gather_rhs(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm, int32_t lo,

@@ -1259,6 +1259,120 @@ __global__ __launch_bounds__(256) void denset_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Matrix-free last level (forward, one RHS, complex FP32).  The P blocks of
+// the point kernel are kernel entries G(x_i, y_c) between the points of an
+// observer leaf and the skeleton points of its P columns, so the level is
+// computed from the points instead of read: one workgroup per observer
+// leaf, the four wavefronts on quarters of its flat column list, a lane per
+// column, rows in chunks of 16 accumulated per lane (row data are uniform
+// loads).  Coordinates are local to the leaf centre c: rows hold
+// u = x - c and |u|^2, columns d = y - c and the phase of |d| in
+// revolutions (from FP64), so the distance enters as
+//   r = |d| + delta,  delta = (|u|^2 - 2 d.u) / (r + |d|),
+// and the phase as frac(phase(|d|) + k delta / (2 pi)): its FP32 error is
+// relative to the leaf's size, not to the distance, and does not grow with
+// the electrical size of the geometry.  1 / (4 pi r) is the reciprocal
+// square root of r^2 (1 / (4 pi) folded into the column's input value);
+// sine and cosine are the hardware ones on revolutions.  The 16 row sums
+// leave the 64 lanes by a halving exchange and the wavefronts meet in LDS,
+// in fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pfree_kernel(const int4 *__restrict__ items,
+                                                    const float4 *__restrict__ cdat,
+                                                    const int32_t *__restrict__ colin,
+                                                    const float4 *__restrict__ rdat,
+                                                    const C<float> *__restrict__ slots, float kr,
+                                                    C<float> *__restrict__ y, const int32_t *__restrict__ perm_out,
+                                                    C<float> alpha, C<float> beta, int fuse_beta) {
+    __shared__ C<float> red[4][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int4 it = items[blockIdx.x];  // first row (tree position), rows, columns [z, w)
+    const int ncol = it.w - it.z;
+    const int q = (((ncol + 3) >> 2) + 63) & ~63;
+    const int cb = it.z + w * q, ce = min(it.w, cb + q);
+    constexpr float inv4pi = 0.07957747154594767f;
+    for (int r0 = 0; r0 < it.y; r0 += 16) {
+        const int nrow = min(16, it.y - r0);
+        float ux[16], uy[16], uz[16], uu[16];
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) {  // rows past nrow repeat the last one, unused
+            const float4 p = rdat[it.x + r0 + min(rr, nrow - 1)];
+            ux[rr] = p.x;
+            uy[rr] = p.y;
+            uz[rr] = p.z;
+            uu[rr] = p.w;
+        }
+        C<float> acc[16];
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) acc[rr] = C<float>{0.f, 0.f};
+        for (int c0 = cb; c0 < ce; c0 += 64) {
+            const int c = c0 + lane;
+            const int cc = min(c, ce - 1);
+            const float4 d = cdat[cc];
+            const C<float> xv0 = slots[colin[cc]];
+            const float sc = c < ce ? inv4pi : 0.f;
+            const C<float> xv{xv0.x * sc, xv0.y * sc};
+            const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+            const float D = D2 * __builtin_amdgcn_rsqf(D2);
+            const float mx = -2.f * d.x, my = -2.f * d.y, mz = -2.f * d.z;
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) {
+                const float ex = d.x - ux[rr], ey = d.y - uy[rr], ez = d.z - uz[rr];
+                const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                const float rs = __builtin_amdgcn_rsqf(r2);
+                const float num = fmaf(mx, ux[rr], fmaf(my, uy[rr], fmaf(mz, uz[rr], uu[rr])));
+                const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
+                const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
+                const float cs = __builtin_amdgcn_cosf(rev), sn = __builtin_amdgcn_sinf(rev);
+                const float tx = rs * xv.x, ty = rs * xv.y;
+                acc[rr].x = fmaf(cs, tx, fmaf(sn, ty, acc[rr].x));  // (cos - i sin) (tx + i ty)
+                acc[rr].y = fmaf(cs, ty, fmaf(-sn, tx, acc[rr].y));
+            }
+        }
+        // halving exchange: after the steps over lane bits 5, 4, 3, 2 a lane
+        // holds row (lane >> 2) & 15 summed over 16 lanes; bits 1, 0 in full
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            const int s = 32 >> st, K = 16 >> st, H = K >> 1;
+            const bool hi = (lane & s) != 0;
+#pragma unroll
+            for (int u = 0; u < H; ++u) {
+                const C<float> lo_v = acc[u], hi_v = acc[u + H];
+                const C<float> send = hi ? lo_v : hi_v;
+                const C<float> recv{__shfl_xor(send.x, s, 64), __shfl_xor(send.y, s, 64)};
+                acc[u] = cadd(hi ? hi_v : lo_v, recv);
+            }
+        }
+#pragma unroll
+        for (int s = 2; s > 0; s >>= 1) acc[0] = cadd(acc[0], C<float>{__shfl_xor(acc[0].x, s, 64), __shfl_xor(acc[0].y, s, 64)});
+        __syncthreads();  // red is free (previous chunk read)
+        if ((lane & 3) == 0) red[w][(lane >> 2) & 15] = acc[0];
+        __syncthreads();
+        if (w == 0 && lane < nrow) {
+            C<float> sum = red[0][lane];
+            sum = cadd(sum, red[1][lane]);
+            sum = cadd(sum, red[2][lane]);
+            sum = cadd(sum, red[3][lane]);
+            C<float> *yp = y + perm_out[it.x + r0 + lane];
+            *yp = cadd(fuse_beta ? cmul(beta, *yp) : *yp, cmul(alpha, sum));
+        }
+    }
+}
+
+void launch_pfree(hipStream_t s, const BF &bf, const void *slots, void *y, int64_t ldy, const double *alpha,
+                  const double *beta, bool fuse) {
+    (void)ldy;
+    if (bf.pf_n_items == 0) return;
+    hipLaunchKernelGGL(pfree_kernel, dim3(unsigned(bf.pf_n_items)), dim3(256), 0, s, bf.pf_items.as<const int4>(),
+                       bf.pf_col.as<const float4>(), bf.pf_in.as<const int32_t>(), bf.pf_row.as<const float4>(),
+                       static_cast<const C<float> *>(slots),
+                       float(bf.pf_k * 0.15915494309189535), static_cast<C<float> *>(y),
+                       bf.perm_test.as<const int32_t>(), C<float>{float(alpha[0]), float(alpha[1])},
+                       C<float>{float(beta[0]), float(beta[1])}, fuse ? 1 : 0);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Multi-RHS forward kernel: right-hand sides across lanes.  Lane r < nc of
 // a wavefront is right-hand side r; a wavefront takes one task at a time
 // (persistent, grid stride; the task index is wave-uniform, so task and

@@ -645,6 +645,91 @@ void pack_tables(BF &bf) {
     }
     bf.y_covered = covered_rows == bf.rows;
     bf.x_covered = covered_cols == bf.cols;
+
+    // ---- matrix-free last level: per observer leaf, its P columns ----
+    bf.pfree_ok = false;
+    bf.pf_n_items = 0;
+    bool pf = bf.precision == BFGPU_C64 && bf.pf_k > 0.0 && nb > 0;
+    for (int64_t b = 0; pf && b < nb; ++b) {
+        const OneBF &ob = bf.bfs[b];
+        int64_t nc = 0;
+        for (const Block &k : ob.st.back().b)  // rows: whole observer leaves (their centres frame them)
+            nc += k.ident || (k.m > 0 && !test.isleaf(k.obs_out)) ? -(int64_t(1) << 40) : int64_t(k.n);
+        pf = ob.st.size() >= 2 && nc == int64_t(ob.pcols.size());  // P reads skeleton slots
+    }
+    if (pf) {
+        const int64_t NV = test.n_nodes;
+        const int64_t nr = std::min<int64_t>(nb, 4 * int64_t(host_threads()));
+        auto rbeg = [&](int64_t r) { return nb * r / nr; };
+        std::vector<int64_t> pos(size_t(nr * NV), 0);
+        parallel_for(nr, [&](int64_t r) {
+            int64_t *c = &pos[size_t(r * NV)];
+            for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b)
+                for (const Block &k : bf.bfs[b].st.back().b)
+                    if (k.m > 0) c[k.obs_out] += k.n;
+        }, 1);
+        std::vector<int4> items;
+        int64_t total = 0;
+        for (int64_t v = 0; v < NV; ++v) {
+            const int64_t start = total;
+            for (int64_t r = 0; r < nr; ++r) {
+                const int64_t n = pos[size_t(r * NV + v)];
+                pos[size_t(r * NV + v)] = total;
+                total += n;
+            }
+            if (total > start) items.push_back(int4{test.val_begin[v], test.nvals(int32_t(v)), int32_t(start),
+                                                     int32_t(total)});
+        }
+        if (total < (int64_t(1) << 31) && !items.empty()) {
+            // columns: the skeleton point relative to the observer leaf's
+            // centre and the phase of its distance (revolutions, from FP64);
+            // rows: the point relative to its leaf's centre and its square norm
+            const double krev = bf.pf_k / (2.0 * M_PI);
+            hvec<float4> col(static_cast<size_t>(total));
+            hvec<int32_t> cin(static_cast<size_t>(total));
+            parallel_for(nr, [&](int64_t r) {
+                int64_t *c = &pos[size_t(r * NV)];
+                for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b) {
+                    const OneBF &ob = bf.bfs[b];
+                    const int32_t s = int32_t(ob.st.size()) - 1;
+                    const int64_t g0 = ix.first[ix.sf[b] + s];
+                    int64_t po = 0;
+                    for (size_t q = 0; q < ob.st[s].b.size(); ++q) {
+                        const Block &k = ob.st[s].b[q];
+                        if (k.m > 0) {
+                            const int64_t in0 = slot_base[b] + sl_in[size_t(g0) + q];
+                            const double *ctr = &test.center[3 * size_t(k.obs_out)];
+                            int64_t &p = c[k.obs_out];
+                            for (int32_t j = 0; j < k.n; ++j, ++p) {
+                                const double *y = &trial.points[3 * size_t(ob.pcols[size_t(po + j)])];
+                                const double d0 = y[0] - ctr[0], d1 = y[1] - ctr[1], d2 = y[2] - ctr[2];
+                                const double ph = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) * krev;
+                                col[size_t(p)] = float4{float(d0), float(d1), float(d2), float(ph - std::floor(ph))};
+                                cin[size_t(p)] = int32_t(in0 + j);
+                            }
+                        }
+                        po += k.n;
+                    }
+                }
+            }, 1);
+            std::vector<float4> row(size_t(std::max(test.n_values, 1)), float4{0.f, 0.f, 0.f, 0.f});
+            parallel_for(NV, [&](int64_t v) {
+                if (!test.isleaf(int32_t(v))) return;
+                const double *ctr = &test.center[3 * size_t(v)];
+                for (int32_t i = test.val_begin[v]; i < test.val_end[v]; ++i) {
+                    const double *x = &test.points[3 * size_t(test.perm[i])];
+                    const double u0 = x[0] - ctr[0], u1 = x[1] - ctr[1], u2 = x[2] - ctr[2];
+                    row[size_t(i)] = float4{float(u0), float(u1), float(u2), float(u0 * u0 + u1 * u1 + u2 * u2)};
+                }
+            }, 256);
+            upload_tab(bf.pf_items, items);
+            upload_tab(bf.pf_col, col);
+            upload_tab(bf.pf_in, cin);
+            upload_tab(bf.pf_row, row);
+            bf.pf_n_items = int32_t(items.size());
+            bf.pfree_ok = true;
+        }
+    }
     bf.gather_ofs = bf.part_ofs + part_used;
     bf.slot_len = bf.gather_ofs + std::max(test.n_values, trial.n_values);
     BFG_REQUIRE(bf.slot_len < (int64_t(1) << 31), "pack: slot buffer exceeds 2^31 entries");
@@ -1057,6 +1142,11 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
     for (int32_t i = 0; i < nL; ++i) {
         int32_t L = fwd ? i : nL - 1 - i;
         const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
+        if (fwd && L == nL - 1 && nrhs == 1 && bf.pfree_ok && bf.use_pfree) {
+            launch_pfree(s, bf, bf.slots.p, y, ldy, alpha, b_eff, fuse);  // P entries from the points
+            if (tl) tl->launched(s, {L, 0, 6, OUT_SCATTER, 0, 1});
+            continue;
+        }
         if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 1);
         run_table(bf, op, t, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 0);
     }

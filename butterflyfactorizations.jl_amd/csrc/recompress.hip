@@ -460,7 +460,9 @@ void recompress(const BF &src, double tau, BF &out) {
     out.n_stage_launch = nL;
     out.n_comp = src.n_comp;
     out.n_retry = src.n_retry;
-    out.bfs = src.bfs;
+    out.bfs = src.bfs;  // Q and P unchanged: P still holds kernel entries
+    out.pf_k = src.pf_k;
+    out.use_pfree = src.use_pfree;
     out.factor_elems.assign(size_t(nL), 0);
     struct Put {  // a block's data and its place in the new level buffer
         const void *from;

@@ -369,7 +369,14 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        0 (default): as many as the skeleton-vector buffer
  *                        of one pass allows in 90 % of the free device
  *                        memory (at most 64 with right-hand sides across
- *                        lanes) */
+ *                        lanes)
+ *   BFGPU_OPT_PFREE      1 (default): forward one-RHS complex FP32 products
+ *                        compute the last level's P entries (point-kernel
+ *                        entries between observer-leaf points and skeleton
+ *                        points) from the points instead of reading them;
+ *                        0: read the stored P blocks.  Applies to operators
+ *                        factorized here with the point kernel (and their
+ *                        recompressed or converted copies). */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -379,7 +386,8 @@ enum {
     BFGPU_OPT_DENSE = 6,
     BFGPU_OPT_RHS_LANES = 7,
     BFGPU_OPT_MFMA = 8,
-    BFGPU_OPT_RHS_CHUNK = 9
+    BFGPU_OPT_RHS_CHUNK = 9,
+    BFGPU_OPT_PFREE = 10
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

@@ -279,3 +279,47 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     op.set_option("mfma", 0)
     valu = (op @ Xs).astype(np.complex128)
     assert rel(valu, want) < 2e-6
+
+
+def test_matrix_free_last_level(bfgpu, oracle, ctx, tmp_path):
+    """Forward one-RHS FP32 products compute the last level's P entries
+    (point-kernel entries between observer-leaf points and skeleton points)
+    in the kernel: the product equals the stored-P product to FP32 rounding
+    of the entries (1e-5), with alpha / beta; FP64 operators and operators
+    read from a file keep the stored blocks."""
+    say("far field sphere N=6000 fp32: matrix-free last level")
+    pts = fibonacci_sphere(6000, 4.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    bt = bfgpu.BlockTree(T, T)
+    op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=1e-4,
+                                admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
+                                farfieldonly=True, ctx=ctx)
+    assert op.get_option("pfree") == 1
+    rng = np.random.default_rng(11)
+    x = crandn(rng, 6000).astype(np.complex64)
+    y0 = crandn(rng, 6000).astype(np.complex64)
+    on = op @ x
+    on_ab = op.mul(y0.copy(), x, 0.5 - 1j, 2.0)
+    import torch
+    xd = torch.from_numpy(x).to("cuda:0")
+    _, desc_on = op.launch_times(xd, torch.zeros_like(xd), iters=2)
+    assert any(int(d[2]) == 6 for d in desc_on)
+    op.set_option("pfree", 0)
+    assert op.get_option("pfree") == 0
+    off = op @ x
+    off_ab = op.mul(y0.copy(), x, 0.5 - 1j, 2.0)
+    assert rel(on, off) < 1e-5
+    assert rel(on_ab, off_ab) < 1e-5
+    # the FP64 form of the same operator through the oracle's matvec
+    imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
+    ref = oracle.matvec_many(imps, x.astype(np.complex128), rows=6000)
+    assert rel(on, ref) < 1e-5
+    op.set_option("pfree", 1)
+    # recompressed operators keep P (and the matrix-free level); files do not carry the points
+    assert op.recompress(1e-4).get_option("pfree") == 1
+    path = str(tmp_path / "s.bfgpu")
+    op.save(path)
+    assert bfgpu.load(path, bt, ctx=ctx).get_option("pfree") == 0
+    op64 = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=1e-4,
+                                  admissibility=bfgpu.CenterDistanceAdmissibility(2.0), farfieldonly=True, ctx=ctx)
+    assert op64.get_option("pfree") == 0

@@ -152,8 +152,15 @@ def test_fp32_paths(bfgpu, oracle, ctx):
     y64 = g64 @ x
     # FP32 application of the FP64 factors: rounding-level difference
     g32c = g64.convert(np.complex64)
+    g32c.set_option("pfree", 0)
     y32c = (g32c @ x.astype(np.complex64)).astype(np.complex128)
     assert rel(y32c, y64) < 1e-5
+    # the last level's kernel entries computed in FP32 arithmetic (on by
+    # default): a few FP32 rounding errors per entry more
+    g32c.set_option("pfree", 1)
+    assert g32c.get_option("pfree") == 1
+    y32f = (g32c @ x.astype(np.complex64)).astype(np.complex128)
+    assert rel(y32f, y64) < 3e-5
     # FP32 factorization: within the compression tolerance of the dense product
     g32 = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, tol, acctype=np.complex64,
                             ctx=ctx)
@@ -332,6 +339,11 @@ def test_saved_factorization_applies_identically(bfgpu, oracle, ctx, tmp_path):
     op2 = bfgpu.load(p2, B, ctx=ctx)
     assert op2.eltype == np.complex64 and op2.info().n_bf == len(op.farinteractions)
     xs = crandn(np.random.default_rng(10), 6000).astype(np.complex64)
+    # a file carries the stored blocks, not the points: the loaded operator
+    # reads its last level, bit-identical to the original's stored-block product
+    assert op.get_option("pfree") == 1 and op2.get_option("pfree") == 0
+    assert rel(op2 @ xs, op @ xs) < 1e-5
+    op.set_option("pfree", 0)
     assert np.array_equal(op2 @ xs, op @ xs)
     Xs = np.asfortranarray(crandn(np.random.default_rng(11), 6000, 40)).astype(np.complex64)
     assert np.array_equal(op2 @ Xs, op @ Xs)
