@@ -271,6 +271,7 @@ struct BF {
     int32_t pf_n_items = 0;
     double pf_k = 0.0;            // wavenumber of the point kernel
     DevBuf pf_items, pf_col, pf_in, pf_row;  // matrix-free last level tables
+    bool pfree_t_ok = false;                 // every P column's skeleton slot written once
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
     ~BF();
@@ -397,6 +398,7 @@ void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, vo
 // the P columns, entries computed from the points.
 void launch_pfree(hipStream_t s, const BF &bf, const void *slots, void *y, int64_t ldy, const double *alpha,
                   const double *beta, bool fuse);
+void launch_pfree_t(hipStream_t s, const BF &bf, const void *x, void *slots, bool conj);
 // Matrix-core tiles of a forward table, at most `rows` rows each (matvec.hip).
 void build_mfma(StageTable &t, int32_t rows);
 void launch_gather_rhs(hipStream_t s, int prec, const void *x, int64_t ldx, const int32_t *perm, int32_t lo,

@@ -1372,6 +1372,73 @@ void launch_pfree(hipStream_t s, const BF &bf, const void *slots, void *y, int64
     BFG_HIP(hipGetLastError());
 }
 
+// Matrix-free last level of the transposed / adjoint product (one RHS,
+// complex FP32): the first step of the transposed sweep writes every P
+// column's skeleton entry, sum over the leaf's rows of G(x_i, y_c) x_i
+// (conjugated for the adjoint).  One workgroup per observer leaf, a thread
+// per column (256 columns per pass, each written by exactly one thread), the
+// leaf's rows (local coordinates and input values) staged in LDS 256 at a
+// time and read as broadcasts.  Entries as in the forward kernel.
+template <bool CONJ>
+__global__ __launch_bounds__(256) void pfree_t_kernel(const int4 *__restrict__ items,
+                                                      const float4 *__restrict__ cdat,
+                                                      const int32_t *__restrict__ colin,
+                                                      const float4 *__restrict__ rdat,
+                                                      const C<float> *__restrict__ x,
+                                                      const int32_t *__restrict__ perm_in, float kr,
+                                                      C<float> *__restrict__ slots) {
+    __shared__ float4 su[256];
+    __shared__ C<float> sx[256];
+    const int t = threadIdx.x;
+    const int4 it = items[blockIdx.x];  // first row (tree position), rows, columns [z, w)
+    constexpr float inv4pi = 0.07957747154594767f;
+    constexpr float sgn = CONJ ? 1.f : -1.f;  // exp(-i k r), conjugated for the adjoint
+    for (int cb = it.z; cb < it.w; cb += 256) {
+        const int c = cb + t, cc = min(c, it.w - 1);
+        const float4 d = cdat[cc];
+        const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+        const float D = D2 * __builtin_amdgcn_rsqf(D2);
+        const float mx = -2.f * d.x, my = -2.f * d.y, mz = -2.f * d.z;
+        C<float> acc{0.f, 0.f};
+        for (int r0 = 0; r0 < it.y; r0 += 256) {
+            const int nr = min(256, it.y - r0);
+            __syncthreads();  // the previous rows are read
+            if (t < nr) {
+                su[t] = rdat[it.x + r0 + t];
+                const C<float> xv = x[perm_in[it.x + r0 + t]];
+                sx[t] = C<float>{xv.x * inv4pi, xv.y * inv4pi};
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int r = 0; r < nr; ++r) {
+                const float4 u = su[r];
+                const C<float> xv = sx[r];
+                const float ex = d.x - u.x, ey = d.y - u.y, ez = d.z - u.z;
+                const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                const float rs = __builtin_amdgcn_rsqf(r2);
+                const float num = fmaf(mx, u.x, fmaf(my, u.y, fmaf(mz, u.z, u.w)));
+                const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
+                const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
+                const float cs = __builtin_amdgcn_cosf(rev), sn = sgn * __builtin_amdgcn_sinf(rev);
+                const float tx = rs * xv.x, ty = rs * xv.y;
+                acc.x = fmaf(cs, tx, fmaf(-sn, ty, acc.x));  // (cos + i sn) (tx + i ty)
+                acc.y = fmaf(cs, ty, fmaf(sn, tx, acc.y));
+            }
+        }
+        if (c < it.w) slots[colin[c]] = acc;
+    }
+}
+
+void launch_pfree_t(hipStream_t s, const BF &bf, const void *x, void *slots, bool conj) {
+    if (bf.pf_n_items == 0) return;
+    auto k = conj ? pfree_t_kernel<true> : pfree_t_kernel<false>;
+    hipLaunchKernelGGL(k, dim3(unsigned(bf.pf_n_items)), dim3(256), 0, s, bf.pf_items.as<const int4>(),
+                       bf.pf_col.as<const float4>(), bf.pf_in.as<const int32_t>(), bf.pf_row.as<const float4>(),
+                       static_cast<const C<float> *>(x), bf.perm_test.as<const int32_t>(),
+                       float(bf.pf_k * 0.15915494309189535), static_cast<C<float> *>(slots));
+    BFG_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------------------------
 // Multi-RHS forward kernel: right-hand sides across lanes.  Lane r < nc of
 // a wavefront is right-hand side r; a wavefront takes one task at a time

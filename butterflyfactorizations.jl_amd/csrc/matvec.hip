@@ -647,7 +647,7 @@ void pack_tables(BF &bf) {
     bf.x_covered = covered_cols == bf.cols;
 
     // ---- matrix-free last level: per observer leaf, its P columns ----
-    bf.pfree_ok = false;
+    bf.pfree_ok = bf.pfree_t_ok = false;
     bf.pf_n_items = 0;
     bool pf = bf.precision == BFGPU_C64 && bf.pf_k > 0.0 && nb > 0;
     for (int64_t b = 0; pf && b < nb; ++b) {
@@ -665,8 +665,7 @@ void pack_tables(BF &bf) {
         parallel_for(nr, [&](int64_t r) {
             int64_t *c = &pos[size_t(r * NV)];
             for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b)
-                for (const Block &k : bf.bfs[b].st.back().b)
-                    if (k.m > 0) c[k.obs_out] += k.n;
+                for (const Block &k : bf.bfs[b].st.back().b) c[k.obs_out] += k.n;
         }, 1);
         std::vector<int4> items;
         int64_t total = 0;
@@ -696,7 +695,7 @@ void pack_tables(BF &bf) {
                     int64_t po = 0;
                     for (size_t q = 0; q < ob.st[s].b.size(); ++q) {
                         const Block &k = ob.st[s].b[q];
-                        if (k.m > 0) {
+                        {  // blocks of empty leaves too: the transposed product writes their slots
                             const int64_t in0 = slot_base[b] + sl_in[size_t(g0) + q];
                             const double *ctr = &test.center[3 * size_t(k.obs_out)];
                             int64_t &p = c[k.obs_out];
@@ -722,6 +721,12 @@ void pack_tables(BF &bf) {
                     row[size_t(i)] = float4{float(u0), float(u1), float(u2), float(u0 * u0 + u1 * u1 + u2 * u2)};
                 }
             }, 256);
+            {  // the transposed kernel writes each column's slot: one writer per slot
+                std::vector<uint8_t> seen(size_t(std::max<int64_t>(counter, 1)), 0);
+                bool once = true;
+                for (int64_t i = 0; i < total && once; ++i) once = seen[size_t(cin[size_t(i)])]++ == 0;
+                bf.pfree_t_ok = once;
+            }
             upload_tab(bf.pf_items, items);
             upload_tab(bf.pf_col, col);
             upload_tab(bf.pf_in, cin);
@@ -1145,6 +1150,11 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         if (fwd && L == nL - 1 && nrhs == 1 && bf.pfree_ok && bf.use_pfree) {
             launch_pfree(s, bf, bf.slots.p, y, ldy, alpha, b_eff, fuse);  // P entries from the points
             if (tl) tl->launched(s, {L, 0, 6, OUT_SCATTER, 0, 1});
+            continue;
+        }
+        if (!fwd && L == nL - 1 && nrhs == 1 && bf.pfree_t_ok && bf.use_pfree) {
+            launch_pfree_t(s, bf, x, bf.slots.p, op == BFGPU_OP_C);  // skeleton entries from the points
+            if (tl) tl->launched(s, {L, 0, 7, OUT_SLOT, 0, 1});
             continue;
         }
         if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 1);

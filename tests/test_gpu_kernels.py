@@ -282,11 +282,11 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
 
 
 def test_matrix_free_last_level(bfgpu, oracle, ctx, tmp_path):
-    """Forward one-RHS FP32 products compute the last level's P entries
-    (point-kernel entries between observer-leaf points and skeleton points)
-    in the kernel: the product equals the stored-P product to FP32 rounding
-    of the entries (1e-5), with alpha / beta; FP64 operators and operators
-    read from a file keep the stored blocks."""
+    """One-RHS FP32 products compute the last level's P entries (point-
+    kernel entries between observer-leaf points and skeleton points) in the
+    kernel, forward, adjoint and transpose: each product equals the stored-P
+    product to FP32 rounding of the entries (1e-5), with alpha / beta; FP64
+    operators and operators read from a file keep the stored blocks."""
     say("far field sphere N=6000 fp32: matrix-free last level")
     pts = fibonacci_sphere(6000, 4.0)
     T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
@@ -304,12 +304,22 @@ def test_matrix_free_last_level(bfgpu, oracle, ctx, tmp_path):
     xd = torch.from_numpy(x).to("cuda:0")
     _, desc_on = op.launch_times(xd, torch.zeros_like(xd), iters=2)
     assert any(int(d[2]) == 6 for d in desc_on)
+    # adjoint and transpose: the first transposed step writes the skeleton entries
+    adj_on, tr_on = op.adjoint() @ x, op.transpose() @ x
+    adj_ab = op.adjoint().mul(y0.copy(), x, 0.5 - 1j, 2.0)
+    _, desc_adj = op.adjoint().launch_times(xd, torch.zeros_like(xd), iters=2)
+    assert any(int(d[2]) == 7 for d in desc_adj)
     op.set_option("pfree", 0)
     assert op.get_option("pfree") == 0
     off = op @ x
     off_ab = op.mul(y0.copy(), x, 0.5 - 1j, 2.0)
     assert rel(on, off) < 1e-5
     assert rel(on_ab, off_ab) < 1e-5
+    assert rel(adj_on, op.adjoint() @ x) < 1e-5
+    assert rel(tr_on, op.transpose() @ x) < 1e-5
+    assert rel(adj_ab, op.adjoint().mul(y0.copy(), x, 0.5 - 1j, 2.0)) < 1e-5
+    # the point kernel is symmetric: transpose(F) x = F x up to the compression
+    assert rel(tr_on, on) < 1e-3
     # the FP64 form of the same operator through the oracle's matvec
     imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
     ref = oracle.matvec_many(imps, x.astype(np.complex128), rows=6000)
