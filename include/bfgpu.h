@@ -370,10 +370,11 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        of one pass allows in 90 % of the free device
  *                        memory (at most 64 with right-hand sides across
  *                        lanes)
- *   BFGPU_OPT_PFREE      1 (default): forward one-RHS complex FP32 products
- *                        compute the last level's P entries (point-kernel
- *                        entries between observer-leaf points and skeleton
- *                        points) from the points instead of reading them;
+ *   BFGPU_OPT_PFREE      1 (default): one-RHS complex FP32 products (forward,
+ *                        adjoint, transpose) compute the last level's P
+ *                        entries (point-kernel entries between observer-
+ *                        leaf points and skeleton points) from the points
+ *                        instead of reading them;
  *                        0: read the stored P blocks.  Applies to operators
  *                        factorized here with the point kernel (and their
  *                        recompressed or converted copies). */
