@@ -1008,8 +1008,8 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
     // right-hand sides across lanes), fewer when the caller caps them or
     // when the slot buffer of one pass would not fit in 90 % of the device
     // memory it may take
-    const bool lanes = op == BFGPU_OP_N && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min;
-    const int64_t full = lanes ? std::min<int64_t>(64, nrhs) : nrhs;
+    bool lanes = op == BFGPU_OP_N && bf.rhs_lanes_min > 0 && nrhs >= bf.rhs_lanes_min;
+    int64_t full = lanes ? std::min<int64_t>(64, nrhs) : nrhs;
     int64_t per = full;
     if (bf.rhs_chunk > 0) per = std::min<int64_t>(per, bf.rhs_chunk);
     auto cols_of = [&](int64_t c) { return size_t(lanes ? (c + 15) / 16 * 16 : c); };
@@ -1018,6 +1018,12 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
         size_t free_b = 0, total_b = 0;
         BFG_HIP(hipMemGetInfo(&free_b, &total_b));
         const size_t avail = size_t(0.9 * double(free_b + bf.slots_cap));
+        if (lanes && cols_of(1) * col_bytes > avail) {
+            // not even 16 interleaved columns fit: one column per RHS
+            lanes = false;
+            full = nrhs;
+            per = bf.rhs_chunk > 0 ? std::min<int64_t>(nrhs, bf.rhs_chunk) : nrhs;
+        }
         while (per > 1 && cols_of(per) * col_bytes > avail) per = lanes && per > 16 ? per - 16 : (per + 1) / 2;
         BFG_REQUIRE(cols_of(per) * col_bytes <= avail, "matvec: not enough device memory for the skeleton vectors");
     }
