@@ -722,9 +722,13 @@ void pack_tables(BF &bf) {
                 }
             }, 256);
             {  // the transposed kernel writes each column's slot: one writer per slot
-                std::vector<uint8_t> seen(size_t(std::max<int64_t>(counter, 1)), 0);
+                std::vector<bool> seen(size_t(std::max<int64_t>(counter, 1)), false);  // a bit per slot
                 bool once = true;
-                for (int64_t i = 0; i < total && once; ++i) once = seen[size_t(cin[size_t(i)])]++ == 0;
+                for (int64_t i = 0; i < total && once; ++i) {
+                    const size_t k = size_t(cin[size_t(i)]);
+                    once = !seen[k];
+                    seen[k] = true;
+                }
                 bf.pfree_t_ok = once;
             }
             upload_tab(bf.pf_items, items);
