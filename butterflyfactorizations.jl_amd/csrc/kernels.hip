@@ -1264,8 +1264,8 @@ __global__ __launch_bounds__(256) void denset_kernel(
 // observer leaf and the skeleton points of its P columns, so the level is
 // computed from the points instead of read: one workgroup per observer
 // leaf, the four wavefronts on quarters of its flat column list, a lane per
-// column, rows in chunks of 16 accumulated per lane (row data are uniform
-// loads).  Coordinates are local to the leaf centre c: rows hold
+// column, rows in chunks of 16 (8 for a leaf's last few rows) accumulated
+// per lane (row data are uniform loads).  Coordinates are local to the leaf centre c: rows hold
 // u = x - c and |u|^2, columns d = y - c and the phase of |d| in
 // revolutions (from FP64), so the distance enters as
 //   r = |d| + delta,  delta = (|u|^2 - 2 d.u) / (r + |d|),
@@ -1277,6 +1277,84 @@ __global__ __launch_bounds__(256) void denset_kernel(
 // leave the 64 lanes by a halving exchange and the wavefronts meet in LDS,
 // in fixed order.
 // ---------------------------------------------------------------------------
+// One chunk of NR rows (16, or 8 for a leaf's last few rows) of an item.
+template <int NR>
+__device__ __forceinline__ void pfree_rows(const int4 it, int r0, int cb, int ce, int lane, int w,
+                                           const float4 *__restrict__ cdat, const int32_t *__restrict__ colin,
+                                           const float4 *__restrict__ rdat, const C<float> *__restrict__ slots,
+                                           float kr, C<float> (*red)[16], C<float> *__restrict__ y,
+                                           const int32_t *__restrict__ perm_out, C<float> alpha, C<float> beta,
+                                           int fuse_beta) {
+    constexpr float inv4pi = 0.07957747154594767f;
+    constexpr int NS = NR == 16 ? 4 : 3;  // exchange steps: 2^NS = NR rows
+    const int nrow = min(NR, it.y - r0);
+    float ux[NR], uy[NR], uz[NR], uu[NR];
+#pragma unroll
+    for (int rr = 0; rr < NR; ++rr) {  // rows past nrow repeat the last one, unused
+        const float4 p = rdat[it.x + r0 + min(rr, nrow - 1)];
+        ux[rr] = p.x;
+        uy[rr] = p.y;
+        uz[rr] = p.z;
+        uu[rr] = p.w;
+    }
+    C<float> acc[NR];
+#pragma unroll
+    for (int rr = 0; rr < NR; ++rr) acc[rr] = C<float>{0.f, 0.f};
+    for (int c0 = cb; c0 < ce; c0 += 64) {
+        const int c = c0 + lane;
+        const int cc = min(c, ce - 1);
+        const float4 d = cdat[cc];
+        const C<float> xv0 = slots[colin[cc]];
+        const float sc = c < ce ? inv4pi : 0.f;
+        const C<float> xv{xv0.x * sc, xv0.y * sc};
+        const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+        const float D = D2 * __builtin_amdgcn_rsqf(D2);
+        const float mx = -2.f * d.x, my = -2.f * d.y, mz = -2.f * d.z;
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) {
+            const float ex = d.x - ux[rr], ey = d.y - uy[rr], ez = d.z - uz[rr];
+            const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+            const float rs = __builtin_amdgcn_rsqf(r2);
+            const float num = fmaf(mx, ux[rr], fmaf(my, uy[rr], fmaf(mz, uz[rr], uu[rr])));
+            const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
+            const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
+            const float cs = __builtin_amdgcn_cosf(rev), sn = __builtin_amdgcn_sinf(rev);
+            const float tx = rs * xv.x, ty = rs * xv.y;
+            acc[rr].x = fmaf(cs, tx, fmaf(sn, ty, acc[rr].x));  // (cos - i sin) (tx + i ty)
+            acc[rr].y = fmaf(cs, ty, fmaf(-sn, tx, acc[rr].y));
+        }
+    }
+    // halving exchange: after the steps over lane bits 5 .. 6 - NS a lane
+    // holds row (lane >> (6 - NS)) & (NR - 1) summed over 2^NS lanes; the
+    // lower bits in full
+#pragma unroll
+    for (int st = 0; st < NS; ++st) {
+        const int s = 32 >> st, K = NR >> st, H = K >> 1;
+        const bool hi = (lane & s) != 0;
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            const C<float> lo_v = acc[u], hi_v = acc[u + H];
+            const C<float> send = hi ? lo_v : hi_v;
+            const C<float> recv{__shfl_xor(send.x, s, 64), __shfl_xor(send.y, s, 64)};
+            acc[u] = cadd(hi ? hi_v : lo_v, recv);
+        }
+    }
+#pragma unroll
+    for (int s = 32 >> NS; s > 0; s >>= 1)
+        acc[0] = cadd(acc[0], C<float>{__shfl_xor(acc[0].x, s, 64), __shfl_xor(acc[0].y, s, 64)});
+    __syncthreads();  // red is free (previous chunk read)
+    if ((lane & ((1 << (6 - NS)) - 1)) == 0) red[w][(lane >> (6 - NS)) & (NR - 1)] = acc[0];
+    __syncthreads();
+    if (w == 0 && lane < nrow) {
+        C<float> sum = red[0][lane];
+        sum = cadd(sum, red[1][lane]);
+        sum = cadd(sum, red[2][lane]);
+        sum = cadd(sum, red[3][lane]);
+        C<float> *yp = y + perm_out[it.x + r0 + lane];
+        *yp = cadd(fuse_beta ? cmul(beta, *yp) : *yp, cmul(alpha, sum));
+    }
+}
+
 __global__ __launch_bounds__(256) void pfree_kernel(const int4 *__restrict__ items,
                                                     const float4 *__restrict__ cdat,
                                                     const int32_t *__restrict__ colin,
@@ -1290,71 +1368,15 @@ __global__ __launch_bounds__(256) void pfree_kernel(const int4 *__restrict__ ite
     const int ncol = it.w - it.z;
     const int q = (((ncol + 3) >> 2) + 63) & ~63;
     const int cb = it.z + w * q, ce = min(it.w, cb + q);
-    constexpr float inv4pi = 0.07957747154594767f;
-    for (int r0 = 0; r0 < it.y; r0 += 16) {
-        const int nrow = min(16, it.y - r0);
-        float ux[16], uy[16], uz[16], uu[16];
-#pragma unroll
-        for (int rr = 0; rr < 16; ++rr) {  // rows past nrow repeat the last one, unused
-            const float4 p = rdat[it.x + r0 + min(rr, nrow - 1)];
-            ux[rr] = p.x;
-            uy[rr] = p.y;
-            uz[rr] = p.z;
-            uu[rr] = p.w;
-        }
-        C<float> acc[16];
-#pragma unroll
-        for (int rr = 0; rr < 16; ++rr) acc[rr] = C<float>{0.f, 0.f};
-        for (int c0 = cb; c0 < ce; c0 += 64) {
-            const int c = c0 + lane;
-            const int cc = min(c, ce - 1);
-            const float4 d = cdat[cc];
-            const C<float> xv0 = slots[colin[cc]];
-            const float sc = c < ce ? inv4pi : 0.f;
-            const C<float> xv{xv0.x * sc, xv0.y * sc};
-            const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
-            const float D = D2 * __builtin_amdgcn_rsqf(D2);
-            const float mx = -2.f * d.x, my = -2.f * d.y, mz = -2.f * d.z;
-#pragma unroll
-            for (int rr = 0; rr < 16; ++rr) {
-                const float ex = d.x - ux[rr], ey = d.y - uy[rr], ez = d.z - uz[rr];
-                const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                const float rs = __builtin_amdgcn_rsqf(r2);
-                const float num = fmaf(mx, ux[rr], fmaf(my, uy[rr], fmaf(mz, uz[rr], uu[rr])));
-                const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
-                const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
-                const float cs = __builtin_amdgcn_cosf(rev), sn = __builtin_amdgcn_sinf(rev);
-                const float tx = rs * xv.x, ty = rs * xv.y;
-                acc[rr].x = fmaf(cs, tx, fmaf(sn, ty, acc[rr].x));  // (cos - i sin) (tx + i ty)
-                acc[rr].y = fmaf(cs, ty, fmaf(-sn, tx, acc[rr].y));
-            }
-        }
-        // halving exchange: after the steps over lane bits 5, 4, 3, 2 a lane
-        // holds row (lane >> 2) & 15 summed over 16 lanes; bits 1, 0 in full
-#pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            const int s = 32 >> st, K = 16 >> st, H = K >> 1;
-            const bool hi = (lane & s) != 0;
-#pragma unroll
-            for (int u = 0; u < H; ++u) {
-                const C<float> lo_v = acc[u], hi_v = acc[u + H];
-                const C<float> send = hi ? lo_v : hi_v;
-                const C<float> recv{__shfl_xor(send.x, s, 64), __shfl_xor(send.y, s, 64)};
-                acc[u] = cadd(hi ? hi_v : lo_v, recv);
-            }
-        }
-#pragma unroll
-        for (int s = 2; s > 0; s >>= 1) acc[0] = cadd(acc[0], C<float>{__shfl_xor(acc[0].x, s, 64), __shfl_xor(acc[0].y, s, 64)});
-        __syncthreads();  // red is free (previous chunk read)
-        if ((lane & 3) == 0) red[w][(lane >> 2) & 15] = acc[0];
-        __syncthreads();
-        if (w == 0 && lane < nrow) {
-            C<float> sum = red[0][lane];
-            sum = cadd(sum, red[1][lane]);
-            sum = cadd(sum, red[2][lane]);
-            sum = cadd(sum, red[3][lane]);
-            C<float> *yp = y + perm_out[it.x + r0 + lane];
-            *yp = cadd(fuse_beta ? cmul(beta, *yp) : *yp, cmul(alpha, sum));
+    for (int r0 = 0; r0 < it.y;) {  // chunks of 16 rows; at most 8 left: one chunk of 8
+        if (it.y - r0 > 8) {
+            pfree_rows<16>(it, r0, cb, ce, lane, w, cdat, colin, rdat, slots, kr, red, y, perm_out, alpha, beta,
+                           fuse_beta);
+            r0 += 16;
+        } else {
+            pfree_rows<8>(it, r0, cb, ce, lane, w, cdat, colin, rdat, slots, kr, red, y, perm_out, alpha, beta,
+                          fuse_beta);
+            r0 += 8;
         }
     }
 }
