@@ -214,14 +214,14 @@ class HelmholtzKernelMatrix:
 
 @dataclass
 class ButterflyRankEstimator:
-    """rankestimation.jl:197-205: n = max(ceil(Ceps * ln(1/eps)), Rmin)."""
+    """rankestimation.jl:32-40,93-98: n = max(ceil(Ceps * ln(1/eps)), Rmin)."""
     Ceps: float = 4.0
     Rmin: int = 10
 
 
 @dataclass
 class GeometricRankEstimator:
-    """rankestimation.jl:178-189 with estimate_rank_3d on trees."""
+    """rankestimation.jl:13-24,70-88: estimate_rank_3d on trees."""
     C: float = 1.5
     Ceps: float = 1.5
     Rmin: int = 3
@@ -526,7 +526,7 @@ class ButterflyFactorization:
         return np.concatenate(recs), np.concatenate(feats)
 
     def set_fused(self, enable=True):
-        """Fused single-launch forward sweep (default) or one launch per level."""
+        """Fused single-launch forward sweep (opt-in) or one launch per level (the default)."""
         check(lib().bfgpu_bf_set_fused(self.handle, 1 if enable else 0))
         return self
 

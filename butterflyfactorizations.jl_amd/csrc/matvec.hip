@@ -207,6 +207,18 @@ void pack_tables(BF &bf) {
     auto tp0 = clk::now();
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     const BlockIndex ix = index_blocks(bf);
+    // block keys and roots index the trees' node arrays below
+    parallel_for(nb, [&](int64_t b) {
+        const OneBF &ob = bf.bfs[b];
+        BFG_REQUIRE(ob.no >= 0 && ob.no < test.n_nodes && ob.ns >= 0 && ob.ns < trial.n_nodes,
+                    "pack: root node outside the trees");
+        for (const StageBlocks &st : ob.st)
+            for (const Block &k : st.b)
+                BFG_REQUIRE(k.obs_out >= 0 && k.obs_out < test.n_nodes && k.obs_in >= 0 && k.obs_in < test.n_nodes &&
+                                k.src_out >= 0 && k.src_out < trial.n_nodes && k.src_in >= 0 &&
+                                k.src_in < trial.n_nodes,
+                            "pack: block key outside the trees");
+    }, 256);
     // ---- slot offsets (per factorization, local), reference slot ids ----
     std::vector<int32_t> sl_out(size_t(ix.total), -1), sl_in(size_t(ix.total), -1);
     std::vector<int64_t> slot_base(nb + 1, 0);

@@ -203,15 +203,29 @@ extern "C" int bfgpu_bf_load(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu
             BFG_HIP(hipMemcpy(static_cast<char *>(bf.factors[L].p) + o, buf.data(), n, hipMemcpyHostToDevice));
         }
     }
-    // block data must lie in its level's buffer; the table packer checks
-    // every other range (slots, inputs, outputs) before any launch
+    // Block keys index the trees' node arrays in the table packer, and block
+    // data must lie in its level's buffer: both are checked here, before
+    // the packer reads any of them; the packer checks the slot, input and
+    // output ranges it derives before any launch.
+    const Tree &te = test->t, &tr = trial->t;
     for (const OneBF &ob : bf.bfs)
         for (size_t s = 0; s < ob.st.size(); ++s) {
             const int32_t L = (s + 1 == ob.st.size()) ? bf.n_stage_launch - 1 : int32_t(s);
-            for (const Block &b : ob.st[s].b)
+            const bool first = s == 0, last = s + 1 == ob.st.size();
+            for (const Block &b : ob.st[s].b) {
+                BFG_REQUIRE(b.obs_out >= 0 && b.obs_out < te.n_nodes && b.obs_in >= 0 && b.obs_in < te.n_nodes &&
+                                b.src_out >= 0 && b.src_out < tr.n_nodes && b.src_in >= 0 && b.src_in < tr.n_nodes,
+                            "corrupt file (block key outside the trees)");
                 BFG_REQUIRE(b.m >= 0 && b.n >= 0 &&
                                 (b.ident ? b.m == b.n : b.fofs >= 0 && b.fofs + int64_t(b.m) * b.n <= bf.factor_elems[L]),
                             "corrupt file (block outside its factor buffer)");
+                // Q blocks read the values of their source node, P blocks
+                // write the values of their observer node
+                BFG_REQUIRE(!first || b.ident || b.n == tr.nvals(b.src_in),
+                            "corrupt file (Q block columns differ from its source node's values)");
+                BFG_REQUIRE(!last || b.ident || b.m == te.nvals(b.obs_out),
+                            "corrupt file (P block rows differ from its observer node's values)");
+            }
         }
     pack_tables(bf);
     *out = h.release();

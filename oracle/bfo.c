@@ -70,8 +70,8 @@ void bfo_helmholtz_fill(double k, const double *obs_xyz, const int32_t *rows, in
 }
 
 /* ------------------------------------------------------------------------ */
-/* Rank estimators: src/ButterflyFactorization/rankestimation.jl:255-263     */
-/* (estimate_rank_butterfly) and :235-253 / ext/BFH2Trees/h2rankest.jl:8-38  */
+/* Rank estimators: src/ButterflyFactorization/rankestimation.jl:93-98      */
+/* (estimate_rank_butterfly) and :70-88 / ext/BFH2Trees/h2rankest.jl:8-38    */
 /* (estimate_rank_3d on trees).                                              */
 /* ------------------------------------------------------------------------ */
 int32_t bfo_rank_butterfly(double eps, double c_tau, int32_t rmin, double *x1, double *x2) {

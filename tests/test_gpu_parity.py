@@ -48,12 +48,12 @@ def test_from_host_fp64_matvec_matches_oracle(bfgpu, oracle, ctx):
                                                   ex["data"], ctx=ctx)
     rng = np.random.default_rng(0)
     x = crandn(rng, 1024)
-    y = g @ x
+    y = g @ x  # one launch per level (the default)
     assert rel(y, obf.matvec(x)) < 1e-12
-    # one launch per level gives the same product as the fused sweep
-    y_levels = g.set_fused(False) @ x
-    g.set_fused(True)
-    assert rel(y_levels, obf.matvec(x)) < 1e-12
+    # the opt-in fused sweep (all levels in one launch) gives the same product
+    y_fused = g.set_fused(True) @ x
+    g.set_fused(False)
+    assert rel(y_fused, obf.matvec(x)) < 1e-12
     # adjoint / transpose without materialisation vs the oracle's copies
     assert rel(g.adjoint() @ x, obf.transform(1).matvec(x)) < 1e-12
     assert rel(g.transpose() @ x, obf.transform(0).matvec(x)) < 1e-12
@@ -361,6 +361,24 @@ def test_saved_factorization_applies_identically(bfgpu, oracle, ctx, tmp_path):
         f.write(b"not a factorization")
     with pytest.raises(bfgpu.ArgumentError):
         bfgpu.load(short, B, ctx=ctx)
+
+    # block keys index the trees: a key outside them, or a Q block whose
+    # columns are not its source node's values, is rejected at load.  The
+    # first Q block's record follows the 60-byte header, the root record
+    # (3 int32) and the stage's block count (int64): keys obs_out, src_out,
+    # obs_in, src_in, then m, n (int32 each).
+    import struct
+    with open(path, "rb") as f:
+        good = f.read()
+    q0 = 60 + 12 + 8
+    assert struct.unpack_from("<6i", good, q0)[5] > 0
+    for field, val in ((3, 10 ** 6), (0, -1), (2, 1 << 30), (5, 3)):
+        bad = bytearray(good)
+        struct.pack_into("<i", bad, q0 + 4 * field, val)
+        with open(short, "wb") as f:
+            f.write(bytes(bad))
+        with pytest.raises(bfgpu.ArgumentError):
+            bfgpu.load(short, bt, ctx=ctx)
 
 
 def test_gpu_recompression_matches_oracle(bfgpu, oracle, ctx):
