@@ -352,8 +352,11 @@ def _apply(cfun, handle, op, shape, prec, ctx, y, x, alpha, beta):
         if nrhs != nrhs2:
             raise ArgumentError(1, "x and y have different numbers of columns")
         ctx.set_stream(torch.cuda.current_stream(x.device).cuda_stream)
-        check(fn(handle, op, nrhs, C.c_void_p(x.data_ptr()), ldx, C.c_void_p(y.data_ptr()), ldy, a, b,
-                 L.XY_DEVICE))
+        if cfun == "bfgpu_matvec_noalloc":
+            check(fn(handle, op, nrhs, C.c_void_p(x.data_ptr()), ldx, C.c_void_p(y.data_ptr()), ldy, a, b))
+        else:
+            check(fn(handle, op, nrhs, C.c_void_p(x.data_ptr()), ldx, C.c_void_p(y.data_ptr()), ldy, a, b,
+                     L.XY_DEVICE))
         return y
     if not (isinstance(x, np.ndarray) and isinstance(y, np.ndarray)):
         raise ArgumentError(1, "x and y must be numpy arrays or torch tensors")
@@ -444,6 +447,15 @@ class ButterflyFactorization:
     def mul(self, y, x, alpha=1.0, beta=0.0):
         """y = alpha * op(A) x + beta * y  (mul!, bfvector.jl:200-296)."""
         return _apply("bfgpu_matvec", self.handle, self.op, self.shape, self._H.precision, self.ctx,
+                      y, x, alpha, beta)
+
+    def mul_noalloc(self, y, x, alpha=1.0, beta=0.0):
+        """y = alpha * op(A) x + beta * y on GPU tensors through the
+        allocation-free entry point (bfgpu_matvec_noalloc): skeleton vectors
+        in the library's static scratch array, nothing allocated."""
+        if not (_is_torch(x) and _is_torch(y)):
+            raise ArgumentError(1, "mul_noalloc takes torch GPU tensors")
+        return _apply("bfgpu_matvec_noalloc", self.handle, self.op, self.shape, self._H.precision, self.ctx,
                       y, x, alpha, beta)
 
     _layout = staticmethod(lambda *a: _layout(*a))
@@ -1049,6 +1061,14 @@ class PetrovGalerkinBF(ButterflyFactorization):
         if N is not None:
             N.mul(y, x, alpha, 1.0)
         return y
+
+    def mul_noalloc(self, y, x, alpha=1.0, beta=0.0):
+        """The allocation-free product covers the far field (the butterfly
+        factorizations): far-field-only operators, or ``farfield()``."""
+        if self._near_op() is not None:
+            raise ArgumentError(1, "mul_noalloc applies the far field only: call farfield().mul_noalloc and "
+                                   "add the near field's product")
+        return ButterflyFactorization.mul_noalloc(self, y, x, alpha, beta)
 
     def farfield(self):
         """The far-field part alone, as a ButterflyFactorization view."""

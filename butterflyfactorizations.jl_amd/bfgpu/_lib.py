@@ -102,6 +102,7 @@ _SIGS = {
     "bfgpu_bf_launch_stats": (i32, [vp, i32, P(i32), P(i64)]),
     "bfgpu_bf_convert": (i32, [vp, i32, P(vp)]),
     "bfgpu_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
+    "bfgpu_matvec_noalloc": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64)]),
     "bfgpu_matvec_timed": (i32, [vp, i32, i64, vp, i64, vp, i64, i32, P(f64), P(i64), P(i32), P(f64)]),
     "bfgpu_matvec_launch_times": (i32, [vp, i32, i64, vp, i64, vp, i64, i32, i32, P(i32), P(f64), P(i32)]),
     "bfgpu_bf_get_tuning": (i32, [vp, P(i32), P(i32)]),

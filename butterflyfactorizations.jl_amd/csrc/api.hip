@@ -42,6 +42,9 @@ void matvec_launch_times(BF &bf, int op, int64_t nrhs, const void *x, int64_t ld
 void check_sweep_error(BF &bf);
 void release_graph(BF &bf);
 void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *variants);
+void noalloc_init(int device);
+void matvec_noalloc(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                    const double *alpha, const double *beta);
 }  // namespace bfg
 
 using namespace bfg;
@@ -84,6 +87,7 @@ extern "C" int bfgpu_ctx_create(int device, bfgpu_ctx **out) {
     h->c.stream = h->c.own;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->c.n_cu = prop.multiProcessorCount;
+    noalloc_init(device);  // device address of the allocation-free matvec's scratch array
     *out = h;
     BFG_API_END
 }
@@ -552,6 +556,15 @@ extern "C" int bfgpu_matvec(bfgpu_bf *h, int32_t op, int64_t nrhs, const void *x
     BFG_REQUIRE(h, "matvec: null handle");
     static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
     matvec(h->b, op, nrhs, x, ldx, y, ldy, alpha ? alpha : one, beta ? beta : zero, flags);
+    BFG_API_END
+}
+
+extern "C" int bfgpu_matvec_noalloc(bfgpu_bf *h, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
+                                    void *y, int64_t ldy, const double *alpha, const double *beta) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h, "matvec_noalloc: null handle");
+    static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
+    matvec_noalloc(h->b, op, nrhs, x, ldx, y, ldy, alpha ? alpha : one, beta ? beta : zero);
     BFG_API_END
 }
 

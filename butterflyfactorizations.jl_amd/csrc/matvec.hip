@@ -16,6 +16,8 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
+#include <string>
 #include <unordered_map>
 
 #include "internal.hpp"
@@ -976,13 +978,14 @@ struct Timeline {
 static int32_t nr_per_lane(int64_t nrhs) { return nrhs <= 1 ? 1 : nrhs == 2 ? 2 : nrhs <= 4 ? 4 : 8; }
 
 static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl, bool lanes);
+                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl, bool lanes,
+                           void *slots, bool sweep_allowed = true);
 
 // One launch table of level L: the tiled kernel (after permuting an
 // IN_GATHER input into the slot buffer) or the per-task kernels.
 static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nrhs, const void *x,
                       int64_t ldx, void *y, int64_t ldy, const double *alpha, const double *beta,
-                      bool fuse, Timeline *tl = nullptr, int32_t table = 0) {
+                      bool fuse, void *slots, Timeline *tl = nullptr, int32_t table = 0) {
     hipStream_t s = bf.ctx->stream;
     const bool fwd = op == BFGPU_OP_N;
     const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
@@ -993,22 +996,22 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
         if (t.in_kind == IN_GATHER) {
             const int es = bf.precision == BFGPU_C128 ? 16 : 8;
             launch_gather(s, bf.precision, x, ldx, pin, t.in_lo, t.in_hi,
-                          static_cast<char *>(bf.slots.p) + size_t(bf.gather_ofs) * es, bf.slot_len, nrhs);
+                          static_cast<char *>(slots) + size_t(bf.gather_ofs) * es, bf.slot_len, nrhs);
             shift = bf.gather_ofs;
             if (tl) tl->launched(s, {L, 2, 0, 0, 0, 1});
         }
         if (t.variant == 3 && fwd) {
-            launch_dense(s, bf.precision, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout, nrhs,
+            launch_dense(s, bf.precision, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout, nrhs,
                          alpha, beta, fuse, bf.ctx->n_cu);
         } else if (t.variant == 3) {
-            launch_denset(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
+            launch_denset(s, bf.precision, op, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
                           nrhs, alpha, beta, fuse, bf.ctx->n_cu);
         } else {
-            launch_tile(s, bf.precision, op, t, bf.factors[L].p, bf.slots.p, bf.slot_len, shift, y, ldy, pout,
+            launch_tile(s, bf.precision, op, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
                         nrhs, alpha, beta, fuse, bf.ctx->n_cu);
         }
     } else {
-        launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pin,
+        launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, slots, bf.slot_len, pin,
                      pout, nrhs, alpha, beta, fuse);
     }
     if (tl) tl->launched(s, {L, table, t.variant, t.out_kind, int32_t(pack_tile(t.tile)), nr_per_lane(nrhs)});
@@ -1076,7 +1079,7 @@ static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         }
     const bool fused = op == BFGPU_OP_N && !lanes && sweep_ok(bf, nrhs);
     if (tl || !bf.use_graph || fused || !graph_ok) {
-        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl, lanes);
+        stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl, lanes, bf.slots.p);
         return;
     }
     // per-level launches replayed from a captured graph while the call
@@ -1090,7 +1093,7 @@ static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         BFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         hipGraph_t g = nullptr;
         try {
-            stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr, lanes);
+            stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr, lanes, bf.slots.p);
         } catch (...) {
             (void)hipStreamEndCapture(s, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -1106,7 +1109,8 @@ static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
 }
 
 static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y,
-                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl, bool lanes) {
+                           int64_t ldy, const double *alpha, const double *beta, Timeline *tl, bool lanes,
+                           void *slots, bool sweep_allowed) {
     hipStream_t s = bf.ctx->stream;
     const int32_t nL = bf.n_stage_launch;
     const bool fwd = (op == BFGPU_OP_N);
@@ -1117,14 +1121,14 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
     const double *b_eff = fuse ? beta : one;
     const int32_t *pt = bf.perm_test.as<const int32_t>(), *pr = bf.perm_trial.as<const int32_t>();
     if (tl) tl->start(s);
-    if (fwd && !lanes && sweep_ok(bf, nrhs)) {
+    if (fwd && !lanes && sweep_allowed && sweep_ok(bf, nrhs)) {
         SweepTables &sw = bf.sweep;
         if (int64_t(sw.epoch + 1) * sw.max_wait >= (int64_t(1) << 30)) {
             launch_zero_i32(s, sw.counters.as<int32_t>(), sw.n_groups);
             sw.epoch = 0;
         }
         ++sw.epoch;
-        launch_sweep(s, bf.precision, sw, bf.factors, x, ldx, y, ldy, bf.slots.p, bf.slot_len, pr, pt,
+        launch_sweep(s, bf.precision, sw, bf.factors, x, ldx, y, ldy, slots, bf.slot_len, pr, pt,
                      nrhs, alpha, b_eff, fuse, bf.ctx->n_cu);
         if (tl) tl->launched(s, {-1, 3, 0, 0, 0, nr_per_lane(nrhs)});
         return;
@@ -1145,7 +1149,7 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
             char *yc = static_cast<char *>(y) + size_t(c0) * size_t(ldy) * es;
             const StageTable &t0 = bf.fwd[0];
             launch_gather_rhs(s, bf.precision, xc, ldx, pr, t0.in_lo, t0.in_hi,
-                              static_cast<char *>(bf.slots.p) + size_t(bf.gather_ofs) * ns * es, ns, nc);
+                              static_cast<char *>(slots) + size_t(bf.gather_ofs) * ns * es, ns, nc);
             if (tl) tl->launched(s, {0, 2, 4, 0, 0, nc});
             for (int32_t L = 0; L < nL; ++L) {
                 const StageTable &t = bf.fwd[L];
@@ -1154,10 +1158,10 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
                     if (!u) continue;
                     const int64_t shift = u->in_kind == IN_GATHER ? bf.gather_ofs : 0;
                     if (mfma) {
-                        launch_mfma_rhs(s, *u, bf.factors[L].p, bf.slots.p, ns, shift, yc, ldy, pt, nc, alpha,
+                        launch_mfma_rhs(s, *u, bf.factors[L].p, slots, ns, shift, yc, ldy, pt, nc, alpha,
                                         b_eff, fuse, bf.ctx->n_cu);
                     } else {
-                        launch_rhs(s, bf.precision, *u, bf.factors[L].p, bf.slots.p, ns, shift, yc, ldy, pt, nc,
+                        launch_rhs(s, bf.precision, *u, bf.factors[L].p, slots, ns, shift, yc, ldy, pt, nc,
                                    alpha, b_eff, fuse, bf.ctx->n_cu);
                     }
                     if (tl) tl->launched(s, {L, u == &t ? 0 : 1, mfma ? 5 : 4, u->out_kind, 0, nc});
@@ -1170,17 +1174,17 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         int32_t L = fwd ? i : nL - 1 - i;
         const StageTable &t = fwd ? bf.fwd[L] : bf.tr[L];
         if (fwd && L == nL - 1 && nrhs == 1 && bf.pfree_ok && bf.use_pfree) {
-            launch_pfree(s, bf, bf.slots.p, y, ldy, alpha, b_eff, fuse);  // P entries from the points
+            launch_pfree(s, bf, slots, y, ldy, alpha, b_eff, fuse);  // P entries from the points
             if (tl) tl->launched(s, {L, 0, 6, OUT_SCATTER, 0, 1});
             continue;
         }
         if (!fwd && L == nL - 1 && nrhs == 1 && bf.pfree_t_ok && bf.use_pfree) {
-            launch_pfree_t(s, bf, x, bf.slots.p, op == BFGPU_OP_C);  // skeleton entries from the points
+            launch_pfree_t(s, bf, x, slots, op == BFGPU_OP_C);  // skeleton entries from the points
             if (tl) tl->launched(s, {L, 0, 7, OUT_SLOT, 0, 1});
             continue;
         }
-        if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 1);
-        run_table(bf, op, t, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, tl, 0);
+        if (t.pre) run_table(bf, op, *t.pre, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, slots, tl, 1);
+        run_table(bf, op, t, L, nrhs, x, ldx, y, ldy, alpha, b_eff, fuse, slots, tl, 0);
     }
 }
 
@@ -1211,6 +1215,78 @@ void matvec(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, i
                              size_t(nrhs), hipMemcpyDeviceToHost, s));
     BFG_HIP(hipStreamSynchronize(s));
     check_sweep_error(bf);
+}
+
+// ---- allocation-free matvec ----
+// The skeleton vectors of bfgpu_matvec_noalloc live in one static device
+// array instead of the handle's slot buffer; every GPU that loads the
+// library holds its own copy until the process ends.  On the host the
+// array's name is a stand-in, not a device address: the device address is
+// looked up once per device (noalloc_init, called when a context is
+// created on that device, with the device current) and only that pointer
+// is ever handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(BFGPU_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static void *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::string noalloc_err[NOALLOC_MAX_DEVICES];
+
+void noalloc_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> g(noalloc_mu);
+    if (noalloc_ptr[device]) return;
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    void *p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+        // the other entry points do not need the array: keep the reason for
+        // bfgpu_matvec_noalloc to report
+        noalloc_err[device] = std::string("hipGetSymbolAddress(scratch): ") +
+                              (e != hipSuccess ? hipGetErrorString(e) : "null address");
+        return;
+    }
+    noalloc_ptr[device] = p;
+}
+
+// y = alpha op(A) x + beta y with device x, y: the per-level launches of
+// matvec with one column per right-hand side (never right-hand sides
+// across lanes, whose matrix-core tiles are built on first use, never the
+// fused sweep or a captured graph), skeleton vectors in the scratch array.
+// Every skeleton-vector entry a launch reads is written earlier in the same
+// call (the gather of x, then each level's outputs), so the array needs no
+// clearing; calls that overlap on one GPU share it and race.
+void matvec_noalloc(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
+                    const double *alpha, const double *beta) {
+    BFG_REQUIRE(op == BFGPU_OP_N || op == BFGPU_OP_T || op == BFGPU_OP_C, "matvec_noalloc: bad op");
+    BFG_REQUIRE(nrhs >= 0, "matvec_noalloc: nrhs < 0");
+    if (nrhs == 0) return;
+    BFG_REQUIRE(x && y, "matvec_noalloc: null x or y");
+    const bool fwd = (op == BFGPU_OP_N);
+    const int64_t in_len = fwd ? bf.cols : bf.rows, out_len = fwd ? bf.rows : bf.cols;
+    BFG_REQUIRE(ldx >= in_len && ldy >= out_len, "matvec_noalloc: leading dimension too small (DimensionMismatch)");
+    const int dev = bf.ctx->device;
+    BFG_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "matvec_noalloc: device id out of range");
+    void *slots = nullptr;
+    {
+        std::lock_guard<std::mutex> g(noalloc_mu);
+        slots = noalloc_ptr[dev];
+        if (!slots)
+            throw Error(BFGPU_ERR_STATE, "matvec_noalloc: no scratch array on device " + std::to_string(dev) +
+                                             (noalloc_err[dev].empty() ? std::string(" (no context created on it)")
+                                                                       : ": " + noalloc_err[dev]));
+    }
+    const int64_t es = bf.precision == BFGPU_C128 ? 16 : 8;
+    const int64_t max_rhs = int64_t(NOALLOC_SCRATCH_BYTES) / (std::max<int64_t>(bf.slot_len, 1) * es);
+    BFG_REQUIRE(nrhs <= max_rhs,
+                "matvec_noalloc: the skeleton vectors of " + std::to_string(nrhs) + " right-hand sides need " +
+                    std::to_string(std::max<int64_t>(bf.slot_len, 1) * es * nrhs) + " bytes, more than the " +
+                    std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte scratch array (at most " +
+                    std::to_string(max_rhs) + " right-hand sides for this operator)");
+    stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr, false, slots, false);
 }
 
 // Mean duration of every launch over `iters` device matvecs (alpha 1,
@@ -1340,9 +1416,9 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
     BFG_HIP(hipEventCreate(&e0));
     BFG_HIP(hipEventCreate(&e1));
     auto time_table = [&](int op, StageTable &t, int32_t L) {
-        run_table(bf, op, t, L, nrhs, dx.p, nv, dy.p, nv, one, zero, true);
+        run_table(bf, op, t, L, nrhs, dx.p, nv, dy.p, nv, one, zero, true, bf.slots.p);
         BFG_HIP(hipEventRecord(e0, s));
-        for (int32_t k = 0; k < iters; ++k) run_table(bf, op, t, L, nrhs, dx.p, nv, dy.p, nv, one, zero, true);
+        for (int32_t k = 0; k < iters; ++k) run_table(bf, op, t, L, nrhs, dx.p, nv, dy.p, nv, one, zero, true, bf.slots.p);
         BFG_HIP(hipEventRecord(e1, s));
         BFG_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

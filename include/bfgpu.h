@@ -295,6 +295,22 @@ int bfgpu_bf_convert(bfgpu_bf *bf, int32_t precision, bfgpu_bf **out);
 int bfgpu_matvec(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, int64_t ldx, void *y,
                  int64_t ldy, const double *alpha, const double *beta, int32_t flags);
 
+/* Allocation-free form of bfgpu_matvec: x and y are device pointers (work
+ * is enqueued on the context stream) and the call allocates no device or
+ * host memory.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The skeleton vectors of the call live
+ * in a static device array of BFGPU_NOALLOC_SCRATCH_BYTES per GPU, shared
+ * by every call on that GPU: calls that overlap on one GPU race.  A call
+ * whose skeleton vectors (bfgpu_bf_info.slot_len * nrhs * element size)
+ * exceed it fails with BFGPU_ERR_ARG.  One launch per level, one column per
+ * right-hand side (the matrix-core path across right-hand sides, the fused
+ * sweep and graph replay are not used); results equal bfgpu_matvec's to
+ * rounding (identical launches at one right-hand side). */
+#define BFGPU_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int bfgpu_matvec_noalloc(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, int64_t ldx, void *y,
+                         int64_t ldy, const double *alpha, const double *beta);
+
 /* Same as bfgpu_matvec with device x/y, repeated `iters` times with HIP
  * events around every stage launch on the context stream; stage_ms (length
  * n_stages) receives the mean duration per stage, total_ms the mean per
