@@ -6,11 +6,15 @@ right-hand side): 262,144 Fibonacci-lattice points on a sphere of radius 8
 wavelengths, Helmholtz kernel exp(-ik|x-y|)/(4 pi |x-y|) with k = 2 pi,
 octree with at most 64 points per leaf, separation-ratio admissibility 2.0,
 every admissible block pair butterfly-factorized at tolerance 1e-4.  The
-factorization is computed on the GPU in complex FP32 (the measured path)
-and complex FP64 (the plain path); the timed region applies the FP32
+factorization is computed on the GPU in two forms, each twice, interleaved
+(the fastest run of each counts): complex FP32 with learned ranks (the
+committed rank predictor, residual judge on) -- the measured path -- and
+complex FP64 with plain adaptive RRQR.  The timed region applies the FP32
 factorization (all admissible blocks, one batched launch per level) to a
-complex-Gaussian vector already resident in HBM.  `--workload squares`
-selects the single-block two-squares case (65,536 points per square).
+complex-Gaussian vector already resident in HBM; the FP64 form's matvec,
+both factorization times with their parts, and both errors against dense
+FP64 far-field rows are reported beside it.  `--workload squares` selects
+the single-block two-squares case (65,536 points per square, plain ranks).
 
 Prints one JSON line.  --gpus N > 1 is rejected: this build runs on one GPU.
 """
@@ -36,7 +40,10 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_sphere(bfgpu, n, k, tol, acctype, ctx):
+MODEL = os.path.join(ROOT, "butterflyfactorizations.jl_amd", "bfgpu", "data", "rank_mlp.npz")
+
+
+def build_sphere(bfgpu, n, k, tol, acctype, ctx, est=None):
     from bfgpu.geometry import fibonacci_sphere
     pts = fibonacci_sphere(n, 8.0 * np.sqrt(n / 262144))
     T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
@@ -44,18 +51,32 @@ def build_sphere(bfgpu, n, k, tol, acctype, ctx):
     t0 = time.perf_counter()
     op = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(k), bt, k, tol=tol,
                                 admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=acctype,
-                                farfieldonly=True, ctx=ctx)
+                                rankestimator=est, farfieldonly=True, ctx=ctx)
     return pts, pts, bt, op, time.perf_counter() - t0
 
 
-def build_squares(bfgpu, n, k, tol, acctype, ctx):
+def build_squares(bfgpu, n, k, tol, acctype, ctx, est=None):
     from bfgpu.geometry import two_squares
     obs, src = two_squares(n)
     bt = bfgpu.BlockTree(bfgpu.ClusterTree(obs, dims=2, leaf_size=64),
                          bfgpu.ClusterTree(src, dims=2, leaf_size=64))
     t0 = time.perf_counter()
-    op = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(k), bt, 0, 0, k, tol, acctype=acctype, ctx=ctx)
+    op = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(k), bt, 0, 0, k, tol, acctype=acctype, rankestimator=est,
+                           ctx=ctx)
     return obs, src, bt, op, time.perf_counter() - t0
+
+
+def factorization_record(op, t_wall):
+    """Wall time of one factorization with its parts (host planning and
+    block emission, row-sample preparation, fill + pivoted QR + judge on the
+    GPU, interpolation matrices, table packing) and its compression counts."""
+    i = op.info()
+    return {"s": round(t_wall, 3), "plan_s": round(i.t_plan_ms * 1e-3, 3),
+            "sample_s": round(i.t_sample_ms * 1e-3, 3), "fill_qr_s": round(i.t_qr_ms * 1e-3, 3),
+            "interp_s": round(i.t_interp_ms * 1e-3, 3), "pack_s": round(i.t_pack_ms * 1e-3, 3),
+            "compressions": int(i.n_comp), "doublings": int(i.n_retry), "judged": int(i.n_judged),
+            "fallbacks": int(i.n_fallback), "fallback_rate": round(int(i.n_fallback) / max(int(i.n_comp), 1), 6),
+            "factor_bytes": int(i.factor_bytes)}
 
 
 def far_rows_reference(oracle, op, bt, obs, src, k, x, rows):
@@ -193,6 +214,11 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--rank", choices=["learned", "plain"], default="",
+                    help="rank estimator of the FP32 operator (default: learned on the sphere, plain on the "
+                         "squares, for which the predictor is not trained)")
+    ap.add_argument("--reps", type=int, default=2,
+                    help="factorizations of each form, interleaved (the fastest is reported)")
     ap.add_argument("--err-rows", type=int, default=128)
     ap.add_argument("--tuning", default="", help="load launch tuning (.npy) instead of autotuning; "
                     "written there after autotuning when the file does not exist")
@@ -216,8 +242,33 @@ def main():
     torch.cuda.init()
     ctx.set_stream(torch.cuda.current_stream().cuda_stream)
 
-    log(f"{args.workload} n={n}: factorize fp32")
-    obs, src, bt, g32, t_f32 = build(bfgpu, n, k, tol, np.complex64, ctx)
+    rank = args.rank or ("learned" if args.workload == "sphere" else "plain")
+    est = None
+    if rank == "learned":
+        from bfgpu import learned
+        est = bfgpu.LearnedRankEstimator(learned.load(MODEL, ctx=ctx))
+    # the two forms of the north-star comparison, factorized in turn (the
+    # first large factorization of a process also pays for first-touch host
+    # memory, so each form runs `reps` times and the fastest run counts)
+    import gc
+    forms = [("fp64_plain", np.complex128, None)] if not args.no_fp64 else []
+    forms.append((f"fp32_{rank}", np.complex64, est))
+    runs = {f: [] for f, _, _ in forms}
+    ops = {}
+    for rep in range(max(1, args.reps)):
+        for form, dt, e in forms:
+            ops.pop(form, None)
+            gc.collect()
+            log(f"{args.workload} n={n}: factorize {form} (run {rep + 1})")
+            obs, src, bt, op, t = build(bfgpu, n, k, tol, dt, ctx, e)
+            runs[form].append(factorization_record(op, t))
+            ops[form] = op
+            log(f"factorize {form} {t:.2f}s: {runs[form][-1]}")
+    g32 = ops[f"fp32_{rank}"]
+    g64 = ops.get("fp64_plain")
+    best = {f: min(r, key=lambda v: v["s"]) for f, r in runs.items()}
+    t_f32 = best[f"fp32_{rank}"]["s"]
+    t_f64 = best["fp64_plain"]["s"] if g64 is not None else None
     i32 = g32.info()
     log(f"factorize fp32 {t_f32:.2f}s, {i32.n_comp} compressions, {i32.factor_bytes / 1e9:.2f} GB")
     t0 = time.perf_counter()
@@ -290,17 +341,15 @@ def main():
     traffic = read_traffic(kname, tag)
 
     # ---- plain FP64 path ----
-    ms64, t_f64, e64, fb64 = None, None, None, None
+    ms64, e64, fb64 = None, None, None
     y64h = None
-    if not args.no_fp64:
-        log("factorize fp64")
-        _, _, _, g64, t_f64 = build(bfgpu, n, k, tol, np.complex128, ctx)
+    if g64 is not None:
         g64.autotune(nrhs=1, iters=3)
         y64 = torch.zeros(rows_n, dtype=torch.complex128, device="cuda")
         ms64 = timed(g64, x64, y64, max(args.steps // 2, 3), 2)
         y64h = y64.cpu().numpy()
         fb64 = int(g64.info().factor_bytes)
-        log(f"factorize fp64 {t_f64:.2f}s, matvec fp64 {ms64:.4f} ms")
+        log(f"matvec fp64 {ms64:.4f} ms")
 
     # ---- error against dense FP64 rows of the far field ----
     rng = np.random.default_rng(1)
@@ -315,7 +364,7 @@ def main():
     # ---- CPU baseline: the oracle's flat matvec on a sample of blocks ----
     cpu = None
     if not args.no_cpu:
-        src_bf = g64 if not args.no_fp64 else g32
+        src_bf = g64 if g64 is not None else g32
         nbf = src_bf.n_factorizations() if hasattr(src_bf, "n_factorizations") else 1
         pick = np.sort(rng.choice(nbf, min(nbf, 8000), replace=False)) if nbf > 1 else np.array([0])
         threads = max(1, min(16, os.cpu_count() or 1))  # the host share of one GPU
@@ -353,10 +402,10 @@ def main():
                          f"{n1} runs {t1:.2f} ms, scaled {t1 * scale:.0f} ms"}
 
     workload = (f"sphere far field: {n} Fibonacci points, radius {8.0 * np.sqrt(n / 262144):g} wavelengths, "
-                f"k=2pi, octree leaf 64, admissibility 2.0, tol {tol:g}, 1 RHS, FP32 factorization"
+                f"k=2pi, octree leaf 64, admissibility 2.0, tol {tol:g}, 1 RHS, FP32 factorization, {rank} ranks"
                 if args.workload == "sphere" else
                 f"two parallel 16x16 squares, {n} points each, quadtree leaf 64, tol {tol:g}, k=2pi, 1 RHS, "
-                "FP32 factorization")
+                f"FP32 factorization, {rank} ranks")
     out = {
         "metric": METRIC,
         "value": round(ms32, 4), "unit": "ms", "n_gpus": 1, "steps": args.steps,
@@ -379,6 +428,13 @@ def main():
         "fp32_adjoint_ms": round(ms32_adj, 4), "fp32_32rhs_ms": round(ms32_32, 4),
         "factorize_s": {"fp32": round(t_f32, 2), "fp64": None if t_f64 is None else round(t_f64, 2),
                         "autotune_fp32": round(t_tune, 2)},
+        "north_star": {
+            "forms": {f: {"factorize": best[f], "runs_s": [r["s"] for r in runs[f]]} for f in runs},
+            "fp32_form": f"fp32_{rank}",
+            "factorize_speedup_vs_fp64_plain": None if t_f64 is None else round(t_f64 / t_f32, 3),
+            "matvec_ms": {f"fp32_{rank}": round(ms32, 4), "fp64_plain": None if ms64 is None else round(ms64, 4)},
+            "matvec_speedup_vs_fp64_plain": None if ms64 is None else round(ms64 / ms32, 3),
+            "err_vs_dense_rows": {f"fp32_{rank}": e32, "fp64_plain": e64}},
         "err_vs_dense_rows": {"rows": len(rows), "fp32": e32, "fp64": e64},
         "factor_bytes": {"fp32": int(i32.factor_bytes), "fp64": fb64},
         "compressions": int(i32.n_comp), "adaptive_retries": int(i32.n_retry),

@@ -229,9 +229,15 @@ class GeometricRankEstimator:
 
 @dataclass
 class LearnedRankEstimator:
-    """Rank predicted by a trained RankPredictor plus `oversample` rows."""
+    """Rank predicted by a trained RankPredictor plus `oversample` rows.
+    Each compression the pivoted QR accepts is then judged on `judge_rows`
+    held-out rows of the same sample; one whose residual exceeds
+    judge_slack * eps (relative to the held-out tile's largest column) is
+    redone by plain adaptive RRQR (judge_rows 0: no judge)."""
     predictor: "RankPredictor"
     oversample: int = 10
+    judge_rows: int = 8
+    judge_slack: float = 4.0
 
 
 @dataclass
@@ -283,6 +289,7 @@ def _opts(k, tol, rankestimator, adaptive, acctype, seed, workspace_bytes=0):
     elif isinstance(est, LearnedRankEstimator):
         o.rank_mode, o.mlp, o.oversample = L.RANK_LEARNED, est.predictor.handle, int(est.oversample)
         o.rmin = 1
+        o.judge_rows, o.judge_slack = int(est.judge_rows), float(est.judge_slack)
     else:
         raise ArgumentError(1, f"unknown rank estimator {type(est).__name__}")
     dt = np.dtype(acctype)
@@ -510,8 +517,10 @@ class ButterflyFactorization:
         comp = np.zeros((nc.value, 6), np.int32)
         check(lib().bfgpu_bf_export_logs(self.handle, index, ptr(skel, C.c_int32),
                                          ptr(skel_idx, C.c_int32), ptr(comp, C.c_int32)))
+        judge = np.zeros(max(nc.value, 1), np.int32)
+        check(lib().bfgpu_bf_export_judge(self.handle, index, ptr(judge, C.c_int32)))
         return {"n_stages": ns.value, "table": table, "data_ofs": ofs, "data": data[:nd.value],
-                "skel": skel, "skel_idx": skel_idx[:nski.value], "comp": comp,
+                "skel": skel, "skel_idx": skel_idx[:nski.value], "comp": comp, "judge": judge[:nc.value],
                 "roots": (roots[0], roots[1])}
 
     def to_mat(self, index=0):

@@ -34,7 +34,7 @@ class Opts(C.Structure):
         ("k", f64), ("tol", f64), ("adaptive", i32), ("rank_mode", i32),
         ("c_tau", f64), ("rmin", i32), ("c_geo", f64), ("c_eps", f64),
         ("precision", i32), ("oversample", i32), ("seed", u64), ("mlp", vp), ("workspace_bytes", i64),
-        ("kernel", vp),
+        ("kernel", vp), ("judge_rows", i32), ("judge_slack", f64),
     ]
 
 
@@ -45,7 +45,7 @@ class BFInfo(C.Structure):
         ("factor_bytes", i64), ("slot_len", i64), ("n_comp", i64),
         ("n_retry", i64), ("t_factor_ms", f64),
         ("t_plan_ms", f64), ("t_sample_ms", f64), ("t_qr_ms", f64), ("t_interp_ms", f64), ("t_pack_ms", f64),
-        ("device_bytes", i64),
+        ("device_bytes", i64), ("n_judged", i64), ("n_fallback", i64),
     ]
 
 
@@ -97,6 +97,7 @@ _SIGS = {
     "bfgpu_bf_export": (i32, [vp, i32, P(i32), P(i64), P(f64)]),
     "bfgpu_bf_export_logs": (i32, [vp, i32, P(i32), P(i32), P(i32)]),
     "bfgpu_bf_export_features": (i32, [vp, i32, P(f32)]),
+    "bfgpu_bf_export_judge": (i32, [vp, i32, P(i32)]),
     "bfgpu_bf_compressions": (i32, [vp, P(i64), P(i32), P(f32)]),
     "bfgpu_bf_stage_stats": (i32, [vp, P(i32), P(i64)]),
     "bfgpu_bf_launch_stats": (i32, [vp, i32, P(i32), P(i64)]),

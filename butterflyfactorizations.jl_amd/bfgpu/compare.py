@@ -56,7 +56,7 @@ def _form(pts, acctype, rankestimator, x, rows, ref, iters, ctx, log, t_prev=())
            "factorize_plus_matvec_s": round(t_fact + ms * 1e-3, 3),
            "gpu_fill_qr_s": round(inf.t_qr_ms * 1e-3, 3), "gpu_interp_s": round(inf.t_interp_ms * 1e-3, 3),
            "mean_rows": float(rec[:, 5].mean()), "mean_rank": float(rec[:, 6].mean()),
-           "fallbacks": int(inf.n_retry), "factor_bytes": int(inf.factor_bytes), "err_vs_dense": err}
+           "doublings": int(inf.n_retry), "judged": int(inf.n_judged), "fallbacks": int(inf.n_fallback), "factor_bytes": int(inf.factor_bytes), "err_vs_dense": err}
     log(f"{np.dtype(acctype).name} {'learned' if rankestimator else 'plain'}: {out}")
     del op, xd, yd
     torch.cuda.empty_cache()

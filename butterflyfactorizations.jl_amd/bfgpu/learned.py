@@ -10,8 +10,10 @@ a far-field factorization exports six geometric features and its rank
 (ReLU, FP32) is trained on them by the library's own forward, backward and
 Adam kernels.  With `LearnedRankEstimator` the prediction plus 10 rows sizes
 the row sample of PartialQR (compressors.jl:99-101); the adaptive doubling
-of plain RRQR (compressors.jl:134-151) stays as the fallback whenever the
-sampled block turns out nearly full rank.
+of plain RRQR (compressors.jl:134-151) still applies when the sampled block
+turns out nearly full rank, and a residual judge on 8 held-out rows of the
+same sample sends every compression whose interpolative decomposition
+misses the tolerance there back to plain adaptive RRQR.
 
     python -m bfgpu.learned train --geometries 200 --out rank_mlp.npz
     python -m bfgpu.learned evaluate --model rank_mlp.npz --points 65536
@@ -131,8 +133,10 @@ def summary(op, t_wall):
     rec, _ = op.compressions()
     return {"factorize_s": round(t_wall, 3), "gpu_fill_qr_s": round(i.t_qr_ms * 1e-3, 3),
             "gpu_interp_s": round(i.t_interp_ms * 1e-3, 3), "sample_prep_s": round(i.t_sample_ms * 1e-3, 3),
-            "compressions": int(i.n_comp), "fallbacks": int(i.n_retry),
-            "fallback_rate": float(i.n_retry) / max(int(i.n_comp), 1),
+            "compressions": int(i.n_comp), "doublings": int(i.n_retry),
+            "doubling_rate": float(i.n_retry) / max(int(i.n_comp), 1),
+            "judged": int(i.n_judged), "fallbacks": int(i.n_fallback),
+            "fallback_rate": float(i.n_fallback) / max(int(i.n_comp), 1),
             "mean_rows": float(rec[:, 5].mean()), "mean_rank": float(rec[:, 6].mean()),
             "factor_bytes": int(i.factor_bytes)}
 

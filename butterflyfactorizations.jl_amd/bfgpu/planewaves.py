@@ -131,7 +131,7 @@ def run(points=1048576, axes=(16.0, 12.0, 8.0), n_rhs=128, model=None, err_rows=
            "ranks": "learned" if net else "plain",
            "factorize_s": round(t_fact, 2), "matvec_ms": round(ms, 2), "matvec_ms_per_rhs": round(ms / n_rhs, 3),
            "admissible_pairs": int(len(op.farinteractions)), "compressions": int(inf.n_comp),
-           "fallbacks": int(inf.n_retry), "mean_rows": float(rec[:, 5].mean()),
+           "doublings": int(inf.n_retry), "judged": int(inf.n_judged), "fallbacks": int(inf.n_fallback), "mean_rows": float(rec[:, 5].mean()),
            "mean_rank": float(rec[:, 6].mean()), "factor_bytes": int(inf.factor_bytes),
            "operator_device_bytes_after_factorization": int(inf.device_bytes),
            "operator_device_bytes_after_matvec": int(held),

@@ -5,6 +5,8 @@
 #include <cstring>
 #include <string>
 
+#include <cmath>
+
 #include "internal.hpp"
 
 namespace bfg {
@@ -134,6 +136,8 @@ static void check_opts(const bfgpu_opts *o) {
     BFG_REQUIRE(o->k > 0.0, "factorize: wavenumber must be positive");
     BFG_REQUIRE(o->rank_mode >= 0 && o->rank_mode <= 2, "factorize: unknown rank_mode");
     BFG_REQUIRE(o->workspace_bytes >= 0, "factorize: workspace_bytes must be >= 0 (0: automatic)");
+    BFG_REQUIRE(o->judge_rows >= 0 && o->judge_rows <= 4096, "factorize: judge_rows must lie in [0, 4096]");
+    BFG_REQUIRE(std::isfinite(o->judge_slack), "factorize: judge_slack must be finite");
 }
 
 extern "C" int bfgpu_factorize(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
@@ -257,6 +261,8 @@ extern "C" int bfgpu_bf_info_get(const bfgpu_bf *h, bfgpu_bf_info *o) {
     o->slot_len = bf.slot_len;
     o->n_comp = bf.n_comp;
     o->n_retry = bf.n_retry;
+    o->n_judged = bf.n_judged;
+    o->n_fallback = bf.n_fallback;
     o->t_factor_ms = bf.t_factor_ms;
     o->t_plan_ms = bf.t_plan_ms;
     o->t_sample_ms = bf.t_sample_ms;
@@ -390,6 +396,15 @@ extern "C" int bfgpu_bf_export_logs(const bfgpu_bf *h, int32_t index, int32_t *s
             int32_t *o = comp_rec + 6 * i;
             o[0] = r.stage; o[1] = r.s; o[2] = r.o; o[3] = r.guess; o[4] = r.rows; o[5] = r.rank;
         }
+    BFG_API_END
+}
+
+extern "C" int bfgpu_bf_export_judge(const bfgpu_bf *h, int32_t index, int32_t *judge) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && judge, "bf_export_judge: null argument");
+    BFG_REQUIRE(index >= 0 && index < int32_t(h->b.bfs.size()), "bf_export_judge: index out of range");
+    const OneBF &ob = h->b.bfs[index];
+    for (size_t i = 0; i < ob.comp.size(); ++i) judge[i] = ob.comp[i].judge;
     BFG_API_END
 }
 
@@ -527,6 +542,8 @@ extern "C" int bfgpu_bf_convert(bfgpu_bf *h, int32_t precision, bfgpu_bf **out) 
         bf.precision = precision;
         bf.n_comp = src.n_comp;
         bf.n_retry = src.n_retry;
+        bf.n_judged = src.n_judged;
+        bf.n_fallback = src.n_fallback;
         bf.t_factor_ms = src.t_factor_ms;
         bf.pf_k = src.pf_k;
         bf.use_pfree = src.use_pfree;

@@ -117,6 +117,7 @@ struct Comp {
     double eps;
     uint64_t key;
     int32_t guess = 10, rows = 0, rank = 0;
+    int32_t judge = 0;  // 0 not judged, 1 accepted by the residual judge, 2 redone by plain RRQR
     std::vector<int32_t> skel;
     int64_t tofs = 0;
     float feat[6];
@@ -186,8 +187,12 @@ struct Engine {
     size_t es;
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
-    int64_t n_comp = 0, n_retry = 0, n_rank0 = 0;
+    // residual judge: held-out rows, their tiles, interpolation matrices of
+    // the judged compressions, judge tasks and results, copies into the pool
+    DevBuf d_jrows, d_jfill, d_jfill_end, d_jinterp, d_jtask, d_jres, d_J, d_Tj, d_cp, d_cp_end;
+    int64_t n_comp = 0, n_retry = 0, n_rank0 = 0, n_judged = 0, n_fallback = 0;
     double t_host = 0, t_qr = 0, t_interp = 0;  // seconds: sample prep, fill+QR, interpolation
+    double t_judge = 0;                         // seconds: held-out rows, fill, judge (part of t_qr)
 
     Engine(Ctx &c, const Tree &te, const Tree &tr, const bfgpu_opts &op)
         : ctx(c), test(te), trial(tr), o(op), prec(op.precision),
@@ -221,7 +226,9 @@ struct Engine {
     }
 
     void release() {
-        for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z})
+        for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z,
+                          &d_jrows, &d_jfill, &d_jfill_end, &d_jinterp, &d_jtask, &d_jres, &d_J, &d_Tj, &d_cp,
+                          &d_cp_end})
             *d = DevBuf();
     }
 
@@ -265,11 +272,103 @@ struct Engine {
         }
     }
 
+    // Residual judge (learned ranks) of the compressions `acc` of the
+    // current batch whose sample left observer rows unseen: the next
+    // judge_rows rows of the same sample prefix are filled, the
+    // interpolation matrices computed into d_Tj (tj_of[q]: their offset)
+    // and checked on those rows.  A compression passes when the largest
+    // residual column norm is at most judge_slack * eps times the largest
+    // column norm of the held-out tile; one that fails is redone by plain
+    // adaptive RRQR from max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.
+    // The fill, QR tiles, pivots and column lists of the batch must still
+    // be on the device.
+    void judge(std::vector<Comp> &cs, const std::vector<int32_t> &pend, const std::vector<int32_t> &acc,
+               const std::vector<QrTask> &qt, const std::vector<int64_t> &cofs, std::vector<int64_t> &tj_of,
+               std::vector<int32_t> &retry) {
+        auto t0 = clk::now();
+        std::vector<int32_t> jq;
+        for (int32_t q : acc) {
+            const Comp &c = cs[pend[q]];
+            if (c.judge == 0 && c.rows < c.n_obs && !c.cols.empty()) jq.push_back(q);
+        }
+        if (jq.empty()) return;
+        const int64_t nj = int64_t(jq.size());
+        std::vector<int64_t> jr(size_t(nj) + 1, 0);
+        for (int64_t a = 0; a < nj; ++a) {
+            const Comp &c = cs[pend[jq[a]]];
+            jr[a + 1] = jr[a] + std::min(o.judge_rows, c.n_obs - c.rows);
+        }
+        BFG_REQUIRE(jr[nj] < (int64_t(1) << 31), "factorize: judge rows exceed 32-bit index lists");
+        std::vector<int32_t> hj(size_t(std::max<int64_t>(jr[nj], 1)));
+        parallel_for(nj, [&](int64_t a) {
+            const Comp &c = cs[pend[jq[a]]];
+            const int32_t mc = int32_t(jr[a + 1] - jr[a]);
+            thread_local std::vector<int32_t> pre;
+            pre.resize(size_t(c.rows) + size_t(mc));
+            sample_prefix(c.key, c.n_obs, c.rows + mc, pre.data());
+            for (int32_t i = 0; i < mc; ++i) hj[size_t(jr[a] + i)] = c.obs[pre[size_t(c.rows + i)]];
+        });
+        std::vector<FillTask> jf(static_cast<size_t>(nj));
+        std::vector<int64_t> je(static_cast<size_t>(nj));
+        std::vector<InterpTask> ji(static_cast<size_t>(nj));
+        std::vector<JudgeTask> jt(static_cast<size_t>(nj));
+        int64_t zj = 0, tj = 0;
+        int32_t jmax_rank = 0;
+        for (int64_t a = 0; a < nj; ++a) {
+            const int32_t q = jq[a];
+            const Comp &c = cs[pend[q]];
+            const int32_t mc = int32_t(jr[a + 1] - jr[a]), n = int32_t(c.cols.size());
+            jf[a] = FillTask{zj, mc, n, int32_t(jr[a]), int32_t(cofs[q])};
+            zj += int64_t(mc) * n;
+            je[a] = zj;
+            ji[a] = InterpTask{qt[q].zofs, tj, qt[q].m, qt[q].n, c.rank, qt[q].pofs};
+            jt[a] = JudgeTask{jf[a].zofs, tj, mc, n, c.rank, qt[q].pofs};
+            tj_of[q] = tj;
+            tj += int64_t(c.rank) * n;
+            jmax_rank = std::max(jmax_rank, c.rank);
+        }
+        upload(d_jrows, hj, ctx.stream);
+        upload(d_jfill, jf, ctx.stream);
+        upload(d_jfill_end, je, ctx.stream);
+        upload(d_jinterp, ji, ctx.stream);
+        upload(d_jtask, jt, ctx.stream);
+        d_J.reserve(size_t(std::max<int64_t>(zj, 1)) * es);
+        d_Tj.reserve(size_t(std::max<int64_t>(tj, 1)) * es);
+        d_jres.reserve(sizeof(double) * 2 * size_t(nj));
+        fill(d_jrows.as<int32_t>(), d_cols.as<int32_t>(), d_jfill.as<FillTask>(), d_jfill_end.as<int64_t>(),
+             int32_t(nj), zj, d_J.p);
+        launch_interp(ctx.stream, prec, d_Z.p, d_jinterp.as<InterpTask>(), int32_t(nj), d_piv.as<int32_t>(), d_Tj.p,
+                      jmax_rank);
+        launch_judge(ctx.stream, prec, d_J.p, d_Tj.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
+                     d_jres.as<double>());
+        std::vector<double> res(2 * size_t(nj));
+        BFG_HIP(hipMemcpyAsync(res.data(), d_jres.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, ctx.stream));
+        BFG_HIP(hipStreamSynchronize(ctx.stream));
+        const double slack = o.judge_slack > 0.0 ? o.judge_slack : 4.0;
+        for (int64_t a = 0; a < nj; ++a) {
+            const int32_t q = jq[a];
+            Comp &c = cs[pend[q]];
+            ++n_judged;
+            if (res[2 * a] <= slack * c.eps * res[2 * a + 1]) {
+                c.judge = 1;
+                continue;
+            }
+            c.judge = 2;
+            ++n_fallback;
+            tj_of[q] = -1;
+            const int32_t plain = std::max(int32_t(std::ceil(o.c_tau * std::log(1.0 / c.eps))), 10);
+            c.rows = std::min(std::max(plain, 2 * c.rows), c.n_obs);
+            retry.push_back(pend[q]);
+        }
+        t_judge += secs(t0, clk::now());
+    }
+
     // Run every compression of one level in batches: fill -> pivoted QR ->
     // (adaptive doubling of the rows that saturated) -> interpolation
     // matrices written into `pool` (the level's factor buffer).
     void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used) {
         guesses(cs);
+        const bool judging = o.rank_mode == BFGPU_RANK_LEARNED && o.judge_rows > 0;
         std::vector<int32_t> pend;
         for (size_t i = 0; i < cs.size(); ++i) {
             Comp &c = cs[i];
@@ -395,7 +494,7 @@ struct Engine {
             BFG_HIP(hipStreamSynchronize(ctx.stream));
             auto ti = clk::now();
             t_qr += secs(tq, ti);
-            std::vector<int32_t> retry;
+            std::vector<int32_t> retry, acc;
             it.clear();
             int32_t max_rank = 0;
             for (size_t q = 0; q < np; ++q) {
@@ -411,8 +510,34 @@ struct Engine {
                 if (r == 0 && n_rank0++ < 4 && std::getenv("BFGPU_TIMING"))
                     std::fprintf(stderr, "[bfgpu] rank-0 compression: %d sampled rows of %d observers x %d columns\n",
                                  c.rows, c.n_obs, int(c.cols.size()));
+                acc.push_back(int32_t(q));
+            }
+            // residual judge of the compressions whose rows a prediction sized:
+            // accepted ones keep the interpolation matrix computed for the
+            // judge (copied into the pool), rejected ones are redone by plain
+            // adaptive RRQR in a later batch
+            std::vector<int64_t> tj_of(judging ? np : 0, -1);
+            const double tj0 = t_judge;
+            if (judging) judge(cs, pend, acc, qt, cofs, tj_of, retry);
+            t_qr += t_judge - tj0;  // reported with fill + QR
+            std::vector<RunCopy> cp;
+            std::vector<int64_t> cp_end;
+            int64_t cp_total = 0;
+            for (int32_t q : acc) {
+                Comp &c = cs[pend[q]];
+                if (c.judge == 2 && c.rows != ft[q].m) continue;  // rejected: redone with more rows
+                const int32_t r = c.rank;
                 c.tofs = pool_used;
                 pool_used += int64_t(r) * int64_t(c.cols.size());
+                if (judging && tj_of[q] >= 0) {
+                    const int64_t len = int64_t(r) * int64_t(c.cols.size());
+                    if (len > 0) {
+                        cp.push_back(RunCopy{tj_of[q], c.tofs});
+                        cp_total += len;
+                        cp_end.push_back(cp_total);
+                    }
+                    continue;
+                }
                 it.push_back(InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, r, qt[q].pofs});
                 max_rank = std::max(max_rank, r);
             }
@@ -423,14 +548,22 @@ struct Engine {
                 c.skel.resize(c.rank);
                 for (int32_t i = 0; i < c.rank; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
             });
-            if (!it.empty()) {
+            if (!it.empty() || !cp.empty()) {
                 ensure_pool(pool, pool_cap, pool_used + FACTOR_PAD);
-                upload(d_interp, it, ctx.stream);
-                launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), int32_t(it.size()),
-                              d_piv.as<int32_t>(), pool.p, max_rank);
+                if (!it.empty()) {
+                    upload(d_interp, it, ctx.stream);
+                    launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), int32_t(it.size()),
+                                  d_piv.as<int32_t>(), pool.p, max_rank);
+                }
+                if (!cp.empty()) {
+                    upload(d_cp, cp, ctx.stream);
+                    upload(d_cp_end, cp_end, ctx.stream);
+                    launch_copy_runs(ctx.stream, prec, d_Tj.p, pool.p, d_cp.as<RunCopy>(), d_cp_end.as<int64_t>(),
+                                     int32_t(cp.size()), cp_total);
+                }
                 BFG_HIP(hipStreamSynchronize(ctx.stream));
             }
-            t_interp += secs(ti, clk::now());
+            t_interp += secs(ti, clk::now()) - (t_judge - tj0);
             retry.insert(retry.end(), later.begin(), later.end());
             pend.swap(retry);
         }
@@ -526,7 +659,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         ob.skel_idx.insert(ob.skel_idx.end(), v.begin(), v.end());
     };
     auto log_comp = [&](OneBF &ob, const Comp &c) {
-        CompRec r{c.stage, c.S, c.O, c.guess, c.rows, c.rank, {}};
+        CompRec r{c.stage, c.S, c.O, c.guess, c.rows, c.rank, {}, c.judge};
         for (int q = 0; q < 6; ++q) r.feat[q] = c.feat[q];
         ob.comp.push_back(r);
     };
@@ -759,6 +892,8 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     }
     bf.n_comp = E.n_comp;
     bf.n_retry = E.n_retry;
+    bf.n_judged = E.n_judged;
+    bf.n_fallback = E.n_fallback;
     bf.precision = o.precision;
     for (int64_t b = 0; b < nB; ++b) bf.bfs[b].stage_data_base.assign(bf.bfs[b].st.size(), 0);
     auto tfree = clk::now();
@@ -778,10 +913,11 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         std::fprintf(stderr,
                      "[bfgpu] factorize %.3fs: setup %.3fs, plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "
                      "interp %.3fs, emit %.3fs, free %.3fs, pack %.3fs (%lld compressions, %lld retries, "
-                     "%lld of rank 0, %d threads)\n",
+                     "%lld of rank 0, judge %.3fs: %lld judged, %lld fallbacks, %d threads)\n",
                      bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_host, E.t_qr, E.t_interp, t_emit, t_free,
                      secs(tpk, t1),
-                     (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, host_threads());
+                     (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, E.t_judge,
+                     (long long)E.n_judged, (long long)E.n_fallback, host_threads());
 }
 
 }  // namespace bfg

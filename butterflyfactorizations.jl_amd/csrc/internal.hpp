@@ -97,7 +97,9 @@ struct Block {
 };
 
 struct SkelRec { int32_t stage, s, o, ofs, len; };
-struct CompRec { int32_t stage, s, o, guess, rows, rank; float feat[6]; };
+// judge: 0 not judged, 1 accepted by the residual judge, 2 rejected and
+// redone by plain adaptive RRQR
+struct CompRec { int32_t stage, s, o, guess, rows, rank; float feat[6]; int32_t judge; };
 
 // One stage of one factorization in reference layout.
 struct StageBlocks { std::vector<Block> b; };
@@ -281,6 +283,7 @@ struct BF {
     DevBuf xy_host_stage;         // staging for host x / y
     mutable std::vector<std::vector<double>> export_cache;  // host copy of factors (export)
     int64_t n_comp = 0, n_retry = 0;
+    int64_t n_judged = 0, n_fallback = 0;  // learned ranks: judged compressions, plain fallbacks
     int64_t n_blocks = 0, n_data = 0;  // all blocks, factor entries (set by pack_tables)
     double t_factor_ms = 0.0;
     double t_plan_ms = 0.0, t_sample_ms = 0.0, t_qr_ms = 0.0, t_interp_ms = 0.0, t_pack_ms = 0.0;
@@ -374,6 +377,24 @@ struct InterpTask {
 };
 void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tasks,
                    int32_t n_tasks, const int32_t *piv, void *T, int32_t max_rank);
+
+// Residual judge of one compression: the interpolative decomposition T
+// (r x n at tofs, skeleton columns piv[pofs .. pofs + r)) checked on mc
+// held-out rows Zc (mc x n, column-major, at zofs).
+struct JudgeTask {
+    int64_t zofs, tofs;
+    int32_t mc, n, r, pofs;
+};
+// out[t] = (e_max, z_max): largest column norm of Zc - Zc[:, skel] T and of Zc.
+void launch_judge(hipStream_t s, int prec, const void *Zc, const void *T, const JudgeTask *tasks,
+                  int32_t n_tasks, const int32_t *piv, double *out);
+// Copy runs of complex entries: run i copies ends[i] - ends[i-1] entries
+// from src + src_ofs[i] to dst + dst_ofs[i].
+struct RunCopy {
+    int64_t src_ofs, dst_ofs;
+};
+void launch_copy_runs(hipStream_t s, int prec, const void *src, void *dst, const RunCopy *tasks,
+                      const int64_t *ends, int32_t n_tasks, int64_t total);
 
 void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const void *factors,
                   const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,

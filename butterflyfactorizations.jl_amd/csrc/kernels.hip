@@ -531,6 +531,103 @@ void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tas
 }
 
 // ---------------------------------------------------------------------------
+// Residual judge of compressions sized by a rank prediction.  Per task,
+// with Zc the mc held-out rows (column-major, ld mc) and T the
+// interpolation matrix (r x n, column-major; T[:, piv[q]] = e_q):
+//   e_max = max_j |Zc[:, j] - sum_q Zc[:, piv[q]] T[q, j]|_2,
+//   z_max = max_j |Zc[:, j]|_2,
+// accumulated in FP64.  One wavefront per task, a lane per column (strided
+// by 64), the maxima combined by a wave reduction; four tasks per
+// workgroup, whole waves leave together.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Zc, const C<R> *__restrict__ T,
+                                                    const JudgeTask *__restrict__ tasks, int32_t n_tasks,
+                                                    const int32_t *__restrict__ piv_all, double *__restrict__ out) {
+    const int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (t >= n_tasks) return;
+    const int lane = threadIdx.x & 63;
+    const JudgeTask tk = tasks[t];
+    const C<R> *Z = Zc + tk.zofs;
+    const C<R> *Tm = T + tk.tofs;
+    const int32_t *piv = piv_all + tk.pofs;
+    const int mc = tk.mc, n = tk.n, r = tk.r;
+    double emax = 0.0, zmax = 0.0;
+    for (int j = lane; j < n; j += 64) {
+        double en = 0.0, zn = 0.0;
+        for (int i = 0; i < mc; ++i) {
+            const C<R> zv = Z[i + int64_t(j) * mc];
+            C<double> e = {double(zv.x), double(zv.y)};
+            for (int q = 0; q < r; ++q) {
+                const C<R> a = Z[i + int64_t(piv[q]) * mc];
+                const C<R> b = Tm[q + int64_t(j) * r];
+                e = csub(e, cmul(C<double>{double(a.x), double(a.y)}, C<double>{double(b.x), double(b.y)}));
+            }
+            en += e.x * e.x + e.y * e.y;
+            zn += double(zv.x) * double(zv.x) + double(zv.y) * double(zv.y);
+        }
+        emax = fmax(emax, sqrt(en));
+        zmax = fmax(zmax, sqrt(zn));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        emax = fmax(emax, __shfl_xor(emax, o, 64));
+        zmax = fmax(zmax, __shfl_xor(zmax, o, 64));
+    }
+    if (lane == 0) {
+        out[2 * t] = emax;
+        out[2 * t + 1] = zmax;
+    }
+}
+
+void launch_judge(hipStream_t s, int prec, const void *Zc, const void *T, const JudgeTask *tasks,
+                  int32_t n_tasks, const int32_t *piv, double *out) {
+    if (n_tasks <= 0) return;
+    const int64_t nb = (int64_t(n_tasks) + 3) / 4;
+    BFG_REQUIRE(nb < (int64_t(1) << 31), "judge: too many tasks");
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(judge_kernel<double>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<double> *)Zc,
+                           (const C<double> *)T, tasks, n_tasks, piv, out);
+    else
+        hipLaunchKernelGGL(judge_kernel<float>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<float> *)Zc,
+                           (const C<float> *)T, tasks, n_tasks, piv, out);
+    BFG_HIP(hipGetLastError());
+}
+
+// Runs of complex entries copied between buffers: one thread per entry
+// (grid-stride), its run found by binary search over the running ends.
+template <typename R>
+__global__ __launch_bounds__(256) void copy_runs_kernel(const C<R> *__restrict__ src, C<R> *__restrict__ dst,
+                                                        const RunCopy *__restrict__ tasks,
+                                                        const int64_t *__restrict__ ends, int32_t n_tasks,
+                                                        int64_t total) {
+    int32_t lo = 0;
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
+        int32_t hi = n_tasks - 1;  // entries only grow: search from the last run
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (ends[mid] > e) hi = mid;
+            else lo = mid + 1;
+        }
+        const int64_t local = e - (lo > 0 ? ends[lo - 1] : 0);
+        dst[tasks[lo].dst_ofs + local] = src[tasks[lo].src_ofs + local];
+    }
+}
+
+void launch_copy_runs(hipStream_t s, int prec, const void *src, void *dst, const RunCopy *tasks,
+                      const int64_t *ends, int32_t n_tasks, int64_t total) {
+    if (total <= 0 || n_tasks <= 0) return;
+    dim3 grid(unsigned(std::min<int64_t>((total + 255) / 256, int64_t(1) << 22)));
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(copy_runs_kernel<double>, grid, dim3(256), 0, s, (const C<double> *)src,
+                           (C<double> *)dst, tasks, ends, n_tasks, total);
+    else
+        hipLaunchKernelGGL(copy_runs_kernel<float>, grid, dim3(256), 0, s, (const C<float> *)src,
+                           (C<float> *)dst, tasks, ends, n_tasks, total);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Forward matvec stage (y = A x), one workgroup per (task, group of NR
 // right-hand sides).  A task is an output vector (a slot, or rows of y
 // reached through a permutation) with its first contribution inline and

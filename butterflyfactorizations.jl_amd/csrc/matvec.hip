@@ -1090,16 +1090,24 @@ static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
                                 bits(beta[0]), bits(beta[1]), int64_t(reinterpret_cast<intptr_t>(s))};
     if (!bf.graph_exec || key != bf.graph_key) {
         release_graph(bf);
-        BFG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        // capture on the context's own stream (the caller's stream may be
+        // the null stream, which cannot be captured; capturing runs
+        // nothing), replay on the caller's stream
+        hipStream_t cap = bf.ctx->own;
+        BFG_REQUIRE(cap != nullptr, "matvec: graph capture needs the context's own stream");
+        BFG_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
         hipGraph_t g = nullptr;
+        bf.ctx->stream = cap;
         try {
             stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, nullptr, lanes, bf.slots.p);
         } catch (...) {
-            (void)hipStreamEndCapture(s, &g);
+            bf.ctx->stream = s;
+            (void)hipStreamEndCapture(cap, &g);
             if (g) (void)hipGraphDestroy(g);
             throw;
         }
-        BFG_HIP(hipStreamEndCapture(s, &g));
+        bf.ctx->stream = s;
+        BFG_HIP(hipStreamEndCapture(cap, &g));
         hipError_t e = hipGraphInstantiate(&bf.graph_exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         BFG_HIP(e);

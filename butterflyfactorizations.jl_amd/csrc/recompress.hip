@@ -460,6 +460,8 @@ void recompress(const BF &src, double tau, BF &out) {
     out.n_stage_launch = nL;
     out.n_comp = src.n_comp;
     out.n_retry = src.n_retry;
+    out.n_judged = src.n_judged;
+    out.n_fallback = src.n_fallback;
     out.bfs = src.bfs;  // Q and P unchanged: P still holds kernel entries
     out.pf_k = src.pf_k;
     out.use_pfree = src.use_pfree;
@@ -534,6 +536,8 @@ void materialize_transpose(const BF &src, bool conj, BF &out) {
     out.n_stage_launch = nL;
     out.n_comp = src.n_comp;
     out.n_retry = src.n_retry;
+    out.n_judged = src.n_judged;
+    out.n_fallback = src.n_fallback;
     out.bfs.resize(src.bfs.size());
     out.factor_elems.assign(size_t(nL), 0);
     struct Put {

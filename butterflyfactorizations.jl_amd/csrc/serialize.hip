@@ -9,7 +9,7 @@
 // loaded operator applies exactly as the one that was saved.
 //
 // Layout (little endian):
-//   "BFGPUSV1" | u32 version = 1 | i32 precision | i32 launch levels
+//   "BFGPUSV1" | u32 version = 2 | i32 precision | i32 launch levels
 //   | u64 observer-tree fingerprint | u64 source-tree fingerprint
 //   | i64 compressions | i64 adaptive doublings | i64 factorizations
 //   | per factorization: i32 no, ns, stages | per stage: i64 blocks, Block[]
@@ -29,7 +29,7 @@ void pack_tables(BF &bf);
 
 namespace {
 constexpr char MAGIC[8] = {'B', 'F', 'G', 'P', 'U', 'S', 'V', '1'};
-constexpr uint32_t VERSION = 1;
+constexpr uint32_t VERSION = 2;  // 2: compression records carry the judge verdict
 
 // FNV-1a over the arrays that fix a tree's node numbering and values.
 uint64_t fingerprint(const Tree &t) {
@@ -182,6 +182,10 @@ extern "C" int bfgpu_bf_load(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu
         f.get_vec(ob.skel, lim);
         f.get_vec(ob.skel_idx, lim);
         f.get_vec(ob.comp, lim);
+        for (const CompRec &c : ob.comp) {
+            bf.n_judged += c.judge != 0;
+            bf.n_fallback += c.judge == 2;
+        }
         ob.stage_data_base.assign(ob.st.size(), 0);
     }
     const size_t es = bf.precision == BFGPU_C128 ? 16 : 8;

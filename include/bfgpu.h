@@ -179,6 +179,18 @@ typedef struct {
     const bfgpu_kernel *kernel; /* kernel-matrix plug-in (NULL: the Helmholtz
                                    point kernel above); tree values index its
                                    test / trial functions */
+    /* Residual judge of BFGPU_RANK_LEARNED compressions (no counterpart in the
+     * reference, whose adaptive loop compressors.jl:124-151 is the fallback
+     * it triggers): after the pivoted QR accepts a compression whose row
+     * sample was sized by the prediction, its interpolative decomposition T
+     * is checked on the next judge_rows rows of the same sample prefix.  If
+     * max_j |Zc[:,j] - Zc[:,skel] T[:,j]| > judge_slack * eps * max_j
+     * |Zc[:,j]| (the pivoted QR's own stopping rule on rows it never saw),
+     * the compression is redone by plain adaptive RRQR from
+     * max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.  judge_rows 0: no
+     * judge; judge_slack <= 0: 4. */
+    int32_t judge_rows;
+    double judge_slack;
 } bfgpu_opts;
 
 /* assemble_BF (src/ButterflyFactorization/bfassembly.jl:48-200) for n_pairs
@@ -219,6 +231,8 @@ typedef struct {
     double t_plan_ms, t_sample_ms, t_qr_ms, t_interp_ms, t_pack_ms;
     int64_t device_bytes;  /* device memory the handle holds: factors, launch
                               tables, skeleton-vector and staging buffers */
+    int64_t n_judged;      /* learned ranks: compressions checked by the judge */
+    int64_t n_fallback;    /* of these, redone by plain adaptive RRQR */
 } bfgpu_bf_info;
 int bfgpu_bf_info_get(const bfgpu_bf *bf, bfgpu_bf_info *out);
 
@@ -258,6 +272,10 @@ int bfgpu_bf_export(const bfgpu_bf *bf, int32_t index, int32_t *table, int64_t *
  * compression records (stage, snode, onode, guess, rows, rank). */
 int bfgpu_bf_export_logs(const bfgpu_bf *bf, int32_t index, int32_t *skel_rec,
                          int32_t *skel_idx, int32_t *comp_rec);
+/* Judge verdict of each compression record (same order as comp_rec): 0 not
+ * judged, 1 accepted by the residual judge, 2 rejected and redone by plain
+ * adaptive RRQR (bfgpu_opts.judge_rows). */
+int bfgpu_bf_export_judge(const bfgpu_bf *bf, int32_t index, int32_t *judge);
 /* Features of each compression record (6 floats, same order as comp_rec):
  * source and observer radii in wavelengths, centre distance in
  * wavelengths, |cos| of the separation angle to z, level, log10(1/eps).

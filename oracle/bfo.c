@@ -362,6 +362,52 @@ int32_t bfo_compress(double k, const double *obs_xyz, const int32_t *obs_idx, in
     return r;
 }
 
+/* Residual judge of a compression whose row sample was sized by a rank
+ * prediction (no counterpart in the reference; its adaptive loop,
+ * compressors.jl:124-151, is the fallback it triggers): the interpolative
+ * decomposition T (r x n_src, skeleton columns skel) found on the first
+ * `rows` sampled rows is checked on the next m = min(judge_rows, n_obs -
+ * rows) rows of the same sample prefix, rows the QR never saw.  With
+ * Z_c = K(check rows, src) and E = Z_c - K(check rows, skel) T:
+ *   e_max = max_j |E[:, j]|_2,   z_max = max_j |Z_c[:, j]|_2,
+ * the held-out form of the pivoted QR's own stopping rule (every remaining
+ * column norm within rtol of the largest column norm). */
+void bfo_judge(double k, const double *obs_xyz, const int32_t *obs_idx, int32_t n_obs,
+               const double *src_xyz, const int32_t *src_idx, int32_t n_src, uint64_t key, int32_t rows,
+               int32_t judge_rows, const double *Td, const int32_t *skel, int32_t r, double *e_max,
+               double *z_max) {
+    int32_t m = n_obs - rows < judge_rows ? n_obs - rows : judge_rows;
+    *e_max = 0.0;
+    *z_max = 0.0;
+    if (m <= 0 || n_src <= 0) return;
+    const cplx *T = (const cplx *)Td;
+    int32_t *perm = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)(rows + m));
+    int32_t *cr = (int32_t *)xmalloc(sizeof(int32_t) * (size_t)m);
+    bfo_sample_prefix(key, n_obs, rows + m, perm);
+    for (int32_t i = 0; i < m; ++i) cr[i] = obs_idx[perm[rows + i]];
+    cplx *Z = (cplx *)xmalloc(sizeof(cplx) * (size_t)m * n_src);
+    cplx *Zs = (cplx *)xmalloc(sizeof(cplx) * (size_t)m * (r > 0 ? r : 1));
+    bfo_helmholtz_fill(k, obs_xyz, cr, m, src_xyz, src_idx, n_src, (double *)Z);
+    if (r > 0) bfo_helmholtz_fill(k, obs_xyz, cr, m, src_xyz, skel, r, (double *)Zs);
+    for (int32_t j = 0; j < n_src; ++j) {
+        double en = 0.0, zn = 0.0;
+        for (int32_t i = 0; i < m; ++i) {
+            cplx z = Z[i + (int64_t)j * m], e = z;
+            for (int32_t q = 0; q < r; ++q) e -= Zs[i + (int64_t)q * m] * T[q + (int64_t)j * r];
+            en += creal(e) * creal(e) + cimag(e) * cimag(e);
+            zn += creal(z) * creal(z) + cimag(z) * cimag(z);
+        }
+        en = sqrt(en);
+        zn = sqrt(zn);
+        if (en > *e_max) *e_max = en;
+        if (zn > *z_max) *z_max = zn;
+    }
+    free(perm);
+    free(cr);
+    free(Z);
+    free(Zs);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Growable arrays                                                           */
 /* ------------------------------------------------------------------------ */
@@ -539,6 +585,7 @@ struct bfo_bf {
     int64_t nsk, capsk;
     ivec skidx;
     ivec comp; /* 6 ints per compression */
+    ivec judge; /* per compression: 0 not judged, 1 accepted, 2 redone by plain adaptive RRQR */
 };
 
 static blk mkblk(int32_t oo, int32_t so, int32_t oi, int32_t si, int32_t m, int32_t n,
@@ -635,6 +682,23 @@ static cplx *do_compress(bfo_bf *bf, const bfo_opts *o, const double *obs_xyz,
     int32_t rows = 0;
     int32_t r = bfo_compress(o->k, obs_xyz, obs_idx, n_obs, src_xyz, src_idx, n_src, guess, eps,
                              o->adaptive, key, (double *)T, *skel, &rows);
+    int32_t verdict = 0;
+    if (o->judge_rows > 0 && n_src > 0 && rows < n_obs) {
+        double e_max = 0.0, z_max = 0.0;
+        bfo_judge(o->k, obs_xyz, obs_idx, n_obs, src_xyz, src_idx, n_src, key, rows, o->judge_rows,
+                  (const double *)T, *skel, r, &e_max, &z_max);
+        verdict = 1;
+        if (!(e_max <= o->judge_slack * eps * z_max)) {
+            /* plain adaptive RRQR from the butterfly estimator's rows (its
+             * floor of 10), and at least twice the rows the prediction used */
+            int32_t g = bfo_rank_butterfly(eps, o->c_tau, 10, NULL, NULL);
+            if (g < 2 * rows) g = 2 * rows;
+            r = bfo_compress(o->k, obs_xyz, obs_idx, n_obs, src_xyz, src_idx, n_src, g, eps, o->adaptive, key,
+                             (double *)T, *skel, &rows);
+            verdict = 2;
+        }
+    }
+    iv_push(&bf->judge, verdict);
     iv_push(&bf->comp, stage);
     iv_push(&bf->comp, S);
     iv_push(&bf->comp, O);
@@ -875,6 +939,7 @@ void bfo_bf_free(bfo_bf *bf) {
     free(bf->sk);
     iv_free(&bf->skidx);
     iv_free(&bf->comp);
+    iv_free(&bf->judge);
     free(bf);
 }
 
@@ -962,6 +1027,9 @@ void bfo_bf_export_skel(const bfo_bf *bf, int32_t *rec, int32_t *idx) {
     if (bf->skidx.n) memcpy(idx, bf->skidx.v, sizeof(int32_t) * bf->skidx.n);
 }
 int64_t bfo_bf_n_comp(const bfo_bf *bf) { return bf->comp.n / 6; }
+void bfo_bf_export_judge(const bfo_bf *bf, int32_t *v) {
+    if (bf->judge.n) memcpy(v, bf->judge.v, sizeof(int32_t) * bf->judge.n);
+}
 void bfo_bf_export_comp(const bfo_bf *bf, int32_t *rec) {
     if (bf->comp.n) memcpy(rec, bf->comp.v, sizeof(int32_t) * bf->comp.n);
 }

@@ -47,6 +47,8 @@ typedef struct {
     uint64_t seed;     /* row-sample seed */
     const int32_t *fixed_guess; /* rank_mode 2: per-compression guess, consumed in order */
     int32_t n_fixed;
+    int32_t judge_rows; /* > 0: residual judge on this many held-out rows (bfo_judge) */
+    double judge_slack; /* judge accepts when e_max <= judge_slack * eps * z_max */
 } bfo_opts;
 
 typedef struct bfo_bf bfo_bf;
@@ -79,6 +81,15 @@ int32_t bfo_compress(double k, const double *obs_xyz, const int32_t *obs_idx, in
                      double *T_out /* complex, cap n_src*n_src or larger */, int32_t *skel_out,
                      int32_t *rows_used);
 
+/* Residual of an interpolative decomposition on held-out rows of the same
+ * sample prefix (rows rows .. rows + min(judge_rows, n_obs - rows) - 1):
+ * e_max = max column norm of K(check, src) - K(check, skel) T, z_max = max
+ * column norm of K(check, src). */
+void bfo_judge(double k, const double *obs_xyz, const int32_t *obs_idx, int32_t n_obs,
+               const double *src_xyz, const int32_t *src_idx, int32_t n_src, uint64_t key, int32_t rows,
+               int32_t judge_rows, const double *T, const int32_t *skel, int32_t r, double *e_max,
+               double *z_max);
+
 /* ---- factorization ---- */
 bfo_bf *bfo_assemble(const bfo_tree *obsT, const double *obs_xyz, const bfo_tree *srcT,
                      const double *src_xyz, int32_t no, int32_t ns, const bfo_opts *opts);
@@ -100,6 +111,9 @@ void bfo_bf_export_skel(const bfo_bf *bf, int32_t *rec /* 5 per record */, int32
 /* Sample log: one record per compression (stage, snode, onode, guess, rows, rank). */
 int64_t bfo_bf_n_comp(const bfo_bf *bf);
 void bfo_bf_export_comp(const bfo_bf *bf, int32_t *rec /* 6 per record */);
+/* Per compression record: 0 not judged, 1 accepted by the residual judge,
+ * 2 rejected and redone by plain adaptive RRQR. */
+void bfo_bf_export_judge(const bfo_bf *bf, int32_t *v);
 void bfo_bf_dims(const bfo_bf *bf, int64_t *rows, int64_t *cols);
 
 /* Import a factorization from the same block table / data format. */

@@ -26,7 +26,7 @@ class Tree(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("k", f64), ("tol", f64), ("adaptive", i32), ("rank_mode", i32), ("c_tau", f64),
                 ("rmin", i32), ("c_geo", f64), ("c_eps", f64), ("seed", u64),
-                ("fixed_guess", P(i32)), ("n_fixed", i32)]
+                ("fixed_guess", P(i32)), ("n_fixed", i32), ("judge_rows", i32), ("judge_slack", f64)]
 
 
 _SIGS = {
@@ -52,6 +52,9 @@ _SIGS = {
     "bfo_bf_export_skel": (None, [vp, P(i32), P(i32)]),
     "bfo_bf_n_comp": (i64, [vp]),
     "bfo_bf_export_comp": (None, [vp, P(i32)]),
+    "bfo_bf_export_judge": (None, [vp, P(i32)]),
+    "bfo_judge": (None, [f64, P(f64), P(i32), i32, P(f64), P(i32), i32, u64, i32, i32, P(f64), P(i32), i32,
+                         P(f64), P(f64)]),
     "bfo_bf_dims": (None, [vp, P(i64), P(i64)]),
     "bfo_bf_import": (vp, [P(Tree), P(Tree), i32, i64, P(i32), P(i64), P(f64)]),
     "bfo_matvec": (None, [vp, i64, P(f64), i64, P(f64), i64, P(f64), P(f64)]),
@@ -202,6 +205,24 @@ def compress(k, obs_pts, obs_idx, src_pts, src_idx, guess, eps, adaptive, key):
     return T[: r * ns].reshape((r, ns), order="F"), skel[:r], r, rows.value
 
 
+def judge(k, obs_pts, obs_idx, src_pts, src_idx, key, rows, judge_rows, T, skel):
+    """(e_max, z_max) of the interpolative decomposition (T, skel) on the
+    held-out rows rows .. rows + judge_rows - 1 of the sample prefix."""
+    obs_idx = np.ascontiguousarray(obs_idx, np.int32)
+    src_idx = np.ascontiguousarray(src_idx, np.int32)
+    skel = np.ascontiguousarray(skel, np.int32)
+    r = len(skel)
+    T = np.asfortranarray(T, np.complex128).reshape((r, len(src_idx)), order="F")
+    Tf = np.zeros(max(T.size, 1), np.complex128)
+    Tf[: T.size] = T.ravel(order="F")
+    sk = skel if r else np.zeros(1, np.int32)
+    e, z = f64(), f64()
+    lib().bfo_judge(k, _p(np.ascontiguousarray(obs_pts, np.float64), f64), _p(obs_idx, i32), len(obs_idx),
+                    _p(np.ascontiguousarray(src_pts, np.float64), f64), _p(src_idx, i32), len(src_idx), key,
+                    int(rows), int(judge_rows), _dp(Tf), _p(sk, i32), r, C.byref(e), C.byref(z))
+    return e.value, z.value
+
+
 class OBF:
     """An oracle factorization (owns the C object)."""
 
@@ -234,8 +255,11 @@ class OBF:
         nc = lib().bfo_bf_n_comp(self.h)
         comp = np.zeros((nc, 6), np.int32)
         lib().bfo_bf_export_comp(self.h, _p(comp, i32))
+        judge = np.zeros(max(nc, 1), np.int32)
+        lib().bfo_bf_export_judge(self.h, _p(judge, i32))
         return {"n_stages": lib().bfo_bf_n_stages(self.h), "table": table, "data_ofs": ofs,
-                "data": data[:nd], "skel": skel, "skel_idx": skel_idx[:nski], "comp": comp}
+                "data": data[:nd], "skel": skel, "skel_idx": skel_idx[:nski], "comp": comp,
+                "judge": judge[:nc]}
 
     def matvec(self, x, alpha=1.0, beta=0.0, y=None):
         rows, cols = self.dims()
@@ -259,12 +283,15 @@ class OBF:
 
 
 def assemble(test, trial, no, ns, k, tol, adaptive=True, rank_mode=0, c_tau=4.0, rmin=10,
-             c_geo=1.5, c_eps=1.5, seed=0, fixed=None):
-    """Oracle assemble_BF over product trees (bfgpu.ClusterTree)."""
+             c_geo=1.5, c_eps=1.5, seed=0, fixed=None, judge_rows=0, judge_slack=1.0):
+    """Oracle assemble_BF over product trees (bfgpu.ClusterTree).  rank_mode
+    2 takes the per-compression guesses `fixed` in compression order;
+    judge_rows > 0 checks every compression on that many held-out rows and
+    redoes the ones that miss judge_slack * eps with plain adaptive RRQR."""
     ot, os_ = OTree(test), OTree(trial)
     fx = np.ascontiguousarray(fixed if fixed is not None else np.zeros(1), np.int32)
     o = Opts(k, tol, int(adaptive), rank_mode, c_tau, rmin, c_geo, c_eps, seed, _p(fx, i32),
-             len(fixed) if fixed is not None else 0)
+             len(fixed) if fixed is not None else 0, int(judge_rows), float(judge_slack))
     h = lib().bfo_assemble(C.byref(ot.s), _p(ot.points, f64), C.byref(os_.s), _p(os_.points, f64),
                            no, ns, C.byref(o))
     bf = OBF(h, ot, os_)

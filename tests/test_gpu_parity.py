@@ -86,6 +86,7 @@ def test_from_host_fp64_matvec_matches_oracle(bfgpu, oracle, ctx):
 
 def _compare_factorizations(gex, oex, T_tol=1e-9):
     np.testing.assert_array_equal(gex["comp"], oex["comp"])          # stage, S, O, guess, rows, rank
+    np.testing.assert_array_equal(gex["judge"], oex["judge"])        # residual-judge verdicts
     np.testing.assert_array_equal(gex["skel"], oex["skel"])
     np.testing.assert_array_equal(gex["skel_idx"], oex["skel_idx"])  # skeleton index sets
     np.testing.assert_array_equal(gex["table"], oex["table"])        # keys, sizes, slot ids

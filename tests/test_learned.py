@@ -1,7 +1,8 @@
 """Learned rank predictor end to end on the GPU: rank logs of plain
 adaptive RRQR on seeded ellipsoids -> MLP trained by the library's kernels
 -> far-field factorization with predicted ranks.  Bars (written below): the
-fallback (adaptive doubling) rate stays under 5 %, the row samples shrink by
+rates of adaptive doubling and of the residual judge's fallback to plain
+RRQR each stay under 5 %, the row samples shrink by
 at least 10 % against the butterfly estimator, and the learned far field
 stays within 10 tol of dense FP64 rows and of the plain factorization."""
 import sys
@@ -33,7 +34,8 @@ def test_learned_ranks_shrink_samples_without_losing_accuracy(bfgpu):
     say("plain", P)
     say("learned", Lr)
     say("learned vs plain", res["learned_vs_plain"])
-    assert Lr["fallback_rate"] <= 0.05
+    assert Lr["fallback_rate"] <= 0.05 and Lr["doubling_rate"] <= 0.05
+    assert Lr["judged"] > 0
     assert Lr["mean_rows"] < 0.9 * P["mean_rows"]
     assert P["err_vs_dense"] < 10 * tol and Lr["err_vs_dense"] < 10 * tol
     assert res["learned_vs_plain"] < 10 * tol
