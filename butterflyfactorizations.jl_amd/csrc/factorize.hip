@@ -187,9 +187,8 @@ struct Engine {
     size_t es;
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
-    // residual judge: held-out rows, their tiles, interpolation matrices of
-    // the judged compressions, judge tasks and results, copies into the pool
-    DevBuf d_jrows, d_jfill, d_jfill_end, d_jinterp, d_jtask, d_jres, d_J, d_Tj, d_cp, d_cp_end;
+    // residual judge: tasks and results
+    DevBuf d_jtask, d_jres;
     int64_t n_comp = 0, n_retry = 0, n_rank0 = 0, n_judged = 0, n_fallback = 0;
     double t_host = 0, t_qr = 0, t_interp = 0;  // seconds: sample prep, fill+QR, interpolation
     double t_judge = 0;                         // seconds: held-out rows, fill, judge (part of t_qr)
@@ -227,8 +226,7 @@ struct Engine {
 
     void release() {
         for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z,
-                          &d_jrows, &d_jfill, &d_jfill_end, &d_jinterp, &d_jtask, &d_jres, &d_J, &d_Tj, &d_cp,
-                          &d_cp_end})
+                          &d_jtask, &d_jres})
             *d = DevBuf();
     }
 
@@ -272,82 +270,54 @@ struct Engine {
         }
     }
 
-    // Residual judge (learned ranks) of the compressions `acc` of the
-    // current batch whose sample left observer rows unseen: the next
-    // judge_rows rows of the same sample prefix are filled, the
-    // interpolation matrices computed into d_Tj (tj_of[q]: their offset)
-    // and checked on those rows.  A compression passes when the largest
-    // residual column norm is at most judge_slack * eps times the largest
-    // column norm of the held-out tile; one that fails is redone by plain
-    // adaptive RRQR from max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.
-    // The fill, QR tiles, pivots and column lists of the batch must still
-    // be on the device.
+    // Held-out rows the residual judge checks compression c on: the next
+    // judge_rows rows of its sample prefix (fewer when the observers run
+    // out), for learned ranks whose sample left observers unseen and whose
+    // columns fit the judge kernel; 0 when c is not judged.
+    int32_t judge_m(const Comp &c) const {
+        if (o.rank_mode != BFGPU_RANK_LEARNED || o.judge_rows <= 0 || c.judge != 0 || c.rows >= c.n_obs ||
+            c.cols.empty() || int32_t(c.cols.size()) > JUDGE_MAX_COLS)
+            return 0;
+        return std::min(o.judge_rows, c.n_obs - c.rows);
+    }
+
+    // Residual judge of the compressions `acc` of the current batch that
+    // the pivoted QR accepted and that have held-out tiles (jq[a]: batch
+    // index, rows [jr[a], jr[a + 1]) of the held-out rows, tile at jz[a]):
+    // a compression passes when the largest residual column norm of its
+    // interpolative decomposition on the held-out tile is at most
+    // judge_slack * eps times the tile's largest column norm; one that fails
+    // is redone by plain adaptive RRQR from max(ceil(c_tau ln(1/eps)), 10,
+    // 2 * rows) rows.  The QR tiles, pivots and held-out tiles of the batch
+    // are still on the device.
     void judge(std::vector<Comp> &cs, const std::vector<int32_t> &pend, const std::vector<int32_t> &acc,
-               const std::vector<QrTask> &qt, const std::vector<int64_t> &cofs, std::vector<int64_t> &tj_of,
-               std::vector<int32_t> &retry) {
+               const std::vector<QrTask> &qt, const std::vector<int32_t> &jq, const std::vector<int64_t> &jr,
+               const std::vector<int64_t> &jz, const std::vector<int32_t> &jpos, std::vector<int32_t> &retry) {
         auto t0 = clk::now();
-        std::vector<int32_t> jq;
+        std::vector<JudgeTask> jt;
+        std::vector<int32_t> who;
+        jt.reserve(jq.size());
         for (int32_t q : acc) {
+            const int32_t a = jpos[q];
+            if (a < 0) continue;
             const Comp &c = cs[pend[q]];
-            if (c.judge == 0 && c.rows < c.n_obs && !c.cols.empty()) jq.push_back(q);
+            jt.push_back(JudgeTask{jz[a], qt[q].zofs, int32_t(jr[a + 1] - jr[a]), qt[q].m, qt[q].n, c.rank,
+                                   qt[q].pofs, 0});
+            who.push_back(q);
         }
-        if (jq.empty()) return;
-        const int64_t nj = int64_t(jq.size());
-        std::vector<int64_t> jr(size_t(nj) + 1, 0);
-        for (int64_t a = 0; a < nj; ++a) {
-            const Comp &c = cs[pend[jq[a]]];
-            jr[a + 1] = jr[a] + std::min(o.judge_rows, c.n_obs - c.rows);
-        }
-        BFG_REQUIRE(jr[nj] < (int64_t(1) << 31), "factorize: judge rows exceed 32-bit index lists");
-        std::vector<int32_t> hj(size_t(std::max<int64_t>(jr[nj], 1)));
-        parallel_for(nj, [&](int64_t a) {
-            const Comp &c = cs[pend[jq[a]]];
-            const int32_t mc = int32_t(jr[a + 1] - jr[a]);
-            thread_local std::vector<int32_t> pre;
-            pre.resize(size_t(c.rows) + size_t(mc));
-            sample_prefix(c.key, c.n_obs, c.rows + mc, pre.data());
-            for (int32_t i = 0; i < mc; ++i) hj[size_t(jr[a] + i)] = c.obs[pre[size_t(c.rows + i)]];
-        });
-        std::vector<FillTask> jf(static_cast<size_t>(nj));
-        std::vector<int64_t> je(static_cast<size_t>(nj));
-        std::vector<InterpTask> ji(static_cast<size_t>(nj));
-        std::vector<JudgeTask> jt(static_cast<size_t>(nj));
-        int64_t zj = 0, tj = 0;
-        int32_t jmax_rank = 0;
-        for (int64_t a = 0; a < nj; ++a) {
-            const int32_t q = jq[a];
-            const Comp &c = cs[pend[q]];
-            const int32_t mc = int32_t(jr[a + 1] - jr[a]), n = int32_t(c.cols.size());
-            jf[a] = FillTask{zj, mc, n, int32_t(jr[a]), int32_t(cofs[q])};
-            zj += int64_t(mc) * n;
-            je[a] = zj;
-            ji[a] = InterpTask{qt[q].zofs, tj, qt[q].m, qt[q].n, c.rank, qt[q].pofs};
-            jt[a] = JudgeTask{jf[a].zofs, tj, mc, n, c.rank, qt[q].pofs};
-            tj_of[q] = tj;
-            tj += int64_t(c.rank) * n;
-            jmax_rank = std::max(jmax_rank, c.rank);
-        }
-        upload(d_jrows, hj, ctx.stream);
-        upload(d_jfill, jf, ctx.stream);
-        upload(d_jfill_end, je, ctx.stream);
-        upload(d_jinterp, ji, ctx.stream);
+        (void)jq;
+        if (jt.empty()) return;
+        const int64_t nj = int64_t(jt.size());
         upload(d_jtask, jt, ctx.stream);
-        d_J.reserve(size_t(std::max<int64_t>(zj, 1)) * es);
-        d_Tj.reserve(size_t(std::max<int64_t>(tj, 1)) * es);
         d_jres.reserve(sizeof(double) * 2 * size_t(nj));
-        fill(d_jrows.as<int32_t>(), d_cols.as<int32_t>(), d_jfill.as<FillTask>(), d_jfill_end.as<int64_t>(),
-             int32_t(nj), zj, d_J.p);
-        launch_interp(ctx.stream, prec, d_Z.p, d_jinterp.as<InterpTask>(), int32_t(nj), d_piv.as<int32_t>(), d_Tj.p,
-                      jmax_rank);
-        launch_judge(ctx.stream, prec, d_J.p, d_Tj.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
+        launch_judge(ctx.stream, prec, d_Z.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
                      d_jres.as<double>());
         std::vector<double> res(2 * size_t(nj));
         BFG_HIP(hipMemcpyAsync(res.data(), d_jres.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, ctx.stream));
         BFG_HIP(hipStreamSynchronize(ctx.stream));
         const double slack = o.judge_slack > 0.0 ? o.judge_slack : 4.0;
         for (int64_t a = 0; a < nj; ++a) {
-            const int32_t q = jq[a];
-            Comp &c = cs[pend[q]];
+            Comp &c = cs[pend[who[a]]];
             ++n_judged;
             if (res[2 * a] <= slack * c.eps * res[2 * a + 1]) {
                 c.judge = 1;
@@ -355,10 +325,9 @@ struct Engine {
             }
             c.judge = 2;
             ++n_fallback;
-            tj_of[q] = -1;
             const int32_t plain = std::max(int32_t(std::ceil(o.c_tau * std::log(1.0 / c.eps))), 10);
             c.rows = std::min(std::max(plain, 2 * c.rows), c.n_obs);
-            retry.push_back(pend[q]);
+            retry.push_back(pend[who[a]]);
         }
         t_judge += secs(t0, clk::now());
     }
@@ -368,7 +337,6 @@ struct Engine {
     // matrices written into `pool` (the level's factor buffer).
     void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used) {
         guesses(cs);
-        const bool judging = o.rank_mode == BFGPU_RANK_LEARNED && o.judge_rows > 0;
         std::vector<int32_t> pend;
         for (size_t i = 0; i < cs.size(); ++i) {
             Comp &c = cs[i];
@@ -382,9 +350,9 @@ struct Engine {
             c.rows = std::min(std::max(c.guess, 10), c.n_obs);
             pend.push_back(int32_t(i));
         }
-        std::vector<int32_t> hrows, hcols, hrank, hpiv;
+        std::vector<int32_t> hrows, hcols, hrank, hpiv, jq, jpos;
         std::vector<FillTask> ft;
-        std::vector<int64_t> fe, rofs, cofs;
+        std::vector<int64_t> fe, rofs, cofs, jr, jz;
         std::vector<QrTask> qt;
         std::vector<InterpTask> it;
         // kernel-matrix tiles of one batch: at most 32 GiB or 30 % of the
@@ -404,7 +372,7 @@ struct Engine {
                 int64_t zb = 0;
                 for (; nb < pend.size(); ++nb) {
                     const Comp &c = cs[pend[nb]];
-                    const int64_t z = int64_t(c.rows) * int64_t(c.cols.size()) * es;
+                    const int64_t z = int64_t(c.rows + judge_m(c)) * int64_t(c.cols.size()) * es;
                     if (nb > 0 && zb + z > z_budget) break;
                     zb += z;
                 }
@@ -431,16 +399,52 @@ struct Engine {
                 max_n = std::max(max_n, f.n);
                 max_m = std::max(max_m, f.m);
             }
-            BFG_REQUIRE(rofs[np] < (int64_t(1) << 31) && cofs[np] < (int64_t(1) << 31),
+            // held-out rows of the residual judge (learned ranks): the next
+            // rows of the same sample prefix, filled with the main tiles
+            // (their tiles after the main ones in the workspace)
+            jq.clear();
+            jr.assign(1, 0);
+            jpos.assign(np, -1);
+            for (size_t q = 0; q < np; ++q) {
+                const int32_t mc = judge_m(cs[pend[q]]);
+                if (mc == 0) continue;
+                jpos[q] = int32_t(jq.size());
+                jq.push_back(int32_t(q));
+                jr.push_back(jr.back() + mc);
+            }
+            const size_t nj = jq.size();
+            ft.resize(np + nj);
+            fe.resize(np + nj);
+            jz.resize(nj);
+            for (size_t a = 0; a < nj; ++a) {
+                const int32_t q = jq[a];
+                const int32_t mc = int32_t(jr[a + 1] - jr[a]), n = int32_t(cs[pend[q]].cols.size());
+                ft[np + a] = FillTask{zofs, mc, n, int32_t(rofs[np] + jr[a]), int32_t(cofs[q])};
+                jz[a] = zofs;
+                zofs += int64_t(mc) * n;
+                fe[np + a] = zofs;
+            }
+            BFG_REQUIRE(rofs[np] + jr[nj] < (int64_t(1) << 31) && cofs[np] < (int64_t(1) << 31),
                         "factorize: level too large for 32-bit index lists");
             const int32_t pofs = int32_t(cofs[np]);
-            hrows.resize(size_t(rofs[np]));
+            hrows.resize(size_t(rofs[np] + jr[nj]));
             hcols.resize(size_t(cofs[np]));
             parallel_for(int64_t(np), [&](int64_t q) {
                 const Comp &c = cs[pend[q]];
                 int32_t *r = hrows.data() + rofs[q];
-                sample_prefix(c.key, c.n_obs, c.rows, r);
-                for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[r[i]];
+                if (jpos[q] < 0) {
+                    sample_prefix(c.key, c.n_obs, c.rows, r);
+                    for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[r[i]];
+                } else {
+                    const int64_t a = jpos[q];
+                    const int32_t mc = int32_t(jr[a + 1] - jr[a]);
+                    thread_local std::vector<int32_t> pre;
+                    pre.resize(size_t(c.rows) + size_t(mc));
+                    sample_prefix(c.key, c.n_obs, c.rows + mc, pre.data());
+                    for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[pre[size_t(i)]];
+                    int32_t *h = hrows.data() + rofs[np] + jr[a];
+                    for (int32_t i = 0; i < mc; ++i) h[i] = c.obs[pre[size_t(c.rows + i)]];
+                }
                 std::copy(c.cols.begin(), c.cols.end(), hcols.begin() + cofs[q]);
             });
             auto tq = clk::now();
@@ -512,32 +516,18 @@ struct Engine {
                                  c.rows, c.n_obs, int(c.cols.size()));
                 acc.push_back(int32_t(q));
             }
-            // residual judge of the compressions whose rows a prediction sized:
-            // accepted ones keep the interpolation matrix computed for the
-            // judge (copied into the pool), rejected ones are redone by plain
-            // adaptive RRQR in a later batch
-            std::vector<int64_t> tj_of(judging ? np : 0, -1);
+            // residual judge of the compressions whose rows a prediction
+            // sized: rejected ones are redone by plain adaptive RRQR in a
+            // later batch
             const double tj0 = t_judge;
-            if (judging) judge(cs, pend, acc, qt, cofs, tj_of, retry);
+            if (!jq.empty()) judge(cs, pend, acc, qt, jq, jr, jz, jpos, retry);
             t_qr += t_judge - tj0;  // reported with fill + QR
-            std::vector<RunCopy> cp;
-            std::vector<int64_t> cp_end;
-            int64_t cp_total = 0;
             for (int32_t q : acc) {
                 Comp &c = cs[pend[q]];
                 if (c.judge == 2 && c.rows != ft[q].m) continue;  // rejected: redone with more rows
                 const int32_t r = c.rank;
                 c.tofs = pool_used;
                 pool_used += int64_t(r) * int64_t(c.cols.size());
-                if (judging && tj_of[q] >= 0) {
-                    const int64_t len = int64_t(r) * int64_t(c.cols.size());
-                    if (len > 0) {
-                        cp.push_back(RunCopy{tj_of[q], c.tofs});
-                        cp_total += len;
-                        cp_end.push_back(cp_total);
-                    }
-                    continue;
-                }
                 it.push_back(InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, r, qt[q].pofs});
                 max_rank = std::max(max_rank, r);
             }
@@ -548,19 +538,21 @@ struct Engine {
                 c.skel.resize(c.rank);
                 for (int32_t i = 0; i < c.rank; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
             });
-            if (!it.empty() || !cp.empty()) {
+            if (!it.empty()) {
                 ensure_pool(pool, pool_cap, pool_used + FACTOR_PAD);
-                if (!it.empty()) {
-                    upload(d_interp, it, ctx.stream);
-                    launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), int32_t(it.size()),
-                                  d_piv.as<int32_t>(), pool.p, max_rank);
-                }
-                if (!cp.empty()) {
-                    upload(d_cp, cp, ctx.stream);
-                    upload(d_cp_end, cp_end, ctx.stream);
-                    launch_copy_runs(ctx.stream, prec, d_Tj.p, pool.p, d_cp.as<RunCopy>(), d_cp_end.as<int64_t>(),
-                                     int32_t(cp.size()), cp_total);
-                }
+                // low ranks (almost all) on the lane-per-column kernel, the
+                // rest on the wave-per-column one; tasks of each kind contiguous
+                const auto small_end = std::stable_partition(
+                    it.begin(), it.end(), [](const InterpTask &x) { return x.rank <= INTERP_SMALL_RANK; });
+                const int32_t n_small = int32_t(small_end - it.begin());
+                int32_t big_rank = 0;
+                for (auto p = small_end; p != it.end(); ++p) big_rank = std::max(big_rank, p->rank);
+                (void)max_rank;
+                upload(d_interp, it, ctx.stream);
+                launch_interp_small(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), n_small, d_piv.as<int32_t>(),
+                                    pool.p);
+                launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>() + n_small,
+                              int32_t(it.size()) - n_small, d_piv.as<int32_t>(), pool.p, big_rank);
                 BFG_HIP(hipStreamSynchronize(ctx.stream));
             }
             t_interp += secs(ti, clk::now()) - (t_judge - tj0);

@@ -377,24 +377,28 @@ struct InterpTask {
 };
 void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tasks,
                    int32_t n_tasks, const int32_t *piv, void *T, int32_t max_rank);
+// Compressions of rank at most INTERP_SMALL_RANK: a lane per column, the
+// same back substitution in registers (kernels.hip).
+constexpr int32_t INTERP_SMALL_RANK = 16;
+void launch_interp_small(hipStream_t s, int prec, const void *Z, const InterpTask *tasks, int32_t n_tasks,
+                         const int32_t *piv, void *T);
 
-// Residual judge of one compression: the interpolative decomposition T
-// (r x n at tofs, skeleton columns piv[pofs .. pofs + r)) checked on mc
-// held-out rows Zc (mc x n, column-major, at zofs).
+// Residual judge of one compression (kernels.hip): R of its pivoted QR in
+// the tile workspace at rofs (leading dimension ldr, columns in pivot
+// order, r steps taken), its mc held-out rows at cofs (mc x n, column-major,
+// columns in the original order), pivots at pofs.
 struct JudgeTask {
-    int64_t zofs, tofs;
-    int32_t mc, n, r, pofs;
+    int64_t cofs, rofs;
+    int32_t mc, ldr, n, r, pofs, pad;
 };
-// out[t] = (e_max, z_max): largest column norm of Zc - Zc[:, skel] T and of Zc.
-void launch_judge(hipStream_t s, int prec, const void *Zc, const void *T, const JudgeTask *tasks,
-                  int32_t n_tasks, const int32_t *piv, double *out);
-// Copy runs of complex entries: run i copies ends[i] - ends[i-1] entries
-// from src + src_ofs[i] to dst + dst_ofs[i].
-struct RunCopy {
-    int64_t src_ofs, dst_ofs;
-};
-void launch_copy_runs(hipStream_t s, int prec, const void *src, void *dst, const RunCopy *tasks,
-                      const int64_t *ends, int32_t n_tasks, int64_t total);
+// Most columns of a judged compression (the judge kernel keeps four
+// 64-lane chunks of a held-out row in registers).
+constexpr int32_t JUDGE_MAX_COLS = 256;
+// out[2 t], out[2 t + 1] = (e_max, z_max): the largest column norm of the
+// interpolative decomposition's residual on the held-out rows, and of the
+// held-out tile itself.
+void launch_judge(hipStream_t s, int prec, const void *Z, const JudgeTask *tasks, int32_t n_tasks,
+                  const int32_t *piv, double *out);
 
 void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const void *factors,
                   const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,

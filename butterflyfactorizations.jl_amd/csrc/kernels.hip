@@ -514,6 +514,69 @@ __global__ __launch_bounds__(256) void interp_kernel(const C<R> *Z,
     }
 }
 
+// Interpolation matrices of compressions of rank at most RB: one wavefront
+// per compression, a lane per column, the column's right-hand side
+// R[0:r, j] held in registers and solved by the same column-oriented FP64
+// back substitution (x_i = b_i / R_ii, then b_q -= R_qi x_i for q < i, i
+// from r - 1 down) as interp_kernel, so the results are the same; the
+// entries of R11 are read by every lane at once (one address per load).
+template <typename R, int RB>
+__global__ __launch_bounds__(256) void interp_small_kernel(const C<R> *Z, const InterpTask *__restrict__ tasks,
+                                                           int32_t n_tasks, const int32_t *__restrict__ piv_all,
+                                                           C<R> *T) {
+    const int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (t >= n_tasks) return;
+    const int lane = threadIdx.x & 63;
+    const InterpTask tk = tasks[t];
+    const int m = tk.m, n = tk.n, r = tk.rank;
+    const C<R> *A = Z + tk.zofs;
+    const int32_t *piv = piv_all + tk.pofs;
+    C<R> *out = T + tk.tofs;
+    for (int j = lane; j < n; j += 64) {
+        C<R> *tc = out + int64_t(piv[j]) * r;
+        if (j < r) {
+            for (int i = 0; i < r; ++i) tc[i] = (i == j) ? C<R>{R(1), R(0)} : C<R>{R(0), R(0)};
+            continue;
+        }
+        C<double> b[RB];
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const C<R> v = i < r ? A[i + int64_t(j) * m] : C<R>{R(0), R(0)};
+            b[i] = C<double>{double(v.x), double(v.y)};
+        }
+#pragma unroll
+        for (int i = RB - 1; i >= 0; --i) {
+            if (i >= r) continue;
+            const C<R> d = A[i + int64_t(i) * m];
+            const double dr = d.x, di = d.y, den = dr * dr + di * di;
+            const C<double> xi = {(b[i].x * dr + b[i].y * di) / den, (b[i].y * dr - b[i].x * di) / den};
+            b[i] = xi;
+#pragma unroll
+            for (int q = 0; q < i; ++q) {
+                const C<R> rv = A[q + int64_t(i) * m];
+                b[q] = csub(b[q], cmul(C<double>{double(rv.x), double(rv.y)}, xi));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < RB; ++i)
+            if (i < r) tc[i] = C<R>{R(b[i].x), R(b[i].y)};
+    }
+}
+
+void launch_interp_small(hipStream_t s, int prec, const void *Z, const InterpTask *tasks, int32_t n_tasks,
+                         const int32_t *piv, void *T) {
+    if (n_tasks <= 0) return;
+    const int64_t nb = (int64_t(n_tasks) + 3) / 4;
+    BFG_REQUIRE(nb < (int64_t(1) << 31), "interp: too many tasks");
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL((interp_small_kernel<double, INTERP_SMALL_RANK>), dim3(unsigned(nb)), dim3(256), 0, s,
+                           (const C<double> *)Z, tasks, n_tasks, piv, (C<double> *)T);
+    else
+        hipLaunchKernelGGL((interp_small_kernel<float, INTERP_SMALL_RANK>), dim3(unsigned(nb)), dim3(256), 0, s,
+                           (const C<float> *)Z, tasks, n_tasks, piv, (C<float> *)T);
+    BFG_HIP(hipGetLastError());
+}
+
 void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tasks,
                    int32_t n_tasks, const int32_t *piv, void *T, int32_t max_rank) {
     if (n_tasks <= 0) return;
@@ -531,43 +594,77 @@ void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tas
 }
 
 // ---------------------------------------------------------------------------
-// Residual judge of compressions sized by a rank prediction.  Per task,
-// with Zc the mc held-out rows (column-major, ld mc) and T the
-// interpolation matrix (r x n, column-major; T[:, piv[q]] = e_q):
-//   e_max = max_j |Zc[:, j] - sum_q Zc[:, piv[q]] T[q, j]|_2,
-//   z_max = max_j |Zc[:, j]|_2,
-// accumulated in FP64.  One wavefront per task, a lane per column (strided
-// by 64), the maxima combined by a wave reduction; four tasks per
-// workgroup, whole waves leave together.
+// Residual judge of compressions sized by a rank prediction.  For each
+// held-out row z (1 x n) the row elimination
+//     s_k = z[piv[k]],  then for q < r:  w_q = s_q / R[q, q],
+//     s_k -= w_q R[q, k]  (k > q)
+// leaves s_k = z[piv[k]] - z[P1] R11^{-1} R[0:r, k], the residual of the
+// interpolative decomposition at column piv[k] (k >= r), with no
+// interpolation matrix formed.  Per task
+//     e_max = max_{k >= r} |E[:, k]|_2,   z_max = max_k |Zc[:, k]|_2,
+// in FP64.  One wavefront per task, lane per column (four chunks of 64,
+// n <= JUDGE_MAX_COLS), w_q broadcast from the lane that owns column q;
+// four tasks per workgroup, whole waves leave together.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Zc, const C<R> *__restrict__ T,
-                                                    const JudgeTask *__restrict__ tasks, int32_t n_tasks,
-                                                    const int32_t *__restrict__ piv_all, double *__restrict__ out) {
+__global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Z, const JudgeTask *__restrict__ tasks,
+                                                    int32_t n_tasks, const int32_t *__restrict__ piv_all,
+                                                    double *__restrict__ out) {
     const int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (t >= n_tasks) return;
     const int lane = threadIdx.x & 63;
     const JudgeTask tk = tasks[t];
-    const C<R> *Z = Zc + tk.zofs;
-    const C<R> *Tm = T + tk.tofs;
+    const C<R> *Rm = Z + tk.rofs;
+    const C<R> *Zc = Z + tk.cofs;
     const int32_t *piv = piv_all + tk.pofs;
-    const int mc = tk.mc, n = tk.n, r = tk.r;
-    double emax = 0.0, zmax = 0.0;
-    for (int j = lane; j < n; j += 64) {
-        double en = 0.0, zn = 0.0;
-        for (int i = 0; i < mc; ++i) {
-            const C<R> zv = Z[i + int64_t(j) * mc];
-            C<double> e = {double(zv.x), double(zv.y)};
-            for (int q = 0; q < r; ++q) {
-                const C<R> a = Z[i + int64_t(piv[q]) * mc];
-                const C<R> b = Tm[q + int64_t(j) * r];
-                e = csub(e, cmul(C<double>{double(a.x), double(a.y)}, C<double>{double(b.x), double(b.y)}));
+    const int mc = tk.mc, ldr = tk.ldr, n = tk.n, r = tk.r;
+    constexpr int NC = JUDGE_MAX_COLS / 64;
+    double en[NC], zn[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) en[c] = zn[c] = 0.0;
+    for (int i = 0; i < mc; ++i) {
+        C<double> sv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int k = lane + 64 * c;
+            sv[c] = C<double>{0.0, 0.0};
+            if (k < n) {
+                const C<R> v = Zc[i + int64_t(piv[k]) * mc];
+                sv[c] = C<double>{double(v.x), double(v.y)};
+                zn[c] += sv[c].x * sv[c].x + sv[c].y * sv[c].y;
             }
-            en += e.x * e.x + e.y * e.y;
-            zn += double(zv.x) * double(zv.x) + double(zv.y) * double(zv.y);
         }
-        emax = fmax(emax, sqrt(en));
-        zmax = fmax(zmax, sqrt(zn));
+        for (int q = 0; q < r; ++q) {
+            const int oc = q >> 6, ol = q & 63;
+            C<double> sq = {0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (c == oc) sq = sv[c];
+            sq.x = __shfl(sq.x, ol, 64);
+            sq.y = __shfl(sq.y, ol, 64);
+            const C<R> d = Rm[q + int64_t(q) * ldr];
+            const double dr = d.x, di = d.y, den = dr * dr + di * di;
+            const C<double> w = {(sq.x * dr + sq.y * di) / den, (sq.y * dr - sq.x * di) / den};
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const int k = lane + 64 * c;
+                if (k > q && k < n) {
+                    const C<R> rv = Rm[q + int64_t(k) * ldr];
+                    sv[c] = csub(sv[c], cmul(w, C<double>{double(rv.x), double(rv.y)}));
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int k = lane + 64 * c;
+            if (k >= r && k < n) en[c] += sv[c].x * sv[c].x + sv[c].y * sv[c].y;
+        }
+    }
+    double emax = 0.0, zmax = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        emax = fmax(emax, sqrt(en[c]));
+        zmax = fmax(zmax, sqrt(zn[c]));
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -580,50 +677,17 @@ __global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Zc,
     }
 }
 
-void launch_judge(hipStream_t s, int prec, const void *Zc, const void *T, const JudgeTask *tasks,
-                  int32_t n_tasks, const int32_t *piv, double *out) {
+void launch_judge(hipStream_t s, int prec, const void *Z, const JudgeTask *tasks, int32_t n_tasks,
+                  const int32_t *piv, double *out) {
     if (n_tasks <= 0) return;
     const int64_t nb = (int64_t(n_tasks) + 3) / 4;
     BFG_REQUIRE(nb < (int64_t(1) << 31), "judge: too many tasks");
     if (prec == BFGPU_C128)
-        hipLaunchKernelGGL(judge_kernel<double>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<double> *)Zc,
-                           (const C<double> *)T, tasks, n_tasks, piv, out);
+        hipLaunchKernelGGL(judge_kernel<double>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<double> *)Z, tasks,
+                           n_tasks, piv, out);
     else
-        hipLaunchKernelGGL(judge_kernel<float>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<float> *)Zc,
-                           (const C<float> *)T, tasks, n_tasks, piv, out);
-    BFG_HIP(hipGetLastError());
-}
-
-// Runs of complex entries copied between buffers: one thread per entry
-// (grid-stride), its run found by binary search over the running ends.
-template <typename R>
-__global__ __launch_bounds__(256) void copy_runs_kernel(const C<R> *__restrict__ src, C<R> *__restrict__ dst,
-                                                        const RunCopy *__restrict__ tasks,
-                                                        const int64_t *__restrict__ ends, int32_t n_tasks,
-                                                        int64_t total) {
-    int32_t lo = 0;
-    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += int64_t(gridDim.x) * 256) {
-        int32_t hi = n_tasks - 1;  // entries only grow: search from the last run
-        while (lo < hi) {
-            const int32_t mid = (lo + hi) >> 1;
-            if (ends[mid] > e) hi = mid;
-            else lo = mid + 1;
-        }
-        const int64_t local = e - (lo > 0 ? ends[lo - 1] : 0);
-        dst[tasks[lo].dst_ofs + local] = src[tasks[lo].src_ofs + local];
-    }
-}
-
-void launch_copy_runs(hipStream_t s, int prec, const void *src, void *dst, const RunCopy *tasks,
-                      const int64_t *ends, int32_t n_tasks, int64_t total) {
-    if (total <= 0 || n_tasks <= 0) return;
-    dim3 grid(unsigned(std::min<int64_t>((total + 255) / 256, int64_t(1) << 22)));
-    if (prec == BFGPU_C128)
-        hipLaunchKernelGGL(copy_runs_kernel<double>, grid, dim3(256), 0, s, (const C<double> *)src,
-                           (C<double> *)dst, tasks, ends, n_tasks, total);
-    else
-        hipLaunchKernelGGL(copy_runs_kernel<float>, grid, dim3(256), 0, s, (const C<float> *)src,
-                           (C<float> *)dst, tasks, ends, n_tasks, total);
+        hipLaunchKernelGGL(judge_kernel<float>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<float> *)Z, tasks,
+                           n_tasks, piv, out);
     BFG_HIP(hipGetLastError());
 }
 

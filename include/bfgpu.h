@@ -187,8 +187,9 @@ typedef struct {
      * max_j |Zc[:,j] - Zc[:,skel] T[:,j]| > judge_slack * eps * max_j
      * |Zc[:,j]| (the pivoted QR's own stopping rule on rows it never saw),
      * the compression is redone by plain adaptive RRQR from
-     * max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.  judge_rows 0: no
-     * judge; judge_slack <= 0: 4. */
+     * max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.  Compressions of more
+     * than 256 columns, or whose sample holds every observer, are not
+     * judged.  judge_rows 0: no judge; judge_slack <= 0: 4. */
     int32_t judge_rows;
     double judge_slack;
 } bfgpu_opts;

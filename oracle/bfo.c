@@ -683,7 +683,8 @@ static cplx *do_compress(bfo_bf *bf, const bfo_opts *o, const double *obs_xyz,
     int32_t r = bfo_compress(o->k, obs_xyz, obs_idx, n_obs, src_xyz, src_idx, n_src, guess, eps,
                              o->adaptive, key, (double *)T, *skel, &rows);
     int32_t verdict = 0;
-    if (o->judge_rows > 0 && n_src > 0 && rows < n_obs) {
+    /* judged: samples that left observers unseen, at most 256 columns */
+    if (o->judge_rows > 0 && n_src > 0 && n_src <= 256 && rows < n_obs) {
         double e_max = 0.0, z_max = 0.0;
         bfo_judge(o->k, obs_xyz, obs_idx, n_obs, src_xyz, src_idx, n_src, key, rows, o->judge_rows,
                   (const double *)T, *skel, r, &e_max, &z_max);
