@@ -887,8 +887,9 @@ class BlockSparseNear:
     the point kernel exp(-ik r)/(4 pi r) is singular (default 0)."""
 
     def __init__(self, kernelmatrix, tree, pairs, *, acctype=np.complex128, selfterm=0.0, ctx=None):
-        if not isinstance(kernelmatrix, HelmholtzKernelMatrix):
-            raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix")
+        from .rwg import EFIEKernelMatrix
+        if not isinstance(kernelmatrix, (HelmholtzKernelMatrix, EFIEKernelMatrix)):
+            raise ArgumentError(1, "kernelmatrix must be a HelmholtzKernelMatrix or an EFIEKernelMatrix")
         self.ctx = ctx or default_context()
         self.tree = tree
         self.pairs = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
@@ -898,9 +899,16 @@ class BlockSparseNear:
         prec = L.BFGPU_C128 if dt == np.complex128 else L.BFGPU_C64
         sv = (C.c_double * 2)(float(np.real(selfterm)), float(np.imag(selfterm)))
         h = C.c_void_p()
-        check(lib().bfgpu_near_assemble(self.ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,
-                                        float(kernelmatrix.k), prec, sv, self.pairs.shape[0],
-                                        ptr(self.pairs, C.c_int32), C.byref(h)))
+        if isinstance(kernelmatrix, EFIEKernelMatrix):
+            self._kernel = kernelmatrix  # keeps the plug-in handle alive
+            check(lib().bfgpu_near_assemble_kernel(self.ctx.handle, tree.testcluster.handle,
+                                                   tree.trialcluster.handle, kernelmatrix.handle(self.ctx),
+                                                   float(kernelmatrix.k), prec, self.pairs.shape[0],
+                                                   ptr(self.pairs, C.c_int32), C.byref(h)))
+        else:
+            check(lib().bfgpu_near_assemble(self.ctx.handle, tree.testcluster.handle, tree.trialcluster.handle,
+                                            float(kernelmatrix.k), prec, sv, self.pairs.shape[0],
+                                            ptr(self.pairs, C.c_int32), C.byref(h)))
         self._h = h
         self._owner = None
         self.op = L.OP_N
@@ -1000,9 +1008,6 @@ class PetrovGalerkinBF(ButterflyFactorization):
                  workspace_bytes=0, ctx=None):
         ctx = ctx or default_context()
         _check_kernel(kernelmatrix, k)
-        if not farfieldonly and not isinstance(kernelmatrix, HelmholtzKernelMatrix):
-            raise ArgumentError(1, "the EFIE near field needs singular quadrature, which is not part of this "
-                                   "library: pass farfieldonly=True")
         admissibility = admissibility or isFarFunctor(tree_parameters(tree.testcluster).alpha)
         if rankestimator is None:
             rankestimator = ButterflyRankEstimator(tree_parameters(tree, admissibility).Ctau)

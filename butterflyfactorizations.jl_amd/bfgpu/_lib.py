@@ -84,6 +84,9 @@ _SIGS = {
     "bfgpu_kernel_rwg_create": (i32, [vp, P(f64), P(f64), i64, i32, P(f64), i64, i32, P(f64), P(vp)]),
     "bfgpu_kernel_fill": (i32, [vp, f64, i32, i32, P(i32), i32, P(i32), vp]),
     "bfgpu_kernel_destroy": (i32, [vp]),
+    "bfgpu_kernel_rwg_set_near": (i32, [vp, P(f64), P(f64), i32, i32, i32, i32, f64]),
+    "bfgpu_kernel_fill_near": (i32, [vp, f64, i32, i32, P(i32), i32, P(i32), vp]),
+    "bfgpu_near_assemble_kernel": (i32, [vp, vp, vp, vp, f64, i32, i64, P(i32), P(vp)]),
     "bfgpu_near_assemble": (i32, [vp, vp, vp, f64, i32, P(f64), i64, P(i32), P(vp)]),
     "bfgpu_near_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
     "bfgpu_near_info": (i32, [vp, P(i64)]),

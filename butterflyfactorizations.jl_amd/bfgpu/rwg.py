@@ -140,6 +140,24 @@ class RWG:
         V = np.asarray(self.vertices, np.float64)
         return 0.5 * (V[self.edges[:, 0]] + V[self.edges[:, 1]])
 
+    def geometry(self):
+        """Per function, plus then minus triangle: 13 doubles each -- the
+        three vertices, the free vertex v and c with f(x) = c (x - v)
+        (c = +l/(2 A+) on the plus, -l/(2 A-) on the minus triangle, so
+        div f = 2 c): shape (n, 26)."""
+        V = np.asarray(self.vertices, np.float64)
+        T = np.asarray(self.triangles, np.int64)
+        out = np.zeros((len(self), 26))
+        for side, (tri, free, sgn) in enumerate(((self.plus, self.free_plus, 1.0),
+                                                 (self.minus, self.free_minus, -1.0))):
+            P = V[T[tri]]
+            area = 0.5 * np.linalg.norm(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), axis=1)
+            o = 13 * side
+            out[:, o:o + 9] = P.reshape(-1, 9)
+            out[:, o + 9:o + 12] = V[free]
+            out[:, o + 12] = sgn * self.length / (2 * area)
+        return out
+
     def samples(self, degree):
         """Quadrature samples of every function: (n, 2 q, 7) with rows
         (x, y, z, w A f(x) (3), w A div f) over the plus then the minus
@@ -168,13 +186,20 @@ class EFIEKernelMatrix:
     quadrature with rules of degree `rules` = (outer, inner) on test / trial
     triangles.  Z = alpha <f, G g> + beta <div f, G div g> with
     G = exp(-ik|x-y|)/(4 pi |x-y|), alpha = -ik, beta = -1/(ik) by default
-    (gamma = ik)."""
+    (gamma = ik).  Near-field blocks (``near_block``, and the near field of
+    ``PetrovGalerkinBF``) use singularity extraction for triangle pairs
+    closer than ``near_factor`` longest edges (outer / inner rule degrees
+    ``near_rules``; closed-form static part, Wilton et al. 1984) and the
+    double rule ``near_far_rules`` for the other pairs."""
     rwg: RWG
     k: float
     trial_rwg: RWG = None
     alpha: complex = None
     beta: complex = None
     rules: tuple = (2, 3)
+    near_rules: tuple = (5, 5)
+    near_far_rules: tuple = (5, 5)
+    near_factor: float = 3.0
     _handles: dict = field(default_factory=dict, repr=False)
 
     def __post_init__(self):
@@ -202,6 +227,12 @@ class EFIEKernelMatrix:
                                                 ts.ctypes.data_as(dp), ss.shape[0], ss.shape[1],
                                                 ss.ctypes.data_as(dp), C.byref(h)))
             self._handles[key] = (h, ctx)
+            tg = np.ascontiguousarray(self.rwg.geometry())
+            sg = np.ascontiguousarray(self.trial_rwg.geometry())
+            check(lib().bfgpu_kernel_rwg_set_near(h, tg.ctypes.data_as(dp), sg.ctypes.data_as(dp),
+                                                  int(self.near_rules[0]), int(self.near_rules[1]),
+                                                  int(self.near_far_rules[0]), int(self.near_far_rules[1]),
+                                                  float(self.near_factor)))
         return self._handles[key][0]
 
     def block(self, rows, cols, acctype=np.complex128, ctx=None):
@@ -219,6 +250,23 @@ class EFIEKernelMatrix:
         check(lib().bfgpu_kernel_fill(self.handle(ctx), float(self.k), prec, len(rows),
                                       rows.ctypes.data_as(L.P(C.c_int32)), len(cols),
                                       cols.ctypes.data_as(L.P(C.c_int32)), C.c_void_p(Z.ctypes.data)))
+        return Z
+
+    def near_block(self, rows, cols, acctype=np.complex128, ctx=None):
+        """Z[rows, cols] of the near-field quadrature, computed on the device."""
+        import ctypes as C
+        from . import _lib as L
+        from ._lib import check, lib
+        from . import default_context
+        ctx = ctx or default_context()
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        dt = np.dtype(acctype)
+        Z = np.zeros((len(rows), len(cols)), dt, order="F")
+        prec = L.BFGPU_C128 if dt == np.complex128 else L.BFGPU_C64
+        check(lib().bfgpu_kernel_fill_near(self.handle(ctx), float(self.k), prec, len(rows),
+                                           rows.ctypes.data_as(L.P(C.c_int32)), len(cols),
+                                           cols.ctypes.data_as(L.P(C.c_int32)), C.c_void_p(Z.ctypes.data)))
         return Z
 
     def __del__(self, _destroy=_destroy):

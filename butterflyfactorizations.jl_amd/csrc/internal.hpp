@@ -350,9 +350,14 @@ struct FillTask {       // one kernel-matrix block
 struct KernelRWG;
 const KernelRWG *kernel_of(const bfgpu_kernel *h);
 int64_t kernel_functions(const KernelRWG &k, bool test);
+bool kernel_has_near(const KernelRWG &k);
 void launch_fill_rwg(hipStream_t s, int prec, const KernelRWG &kr, const int32_t *rows, const int32_t *cols,
                      const FillTask *tasks, const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z,
                      double k);
+// the plug-in's near-field quadrature (singularity extraction), same tasks
+void launch_fill_rwg_near(hipStream_t s, int prec, const KernelRWG &kr, const int32_t *rows, const int32_t *cols,
+                          const FillTask *tasks, const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z,
+                          double k);
 void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *src_pts,
                  const int32_t *rows, const int32_t *cols, const FillTask *tasks,
                  const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k,

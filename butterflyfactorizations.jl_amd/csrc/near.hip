@@ -445,6 +445,10 @@ struct Near {
     int32_t mfma_min_rhs = 8;        // 0: vector ALU only
     std::unique_ptr<Near> tcopy;
     bool tcopy_failed = false, use_tcopy = true;
+    // kernel-matrix plug-in whose near quadrature fills the blocks (null:
+    // the point kernel); its blocks are not symmetric under the swap, so
+    // T / C products take the two-pass product over the stored blocks
+    const KernelRWG *kern = nullptr;
 };
 
 namespace {
@@ -582,7 +586,7 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
 
 // the transposed copy, built on first use; null when it does not fit
 Near *transposed_copy(Near &nf) {
-    if (!nf.use_tcopy || nf.tcopy_failed) return nullptr;
+    if (!nf.use_tcopy || nf.tcopy_failed || nf.kern) return nullptr;
     if (nf.tcopy) return nf.tcopy.get();
     size_t fr = 0, tot = 0;
     BFG_HIP(hipMemGetInfo(&fr, &tot));
@@ -782,9 +786,13 @@ void near_assemble(Near &nf, double k, double self_re, double self_im, int64_t n
         put(d_src, tr.points, s);
         put(d_fill, fill, s);
         put(d_end, fill_end, s);
-        launch_fill(s, nf.precision, d_obs.as<double>(), d_src.as<double>(), nf.rowx.as<int32_t>(),
-                    nf.colx.as<int32_t>(), d_fill.as<FillTask>(), d_end.as<int64_t>(), int32_t(fill.size()), fofs,
-                    nf.data.p, k, self_re, self_im);
+        if (nf.kern)
+            launch_fill_rwg_near(s, nf.precision, *nf.kern, nf.rowx.as<int32_t>(), nf.colx.as<int32_t>(),
+                                 d_fill.as<FillTask>(), d_end.as<int64_t>(), int32_t(fill.size()), fofs, nf.data.p, k);
+        else
+            launch_fill(s, nf.precision, d_obs.as<double>(), d_src.as<double>(), nf.rowx.as<int32_t>(),
+                        nf.colx.as<int32_t>(), d_fill.as<FillTask>(), d_end.as<int64_t>(), int32_t(fill.size()),
+                        fofs, nf.data.p, k, self_re, self_im);
         BFG_HIP(hipStreamSynchronize(s));
     } else {
         BFG_HIP(hipStreamSynchronize(s));
@@ -814,6 +822,29 @@ struct bfgpu_near { bfg::Near n; };
         return BFGPU_ERR_STATE;           \
     }                                     \
     return BFGPU_OK;
+
+extern "C" int bfgpu_near_assemble_kernel(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
+                                          const bfgpu_kernel *kern, double k, int32_t precision, int64_t n_pairs,
+                                          const int32_t *pairs, bfgpu_near **out) {
+    NF_API_BEGIN
+    BFG_REQUIRE(ctx && test && trial && kern && out, "near_assemble_kernel: null argument");
+    BFG_REQUIRE(n_pairs >= 0 && (n_pairs == 0 || pairs), "near_assemble_kernel: bad pair list");
+    BFG_REQUIRE(precision == BFGPU_C64 || precision == BFGPU_C128, "near_assemble_kernel: precision");
+    const KernelRWG *kr = kernel_of(kern);
+    BFG_REQUIRE(kernel_functions(*kr, true) >= test->t.n_values && kernel_functions(*kr, false) >= trial->t.n_values,
+                "near_assemble_kernel: the plug-in has fewer functions than the trees have values");
+    BFG_REQUIRE(kernel_has_near(*kr), "near_assemble_kernel: call bfgpu_kernel_rwg_set_near first");
+    auto h = std::make_unique<bfgpu_near>();
+    h->n.ctx = &ctx->c;
+    h->n.test = &test->t;
+    h->n.trial = &trial->t;
+    h->n.precision = precision;
+    h->n.kern = kr;
+    h->n.use_tcopy = false;
+    near_assemble(h->n, k, 0.0, 0.0, n_pairs, pairs);
+    *out = h.release();
+    NF_API_END
+}
 
 extern "C" int bfgpu_near_assemble(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial, double k,
                                    int32_t precision, const double *self_value, int64_t n_pairs,

@@ -116,6 +116,14 @@ int bfgpu_nearandfar(const bfgpu_tree *test, const bfgpu_tree *trial, int32_t ki
 int bfgpu_near_assemble(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial, double k,
                         int32_t precision, const double *self_value, int64_t n_pairs,
                         const int32_t *pairs, bfgpu_near **out);
+/* The near field of a kernel-matrix plug-in: blocks filled by the plug-in's
+ * near-field quadrature (bfgpu_kernel_rwg_set_near must have been called);
+ * the trees' values index its test / trial functions.  Otherwise as
+ * bfgpu_near_assemble; its adjoint / transpose products read the stored
+ * blocks (two-pass), since the quadrature is not symmetric under the swap. */
+int bfgpu_near_assemble_kernel(bfgpu_ctx *ctx, const bfgpu_tree *test, const bfgpu_tree *trial,
+                               const bfgpu_kernel *kern, double k, int32_t precision, int64_t n_pairs,
+                               const int32_t *pairs, bfgpu_near **out);
 /* y = alpha * op(N) * x + beta * y (the near mul! of
  * src/matrixalgebra/matrixvector.jl:31-37 and its adjoint/transpose,
  * src/matrixalgebra/matrix_adj_tr.jl); arguments as bfgpu_matvec. */
@@ -158,6 +166,26 @@ int bfgpu_kernel_rwg_create(bfgpu_ctx *ctx, const double *alpha, const double *b
  * device by the plug-in's fill kernel; for checks. */
 int bfgpu_kernel_fill(const bfgpu_kernel *kern, double k, int32_t precision, int32_t m, const int32_t *rows,
                       int32_t n, const int32_t *cols, void *Z);
+/* Near-field quadrature of the plug-in, the :near kernel matrix of
+ * ext/BFBEAST/kernelmatrix.jl:26-36 (BEAST.defaultquadstrat; BEAST is absent,
+ * so the scheme is this restatement): per pair of triangles of a test and a
+ * trial function, when their centroids are closer than near_factor times
+ * the longer of their longest edges, singularity extraction -- the static
+ * kernel 1/(4 pi R) integrated in closed form over the trial triangle
+ * (Wilton, Rao, Glisson, Schaubert, Al-Bundak and Butler, IEEE Trans. AP
+ * 32(3), 1984), the rest (exp(-ikR) - 1)/(4 pi R) by the triangle rule of
+ * degree near_inner, the outer integral by the rule of degree near_outer --
+ * otherwise the double rule of degrees (far_outer, far_inner); degrees 1-5.
+ * test_geom / trial_geom: per function, for its plus then minus triangle,
+ * 13 doubles: the three vertices, the free vertex v and c with f(x) =
+ * c (x - v) on that triangle (so div f = 2 c). */
+int bfgpu_kernel_rwg_set_near(bfgpu_kernel *kern, const double *test_geom, const double *trial_geom,
+                              int32_t near_outer, int32_t near_inner, int32_t far_outer, int32_t far_inner,
+                              double near_factor);
+/* One m x n block of the near-field quadrature (column-major, `precision`),
+ * computed on the device; for checks. */
+int bfgpu_kernel_fill_near(const bfgpu_kernel *kern, double k, int32_t precision, int32_t m, const int32_t *rows,
+                           int32_t n, const int32_t *cols, void *Z);
 int bfgpu_kernel_destroy(bfgpu_kernel *kern);
 
 /* ---- factorization ---- */

@@ -113,3 +113,101 @@ def test_host_samples_reproduce_the_oracle_entries(mesh):
             Z[i, j] = (G * w).sum()
     want = efie.efie_block(V, T, K, rows, cols)
     assert np.linalg.norm(Z - want) <= 1e-13 * np.linalg.norm(want)
+
+
+def _subdivided_integrals(P, X, levels=5):
+    """im1, iv of triangle P at X by the degree-5 rule on 4^levels
+    sub-triangles (accurate away from the triangle)."""
+    tris = [np.asarray(P, float)]
+    for _ in range(levels):
+        nt = []
+        for a, b, c in tris:
+            ab, bc, ca = (a + b) / 2, (b + c) / 2, (c + a) / 2
+            nt += [np.array([a, ab, ca]), np.array([ab, b, bc]), np.array([ca, bc, c]), np.array([ab, bc, ca])]
+        tris = nt
+    bq, w = efie.RULES[5]
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    n /= np.linalg.norm(n)
+    rho = X - ((X - P[0]) @ n)[:, None] * n
+    im = np.zeros(len(X))
+    iv = np.zeros((len(X), 3))
+    for t in tris:
+        A = 0.5 * np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0]))
+        y = bq @ t
+        r = np.linalg.norm(X[:, None, :] - y[None], axis=2)
+        im += (w * A / r).sum(1)
+        iv += np.einsum("nq,nqd->nd", w * A / r, y[None] - rho[:, None])
+    return im, iv
+
+
+def test_potential_integrals_closed_forms():
+    """The closed-form potential integrals (Wilton et al. 1984) against
+    independent evaluations: a subdivided high-order rule off the triangle;
+    for points in the plane inside it, the polar form sum_i h_i
+    [ln(sec t + tan t)] over the sub-triangles seen from the point; and, for
+    the vector integral everywhere, the boundary form sum_i u_i int_edge R dl
+    (the surface gradient of R is (rho_y - rho_x)/R)."""
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 0.9, 0.05]])
+    off = np.array([[0.3, 0.3, 0.5], [2.0, 1.0, 0.3], [1.5, 1.5, 0.0], [-0.5, 0.2, 0.2], [0.4, 0.2, -0.8]])
+    a, b = efie.potential_integrals(P, off)
+    c, d = _subdivided_integrals(P, off)
+    assert np.max(np.abs(a - c) / np.abs(c)) < 1e-10
+    assert np.max(np.linalg.norm(b - d, axis=1) / np.linalg.norm(d, axis=1)) < 1e-10
+    # in the plane, inside: polar closed form
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    n /= np.linalg.norm(n)
+    for bary in ([1 / 3, 1 / 3, 1 / 3], [0.7, 0.2, 0.1], [0.05, 0.05, 0.9]):
+        x = np.asarray(bary) @ P
+        im1, _ = efie.potential_integrals(P, x[None])
+        ref = 0.0
+        for i in range(3):
+            a_, b_ = P[i], P[(i + 1) % 3]
+            l = (b_ - a_) / np.linalg.norm(b_ - a_)
+            foot = a_ + np.dot(x - a_, l) * l
+            h = np.linalg.norm(foot - x)
+            t1, t2 = np.dot(a_ - foot, l) / h, np.dot(b_ - foot, l) / h  # tan of the angles from the normal
+            ref += h * (np.arcsinh(t2) - np.arcsinh(t1))                   # h [ln(sec t + tan t)]
+        assert abs(im1[0] - ref) < 1e-13 * abs(ref)
+    # vector integral: boundary form by Gauss-Legendre on each edge
+    g, gw = np.polynomial.legendre.leggauss(60)
+    pts = np.vstack([off, [np.asarray([0.7, 0.2, 0.1]) @ P]])
+    _, iv = efie.potential_integrals(P, pts)
+    for x, v in zip(pts, iv):
+        ref = np.zeros(3)
+        for i in range(3):
+            a_, b_ = P[i], P[(i + 1) % 3]
+            L = np.linalg.norm(b_ - a_)
+            l = (b_ - a_) / L
+            u = np.cross(l, n)
+            s = 0.5 * (g + 1) * L
+            R = np.linalg.norm(a_[None] + s[:, None] * l[None] - x[None], axis=1)
+            ref += u * np.sum(0.5 * L * gw * R)
+        assert np.linalg.norm(v - ref) < 1e-12 * np.linalg.norm(ref)
+
+
+def test_near_quadrature_entries(mesh):
+    """Near-field entries by singularity extraction: they converge with the
+    rule order (degree 3 -> 5 changes them by more than degree 4 -> 5), the
+    Galerkin matrix is symmetric to the quadrature accuracy, and pairs of
+    functions farther apart than the extraction distance reproduce the far
+    entries of the same double rule."""
+    V, T = mesh
+    from bfgpu import rwg
+    r = rwg.RWG(V, T)
+    c = r.centers
+    rows = np.argsort(np.linalg.norm(c - c[10], axis=1))[:8]
+    Z5 = efie.efie_near_block(V, T, K, rows, rows, near_rules=(5, 5), rules=(5, 5))
+    Z4 = efie.efie_near_block(V, T, K, rows, rows, near_rules=(4, 4), rules=(4, 4))
+    Z3 = efie.efie_near_block(V, T, K, rows, rows, near_rules=(3, 3), rules=(3, 3))
+    d35 = np.linalg.norm(Z3 - Z5) / np.linalg.norm(Z5)
+    d45 = np.linalg.norm(Z4 - Z5) / np.linalg.norm(Z5)
+    assert d45 < 0.25 * d35 and d45 < 0.02
+    assert np.linalg.norm(Z5 - Z5.T) / np.linalg.norm(Z5) < 1e-3
+    far = np.argsort(np.linalg.norm(c - c[10], axis=1))[-5:]
+    Zn = efie.efie_near_block(V, T, K, rows[:4], far, near_rules=(5, 5), rules=(5, 5), near_factor=3.0)
+    Zf = efie.efie_block(V, T, K, rows[:4], far, rules=(5, 5))
+    assert np.linalg.norm(Zn - Zf) < 1e-13 * np.linalg.norm(Zf)
+    # the static part is what makes the self and neighbour entries finite:
+    # the double rule alone misses them by a wide margin
+    Zd = efie.efie_block(V, T, K, rows, rows, rules=(5, 5))
+    assert np.linalg.norm(Zd - Z5) / np.linalg.norm(Z5) > 0.1
