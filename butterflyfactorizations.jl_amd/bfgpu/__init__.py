@@ -552,7 +552,7 @@ class ButterflyFactorization:
         return self
 
     OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7,
-               "mfma": 8, "rhs_chunk": 9, "pfree": 10}
+               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11}
 
     @staticmethod
     def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0):

@@ -736,6 +736,10 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         bf.rhs_chunk = value;
         release_graph(bf);
         break;
+    case BFGPU_OPT_SLOT_CAP:
+        BFG_REQUIRE(value >= 0, "bf_set_option: slot_cap is a number of KiB (0: as device memory allows)");
+        bf.slot_cap_kib = value;
+        break;
     case BFGPU_OPT_PFREE:
         BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: pfree is 0 or 1");
         bf.use_pfree = value != 0;
@@ -784,6 +788,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_RHS_LANES: *value = bf.rhs_lanes_min; break;
     case BFGPU_OPT_MFMA: *value = bf.use_mfma ? bf.mfma_rows : 0; break;
     case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
+    case BFGPU_OPT_SLOT_CAP: *value = bf.slot_cap_kib; break;
     case BFGPU_OPT_PFREE: *value = bf.use_pfree && bf.pfree_ok ? 1 : 0; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }

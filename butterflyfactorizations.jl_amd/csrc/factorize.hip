@@ -230,6 +230,35 @@ struct Engine {
             *d = DevBuf();
     }
 
+    // out[0] = 0, out[i + 1] = out[i] + v(i): chunk sums in parallel, a
+    // serial pass over the chunk totals, then the chunks offset in parallel
+    template <class F> static void prefix_sum(int64_t n, std::vector<int64_t> &out, F v) {
+        out.resize(size_t(n) + 1);
+        out[0] = 0;
+        constexpr int64_t CH = int64_t(1) << 15;
+        const int64_t nch = (n + CH - 1) / CH;
+        std::vector<int64_t> part(size_t(nch) + 1, 0);
+        parallel_for(nch, [&](int64_t c) {
+            int64_t sum = 0;
+            for (int64_t i = c * CH; i < std::min(n, (c + 1) * CH); ++i) {
+                sum += v(i);
+                out[size_t(i) + 1] = sum;
+            }
+            part[size_t(c) + 1] = sum;
+        }, 1);
+        for (int64_t c = 0; c < nch; ++c) part[size_t(c) + 1] += part[size_t(c)];
+        parallel_for(nch, [&](int64_t c) {
+            const int64_t base = part[size_t(c)];
+            if (base)
+                for (int64_t i = c * CH; i < std::min(n, (c + 1) * CH); ++i) out[size_t(i) + 1] += base;
+        }, 1);
+    }
+
+    template <class T, class A> void upload(DevBuf &d, const std::vector<T, A> &h, hipStream_t s) {
+        d.reserve(sizeof(T) * std::max<size_t>(h.size(), 1));
+        if (!h.empty()) BFG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    }
+
     void ensure_pool(DevBuf &pool, int64_t &cap_elems, int64_t need) {
         if (need <= cap_elems) return;
         int64_t nc = std::max<int64_t>(need, cap_elems * 3 / 2 + 1024);
@@ -291,43 +320,49 @@ struct Engine {
     // 2 * rows) rows.  The QR tiles, pivots and held-out tiles of the batch
     // are still on the device.
     void judge(std::vector<Comp> &cs, const std::vector<int32_t> &pend, const std::vector<int32_t> &acc,
-               const std::vector<QrTask> &qt, const std::vector<int32_t> &jq, const std::vector<int64_t> &jr,
+               const hvec<QrTask> &qt, const std::vector<int32_t> &jq, const std::vector<int64_t> &jr,
                const std::vector<int64_t> &jz, const std::vector<int32_t> &jpos, std::vector<int32_t> &retry) {
         auto t0 = clk::now();
-        std::vector<JudgeTask> jt;
-        std::vector<int32_t> who;
-        jt.reserve(jq.size());
-        for (int32_t q : acc) {
-            const int32_t a = jpos[q];
-            if (a < 0) continue;
-            const Comp &c = cs[pend[q]];
-            jt.push_back(JudgeTask{jz[a], qt[q].zofs, int32_t(jr[a + 1] - jr[a]), qt[q].m, qt[q].n, c.rank,
-                                   qt[q].pofs, 0});
-            who.push_back(q);
-        }
         (void)jq;
-        if (jt.empty()) return;
-        const int64_t nj = int64_t(jt.size());
+        // the accepted compressions with held-out tiles, in batch order
+        const int64_t na = int64_t(acc.size());
+        std::vector<int64_t> at;
+        prefix_sum(na, at, [&](int64_t i) { return int64_t(jpos[acc[size_t(i)]] >= 0); });
+        const int64_t nj = at[size_t(na)];
+        if (nj == 0) return;
+        hvec<JudgeTask> jt(static_cast<size_t>(nj));
+        std::vector<int32_t> who(static_cast<size_t>(nj));
+        parallel_for(na, [&](int64_t i) {
+            const int32_t q = acc[size_t(i)];
+            const int32_t a = jpos[q];
+            if (a < 0) return;
+            jt[size_t(at[size_t(i)])] = JudgeTask{jz[a], qt[q].zofs, int32_t(jr[a + 1] - jr[a]), qt[q].m, qt[q].n,
+                                                  cs[pend[q]].rank, qt[q].pofs, 0};
+            who[size_t(at[size_t(i)])] = q;
+        }, 4096);
         upload(d_jtask, jt, ctx.stream);
         d_jres.reserve(sizeof(double) * 2 * size_t(nj));
         launch_judge(ctx.stream, prec, d_Z.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
                      d_jres.as<double>());
-        std::vector<double> res(2 * size_t(nj));
+        hvec<double> res(2 * static_cast<size_t>(nj));
         BFG_HIP(hipMemcpyAsync(res.data(), d_jres.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, ctx.stream));
         BFG_HIP(hipStreamSynchronize(ctx.stream));
         const double slack = o.judge_slack > 0.0 ? o.judge_slack : 4.0;
+        std::vector<uint8_t> fail(static_cast<size_t>(nj));
+        parallel_for(nj, [&](int64_t a) {
+            Comp &c = cs[pend[who[size_t(a)]]];
+            fail[size_t(a)] = !(res[2 * a] <= slack * c.eps * res[2 * a + 1]);
+            c.judge = fail[size_t(a)] ? 2 : 1;
+        }, 4096);
+        n_judged += nj;
+        // rejected ones (about one in a hundred), in batch order
         for (int64_t a = 0; a < nj; ++a) {
-            Comp &c = cs[pend[who[a]]];
-            ++n_judged;
-            if (res[2 * a] <= slack * c.eps * res[2 * a + 1]) {
-                c.judge = 1;
-                continue;
-            }
-            c.judge = 2;
+            if (!fail[size_t(a)]) continue;
+            Comp &c = cs[pend[who[size_t(a)]]];
             ++n_fallback;
             const int32_t plain = std::max(int32_t(std::ceil(o.c_tau * std::log(1.0 / c.eps))), 10);
             c.rows = std::min(std::max(plain, 2 * c.rows), c.n_obs);
-            retry.push_back(pend[who[a]]);
+            retry.push_back(pend[who[size_t(a)]]);
         }
         t_judge += secs(t0, clk::now());
     }
@@ -350,11 +385,13 @@ struct Engine {
             c.rows = std::min(std::max(c.guess, 10), c.n_obs);
             pend.push_back(int32_t(i));
         }
-        std::vector<int32_t> hrows, hcols, hrank, hpiv, jq, jpos;
-        std::vector<FillTask> ft;
-        std::vector<int64_t> fe, rofs, cofs, jr, jz;
-        std::vector<QrTask> qt;
-        std::vector<InterpTask> it;
+        hvec<int32_t> hrows, hcols;
+        std::vector<int32_t> hrank, hpiv, jq, jpos;
+        hvec<FillTask> ft;
+        hvec<int64_t> fe;
+        std::vector<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big;
+        hvec<QrTask> qt;
+        hvec<InterpTask> it;
         // kernel-matrix tiles of one batch: at most 32 GiB or 30 % of the
         // free device memory (besides what the tile buffer already holds)
         int64_t z_budget = o.workspace_bytes;
@@ -380,50 +417,55 @@ struct Engine {
                 pend.resize(nb);
             }
             const size_t np = pend.size();
-            ft.resize(np);
-            fe.resize(np);
-            qt.resize(np);
-            rofs.assign(np + 1, 0);
-            cofs.assign(np + 1, 0);
-            int64_t zofs = 0;
-            int32_t max_n = 1, max_m = 1;
-            for (size_t q = 0; q < np; ++q) {
-                const Comp &c = cs[pend[q]];
-                rofs[q + 1] = rofs[q] + c.rows;
-                cofs[q + 1] = cofs[q] + int64_t(c.cols.size());
-                FillTask f{zofs, c.rows, int32_t(c.cols.size()), int32_t(rofs[q]), int32_t(cofs[q])};
-                ft[q] = f;
-                zofs += int64_t(f.m) * f.n;
-                fe[q] = zofs;
-                qt[q] = QrTask{f.zofs, f.m, f.n, int32_t(cofs[q]), 0};
-                max_n = std::max(max_n, f.n);
-                max_m = std::max(max_m, f.m);
-            }
-            // held-out rows of the residual judge (learned ranks): the next
-            // rows of the same sample prefix, filled with the main tiles
-            // (their tiles after the main ones in the workspace)
-            jq.clear();
-            jr.assign(1, 0);
-            jpos.assign(np, -1);
-            for (size_t q = 0; q < np; ++q) {
-                const int32_t mc = judge_m(cs[pend[q]]);
-                if (mc == 0) continue;
-                jpos[q] = int32_t(jq.size());
-                jq.push_back(int32_t(q));
-                jr.push_back(jr.back() + mc);
-            }
-            const size_t nj = jq.size();
+            // sizes of every tile, then running offsets (parallel prefix
+            // sums): main tiles, then the judge's held-out tiles
+            std::vector<int32_t> jm(np);
+            parallel_for(int64_t(np), [&](int64_t q) { jm[q] = judge_m(cs[pend[q]]); }, 4096);
+            prefix_sum(int64_t(np), rofs, [&](int64_t q) { return int64_t(cs[pend[q]].rows); });
+            prefix_sum(int64_t(np), cofs, [&](int64_t q) { return int64_t(cs[pend[q]].cols.size()); });
+            prefix_sum(int64_t(np), zo, [&](int64_t q) {
+                return int64_t(cs[pend[q]].rows) * int64_t(cs[pend[q]].cols.size());
+            });
+            prefix_sum(int64_t(np), jrf, [&](int64_t q) { return int64_t(jm[q]); });
+            prefix_sum(int64_t(np), jzf, [&](int64_t q) { return int64_t(jm[q]) * int64_t(cs[pend[q]].cols.size()); });
+            prefix_sum(int64_t(np), jcnt, [&](int64_t q) { return int64_t(jm[q] > 0); });
+            const size_t nj = size_t(jcnt[np]);
+            const int64_t zmain = zo[np];
+            int64_t zofs = zmain + jzf[np];
             ft.resize(np + nj);
             fe.resize(np + nj);
+            qt.resize(np);
+            jq.resize(nj);
+            jr.resize(nj + 1);
             jz.resize(nj);
-            for (size_t a = 0; a < nj; ++a) {
-                const int32_t q = jq[a];
-                const int32_t mc = int32_t(jr[a + 1] - jr[a]), n = int32_t(cs[pend[q]].cols.size());
-                ft[np + a] = FillTask{zofs, mc, n, int32_t(rofs[np] + jr[a]), int32_t(cofs[q])};
-                jz[a] = zofs;
-                zofs += int64_t(mc) * n;
-                fe[np + a] = zofs;
-            }
+            jpos.resize(np);
+            std::atomic<int32_t> amax_n{1}, amax_m{1};
+            parallel_for(int64_t(np), [&](int64_t q) {
+                const Comp &c = cs[pend[q]];
+                const int32_t n = int32_t(c.cols.size());
+                ft[q] = FillTask{zo[q], c.rows, n, int32_t(rofs[q]), int32_t(cofs[q])};
+                fe[q] = zo[q + 1];
+                qt[q] = QrTask{zo[q], c.rows, n, int32_t(cofs[q]), 0};
+                int32_t v = amax_n.load(std::memory_order_relaxed);
+                while (n > v && !amax_n.compare_exchange_weak(v, n)) {
+                }
+                v = amax_m.load(std::memory_order_relaxed);
+                while (c.rows > v && !amax_m.compare_exchange_weak(v, c.rows)) {
+                }
+                if (jm[q] == 0) {
+                    jpos[q] = -1;
+                    return;
+                }
+                const int64_t a = jcnt[q];
+                jpos[q] = int32_t(a);
+                jq[a] = int32_t(q);
+                jr[a] = jrf[q];
+                jz[a] = zmain + jzf[q];
+                ft[np + a] = FillTask{jz[a], jm[q], n, int32_t(rofs[np] + jrf[q]), int32_t(cofs[q])};
+                fe[np + a] = jz[a] + int64_t(jm[q]) * n;
+            }, 4096);
+            jr[nj] = jrf[np];
+            const int32_t max_n = amax_n.load(), max_m = amax_m.load();
             BFG_REQUIRE(rofs[np] + jr[nj] < (int64_t(1) << 31) && cofs[np] < (int64_t(1) << 31),
                         "factorize: level too large for 32-bit index lists");
             const int32_t pofs = int32_t(cofs[np]);
@@ -436,13 +478,12 @@ struct Engine {
                     sample_prefix(c.key, c.n_obs, c.rows, r);
                     for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[r[i]];
                 } else {
-                    const int64_t a = jpos[q];
-                    const int32_t mc = int32_t(jr[a + 1] - jr[a]);
+                    const int32_t mc = jm[q];
                     thread_local std::vector<int32_t> pre;
                     pre.resize(size_t(c.rows) + size_t(mc));
                     sample_prefix(c.key, c.n_obs, c.rows + mc, pre.data());
                     for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[pre[size_t(i)]];
-                    int32_t *h = hrows.data() + rofs[np] + jr[a];
+                    int32_t *h = hrows.data() + rofs[np] + jrf[q];
                     for (int32_t i = 0; i < mc; ++i) h[i] = c.obs[pre[size_t(c.rows + i)]];
                 }
                 std::copy(c.cols.begin(), c.cols.end(), hcols.begin() + cofs[q]);
@@ -463,13 +504,22 @@ struct Engine {
             // hold factorizations of different depth, and the blocks that
             // fit a wavefront's LDS take the wavefront kernel while larger
             // ones take the workgroup kernel, whatever else is in the batch
+            // (one group, the common case, is found without building lists)
+            const double eps0 = cs[pend[0]].eps;
+            const bool ok0 = cpqr_wave_ok(prec, qt[0].m, qt[0].n);
+            std::atomic<bool> one_group{true};
+            parallel_for(int64_t(np), [&](int64_t q) {
+                if (cs[pend[q]].eps != eps0 || cpqr_wave_ok(prec, qt[q].m, qt[q].n) != ok0)
+                    one_group.store(false, std::memory_order_relaxed);
+            }, 1 << 14);
             std::map<std::pair<double, int>, std::vector<int32_t>> groups;
-            for (size_t q = 0; q < np; ++q)
-                groups[{cs[pend[q]].eps, cpqr_wave_ok(prec, qt[q].m, qt[q].n) ? 1 : 0}].push_back(int32_t(q));
+            if (!one_group.load())
+                for (size_t q = 0; q < np; ++q)
+                    groups[{cs[pend[q]].eps, cpqr_wave_ok(prec, qt[q].m, qt[q].n) ? 1 : 0}].push_back(int32_t(q));
             hrank.assign(np, 0);
-            if (groups.size() == 1) {
+            if (one_group.load()) {
                 launch_cpqr(ctx.stream, prec, d_Z.p, d_qr.as<QrTask>(), int32_t(np),
-                            d_piv.as<int32_t>(), d_rank.as<int32_t>(), groups.begin()->first.first, max_n, max_m);
+                            d_piv.as<int32_t>(), d_rank.as<int32_t>(), eps0, max_n, max_m);
                 BFG_HIP(hipMemcpyAsync(hrank.data(), d_rank.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, ctx.stream));
             } else {
                 std::vector<QrTask> sub;
@@ -498,20 +548,26 @@ struct Engine {
             BFG_HIP(hipStreamSynchronize(ctx.stream));
             auto ti = clk::now();
             t_qr += secs(tq, ti);
+            // adaptive doubling (compressors.jl:134) decided per compression
+            // in parallel; the retried ones in batch order
+            std::vector<uint8_t> dbl(np);
+            parallel_for(int64_t(np), [&](int64_t q) {
+                Comp &c = cs[pend[q]];
+                const int32_t r = hrank[q];
+                dbl[q] = o.adaptive && r > int32_t(std::floor(0.8 * c.rows)) && c.rows < c.n_obs;
+                if (!dbl[q]) c.rank = r;
+            }, 4096);
             std::vector<int32_t> retry, acc;
-            it.clear();
-            int32_t max_rank = 0;
+            acc.reserve(np);
             for (size_t q = 0; q < np; ++q) {
                 Comp &c = cs[pend[q]];
-                int32_t r = hrank[q];
-                if (o.adaptive && r > int32_t(std::floor(0.8 * c.rows)) && c.rows < c.n_obs) {
+                if (dbl[q]) {
                     c.rows = std::min(2 * c.rows, c.n_obs);
                     retry.push_back(pend[q]);
                     ++n_retry;
                     continue;
                 }
-                c.rank = r;
-                if (r == 0 && n_rank0++ < 4 && std::getenv("BFGPU_TIMING"))
+                if (c.rank == 0 && n_rank0++ < 4 && std::getenv("BFGPU_TIMING"))
                     std::fprintf(stderr, "[bfgpu] rank-0 compression: %d sampled rows of %d observers x %d columns\n",
                                  c.rows, c.n_obs, int(c.cols.size()));
                 acc.push_back(int32_t(q));
@@ -522,15 +578,38 @@ struct Engine {
             const double tj0 = t_judge;
             if (!jq.empty()) judge(cs, pend, acc, qt, jq, jr, jz, jpos, retry);
             t_qr += t_judge - tj0;  // reported with fill + QR
-            for (int32_t q : acc) {
+            // interpolation matrices of the accepted compressions, in batch
+            // order, at running offsets of the level's factor buffer
+            const int64_t na = int64_t(acc.size());
+            auto kept = [&](int64_t a) {
+                const Comp &c = cs[pend[acc[size_t(a)]]];
+                return !(c.judge == 2 && c.rows != ft[acc[size_t(a)]].m);  // rejected: redone with more rows
+            };
+            prefix_sum(na, tofs_run, [&](int64_t a) {
+                const Comp &c = cs[pend[acc[size_t(a)]]];
+                return kept(a) ? int64_t(c.rank) * int64_t(c.cols.size()) : int64_t(0);
+            });
+            // low ranks (almost all: the lane-per-column kernel) first, the
+            // rest after them
+            auto small = [&](int64_t a) { return cs[pend[acc[size_t(a)]]].rank <= INTERP_SMALL_RANK; };
+            prefix_sum(na, it_pos, [&](int64_t a) { return int64_t(kept(a) && small(a)); });
+            prefix_sum(na, it_big, [&](int64_t a) { return int64_t(kept(a) && !small(a)); });
+            const int64_t n_small = it_pos[size_t(na)];
+            it.resize(size_t(n_small + it_big[size_t(na)]));
+            std::atomic<int32_t> amax_rank{0};
+            parallel_for(na, [&](int64_t a) {
+                if (!kept(a)) return;
+                const int32_t q = acc[size_t(a)];
                 Comp &c = cs[pend[q]];
-                if (c.judge == 2 && c.rows != ft[q].m) continue;  // rejected: redone with more rows
-                const int32_t r = c.rank;
-                c.tofs = pool_used;
-                pool_used += int64_t(r) * int64_t(c.cols.size());
-                it.push_back(InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, r, qt[q].pofs});
-                max_rank = std::max(max_rank, r);
-            }
+                c.tofs = pool_used + tofs_run[size_t(a)];
+                const int64_t at = small(a) ? it_pos[size_t(a)] : n_small + it_big[size_t(a)];
+                it[size_t(at)] = InterpTask{qt[q].zofs, c.tofs, qt[q].m, qt[q].n, c.rank, qt[q].pofs};
+                int32_t v = amax_rank.load(std::memory_order_relaxed);
+                while (c.rank > v && !amax_rank.compare_exchange_weak(v, c.rank)) {
+                }
+            }, 4096);
+            pool_used += tofs_run[size_t(na)];
+            const int32_t max_rank = amax_rank.load();
             // skeletons (pivot prefixes) in parallel
             parallel_for(int64_t(np), [&](int64_t q) {
                 Comp &c = cs[pend[q]];
@@ -540,19 +619,11 @@ struct Engine {
             });
             if (!it.empty()) {
                 ensure_pool(pool, pool_cap, pool_used + FACTOR_PAD);
-                // low ranks (almost all) on the lane-per-column kernel, the
-                // rest on the wave-per-column one; tasks of each kind contiguous
-                const auto small_end = std::stable_partition(
-                    it.begin(), it.end(), [](const InterpTask &x) { return x.rank <= INTERP_SMALL_RANK; });
-                const int32_t n_small = int32_t(small_end - it.begin());
-                int32_t big_rank = 0;
-                for (auto p = small_end; p != it.end(); ++p) big_rank = std::max(big_rank, p->rank);
-                (void)max_rank;
                 upload(d_interp, it, ctx.stream);
-                launch_interp_small(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), n_small, d_piv.as<int32_t>(),
-                                    pool.p);
+                launch_interp_small(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>(), int32_t(n_small),
+                                    d_piv.as<int32_t>(), pool.p);
                 launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>() + n_small,
-                              int32_t(it.size()) - n_small, d_piv.as<int32_t>(), pool.p, big_rank);
+                              int32_t(int64_t(it.size()) - n_small), d_piv.as<int32_t>(), pool.p, max_rank);
                 BFG_HIP(hipStreamSynchronize(ctx.stream));
             }
             t_interp += secs(ti, clk::now()) - (t_judge - tj0);

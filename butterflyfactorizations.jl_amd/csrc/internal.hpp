@@ -266,6 +266,7 @@ struct BF {
     bool use_mfma = true;         // RHS across lanes in complex FP32: matrix-core kernel
     int32_t mfma_rows = 16;       // rows of its tiles (16, 32, 48 or 64)
     int32_t rhs_chunk = 0;        // most right-hand sides per pass (0: as device memory allows)
+    int32_t slot_cap_kib = 0;     // most KiB of skeleton vectors per pass (0: 90 % of the free memory)
     // matrix-free last level (forward, one RHS, complex FP32): per observer
     // leaf, its rows and the flat list of P columns (source point, input
     // slot); the entries G(x, y) are computed from the points

@@ -618,6 +618,54 @@ __global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Z, 
     const C<R> *Zc = Z + tk.cofs;
     const int32_t *piv = piv_all + tk.pofs;
     const int mc = tk.mc, ldr = tk.ldr, n = tk.n, r = tk.r;
+    if (n <= 64) {
+        // one column per lane; eight held-out rows at a time, so each
+        // entry of R is loaded once per eight rows
+        constexpr int RBK = 8;
+        const bool col = lane < n;
+        const int k = col ? lane : 0;
+        double en1 = 0.0, zn1 = 0.0;
+        const int pk = piv[k];
+        for (int i0 = 0; i0 < mc; i0 += RBK) {
+            C<double> sv[RBK];
+#pragma unroll
+            for (int b = 0; b < RBK; ++b) {
+                sv[b] = C<double>{0.0, 0.0};
+                if (col && i0 + b < mc) {
+                    const C<R> v = Zc[i0 + b + int64_t(pk) * mc];
+                    sv[b] = C<double>{double(v.x), double(v.y)};
+                    zn1 += sv[b].x * sv[b].x + sv[b].y * sv[b].y;
+                }
+            }
+            for (int q = 0; q < r; ++q) {
+                const C<R> d = Rm[q + int64_t(q) * ldr];
+                const double dr = d.x, di = d.y, inv = 1.0 / (dr * dr + di * di);
+                const C<R> rv = Rm[q + int64_t(k) * ldr];
+                const C<double> rq = {double(rv.x), double(rv.y)};
+#pragma unroll
+                for (int b = 0; b < RBK; ++b) {
+                    const double sx = __shfl(sv[b].x, q, 64), sy = __shfl(sv[b].y, q, 64);
+                    const C<double> w = {(sx * dr + sy * di) * inv, (sy * dr - sx * di) * inv};
+                    if (k > q) sv[b] = csub(sv[b], cmul(w, rq));
+                }
+            }
+            if (col && k >= r)
+#pragma unroll
+                for (int b = 0; b < RBK; ++b)
+                    if (i0 + b < mc) en1 += sv[b].x * sv[b].x + sv[b].y * sv[b].y;
+        }
+        double emax = col && k >= r ? sqrt(en1) : 0.0, zmax = col ? sqrt(zn1) : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            emax = fmax(emax, __shfl_xor(emax, o, 64));
+            zmax = fmax(zmax, __shfl_xor(zmax, o, 64));
+        }
+        if (lane == 0) {
+            out[2 * t] = emax;
+            out[2 * t + 1] = zmax;
+        }
+        return;
+    }
     constexpr int NC = JUDGE_MAX_COLS / 64;
     double en[NC], zn[NC];
 #pragma unroll

@@ -1033,10 +1033,11 @@ static void stage_seq(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, 
     if (bf.rhs_chunk > 0) per = std::min<int64_t>(per, bf.rhs_chunk);
     auto cols_of = [&](int64_t c) { return size_t(lanes ? (c + 15) / 16 * 16 : c); };
     const size_t col_bytes = size_t(std::max<int64_t>(bf.slot_len, 1)) * es;
-    if (cols_of(per) * col_bytes > bf.slots_cap) {
+    const size_t cap = bf.slot_cap_kib > 0 ? size_t(bf.slot_cap_kib) << 10 : SIZE_MAX;
+    if (cols_of(per) * col_bytes > std::min(bf.slots_cap, cap)) {
         size_t free_b = 0, total_b = 0;
         BFG_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t avail = size_t(0.9 * double(free_b + bf.slots_cap));
+        const size_t avail = std::min(cap, size_t(0.9 * double(free_b + bf.slots_cap)));
         if (lanes && cols_of(1) * col_bytes > avail) {
             // not even 16 interleaved columns fit: one column per RHS
             lanes = false;

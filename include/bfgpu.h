@@ -440,7 +440,12 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        instead of reading them;
  *                        0: read the stored P blocks.  Applies to operators
  *                        factorized here with the point kernel (and their
- *                        recompressed or converted copies). */
+ *                        recompressed or converted copies).
+ *   BFGPU_OPT_SLOT_CAP   most KiB the skeleton vectors of one pass over the
+ *                        levels may take (0, default: 90 % of the free
+ *                        device memory); when 16 interleaved columns do not
+ *                        fit, right-hand sides go one column each, in
+ *                        passes of as many as fit. */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -451,7 +456,8 @@ enum {
     BFGPU_OPT_RHS_LANES = 7,
     BFGPU_OPT_MFMA = 8,
     BFGPU_OPT_RHS_CHUNK = 9,
-    BFGPU_OPT_PFREE = 10
+    BFGPU_OPT_PFREE = 10,
+    BFGPU_OPT_SLOT_CAP = 11
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

@@ -206,6 +206,27 @@ def test_rhs_across_lanes_matches_oracle(bfgpu, oracle, ctx):
     op.set_option("rhs_lanes", 16)
     assert rel(op @ Xs, want) < 1e-12
 
+    say("far field sphere N=6000 fp32, 40 RHS: skeleton vectors capped below 16 interleaved columns")
+    op32 = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
+                                  admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
+                                  farfieldonly=True, ctx=ctx)
+    X40 = np.asfortranarray(crandn(rng, 6000, 40).astype(np.complex64))
+    lanes = op32 @ X40                                  # right-hand sides across lanes (matrix cores)
+    col_kib = op32.info().slot_len * 8 / 1024
+    op32.set_option("slot_cap", int(5.5 * col_kib))    # 5 columns fit, 16 interleaved do not
+    assert op32.get_option("slot_cap") == int(5.5 * col_kib)
+    split = op32 @ X40                                  # one column per RHS, passes of at most 5
+    op32.set_option("slot_cap", 0)
+    assert rel(split, lanes) < 2e-6
+    imps32 = [oracle.import_bf(T, T, op32.export(i)) for i in range(op32.n_factorizations())]
+    want32 = np.stack([oracle.matvec_many(imps32, X40[:, j].astype(np.complex128), nthreads=8, rows=6000)
+                       for j in (0, 17, 39)], axis=1)
+    assert rel(split[:, [0, 17, 39]], want32) < 2e-6
+    op32.set_option("slot_cap", 1)                      # not even one column: an error, not a fault
+    with pytest.raises(bfgpu.ArgumentError):
+        op32 @ X40
+    op32.set_option("slot_cap", 0)
+
 
 def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     """Complex FP32 right-hand sides across lanes on the matrix cores
@@ -326,7 +347,20 @@ def test_matrix_free_last_level(bfgpu, oracle, ctx, tmp_path):
     assert rel(on, ref) < 1e-5
     op.set_option("pfree", 1)
     # recompressed operators keep P (and the matrix-free level); files do not carry the points
-    assert op.recompress(1e-4).get_option("pfree") == 1
+    rc = op.recompress(1e-4)
+    assert rc.get_option("pfree") == 1
+    # the recompressed operator's computed-P products against its stored-P
+    # products and against the oracle's product of its exported blocks
+    rc_on, rc_adj_on = rc @ x, rc.adjoint() @ x
+    rc.set_option("pfree", 0)
+    rc_off, rc_adj_off = rc @ x, rc.adjoint() @ x
+    rc.set_option("pfree", 1)
+    assert rel(rc_on, rc_off) < 1e-5 and rel(rc_adj_on, rc_adj_off) < 1e-5
+    rimps = [oracle.import_bf(T, T, rc.export(i)) for i in range(rc.n_factorizations())]
+    x64 = x.astype(np.complex128)
+    assert rel(rc_on, oracle.matvec_many(rimps, x64, rows=6000)) < 1e-5
+    radj = [b.transform(1) for b in rimps]
+    assert rel(rc_adj_on, oracle.matvec_many(radj, x64, rows=6000)) < 1e-5
     path = str(tmp_path / "s.bfgpu")
     op.save(path)
     assert bfgpu.load(path, bt, ctx=ctx).get_option("pfree") == 0
