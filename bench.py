@@ -126,7 +126,9 @@ def kernel_name(d, op, real):
         return f"mfma_rhs_kernel<{outk}, {(nr + 15) // 16}>"
     if variant == 4:
         return f"rhs_kernel<{real}, {outk}, 8>"
-    if variant == 3:
+    if variant == 3 and op != 0:
+        return f"denset_kernel<{real}, {outk}, {nr}, {'true' if (cs >> 8) & 1 else 'false'}>"
+    if variant in (3, 8):  # 8: the forward kernel over the transposed copy
         qw = (4 if (cs >> 24) & 1 else 8) if nr == 1 else (4 if nr == 2 else 2)
         return f"dense_kernel<{real}, {outk}, {nr}, {qw}, {'true' if (cs >> 8) & 1 else 'false'}>"
     if variant == 2:

@@ -552,7 +552,7 @@ class ButterflyFactorization:
         return self
 
     OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7,
-               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11}
+               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11, "tcopy": 12}
 
     @staticmethod
     def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0):
@@ -569,7 +569,12 @@ class ButterflyFactorization:
         see tile_shape), dense (dense shape of the forward tables),
         rhs_lanes (fewest right-hand sides put across lanes), mfma (1:
         FP32 right-hand sides across lanes on the matrix cores), rhs_chunk
-        (most right-hand sides per pass; 0: as device memory allows)."""
+        (most right-hand sides per pass; 0: as device memory allows),
+        pfree (1: compute the last level's point-kernel entries), slot_cap
+        (most KiB of skeleton vectors per pass), tcopy (1: transposed copy
+        of the stored blocks for the adjoint / transpose, built when device
+        memory allows and used where the autotuner finds it faster; 0:
+        release it)."""
         check(lib().bfgpu_bf_set_option(self.handle, self.OPTIONS[key], int(value)))
         return self
 
@@ -579,8 +584,10 @@ class ButterflyFactorization:
         return v.value
 
     def autotune(self, nrhs=1, iters=10):
-        """Pick the fastest forward kernel variant of every level for `nrhs`
-        right-hand sides; returns (per-level ms, per-level variant)."""
+        """Pick the fastest kernel variant of every level for `nrhs`
+        right-hand sides, forward and adjoint / transpose (building the
+        transposed copy first when the tcopy option and device memory
+        allow); returns the forward (per-level ms, per-level variant)."""
         n = self.info().max_stages
         ms = (C.c_double * n)()
         var = (C.c_int32 * n)()

@@ -636,6 +636,10 @@ extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) 
     BFG_REQUIRE(h && vals, "bf_set_tuning: null argument");
     BF &bf = h->b;
     BFG_REQUIRE(n == 8 * bf.n_stage_launch, "bf_set_tuning: wrong length (levels differ)");
+    // the transposed copy, where a transposed table asks for it
+    for (int32_t L = 0; L < bf.n_stage_launch; ++L)
+        for (int q = 2; q < 4; ++q)
+            if (vals[8 * L + 2 * q] == 8) ensure_tcopy(bf);
     // validate everything before changing anything
     for (int pass = 0; pass < 2; ++pass)
         for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
@@ -647,9 +651,13 @@ extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) 
                     continue;
                 }
                 const TileShape sh = unpack_tile(p);
-                BFG_REQUIRE(v >= 0 && v <= 3 && !(q >= 2 && v == 1) && p >= 0 && p < (1 << 28) &&
-                                tile_ok(sh) && !(v == 3 && sh.rp_shift > 6),
+                BFG_REQUIRE(((v >= 0 && v <= 3) || (v == 8 && q >= 2)) && !(q >= 2 && v == 1) && p >= 0 &&
+                                p < (1 << 28) && tile_ok(sh) && !(v >= 3 && sh.g_shift != 0),
                             "bf_set_tuning: bad variant or shape");
+                BFG_REQUIRE(v != 8 || (bf.tcopy_built && (bf.factor_elems[L] == 0 ||
+                                                          (L < int32_t(bf.factors_t.size()) && bf.factors_t[L].p))),
+                            "bf_set_tuning: the transposed copy of a level is not available (tcopy off, or "
+                            "device memory short)");
                 if (pass == 1) {
                     t[q]->variant = v;
                     t[q]->tile = sh;
@@ -740,6 +748,17 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         BFG_REQUIRE(value >= 0, "bf_set_option: slot_cap is a number of KiB (0: as device memory allows)");
         bf.slot_cap_kib = value;
         break;
+    case BFGPU_OPT_TCOPY:
+        BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: tcopy is 0 or 1");
+        if (value == 0) {
+            release_tcopy(bf);
+            bf.tcopy_mode = 0;
+        } else {
+            bf.tcopy_mode = 1;
+            bf.tcopy_tried = false;
+            ensure_tcopy(bf);
+        }
+        break;
     case BFGPU_OPT_PFREE:
         BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: pfree is 0 or 1");
         bf.use_pfree = value != 0;
@@ -789,6 +808,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_MFMA: *value = bf.use_mfma ? bf.mfma_rows : 0; break;
     case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
     case BFGPU_OPT_SLOT_CAP: *value = bf.slot_cap_kib; break;
+    case BFGPU_OPT_TCOPY: *value = bf.tcopy_built ? 1 : 0; break;
     case BFGPU_OPT_PFREE: *value = bf.use_pfree && bf.pfree_ok ? 1 : 0; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }

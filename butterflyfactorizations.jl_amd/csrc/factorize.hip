@@ -212,16 +212,25 @@ struct Engine {
     }
 
     // Shrink a level's factor buffer to what it holds (the growth policy
-    // below leaves up to a third of it unused).
+    // below leaves up to a third of it unused) and zero its tail pad: the
+    // matvec kernels clamp their loads into the blocks and may read the pad
+    // entry after the last one, which then meets a zero weight and must be
+    // finite.
     void trim_pool(DevBuf &pool, int64_t &cap_elems, int64_t used) {
         const int64_t need = used + FACTOR_PAD;
-        if (cap_elems <= need + need / 32) return;
-        DevBuf nb;
-        nb.alloc(size_t(need) * es);
-        BFG_HIP(hipMemcpyAsync(nb.p, pool.p, size_t(need) * es, hipMemcpyDeviceToDevice, ctx.stream));
+        if (used == 0 && cap_elems == 0) return;  // no blocks on this level
+        BFG_REQUIRE(cap_elems >= need, "factorize: factor buffer without room for its tail pad");
+        if (cap_elems > need + need / 32) {
+            DevBuf nb;
+            nb.alloc(size_t(need) * es);
+            BFG_HIP(hipMemcpyAsync(nb.p, pool.p, size_t(used) * es, hipMemcpyDeviceToDevice, ctx.stream));
+            BFG_HIP(hipStreamSynchronize(ctx.stream));
+            pool = std::move(nb);
+            cap_elems = need;
+        }
+        BFG_HIP(hipMemsetAsync(static_cast<char *>(pool.p) + size_t(used) * es, 0, size_t(FACTOR_PAD) * es,
+                               ctx.stream));
         BFG_HIP(hipStreamSynchronize(ctx.stream));
-        pool = std::move(nb);
-        cap_elems = need;
     }
 
     void release() {

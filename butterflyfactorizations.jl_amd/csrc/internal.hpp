@@ -173,12 +173,13 @@ enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
 // Launch shape of the tiled stage kernel: 2^rp_shift output entries times
 // 2^g_shift lane groups per task; groups split the inner index (csplit 0)
 // or the contributions (csplit 1); persistent grid of wg_per_cu
-// workgroups per CU.
+// workgroups per CU.  For the transposed dense kernel csplit 1 selects
+// grouped rows (lanes on the rows of the block columns).
 struct TileShape {
     int8_t rp_shift = 3, g_shift = 0, csplit = 0;
     int16_t wg_per_cu = 8;
     int8_t xcd = 0;  // 1: each XCD walks one contiguous range of tasks
-    int8_t qw4 = 0;  // dense kernel, one RHS: 4 loads per lane and pass (else 8)
+    int8_t qw4 = 0;  // dense kernels, one RHS: 4 loads per lane and pass (else 8)
 };
 inline int64_t pack_tile(const TileShape &t) {
     return int64_t(t.rp_shift) | int64_t(t.g_shift) << 4 | int64_t(t.csplit) << 8 | int64_t(t.wg_per_cu) << 12 |
@@ -210,12 +211,17 @@ struct StageTable {
     // kernel: 0 workgroup per task, 1 wavefront per task (forward only),
     // 2 tiled (several tasks per wavefront, persistent), 3 dense (forward:
     // lanes packed to each block's row count; transposed: a lane per output
-    // column; tile.rp_shift is the log2 of the lanes per task)
+    // column; tile.rp_shift is the log2 of the lanes per task), 8
+    // (transposed only) the forward dense kernel over the transposed copy of
+    // the blocks (BF::factors_t) with tasks_sw / contribs_sw
     int32_t variant = 0;
     TileShape tile;
     int32_t in_lo = 0, in_hi = 0;      // IN_GATHER: value range read through the permutation
     int32_t max_nc = 0;                // most contributions of one item
     DevBuf items, contribs, tasks, tasks_c;
+    // transposed tables over the transposed copy: tasks_c / contribs with
+    // every block's rows and columns swapped (built with the copy)
+    DevBuf tasks_sw, contribs_sw;
     hvec<MvItem> h_items;
     hvec<MvContrib> h_contribs;
     hvec<MvTask> h_tasks;
@@ -275,6 +281,14 @@ struct BF {
     double pf_k = 0.0;            // wavenumber of the point kernel
     DevBuf pf_items, pf_col, pf_in, pf_row;  // matrix-free last level tables
     bool pfree_t_ok = false;                 // every P column's skeleton slot written once
+    // transposed copy of the stored blocks for the adjoint / transpose
+    // (variant 8 of the transposed tables): per level, each block n x m
+    // column-major at its own offset; levels whose transposed product is
+    // matrix-free are not copied.  Mode 0 off, 1 built on first use when
+    // device memory allows.
+    std::vector<DevBuf> factors_t;
+    int32_t tcopy_mode = 1;
+    bool tcopy_built = false, tcopy_tried = false;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<int64_t> graph_key;  // op, nrhs, x, ldx, y, ldy, alpha, beta, stream
     ~BF();
@@ -413,9 +427,21 @@ void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const vo
 void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
                  int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
+// Forward dense kernel over table t; with swapped true it reads the
+// swapped task tables (t.tasks_sw, t.contribs_sw) and `factors` is the
+// transposed copy; conj conjugates the factor entries.
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
-                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);
+                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu,
+                  bool swapped = false, int conj = 0);
+// Block transposition: for block q, dst[fofs + i n + k] = src[fofs + k m + i]
+// (int4 {fofs lo, fofs hi, m, n} per block).
+void launch_transpose_blocks(hipStream_t s, int prec, const int4 *blocks, int64_t n_blocks, const void *src,
+                             void *dst);
+// Transposed copy of the stored blocks (built once when mode and device
+// memory allow; true when built) and its release.
+bool ensure_tcopy(BF &bf);
+void release_tcopy(BF &bf);
 void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const void *factors, void *slots,
                    int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                    int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu);

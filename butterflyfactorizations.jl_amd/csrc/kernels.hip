@@ -1168,7 +1168,7 @@ __global__ __launch_bounds__(256) void dense_kernel(
     const MvTaskC *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
     const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
     C<R> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out, C<R> alpha, C<R> beta,
-    int fuse_beta, int nrhs, int lpt_shift, int xcd_map) {
+    int fuse_beta, int nrhs, int lpt_shift, int xcd_map, int conj) {
     const int lane = threadIdx.x & 63;
     const int LPT = 1 << lpt_shift;
     const int tpw = 64 >> lpt_shift;
@@ -1251,6 +1251,10 @@ __global__ __launch_bounds__(256) void dense_kernel(
                     const bool ok = act && kc < w && k < K;
                     __builtin_memcpy(&a[q], __builtin_assume_aligned(pa + min(k, last) * m, sizeof(C<R>)),
                                      sizeof(Pair));
+                    if (conj) {  // conjugated entries (adjoint over a transposed copy)
+                        a[q].a.y = -a[q].a.y;
+                        a[q].b.y = -a[q].b.y;
+                    }
                     const int src = xbase + min(kc, w - 1);
 #pragma unroll
                     for (int rr = 0; rr < NR; ++rr) {
@@ -1346,11 +1350,14 @@ __global__ __launch_bounds__(256) void dense_kernel(
 // cross-lane reduction is needed; the lane walks its block column, which
 // is contiguous, with 16-byte loads of two entries.  The inputs of an inner
 // chunk of LPT entries are loaded once, one lane per entry, and handed to
-// every lane of the task by cross-lane permutes.  Loads carry no predicate
-// (column clamped into the block; the entry after the last row of a column
-// reads the next column or the buffer's tail pad and meets a zeroed input).
+// every lane of the task by cross-lane permutes (XL false), or stored by
+// those lanes into the wavefront's own LDS rows and read back by every lane
+// of the task as 16-byte broadcasts, two entries per read (XL true: one LDS
+// read where the permutes take four).  Loads carry no predicate (column
+// clamped into the block; the entry after the last row of a column reads
+// the next column or the buffer's tail pad and meets a zeroed input).
 // ---------------------------------------------------------------------------
-template <typename R, int OUTK, int NR>
+template <typename R, int OUTK, int NR, bool XL>
 __global__ __launch_bounds__(256) void denset_kernel(
     const MvTaskC *__restrict__ tasks, int32_t n_tasks, const MvContrib *__restrict__ contribs,
     const C<R> *__restrict__ F, C<R> *__restrict__ slots, int64_t slot_len, int64_t in_shift,
@@ -1386,6 +1393,9 @@ __global__ __launch_bounds__(256) void denset_kernel(
         return ci < tk.c_end ? contribs[ci] : none;
     };
     struct Pair { C<R> a, b; };
+    // inputs of the current chunk, per wavefront: [rhs][lane] (XL only)
+    __shared__ __align__(32) C<R> xlds[XL ? 4 * 64 * NR : 1];
+    C<R> *xrow = xlds + (XL ? (threadIdx.x >> 6) * 64 * NR : 0);
 
     uint2 tv = load_task_group(tasks, n_tasks, grp, tpw, lane);
     while (grp < grp_hi) {
@@ -1420,6 +1430,15 @@ __global__ __launch_bounds__(256) void denset_kernel(
                             const C<R> v = px[int64_t(rr < nr ? rr : nr - 1) * slot_len + i0 + min(l, iw - 1)];
                             xs[rr] = C<R>{l < iw ? v.x : R(0), l < iw ? v.y : R(0)};
                         }
+                        if (XL) {
+                            // the task's LDS row holds this chunk's inputs (zero past iw);
+                            // earlier reads of the row were issued before these stores
+#pragma unroll
+                            for (int rr = 0; rr < NR; ++rr) xrow[rr * 64 + lane] = xs[rr];
+                            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                            __builtin_amdgcn_wave_barrier();
+                            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                        }
                         for (int ii = 0; ii < iw; ii += 2 * UP) {
                             Pair p[UP];
 #pragma unroll
@@ -1431,6 +1450,19 @@ __global__ __launch_bounds__(256) void denset_kernel(
 #pragma unroll
                             for (int u = 0; u < UP; ++u) {
                                 const int e0 = ii + 2 * u, e1 = e0 + 1;
+                                if (XL) {
+                                    // entries e0 < iw and e1 <= LPT - 1 of the task's row (e0
+                                    // even; the row is zero from iw on); steps past iw add nothing
+                                    if (e0 >= iw) continue;
+#pragma unroll
+                                    for (int rr = 0; rr < NR; ++rr) {
+                                        Pair xp;
+                                        __builtin_memcpy(&xp, &xrow[rr * 64 + base + e0], sizeof(Pair));
+                                        cfma(acc[rr], C<R>{p[u].a.x, p[u].a.y * csign}, xp.a);
+                                        cfma(acc[rr], C<R>{p[u].b.x, p[u].b.y * csign}, xp.b);
+                                    }
+                                    continue;
+                                }
                                 const int s0 = base + min(e0, iw - 1), s1 = base + min(e1, iw - 1);
 #pragma unroll
                                 for (int rr = 0; rr < NR; ++rr) {
@@ -2127,7 +2159,8 @@ static void launch_tile_t(hipStream_t s, const StageTable &t, const void *factor
 template <typename R, int NR, int QW>
 static void launch_dense_t(hipStream_t s, const StageTable &t, const void *factors, void *slots,
                            int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
-                           int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu) {
+                           int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu,
+                           bool swapped, int conj) {
     C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
     const int lpt = t.tile.rp_shift;
     BFG_REQUIRE(tile_ok(t.tile) && t.tile.g_shift == 0, "matvec: bad dense shape");
@@ -2138,11 +2171,12 @@ static void launch_dense_t(hipStream_t s, const StageTable &t, const void *facto
     auto F = (const C<R> *)factors;
     auto S = (C<R> *)slots;
     auto Y = (C<R> *)y;
-    auto T = t.tasks_c.as<const MvTaskC>();
-    auto P = t.contribs.as<const MvContrib>();
+    auto T = (swapped ? t.tasks_sw : t.tasks_c).as<const MvTaskC>();
+    auto P = (swapped ? t.contribs_sw : t.contribs).as<const MvContrib>();
+    BFG_REQUIRE(T != nullptr && P != nullptr, "matvec: dense tables missing");
 #define BFG_DENSE(OUTK, CS)                                                                              \
     hipLaunchKernelGGL((dense_kernel<R, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S,  \
-                       slot_len, in_shift, Y, ldy, perm_out, a, b, int(fuse), int(nrhs), lpt, int(t.tile.xcd))
+                       slot_len, in_shift, Y, ldy, perm_out, a, b, int(fuse), int(nrhs), lpt, int(t.tile.xcd), conj)
     if (t.out_kind == OUT_SLOT) {
         if (t.tile.csplit) BFG_DENSE(OUT_SLOT, true);
         else BFG_DENSE(OUT_SLOT, false);
@@ -2157,20 +2191,21 @@ static void launch_dense_t(hipStream_t s, const StageTable &t, const void *facto
 template <typename R>
 static void launch_dense_nr(hipStream_t s, const StageTable &t, const void *factors, void *slots,
                             int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
-                            int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu) {
+                            int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu,
+                            bool sw, int conj) {
     if (nrhs == 1) {
-        if (t.tile.qw4) launch_dense_t<R, 1, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
-        else launch_dense_t<R, 1, 8>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+        if (t.tile.qw4) launch_dense_t<R, 1, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
+        else launch_dense_t<R, 1, 8>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
     } else if (nrhs == 2) {
-        launch_dense_t<R, 2, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+        launch_dense_t<R, 2, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
     } else if (nrhs <= 4) {
-        launch_dense_t<R, 4, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+        launch_dense_t<R, 4, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
     } else {
-        launch_dense_t<R, 8, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu);
+        launch_dense_t<R, 8, 2>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
     }
 }
 
-template <typename R, int NR>
+template <typename R, int NR, bool XL>
 static void launch_denset_t(hipStream_t s, const StageTable &t, const void *factors, void *slots,
                             int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                             int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu, int conj) {
@@ -2184,7 +2219,7 @@ static void launch_denset_t(hipStream_t s, const StageTable &t, const void *fact
     auto T = t.tasks_c.as<const MvTaskC>();
     auto P = t.contribs.as<const MvContrib>();
 #define BFG_DENSET(OUTK)                                                                                    \
-    hipLaunchKernelGGL((denset_kernel<R, OUTK, NR>), grid, dim3(256), 0, s, T, t.n_items, P, (const C<R> *)factors, \
+    hipLaunchKernelGGL((denset_kernel<R, OUTK, NR, XL>), grid, dim3(256), 0, s, T, t.n_items, P, (const C<R> *)factors, \
                        (C<R> *)slots, slot_len, in_shift, (C<R> *)y, ldy, perm_out, a, b, int(fuse), int(nrhs), lpt,  \
                        int(t.tile.xcd), conj)
     if (t.out_kind == OUT_SLOT) BFG_DENSET(OUT_SLOT);
@@ -2199,8 +2234,15 @@ void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const v
     if (t.n_items == 0 || nrhs == 0) return;
     BFG_REQUIRE(nrhs < 65536 * 8, "matvec: nrhs too large");
     const int conj = op == BFGPU_OP_C ? 1 : 0;
-#define BFG_DT(RT, NRv) \
-    launch_denset_t<RT, NRv>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu, conj)
+#define BFG_DT(RT, NRv)                                                                                    \
+    do {                                                                                                   \
+        if (t.tile.csplit)                                                                                 \
+            launch_denset_t<RT, NRv, true>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, \
+                                           alpha, beta, fuse_beta, n_cu, conj);                           \
+        else                                                                                               \
+            launch_denset_t<RT, NRv, false>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, \
+                                            alpha, beta, fuse_beta, n_cu, conj);                          \
+    } while (0)
     if (prec == BFGPU_C128) {
         if (nrhs == 1) BFG_DT(double, 1);
         else if (nrhs == 2) BFG_DT(double, 2);
@@ -2217,13 +2259,48 @@ void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const v
 
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
-                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu) {
+                  int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu,
+                  bool swapped, int conj) {
     if (t.n_items == 0 || nrhs == 0) return;
     BFG_REQUIRE(nrhs < 65536 * 8, "matvec: nrhs too large");
     if (prec == BFGPU_C128)
-        launch_dense_nr<double>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu);
+        launch_dense_nr<double>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta,
+                                fuse_beta, n_cu, swapped, conj);
     else
-        launch_dense_nr<float>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse_beta, n_cu);
+        launch_dense_nr<float>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta,
+                               fuse_beta, n_cu, swapped, conj);
+}
+
+// ---------------------------------------------------------------------------
+// Block transposition for the transposed copy: a workgroup per block, entries
+// read in column-major order (coalesced) and written transposed.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void transpose_blocks_kernel(const int4 *__restrict__ blocks, int64_t n_blocks,
+                                                               const C<R> *__restrict__ src, C<R> *__restrict__ dst) {
+    for (int64_t q = blockIdx.x; q < n_blocks; q += gridDim.x) {
+        const int4 b = blocks[q];
+        const int64_t fofs = int64_t((uint64_t(uint32_t(b.y)) << 32) | uint64_t(uint32_t(b.x)));
+        const int m = b.z, n = b.w;
+        const int64_t total = int64_t(m) * n;
+        for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
+            const int k = int(e / m), i = int(e - int64_t(k) * m);
+            dst[fofs + int64_t(i) * n + k] = src[fofs + e];
+        }
+    }
+}
+
+void launch_transpose_blocks(hipStream_t s, int prec, const int4 *blocks, int64_t n_blocks, const void *src,
+                             void *dst) {
+    if (n_blocks <= 0) return;
+    dim3 grid(unsigned(std::min<int64_t>(n_blocks, int64_t(1) << 20)));
+    if (prec == BFGPU_C128)
+        hipLaunchKernelGGL(transpose_blocks_kernel<double>, grid, dim3(256), 0, s, blocks, n_blocks,
+                           (const C<double> *)src, (C<double> *)dst);
+    else
+        hipLaunchKernelGGL(transpose_blocks_kernel<float>, grid, dim3(256), 0, s, blocks, n_blocks,
+                           (const C<float> *)src, (C<float> *)dst);
+    BFG_HIP(hipGetLastError());
 }
 
 template <typename R, int OP>

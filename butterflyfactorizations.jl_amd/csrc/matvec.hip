@@ -1003,6 +1003,12 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
         if (t.variant == 3 && fwd) {
             launch_dense(s, bf.precision, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout, nrhs,
                          alpha, beta, fuse, bf.ctx->n_cu);
+        } else if (t.variant == 8) {  // forward dense kernel over the transposed copy
+            const void *ft = L < int32_t(bf.factors_t.size()) ? bf.factors_t[L].p : nullptr;
+            BFG_REQUIRE(!fwd && bf.tcopy_built && (ft || bf.factor_elems[L] == 0),
+                        "matvec: transposed copy of the blocks missing");
+            launch_dense(s, bf.precision, t, ft ? ft : bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
+                         nrhs, alpha, beta, fuse, bf.ctx->n_cu, true, op == BFGPU_OP_C ? 1 : 0);
         } else if (t.variant == 3) {
             launch_denset(s, bf.precision, op, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
                           nrhs, alpha, beta, fuse, bf.ctx->n_cu);
@@ -1396,6 +1402,93 @@ void check_sweep_error(BF &bf) {
 
 namespace bfg {
 
+// Transposed copy of the stored blocks: every non-identity contribution of a
+// level's transposed table (and of its partial-sum table) is one block,
+// transposed in place of its own offset in the copy; the swapped task
+// tables read each block as n x m.  The level whose transposed product is
+// matrix-free is not copied.  Built once, when device memory holds the copy
+// with a tenth and 1 GiB to spare.
+bool ensure_tcopy(BF &bf) {
+    if (bf.tcopy_built) return true;
+    if (bf.tcopy_mode == 0 || bf.tcopy_tried) return false;
+    bf.tcopy_tried = true;
+    const int32_t nL = bf.n_stage_launch, Lmax = nL - 1;
+    const size_t es = bf.precision == BFGPU_C128 ? 16 : 8;
+    auto copied = [&](int32_t L) {
+        return L < int32_t(bf.factor_elems.size()) && bf.factor_elems[L] > 0 && !(L == Lmax && bf.pfree_t_ok);
+    };
+    size_t bytes = 0;
+    for (int32_t L = 0; L < nL; ++L)
+        if (copied(L)) bytes += size_t(bf.factor_elems[L] + FACTOR_PAD) * es;
+    if (bytes == 0) return false;
+    size_t fr = 0, tot = 0;
+    BFG_HIP(hipMemGetInfo(&fr, &tot));
+    if (bytes + bytes / 10 + (size_t(1) << 30) > fr) return false;
+    hipStream_t s = bf.ctx->stream;
+    std::vector<DevBuf> ft(static_cast<size_t>(nL));
+    DevBuf d_blk;
+    for (int32_t L = 0; L < nL; ++L) {
+        if (!copied(L)) continue;
+        ft[L].alloc(size_t(bf.factor_elems[L] + FACTOR_PAD) * es);
+        BFG_HIP(hipMemsetAsync(static_cast<char *>(ft[L].p) + size_t(bf.factor_elems[L]) * es, 0,
+                               size_t(FACTOR_PAD) * es, s));
+        hvec<int4> blk;
+        for (const StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()})
+            if (u)
+                for (const MvContrib &c : u->h_contribs)
+                    if (c.fofs >= 0 && int64_t(c.m) * c.n > 0) {
+                        BFG_REQUIRE(c.fofs + int64_t(c.m) * c.n <= bf.factor_elems[L],
+                                    "transposed copy: block outside its level's factors");
+                        blk.push_back(int4{int32_t(uint32_t(uint64_t(c.fofs))),
+                                           int32_t(uint32_t(uint64_t(c.fofs) >> 32)), c.m, c.n});
+                    }
+        if (blk.empty()) continue;
+        d_blk.reserve(blk.size() * sizeof(int4));
+        BFG_HIP(hipMemcpyAsync(d_blk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        launch_transpose_blocks(s, bf.precision, d_blk.as<const int4>(), int64_t(blk.size()), bf.factors[L].p,
+                                ft[L].p);
+        BFG_HIP(hipStreamSynchronize(s));
+    }
+    for (int32_t L = 0; L < nL; ++L)
+        for (StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()}) {
+            if (!u) continue;
+            hvec<MvTaskC> tc(u->h_tasks.size());
+            parallel_for(int64_t(tc.size()), [&](int64_t i) {
+                const MvTask &t = u->h_tasks[i];
+                tc[i] = MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.n, t.m, t.c_begin, t.c_end};
+            }, 4096);
+            hvec<MvContrib> cc(u->h_contribs.size());
+            parallel_for(int64_t(cc.size()), [&](int64_t i) {
+                const MvContrib &c = u->h_contribs[i];
+                cc[i] = MvContrib{c.fofs, c.n, c.m, c.in_ofs, c.pad};
+            }, 4096);
+            upload_tab(u->tasks_sw, tc);
+            upload_tab(u->contribs_sw, cc);
+        }
+    bf.factors_t.swap(ft);
+    bf.tcopy_built = true;
+    release_graph(bf);
+    return true;
+}
+
+void release_tcopy(BF &bf) {
+    for (int32_t L = 0; L < int32_t(bf.tr.size()); ++L)
+        for (StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()}) {
+            if (!u) continue;
+            if (u->variant == 8) {  // back to a lane per output column, same lanes per task
+                u->variant = 3;
+                u->tile.csplit = 0;
+                u->tile.qw4 = 0;
+            }
+            u->tasks_sw = DevBuf();
+            u->contribs_sw = DevBuf();
+        }
+    bf.factors_t.clear();
+    bf.tcopy_built = false;
+    bf.tcopy_tried = false;
+    release_graph(bf);
+}
+
 // Launch-shape autotuner: for every launch table of both directions
 // (forward, and the adjoint whose tables the transpose shares), time each
 // kernel variant and tiled shape on device buffers and keep the fastest;
@@ -1419,6 +1512,7 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
     }
     BFG_HIP(hipMemsetAsync(bf.slots.p, 0, need, s));
     release_graph(bf);
+    ensure_tcopy(bf);  // when the mode and device memory allow
     const int32_t nL = bf.n_stage_launch;
     static const double one[2] = {1.0, 0.0}, zero[2] = {0.0, 0.0};
     hipEvent_t e0, e1;
@@ -1460,16 +1554,20 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                     sh.csplit = int8_t(cs);
                     consider(2, sh);
                 }
-        if (op != BFGPU_OP_N)  // transposed dense: a lane per output column
-            for (int lp = 2; lp <= 6; ++lp) {
-                TileShape sh = t0;
-                sh.rp_shift = int8_t(lp);
-                sh.g_shift = 0;
-                sh.csplit = 0;
-                sh.qw4 = 0;
-                consider(3, sh);
-            }
-        if (op == BFGPU_OP_N)
+        if (op != BFGPU_OP_N)  // transposed dense: a lane per output column, inputs by permutes or LDS
+            for (int lp = 2; lp <= 6; ++lp)
+                for (int xl = 0; xl <= 1; ++xl) {
+                    TileShape sh = t0;
+                    sh.rp_shift = int8_t(lp);
+                    sh.g_shift = 0;
+                    sh.csplit = int8_t(xl);
+                    sh.qw4 = 0;
+                    consider(3, sh);
+                }
+        // the forward dense kernel over the transposed copy of the blocks
+        const bool tc = op != BFGPU_OP_N && bf.tcopy_built &&
+                        (bf.factor_elems[L] == 0 || (L < int32_t(bf.factors_t.size()) && bf.factors_t[L].p));
+        if (op == BFGPU_OP_N || tc)
             for (int lp = 2; lp <= 6; ++lp)
                 for (int cs = 0; cs <= (t.max_nc >= 2 ? 1 : 0); ++cs)
                     for (int q4 = 0; q4 <= 1; ++q4) {
@@ -1478,7 +1576,7 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                         sh.g_shift = 0;
                         sh.csplit = int8_t(cs);
                         sh.qw4 = int8_t(q4);
-                        consider(3, sh);
+                        consider(op == BFGPU_OP_N ? 3 : 8, sh);
                     }
         if (bv >= 2) {  // the other task-to-XCD mapping, then other occupancies, for the best shape
             TileShape sh = bs;

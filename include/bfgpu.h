@@ -398,7 +398,9 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        1 wavefront per task (forward; transposed tables
  *                        take 0), 2 tiled (several tasks per wavefront),
  *                        3 dense (forward: lanes packed to each block's row
- *                        count; transposed: a lane per output column); get:
+ *                        count; transposed: a lane per output column, or
+ *                        with csplit 1 lanes on the rows of the block
+ *                        columns and a group per column); get:
  *                        the common value, or -1 when levels differ
  *   BFGPU_OPT_TILE       tiled shape of every table, packed as rp_shift |
  *                        g_shift << 4 | csplit << 8 | wg_per_cu << 12 |
@@ -413,8 +415,8 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *   BFGPU_OPT_DENSE      dense shape of every table, packed as for
  *                        BFGPU_OPT_TILE with rp_shift = log2(lanes per
  *                        task) and g_shift = 0 (the transposed tables use
- *                        only the lanes per task and the XCD mapping);
- *                        selects variant 3
+ *                        the lanes per task, the XCD mapping, csplit 1 for
+ *                        grouped rows and qw4 with it); selects variant 3
  *   BFGPU_OPT_RHS_LANES  forward matvecs with at least this many right-hand
  *                        sides (default 32; 0 = never) put the right-hand
  *                        sides across lanes, 64 per sweep, with the factor
@@ -445,7 +447,15 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        levels may take (0, default: 90 % of the free
  *                        device memory); when 16 interleaved columns do not
  *                        fit, right-hand sides go one column each, in
- *                        passes of as many as fit. */
+ *                        passes of as many as fit.
+ *   BFGPU_OPT_TCOPY      1 (default): the autotuner may run adjoint and
+ *                        transpose levels with the forward dense kernel over
+ *                        a transposed copy of the stored blocks (tuning
+ *                        variant 8 of a transposed table), built on the
+ *                        first autotune (or on this setting) when device
+ *                        memory holds it with a tenth and 1 GiB to spare;
+ *                        the matrix-free level is not copied; 0 releases
+ *                        the copy.  get: 1 while the copy exists. */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -457,7 +467,8 @@ enum {
     BFGPU_OPT_MFMA = 8,
     BFGPU_OPT_RHS_CHUNK = 9,
     BFGPU_OPT_PFREE = 10,
-    BFGPU_OPT_SLOT_CAP = 11
+    BFGPU_OPT_SLOT_CAP = 11,
+    BFGPU_OPT_TCOPY = 12
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

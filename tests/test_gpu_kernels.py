@@ -2,8 +2,9 @@
 
 Every launch table can run on the workgroup-per-task kernel (variant 0), the
 wavefront-per-task kernel (variant 1, forward), the tiled kernel (variant 2)
-or the dense kernel (variant 3: forward packed rows, transposed a lane per
-output column) in any legal shape; observer-leaf sums over many
+or the dense kernel (variant 3: forward packed rows; transposed a lane per
+output column) in any legal shape, and the transposed tables on the forward
+dense kernel over a transposed copy of the blocks (variant 8); observer-leaf sums over many
 factorizations are split into partial sums plus a combine pass.  Bar
 (written in each test): every variant and shape gives the oracle's FP64
 matvec of the same factorization to 1e-12 relative, in all three
@@ -53,12 +54,27 @@ def _shapes(bfgpu):
 
 
 def _dense_shapes(bfgpu):
-    """Dense forward shapes: lanes per task 4..64, both group modes, 4 or
-    8 loads per pass, both task-to-XCD mappings."""
+    """Dense shapes: lanes per task 4..64, both group modes (transposed
+    lane-per-column kernel: inputs by permutes or from LDS), 4 or 8 loads
+    per pass, both task-to-XCD mappings."""
     sh = bfgpu.ButterflyFactorization.tile_shape
     out = [sh(lp, 0, cs, 8, 0, q4) for lp in range(2, 7) for cs in (0, 1) for q4 in (0, 1)]
     out += [sh(4, 0, 0, 8, 1, 0), sh(5, 0, 1, 2, 1, 1), sh(6, 0, 0, 1, 1, 0)]
     return out
+
+
+def _force_tcopy(g, sh, skip_last=False):
+    """Every transposed table (and partial-sum table) on variant 8, the
+    forward dense kernel over the transposed copy, with dense shape sh;
+    skip_last leaves the last level (matrix-free, not copied) alone."""
+    v = g.tuning().copy()
+    n = len(v) // 8
+    for L in range(n - 1 if skip_last else n):
+        for q in (2, 3):
+            if v[8 * L + 2 * q] >= 0:
+                v[8 * L + 2 * q] = 8
+                v[8 * L + 2 * q + 1] = sh
+    g.set_tuning(v)
 
 
 def _check_all_directions(g, refs, X):
@@ -109,6 +125,20 @@ def test_variants_two_squares_match_oracle(bfgpu, oracle, ctx):
         g.set_option("dense", sh)
         assert g.get_option("dense") == sh
         _check_all_directions(g, refs, X)
+    # adjoint / transpose through the transposed copy of the blocks
+    g.set_option("tcopy", 1)
+    assert g.get_option("tcopy") == 1
+    for sh in _dense_shapes(bfgpu)[::3]:
+        say("transposed copy, dense", hex(sh))
+        _force_tcopy(g, sh)
+        assert all(g.tuning()[8 * L + 4] == 8 for L in range(len(g)))
+        _check_all_directions(g, refs, X)
+    g.set_option("tcopy", 0)  # released: the transposed tables go back to the lazy kernels
+    assert g.get_option("tcopy") == 0 and all(g.tuning()[8 * L + 4] != 8 for L in range(len(g)))
+    _check_all_directions(g, refs, X)
+    with pytest.raises(bfgpu.ArgumentError):  # no copy to run on
+        _force_tcopy(g, bfgpu.ButterflyFactorization.tile_shape(4, 0))
+    g.set_option("tcopy", 1)
     # the autotuner's choice (both directions) keeps the product
     ms, var = g.autotune(nrhs=1, iters=2)
     assert len(ms) == len(var) == len(g)
@@ -158,6 +188,10 @@ def test_variants_sphere_partial_sums_match_oracle(bfgpu, oracle, ctx):
     for sh in _dense_shapes(bfgpu)[::2]:
         say("dense", hex(sh))
         op.set_option("dense", sh)
+        _check_all_directions(op, refs, X)
+    for sh in _dense_shapes(bfgpu)[1::5]:  # partial-sum tables over the transposed copy too
+        say("transposed copy, dense", hex(sh))
+        _force_tcopy(op, sh)
         _check_all_directions(op, refs, X)
     op.autotune(nrhs=1, iters=2)
     _check_all_directions(op, refs, X)
@@ -341,6 +375,18 @@ def test_matrix_free_last_level(bfgpu, oracle, ctx, tmp_path):
     assert rel(adj_ab, op.adjoint().mul(y0.copy(), x, 0.5 - 1j, 2.0)) < 1e-5
     # the point kernel is symmetric: transpose(F) x = F x up to the compression
     assert rel(tr_on, on) < 1e-3
+    # the transposed copy covers the stored levels; the matrix-free level is
+    # not copied and stays on its own kernel
+    op.set_option("pfree", 1)
+    op.set_option("tcopy", 1)
+    assert op.get_option("tcopy") == 1
+    with pytest.raises(bfgpu.ArgumentError):
+        _force_tcopy(op, bfgpu.ButterflyFactorization.tile_shape(5, 0))
+    _force_tcopy(op, bfgpu.ButterflyFactorization.tile_shape(5, 0), skip_last=True)
+    assert rel(op.adjoint() @ x, adj_on) < 1e-5 and rel(op.transpose() @ x, tr_on) < 1e-5
+    assert rel(op.adjoint().mul(y0.copy(), x, 0.5 - 1j, 2.0), adj_ab) < 1e-5
+    op.set_option("tcopy", 0)
+    op.set_option("pfree", 0)
     # the FP64 form of the same operator through the oracle's matvec
     imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
     ref = oracle.matvec_many(imps, x.astype(np.complex128), rows=6000)
