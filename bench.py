@@ -251,14 +251,17 @@ def main():
         est = bfgpu.LearnedRankEstimator(learned.load(MODEL, ctx=ctx))
     # the two forms of the north-star comparison, factorized in turn (the
     # first large factorization of a process also pays for first-touch host
-    # memory, so each form runs `reps` times and the fastest run counts)
+    # memory, and the host-side emission time alternates between
+    # factorizations of one process, so each form runs `reps` times, the
+    # order of the forms alternating from one repetition to the next, and
+    # the fastest run of each form counts)
     import gc
     forms = [("fp64_plain", np.complex128, None)] if not args.no_fp64 else []
     forms.append((f"fp32_{rank}", np.complex64, est))
     runs = {f: [] for f, _, _ in forms}
     ops = {}
     for rep in range(max(1, args.reps)):
-        for form, dt, e in forms:
+        for form, dt, e in (forms if rep % 2 == 0 else forms[::-1]):
             ops.pop(form, None)
             gc.collect()
             log(f"{args.workload} n={n}: factorize {form} (run {rep + 1})")

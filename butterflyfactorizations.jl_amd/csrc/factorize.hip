@@ -17,7 +17,7 @@
 #include <map>
 #include <mutex>
 #include <thread>
-#include <unordered_map>
+#include <type_traits>
 
 #include "internal.hpp"
 
@@ -111,16 +111,69 @@ namespace {
 
 struct Comp {
     int32_t bf, stage, S, O;
-    const int32_t *obs;  // candidate observer point ids
+    const int32_t *obs;   // candidate observer point ids
     int32_t n_obs;
-    std::vector<int32_t> cols;  // source point ids
+    int32_t n_cols;       // source point ids: cols[0, n_cols), in the level's column pool or
+    const int32_t *cols;  // the tree's value list
     double eps;
     uint64_t key;
     int32_t guess = 10, rows = 0, rank = 0;
-    int32_t judge = 0;  // 0 not judged, 1 accepted by the residual judge, 2 redone by plain RRQR
-    std::vector<int32_t> skel;
+    int32_t judge = 0;      // 0 not judged, 1 accepted by the residual judge, 2 redone by plain RRQR
+    int64_t skel_ofs = -1;  // skeleton (rank source ids) in the level's skeleton pool
     int64_t tofs = 0;
     float feat[6];
+};
+
+// Skeleton index of one factorization: (source node, observer node) ->
+// (offset, length) of the skeleton in the factorization's skeleton log;
+// open addressing, doubled at half load.
+struct SkelMap {
+    std::vector<uint64_t> k;  // key + 1 (0: empty)
+    std::vector<uint64_t> v;  // offset << 32 | length
+    size_t n = 0;
+    static size_t hash(uint64_t key) {
+        const uint64_t x = key * 0x9E3779B97F4A7C15ULL;
+        return size_t(x ^ (x >> 29));
+    }
+    size_t find(uint64_t key) const {
+        const size_t mask = k.size() - 1;
+        size_t h = hash(key) & mask;
+        while (k[h] != 0 && k[h] != key + 1) h = (h + 1) & mask;
+        return h;
+    }
+    void put(uint64_t key, int64_t ofs, int32_t len) {
+        BFG_REQUIRE(ofs >= 0 && ofs < (int64_t(1) << 31), "factorize: skeleton log exceeds 2^31 entries");
+        if (2 * (n + 1) > k.size()) {
+            std::vector<uint64_t> ok, ov;
+            ok.swap(k);
+            ov.swap(v);
+            k.assign(std::max<size_t>(16, 2 * ok.size()), 0);
+            v.assign(k.size(), 0);
+            for (size_t i = 0; i < ok.size(); ++i)
+                if (ok[i]) {
+                    const size_t h = find(ok[i] - 1);
+                    k[h] = ok[i];
+                    v[h] = ov[i];
+                }
+        }
+        const size_t h = find(key);
+        if (k[h] == 0) {
+            k[h] = key + 1;
+            ++n;
+        }
+        v[h] = (uint64_t(uint32_t(ofs)) << 32) | uint32_t(len);
+    }
+    // offset and length of key's skeleton; the key must be present
+    std::pair<int64_t, int32_t> at(uint64_t key) const {
+        const size_t h = k.empty() ? 0 : find(key);
+        BFG_REQUIRE(!k.empty() && k[h] == key + 1, "factorize: skeleton of a block missing");
+        return {int64_t(v[h] >> 32), int32_t(uint32_t(v[h]))};
+    }
+    void swap(SkelMap &o) {
+        k.swap(o.k);
+        v.swap(o.v);
+        std::swap(n, o.n);
+    }
 };
 
 // Per-factorization assembly state.
@@ -141,11 +194,39 @@ public:
     }
 };
 
+// Node lists of the levels of a tree under a root, flat: level i is
+// v[ofs[i], ofs[i + 1]) (two allocations per tree instead of one per level).
+struct Levels {
+    std::vector<int32_t> v, ofs;
+    struct Span {
+        const int32_t *p;
+        size_t n;
+        const int32_t *begin() const { return p; }
+        const int32_t *end() const { return p + n; }
+        size_t size() const { return n; }
+        int32_t operator[](size_t i) const { return p[i]; }
+    };
+    Levels() = default;
+    explicit Levels(const std::vector<std::vector<int32_t>> &lv) {
+        ofs.resize(lv.size() + 1, 0);
+        for (size_t i = 0; i < lv.size(); ++i) ofs[i + 1] = ofs[i] + int32_t(lv[i].size());
+        v.reserve(size_t(ofs.back()));
+        for (const auto &l : lv) v.insert(v.end(), l.begin(), l.end());
+    }
+    size_t size() const { return ofs.empty() ? 0 : ofs.size() - 1; }
+    Span operator[](size_t i) const { return Span{v.data() + ofs[i], size_t(ofs[i + 1] - ofs[i])}; }
+    Span back() const { return (*this)[size() - 1]; }
+    void swap(Levels &o) {
+        v.swap(o.v);
+        ofs.swap(o.ofs);
+    }
+};
+
 struct Build {
     int32_t no, ns, LS, LO, L;
     double eps;
-    std::vector<std::vector<int32_t>> tS, tO;
-    std::unordered_map<uint64_t, std::vector<int32_t>> K;
+    Levels tS, tO;
+    SkelMap K;  // skeletons so far, as spans of the factorization's skeleton log
     // release o's containers here (called from worker threads, so the
     // many small frees of a large batch run in parallel)
     void swap_out(Build &o) {
@@ -192,6 +273,27 @@ struct Engine {
     int64_t n_comp = 0, n_retry = 0, n_rank0 = 0, n_judged = 0, n_fallback = 0;
     double t_host = 0, t_qr = 0, t_interp = 0;  // seconds: sample prep, fill+QR, interpolation
     double t_judge = 0;                         // seconds: held-out rows, fill, judge (part of t_qr)
+    double t_guess = 0, t_trim = 0;             // seconds: rank guesses, factor-buffer trims
+    // host lists of a level's batches, kept across levels: their capacity is
+    // reused instead of hundreds of MB being unmapped and faulted in again
+    // per level (unmapping stalls every worker thread)
+    std::vector<int32_t> pend, later, retry, acc;
+    hvec<int32_t> hrows, hcols;
+    std::vector<int32_t> hrank, hpiv, jq, jpos;
+    hvec<FillTask> ft;
+    hvec<int64_t> fe;
+    std::vector<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big, sk_run;
+    hvec<QrTask> qt;
+    hvec<InterpTask> it;
+    std::vector<float> gfeat;
+    std::vector<int32_t> gpred;
+    std::vector<int64_t> j_at;  // residual judge
+    hvec<JudgeTask> j_task;
+    std::vector<int32_t> j_who;
+    hvec<double> j_res;
+    std::vector<uint8_t> j_fail;
+    std::vector<uint8_t> e_dbl;  // per batch: doubled, held-out rows
+    std::vector<int32_t> e_jm;
 
     Engine(Ctx &c, const Tree &te, const Tree &tr, const bfgpu_opts &op)
         : ctx(c), test(te), trial(tr), o(op), prec(op.precision),
@@ -217,6 +319,12 @@ struct Engine {
     // entry after the last one, which then meets a zero weight and must be
     // finite.
     void trim_pool(DevBuf &pool, int64_t &cap_elems, int64_t used) {
+        const auto t0 = clk::now();
+        struct Acc {
+            double &t;
+            clk::time_point a;
+            ~Acc() { t += secs(a, clk::now()); }
+        } acc{t_trim, t0};
         const int64_t need = used + FACTOR_PAD;
         if (used == 0 && cap_elems == 0) return;  // no blocks on this level
         BFG_REQUIRE(cap_elems >= need, "factorize: factor buffer without room for its tail pad");
@@ -237,6 +345,25 @@ struct Engine {
         for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z,
                           &d_jtask, &d_jres})
             *d = DevBuf();
+        Engine *e = this;
+        auto drop = [](auto &v) { std::remove_reference_t<decltype(v)>().swap(v); };
+        for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->hpiv, &e->jq, &e->jpos, &e->gpred,
+                        &e->j_who, &e->e_jm})
+            drop(*v);
+        for (auto *v : {&e->rofs, &e->cofs, &e->zo, &e->jr, &e->jz, &e->jrf, &e->jzf, &e->jcnt, &e->tofs_run,
+                        &e->it_pos, &e->it_big, &e->sk_run, &e->j_at})
+            drop(*v);
+        drop(hrows);
+        drop(hcols);
+        drop(ft);
+        drop(fe);
+        drop(qt);
+        drop(it);
+        drop(gfeat);
+        drop(j_task);
+        drop(j_res);
+        drop(j_fail);
+        drop(e_dbl);
     }
 
     // out[0] = 0, out[i + 1] = out[i] + v(i): chunk sums in parallel, a
@@ -283,12 +410,16 @@ struct Engine {
     void guesses(std::vector<Comp> &cs) {
         if (o.rank_mode == BFGPU_RANK_LEARNED) {
             BFG_REQUIRE(o.mlp != nullptr, "factorize: learned ranks need an MLP");
-            std::vector<float> f(6 * cs.size());
-            for (size_t i = 0; i < cs.size(); ++i)
-                for (int q = 0; q < 6; ++q) f[6 * i + q] = cs[i].feat[q];
-            std::vector<int32_t> pr;
+            std::vector<float> &f = gfeat;
+            f.resize(6 * cs.size());
+            parallel_for(int64_t(cs.size()), [&](int64_t i) {
+                for (int q = 0; q < 6; ++q) f[6 * size_t(i) + q] = cs[size_t(i)].feat[q];
+            }, 1 << 14);
+            std::vector<int32_t> &pr = gpred;
             mlp_predict_ranks(o.mlp, f, int64_t(cs.size()), pr);
-            for (size_t i = 0; i < cs.size(); ++i) cs[i].guess = std::max(pr[i], 1) + o.oversample;
+            parallel_for(int64_t(cs.size()), [&](int64_t i) {
+                cs[size_t(i)].guess = std::max(pr[size_t(i)], 1) + o.oversample;
+            }, 1 << 14);
             return;
         }
         for (auto &c : cs) {
@@ -314,7 +445,7 @@ struct Engine {
     // columns fit the judge kernel; 0 when c is not judged.
     int32_t judge_m(const Comp &c) const {
         if (o.rank_mode != BFGPU_RANK_LEARNED || o.judge_rows <= 0 || c.judge != 0 || c.rows >= c.n_obs ||
-            c.cols.empty() || int32_t(c.cols.size()) > JUDGE_MAX_COLS)
+            c.n_cols == 0 || c.n_cols > JUDGE_MAX_COLS)
             return 0;
         return std::min(o.judge_rows, c.n_obs - c.rows);
     }
@@ -335,12 +466,14 @@ struct Engine {
         (void)jq;
         // the accepted compressions with held-out tiles, in batch order
         const int64_t na = int64_t(acc.size());
-        std::vector<int64_t> at;
+        std::vector<int64_t> &at = j_at;
         prefix_sum(na, at, [&](int64_t i) { return int64_t(jpos[acc[size_t(i)]] >= 0); });
         const int64_t nj = at[size_t(na)];
         if (nj == 0) return;
-        hvec<JudgeTask> jt(static_cast<size_t>(nj));
-        std::vector<int32_t> who(static_cast<size_t>(nj));
+        hvec<JudgeTask> &jt = j_task;
+        jt.resize(static_cast<size_t>(nj));
+        std::vector<int32_t> &who = j_who;
+        who.resize(static_cast<size_t>(nj));
         parallel_for(na, [&](int64_t i) {
             const int32_t q = acc[size_t(i)];
             const int32_t a = jpos[q];
@@ -353,11 +486,13 @@ struct Engine {
         d_jres.reserve(sizeof(double) * 2 * size_t(nj));
         launch_judge(ctx.stream, prec, d_Z.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
                      d_jres.as<double>());
-        hvec<double> res(2 * static_cast<size_t>(nj));
+        hvec<double> &res = j_res;
+        res.resize(2 * static_cast<size_t>(nj));
         BFG_HIP(hipMemcpyAsync(res.data(), d_jres.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, ctx.stream));
         BFG_HIP(hipStreamSynchronize(ctx.stream));
         const double slack = o.judge_slack > 0.0 ? o.judge_slack : 4.0;
-        std::vector<uint8_t> fail(static_cast<size_t>(nj));
+        std::vector<uint8_t> &fail = j_fail;
+        fail.assign(static_cast<size_t>(nj), 0);
         parallel_for(nj, [&](int64_t a) {
             Comp &c = cs[pend[who[size_t(a)]]];
             fail[size_t(a)] = !(res[2 * a] <= slack * c.eps * res[2 * a + 1]);
@@ -378,14 +513,16 @@ struct Engine {
 
     // Run every compression of one level in batches: fill -> pivoted QR ->
     // (adaptive doubling of the rows that saturated) -> interpolation
-    // matrices written into `pool` (the level's factor buffer).
-    void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used) {
+    // matrices written into `pool` (the level's factor buffer), skeletons
+    // appended to `skp` (the level's skeleton pool, Comp::skel_ofs).
+    void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used, std::vector<int32_t> &skp) {
+        const auto tg = clk::now();
         guesses(cs);
-        std::vector<int32_t> pend;
+        pend.clear();
         for (size_t i = 0; i < cs.size(); ++i) {
             Comp &c = cs[i];
             ++n_comp;
-            if (c.n_obs == 0 || c.cols.empty()) {
+            if (c.n_obs == 0 || c.n_cols == 0) {
                 c.rank = 0;
                 c.rows = 0;
                 c.tofs = pool_used;
@@ -394,13 +531,7 @@ struct Engine {
             c.rows = std::min(std::max(c.guess, 10), c.n_obs);
             pend.push_back(int32_t(i));
         }
-        hvec<int32_t> hrows, hcols;
-        std::vector<int32_t> hrank, hpiv, jq, jpos;
-        hvec<FillTask> ft;
-        hvec<int64_t> fe;
-        std::vector<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big;
-        hvec<QrTask> qt;
-        hvec<InterpTask> it;
+        t_guess += secs(tg, clk::now());
         // kernel-matrix tiles of one batch: at most 32 GiB or 30 % of the
         // free device memory (besides what the tile buffer already holds)
         int64_t z_budget = o.workspace_bytes;
@@ -410,7 +541,6 @@ struct Engine {
             z_budget = std::max<int64_t>(int64_t(1) << 30,
                                          std::min<int64_t>(int64_t(32) << 30, int64_t(0.3 * double(free_b + d_Z.bytes))));
         }
-        std::vector<int32_t> later;
         while (!pend.empty()) {
             auto th = clk::now();
             {
@@ -418,7 +548,7 @@ struct Engine {
                 int64_t zb = 0;
                 for (; nb < pend.size(); ++nb) {
                     const Comp &c = cs[pend[nb]];
-                    const int64_t z = int64_t(c.rows + judge_m(c)) * int64_t(c.cols.size()) * es;
+                    const int64_t z = int64_t(c.rows + judge_m(c)) * int64_t(c.n_cols) * es;
                     if (nb > 0 && zb + z > z_budget) break;
                     zb += z;
                 }
@@ -428,15 +558,16 @@ struct Engine {
             const size_t np = pend.size();
             // sizes of every tile, then running offsets (parallel prefix
             // sums): main tiles, then the judge's held-out tiles
-            std::vector<int32_t> jm(np);
+            std::vector<int32_t> &jm = e_jm;
+            jm.resize(np);
             parallel_for(int64_t(np), [&](int64_t q) { jm[q] = judge_m(cs[pend[q]]); }, 4096);
             prefix_sum(int64_t(np), rofs, [&](int64_t q) { return int64_t(cs[pend[q]].rows); });
-            prefix_sum(int64_t(np), cofs, [&](int64_t q) { return int64_t(cs[pend[q]].cols.size()); });
+            prefix_sum(int64_t(np), cofs, [&](int64_t q) { return int64_t(cs[pend[q]].n_cols); });
             prefix_sum(int64_t(np), zo, [&](int64_t q) {
-                return int64_t(cs[pend[q]].rows) * int64_t(cs[pend[q]].cols.size());
+                return int64_t(cs[pend[q]].rows) * int64_t(cs[pend[q]].n_cols);
             });
             prefix_sum(int64_t(np), jrf, [&](int64_t q) { return int64_t(jm[q]); });
-            prefix_sum(int64_t(np), jzf, [&](int64_t q) { return int64_t(jm[q]) * int64_t(cs[pend[q]].cols.size()); });
+            prefix_sum(int64_t(np), jzf, [&](int64_t q) { return int64_t(jm[q]) * int64_t(cs[pend[q]].n_cols); });
             prefix_sum(int64_t(np), jcnt, [&](int64_t q) { return int64_t(jm[q] > 0); });
             const size_t nj = size_t(jcnt[np]);
             const int64_t zmain = zo[np];
@@ -451,7 +582,7 @@ struct Engine {
             std::atomic<int32_t> amax_n{1}, amax_m{1};
             parallel_for(int64_t(np), [&](int64_t q) {
                 const Comp &c = cs[pend[q]];
-                const int32_t n = int32_t(c.cols.size());
+                const int32_t n = c.n_cols;
                 ft[q] = FillTask{zo[q], c.rows, n, int32_t(rofs[q]), int32_t(cofs[q])};
                 fe[q] = zo[q + 1];
                 qt[q] = QrTask{zo[q], c.rows, n, int32_t(cofs[q]), 0};
@@ -495,7 +626,7 @@ struct Engine {
                     int32_t *h = hrows.data() + rofs[np] + jrf[q];
                     for (int32_t i = 0; i < mc; ++i) h[i] = c.obs[pre[size_t(c.rows + i)]];
                 }
-                std::copy(c.cols.begin(), c.cols.end(), hcols.begin() + cofs[q]);
+                std::copy(c.cols, c.cols + c.n_cols, hcols.begin() + cofs[q]);
             });
             auto tq = clk::now();
             t_host += secs(th, tq);
@@ -559,14 +690,16 @@ struct Engine {
             t_qr += secs(tq, ti);
             // adaptive doubling (compressors.jl:134) decided per compression
             // in parallel; the retried ones in batch order
-            std::vector<uint8_t> dbl(np);
+            std::vector<uint8_t> &dbl = e_dbl;
+            dbl.assign(np, 0);
             parallel_for(int64_t(np), [&](int64_t q) {
                 Comp &c = cs[pend[q]];
                 const int32_t r = hrank[q];
                 dbl[q] = o.adaptive && r > int32_t(std::floor(0.8 * c.rows)) && c.rows < c.n_obs;
                 if (!dbl[q]) c.rank = r;
             }, 4096);
-            std::vector<int32_t> retry, acc;
+            retry.clear();
+            acc.clear();
             acc.reserve(np);
             for (size_t q = 0; q < np; ++q) {
                 Comp &c = cs[pend[q]];
@@ -578,7 +711,7 @@ struct Engine {
                 }
                 if (c.rank == 0 && n_rank0++ < 4 && std::getenv("BFGPU_TIMING"))
                     std::fprintf(stderr, "[bfgpu] rank-0 compression: %d sampled rows of %d observers x %d columns\n",
-                                 c.rows, c.n_obs, int(c.cols.size()));
+                                 c.rows, c.n_obs, c.n_cols);
                 acc.push_back(int32_t(q));
             }
             // residual judge of the compressions whose rows a prediction
@@ -596,7 +729,7 @@ struct Engine {
             };
             prefix_sum(na, tofs_run, [&](int64_t a) {
                 const Comp &c = cs[pend[acc[size_t(a)]]];
-                return kept(a) ? int64_t(c.rank) * int64_t(c.cols.size()) : int64_t(0);
+                return kept(a) ? int64_t(c.rank) * int64_t(c.n_cols) : int64_t(0);
             });
             // low ranks (almost all: the lane-per-column kernel) first, the
             // rest after them
@@ -619,13 +752,20 @@ struct Engine {
             }, 4096);
             pool_used += tofs_run[size_t(na)];
             const int32_t max_rank = amax_rank.load();
-            // skeletons (pivot prefixes) in parallel
+            // skeletons (pivot prefixes) at running offsets of the level's
+            // skeleton pool, in parallel; compressions being retried with more
+            // rows take theirs from the later batch
+            auto done = [&](int64_t q) { return cs[pend[q]].rows == ft[q].m; };
+            prefix_sum(int64_t(np), sk_run, [&](int64_t q) { return done(q) ? int64_t(cs[pend[q]].rank) : int64_t(0); });
+            const int64_t sk0 = int64_t(skp.size());
+            skp.resize(size_t(sk0 + sk_run[np]));
             parallel_for(int64_t(np), [&](int64_t q) {
+                if (!done(q)) return;
                 Comp &c = cs[pend[q]];
-                if (c.rows != ft[q].m) return;  // being retried with more rows
-                c.skel.resize(c.rank);
-                for (int32_t i = 0; i < c.rank; ++i) c.skel[i] = c.cols[hpiv[qt[q].pofs + i]];
-            });
+                c.skel_ofs = sk0 + sk_run[q];
+                int32_t *o = skp.data() + c.skel_ofs;
+                for (int32_t i = 0; i < c.rank; ++i) o[i] = c.cols[hpiv[qt[q].pofs + i]];
+            }, 4096);
             if (!it.empty()) {
                 ensure_pool(pool, pool_cap, pool_used + FACTOR_PAD);
                 upload(d_interp, it, ctx.stream);
@@ -694,8 +834,8 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         Build &x = B[b];
         x.no = pairs[b].first;
         x.ns = pairs[b].second;
-        x.tS = traverse_and_pad(trial, x.ns);
-        x.tO = traverse_and_pad(test, x.no);
+        x.tS = Levels(traverse_and_pad(trial, x.ns));
+        x.tO = Levels(traverse_and_pad(test, x.no));
         x.LS = int32_t(x.tS.size());
         x.LO = int32_t(x.tO.size());
         x.L = std::max(x.LS, x.LO);
@@ -712,7 +852,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     bf.factors.resize(bf.n_stage_launch);
     bf.factor_elems.assign(bf.n_stage_launch, 0);
 
-    auto mkcomp = [&](int32_t b, int32_t stage, int32_t S, int32_t O, std::vector<int32_t> cols) {
+    auto mkcomp = [&](int32_t b, int32_t stage, int32_t S, int32_t O, const int32_t *cols, int32_t n_cols) {
         Comp c;
         c.bf = b;
         c.stage = stage;
@@ -720,137 +860,200 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         c.O = O;
         c.obs = test.vals(O);
         c.n_obs = test.nvals(O);
-        c.cols = std::move(cols);
+        c.cols = cols;
+        c.n_cols = n_cols;
         c.eps = B[b].eps;
         c.key = sample_key(o.seed, B[b].no, B[b].ns, stage, S, O);
         features(test, trial, o.k, S, O, stage, c.eps, c.feat);
         return c;
-    };
-    auto log_skel = [&](OneBF &ob, int32_t stage, int32_t s, int32_t on, const std::vector<int32_t> &v) {
-        ob.skel.push_back(SkelRec{stage, s, on, int32_t(ob.skel_idx.size()), int32_t(v.size())});
-        ob.skel_idx.insert(ob.skel_idx.end(), v.begin(), v.end());
     };
     auto log_comp = [&](OneBF &ob, const Comp &c) {
         CompRec r{c.stage, c.S, c.O, c.guess, c.rows, c.rank, {}, c.judge};
         for (int q = 0; q < 6; ++q) r.feat[q] = c.feat[q];
         ob.comp.push_back(r);
     };
-    double t_plan = 0, t_emit = 0;
+    // A skeleton update of one level, applied after the level's blocks (the
+    // reference's sequential update): the skeleton at `ofs` of the
+    // factorization's own log (an identity block) or of the level's pool.
+    struct Upd {
+        uint64_t key;
+        int64_t ofs;
+        int32_t len;
+        bool from_log;
+    };
+    // Append u's skeleton to the factorization's skeleton log (a record of
+    // stage `stage`) and point K at it.
+    auto apply_upd = [&](OneBF &ob, Build &x, int32_t stage, const Upd &u, const std::vector<int32_t> &skp) {
+        const int64_t so = int64_t(ob.skel_idx.size());
+        BFG_REQUIRE(so + u.len < (int64_t(1) << 31), "factorize: skeleton log exceeds 2^31 entries");
+        ob.skel.push_back(SkelRec{stage, int32_t(u.key >> 32), int32_t(u.key & 0xffffffffu), int32_t(so), u.len});
+        ob.skel_idx.resize(size_t(so + u.len));
+        if (u.len > 0) {
+            if (u.from_log)  // an earlier entry of the same log: [ofs, ofs + len) lies before so
+                std::copy_n(ob.skel_idx.begin() + u.ofs, u.len, ob.skel_idx.begin() + so);
+            else
+                std::copy_n(skp.begin() + u.ofs, u.len, ob.skel_idx.begin() + so);
+        }
+        x.K.put(u.key, so, u.len);
+    };
+    double t_plan = 0, t_emit = 0, t_drop = 0;
+    // per-level arrays, reused level to level (capacity kept): the level's
+    // skeletons (Comp::skel_ofs), compressions, actions and column pool
+    std::vector<int32_t> skp;
+    std::vector<Comp> cs;
+    std::vector<Act> acts;
+    std::vector<int32_t> up;
 
     // ---- stage 0: Q (bfassembly.jl:82-102) ----
     {
         auto tp = clk::now();
         std::vector<int64_t> off(nB + 1, 0);
         for (int64_t b = 0; b < nB; ++b) off[b + 1] = off[b] + int64_t(B[b].tS.back().size());
-        std::vector<Comp> cs(static_cast<size_t>(off[nB]));
+        cs.assign(static_cast<size_t>(off[nB]), Comp{});
         parallel_for(nB, [&](int64_t b) {
             const auto &lv = B[b].tS.back();
             for (size_t i = 0; i < lv.size(); ++i) {
-                int32_t Sl = lv[i];
-                cs[off[b] + int64_t(i)] =
-                    mkcomp(int32_t(b), 0, Sl, B[b].no,
-                           std::vector<int32_t>(trial.vals(Sl), trial.vals(Sl) + trial.nvals(Sl)));
+                const int32_t Sl = lv[i];
+                cs[off[b] + int64_t(i)] = mkcomp(int32_t(b), 0, Sl, B[b].no, trial.vals(Sl), trial.nvals(Sl));
             }
-        });
+        }, 1024);
         t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
-        E.run(cs, bf.factors[0], cap, used);
+        skp.clear();
+        E.run(cs, bf.factors[0], cap, used, skp);
         E.trim_pool(bf.factors[0], cap, used);
         bf.factor_elems[0] = used;
         auto te = clk::now();
         parallel_for(nB, [&](int64_t b) {
             OneBF &ob = bf.bfs[b];
             Build &x = B[b];
+            ob.st[0].b.reserve(size_t(off[b + 1] - off[b]));
             for (int64_t i = off[b]; i < off[b + 1]; ++i) {
-                Comp &c = cs[i];
+                const Comp &c = cs[i];
                 log_comp(ob, c);
-                ob.st[0].b.push_back(Block{x.no, c.S, x.no, c.S, c.rank, int32_t(c.cols.size()), 0, c.tofs, -1, -1});
-                log_skel(ob, 0, c.S, x.no, c.skel);
-                x.K[fastkey(c.S, x.no)] = std::move(c.skel);
+                ob.st[0].b.push_back(Block{x.no, c.S, x.no, c.S, c.rank, c.n_cols, 0, c.tofs, -1, -1});
+                apply_upd(ob, x, 0, Upd{fastkey(c.S, x.no), c.skel_ofs, c.skel_ofs >= 0 ? c.rank : 0, false}, skp);
             }
-        });
+        }, 1024);
         t_emit += secs(te, clk::now());
+        const auto td = clk::now();
+        cs.clear();
+        t_drop += secs(td, clk::now());
     }
 
     // ---- R levels (bfassembly.jl:107-181) ----
     for (int32_t l = 1; l <= Lmax - 1; ++l) {
         auto tp = clk::now();
-        std::vector<std::vector<Comp>> csb(nB);
-        std::vector<std::vector<Act>> actb(nB);
-        parallel_for(nB, [&](int64_t b) {
+        // Two walks over the actions of every factorization: the first counts
+        // compressions, actions and column entries, the second writes them at
+        // their offsets in the level's arrays.  The columns U of a
+        // compression (the skeletons of the source children under the
+        // observer parent, or the frozen source's) are written once per
+        // (Svert, Overt) and shared by the observer children's compressions.
+        std::vector<int64_t> nc(nB + 1, 0), na(nB + 1, 0), nu(nB + 1, 0);
+        auto walk = [&](int64_t b, bool fill) {
             Build &x = B[b];
-            if (l > x.L - 1) return;
-            auto &cs = csb[b];
-            auto &acts = actb[b];
-            auto U_of = [&](int32_t Sv, int32_t Ov) {
-                if (trial.isleaf(Sv)) return x.K.at(fastkey(Sv, Ov));
-                std::vector<int32_t> u;
-                for (int32_t c = 0; c < trial.n_children[Sv]; ++c) {
-                    const auto &k = x.K.at(fastkey(trial.first_child[Sv] + c, Ov));
-                    u.insert(u.end(), k.begin(), k.end());
-                }
-                return u;
-            };
-            auto add_comp = [&](int32_t S, int32_t O, std::vector<int32_t> U, int32_t kind, int32_t Oout,
-                                int32_t Sv, int32_t Ov) {
-                cs.push_back(mkcomp(int32_t(b), l, S, O, std::move(U)));
-                acts.push_back(Act{kind, int32_t(cs.size() - 1), Oout, Sv, Ov, 0});
-            };
-            auto add_ident = [&](int32_t Ov, int32_t Sv) {
-                acts.push_back(Act{1, -1, Ov, Sv, Ov, int32_t(x.K.at(fastkey(Sv, Ov)).size())});
-            };
-            const bool sf = l >= x.LS, of = l >= x.LO;
-            if (!sf && !of) {
-                const auto &Os = x.tO[l - 1];
-                const auto &Ss = x.tS[x.LS - l - 1];
-                for (int32_t Sv : Ss)
+            int64_t kc = 0, ka = 0, ku = 0;
+            if (l <= x.L - 1) {
+                const int32_t *sk = bf.bfs[b].skel_idx.data();
+                Comp *C = fill ? cs.data() + nc[b] : nullptr;
+                Act *A = fill ? acts.data() + na[b] : nullptr;
+                int32_t *U = fill ? up.data() + nu[b] : nullptr;
+                auto put = [&](uint64_t key) {
+                    const auto sp = x.K.at(key);
+                    if (fill) std::copy_n(sk + sp.first, sp.second, U + ku);
+                    ku += sp.second;
+                };
+                // the columns of (Sv, Ov): offset within this factorization's U, length
+                auto U_of = [&](int32_t Sv, int32_t Ov) {
+                    const int64_t o = ku;
+                    if (trial.isleaf(Sv)) put(fastkey(Sv, Ov));
+                    else
+                        for (int32_t c = 0; c < trial.n_children[Sv]; ++c)
+                            put(fastkey(trial.first_child[Sv] + c, Ov));
+                    return std::pair<int64_t, int32_t>(o, int32_t(ku - o));
+                };
+                auto frozen = [&](int32_t Sv, int32_t Ov) {
+                    const int64_t o = ku;
+                    put(fastkey(Sv, Ov));
+                    return std::pair<int64_t, int32_t>(o, int32_t(ku - o));
+                };
+                auto add_comp = [&](int32_t S, int32_t O, std::pair<int64_t, int32_t> u, int32_t kind, int32_t Oout,
+                                    int32_t Sv, int32_t Ov) {
+                    if (fill) {
+                        C[kc] = mkcomp(int32_t(b), l, S, O, U + u.first, u.second);
+                        A[ka] = Act{kind, int32_t(kc), Oout, Sv, Ov, 0};
+                    }
+                    ++kc;
+                    ++ka;
+                };
+                auto add_ident = [&](int32_t Ov, int32_t Sv) {
+                    if (fill) A[ka] = Act{1, -1, Ov, Sv, Ov, x.K.at(fastkey(Sv, Ov)).second};
+                    ++ka;
+                };
+                const bool sf = l >= x.LS, of = l >= x.LO;
+                if (!sf && !of) {
+                    const auto &Os = x.tO[l - 1];
+                    const auto &Ss = x.tS[x.LS - l - 1];
+                    for (int32_t Sv : Ss)
+                        for (int32_t Ov : Os) {
+                            if (!test.isleaf(Ov)) {
+                                const auto u = U_of(Sv, Ov);
+                                for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
+                                    const int32_t Oc = test.first_child[Ov] + c;
+                                    add_comp(Sv, Oc, u, 0, Oc, Sv, Ov);
+                                }
+                            } else if (!trial.isleaf(Sv)) {
+                                add_comp(Sv, Ov, U_of(Sv, Ov), 0, Ov, Sv, Ov);
+                            } else {
+                                add_ident(Ov, Sv);
+                            }
+                        }
+                } else if (sf && !of) {
+                    const auto &Os = x.tO[l - 1];
+                    const int32_t Sv = x.ns;
                     for (int32_t Ov : Os) {
                         if (!test.isleaf(Ov)) {
-                            std::vector<int32_t> U = U_of(Sv, Ov);
+                            // source frozen: the input is the single slot (Ov, ns)
+                            const auto u = frozen(Sv, Ov);
                             for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
-                                int32_t Oc = test.first_child[Ov] + c;
-                                add_comp(Sv, Oc, U, 0, Oc, Sv, Ov);
+                                const int32_t Oc = test.first_child[Ov] + c;
+                                add_comp(Sv, Oc, u, 2, Oc, Sv, Ov);
                             }
-                        } else if (!trial.isleaf(Sv)) {
-                            add_comp(Sv, Ov, U_of(Sv, Ov), 0, Ov, Sv, Ov);
                         } else {
                             add_ident(Ov, Sv);
                         }
                     }
-            } else if (sf && !of) {
-                const auto &Os = x.tO[l - 1];
-                const int32_t Sv = x.ns;
-                for (int32_t Ov : Os) {
-                    if (!test.isleaf(Ov)) {
-                        for (int32_t c = 0; c < test.n_children[Ov]; ++c) {
-                            int32_t Oc = test.first_child[Ov] + c;
-                            // source frozen: the input is the single slot (Ov, ns)
-                            add_comp(Sv, Oc, x.K.at(fastkey(Sv, Ov)), 2, Oc, Sv, Ov);
+                } else if (!sf && of) {
+                    const auto &Os = x.tO[x.LO - 1];
+                    const auto &Ss = x.tS[x.LS - l - 1];
+                    for (int32_t Sv : Ss)
+                        for (int32_t Ov : Os) {
+                            if (!trial.isleaf(Sv)) add_comp(Sv, Ov, U_of(Sv, Ov), 0, Ov, Sv, Ov);
+                            else add_ident(Ov, Sv);
                         }
-                    } else {
-                        add_ident(Ov, Sv);
-                    }
                 }
-            } else if (!sf && of) {
-                const auto &Os = x.tO[x.LO - 1];
-                const auto &Ss = x.tS[x.LS - l - 1];
-                for (int32_t Sv : Ss)
-                    for (int32_t Ov : Os) {
-                        if (!trial.isleaf(Sv)) add_comp(Sv, Ov, U_of(Sv, Ov), 0, Ov, Sv, Ov);
-                        else add_ident(Ov, Sv);
-                    }
             }
-        });
-        std::vector<int64_t> off(nB + 1, 0);
-        for (int64_t b = 0; b < nB; ++b) off[b + 1] = off[b] + int64_t(csb[b].size());
-        std::vector<Comp> cs(static_cast<size_t>(off[nB]));
-        parallel_for(nB, [&](int64_t b) {
-            for (size_t i = 0; i < csb[b].size(); ++i) cs[off[b] + int64_t(i)] = std::move(csb[b][i]);
-            std::vector<Comp>().swap(csb[b]);
-        });
+            if (!fill) {
+                nc[b + 1] = kc;
+                na[b + 1] = ka;
+                nu[b + 1] = ku;
+            }
+        };
+        parallel_for(nB, [&](int64_t b) { walk(b, false); }, 1024);
+        for (int64_t b = 0; b < nB; ++b) {
+            nc[b + 1] += nc[b];
+            na[b + 1] += na[b];
+            nu[b + 1] += nu[b];
+        }
+        cs.resize(size_t(nc[nB]));
+        acts.resize(size_t(na[nB]));
+        up.resize(size_t(nu[nB]));
+        parallel_for(nB, [&](int64_t b) { walk(b, true); }, 1024);
         t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
-        E.run(cs, bf.factors[l], cap, used);
+        skp.clear();
+        E.run(cs, bf.factors[l], cap, used, skp);
         E.trim_pool(bf.factors[l], cap, used);
         bf.factor_elems[l] = used;
         // emit blocks in action order; K updates in the same order, applied
@@ -861,37 +1064,50 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
             if (l > x.L - 1) return;
             OneBF &ob = bf.bfs[b];
             auto &st = ob.st[l].b;
-            std::vector<std::pair<uint64_t, std::vector<int32_t>>> upd;
-            for (const Act &a : actb[b]) {
+            {  // the level's blocks: one per identity or whole-T compression, else one per source child
+                size_t nblk = 0;
+                for (int64_t ai = na[b]; ai < na[b + 1]; ++ai) {
+                    const Act &a = acts[size_t(ai)];
+                    nblk += (a.kind != 0 || trial.isleaf(a.Sv)) ? 1 : size_t(trial.n_children[a.Sv]);
+                }
+                st.reserve(nblk);
+            }
+            thread_local std::vector<Upd> upd;
+            upd.clear();
+            for (int64_t ai = na[b]; ai < na[b + 1]; ++ai) {
+                const Act &a = acts[size_t(ai)];
                 if (a.kind == 1) {
                     st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, a.len, a.len, 1, -1, -1, -1});
-                    upd.emplace_back(fastkey(a.Sv, a.Ov), x.K.at(fastkey(a.Sv, a.Ov)));
+                    const auto sp = x.K.at(fastkey(a.Sv, a.Ov));
+                    upd.push_back(Upd{fastkey(a.Sv, a.Ov), sp.first, sp.second, true});
                     continue;
                 }
-                Comp &c = cs[off[b] + a.comp];
+                const Comp &c = cs[size_t(nc[b] + a.comp)];
                 log_comp(ob, c);
-                const int32_t nU = int32_t(c.cols.size());
+                const int32_t nU = c.n_cols;
                 if (a.kind == 2 || trial.isleaf(a.Sv)) {
                     // whole T: source frozen (Oc, ns) <- (Ov, ns), or leaf Svert
                     st.push_back(Block{a.Oout, a.Sv, a.Ov, a.Sv, c.rank, nU, 0, c.tofs, -1, -1});
                 } else {
                     int32_t last = 0;
                     for (int32_t q = 0; q < trial.n_children[a.Sv]; ++q) {
-                        int32_t Sc = trial.first_child[a.Sv] + q;
-                        int32_t ks = int32_t(x.K.at(fastkey(Sc, a.Ov)).size());
+                        const int32_t Sc = trial.first_child[a.Sv] + q;
+                        const int32_t ks = x.K.at(fastkey(Sc, a.Ov)).second;
                         st.push_back(Block{a.Oout, a.Sv, a.Ov, Sc, c.rank, ks, 0,
                                            c.tofs + int64_t(c.rank) * last, -1, -1});
                         last += ks;
                     }
                 }
-                upd.emplace_back(fastkey(c.S, c.O), std::move(c.skel));
+                upd.push_back(Upd{fastkey(c.S, c.O), c.skel_ofs, c.skel_ofs >= 0 ? c.rank : 0, false});
             }
-            for (auto &u : upd) {
-                log_skel(ob, l, int32_t(u.first >> 32), int32_t(u.first & 0xffffffffu), u.second);
-                x.K[u.first] = std::move(u.second);
-            }
-        });
+            for (const Upd &u : upd) apply_upd(ob, x, l, u, skp);
+        }, 1024);
         t_emit += secs(te, clk::now());
+        const auto td = clk::now();
+        cs.clear();
+        acts.clear();
+        up.clear();
+        t_drop += secs(td, clk::now());
     }
 
     // ---- P (bfassembly.jl:186-197): dense kernel blocks ----
@@ -908,7 +1124,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
             for (size_t i = 0; i < lv.size(); ++i) {
                 int64_t q = noff[b] + int64_t(i);
                 rofs[q + 1] = test.nvals(lv[i]);
-                cofs[q + 1] = int64_t(x.K.at(fastkey(x.ns, lv[i])).size());
+                cofs[q + 1] = int64_t(x.K.at(fastkey(x.ns, lv[i])).second);
             }
         });
         for (int64_t q = 0; q < nP; ++q) {
@@ -930,13 +1146,14 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
             for (size_t i = 0; i < lv.size(); ++i) {
                 int64_t q = noff[b] + int64_t(i);
                 int32_t Ol = lv[i];
-                const auto &col = x.K.at(fastkey(x.ns, Ol));
-                FillTask f{zo[q], test.nvals(Ol), int32_t(col.size()), int32_t(rofs[q]), int32_t(cofs[q])};
+                const auto sp = x.K.at(fastkey(x.ns, Ol));
+                const int32_t *col = bf.bfs[b].skel_idx.data() + sp.first;
+                FillTask f{zo[q], test.nvals(Ol), sp.second, int32_t(rofs[q]), int32_t(cofs[q])};
                 std::copy(test.vals(Ol), test.vals(Ol) + f.m, hrows.begin() + rofs[q]);
-                std::copy(col.begin(), col.end(), hcols.begin() + cofs[q]);
+                std::copy_n(col, sp.second, hcols.begin() + cofs[q]);
                 ftall[q] = f;
                 st.push_back(Block{Ol, x.ns, Ol, x.ns, f.m, f.n, 0, zo[q], -1, -1});
-                if (kernel_entries) bf.bfs[b].pcols.insert(bf.bfs[b].pcols.end(), col.begin(), col.end());
+                if (kernel_entries) bf.bfs[b].pcols.insert(bf.bfs[b].pcols.end(), col, col + sp.second);
             }
         });
         std::vector<FillTask> ft;
@@ -970,6 +1187,9 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     for (int64_t b = 0; b < nB; ++b) bf.bfs[b].stage_data_base.assign(bf.bfs[b].st.size(), 0);
     auto tfree = clk::now();
     E.release();  // tile workspace and index lists: not needed for packing
+    for (auto *v : {&skp, &up}) std::vector<int32_t>().swap(*v);
+    std::vector<Comp>().swap(cs);
+    std::vector<Act>().swap(acts);
     reaper().post(std::move(B));
     const double t_free = secs(tfree, clk::now());
     auto tpk = clk::now();
@@ -983,11 +1203,12 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     bf.t_pack_ms = 1e3 * secs(tpk, t1);
     if (std::getenv("BFGPU_TIMING"))
         std::fprintf(stderr,
-                     "[bfgpu] factorize %.3fs: setup %.3fs, plan %.3fs, sample-prep %.3fs, fill+qr %.3fs, "
-                     "interp %.3fs, emit %.3fs, free %.3fs, pack %.3fs (%lld compressions, %lld retries, "
+                     "[bfgpu] factorize %.3fs: setup %.3fs, plan %.3fs, guess %.3fs, sample-prep %.3fs, "
+                     "fill+qr %.3fs, interp %.3fs, trim %.3fs, emit %.3fs, drop %.3fs, free %.3fs, pack %.3fs "
+                     "(%lld compressions, %lld retries, "
                      "%lld of rank 0, judge %.3fs: %lld judged, %lld fallbacks, %d threads)\n",
-                     bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_host, E.t_qr, E.t_interp, t_emit, t_free,
-                     secs(tpk, t1),
+                     bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_guess, E.t_host, E.t_qr, E.t_interp, E.t_trim,
+                     t_emit, t_drop, t_free, secs(tpk, t1),
                      (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, E.t_judge,
                      (long long)E.n_judged, (long long)E.n_fallback, host_threads());
 }

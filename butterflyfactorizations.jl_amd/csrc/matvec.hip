@@ -209,6 +209,7 @@ void pack_tables(BF &bf) {
     auto tp0 = clk::now();
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     const BlockIndex ix = index_blocks(bf);
+    double t_keys = 0, t_slots = 0, t_pf = 0;
     // block keys and roots index the trees' node arrays below
     parallel_for(nb, [&](int64_t b) {
         const OneBF &ob = bf.bfs[b];
@@ -221,6 +222,8 @@ void pack_tables(BF &bf) {
                                 k.src_in < trial.n_nodes,
                             "pack: block key outside the trees");
     }, 256);
+    t_keys = since(tp0);
+    auto tsl = clk::now();
     // ---- slot offsets (per factorization, local), reference slot ids ----
     std::vector<int32_t> sl_out(size_t(ix.total), -1), sl_in(size_t(ix.total), -1);
     std::vector<int64_t> slot_base(nb + 1, 0);
@@ -285,6 +288,7 @@ void pack_tables(BF &bf) {
     }
     BFG_REQUIRE(counter < (int64_t(1) << 31), "pack: skeleton vectors exceed 2^31 entries");
     bf.part_ofs = counter;
+    t_slots = since(tsl);
 
     // ---- size (bfdims.jl:54-64) and coverage ----
     if (nb == 1) {
@@ -661,6 +665,7 @@ void pack_tables(BF &bf) {
     bf.x_covered = covered_cols == bf.cols;
 
     // ---- matrix-free last level: per observer leaf, its P columns ----
+    auto tpf = clk::now();
     bf.pfree_ok = bf.pfree_t_ok = false;
     bf.pf_n_items = 0;
     bool pf = bf.precision == BFGPU_C64 && bf.pf_k > 0.0 && nb > 0;
@@ -753,6 +758,7 @@ void pack_tables(BF &bf) {
             bf.pfree_ok = true;
         }
     }
+    t_pf = since(tpf);
     bf.gather_ofs = bf.part_ofs + part_used;
     bf.slot_len = bf.gather_ofs + std::max(test.n_values, trial.n_values);
     BFG_REQUIRE(bf.slot_len < (int64_t(1) << 31), "pack: slot buffer exceeds 2^31 entries");
@@ -764,10 +770,9 @@ void pack_tables(BF &bf) {
     bf.sweep = SweepTables{};
     if (bf.use_sweep) build_sweep(bf);
     if (timing)
-        std::fprintf(stderr, "[bfgpu] pack %.3fs: slots %.3fs, group %.3fs, split %.3fs, finish+upload %.3fs, "
-                             "validate %.3fs\n",
-                     since(tp0), since(tp0) - t_group - t_split - t_finish - t_valid, t_group, t_split, t_finish,
-                     t_valid);
+        std::fprintf(stderr, "[bfgpu] pack %.3fs: index+keys %.3fs, slots %.3fs, group %.3fs, split %.3fs, "
+                             "finish+upload %.3fs, matrix-free tables %.3fs, validate %.3fs\n",
+                     since(tp0), t_keys, t_slots, t_group, t_split, t_finish, t_pf, t_valid);
 }
 
 // Fused-sweep tables: every forward task of every stage in stage order.

@@ -237,14 +237,14 @@ static void forward_chunked(bfgpu_mlp &m, int64_t n, const float *features, floa
 int32_t mlp_predict_ranks(const bfgpu_mlp *mc, const std::vector<float> &feat, int64_t n,
                           std::vector<int32_t> &out) {
     bfgpu_mlp &m = const_cast<bfgpu_mlp &>(*mc);
-    out.assign(size_t(n), 1);
+    out.resize(size_t(n));
     if (n == 0) return 0;
     std::vector<float> y(static_cast<size_t>(n));
     forward_chunked(m, n, feat.data(), y.data());
-    for (int64_t i = 0; i < n; ++i) {
-        float rk = y[i] * m.out_scale + m.out_shift;
-        out[i] = std::max(1, int32_t(std::ceil(rk)));
-    }
+    parallel_for(n, [&](int64_t i) {
+        const float rk = y[size_t(i)] * m.out_scale + m.out_shift;
+        out[size_t(i)] = std::max(1, int32_t(std::ceil(rk)));
+    }, 1 << 16);
     return 0;
 }
 
