@@ -18,6 +18,8 @@
 #include <mutex>
 #include <thread>
 #include <type_traits>
+#include <malloc.h>
+#include <climits>
 
 #include "internal.hpp"
 
@@ -818,6 +820,22 @@ void compression_features(const Tree &test, const Tree &trial, double k, int32_t
 
 // Factorize n_pairs blocks; fills bf.bfs (reference layout) and bf.factors.
 void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int32_t, int32_t>> &pairs) {
+    // Host heap: a factorization writes several GB of small per-block
+    // records from 16 threads; with glibc's defaults the per-thread heaps
+    // are trimmed and grown again around that, and the page-table updates
+    // stall every worker (block emission took 0.4 s or 3 s from one run to
+    // the next).  On the first factorization the process keeps freed memory
+    // for reuse instead (trim threshold INT_MAX, 64 MiB top pad, blocks up
+    // to 32 MiB from the heaps); BFGPU_HEAP_KEEP=0 leaves glibc's defaults.
+    static const bool heap_keep = [] {
+        const char *e = std::getenv("BFGPU_HEAP_KEEP");
+        if (e && std::atoi(e) == 0) return false;
+        mallopt(M_TRIM_THRESHOLD, INT_MAX);
+        mallopt(M_TOP_PAD, 64 << 20);
+        mallopt(M_MMAP_THRESHOLD, 32 << 20);
+        return true;
+    }();
+    (void)heap_keep;
     auto t0 = clk::now();
     bf.pf_k = o.kernel ? 0.0 : o.k;  // P blocks of the point kernel can be recomputed
     Ctx &ctx = *bf.ctx;

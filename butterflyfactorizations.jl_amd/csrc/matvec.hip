@@ -443,10 +443,20 @@ void pack_tables(BF &bf) {
                 }
             });
         }
-        int64_t fe = 0;
-        for (const MvContrib &c : con)
-            if (c.fofs >= 0) fe += int64_t(c.m) * c.n;
-        S.factor_elems = fe;
+        {  // factor entries read, summed in parallel over chunks
+            constexpr int64_t CH = int64_t(1) << 16;
+            const int64_t nk = int64_t(con.size()), nch = (nk + CH - 1) / CH;
+            std::vector<int64_t> part(size_t(std::max<int64_t>(nch, 1)), 0);
+            parallel_for(nch, [&](int64_t q) {
+                int64_t f = 0;
+                for (int64_t i = q * CH; i < std::min(nk, (q + 1) * CH); ++i)
+                    if (con[size_t(i)].fofs >= 0) f += int64_t(con[size_t(i)].m) * con[size_t(i)].n;
+                part[size_t(q)] = f;
+            }, 1);
+            int64_t fe = 0;
+            for (int64_t f : part) fe += f;
+            S.factor_elems = fe;
+        }
         if (fwd) {
             // merge within each item, then compact
             std::vector<int32_t> kept(items.size() + 1, 0);
@@ -476,37 +486,78 @@ void pack_tables(BF &bf) {
     auto split = [&](StageTable &S) -> int64_t {
         constexpr int64_t CHUNK = 4096;
         auto elems = [](const MvContrib &c) { return c.fofs < 0 ? int64_t(c.m) : int64_t(c.m) * c.n; };
-        hvec<MvItem> mi, pi;
-        hvec<MvContrib> mc, pc;
-        int64_t cur = 0;
-        for (const MvItem &it : S.h_items) {
+        const int64_t ni = int64_t(S.h_items.size());
+        // chunk boundaries of item i: contributions [start, c] close a chunk
+        // when its entries reach CHUNK or the item ends
+        auto is_split = [&](const MvItem &it) {
+            if (it.c_end - it.c_begin <= 2) return false;
             int64_t total = 0;
             for (int32_t c = it.c_begin; c < it.c_end; ++c) total += elems(S.h_contribs[c]);
-            MvItem m = it;
-            m.c_begin = int32_t(mc.size());
-            if (it.c_end - it.c_begin <= 2 || total <= 2 * CHUNK) {
-                mc.insert(mc.end(), S.h_contribs.begin() + it.c_begin, S.h_contribs.begin() + it.c_end);
-            } else {
-                int32_t start = it.c_begin;
-                int64_t acc = 0;
-                for (int32_t c = it.c_begin; c < it.c_end; ++c) {
-                    acc += elems(S.h_contribs[c]);
-                    if (acc < CHUNK && c + 1 < it.c_end) continue;
-                    const int64_t po = bf.part_ofs + cur;
-                    BFG_REQUIRE(po + it.out_len < (int64_t(1) << 31), "pack: partial sums exceed 2^31 entries");
-                    pi.push_back(MvItem{int32_t(po), it.out_len, int32_t(pc.size()), 0});
-                    pc.insert(pc.end(), S.h_contribs.begin() + start, S.h_contribs.begin() + c + 1);
-                    pi.back().c_end = int32_t(pc.size());
-                    mc.push_back(MvContrib{-1, it.out_len, it.out_len, int32_t(po), 0});
-                    cur += it.out_len;
-                    start = c + 1;
-                    acc = 0;
-                }
+            return total > 2 * CHUNK;
+        };
+        auto n_chunks = [&](const MvItem &it) {
+            int64_t n = 0, acc = 0;
+            for (int32_t c = it.c_begin; c < it.c_end; ++c) {
+                acc += elems(S.h_contribs[c]);
+                if (acc < CHUNK && c + 1 < it.c_end) continue;
+                ++n;
+                acc = 0;
             }
-            m.c_end = int32_t(mc.size());
-            mi.push_back(m);
+            return n;
+        };
+        // first pass (parallel): per item, its chunks (0: kept whole)
+        std::vector<int64_t> nchk(size_t(ni) + 1, 0);
+        parallel_for(ni, [&](int64_t i) {
+            const MvItem &it = S.h_items[size_t(i)];
+            nchk[size_t(i) + 1] = is_split(it) ? n_chunks(it) : 0;
+        }, 1024);
+        // running offsets: main contributions, partial items, partial contributions, partial entries
+        std::vector<int64_t> omc(size_t(ni) + 1, 0), opi(size_t(ni) + 1, 0), opc(size_t(ni) + 1, 0),
+            opo(size_t(ni) + 1, 0);
+        for (int64_t i = 0; i < ni; ++i) {
+            const MvItem &it = S.h_items[size_t(i)];
+            const int64_t k = nchk[size_t(i) + 1], nc = it.c_end - it.c_begin;
+            omc[size_t(i) + 1] = omc[size_t(i)] + (k ? k : nc);
+            opi[size_t(i) + 1] = opi[size_t(i)] + k;
+            opc[size_t(i) + 1] = opc[size_t(i)] + (k ? nc : 0);
+            opo[size_t(i) + 1] = opo[size_t(i)] + k * it.out_len;
         }
-        if (pi.empty()) return 0;
+        if (opi[size_t(ni)] == 0) return 0;
+        BFG_REQUIRE(bf.part_ofs + opo[size_t(ni)] < (int64_t(1) << 31) && omc[size_t(ni)] < (int64_t(1) << 31) &&
+                        opc[size_t(ni)] < (int64_t(1) << 31),
+                    "pack: partial sums exceed 2^31 entries");
+        hvec<MvItem> mi(static_cast<size_t>(ni)), pi(static_cast<size_t>(opi[size_t(ni)]));
+        hvec<MvContrib> mc(static_cast<size_t>(omc[size_t(ni)])), pc(static_cast<size_t>(opc[size_t(ni)]));
+        // second pass (parallel): write every item at its offsets
+        parallel_for(ni, [&](int64_t i) {
+            const MvItem &it = S.h_items[size_t(i)];
+            MvItem m = it;
+            m.c_begin = int32_t(omc[size_t(i)]);
+            m.c_end = int32_t(omc[size_t(i) + 1]);
+            mi[size_t(i)] = m;
+            if (nchk[size_t(i) + 1] == 0) {
+                std::copy(S.h_contribs.begin() + it.c_begin, S.h_contribs.begin() + it.c_end,
+                          mc.begin() + omc[size_t(i)]);
+                return;
+            }
+            int64_t q = opi[size_t(i)], wm = omc[size_t(i)], wp = opc[size_t(i)], po = bf.part_ofs + opo[size_t(i)];
+            int32_t start = it.c_begin;
+            int64_t acc = 0;
+            for (int32_t c = it.c_begin; c < it.c_end; ++c) {
+                acc += elems(S.h_contribs[c]);
+                if (acc < CHUNK && c + 1 < it.c_end) continue;
+                const int64_t n = c + 1 - start;
+                pi[size_t(q)] = MvItem{int32_t(po), it.out_len, int32_t(wp), int32_t(wp + n)};
+                std::copy(S.h_contribs.begin() + start, S.h_contribs.begin() + c + 1, pc.begin() + wp);
+                mc[size_t(wm)] = MvContrib{-1, it.out_len, it.out_len, int32_t(po), 0};
+                ++q;
+                ++wm;
+                wp += n;
+                po += it.out_len;
+                start = c + 1;
+                acc = 0;
+            }
+        }, 1024);
         S.pre = std::make_unique<StageTable>();
         S.pre->in_kind = S.in_kind;
         S.pre->out_kind = OUT_SLOT;
@@ -514,7 +565,7 @@ void pack_tables(BF &bf) {
         S.pre->h_contribs.swap(pc);
         S.h_items.swap(mi);
         S.h_contribs.swap(mc);
-        return cur;
+        return opo[size_t(ni)];
     };
     // p-quantile of a histogram over power-of-two buckets: the smallest
     // 2^b with at least q of the counted values <= 2^b
