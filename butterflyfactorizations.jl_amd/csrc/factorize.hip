@@ -424,7 +424,8 @@ struct Engine {
             }, 1 << 14);
             return;
         }
-        for (auto &c : cs) {
+        parallel_for(int64_t(cs.size()), [&](int64_t i) {
+            Comp &c = cs[size_t(i)];
             if (o.rank_mode == BFGPU_RANK_GEOMETRIC) {
                 const double *csrc = &trial.center[3 * c.S], *cobs = &test.center[3 * c.O];
                 double d = std::sqrt((csrc[0] - cobs[0]) * (csrc[0] - cobs[0]) +
@@ -438,7 +439,7 @@ struct Engine {
                 int32_t R = int32_t(std::ceil(o.c_tau * std::log(1.0 / c.eps)));
                 c.guess = std::max(R, o.rmin);
             }
-        }
+        }, 1 << 14);
     }
 
     // Held-out rows the residual judge checks compression c on: the next

@@ -792,14 +792,14 @@ void pack_tables(BF &bf) {
                 }
             }, 256);
             {  // the transposed kernel writes each column's slot: one writer per slot
-                std::vector<bool> seen(size_t(std::max<int64_t>(counter, 1)), false);  // a bit per slot
-                bool once = true;
-                for (int64_t i = 0; i < total && once; ++i) {
+                std::unique_ptr<std::atomic<uint8_t>[]> seen(new std::atomic<uint8_t>[size_t(std::max<int64_t>(counter, 1))]);
+                parallel_for(std::max<int64_t>(counter, 1), [&](int64_t k) { seen[size_t(k)].store(0, std::memory_order_relaxed); }, 1 << 16);
+                std::atomic<bool> once{true};
+                parallel_for(total, [&](int64_t i) {
                     const size_t k = size_t(cin[size_t(i)]);
-                    once = !seen[k];
-                    seen[k] = true;
-                }
-                bf.pfree_t_ok = once;
+                    if (seen[k].exchange(1, std::memory_order_relaxed)) once.store(false, std::memory_order_relaxed);
+                }, 1 << 16);
+                bf.pfree_t_ok = once.load();
             }
             upload_tab(bf.pf_items, items);
             upload_tab(bf.pf_col, col);
