@@ -552,14 +552,15 @@ class ButterflyFactorization:
         return self
 
     OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7,
-               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11, "tcopy": 12}
+               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11, "tcopy": 12, "stream": 13}
 
     @staticmethod
-    def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0):
+    def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0, snld4=0):
         """Packed value of the "tile" option (and of "dense" with g_shift 0,
-        rp_shift = log2 of the lanes per task)."""
+        rp_shift = log2 of the lanes per task; snld4 1 selects 4 KiB chunks
+        for the streaming kernel)."""
         return (int(rp_shift) | int(g_shift) << 4 | int(csplit) << 8 | int(wg_per_cu) << 12
-                | int(xcd) << 20 | int(qw4) << 24)
+                | int(xcd) << 20 | int(qw4) << 24 | int(snld4) << 28)
 
     def set_option(self, key, value):
         """Launch tuning: sweep (0 per-level, 1 fused dynamic, 2 fused
@@ -570,7 +571,9 @@ class ButterflyFactorization:
         rhs_lanes (fewest right-hand sides put across lanes), mfma (1:
         FP32 right-hand sides across lanes on the matrix cores), rhs_chunk
         (most right-hand sides per pass; 0: as device memory allows),
-        pfree (1: compute the last level's point-kernel entries), slot_cap
+        pfree (1: compute the last level's point-kernel entries), stream
+        (4 or 8: forward levels on the streaming kernel with chunks of that
+        many KiB at one RHS; 0: dense kernel), slot_cap
         (most KiB of skeleton vectors per pass), tcopy (1: transposed copy
         of the stored blocks for the adjoint / transpose, built when device
         memory allows and used where the autotuner finds it faster; 0:
@@ -619,6 +622,16 @@ class ButterflyFactorization:
         check(lib().bfgpu_bf_launch_stats(self.handle, self.op, C.byref(n), None))
         st = np.zeros((n.value, 10), np.int64)
         check(lib().bfgpu_bf_launch_stats(self.handle, self.op, C.byref(n), ptr(st, C.c_int64)))
+        return st
+
+    def stream_stats(self):
+        """Per forward level: streaming chunk KiB (0: tables not built),
+        chunks, tasks in chunks, tasks left to the dense kernel, factor
+        entries copied that belong to no block, input entries copied."""
+        n = C.c_int32()
+        check(lib().bfgpu_bf_stream_stats(self.handle, C.byref(n), None))
+        st = np.zeros((n.value, 6), np.int64)
+        check(lib().bfgpu_bf_stream_stats(self.handle, C.byref(n), ptr(st, C.c_int64)))
         return st
 
     def stage_stats(self):

@@ -104,6 +104,7 @@ _SIGS = {
     "bfgpu_bf_compressions": (i32, [vp, P(i64), P(i32), P(f32)]),
     "bfgpu_bf_stage_stats": (i32, [vp, P(i32), P(i64)]),
     "bfgpu_bf_launch_stats": (i32, [vp, i32, P(i32), P(i64)]),
+    "bfgpu_bf_stream_stats": (i32, [vp, P(i32), P(i64)]),
     "bfgpu_bf_convert": (i32, [vp, i32, P(vp)]),
     "bfgpu_matvec": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64), i32]),
     "bfgpu_matvec_noalloc": (i32, [vp, i32, i64, vp, i64, vp, i64, P(f64), P(f64)]),

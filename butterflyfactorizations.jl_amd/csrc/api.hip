@@ -479,6 +479,28 @@ extern "C" int bfgpu_bf_stage_stats(const bfgpu_bf *h, int32_t *n_stages, int64_
     BFG_API_END
 }
 
+extern "C" int bfgpu_bf_stream_stats(const bfgpu_bf *h, int32_t *n_levels, int64_t *stats) {
+    BFG_API_BEGIN
+    BFG_REQUIRE(h && n_levels, "bf_stream_stats: null argument");
+    const BF &bf = h->b;
+    *n_levels = bf.n_stage_launch;
+    if (!stats) return BFGPU_OK;
+    for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
+        int64_t *o = stats + 6 * L;
+        std::fill(o, o + 6, int64_t(0));
+        for (const StageTable *u : {&bf.fwd[L], static_cast<const StageTable *>(bf.fwd[L].pre.get())}) {
+            if (!u || !u->stream_built) continue;
+            o[0] = u->stream_nld;
+            o[1] += u->n_schunks;
+            o[2] += u->n_stasks;
+            o[3] += u->n_srest;
+            o[4] += u->stream_gap;
+            o[5] += u->stream_xent;
+        }
+    }
+    BFG_API_END
+}
+
 extern "C" int bfgpu_bf_launch_stats(const bfgpu_bf *h, int32_t op, int32_t *n_levels, int64_t *stats) {
     BFG_API_BEGIN
     BFG_REQUIRE(h && n_levels, "bf_launch_stats: null argument");
@@ -651,14 +673,18 @@ extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) 
                     continue;
                 }
                 const TileShape sh = unpack_tile(p);
-                BFG_REQUIRE(((v >= 0 && v <= 3) || (v == 8 && q >= 2)) && !(q >= 2 && v == 1) && p >= 0 &&
-                                p < (1 << 28) && tile_ok(sh) && !(v >= 3 && sh.g_shift != 0),
+                BFG_REQUIRE(((v >= 0 && v <= 3) || (v == 8 && q >= 2) || (v == 9 && q < 2)) &&
+                                !(q >= 2 && v == 1) && p >= 0 &&
+                                p < TILE_PACK_LIMIT && tile_ok(sh) && !(v >= 3 && sh.g_shift != 0),
                             "bf_set_tuning: bad variant or shape");
                 BFG_REQUIRE(v != 8 || (bf.tcopy_built && (bf.factor_elems[L] == 0 ||
                                                           (L < int32_t(bf.factors_t.size()) && bf.factors_t[L].p))),
                             "bf_set_tuning: the transposed copy of a level is not available (tcopy off, or "
                             "device memory short)");
                 if (pass == 1) {
+                    if (v == 9)
+                        build_stream(*t[q], bf.precision == BFGPU_C128 ? 16 : 8, sh.snld4 ? 4 : 8,
+                                     bf.factor_elems[L]);
                     t[q]->variant = v;
                     t[q]->tile = sh;
                 }
@@ -714,7 +740,7 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         break;
     case BFGPU_OPT_TILE: {
         const TileShape sh = unpack_tile(value);
-        BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh), "bf_set_option: bad tile shape");
+        BFG_REQUIRE(value >= 0 && value < TILE_PACK_LIMIT && tile_ok(sh), "bf_set_option: bad tile shape");
         for (int dir = 0; dir < 2; ++dir)
             for (StageTable &t : dir == 0 ? bf.fwd : bf.tr) {
                 t.variant = 2;
@@ -759,6 +785,27 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
             ensure_tcopy(bf);
         }
         break;
+    case BFGPU_OPT_STREAM: {
+        BFG_REQUIRE(value == 0 || value == 4 || value == 8, "bf_set_option: stream is 0, 4 or 8");
+        const int es = bf.precision == BFGPU_C128 ? 16 : 8;
+        for (int32_t L = 0; L < bf.n_stage_launch; ++L)
+            for (StageTable *u : {&bf.fwd[L], bf.fwd[L].pre.get()}) {
+                if (!u) continue;
+                if (value == 0) {
+                    if (u->variant == 9) u->variant = 3;
+                    continue;
+                }
+                if (u->variant != 3 && u->variant != 9) {  // dense shape for the tasks that do not stream
+                    u->tile.rp_shift = int8_t(std::max<int>(4, std::min<int>(6, u->tile.rp_shift + u->tile.g_shift)));
+                    u->tile.g_shift = 0;
+                }
+                u->tile.snld4 = int8_t(value == 4 ? 1 : 0);
+                build_stream(*u, es, value, bf.factor_elems[L]);
+                u->variant = 9;
+            }
+        release_graph(bf);
+        break;
+    }
     case BFGPU_OPT_PFREE:
         BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: pfree is 0 or 1");
         bf.use_pfree = value != 0;
@@ -766,7 +813,7 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         break;
     case BFGPU_OPT_DENSE: {
         const TileShape sh = unpack_tile(value);
-        BFG_REQUIRE(value >= 0 && value < (1 << 28) && tile_ok(sh) && sh.g_shift == 0,
+        BFG_REQUIRE(value >= 0 && value < TILE_PACK_LIMIT && tile_ok(sh) && sh.g_shift == 0,
                     "bf_set_option: bad dense shape");
         for (int dir = 0; dir < 2; ++dir)
             for (StageTable &t : dir == 0 ? bf.fwd : bf.tr)
@@ -809,6 +856,17 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
     case BFGPU_OPT_SLOT_CAP: *value = bf.slot_cap_kib; break;
     case BFGPU_OPT_TCOPY: *value = bf.tcopy_built ? 1 : 0; break;
+    case BFGPU_OPT_STREAM: {
+        int32_t v = -1;
+        for (const StageTable &t : bf.fwd)
+            for (const StageTable *u : {&t, static_cast<const StageTable *>(t.pre.get())}) {
+                if (!u) continue;
+                const int32_t w = u->variant == 9 ? (u->tile.snld4 ? 4 : 8) : 0;
+                v = v < 0 || v == w ? w : 0;
+            }
+        *value = std::max(v, 0);
+        break;
+    }
     case BFGPU_OPT_PFREE: *value = bf.use_pfree && bf.pfree_ok ? 1 : 0; break;
     default: BFG_REQUIRE(false, "bf_get_option: unknown key");
     }

@@ -167,6 +167,41 @@ struct MvTaskC {
 };
 static_assert(sizeof(MvTaskC) == 32, "MvTaskC is 8 dwords");
 
+// Streaming forward kernel (one right-hand side).  A chunk is a run of
+// consecutive single-block tasks whose blocks lie back to back in the
+// level's factor buffer; a wavefront copies the chunk's factor range (nw
+// 16-byte words from word f0w) and the distinct input vectors its tasks read
+// (X entries) into its own LDS region and computes the chunk from there.
+//   w0    = f0w | nw << 40 | X << 51
+//   info  = nt | H << 7 | G << 14: tasks, row pairs, lane groups per row pair
+//   rmask = bit p set where a task's row pairs start (its first is pair p)
+//   xmask = bit e (word e / 64) set where a distinct input vector starts in
+//           the chunk's LDS input region (entry e)
+struct StreamChunk {
+    uint64_t w0;
+    int32_t t0;       // first task (index into the stream task array)
+    uint32_t info;
+    uint64_t rmask;
+    uint64_t xmask[4];
+    uint64_t pad;
+};
+static_assert(sizeof(StreamChunk) == 64, "StreamChunk is 16 dwords");
+// Task t of a chunk: out_ofs its output slot; a, xo the LDS offsets
+// (entries) of its block and of its input vector; m, n its block; hp the
+// chunk's row pairs before it.  (lin, key): the slot offset and the LDS
+// offset of the chunk's t-th distinct input vector (when it has one).
+//   ax  = a | xo << 11 | key << 21
+//   mhn = m | hp << 8 | n << 15
+struct StreamTask {
+    int32_t lin, out_ofs;
+    uint32_t ax, mhn;
+};
+static_assert(sizeof(StreamTask) == 16, "StreamTask is 4 dwords");
+// Input entries per chunk in LDS (1.5 KiB in FP32, 2 KiB in FP64; a
+// multiple of 64 and at most 256, the bits of xmask)
+template <typename R> constexpr int stream_xcap() { return sizeof(R) == 4 ? 192 : 128; }
+constexpr int STREAM_HMAX = 32;  // row pairs per chunk: at least two lane groups per row pair
+
 enum InKind { IN_SLOT = 0, IN_GATHER = 1 };
 enum OutKind { OUT_SLOT = 0, OUT_SCATTER = 1 };
 
@@ -180,11 +215,13 @@ struct TileShape {
     int16_t wg_per_cu = 8;
     int8_t xcd = 0;  // 1: each XCD walks one contiguous range of tasks
     int8_t qw4 = 0;  // dense kernels, one RHS: 4 loads per lane and pass (else 8)
+    int8_t snld4 = 0;  // streaming kernel: chunks of 4 KiB (else 8 KiB)
 };
 inline int64_t pack_tile(const TileShape &t) {
     return int64_t(t.rp_shift) | int64_t(t.g_shift) << 4 | int64_t(t.csplit) << 8 | int64_t(t.wg_per_cu) << 12 |
-           int64_t(t.xcd) << 20 | int64_t(t.qw4) << 24;
+           int64_t(t.xcd) << 20 | int64_t(t.qw4) << 24 | int64_t(t.snld4) << 28;
 }
+constexpr int64_t TILE_PACK_LIMIT = int64_t(1) << 29;  // packed shapes are below this
 inline TileShape unpack_tile(int64_t v) {
     TileShape t;
     t.rp_shift = int8_t(v & 15);
@@ -193,13 +230,14 @@ inline TileShape unpack_tile(int64_t v) {
     t.wg_per_cu = int16_t((v >> 12) & 255);
     t.xcd = int8_t((v >> 20) & 15);
     t.qw4 = int8_t((v >> 24) & 15);
+    t.snld4 = int8_t((v >> 28) & 1);
     return t;
 }
 // at least 4 lanes per task (a wave's task group is at most 16 descriptors)
 inline bool tile_ok(const TileShape &t) {
     return t.rp_shift >= 0 && t.g_shift >= 0 && t.rp_shift + t.g_shift >= 2 && t.rp_shift + t.g_shift <= 6 &&
            t.csplit <= 1 &&
-           t.wg_per_cu >= 1 && t.wg_per_cu <= 64 && t.xcd <= 1 && t.qw4 <= 1;
+           t.wg_per_cu >= 1 && t.wg_per_cu <= 64 && t.xcd <= 1 && t.qw4 <= 1 && t.snld4 <= 1;
 }
 
 struct StageTable {
@@ -236,6 +274,16 @@ struct StageTable {
     int32_t n_mtiles = 0;
     int32_t mtile_ofs[5] = {0, 0, 0, 0, 0};  // tiles of 1, 2, 3, 4 row blocks: [ofs[rb-1], ofs[rb])
     DevBuf mtiles, mtasks;
+    // streaming kernel (variant 9, forward, one RHS; built on first use for
+    // chunks of stream_nld 1 KiB rounds): chunks over the tasks it takes,
+    // and the other tasks (several contributions, identity blocks, blocks
+    // too large for a chunk) as a dense-kernel task list
+    bool stream_built = false;
+    int32_t stream_nld = 0;
+    int32_t n_schunks = 0, n_stasks = 0, n_srest = 0;
+    int64_t stream_gap = 0;   // entries copied that belong to no block
+    int64_t stream_xent = 0;  // input entries copied (distinct per chunk)
+    DevBuf s_chunks, s_tasks, s_rest;
 };
 
 constexpr int SWEEP_MAX_STAGES = 48;
@@ -319,6 +367,8 @@ constexpr int64_t FACTOR_PAD = 16;
 void pack_tables(BF &bf);
 // Fused-sweep tables, built on first use (matvec.hip).
 void build_sweep(BF &bf);
+// Streaming-kernel tables of a forward table (matvec.hip); es: entry bytes.
+void build_stream(StageTable &S, int es, int nld, int64_t factor_elems);
 
 // Host worker threads (OMP_NUM_THREADS caps the hardware count).
 int host_threads();
@@ -430,10 +480,17 @@ void launch_tile(hipStream_t s, int prec, int op, const StageTable &t, const voi
 // Forward dense kernel over table t; with swapped true it reads the
 // swapped task tables (t.tasks_sw, t.contribs_sw) and `factors` is the
 // transposed copy; conj conjugates the factor entries.
+// tab: the table's own tasks (DENSE_TASKS), the tasks over the transposed
+// copy (DENSE_SWAPPED), or the tasks the streaming kernel leaves (DENSE_REST)
+enum DenseTab { DENSE_TASKS = 0, DENSE_SWAPPED = 1, DENSE_REST = 2 };
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu,
-                  bool swapped = false, int conj = 0);
+                  int tab = DENSE_TASKS, int conj = 0);
+// Streaming forward kernel over the chunks of a table (one RHS); the
+// table's other tasks are not launched here.
+void launch_stream(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
+                   int64_t in_shift, int n_cu);
 // Block transposition: for block q, dst[fofs + i n + k] = src[fofs + k m + i]
 // (int4 {fofs lo, fofs hi, m, n} per block).
 void launch_transpose_blocks(hipStream_t s, int prec, const int4 *blocks, int64_t n_blocks, const void *src,

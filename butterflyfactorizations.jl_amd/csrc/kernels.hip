@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "device.hpp"
 #include "internal.hpp"
@@ -1344,6 +1345,210 @@ __global__ __launch_bounds__(256) void dense_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Streaming forward kernel (one right-hand side, single-block tasks).
+// Blocks of consecutive tasks lie back to back in the factor buffer, so a
+// chunk of tasks is one contiguous range of at most NLD KiB.  Each wavefront
+// owns an LDS region; per chunk it
+//   1. copies the range with aligned 16-byte loads (word q * 64 + lane in
+//      round q: every load instruction reads 1 KiB of consecutive bytes),
+//      and the chunk's distinct input vectors (an entry per lane and round;
+//      the entry's vector is the number of vector starts at or below it in
+//      the chunk's xmask, minus one), into registers, then into its LDS
+//      region;
+//   2. computes the chunk from LDS: lane (g, rho) takes row pair rho of the
+//      chunk (rows 2 i, 2 i + 1 of the task that holds it: the number of
+//      task starts at or below rho in rmask, minus one) over the columns
+//      k = g, g + G, ...; the G groups meet in a fixed shuffle tree and
+//      group 0 writes the outputs.
+// The copy of chunk c + 1 is issued before chunk c is computed, its task
+// descriptors one chunk earlier and its chunk record two chunks earlier, so
+// the loads of the next chunk are in flight while the current one is
+// computed.  No barriers: a wavefront reads only what it wrote (LDS
+// instructions of one wavefront complete in order).  The second row of an
+// odd-height block reads one entry past it (the region has a one-word pad)
+// and is dropped.
+// ---------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ C<R> lds_c(const C<R> *p) {  // one entry, 8-byte aligned
+    using V = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+    const V v = *reinterpret_cast<const V *>(__builtin_assume_aligned(p, 8));
+    return C<R>{v.x, v.y};
+}
+
+template <typename R, int NLD>
+__global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restrict__ chunks, int32_t n_chunks,
+                                                     const StreamTask *__restrict__ tasks,
+                                                     const C<R> *__restrict__ F, C<R> *__restrict__ slots,
+                                                     int64_t in_shift, int xcd_map) {
+    constexpr int NW = NLD * 64;  // words per chunk
+    constexpr int XCAP = stream_xcap<R>();
+    constexpr int XLD = XCAP / 64;
+    static_assert(XCAP % 64 == 0 && XLD <= 4, "stream_kernel: input region");
+    __shared__ uint4 fbuf[2][NW + 1];
+    __shared__ C<R> xbuf[2][XCAP];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    uint4 *fw = fbuf[wv];
+    const C<R> *fl = reinterpret_cast<const C<R> *>(fbuf[wv]);
+    C<R> *xl = xbuf[wv];
+    const uint4 *FW = reinterpret_cast<const uint4 *>(F);
+    const C<R> *in = slots + in_shift;
+    const uint64_t at_or_below = ~uint64_t(0) >> (63 - lane);  // bits 0..lane
+    int64_t c, c_hi, stride;
+    if (xcd_map && gridDim.x >= 8) {  // workgroup b runs on XCD b % 8: each XCD walks one eighth
+        const int64_t nwg = gridDim.x, x = blockIdx.x % 8;
+        stride = ((nwg - x + 7) / 8) * 2;
+        c = int64_t(n_chunks) * x / 8 + (int64_t(blockIdx.x) / 8) * 2 + wv;
+        c_hi = int64_t(n_chunks) * (x + 1) / 8;
+    } else {
+        stride = int64_t(gridDim.x) * 2;
+        c = int64_t(blockIdx.x) * 2 + wv;
+        c_hi = n_chunks;
+    }
+    if (c >= c_hi) return;
+
+    struct Rec {
+        int64_t f0w;
+        int32_t nw, X, t0, nt, H, G;
+        uint64_t rm, xm[XLD];
+    };
+    auto rec = [&](int64_t ci) -> Rec {  // nt = 0 past the wave's last chunk
+        Rec r;
+        r.f0w = 0;
+        r.nw = r.X = r.t0 = r.nt = 0;
+        r.H = r.G = 1;
+        r.rm = 1;
+#pragma unroll
+        for (int q = 0; q < XLD; ++q) r.xm[q] = 0;
+        if (ci < c_hi) {
+            const StreamChunk &k = chunks[ci];
+            r.f0w = int64_t(k.w0 & ((uint64_t(1) << 40) - 1));
+            r.nw = int32_t((k.w0 >> 40) & 2047);
+            r.X = int32_t((k.w0 >> 51) & 2047);
+            r.t0 = k.t0;
+            r.nt = int32_t(k.info & 127);
+            r.H = int32_t((k.info >> 7) & 127);
+            r.G = int32_t((k.info >> 14) & 127);
+            r.rm = k.rmask;
+#pragma unroll
+            for (int q = 0; q < XLD; ++q) r.xm[q] = k.xmask[q];
+        }
+        return r;
+    };
+    auto desc = [&](const Rec &r) -> uint4 {  // lane t holds task t of the chunk
+        return lane < r.nt ? reinterpret_cast<const uint4 *>(tasks)[r.t0 + lane] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto issue = [&](const Rec &r, const uint4 &d, uint4(&V)[NLD], C<R>(&XV)[XLD]) {
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int wi = q * 64 + lane;
+            if (wi < r.nw) V[q] = FW[r.f0w + wi];
+        }
+        int before = 0;  // input vectors starting in earlier rounds
+#pragma unroll
+        for (int q = 0; q < XLD; ++q) {
+            if (q * 64 >= r.X) break;
+            const int e = q * 64 + lane;
+            const int v = before + __popcll(r.xm[q] & at_or_below) - 1;
+            before += __popcll(r.xm[q]);
+            const int kv = __shfl(int((d.z >> 21) & 1023), v, 64), lv = __shfl(int(d.x), v, 64);
+            if (e < r.X) XV[q] = in[int64_t(lv) + (e - kv)];
+        }
+    };
+    auto store = [&](const Rec &r, const uint4(&V)[NLD], const C<R>(&XV)[XLD]) {
+#pragma unroll
+        for (int q = 0; q < NLD; ++q)
+            if (q * 64 < r.nw) fw[q * 64 + lane] = V[q];
+#pragma unroll
+        for (int q = 0; q < XLD; ++q)
+            if (q * 64 < r.X) xl[q * 64 + lane] = XV[q];
+    };
+    auto compute = [&](const Rec &r, const uint4 &d) {
+        const int H = r.H, G = r.G;
+        // g = lane / H through a reciprocal: exact for operands below 2^7
+        const int g = int((float(lane) + 0.5f) * __builtin_amdgcn_rcpf(float(H)));
+        const int rho = lane - g * H;
+        const int t = __popcll(r.rm & (~uint64_t(0) >> (63 - rho))) - 1;
+        const uint32_t tz = __shfl(d.z, t, 64), tw = __shfl(d.w, t, 64);
+        const int oo = __shfl(int(d.y), t, 64);
+        const int a = int(tz & 2047), xo = int((tz >> 11) & 1023);
+        const int m = int(tw & 255), hp = int((tw >> 8) & 127), n = int(tw >> 15);
+        const int i = 2 * (rho - hp);
+        C<R> acc0{R(0), R(0)}, acc1{R(0), R(0)};
+        if (g < G) {
+            const int step = G * m;
+            const C<R> *pa = fl + a + i + g * m;
+            const C<R> *px = xl + xo + g;
+            int k = g;
+            for (; k + 3 * G < n; k += 4 * G) {
+                C<R> a0[4], a1[4], xv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a0[u] = lds_c(pa + u * step);
+                    a1[u] = lds_c(pa + u * step + 1);
+                    xv[u] = lds_c(px + u * G);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cfma(acc0, a0[u], xv[u]);
+                    cfma(acc1, a1[u], xv[u]);
+                }
+                pa += 4 * step;
+                px += 4 * G;
+            }
+            for (; k < n; k += G) {
+                const C<R> x = lds_c(px);
+                cfma(acc0, lds_c(pa), x);
+                cfma(acc1, lds_c(pa + 1), x);
+                pa += step;
+                px += G;
+            }
+        }
+        for (int sft = 1; sft < G; sft <<= 1) {
+            const int src = min(lane + sft * H, 63);
+            const bool take = (g & (2 * sft - 1)) == 0 && g + sft < G;
+            const R o0x = __shfl(acc0.x, src, 64), o0y = __shfl(acc0.y, src, 64);
+            const R o1x = __shfl(acc1.x, src, 64), o1y = __shfl(acc1.y, src, 64);
+            if (take) {
+                acc0.x += o0x;
+                acc0.y += o0y;
+                acc1.x += o1x;
+                acc1.y += o1y;
+            }
+        }
+        if (g == 0) {  // lane rho < H
+            C<R> *o = slots + int64_t(oo) + i;
+            o[0] = acc0;
+            if (i + 1 < m) o[1] = acc1;
+        }
+    };
+
+    uint4 V[NLD];
+    C<R> XV[XLD];
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) V[q] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int q = 0; q < XLD; ++q) XV[q] = C<R>{R(0), R(0)};
+    Rec r0 = rec(c), r1 = rec(c + stride), r2 = rec(c + 2 * stride);
+    uint4 d0 = desc(r0), d1 = desc(r1);
+    issue(r0, d0, V, XV);
+    for (;;) {
+        store(r0, V, XV);
+        if (r1.nt > 0) issue(r1, d1, V, XV);
+        const uint4 d2 = desc(r2);
+        const Rec r3 = rec(c + 3 * stride);
+        compute(r0, d0);
+        if (r1.nt == 0) break;
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        d0 = d1;
+        d1 = d2;
+        c += stride;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dense transposed kernel (adjoint / transpose): out[j] = sum_i op(A[i, j])
 // in[i].  Lane l of a task owns output column j = j0 + l (LPT outputs per
 // pass), so its accumulator persists over all contributions and no
@@ -2160,22 +2365,24 @@ template <typename R, int NR, int QW>
 static void launch_dense_t(hipStream_t s, const StageTable &t, const void *factors, void *slots,
                            int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                            int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu,
-                           bool swapped, int conj) {
+                           int tab, int conj) {
     C<R> a = {R(alpha[0]), R(alpha[1])}, b = {R(beta[0]), R(beta[1])};
     const int lpt = t.tile.rp_shift;
     BFG_REQUIRE(tile_ok(t.tile) && t.tile.g_shift == 0, "matvec: bad dense shape");
+    const int32_t n_tasks = tab == DENSE_REST ? t.n_srest : t.n_items;
+    if (n_tasks == 0) return;
     const int64_t tpw = 64 >> lpt;
-    const int64_t wgs = ((int64_t(t.n_items) + tpw - 1) / tpw + 3) / 4;
+    const int64_t wgs = ((int64_t(n_tasks) + tpw - 1) / tpw + 3) / 4;
     const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
     dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))), unsigned((nrhs + NR - 1) / NR));
     auto F = (const C<R> *)factors;
     auto S = (C<R> *)slots;
     auto Y = (C<R> *)y;
-    auto T = (swapped ? t.tasks_sw : t.tasks_c).as<const MvTaskC>();
-    auto P = (swapped ? t.contribs_sw : t.contribs).as<const MvContrib>();
+    auto T = (tab == DENSE_SWAPPED ? t.tasks_sw : tab == DENSE_REST ? t.s_rest : t.tasks_c).as<const MvTaskC>();
+    auto P = (tab == DENSE_SWAPPED ? t.contribs_sw : t.contribs).as<const MvContrib>();
     BFG_REQUIRE(T != nullptr && P != nullptr, "matvec: dense tables missing");
 #define BFG_DENSE(OUTK, CS)                                                                              \
-    hipLaunchKernelGGL((dense_kernel<R, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, t.n_items, P, F, S,  \
+    hipLaunchKernelGGL((dense_kernel<R, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, n_tasks, P, F, S,  \
                        slot_len, in_shift, Y, ldy, perm_out, a, b, int(fuse), int(nrhs), lpt, int(t.tile.xcd), conj)
     if (t.out_kind == OUT_SLOT) {
         if (t.tile.csplit) BFG_DENSE(OUT_SLOT, true);
@@ -2192,7 +2399,7 @@ template <typename R>
 static void launch_dense_nr(hipStream_t s, const StageTable &t, const void *factors, void *slots,
                             int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                             int64_t nrhs, const double *alpha, const double *beta, bool fuse, int n_cu,
-                            bool sw, int conj) {
+                            int sw, int conj) {
     if (nrhs == 1) {
         if (t.tile.qw4) launch_dense_t<R, 1, 4>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
         else launch_dense_t<R, 1, 8>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta, fuse, n_cu, sw, conj);
@@ -2257,18 +2464,43 @@ void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const v
 #undef BFG_DT
 }
 
+template <typename R, int NLD>
+static void launch_stream_t(hipStream_t s, const StageTable &t, const void *factors, void *slots, int64_t in_shift,
+                            int n_cu) {
+    const int64_t wgs = (int64_t(t.n_schunks) + 1) / 2;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
+    hipLaunchKernelGGL((stream_kernel<R, NLD>), grid, dim3(128), 0, s, t.s_chunks.as<const StreamChunk>(),
+                       t.n_schunks, t.s_tasks.as<const StreamTask>(), (const C<R> *)factors, (C<R> *)slots,
+                       in_shift, int(t.tile.xcd));
+    BFG_HIP(hipGetLastError());
+}
+
+void launch_stream(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
+                   int64_t in_shift, int n_cu) {
+    BFG_REQUIRE(t.stream_built && (t.stream_nld == 4 || t.stream_nld == 8), "matvec: streaming tables missing");
+    if (t.n_schunks == 0) return;
+    if (prec == BFGPU_C128) {
+        if (t.stream_nld == 4) launch_stream_t<double, 4>(s, t, factors, slots, in_shift, n_cu);
+        else launch_stream_t<double, 8>(s, t, factors, slots, in_shift, n_cu);
+    } else {
+        if (t.stream_nld == 4) launch_stream_t<float, 4>(s, t, factors, slots, in_shift, n_cu);
+        else launch_stream_t<float, 8>(s, t, factors, slots, in_shift, n_cu);
+    }
+}
+
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                   int64_t slot_len, int64_t in_shift, void *y, int64_t ldy, const int32_t *perm_out,
                   int64_t nrhs, const double *alpha, const double *beta, bool fuse_beta, int n_cu,
-                  bool swapped, int conj) {
-    if (t.n_items == 0 || nrhs == 0) return;
+                  int tab, int conj) {
+    if ((tab == DENSE_REST ? t.n_srest : t.n_items) == 0 || nrhs == 0) return;
     BFG_REQUIRE(nrhs < 65536 * 8, "matvec: nrhs too large");
     if (prec == BFGPU_C128)
         launch_dense_nr<double>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta,
-                                fuse_beta, n_cu, swapped, conj);
+                                fuse_beta, n_cu, tab, conj);
     else
         launch_dense_nr<float>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta,
-                               fuse_beta, n_cu, swapped, conj);
+                               fuse_beta, n_cu, tab, conj);
 }
 
 // ---------------------------------------------------------------------------

@@ -826,6 +826,171 @@ void pack_tables(BF &bf) {
                      since(tp0), t_keys, t_slots, t_group, t_split, t_finish, t_pf, t_valid);
 }
 
+// Streaming-kernel tables of a forward table (variant 9).  Consecutive
+// tasks that hold one factor block each (no identity block, output rows =
+// block rows, at most 128 rows), whose blocks follow each other in the
+// factor buffer with gaps of at most 16 entries, are cut into chunks of at
+// most nld KiB of 16-byte words, 64 tasks, STREAM_HMAX row pairs and
+// stream_xcap distinct input entries (tasks that read the same input vector
+// share one LDS copy of it).  Ranges of tasks are chunked in parallel (a
+// chunk never spans two ranges); every other task goes to a dense-kernel
+// list.
+void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
+    if (S.stream_built && S.stream_nld == nld) return;
+    BFG_REQUIRE(nld == 4 || nld == 8, "pack: stream chunks of 4 or 8 KiB");
+    BFG_REQUIRE(es == 8 || es == 16, "pack: entry size 8 or 16 bytes");
+    const int64_t E = 16 / es;                   // entries per word
+    const int64_t cape = int64_t(nld) * 64 * E;  // entries per chunk
+    const int32_t xcap = es == 8 ? stream_xcap<float>() : stream_xcap<double>();
+    constexpr int64_t GAP = 16;
+    const int64_t n = int64_t(S.h_tasks.size());
+    const bool slot_out = S.out_kind == OUT_SLOT;
+    auto streams = [&](const MvTask &t) {
+        return slot_out && t.c_begin == t.c_end && t.fofs >= 0 && t.m >= 1 && t.m <= 2 * STREAM_HMAX && t.n >= 1 &&
+               t.out_len == t.m && int64_t(t.m) * t.n + (E - 1) <= cape && t.n <= xcap;
+    };
+    struct Part {
+        hvec<StreamChunk> ch;
+        hvec<StreamTask> tk;
+        hvec<MvTaskC> rest;
+        int64_t gap = 0, xent = 0;
+    };
+    const int64_t nr = std::max<int64_t>(1, std::min<int64_t>(4 * int64_t(host_threads()), n / 4096 + 1));
+    std::vector<Part> parts(static_cast<size_t>(nr));
+    parallel_for(nr, [&](int64_t r) {
+        Part &P = parts[size_t(r)];
+        const int64_t lo = n * r / nr, hi = n * (r + 1) / nr;
+        struct Run { int32_t in_ofs, n, xo; };
+        Run runs[64];
+        int nruns = 0;
+        bool open = false;
+        int64_t f0w = 0, fend = 0;
+        int32_t t0 = 0, nt = 0, H = 0, X = 0;
+        uint64_t rmask = 0, xmask[4] = {0, 0, 0, 0};
+        auto close = [&]() {
+            if (!open) return;
+            const int64_t nw = (fend - f0w * E + E - 1) / E;
+            StreamChunk k;
+            k.w0 = uint64_t(f0w) | uint64_t(nw) << 40 | uint64_t(X) << 51;
+            k.t0 = t0;  // part-relative until the parts are joined
+            k.info = uint32_t(nt) | uint32_t(H) << 7 | uint32_t(64 / H) << 14;
+            k.rmask = rmask;
+            for (int q = 0; q < 4; ++q) k.xmask[q] = xmask[q];
+            k.pad = 0;
+            P.ch.push_back(k);
+            // task q of the chunk carries the chunk's q-th distinct input vector
+            for (int q = 0; q < nruns; ++q) {
+                StreamTask &st = P.tk[size_t(t0 + q)];
+                st.lin = runs[q].in_ofs;
+                st.ax |= uint32_t(runs[q].xo) << 21;
+            }
+            P.xent += X;
+            open = false;
+        };
+        for (int64_t i = lo; i < hi; ++i) {
+            const MvTask &t = S.h_tasks[size_t(i)];
+            if (!streams(t)) {
+                close();
+                P.rest.push_back(MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.m, t.n, t.c_begin, t.c_end});
+                continue;
+            }
+            const int32_t h = (t.m + 1) / 2;
+            const int64_t end = t.fofs + int64_t(t.m) * t.n;
+            int ri = -1;
+            if (open)
+                for (int q = nruns - 1; q >= 0; --q)
+                    if (runs[q].in_ofs == t.in_ofs && runs[q].n == t.n) {
+                        ri = q;
+                        break;
+                    }
+            const bool fits = open && t.fofs >= fend && t.fofs - fend <= GAP && end - f0w * E <= cape && nt < 64 &&
+                              H + h <= STREAM_HMAX && X + (ri < 0 ? t.n : 0) <= xcap;
+            if (!fits) {
+                close();
+                open = true;
+                f0w = t.fofs / E;
+                fend = f0w * E;
+                t0 = int32_t(P.tk.size());
+                nt = H = X = 0;
+                nruns = 0;
+                rmask = 0;
+                xmask[0] = xmask[1] = xmask[2] = xmask[3] = 0;
+                ri = -1;
+            }
+            P.gap += t.fofs - fend;
+            int32_t xo;
+            if (ri < 0) {  // a new distinct input vector at LDS entry X
+                xo = X;
+                runs[nruns++] = Run{t.in_ofs, t.n, X};
+                xmask[X / 64] |= uint64_t(1) << (X % 64);
+                X += t.n;
+            } else {
+                xo = runs[ri].xo;
+            }
+            rmask |= uint64_t(1) << H;
+            StreamTask st;
+            st.lin = 0;
+            st.out_ofs = t.out_ofs;
+            st.ax = uint32_t(t.fofs - f0w * E) | uint32_t(xo) << 11;
+            st.mhn = uint32_t(t.m) | uint32_t(H) << 8 | uint32_t(t.n) << 15;
+            P.tk.push_back(st);
+            ++nt;
+            H += h;
+            fend = end;
+        }
+        close();
+    }, 1);
+    int64_t n_ch = 0, n_tk = 0, n_rest = 0, gap = 0, xent = 0;
+    for (const Part &P : parts) {
+        n_ch += int64_t(P.ch.size());
+        n_tk += int64_t(P.tk.size());
+        n_rest += int64_t(P.rest.size());
+        gap += P.gap;
+        xent += P.xent;
+    }
+    BFG_REQUIRE(n_tk < (int64_t(1) << 31) && n_ch < (int64_t(1) << 31), "pack: too many streaming tasks");
+    hvec<StreamChunk> ch(static_cast<size_t>(n_ch));
+    hvec<StreamTask> tk(static_cast<size_t>(n_tk));
+    hvec<MvTaskC> rest(static_cast<size_t>(n_rest));
+    {
+        int64_t oc = 0, ot = 0, orr = 0;
+        std::vector<int64_t> bc(static_cast<size_t>(nr)), bt(static_cast<size_t>(nr)), br(static_cast<size_t>(nr));
+        for (int64_t r = 0; r < nr; ++r) {
+            bc[size_t(r)] = oc;
+            bt[size_t(r)] = ot;
+            br[size_t(r)] = orr;
+            oc += int64_t(parts[size_t(r)].ch.size());
+            ot += int64_t(parts[size_t(r)].tk.size());
+            orr += int64_t(parts[size_t(r)].rest.size());
+        }
+        std::atomic<bool> bad{false};
+        parallel_for(nr, [&](int64_t r) {
+            const Part &P = parts[size_t(r)];
+            for (size_t q = 0; q < P.ch.size(); ++q) {
+                StreamChunk k = P.ch[q];
+                k.t0 += int32_t(bt[size_t(r)]);
+                const int64_t f0 = int64_t(k.w0 & ((uint64_t(1) << 40) - 1)) * E;
+                const int64_t nw = int64_t((k.w0 >> 40) & 2047);
+                if (f0 < 0 || f0 + nw * E > factor_elems + FACTOR_PAD || nw > cape / E) bad.store(true);
+                ch[size_t(bc[size_t(r)]) + q] = k;
+            }
+            std::copy(P.tk.begin(), P.tk.end(), tk.begin() + bt[size_t(r)]);
+            std::copy(P.rest.begin(), P.rest.end(), rest.begin() + br[size_t(r)]);
+        }, 1);
+        BFG_REQUIRE(!bad.load(), "pack: streaming chunk outside its factor buffer");
+    }
+    upload_tab(S.s_chunks, ch);
+    upload_tab(S.s_tasks, tk);
+    upload_tab(S.s_rest, rest);
+    S.n_schunks = int32_t(n_ch);
+    S.n_stasks = int32_t(n_tk);
+    S.n_srest = int32_t(n_rest);
+    S.stream_gap = gap;
+    S.stream_xent = xent;
+    S.stream_nld = nld;
+    S.stream_built = true;
+}
+
 // Fused-sweep tables: every forward task of every stage in stage order.
 // Items that read the same set of skeleton vectors form one dependency
 // group with one counter; each produced vector is read by exactly one group
@@ -1056,7 +1221,12 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
             shift = bf.gather_ofs;
             if (tl) tl->launched(s, {L, 2, 0, 0, 0, 1});
         }
-        if (t.variant == 3 && fwd) {
+        if (t.variant == 9 && fwd && nrhs == 1) {
+            // streaming kernel over its chunks, dense kernel over the other tasks
+            launch_stream(s, bf.precision, t, bf.factors[L].p, slots, shift, bf.ctx->n_cu);
+            launch_dense(s, bf.precision, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout, nrhs,
+                         alpha, beta, fuse, bf.ctx->n_cu, DENSE_REST);
+        } else if ((t.variant == 3 || t.variant == 9) && fwd) {
             launch_dense(s, bf.precision, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout, nrhs,
                          alpha, beta, fuse, bf.ctx->n_cu);
         } else if (t.variant == 8) {  // forward dense kernel over the transposed copy
@@ -1064,7 +1234,7 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
             BFG_REQUIRE(!fwd && bf.tcopy_built && (ft || bf.factor_elems[L] == 0),
                         "matvec: transposed copy of the blocks missing");
             launch_dense(s, bf.precision, t, ft ? ft : bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
-                         nrhs, alpha, beta, fuse, bf.ctx->n_cu, true, op == BFGPU_OP_C ? 1 : 0);
+                         nrhs, alpha, beta, fuse, bf.ctx->n_cu, DENSE_SWAPPED, op == BFGPU_OP_C ? 1 : 0);
         } else if (t.variant == 3) {
             launch_denset(s, bf.precision, op, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
                           nrhs, alpha, beta, fuse, bf.ctx->n_cu);
@@ -1076,7 +1246,8 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
         launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, slots, bf.slot_len, pin,
                      pout, nrhs, alpha, beta, fuse);
     }
-    if (tl) tl->launched(s, {L, table, t.variant, t.out_kind, int32_t(pack_tile(t.tile)), nr_per_lane(nrhs)});
+    const int32_t ran = t.variant == 9 && (!fwd || nrhs != 1) ? 3 : t.variant;  // streams at one RHS only
+    if (tl) tl->launched(s, {L, table, ran, t.out_kind, int32_t(pack_tile(t.tile)), nr_per_lane(nrhs)});
 }
 
 static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t ldx, void *y, int64_t ldy,
@@ -1634,6 +1805,25 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                         sh.qw4 = int8_t(q4);
                         consider(op == BFGPU_OP_N ? 3 : 8, sh);
                     }
+        // the streaming kernel (forward, one RHS): chunks of 8 and 4 KiB;
+        // its other tasks keep the best dense shape
+        if (op == BFGPU_OP_N && nrhs == 1 && t.out_kind == OUT_SLOT) {
+            TileShape base = bv == 3 ? bs : t0;
+            if (bv != 3) {
+                base.rp_shift = int8_t(std::max<int>(4, std::min<int>(6, base.rp_shift + base.g_shift)));
+                base.g_shift = 0;
+            }
+            for (int nld : {8, 4}) {
+                build_stream(t, es, nld, bf.factor_elems[L]);
+                for (int wg : {4, 8}) {
+                    TileShape sh = base;
+                    sh.snld4 = int8_t(nld == 4 ? 1 : 0);
+                    sh.wg_per_cu = int16_t(wg);
+                    sh.xcd = 0;
+                    consider(9, sh);
+                }
+            }
+        }
         if (bv >= 2) {  // the other task-to-XCD mapping, then other occupancies, for the best shape
             TileShape sh = bs;
             sh.xcd = int8_t(1 - sh.xcd);
@@ -1646,6 +1836,7 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                 consider(bv, sh);
             }
         }
+        if (bv == 9) build_stream(t, es, bs.snld4 ? 4 : 8, bf.factor_elems[L]);
         t.variant = bv;
         t.tile = bs;
         return best;

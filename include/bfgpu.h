@@ -327,6 +327,12 @@ int bfgpu_bf_stage_stats(const bfgpu_bf *bf, int32_t *n_stages, int64_t *stats);
  * written (twice for y, which is read and written).  10 int64 per level;
  * the last two give the level's algorithmic vector traffic. */
 int bfgpu_bf_launch_stats(const bfgpu_bf *bf, int32_t op, int32_t *n_levels, int64_t *stats);
+/* Streaming-kernel tables of the forward levels (BFGPU_OPT_STREAM; no
+ * counterpart in the reference): per level, chunk KiB (0: not built),
+ * chunks, tasks in chunks, tasks left to the dense kernel, factor entries
+ * copied that belong to no block, input entries copied.  6 int64 per
+ * level. */
+int bfgpu_bf_stream_stats(const bfgpu_bf *bf, int32_t *n_levels, int64_t *stats);
 
 /* Convert the factor storage to another precision (e.g. FP64 -> FP32). */
 int bfgpu_bf_convert(bfgpu_bf *bf, int32_t precision, bfgpu_bf **out);
@@ -455,7 +461,18 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        first autotune (or on this setting) when device
  *                        memory holds it with a tenth and 1 GiB to spare;
  *                        the matrix-free level is not copied; 0 releases
- *                        the copy.  get: 1 while the copy exists. */
+ *                        the copy.  get: 1 while the copy exists.
+ *   BFGPU_OPT_STREAM     4 or 8: every forward table runs the streaming
+ *                        kernel at one right-hand side (variant 9): runs of
+ *                        consecutive single-block tasks whose blocks lie
+ *                        back to back are copied, 4 or 8 KiB at a time, into
+ *                        the wavefront's LDS with 1 KiB coalesced loads and
+ *                        computed there; other tasks, and products of
+ *                        several right-hand sides, take the dense kernel
+ *                        (the table's dense shape, bit 28 of the packed
+ *                        shape selects 4 KiB chunks); 0: variant 3 (dense).
+ *                        get: the chunk KiB when every forward table
+ *                        streams, else 0. */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -468,7 +485,8 @@ enum {
     BFGPU_OPT_RHS_CHUNK = 9,
     BFGPU_OPT_PFREE = 10,
     BFGPU_OPT_SLOT_CAP = 11,
-    BFGPU_OPT_TCOPY = 12
+    BFGPU_OPT_TCOPY = 12,
+    BFGPU_OPT_STREAM = 13
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);
