@@ -2034,12 +2034,17 @@ __global__ __launch_bounds__(256) void rhs_kernel(
 // ---------------------------------------------------------------------------
 typedef float bfg_v4f __attribute__((ext_vector_type(4)));
 
-// Occupancy target per RHS group count: 6 / 5 / 4 waves per SIMD for
-// <= 32 / 48 / 64 RHS, the most that fit the accumulators without more than
-// a couple of spilled registers (32 RHS: 28.5 -> 27.2 ms on the 262,144-point
-// sphere against the compiler's own choice of 4).
+// Occupancy target per RHS group count and tile height: 16-row tiles take
+// 6 / 5 / 4 waves per SIMD for <= 32 / 48 / 64 RHS, the most that fit the
+// accumulators without more than a couple of spilled registers (32 RHS:
+// 28.5 -> 27.2 ms on the 262,144-point sphere against the compiler's own
+// choice of 4); taller tiles hold RB times the accumulators and get the
+// occupancy that holds them (the 16-row target spilled 121-318 registers).
+constexpr int mfma_waves(int NT, int RB) {
+    return RB == 1 ? (NT <= 2 ? 6 : NT == 3 ? 5 : 4) : (NT * RB <= 3 ? 4 : 2);
+}
 template <int OUTK, int NT, int RB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 6 : NT == 3 ? 5 : 4))) void mfma_rhs_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(NT, RB)))) void mfma_rhs_kernel(
     const int4 *__restrict__ tiles, int32_t n_tiles, const MvTaskC *__restrict__ mtasks,
     const MvContrib *__restrict__ contribs, const C<float> *__restrict__ F, C<float> *__restrict__ slots,
     int32_t ns, int64_t in_shift, C<float> *__restrict__ y, int64_t ldy, const int32_t *__restrict__ perm_out,

@@ -427,15 +427,14 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        sides (default 32; 0 = never) put the right-hand
  *                        sides across lanes, 64 per sweep, with the factor
  *                        entries read once per sweep
- *   BFGPU_OPT_MFMA       1 (default): with right-hand sides across lanes,
- *                        complex FP32 factorizations multiply on the matrix
- *                        cores (v_mfma_f32_16x16x4_f32, small blocks that
- *                        read the same inputs stacked into tiles of 16
- *                        rows); 16, 32, 48 or 64: the same with tiles of up
- *                        to that many rows (taller tiles share each input
- *                        load between more rows but hold more registers;
- *                        measured slower); 0: vector-ALU kernel; get: 0 or
- *                        the tile height
+ *   BFGPU_OPT_MFMA       with right-hand sides across lanes, complex FP32
+ *                        factorizations multiply on the matrix cores
+ *                        (v_mfma_f32_16x16x4_f32; small blocks that read the
+ *                        same inputs stacked into tiles); 16, 32 (default),
+ *                        48 or 64: the most rows of a tile (taller tiles share
+ *                        each input load between more rows and hold more
+ *                        registers); 1: on, height unchanged; 0: vector-ALU
+ *                        kernel; get: 0 or the tile height
  *   BFGPU_OPT_RHS_CHUNK  most right-hand sides per pass over the levels;
  *                        0 (default): as many as the skeleton-vector buffer
  *                        of one pass allows in 90 % of the free device

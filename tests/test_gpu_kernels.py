@@ -281,7 +281,7 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     rng = np.random.default_rng(12)
     X = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
     want = np.stack([imp.matvec(X[:, j].astype(np.complex128)) for j in range(70)], axis=1)
-    assert g.get_option("mfma") == 16
+    assert g.get_option("mfma") == 32
     g.set_option("rhs_lanes", 16)
     for nr in (16, 17, 33, 64, 70):
         got = (g @ np.asfortranarray(X[:, :nr])).astype(np.complex128)
