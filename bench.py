@@ -122,6 +122,8 @@ def kernel_name(d, op, real):
     ink = 0 if (table == 1 or level > 0) else 1
     if variant == 6:
         return "pfree_kernel"
+    if variant == 10:  # matrix-free last level on the matrix cores
+        return f"pfree_mfma_kernel<{(nr + 15) // 16}>"
     if variant == 9:  # streaming kernel (its other tasks, if any, run the dense kernel in the same slot)
         return f"stream_kernel<{real}, {4 if (cs >> 28) & 1 else 8}>"
     if variant == 5:

@@ -392,8 +392,16 @@ struct Engine {
         }, 1);
     }
 
+    // Grow a device buffer of the engine; kernels still queued (the
+    // interpolation of the previous batch is not waited for) may read the
+    // old one, so the stream is drained before it is freed.
+    void need(DevBuf &d, size_t n) {
+        if (d.p && n <= d.bytes) return;
+        BFG_HIP(hipStreamSynchronize(ctx.stream));
+        d.reserve(n);
+    }
     template <class T, class A> void upload(DevBuf &d, const std::vector<T, A> &h, hipStream_t s) {
-        d.reserve(sizeof(T) * std::max<size_t>(h.size(), 1));
+        need(d, sizeof(T) * std::max<size_t>(h.size(), 1));
         if (!h.empty()) BFG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
     }
 
@@ -486,7 +494,7 @@ struct Engine {
             who[size_t(at[size_t(i)])] = q;
         }, 4096);
         upload(d_jtask, jt, ctx.stream);
-        d_jres.reserve(sizeof(double) * 2 * size_t(nj));
+        need(d_jres, sizeof(double) * 2 * size_t(nj));
         launch_judge(ctx.stream, prec, d_Z.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
                      d_jres.as<double>());
         hvec<double> &res = j_res;
@@ -638,9 +646,9 @@ struct Engine {
             upload(d_fill, ft, ctx.stream);
             upload(d_fill_end, fe, ctx.stream);
             upload(d_qr, qt, ctx.stream);
-            d_Z.reserve(size_t(zofs) * es);
-            d_piv.reserve(sizeof(int32_t) * size_t(std::max(pofs, 1)));
-            d_rank.reserve(sizeof(int32_t) * np);
+            need(d_Z, size_t(zofs) * es);
+            need(d_piv, sizeof(int32_t) * size_t(std::max(pofs, 1)));
+            need(d_rank, sizeof(int32_t) * np);
             fill(d_rows.as<int32_t>(), d_cols.as<int32_t>(), d_fill.as<FillTask>(), d_fill_end.as<int64_t>(),
                  int32_t(ft.size()), zofs, d_Z.p);
             // QR launches: one per (tolerance, kernel) group -- the batch may
@@ -677,7 +685,7 @@ struct Engine {
                         gm = std::max(gm, sub[i].m);
                     }
                     upload(d_sub, sub, ctx.stream);
-                    d_subrank.reserve(sizeof(int32_t) * sub.size());
+                    need(d_subrank, sizeof(int32_t) * sub.size());
                     launch_cpqr(ctx.stream, prec, d_Z.p, d_sub.as<QrTask>(), int32_t(sub.size()),
                                 d_piv.as<int32_t>(), d_subrank.as<int32_t>(), gr.first.first, gn, gm);
                     std::vector<int32_t> r(sub.size());
@@ -776,7 +784,10 @@ struct Engine {
                                     d_piv.as<int32_t>(), pool.p);
                 launch_interp(ctx.stream, prec, d_Z.p, d_interp.as<InterpTask>() + n_small,
                               int32_t(int64_t(it.size()) - n_small), d_piv.as<int32_t>(), pool.p, max_rank);
-                BFG_HIP(hipStreamSynchronize(ctx.stream));
+                // no wait: the host goes on with the next batch or level
+                // (sampling, planning) while the interpolation matrices are
+                // formed; everything that next touches the tiles, pivots or
+                // the pool is ordered after them on the same stream
             }
             t_interp += secs(ti, clk::now()) - (t_judge - tj0);
             retry.insert(retry.end(), later.begin(), later.end());

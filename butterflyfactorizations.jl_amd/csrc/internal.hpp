@@ -328,6 +328,10 @@ struct BF {
     int32_t pf_n_items = 0;
     double pf_k = 0.0;            // wavenumber of the point kernel
     DevBuf pf_items, pf_col, pf_in, pf_row;  // matrix-free last level tables
+    // matrix-core form of the matrix-free last level (right-hand sides
+    // across lanes): items in order of decreasing work, built on first use
+    bool pf_order_built = false;
+    DevBuf pf_order;
     bool pfree_t_ok = false;                 // every P column's skeleton slot written once
     // transposed copy of the stored blocks for the adjoint / transpose
     // (variant 8 of the transposed tables): per level, each block n x m
@@ -367,6 +371,9 @@ constexpr int64_t FACTOR_PAD = 16;
 void pack_tables(BF &bf);
 // Fused-sweep tables, built on first use (matvec.hip).
 void build_sweep(BF &bf);
+// Order of the matrix-free last level's items for its matrix-core kernel
+// (matvec.hip), built on first use.
+void build_pf_order(BF &bf);
 // Streaming-kernel tables of a forward table (matvec.hip); es: entry bytes.
 void build_stream(StageTable &S, int es, int nld, int64_t factor_elems);
 
@@ -510,6 +517,10 @@ void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, vo
                      const double *beta, bool fuse, int n_cu);
 // Matrix-free last level (kernels.hip): y rows of every observer leaf from
 // the P columns, entries computed from the points.
+// Matrix-free last level with right-hand sides across lanes (ns = 16, 32,
+// 48 or 64 interleaved slot columns, nc of them used) on the matrix cores.
+void launch_pfree_mfma(hipStream_t s, const BF &bf, const void *slots, int32_t ns, void *y, int64_t ldy, int nc,
+                       const double *alpha, const double *beta, bool fuse, int n_cu);
 void launch_pfree(hipStream_t s, const BF &bf, const void *slots, void *y, int64_t ldy, const double *alpha,
                   const double *beta, bool fuse);
 void launch_pfree_t(hipStream_t s, const BF &bf, const void *x, void *slots, bool conj);

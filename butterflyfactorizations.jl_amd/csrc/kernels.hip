@@ -2260,6 +2260,163 @@ void launch_mfma_rhs(hipStream_t s, const StageTable &t, const void *factors, vo
     }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free last level on the matrix cores (forward, right-hand sides
+// across lanes, complex FP32).  The last level's blocks are point-kernel
+// entries between an observer leaf's points and the skeleton points of its
+// columns, so, as in pfree_kernel, they are computed instead of read: one
+// wavefront per observer leaf (largest first), all of the leaf's row blocks
+// of 16 at once (at most 4: a leaf of more than 64 rows is taken 64 rows at
+// a time).  Per step of 4 columns, lane i + 16 k computes
+// the entry of row 16 rb + i and column k of every row block rb (leaf-local
+// coordinates, as pfree_kernel) -- the A operand -- and loads, per group of
+// 16 right-hand sides, the column's skeleton entries (lane j + 16 k) -- the
+// B operand, read once for all row blocks; four v_mfma_f32_16x16x4_f32 per
+// row block and group.  Column data and inputs of the next step are loaded
+// while the current one is multiplied.  Rows past the leaf and columns past
+// its end carry zero entries; right-hand sides >= nc are never stored.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 4 : 2))) void pfree_mfma_kernel(
+    const int4 *__restrict__ items, const int32_t *__restrict__ order, int32_t n_items,
+    const float4 *__restrict__ cdat, const int32_t *__restrict__ colin, const float4 *__restrict__ rdat,
+    const C<float> *__restrict__ slots, int32_t ns, float kr, C<float> *__restrict__ y, int64_t ldy,
+    const int32_t *__restrict__ perm_out, C<float> alpha, C<float> beta, int fuse_beta, int nc) {
+    constexpr float inv4pi = 0.07957747154594767f;
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int64_t n_waves = int64_t(gridDim.x) * 4;
+    for (int64_t t = int64_t(blockIdx.x) * 4 + wave; t < n_items; t += n_waves)
+    for (int r0 = 0; r0 < items[order[t]].y; r0 += 64) {  // rows in groups of 64 (leaves hold at most 64 as a rule)
+        const int4 it = items[order[t]];  // first row (tree position), rows, columns [z, w)
+        const int nrb = min(4, (it.y - r0 + 15) >> 4);  // row blocks of this group, 1..4
+        float ux[4], uy[4], uz[4], uu[4];
+        bool rok[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const int row = r0 + 16 * rb + li;
+            rok[rb] = row < it.y;
+            const float4 p = rdat[it.x + min(row, it.y - 1)];
+            ux[rb] = p.x;
+            uy[rb] = p.y;
+            uz[rb] = p.z;
+            uu[rb] = p.w;
+        }
+        bfg_v4f accr[4][NT], acci[4][NT];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int g = 0; g < NT; ++g) {
+                accr[rb][g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+                acci[rb][g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        auto fetch = [&](int c0, float4 &d, C<float>(&b)[NT], bool &ok) {
+            const int c = c0 + lk;
+            ok = c < it.w;
+            const int cc = min(c, it.w - 1);
+            d = cdat[cc];
+            const int64_t e = int64_t(colin[cc]) * ns + li;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) b[g] = slots[e + 16 * g];
+        };
+        float4 d;
+        C<float> b[NT];
+        bool cok;
+        fetch(it.z, d, b, cok);
+        for (int c0 = it.z; c0 < it.w; c0 += 4) {
+            float4 dn = d;
+            C<float> bn[NT];
+            bool nok = false;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) bn[g] = b[g];
+            if (c0 + 4 < it.w) fetch(c0 + 4, dn, bn, nok);
+            const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+            const float D = D2 * __builtin_amdgcn_rsqf(D2);
+            const float mx = -2.f * d.x, my = -2.f * d.y, mz = -2.f * d.z;
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                if (rb >= nrb) break;
+                const float ex = d.x - ux[rb], ey = d.y - uy[rb], ez = d.z - uz[rb];
+                const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                const float rs = __builtin_amdgcn_rsqf(r2);
+                const float num = fmaf(mx, ux[rb], fmaf(my, uy[rb], fmaf(mz, uz[rb], uu[rb])));
+                const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
+                const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
+                const float amp = cok && rok[rb] ? rs * inv4pi : 0.f;
+                const float ar = amp * __builtin_amdgcn_cosf(rev), ai = -amp * __builtin_amdgcn_sinf(rev);
+#pragma unroll
+                for (int g = 0; g < NT; ++g)
+                    accr[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b[g].x, accr[rb][g], 0, 0, 0);
+#pragma unroll
+                for (int g = 0; g < NT; ++g)
+                    acci[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, b[g].y, acci[rb][g], 0, 0, 0);
+#pragma unroll
+                for (int g = 0; g < NT; ++g)
+                    accr[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(-ai, b[g].y, accr[rb][g], 0, 0, 0);
+#pragma unroll
+                for (int g = 0; g < NT; ++g)
+                    acci[rb][g] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, b[g].x, acci[rb][g], 0, 0, 0);
+            }
+            d = dn;
+            cok = nok;
+#pragma unroll
+            for (int g = 0; g < NT; ++g) b[g] = bn[g];
+        }
+        // lane li + 16 lk holds rows 16 rb + 4 lk + v of right-hand side li (+ 16 g)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            if (rb >= nrb) break;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = r0 + 16 * rb + 4 * lk + v;
+                if (row >= it.y) continue;
+                const int64_t yr = perm_out[it.x + row];
+#pragma unroll
+                for (int g = 0; g < NT; ++g) {
+                    const int j = 16 * g + li;
+                    if (j >= nc) continue;
+                    C<float> *yp = y + int64_t(j) * ldy + yr;
+                    const C<float> old = *yp;
+                    *yp = cadd(fuse_beta ? cmul(beta, old) : old,
+                               cmul(alpha, C<float>{accr[rb][g][v], acci[rb][g][v]}));
+                }
+            }
+        }
+    }
+}
+
+void launch_pfree_mfma(hipStream_t s, const BF &bf, const void *slots, int32_t ns, void *y, int64_t ldy, int nc,
+                       const double *alpha, const double *beta, bool fuse, int n_cu) {
+    if (bf.pf_n_items == 0) return;
+    BFG_REQUIRE(bf.pf_order_built, "matvec: matrix-free matrix-core tables missing");
+    BFG_REQUIRE(ns % 16 == 0 && ns >= 16 && ns <= 64 && nc >= 1 && nc <= ns,
+                "matvec: matrix-core chunk of 16, 32, 48 or 64 right-hand sides");
+    const C<float> al{float(alpha[0]), float(alpha[1])}, be{float(beta[0]), float(beta[1])};
+    // persistent: as many workgroups as are resident (one wave per SIMD
+    // each; 5 / 4 / 3 / 2 waves per SIMD for 16 / 32 / 48 / 64 RHS), the
+    // items (largest first) dealt round robin
+    const int64_t wgs = (int64_t(bf.pf_n_items) + 3) / 4;
+    const int per_cu = ns <= 16 ? 5 : ns <= 32 ? 4 : ns <= 48 ? 3 : 2;
+    const int64_t cap = int64_t(std::max(n_cu, 1)) * per_cu;
+    dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
+    const float kr = float(bf.pf_k * 0.15915494309189535);
+#define BFG_PFM(NT)                                                                                            \
+    hipLaunchKernelGGL((pfree_mfma_kernel<NT>), grid, dim3(256), 0, s, bf.pf_items.as<const int4>(),            \
+                       bf.pf_order.as<const int32_t>(), bf.pf_n_items, bf.pf_col.as<const float4>(),              \
+                       bf.pf_in.as<const int32_t>(), bf.pf_row.as<const float4>(),                               \
+                       static_cast<const C<float> *>(slots), ns, kr, static_cast<C<float> *>(y), ldy,             \
+                       bf.perm_test.as<const int32_t>(), al, be, fuse ? 1 : 0, nc)
+    switch (ns / 16) {
+    case 1: BFG_PFM(1); break;
+    case 2: BFG_PFM(2); break;
+    case 3: BFG_PFM(3); break;
+    default: BFG_PFM(4); break;
+    }
+#undef BFG_PFM
+    BFG_HIP(hipGetLastError());
+}
+
 // dst[v * ns + r] = x[r][perm[v]] for v in [lo, hi), r < nc: inputs of the
 // first level in tree order, right-hand sides interleaved.
 template <typename R>

@@ -801,6 +801,7 @@ void pack_tables(BF &bf) {
                 }, 1 << 16);
                 bf.pfree_t_ok = once.load();
             }
+            bf.pf_order_built = false;
             upload_tab(bf.pf_items, items);
             upload_tab(bf.pf_col, col);
             upload_tab(bf.pf_in, cin);
@@ -989,6 +990,22 @@ void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
     S.stream_xent = xent;
     S.stream_nld = nld;
     S.stream_built = true;
+}
+
+// Order of the matrix-free last level's items for the matrix-core kernel:
+// decreasing work (row blocks of 16 times columns), so that the largest
+// leaves start first; built on first use (host sort of the leaves).
+void build_pf_order(BF &bf) {
+    if (bf.pf_order_built || !bf.pfree_ok) return;
+    const int32_t n = bf.pf_n_items;
+    std::vector<int4> items(size_t(std::max(n, 1)));
+    if (n > 0) BFG_HIP(hipMemcpy(items.data(), bf.pf_items.p, sizeof(int4) * size_t(n), hipMemcpyDeviceToHost));
+    std::vector<int32_t> ord(static_cast<size_t>(n));
+    for (int32_t i = 0; i < n; ++i) ord[size_t(i)] = i;
+    auto work = [&](int32_t i) { const int4 &t = items[size_t(i)]; return int64_t((t.y + 15) / 16) * (t.w - t.z); };
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return work(a) > work(b); });
+    upload_tab(bf.pf_order, ord);
+    bf.pf_order_built = true;
 }
 
 // Fused-sweep tables: every forward task of every stage in stage order.
@@ -1306,11 +1323,13 @@ static void stage_seq_pass(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
         bf.slots.alloc(need);
         bf.slots_cap = need;
     }
-    if (lanes && bf.use_mfma && bf.precision == BFGPU_C64)  // before any capture: the build uploads
+    if (lanes && bf.use_mfma && bf.precision == BFGPU_C64) {  // before any capture: the builds upload
         for (StageTable &t : bf.fwd) {
             if (t.pre) build_mfma(*t.pre, bf.mfma_rows);
             build_mfma(t, bf.mfma_rows);
         }
+        if (bf.use_pfree) build_pf_order(bf);
+    }
     const bool fused = op == BFGPU_OP_N && !lanes && sweep_ok(bf, nrhs);
     if (tl || !bf.use_graph || fused || !graph_ok) {
         stage_launches(bf, op, nrhs, x, ldx, y, ldy, alpha, beta, tl, lanes, bf.slots.p);
@@ -1395,6 +1414,12 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
             if (tl) tl->launched(s, {0, 2, 4, 0, 0, nc});
             for (int32_t L = 0; L < nL; ++L) {
                 const StageTable &t = bf.fwd[L];
+                if (mfma && L == nL - 1 && bf.pfree_ok && bf.use_pfree && bf.pf_order_built) {
+                    // last level's entries from the points, on the matrix cores
+                    launch_pfree_mfma(s, bf, slots, ns, yc, ldy, nc, alpha, b_eff, fuse, bf.ctx->n_cu);
+                    if (tl) tl->launched(s, {L, 0, 10, OUT_SCATTER, 0, nc});
+                    continue;
+                }
                 const StageTable *pre = t.pre.get();
                 for (const StageTable *u : {pre, &t}) {
                     if (!u) continue;

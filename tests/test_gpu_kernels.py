@@ -413,3 +413,49 @@ def test_matrix_free_last_level(bfgpu, oracle, ctx, tmp_path):
     op64 = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bt, K, tol=1e-4,
                                   admissibility=bfgpu.CenterDistanceAdmissibility(2.0), farfieldonly=True, ctx=ctx)
     assert op64.get_option("pfree") == 0
+
+
+def test_matrix_free_last_level_many_rhs(bfgpu, oracle, ctx):
+    """Right-hand sides across lanes (16-64 per pass) compute the last
+    level's point-kernel entries on the fly and multiply them on the matrix
+    cores (one wavefront per observer leaf, up to four 16-row blocks): the
+    products equal the stored-P matrix-core products to FP32 rounding of the
+    entries (1e-5, the one-RHS matrix-free bar) and the oracle's FP64
+    product of the same factors (1e-5); alpha / beta, two passes (70 RHS),
+    leaves of a few to 64 rows (the sphere) and of more than 64 rows (the
+    two squares' balanced trees, taken 64 rows at a time)."""
+    import torch
+    rng = np.random.default_rng(31)
+    say("far field sphere N=6000 fp32 and two squares N=4096 fp32: matrix-free last level, many RHS")
+    pts = fibonacci_sphere(6000, 4.0)
+    T = bfgpu.ClusterTree(pts, dims=3, leaf_size=64, mode="maxpoints")
+    sphere = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
+                                    admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
+                                    farfieldonly=True, ctx=ctx)
+    obs, src = two_squares(4096, seed=5)
+    To = bfgpu.ClusterTree(obs, dims=2, leaf_size=64)
+    Ts = bfgpu.ClusterTree(src, dims=2, leaf_size=64)
+    sq = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(To, Ts), 0, 0, K, 1e-4,
+                           acctype=np.complex64, ctx=ctx)
+    for name, op, n, trees in (("sphere", sphere, 6000, (T, T)), ("squares", sq, 4096, (To, Ts))):
+        assert op.get_option("pfree") == 1
+        X = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
+        Y0 = np.asfortranarray(crandn(rng, n, 33)).astype(np.complex64)
+        a, b = 0.5 - 0.25j, -1.5 + 0.5j
+        on = op @ X
+        on_ab = op.mul(Y0.copy(order="F"), np.asfortranarray(X[:, :33]), a, b)
+        Xd = torch.from_numpy(np.ascontiguousarray(X[:, :40].T)).to("cuda:0").t()
+        _, desc = op.launch_times(Xd, torch.zeros_like(Xd), iters=1, nrhs=40)
+        assert any(int(d[2]) == 10 for d in desc), name
+        op.set_option("pfree", 0)
+        off = op @ X
+        off_ab = op.mul(Y0.copy(order="F"), np.asfortranarray(X[:, :33]), a, b)
+        op.set_option("pfree", 1)
+        say(name, "matrix-free vs stored P", rel(on, off), "alpha/beta", rel(on_ab, off_ab))
+        assert rel(on, off) < 1e-5 and rel(on_ab, off_ab) < 1e-5, name
+        nb = len(op.farinteractions) if hasattr(op, "farinteractions") else 1
+        imps = [oracle.import_bf(trees[0], trees[1], op.export(i)) for i in range(nb)]
+        cols = (0, 37, 69)
+        want = np.stack([oracle.matvec_many(imps, X[:, j].astype(np.complex128), nthreads=8, rows=n)
+                         for j in cols], axis=1)
+        assert rel(on[:, cols].astype(np.complex128), want) < 1e-5, name
