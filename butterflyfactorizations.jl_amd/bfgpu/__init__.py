@@ -552,7 +552,7 @@ class ButterflyFactorization:
         return self
 
     OPTIONS = {"sweep": 1, "wg_per_cu": 2, "graph": 3, "variant": 4, "tile": 5, "dense": 6, "rhs_lanes": 7,
-               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11, "tcopy": 12, "stream": 13}
+               "mfma": 8, "rhs_chunk": 9, "pfree": 10, "slot_cap": 11, "tcopy": 12, "stream": 13, "mfma_emul": 14}
 
     @staticmethod
     def tile_shape(rp_shift, g_shift, csplit=0, wg_per_cu=8, xcd=0, qw4=0, snld4=0):
@@ -573,7 +573,9 @@ class ButterflyFactorization:
         (most right-hand sides per pass; 0: as device memory allows),
         pfree (1: compute the last level's point-kernel entries), stream
         (4 or 8: forward levels on the streaming kernel with chunks of that
-        many KiB at one RHS; 0: dense kernel), slot_cap
+        many KiB at one RHS; 0: dense kernel), mfma_emul (1, default:
+        the matrix-free last level of products of many RHS emulates FP32 on
+        the bf16 matrix cores, bf16x3; 0: FP32 matrix cores), slot_cap
         (most KiB of skeleton vectors per pass), tcopy (1: transposed copy
         of the stored blocks for the adjoint / transpose, built when device
         memory allows and used where the autotuner finds it faster; 0:

@@ -806,6 +806,11 @@ extern "C" int bfgpu_bf_set_option(bfgpu_bf *h, int32_t key, int32_t value) {
         release_graph(bf);
         break;
     }
+    case BFGPU_OPT_MFMA_EMUL:
+        BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: mfma_emul is 0 or 1");
+        bf.mfma_emul = value != 0;
+        release_graph(bf);
+        break;
     case BFGPU_OPT_PFREE:
         BFG_REQUIRE(value == 0 || value == 1, "bf_set_option: pfree is 0 or 1");
         bf.use_pfree = value != 0;
@@ -856,6 +861,7 @@ extern "C" int bfgpu_bf_get_option(const bfgpu_bf *h, int32_t key, int32_t *valu
     case BFGPU_OPT_RHS_CHUNK: *value = bf.rhs_chunk; break;
     case BFGPU_OPT_SLOT_CAP: *value = bf.slot_cap_kib; break;
     case BFGPU_OPT_TCOPY: *value = bf.tcopy_built ? 1 : 0; break;
+    case BFGPU_OPT_MFMA_EMUL: *value = bf.mfma_emul ? 1 : 0; break;
     case BFGPU_OPT_STREAM: {
         int32_t v = -1;
         for (const StageTable &t : bf.fwd)

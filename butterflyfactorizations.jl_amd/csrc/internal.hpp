@@ -319,6 +319,7 @@ struct BF {
     int32_t rhs_lanes_min = 32;   // forward matvecs with >= this many RHS: RHS across lanes (0: never)
     bool use_mfma = true;         // RHS across lanes in complex FP32: matrix-core kernel
     int32_t mfma_rows = 32;       // most rows of its tiles (16, 32, 48 or 64)
+    bool mfma_emul = true;        // matrix-free last level: FP32 products emulated on the bf16 matrix cores (bf16x3)
     int32_t rhs_chunk = 0;        // most right-hand sides per pass (0: as device memory allows)
     int32_t slot_cap_kib = 0;     // most KiB of skeleton vectors per pass (0: 90 % of the free memory)
     // matrix-free last level (forward, one RHS, complex FP32): per observer

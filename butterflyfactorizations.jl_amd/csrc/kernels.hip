@@ -2386,6 +2386,157 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 4
     }
 }
 
+// FP32 products emulated on the bf16 matrix cores ("bf16x3"): x = h + l
+// with h = bf16(x) and l = bf16(x - h) (|x - h - l| <= 2^-16 |x|), and a
+// product a b taken as a_h b_h + a_h b_l + a_l b_h (the dropped a_l b_l is
+// below 2^-16 |a b|), accumulated in FP32 by v_mfma_f32_16x16x16_bf16
+// (lane i + 16 q holds A[i][4q .. 4q+3] and B[4q .. 4q+3][i]; rows
+// 4q .. 4q+3 of the product, as for the FP32 form).  A complex product
+// takes 12 such instructions per 16 columns (6 into the real part, 6 into
+// the imaginary part; -Im b is a negated bf16 pair of the shared B operand)
+// against 16 FP32 16x16x4 instructions; each operand carries about 17 bits
+// instead of 24.
+typedef short bfg_v4s __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void bf16_split(float x, short &h, short &l) {
+    const __bf16 hb = __bf16(x);
+    const __bf16 lb = __bf16(x - float(hb));
+    h = __builtin_bit_cast(short, hb);
+    l = __builtin_bit_cast(short, lb);
+}
+
+// The matrix-free last level on the bf16 matrix cores: pfree_mfma_kernel
+// with steps of 16 columns (lane i + 16 k computes the entries of columns
+// 4 k .. 4 k + 3 of row i of each row block, and splits them and the
+// columns' skeleton entries into bf16 halves).
+template <int NT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 3 : 2))) void pfree_mfma3_kernel(
+    const int4 *__restrict__ items, const int32_t *__restrict__ order, int32_t n_items,
+    const float4 *__restrict__ cdat, const int32_t *__restrict__ colin, const float4 *__restrict__ rdat,
+    const C<float> *__restrict__ slots, int32_t ns, float kr, C<float> *__restrict__ y, int64_t ldy,
+    const int32_t *__restrict__ perm_out, C<float> alpha, C<float> beta, int fuse_beta, int nc) {
+    constexpr float inv4pi = 0.07957747154594767f;
+    const int l = threadIdx.x & 63;
+    const int li = l & 15, lk = l >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int64_t n_waves = int64_t(gridDim.x) * 4;
+    for (int64_t t = int64_t(blockIdx.x) * 4 + wave; t < n_items; t += n_waves)
+    for (int r0 = 0; r0 < items[order[t]].y; r0 += 64) {
+        const int4 it = items[order[t]];
+        const int nrb = min(4, (it.y - r0 + 15) >> 4);
+        float ux[4], uy[4], uz[4], uu[4];
+        bool rok[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const int row = r0 + 16 * rb + li;
+            rok[rb] = row < it.y;
+            const float4 p = rdat[it.x + min(row, it.y - 1)];
+            ux[rb] = p.x;
+            uy[rb] = p.y;
+            uz[rb] = p.z;
+            uu[rb] = p.w;
+        }
+        bfg_v4f accr[4][NT], acci[4][NT];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int g = 0; g < NT; ++g) {
+                accr[rb][g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+                acci[rb][g] = bfg_v4f{0.f, 0.f, 0.f, 0.f};
+            }
+        for (int c0 = it.z; c0 < it.w; c0 += 16) {
+            float4 dd[4];
+            bool okc[4];
+            // B halves per group: Re, Im, -Im (hi and lo)
+            bfg_v4s brh[NT], brl[NT], bih[NT], bil[NT], nbih[NT], nbil[NT];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = c0 + 4 * lk + u;
+                okc[u] = c < it.w;
+                const int cc = min(c, it.w - 1);
+                dd[u] = cdat[cc];
+                const int64_t e = int64_t(colin[cc]) * ns + li;
+#pragma unroll
+                for (int g = 0; g < NT; ++g) {
+                    const C<float> v = slots[e + 16 * g];
+                    short h, lo;
+                    bf16_split(v.x, h, lo);
+                    brh[g][u] = h;
+                    brl[g][u] = lo;
+                    bf16_split(v.y, h, lo);
+                    bih[g][u] = h;
+                    bil[g][u] = lo;
+                    nbih[g][u] = short(h ^ short(0x8000));
+                    nbil[g][u] = short(lo ^ short(0x8000));
+                }
+            }
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                if (rb >= nrb) break;
+                bfg_v4s arh, arl, aih, ail;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float4 d = dd[u];
+                    const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+                    const float D = D2 * __builtin_amdgcn_rsqf(D2);
+                    const float ex = d.x - ux[rb], ey = d.y - uy[rb], ez = d.z - uz[rb];
+                    const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+                    const float rs = __builtin_amdgcn_rsqf(r2);
+                    const float num =
+                        fmaf(-2.f * d.x, ux[rb], fmaf(-2.f * d.y, uy[rb], fmaf(-2.f * d.z, uz[rb], uu[rb])));
+                    const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
+                    const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
+                    const float amp = okc[u] && rok[rb] ? rs * inv4pi : 0.f;
+                    short h, lo;
+                    bf16_split(amp * __builtin_amdgcn_cosf(rev), h, lo);
+                    arh[u] = h;
+                    arl[u] = lo;
+                    bf16_split(-amp * __builtin_amdgcn_sinf(rev), h, lo);
+                    aih[u] = h;
+                    ail[u] = lo;
+                }
+#pragma unroll
+                for (int g = 0; g < NT; ++g) {
+                    bfg_v4f r = accr[rb][g], i = acci[rb][g];
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, brh[g], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, bih[g], i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, brl[g], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arh, bil[g], i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl, brh[g], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(arl, bih[g], i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih, nbih[g], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih, brh[g], i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih, nbil[g], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aih, brl[g], i, 0, 0, 0);
+                    r = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ail, nbih[g], r, 0, 0, 0);
+                    i = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ail, brh[g], i, 0, 0, 0);
+                    accr[rb][g] = r;
+                    acci[rb][g] = i;
+                }
+            }
+        }
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            if (rb >= nrb) break;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = r0 + 16 * rb + 4 * lk + v;
+                if (row >= it.y) continue;
+                const int64_t yr = perm_out[it.x + row];
+#pragma unroll
+                for (int g = 0; g < NT; ++g) {
+                    const int j = 16 * g + li;
+                    if (j >= nc) continue;
+                    C<float> *yp = y + int64_t(j) * ldy + yr;
+                    const C<float> old = *yp;
+                    *yp = cadd(fuse_beta ? cmul(beta, old) : old,
+                               cmul(alpha, C<float>{accr[rb][g][v], acci[rb][g][v]}));
+                }
+            }
+        }
+    }
+}
+
 void launch_pfree_mfma(hipStream_t s, const BF &bf, const void *slots, int32_t ns, void *y, int64_t ldy, int nc,
                        const double *alpha, const double *beta, bool fuse, int n_cu) {
     if (bf.pf_n_items == 0) return;
@@ -2397,12 +2548,13 @@ void launch_pfree_mfma(hipStream_t s, const BF &bf, const void *slots, int32_t n
     // each; 5 / 4 / 3 / 2 waves per SIMD for 16 / 32 / 48 / 64 RHS), the
     // items (largest first) dealt round robin
     const int64_t wgs = (int64_t(bf.pf_n_items) + 3) / 4;
-    const int per_cu = ns <= 16 ? 5 : ns <= 32 ? 4 : ns <= 48 ? 3 : 2;
+    const int per_cu = bf.mfma_emul ? (ns <= 32 ? 3 : 2) : ns <= 16 ? 5 : ns <= 32 ? 4 : ns <= 48 ? 3 : 2;
     const int64_t cap = int64_t(std::max(n_cu, 1)) * per_cu;
     dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
     const float kr = float(bf.pf_k * 0.15915494309189535);
 #define BFG_PFM(NT)                                                                                            \
-    hipLaunchKernelGGL((pfree_mfma_kernel<NT>), grid, dim3(256), 0, s, bf.pf_items.as<const int4>(),            \
+    hipLaunchKernelGGL((bf.mfma_emul ? pfree_mfma3_kernel<NT> : pfree_mfma_kernel<NT>), grid, dim3(256), 0, s,     \
+                       bf.pf_items.as<const int4>(),                                                             \
                        bf.pf_order.as<const int32_t>(), bf.pf_n_items, bf.pf_col.as<const float4>(),              \
                        bf.pf_in.as<const int32_t>(), bf.pf_row.as<const float4>(),                               \
                        static_cast<const C<float> *>(slots), ns, kr, static_cast<C<float> *>(y), ldy,             \

@@ -471,7 +471,14 @@ int bfgpu_bf_set_fused(bfgpu_bf *bf, int32_t enable);
  *                        (the table's dense shape, bit 28 of the packed
  *                        shape selects 4 KiB chunks); 0: variant 3 (dense).
  *                        get: the chunk KiB when every forward table
- *                        streams, else 0. */
+ *                        streams, else 0.
+ *   BFGPU_OPT_MFMA_EMUL  1 (default): with right-hand sides across lanes,
+ *                        the matrix-free last level emulates its FP32
+ *                        products on the bf16 matrix cores (bf16x3: each
+ *                        operand as a high and a low bf16 half, three
+ *                        products per real product, FP32 accumulation;
+ *                        about 17 bits per operand, products within about
+ *                        1e-5 of the FP32 ones); 0: FP32 matrix cores. */
 enum {
     BFGPU_OPT_SWEEP = 1,
     BFGPU_OPT_WG_PER_CU = 2,
@@ -485,7 +492,8 @@ enum {
     BFGPU_OPT_PFREE = 10,
     BFGPU_OPT_SLOT_CAP = 11,
     BFGPU_OPT_TCOPY = 12,
-    BFGPU_OPT_STREAM = 13
+    BFGPU_OPT_STREAM = 13,
+    BFGPU_OPT_MFMA_EMUL = 14
 };
 int bfgpu_bf_set_option(bfgpu_bf *bf, int32_t key, int32_t value);
 int bfgpu_bf_get_option(const bfgpu_bf *bf, int32_t key, int32_t *value);

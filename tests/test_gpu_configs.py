@@ -5,9 +5,11 @@
   scaled by sqrt(16384 / 1048576)): learned-rank FP32 factorization with
   the committed predictor, 128 plane-wave right-hand sides on the matrix
   cores in two passes of 64 (bfmatrix.jl:10-50 semantics: column by column
-  the single-RHS product).  The FP32 product must match the oracle's FP64
-  product of the same factors to 2e-6, the plain FP64 GPU operator to
-  10 tau, and dense FP64 far-field rows to 10 tau.
+  the single-RHS product), the last level emulating FP32 on the bf16 matrix
+  cores (bf16x3).  The FP32 product must match the oracle's FP64 product
+  of the same factors to 2e-5 (FP32 matrix cores: 2e-6; the emulated last
+  level adds about 5e-6), the plain FP64 GPU operator to 10 tau, and dense
+  FP64 far-field rows to 10 tau.
 * The sphere far field at its full size (262,144 Fibonacci points, radius
   8 wavelengths): FP32 and FP64 factorizations each within 10 tau of 128
   dense FP64 far-field rows, and within 10 tau of each other.
@@ -62,8 +64,14 @@ def test_plane_wave_ellipsoid_reduced(bfgpu, oracle):
     Yo = oracle.matvec_many(imps, np.asfortranarray(X[:, cols]), nthreads=min(16, os.cpu_count() or 1), rows=n)
     del imps
     e_or = rel(Y32[:, cols], Yo)
-    say("fp32 matrix cores vs oracle of the same factors", e_or)
-    assert e_or < 2e-6
+    say("fp32 matrix cores (bf16x3 last level) vs oracle of the same factors", e_or)
+    assert e_or < 2e-5
+    op32.set_option("mfma_emul", 0)
+    Y32n = (op32 @ np.asfortranarray(X.astype(np.complex64))).astype(np.complex128)
+    op32.set_option("mfma_emul", 1)
+    e_or_n = rel(Y32n[:, cols], Yo)
+    say("fp32 matrix cores throughout vs oracle", e_or_n)
+    assert e_or_n < 2e-6
 
     say("plain fp64 factorization, 128 right-hand sides")
     op64, _ = learned.far_operator(pts, acctype=np.complex128)

@@ -244,6 +244,7 @@ def test_rhs_across_lanes_matches_oracle(bfgpu, oracle, ctx):
     op32 = bfgpu.PetrovGalerkinBF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(T, T), K, tol=1e-4,
                                   admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
                                   farfieldonly=True, ctx=ctx)
+    op32.set_option("pfree", 0)  # stored last-level blocks on both paths: FP32 arithmetic compared at 2e-6
     X40 = np.asfortranarray(crandn(rng, 6000, 40).astype(np.complex64))
     lanes = op32 @ X40                                  # right-hand sides across lanes (matrix cores)
     col_kib = op32.info().slot_len * 8 / 1024
@@ -276,7 +277,9 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
     Ts = bfgpu.ClusterTree(src, dims=2, leaf_size=64)
     g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(To, Ts), 0, 0, K, 1e-4,
                           acctype=np.complex64, ctx=ctx)
-    # the P level's items have 64 rows (a leaf): several row tiles each
+    # stored last-level blocks (the matrix-free last level has a test of its
+    # own): the P level's items have 64 rows (a leaf), several row tiles each
+    g.set_option("pfree", 0)
     imp = oracle.import_bf(To, Ts, g.export())
     rng = np.random.default_rng(12)
     X = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
@@ -320,6 +323,7 @@ def test_matrix_core_rhs_matches_oracle(bfgpu, oracle, ctx):
                                 admissibility=bfgpu.CenterDistanceAdmissibility(2.0), acctype=np.complex64,
                                 farfieldonly=True, ctx=ctx)
     assert op.launch_stats()[-1, 7] > 0
+    op.set_option("pfree", 0)  # stored P: partial sums and their combine pass on the matrix cores
     imps = [oracle.import_bf(T, T, op.export(i)) for i in range(len(op.farinteractions))]
     Xs = np.asfortranarray(crandn(rng, 6000, 40)).astype(np.complex64)
     want = np.stack([oracle.matvec_many(imps, Xs[:, j].astype(np.complex128), nthreads=8, rows=6000)
@@ -459,3 +463,14 @@ def test_matrix_free_last_level_many_rhs(bfgpu, oracle, ctx):
         want = np.stack([oracle.matvec_many(imps, X[:, j].astype(np.complex128), nthreads=8, rows=n)
                          for j in cols], axis=1)
         assert rel(on[:, cols].astype(np.complex128), want) < 1e-5, name
+        # FP32 emulated on the bf16 matrix cores (bf16x3): operands carry
+        # about 17 bits (|x - x_h - x_l| <= 2^-16 |x|), so the bar is 1e-4,
+        # the compression tolerance, and the measured difference is printed
+        op.set_option("mfma_emul", 1)
+        assert op.get_option("mfma_emul") == 1
+        em = op @ X
+        em_ab = op.mul(Y0.copy(order="F"), np.asfortranarray(X[:, :33]), a, b)
+        op.set_option("mfma_emul", 0)
+        say(name, "bf16x3 vs FP32 matrix cores", rel(em, on), "vs oracle", rel(em[:, cols].astype(np.complex128), want))
+        assert rel(em, on) < 1e-4 and rel(em_ab, on_ab) < 1e-4, name
+        assert rel(em[:, cols].astype(np.complex128), want) < 1e-4, name
