@@ -720,13 +720,20 @@ void pack_tables(BF &bf) {
     bf.pfree_ok = bf.pfree_t_ok = false;
     bf.pf_n_items = 0;
     bool pf = bf.precision == BFGPU_C64 && bf.pf_k > 0.0 && nb > 0;
-    for (int64_t b = 0; pf && b < nb; ++b) {
-        const OneBF &ob = bf.bfs[b];
-        int64_t nc = 0;
-        for (const Block &k : ob.st.back().b)  // rows: whole observer leaves (their centres frame them)
-            nc += k.ident || (k.m > 0 && !test.isleaf(k.obs_out)) ? -(int64_t(1) << 40) : int64_t(k.n);
-        pf = ob.st.size() >= 2 && nc == int64_t(ob.pcols.size());  // P reads skeleton slots
+    double t_pf_check = 0, t_pf_cols = 0, t_pf_up = 0;
+    if (pf) {
+        std::atomic<bool> ok{true};
+        parallel_for(nb, [&](int64_t b) {
+            const OneBF &ob = bf.bfs[b];
+            int64_t nc = 0;
+            for (const Block &k : ob.st.back().b)  // rows: whole observer leaves (their centres frame them)
+                nc += k.ident || (k.m > 0 && !test.isleaf(k.obs_out)) ? -(int64_t(1) << 40) : int64_t(k.n);
+            if (!(ob.st.size() >= 2 && nc == int64_t(ob.pcols.size())))  // P reads skeleton slots
+                ok.store(false, std::memory_order_relaxed);
+        }, 1024);
+        pf = ok.load();
     }
+    t_pf_check = since(tpf);
     if (pf) {
         const int64_t NV = test.n_nodes;
         const int64_t nr = std::min<int64_t>(nb, 4 * int64_t(host_threads()));
@@ -791,21 +798,33 @@ void pack_tables(BF &bf) {
                     row[size_t(i)] = float4{float(u0), float(u1), float(u2), float(u0 * u0 + u1 * u1 + u2 * u2)};
                 }
             }, 256);
-            {  // the transposed kernel writes each column's slot: one writer per slot
-                std::unique_ptr<std::atomic<uint8_t>[]> seen(new std::atomic<uint8_t>[size_t(std::max<int64_t>(counter, 1))]);
-                parallel_for(std::max<int64_t>(counter, 1), [&](int64_t k) { seen[size_t(k)].store(0, std::memory_order_relaxed); }, 1 << 16);
+            {  // the transposed kernel writes each column's slot: one writer
+               // per slot.  Slots of different factorizations are disjoint
+               // ([slot_base[b], slot_base[b + 1])), so it suffices that the
+               // last-level blocks of each factorization read disjoint
+               // ranges of its skeleton vectors.
                 std::atomic<bool> once{true};
-                parallel_for(total, [&](int64_t i) {
-                    const size_t k = size_t(cin[size_t(i)]);
-                    if (seen[k].exchange(1, std::memory_order_relaxed)) once.store(false, std::memory_order_relaxed);
-                }, 1 << 16);
+                parallel_for(nb, [&](int64_t b) {
+                    thread_local std::vector<std::pair<int32_t, int32_t>> rg;
+                    const OneBF &ob = bf.bfs[b];
+                    const int32_t s = int32_t(ob.st.size()) - 1;
+                    const int64_t g0 = ix.first[ix.sf[b] + s];
+                    rg.clear();
+                    for (size_t q = 0; q < ob.st[s].b.size(); ++q)
+                        if (ob.st[s].b[q].n > 0) rg.emplace_back(sl_in[size_t(g0) + q], ob.st[s].b[q].n);
+                    std::sort(rg.begin(), rg.end());
+                    for (size_t q = 1; q < rg.size(); ++q)
+                        if (rg[q].first < rg[q - 1].first + rg[q - 1].second) once.store(false, std::memory_order_relaxed);
+                }, 256);
                 bf.pfree_t_ok = once.load();
             }
             bf.pf_order_built = false;
+            t_pf_cols = since(tpf) - t_pf_check;
             upload_tab(bf.pf_items, items);
             upload_tab(bf.pf_col, col);
             upload_tab(bf.pf_in, cin);
             upload_tab(bf.pf_row, row);
+            t_pf_up = since(tpf) - t_pf_check - t_pf_cols;
             bf.pf_n_items = int32_t(items.size());
             bf.pfree_ok = true;
         }
@@ -823,8 +842,10 @@ void pack_tables(BF &bf) {
     if (bf.use_sweep) build_sweep(bf);
     if (timing)
         std::fprintf(stderr, "[bfgpu] pack %.3fs: index+keys %.3fs, slots %.3fs, group %.3fs, split %.3fs, "
-                             "finish+upload %.3fs, matrix-free tables %.3fs, validate %.3fs\n",
-                     since(tp0), t_keys, t_slots, t_group, t_split, t_finish, t_pf, t_valid);
+                             "finish+upload %.3fs, matrix-free tables %.3fs (check %.3fs, columns %.3fs, upload %.3fs), "
+                             "validate %.3fs\n",
+                     since(tp0), t_keys, t_slots, t_group, t_split, t_finish, t_pf, t_pf_check, t_pf_cols, t_pf_up,
+                     t_valid);
 }
 
 // Streaming-kernel tables of a forward table (variant 9).  Consecutive

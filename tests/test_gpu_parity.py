@@ -236,10 +236,17 @@ def test_single_block_65536_properties(bfgpu, oracle, ctx):
     Xt = torch.from_numpy(np.ascontiguousarray(X[:, :4].T)).cuda().t()
     Yt = torch.zeros(4, n, dtype=torch.complex64, device="cuda").t()
     g32.mul(Yt, Xt)
-    assert rel(Yt.cpu().numpy().astype(np.complex128), Y[:, :4].astype(np.complex128)) < 1e-6
+    # the 64-RHS product emulates FP32 on the bf16 matrix cores in its last
+    # level (bf16x3, about 5e-6 from FP32): 2e-5 against the 1- and 4-RHS
+    # FP32 products; 1e-6 with FP32 matrix cores throughout
+    assert rel(Yt.cpu().numpy().astype(np.complex128), Y[:, :4].astype(np.complex128)) < 2e-5
     Y64 = g @ X.astype(np.complex128)
-    assert rel(Y.astype(np.complex128), Y64) < 1e-5
-    assert rel(Y[:, 0].astype(np.complex128), (g32 @ X[:, 0]).astype(np.complex128)) < 1e-6
+    assert rel(Y.astype(np.complex128), Y64) < 2e-5
+    assert rel(Y[:, 0].astype(np.complex128), (g32 @ X[:, 0]).astype(np.complex128)) < 2e-5
+    g32.set_option("mfma_emul", 0)
+    Yn = g32 @ X
+    g32.set_option("mfma_emul", 1)
+    assert rel(Yt.cpu().numpy().astype(np.complex128), Yn[:, :4].astype(np.complex128)) < 1e-6
     # fused sweep (1..8 RHS) and per-level launches agree; repeated sweeps too
     for nr in (1, 3, 8):
         Xs = np.asfortranarray(X[:, :nr])
