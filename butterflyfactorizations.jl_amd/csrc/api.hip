@@ -661,7 +661,7 @@ extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) 
     // the transposed copy, where a transposed table asks for it
     for (int32_t L = 0; L < bf.n_stage_launch; ++L)
         for (int q = 2; q < 4; ++q)
-            if (vals[8 * L + 2 * q] == 8) ensure_tcopy(bf);
+            if (vals[8 * L + 2 * q] == 8 || vals[8 * L + 2 * q] == 9) ensure_tcopy(bf);
     // validate everything before changing anything
     for (int pass = 0; pass < 2; ++pass)
         for (int32_t L = 0; L < bf.n_stage_launch; ++L) {
@@ -673,18 +673,17 @@ extern "C" int bfgpu_bf_set_tuning(bfgpu_bf *h, int32_t n, const int32_t *vals) 
                     continue;
                 }
                 const TileShape sh = unpack_tile(p);
-                BFG_REQUIRE(((v >= 0 && v <= 3) || (v == 8 && q >= 2) || (v == 9 && q < 2)) &&
-                                !(q >= 2 && v == 1) && p >= 0 &&
+                BFG_REQUIRE(((v >= 0 && v <= 3) || (v == 8 && q >= 2) || v == 9) && !(q >= 2 && v == 1) && p >= 0 &&
                                 p < TILE_PACK_LIMIT && tile_ok(sh) && !(v >= 3 && sh.g_shift != 0),
                             "bf_set_tuning: bad variant or shape");
-                BFG_REQUIRE(v != 8 || (bf.tcopy_built && (bf.factor_elems[L] == 0 ||
+                BFG_REQUIRE((v != 8 && !(v == 9 && q >= 2)) || (bf.tcopy_built && (bf.factor_elems[L] == 0 ||
                                                           (L < int32_t(bf.factors_t.size()) && bf.factors_t[L].p))),
                             "bf_set_tuning: the transposed copy of a level is not available (tcopy off, or "
                             "device memory short)");
                 if (pass == 1) {
                     if (v == 9)
                         build_stream(*t[q], bf.precision == BFGPU_C128 ? 16 : 8, sh.snld4 ? 4 : 8,
-                                     bf.factor_elems[L]);
+                                     bf.factor_elems[L], q >= 2);
                     t[q]->variant = v;
                     t[q]->tile = sh;
                 }

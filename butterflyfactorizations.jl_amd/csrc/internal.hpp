@@ -175,28 +175,33 @@ static_assert(sizeof(MvTaskC) == 32, "MvTaskC is 8 dwords");
 //   w0    = f0w | nw << 40 | X << 51
 //   info  = nt | H << 7 | G << 14: tasks, row pairs, lane groups per row pair
 //   rmask = bit p set where a task's row pairs start (its first is pair p)
-//   xmask = bit e (word e / 64) set where a distinct input vector starts in
-//           the chunk's LDS input region (entry e)
+//   xmask = bit e (word e / 64) set where an input vector starts in the
+//           chunk's LDS input region (entry e)
+//   input vectors: runs[r0 .. r0 + nruns), in LDS order
 struct StreamChunk {
     uint64_t w0;
     int32_t t0;       // first task (index into the stream task array)
     uint32_t info;
     uint64_t rmask;
     uint64_t xmask[4];
-    uint64_t pad;
+    int32_t r0, nruns;
 };
 static_assert(sizeof(StreamChunk) == 64, "StreamChunk is 16 dwords");
 // Task t of a chunk: out_ofs its output slot; a, xo the LDS offsets
-// (entries) of its block and of its input vector; m, n its block; hp the
-// chunk's row pairs before it.  (lin, key): the slot offset and the LDS
-// offset of the chunk's t-th distinct input vector (when it has one).
-//   ax  = a | xo << 11 | key << 21
+// (entries) of its block (m x n: its blocks side by side) and of its input
+// (its input vectors one after the other); hp the chunk's row pairs before
+// it.
+//   ax  = a | xo << 11
 //   mhn = m | hp << 8 | n << 15
 struct StreamTask {
-    int32_t lin, out_ofs;
+    int32_t lin, out_ofs;  // lin unused
     uint32_t ax, mhn;
 };
 static_assert(sizeof(StreamTask) == 16, "StreamTask is 4 dwords");
+// Input vector of a chunk: slot offset, LDS offset (entries).
+struct StreamRun {
+    int32_t lin, key;
+};
 // Input entries per chunk in LDS (1.5 KiB in FP32, 2 KiB in FP64; a
 // multiple of 64 and at most 256, the bits of xmask)
 template <typename R> constexpr int stream_xcap() { return sizeof(R) == 4 ? 192 : 128; }
@@ -260,6 +265,7 @@ struct StageTable {
     // transposed tables over the transposed copy: tasks_c / contribs with
     // every block's rows and columns swapped (built with the copy)
     DevBuf tasks_sw, contribs_sw;
+    hvec<int64_t> h_cofs;  // per contribution: its block's offset in the transposed copy (-1 identity)
     hvec<MvItem> h_items;
     hvec<MvContrib> h_contribs;
     hvec<MvTask> h_tasks;
@@ -279,11 +285,12 @@ struct StageTable {
     // and the other tasks (several contributions, identity blocks, blocks
     // too large for a chunk) as a dense-kernel task list
     bool stream_built = false;
+    bool stream_copy = false;  // built over the transposed copy (transposed tables)
     int32_t stream_nld = 0;
     int32_t n_schunks = 0, n_stasks = 0, n_srest = 0;
     int64_t stream_gap = 0;   // entries copied that belong to no block
     int64_t stream_xent = 0;  // input entries copied (distinct per chunk)
-    DevBuf s_chunks, s_tasks, s_rest;
+    DevBuf s_chunks, s_tasks, s_runs, s_rest;
 };
 
 constexpr int SWEEP_MAX_STAGES = 48;
@@ -376,7 +383,7 @@ void build_sweep(BF &bf);
 // (matvec.hip), built on first use.
 void build_pf_order(BF &bf);
 // Streaming-kernel tables of a forward table (matvec.hip); es: entry bytes.
-void build_stream(StageTable &S, int es, int nld, int64_t factor_elems);
+void build_stream(StageTable &S, int es, int nld, int64_t factor_elems, bool over_copy = false);
 
 // Host worker threads (OMP_NUM_THREADS caps the hardware count).
 int host_threads();
@@ -498,10 +505,13 @@ void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *fact
 // Streaming forward kernel over the chunks of a table (one RHS); the
 // table's other tasks are not launched here.
 void launch_stream(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
-                   int64_t in_shift, int n_cu);
-// Block transposition: for block q, dst[fofs + i n + k] = src[fofs + k m + i]
-// (int4 {fofs lo, fofs hi, m, n} per block).
-void launch_transpose_blocks(hipStream_t s, int prec, const int4 *blocks, int64_t n_blocks, const void *src,
+                   int64_t in_shift, int n_cu, bool conj = false);
+// Block transposition: for block q, dst[dofs + i n + k] = src[sofs + k m + i].
+struct TBlock {
+    int64_t sofs, dofs;
+    int32_t m, n;
+};
+void launch_transpose_blocks(hipStream_t s, int prec, const TBlock *blocks, int64_t n_blocks, const void *src,
                              void *dst);
 // Transposed copy of the stored blocks (built once when mode and device
 // memory allow; true when built) and its release.

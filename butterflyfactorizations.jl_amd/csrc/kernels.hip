@@ -1351,17 +1351,20 @@ __global__ __launch_bounds__(256) void dense_kernel(
 // owns an LDS region; per chunk it
 //   1. copies the range with aligned 16-byte loads (word q * 64 + lane in
 //      round q: every load instruction reads 1 KiB of consecutive bytes),
-//      and the chunk's distinct input vectors (an entry per lane and round;
-//      the entry's vector is the number of vector starts at or below it in
-//      the chunk's xmask, minus one), into registers, then into its LDS
-//      region;
+//      and the chunk's input vectors (an entry per lane and round; the
+//      entry's vector is the number of vector starts at or below it in the
+//      chunk's xmask, minus one; lane v holds vector v's slot and LDS
+//      offsets), into registers, then into its LDS region (a task that reads
+//      several vectors finds them one after the other);
 //   2. computes the chunk from LDS: lane (g, rho) takes row pair rho of the
 //      chunk (rows 2 i, 2 i + 1 of the task that holds it: the number of
 //      task starts at or below rho in rmask, minus one) over the columns
 //      k = g, g + G, ...; the G groups meet in a fixed shuffle tree and
 //      group 0 writes the outputs.
-// The copy of chunk c + 1 is issued before chunk c is computed, its task
-// descriptors one chunk earlier and its chunk record two chunks earlier, so
+// Entries are conjugated when CONJ (the adjoint over the transposed copy of
+// the blocks).  The copy of chunk c + 1 is issued before chunk c is
+// computed, its task and vector descriptors one chunk earlier and its chunk
+// record two chunks earlier, so
 // the loads of the next chunk are in flight while the current one is
 // computed.  No barriers: a wavefront reads only what it wrote (LDS
 // instructions of one wavefront complete in order).  The second row of an
@@ -1375,9 +1378,10 @@ __device__ __forceinline__ C<R> lds_c(const C<R> *p) {  // one entry, 8-byte ali
     return C<R>{v.x, v.y};
 }
 
-template <typename R, int NLD>
+template <typename R, int NLD, bool CONJ>
 __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restrict__ chunks, int32_t n_chunks,
                                                      const StreamTask *__restrict__ tasks,
+                                                     const StreamRun *__restrict__ runs,
                                                      const C<R> *__restrict__ F, C<R> *__restrict__ slots,
                                                      int64_t in_shift, int xcd_map) {
     constexpr int NW = NLD * 64;  // words per chunk
@@ -1409,13 +1413,17 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
 
     struct Rec {
         int64_t f0w;
-        int32_t nw, X, t0, nt, H, G;
+        int32_t nw, X, t0, nt, H, G, r0, nr;
         uint64_t rm, xm[XLD];
+    };
+    struct Desc {
+        uint4 t;  // lane t: task t of the chunk
+        int2 v;   // lane v: input vector v of the chunk (slot offset, LDS offset)
     };
     auto rec = [&](int64_t ci) -> Rec {  // nt = 0 past the wave's last chunk
         Rec r;
         r.f0w = 0;
-        r.nw = r.X = r.t0 = r.nt = 0;
+        r.nw = r.X = r.t0 = r.nt = r.r0 = r.nr = 0;
         r.H = r.G = 1;
         r.rm = 1;
 #pragma unroll
@@ -1432,13 +1440,18 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
             r.rm = k.rmask;
 #pragma unroll
             for (int q = 0; q < XLD; ++q) r.xm[q] = k.xmask[q];
+            r.r0 = k.r0;
+            r.nr = k.nruns;
         }
         return r;
     };
-    auto desc = [&](const Rec &r) -> uint4 {  // lane t holds task t of the chunk
-        return lane < r.nt ? reinterpret_cast<const uint4 *>(tasks)[r.t0 + lane] : make_uint4(0u, 0u, 0u, 0u);
+    auto desc = [&](const Rec &r) -> Desc {
+        Desc d;
+        d.t = lane < r.nt ? reinterpret_cast<const uint4 *>(tasks)[r.t0 + lane] : make_uint4(0u, 0u, 0u, 0u);
+        d.v = lane < r.nr ? reinterpret_cast<const int2 *>(runs)[r.r0 + lane] : make_int2(0, 0);
+        return d;
     };
-    auto issue = [&](const Rec &r, const uint4 &d, uint4(&V)[NLD], C<R>(&XV)[XLD]) {
+    auto issue = [&](const Rec &r, const Desc &d, uint4(&V)[NLD], C<R>(&XV)[XLD]) {
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int wi = q * 64 + lane;
@@ -1451,7 +1464,7 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
             const int e = q * 64 + lane;
             const int v = before + __popcll(r.xm[q] & at_or_below) - 1;
             before += __popcll(r.xm[q]);
-            const int kv = __shfl(int((d.z >> 21) & 1023), v, 64), lv = __shfl(int(d.x), v, 64);
+            const int kv = __shfl(d.v.y, v, 64), lv = __shfl(d.v.x, v, 64);
             if (e < r.X) XV[q] = in[int64_t(lv) + (e - kv)];
         }
     };
@@ -1463,7 +1476,8 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
         for (int q = 0; q < XLD; ++q)
             if (q * 64 < r.X) xl[q * 64 + lane] = XV[q];
     };
-    auto compute = [&](const Rec &r, const uint4 &d) {
+    auto compute = [&](const Rec &r, const Desc &dd) {
+        const uint4 &d = dd.t;
         const int H = r.H, G = r.G;
         // g = lane / H through a reciprocal: exact for operands below 2^7
         const int g = int((float(lane) + 0.5f) * __builtin_amdgcn_rcpf(float(H)));
@@ -1490,16 +1504,26 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    cfma(acc0, a0[u], xv[u]);
-                    cfma(acc1, a1[u], xv[u]);
+                    if (CONJ) {
+                        cfmac(acc0, a0[u], xv[u]);
+                        cfmac(acc1, a1[u], xv[u]);
+                    } else {
+                        cfma(acc0, a0[u], xv[u]);
+                        cfma(acc1, a1[u], xv[u]);
+                    }
                 }
                 pa += 4 * step;
                 px += 4 * G;
             }
             for (; k < n; k += G) {
                 const C<R> x = lds_c(px);
-                cfma(acc0, lds_c(pa), x);
-                cfma(acc1, lds_c(pa + 1), x);
+                if (CONJ) {
+                    cfmac(acc0, lds_c(pa), x);
+                    cfmac(acc1, lds_c(pa + 1), x);
+                } else {
+                    cfma(acc0, lds_c(pa), x);
+                    cfma(acc1, lds_c(pa + 1), x);
+                }
                 pa += step;
                 px += G;
             }
@@ -1530,12 +1554,12 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
 #pragma unroll
     for (int q = 0; q < XLD; ++q) XV[q] = C<R>{R(0), R(0)};
     Rec r0 = rec(c), r1 = rec(c + stride), r2 = rec(c + 2 * stride);
-    uint4 d0 = desc(r0), d1 = desc(r1);
+    Desc d0 = desc(r0), d1 = desc(r1);
     issue(r0, d0, V, XV);
     for (;;) {
         store(r0, V, XV);
         if (r1.nt > 0) issue(r1, d1, V, XV);
-        const uint4 d2 = desc(r2);
+        const Desc d2 = desc(r2);
         const Rec r3 = rec(c + 3 * stride);
         compute(r0, d0);
         if (r1.nt == 0) break;
@@ -2693,7 +2717,8 @@ static void launch_dense_t(hipStream_t s, const StageTable &t, const void *facto
     auto S = (C<R> *)slots;
     auto Y = (C<R> *)y;
     auto T = (tab == DENSE_SWAPPED ? t.tasks_sw : tab == DENSE_REST ? t.s_rest : t.tasks_c).as<const MvTaskC>();
-    auto P = (tab == DENSE_SWAPPED ? t.contribs_sw : t.contribs).as<const MvContrib>();
+    auto P = (tab == DENSE_SWAPPED || (tab == DENSE_REST && t.stream_copy) ? t.contribs_sw : t.contribs)
+                 .as<const MvContrib>();
     BFG_REQUIRE(T != nullptr && P != nullptr, "matvec: dense tables missing");
 #define BFG_DENSE(OUTK, CS)                                                                              \
     hipLaunchKernelGGL((dense_kernel<R, OUTK, NR, QW, CS>), grid, dim3(256), 0, s, T, n_tasks, P, F, S,  \
@@ -2778,29 +2803,36 @@ void launch_denset(hipStream_t s, int prec, int op, const StageTable &t, const v
 #undef BFG_DT
 }
 
-template <typename R, int NLD>
+template <typename R, int NLD, bool CONJ>
 static void launch_stream_t(hipStream_t s, const StageTable &t, const void *factors, void *slots, int64_t in_shift,
                             int n_cu) {
     const int64_t wgs = (int64_t(t.n_schunks) + 1) / 2;
     const int64_t cap = int64_t(std::max(n_cu, 1)) * std::max<int64_t>(t.tile.wg_per_cu, 1);
     dim3 grid(unsigned(std::max<int64_t>(1, std::min(wgs, cap))));
-    hipLaunchKernelGGL((stream_kernel<R, NLD>), grid, dim3(128), 0, s, t.s_chunks.as<const StreamChunk>(),
-                       t.n_schunks, t.s_tasks.as<const StreamTask>(), (const C<R> *)factors, (C<R> *)slots,
-                       in_shift, int(t.tile.xcd));
+    hipLaunchKernelGGL((stream_kernel<R, NLD, CONJ>), grid, dim3(128), 0, s, t.s_chunks.as<const StreamChunk>(),
+                       t.n_schunks, t.s_tasks.as<const StreamTask>(), t.s_runs.as<const StreamRun>(),
+                       (const C<R> *)factors, (C<R> *)slots, in_shift, int(t.tile.xcd));
     BFG_HIP(hipGetLastError());
 }
 
+template <typename R>
+static void launch_stream_r(hipStream_t s, const StageTable &t, const void *factors, void *slots, int64_t in_shift,
+                            int n_cu, bool conj) {
+    if (t.stream_nld == 4) {
+        if (conj) launch_stream_t<R, 4, true>(s, t, factors, slots, in_shift, n_cu);
+        else launch_stream_t<R, 4, false>(s, t, factors, slots, in_shift, n_cu);
+    } else {
+        if (conj) launch_stream_t<R, 8, true>(s, t, factors, slots, in_shift, n_cu);
+        else launch_stream_t<R, 8, false>(s, t, factors, slots, in_shift, n_cu);
+    }
+}
+
 void launch_stream(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
-                   int64_t in_shift, int n_cu) {
+                   int64_t in_shift, int n_cu, bool conj) {
     BFG_REQUIRE(t.stream_built && (t.stream_nld == 4 || t.stream_nld == 8), "matvec: streaming tables missing");
     if (t.n_schunks == 0) return;
-    if (prec == BFGPU_C128) {
-        if (t.stream_nld == 4) launch_stream_t<double, 4>(s, t, factors, slots, in_shift, n_cu);
-        else launch_stream_t<double, 8>(s, t, factors, slots, in_shift, n_cu);
-    } else {
-        if (t.stream_nld == 4) launch_stream_t<float, 4>(s, t, factors, slots, in_shift, n_cu);
-        else launch_stream_t<float, 8>(s, t, factors, slots, in_shift, n_cu);
-    }
+    if (prec == BFGPU_C128) launch_stream_r<double>(s, t, factors, slots, in_shift, n_cu, conj);
+    else launch_stream_r<float>(s, t, factors, slots, in_shift, n_cu, conj);
 }
 
 void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
@@ -2822,21 +2854,20 @@ void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *fact
 // read in column-major order (coalesced) and written transposed.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void transpose_blocks_kernel(const int4 *__restrict__ blocks, int64_t n_blocks,
+__global__ __launch_bounds__(256) void transpose_blocks_kernel(const TBlock *__restrict__ blocks, int64_t n_blocks,
                                                                const C<R> *__restrict__ src, C<R> *__restrict__ dst) {
     for (int64_t q = blockIdx.x; q < n_blocks; q += gridDim.x) {
-        const int4 b = blocks[q];
-        const int64_t fofs = int64_t((uint64_t(uint32_t(b.y)) << 32) | uint64_t(uint32_t(b.x)));
-        const int m = b.z, n = b.w;
+        const TBlock b = blocks[q];
+        const int m = b.m, n = b.n;
         const int64_t total = int64_t(m) * n;
         for (int64_t e = threadIdx.x; e < total; e += blockDim.x) {
             const int k = int(e / m), i = int(e - int64_t(k) * m);
-            dst[fofs + int64_t(i) * n + k] = src[fofs + e];
+            dst[b.dofs + int64_t(i) * n + k] = src[b.sofs + e];
         }
     }
 }
 
-void launch_transpose_blocks(hipStream_t s, int prec, const int4 *blocks, int64_t n_blocks, const void *src,
+void launch_transpose_blocks(hipStream_t s, int prec, const TBlock *blocks, int64_t n_blocks, const void *src,
                              void *dst) {
     if (n_blocks <= 0) return;
     dim3 grid(unsigned(std::min<int64_t>(n_blocks, int64_t(1) << 20)));

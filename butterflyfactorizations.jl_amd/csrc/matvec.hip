@@ -848,32 +848,59 @@ void pack_tables(BF &bf) {
                      t_valid);
 }
 
-// Streaming-kernel tables of a forward table (variant 9).  Consecutive
-// tasks that hold one factor block each (no identity block, output rows =
-// block rows, at most 128 rows), whose blocks follow each other in the
-// factor buffer with gaps of at most 16 entries, are cut into chunks of at
-// most nld KiB of 16-byte words, 64 tasks, STREAM_HMAX row pairs and
-// stream_xcap distinct input entries (tasks that read the same input vector
-// share one LDS copy of it).  Ranges of tasks are chunked in parallel (a
-// chunk never spans two ranges); every other task goes to a dense-kernel
-// list.
-void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
-    if (S.stream_built && S.stream_nld == nld) return;
+// Streaming-kernel tables of a launch table (variant 9): the forward tables
+// over the stored blocks, or (over_copy) the transposed tables over the
+// transposed copy of the blocks, where an item's blocks lie back to back.
+// A task streams when its output is a skeleton vector, it has no identity
+// block, its blocks have its output's rows (at most 128) and lie back to
+// back in the buffer (one block of m x sum n), and it reads at most 64
+// input vectors.  Consecutive streaming tasks whose blocks follow each
+// other in the buffer with gaps of at most 16 entries are cut into chunks of
+// at most nld KiB of 16-byte words, 64 tasks, STREAM_HMAX row pairs, 64
+// distinct input vectors and stream_xcap input entries (a task that reads
+// the same vectors as the one before it shares their LDS copy).  Ranges of
+// tasks are chunked in parallel (a chunk never spans two ranges); every
+// other task goes to a dense-kernel list.
+void build_stream(StageTable &S, int es, int nld, int64_t factor_elems, bool over_copy) {
+    if (S.stream_built && S.stream_nld == nld && S.stream_copy == over_copy) return;
     BFG_REQUIRE(nld == 4 || nld == 8, "pack: stream chunks of 4 or 8 KiB");
     BFG_REQUIRE(es == 8 || es == 16, "pack: entry size 8 or 16 bytes");
+    BFG_REQUIRE(!over_copy || S.h_cofs.size() == S.h_contribs.size(), "pack: transposed copy offsets missing");
     const int64_t E = 16 / es;                   // entries per word
     const int64_t cape = int64_t(nld) * 64 * E;  // entries per chunk
     const int32_t xcap = es == 8 ? stream_xcap<float>() : stream_xcap<double>();
     constexpr int64_t GAP = 16;
-    const int64_t n = int64_t(S.h_tasks.size());
+    const int64_t n = int64_t(S.h_items.size());
     const bool slot_out = S.out_kind == OUT_SLOT;
-    auto streams = [&](const MvTask &t) {
-        return slot_out && t.c_begin == t.c_end && t.fofs >= 0 && t.m >= 1 && t.m <= 2 * STREAM_HMAX && t.n >= 1 &&
-               t.out_len == t.m && int64_t(t.m) * t.n + (E - 1) <= cape && t.n <= xcap;
+    // contribution k of the table as seen by the kernel: block offset, rows,
+    // columns, input
+    auto cf = [&](int32_t k) { return over_copy ? S.h_cofs[size_t(k)] : S.h_contribs[size_t(k)].fofs; };
+    auto cm = [&](int32_t k) { return over_copy ? S.h_contribs[size_t(k)].n : S.h_contribs[size_t(k)].m; };
+    auto cn = [&](int32_t k) { return over_copy ? S.h_contribs[size_t(k)].m : S.h_contribs[size_t(k)].n; };
+    struct TV {
+        bool ok;
+        int64_t f0;
+        int32_t m, ntot;
+    };
+    auto view = [&](int64_t i) {
+        const MvItem &it = S.h_items[size_t(i)];
+        TV v{false, -1, it.out_len, 0};
+        if (!slot_out || it.c_end <= it.c_begin || it.c_end - it.c_begin > 64 || v.m < 1 || v.m > 2 * STREAM_HMAX)
+            return v;
+        v.f0 = cf(it.c_begin);
+        int64_t f = v.f0;
+        for (int32_t k = it.c_begin; k < it.c_end; ++k) {
+            if (cf(k) < 0 || cf(k) != f || cm(k) != v.m || cn(k) < 1) return v;
+            f += int64_t(v.m) * cn(k);
+            v.ntot += cn(k);
+        }
+        v.ok = int64_t(v.m) * v.ntot + (E - 1) <= cape && v.ntot <= xcap;
+        return v;
     };
     struct Part {
         hvec<StreamChunk> ch;
         hvec<StreamTask> tk;
+        hvec<StreamRun> rn;
         hvec<MvTaskC> rest;
         int64_t gap = 0, xent = 0;
     };
@@ -882,13 +909,21 @@ void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
     parallel_for(nr, [&](int64_t r) {
         Part &P = parts[size_t(r)];
         const int64_t lo = n * r / nr, hi = n * (r + 1) / nr;
-        struct Run { int32_t in_ofs, n, xo; };
-        Run runs[64];
-        int nruns = 0;
         bool open = false;
         int64_t f0w = 0, fend = 0;
-        int32_t t0 = 0, nt = 0, H = 0, X = 0;
+        int32_t t0 = 0, nt = 0, H = 0, X = 0, r0 = 0;
         uint64_t rmask = 0, xmask[4] = {0, 0, 0, 0};
+        int64_t prev = -1;      // previous task of the chunk (its input vectors)
+        int32_t prev_xo = 0;
+        auto same_inputs = [&](int64_t a, int64_t b) {
+            const MvItem &x = S.h_items[size_t(a)], &y = S.h_items[size_t(b)];
+            if (x.c_end - x.c_begin != y.c_end - y.c_begin) return false;
+            for (int32_t q = 0; q < x.c_end - x.c_begin; ++q)
+                if (S.h_contribs[size_t(x.c_begin + q)].in_ofs != S.h_contribs[size_t(y.c_begin + q)].in_ofs ||
+                    cn(x.c_begin + q) != cn(y.c_begin + q))
+                    return false;
+            return true;
+        };
         auto close = [&]() {
             if (!open) return;
             const int64_t nw = (fend - f0w * E + E - 1) / E;
@@ -898,74 +933,77 @@ void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
             k.info = uint32_t(nt) | uint32_t(H) << 7 | uint32_t(64 / H) << 14;
             k.rmask = rmask;
             for (int q = 0; q < 4; ++q) k.xmask[q] = xmask[q];
-            k.pad = 0;
+            k.r0 = r0;  // part-relative until the parts are joined
+            k.nruns = int32_t(P.rn.size()) - r0;
             P.ch.push_back(k);
-            // task q of the chunk carries the chunk's q-th distinct input vector
-            for (int q = 0; q < nruns; ++q) {
-                StreamTask &st = P.tk[size_t(t0 + q)];
-                st.lin = runs[q].in_ofs;
-                st.ax |= uint32_t(runs[q].xo) << 21;
-            }
             P.xent += X;
             open = false;
         };
         for (int64_t i = lo; i < hi; ++i) {
-            const MvTask &t = S.h_tasks[size_t(i)];
-            if (!streams(t)) {
+            const MvItem &it = S.h_items[size_t(i)];
+            const TV v = view(i);
+            if (!v.ok) {
                 close();
-                P.rest.push_back(MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.m, t.n, t.c_begin, t.c_end});
+                const MvTask &t = S.h_tasks[size_t(i)];
+                if (over_copy)
+                    P.rest.push_back(MvTaskC{it.c_begin < it.c_end ? cf(it.c_begin) : t.fofs, t.in_ofs, t.out_ofs,
+                                             t.n, t.m, t.c_begin, t.c_end});
+                else
+                    P.rest.push_back(MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.m, t.n, t.c_begin, t.c_end});
                 continue;
             }
-            const int32_t h = (t.m + 1) / 2;
-            const int64_t end = t.fofs + int64_t(t.m) * t.n;
-            int ri = -1;
-            if (open)
-                for (int q = nruns - 1; q >= 0; --q)
-                    if (runs[q].in_ofs == t.in_ofs && runs[q].n == t.n) {
-                        ri = q;
-                        break;
-                    }
-            const bool fits = open && t.fofs >= fend && t.fofs - fend <= GAP && end - f0w * E <= cape && nt < 64 &&
-                              H + h <= STREAM_HMAX && X + (ri < 0 ? t.n : 0) <= xcap;
+            const int32_t h = (v.m + 1) / 2;
+            const int32_t nb = it.c_end - it.c_begin;
+            const int64_t end = v.f0 + int64_t(v.m) * v.ntot;
+            const bool share = open && prev >= 0 && same_inputs(prev, i);
+            const int32_t nruns = open ? int32_t(P.rn.size()) - r0 : 0;
+            const bool fits = open && v.f0 >= fend && v.f0 - fend <= GAP && end - f0w * E <= cape && nt < 64 &&
+                              H + h <= STREAM_HMAX &&
+                              (share || (X + v.ntot <= xcap && nruns + nb <= 64));
             if (!fits) {
                 close();
                 open = true;
-                f0w = t.fofs / E;
+                f0w = v.f0 / E;
                 fend = f0w * E;
                 t0 = int32_t(P.tk.size());
+                r0 = int32_t(P.rn.size());
                 nt = H = X = 0;
-                nruns = 0;
                 rmask = 0;
                 xmask[0] = xmask[1] = xmask[2] = xmask[3] = 0;
-                ri = -1;
+                prev = -1;
             }
-            P.gap += t.fofs - fend;
+            P.gap += v.f0 - fend;
             int32_t xo;
-            if (ri < 0) {  // a new distinct input vector at LDS entry X
+            if (open && prev >= 0 && same_inputs(prev, i)) {
+                xo = prev_xo;
+            } else {  // the task's input vectors, listed at LDS entries X, X + n_0, ...
                 xo = X;
-                runs[nruns++] = Run{t.in_ofs, t.n, X};
-                xmask[X / 64] |= uint64_t(1) << (X % 64);
-                X += t.n;
-            } else {
-                xo = runs[ri].xo;
+                for (int32_t k = it.c_begin; k < it.c_end; ++k) {
+                    xmask[X / 64] |= uint64_t(1) << (X % 64);
+                    P.rn.push_back(StreamRun{S.h_contribs[size_t(k)].in_ofs, X});
+                    X += cn(k);
+                }
             }
             rmask |= uint64_t(1) << H;
             StreamTask st;
             st.lin = 0;
-            st.out_ofs = t.out_ofs;
-            st.ax = uint32_t(t.fofs - f0w * E) | uint32_t(xo) << 11;
-            st.mhn = uint32_t(t.m) | uint32_t(H) << 8 | uint32_t(t.n) << 15;
+            st.out_ofs = it.out_ofs;
+            st.ax = uint32_t(v.f0 - f0w * E) | uint32_t(xo) << 11;
+            st.mhn = uint32_t(v.m) | uint32_t(H) << 8 | uint32_t(v.ntot) << 15;
             P.tk.push_back(st);
             ++nt;
             H += h;
             fend = end;
+            prev = i;
+            prev_xo = xo;
         }
         close();
     }, 1);
-    int64_t n_ch = 0, n_tk = 0, n_rest = 0, gap = 0, xent = 0;
+    int64_t n_ch = 0, n_tk = 0, n_rest = 0, n_rn = 0, gap = 0, xent = 0;
     for (const Part &P : parts) {
         n_ch += int64_t(P.ch.size());
         n_tk += int64_t(P.tk.size());
+        n_rn += int64_t(P.rn.size());
         n_rest += int64_t(P.rest.size());
         gap += P.gap;
         xent += P.xent;
@@ -973,17 +1011,22 @@ void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
     BFG_REQUIRE(n_tk < (int64_t(1) << 31) && n_ch < (int64_t(1) << 31), "pack: too many streaming tasks");
     hvec<StreamChunk> ch(static_cast<size_t>(n_ch));
     hvec<StreamTask> tk(static_cast<size_t>(n_tk));
+    BFG_REQUIRE(n_rn < (int64_t(1) << 31), "pack: too many streaming input vectors");
     hvec<MvTaskC> rest(static_cast<size_t>(n_rest));
+    hvec<StreamRun> rn(static_cast<size_t>(n_rn));
     {
-        int64_t oc = 0, ot = 0, orr = 0;
-        std::vector<int64_t> bc(static_cast<size_t>(nr)), bt(static_cast<size_t>(nr)), br(static_cast<size_t>(nr));
+        int64_t oc = 0, ot = 0, orr = 0, orn = 0;
+        std::vector<int64_t> bc(static_cast<size_t>(nr)), bt(static_cast<size_t>(nr)), br(static_cast<size_t>(nr)),
+            bn(static_cast<size_t>(nr));
         for (int64_t r = 0; r < nr; ++r) {
             bc[size_t(r)] = oc;
             bt[size_t(r)] = ot;
             br[size_t(r)] = orr;
+            bn[size_t(r)] = orn;
             oc += int64_t(parts[size_t(r)].ch.size());
             ot += int64_t(parts[size_t(r)].tk.size());
             orr += int64_t(parts[size_t(r)].rest.size());
+            orn += int64_t(parts[size_t(r)].rn.size());
         }
         std::atomic<bool> bad{false};
         parallel_for(nr, [&](int64_t r) {
@@ -991,19 +1034,25 @@ void build_stream(StageTable &S, int es, int nld, int64_t factor_elems) {
             for (size_t q = 0; q < P.ch.size(); ++q) {
                 StreamChunk k = P.ch[q];
                 k.t0 += int32_t(bt[size_t(r)]);
+                k.r0 += int32_t(bn[size_t(r)]);
                 const int64_t f0 = int64_t(k.w0 & ((uint64_t(1) << 40) - 1)) * E;
                 const int64_t nw = int64_t((k.w0 >> 40) & 2047);
-                if (f0 < 0 || f0 + nw * E > factor_elems + FACTOR_PAD || nw > cape / E) bad.store(true);
+                if (f0 < 0 || f0 + nw * E > factor_elems + FACTOR_PAD || nw > cape / E || k.nruns < 1 ||
+                    k.nruns > 64)
+                    bad.store(true);
                 ch[size_t(bc[size_t(r)]) + q] = k;
             }
             std::copy(P.tk.begin(), P.tk.end(), tk.begin() + bt[size_t(r)]);
+            std::copy(P.rn.begin(), P.rn.end(), rn.begin() + bn[size_t(r)]);
             std::copy(P.rest.begin(), P.rest.end(), rest.begin() + br[size_t(r)]);
         }, 1);
         BFG_REQUIRE(!bad.load(), "pack: streaming chunk outside its factor buffer");
     }
     upload_tab(S.s_chunks, ch);
     upload_tab(S.s_tasks, tk);
+    upload_tab(S.s_runs, rn);
     upload_tab(S.s_rest, rest);
+    S.stream_copy = over_copy;
     S.n_schunks = int32_t(n_ch);
     S.n_stasks = int32_t(n_tk);
     S.n_srest = int32_t(n_rest);
@@ -1259,16 +1308,24 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
             shift = bf.gather_ofs;
             if (tl) tl->launched(s, {L, 2, 0, 0, 0, 1});
         }
+        const void *ft = L < int32_t(bf.factors_t.size()) ? bf.factors_t[L].p : nullptr;
         if (t.variant == 9 && fwd && nrhs == 1) {
             // streaming kernel over its chunks, dense kernel over the other tasks
             launch_stream(s, bf.precision, t, bf.factors[L].p, slots, shift, bf.ctx->n_cu);
             launch_dense(s, bf.precision, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout, nrhs,
                          alpha, beta, fuse, bf.ctx->n_cu, DENSE_REST);
+        } else if (t.variant == 9 && !fwd && nrhs == 1) {
+            // the same over the transposed copy (its items' blocks back to back)
+            BFG_REQUIRE(bf.tcopy_built && t.stream_built && t.stream_copy && (ft || bf.factor_elems[L] == 0),
+                        "matvec: transposed copy of the blocks or its streaming tables missing");
+            const int conj = op == BFGPU_OP_C ? 1 : 0;
+            launch_stream(s, bf.precision, t, ft ? ft : bf.factors[L].p, slots, shift, bf.ctx->n_cu, conj != 0);
+            launch_dense(s, bf.precision, t, ft ? ft : bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
+                         nrhs, alpha, beta, fuse, bf.ctx->n_cu, DENSE_REST, conj);
         } else if ((t.variant == 3 || t.variant == 9) && fwd) {
             launch_dense(s, bf.precision, t, bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout, nrhs,
                          alpha, beta, fuse, bf.ctx->n_cu);
-        } else if (t.variant == 8) {  // forward dense kernel over the transposed copy
-            const void *ft = L < int32_t(bf.factors_t.size()) ? bf.factors_t[L].p : nullptr;
+        } else if (t.variant == 8 || t.variant == 9) {  // forward dense kernel over the transposed copy
             BFG_REQUIRE(!fwd && bf.tcopy_built && (ft || bf.factor_elems[L] == 0),
                         "matvec: transposed copy of the blocks missing");
             launch_dense(s, bf.precision, t, ft ? ft : bf.factors[L].p, slots, bf.slot_len, shift, y, ldy, pout,
@@ -1284,7 +1341,7 @@ static void run_table(BF &bf, int op, const StageTable &t, int32_t L, int64_t nr
         launch_stage(s, bf.precision, op, t, bf.factors[L].p, x, ldx, y, ldy, slots, bf.slot_len, pin,
                      pout, nrhs, alpha, beta, fuse);
     }
-    const int32_t ran = t.variant == 9 && (!fwd || nrhs != 1) ? 3 : t.variant;  // streams at one RHS only
+    const int32_t ran = t.variant == 9 && nrhs != 1 ? (fwd ? 3 : 8) : t.variant;  // streams at one RHS only
     if (tl) tl->launched(s, {L, table, ran, t.out_kind, int32_t(pack_tile(t.tile)), nr_per_lane(nrhs)});
 }
 
@@ -1705,35 +1762,51 @@ bool ensure_tcopy(BF &bf) {
         ft[L].alloc(size_t(bf.factor_elems[L] + FACTOR_PAD) * es);
         BFG_HIP(hipMemsetAsync(static_cast<char *>(ft[L].p) + size_t(bf.factor_elems[L]) * es, 0,
                                size_t(FACTOR_PAD) * es, s));
-        hvec<int4> blk;
-        for (const StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()})
-            if (u)
-                for (const MvContrib &c : u->h_contribs)
-                    if (c.fofs >= 0 && int64_t(c.m) * c.n > 0) {
-                        BFG_REQUIRE(c.fofs + int64_t(c.m) * c.n <= bf.factor_elems[L],
-                                    "transposed copy: block outside its level's factors");
-                        blk.push_back(int4{int32_t(uint32_t(uint64_t(c.fofs))),
-                                           int32_t(uint32_t(uint64_t(c.fofs) >> 32)), c.m, c.n});
-                    }
+        // the blocks of the level's transposed tables (main, then partial
+        // sums) at running offsets in their contribution order: every
+        // transposed item's blocks lie back to back, item after item
+        hvec<TBlock> blk;
+        int64_t run = 0;
+        for (StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()}) {
+            if (!u) continue;
+            u->h_cofs.resize(u->h_contribs.size());
+            for (size_t q = 0; q < u->h_contribs.size(); ++q) {
+                const MvContrib &c = u->h_contribs[q];
+                if (c.fofs < 0 || int64_t(c.m) * c.n == 0) {
+                    u->h_cofs[q] = c.fofs < 0 ? -1 : run;
+                    continue;
+                }
+                BFG_REQUIRE(c.fofs + int64_t(c.m) * c.n <= bf.factor_elems[L],
+                            "transposed copy: block outside its level's factors");
+                blk.push_back(TBlock{c.fofs, run, c.m, c.n});
+                u->h_cofs[q] = run;
+                run += int64_t(c.m) * c.n;
+            }
+        }
+        BFG_REQUIRE(run <= bf.factor_elems[L], "transposed copy: blocks overlap");
         if (blk.empty()) continue;
-        d_blk.reserve(blk.size() * sizeof(int4));
-        BFG_HIP(hipMemcpyAsync(d_blk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-        launch_transpose_blocks(s, bf.precision, d_blk.as<const int4>(), int64_t(blk.size()), bf.factors[L].p,
+        d_blk.reserve(blk.size() * sizeof(TBlock));
+        BFG_HIP(hipMemcpyAsync(d_blk.p, blk.data(), blk.size() * sizeof(TBlock), hipMemcpyHostToDevice, s));
+        launch_transpose_blocks(s, bf.precision, d_blk.as<const TBlock>(), int64_t(blk.size()), bf.factors[L].p,
                                 ft[L].p);
         BFG_HIP(hipStreamSynchronize(s));
     }
     for (int32_t L = 0; L < nL; ++L)
         for (StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()}) {
             if (!u) continue;
+            const bool cp = copied(L) && u->h_cofs.size() == u->h_contribs.size();
+            auto cofs = [&](size_t q) { return cp ? u->h_cofs[q] : u->h_contribs[q].fofs; };
             hvec<MvTaskC> tc(u->h_tasks.size());
             parallel_for(int64_t(tc.size()), [&](int64_t i) {
                 const MvTask &t = u->h_tasks[i];
-                tc[i] = MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.n, t.m, t.c_begin, t.c_end};
+                const int64_t f = u->h_items[i].c_begin < u->h_items[i].c_end ? cofs(size_t(u->h_items[i].c_begin))
+                                                                            : t.fofs;
+                tc[i] = MvTaskC{f, t.in_ofs, t.out_ofs, t.n, t.m, t.c_begin, t.c_end};
             }, 4096);
             hvec<MvContrib> cc(u->h_contribs.size());
             parallel_for(int64_t(cc.size()), [&](int64_t i) {
                 const MvContrib &c = u->h_contribs[i];
-                cc[i] = MvContrib{c.fofs, c.n, c.m, c.in_ofs, c.pad};
+                cc[i] = MvContrib{cofs(size_t(i)), c.n, c.m, c.in_ofs, c.pad};
             }, 4096);
             upload_tab(u->tasks_sw, tc);
             upload_tab(u->contribs_sw, cc);
@@ -1748,13 +1821,22 @@ void release_tcopy(BF &bf) {
     for (int32_t L = 0; L < int32_t(bf.tr.size()); ++L)
         for (StageTable *u : {&bf.tr[L], bf.tr[L].pre.get()}) {
             if (!u) continue;
-            if (u->variant == 8) {  // back to a lane per output column, same lanes per task
+            if (u->variant == 8 || u->variant == 9) {  // back to a lane per output column, same lanes per task
                 u->variant = 3;
                 u->tile.csplit = 0;
                 u->tile.qw4 = 0;
+                u->tile.snld4 = 0;
             }
             u->tasks_sw = DevBuf();
             u->contribs_sw = DevBuf();
+            u->h_cofs = hvec<int64_t>();
+            if (u->stream_copy) {
+                u->stream_built = false;
+                u->s_chunks = DevBuf();
+                u->s_tasks = DevBuf();
+                u->s_runs = DevBuf();
+                u->s_rest = DevBuf();
+            }
         }
     bf.factors_t.clear();
     bf.tcopy_built = false;
@@ -1851,16 +1933,19 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                         sh.qw4 = int8_t(q4);
                         consider(op == BFGPU_OP_N ? 3 : 8, sh);
                     }
-        // the streaming kernel (forward, one RHS): chunks of 8 and 4 KiB;
-        // its other tasks keep the best dense shape
-        if (op == BFGPU_OP_N && nrhs == 1 && t.out_kind == OUT_SLOT) {
-            TileShape base = bv == 3 ? bs : t0;
-            if (bv != 3) {
+        // the streaming kernel (one RHS; forward over the stored blocks,
+        // transposed over the transposed copy): chunks of 8 and 4 KiB; its
+        // other tasks keep the best dense shape
+        if (nrhs == 1 && t.out_kind == OUT_SLOT && (op == BFGPU_OP_N || tc)) {
+            const bool dense_best = bv == 3 || bv == 8;
+            TileShape base = dense_best ? bs : t0;
+            if (!dense_best) {
                 base.rp_shift = int8_t(std::max<int>(4, std::min<int>(6, base.rp_shift + base.g_shift)));
                 base.g_shift = 0;
+                base.csplit = 0;
             }
             for (int nld : {8, 4}) {
-                build_stream(t, es, nld, bf.factor_elems[L]);
+                build_stream(t, es, nld, bf.factor_elems[L], op != BFGPU_OP_N);
                 for (int wg : {4, 8}) {
                     TileShape sh = base;
                     sh.snld4 = int8_t(nld == 4 ? 1 : 0);
@@ -1882,7 +1967,7 @@ void autotune(BF &bf, int64_t nrhs, int32_t iters, double *stage_ms, int32_t *va
                 consider(bv, sh);
             }
         }
-        if (bv == 9) build_stream(t, es, bs.snld4 ? 4 : 8, bf.factor_elems[L]);
+        if (bv == 9) build_stream(t, es, bs.snld4 ? 4 : 8, bf.factor_elems[L], op != BFGPU_OP_N);
         t.variant = bv;
         t.tile = bs;
         return best;

@@ -442,7 +442,8 @@ def test_matrix_free_last_level_many_rhs(bfgpu, oracle, ctx):
     sq = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bfgpu.BlockTree(To, Ts), 0, 0, K, 1e-4,
                            acctype=np.complex64, ctx=ctx)
     for name, op, n, trees in (("sphere", sphere, 6000, (T, T)), ("squares", sq, 4096, (To, Ts))):
-        assert op.get_option("pfree") == 1
+        assert op.get_option("pfree") == 1 and op.get_option("mfma_emul") == 1
+        op.set_option("mfma_emul", 0)  # FP32 matrix cores first
         X = np.asfortranarray(crandn(rng, n, 70)).astype(np.complex64)
         Y0 = np.asfortranarray(crandn(rng, n, 33)).astype(np.complex64)
         a, b = 0.5 - 0.25j, -1.5 + 0.5j

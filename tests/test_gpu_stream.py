@@ -36,6 +36,21 @@ def ctx(bfgpu):
     return bfgpu.default_context()
 
 
+def _force_stream_tr(bfgpu, g, kib, skip_last=False):
+    """Every transposed table (and partial-sum table) on the streaming kernel
+    over the transposed copy; skip_last leaves the last level (matrix-free,
+    not copied) alone."""
+    sh = bfgpu.ButterflyFactorization.tile_shape(4, 0, 0, 7, 0, 0, 1 if kib == 4 else 0)
+    v = g.tuning().copy()
+    n = len(v) // 8
+    for L in range(n - 1 if skip_last else n):
+        for q in (2, 3):
+            if v[8 * L + 2 * q] >= 0:
+                v[8 * L + 2 * q] = 9
+                v[8 * L + 2 * q + 1] = sh
+    g.set_tuning(v)
+
+
 def _check(g, refs, X, bar):
     """One RHS on every direction (the streaming kernel runs forward at one
     RHS), and 2 / 8 RHS forward (the dense kernel takes those)."""
@@ -100,6 +115,27 @@ def test_stream_two_squares_fp64_matches_oracle(bfgpu, oracle, ctx):
     _check(g, refs, X, 1e-12)
     with pytest.raises(bfgpu.ArgumentError):
         g.set_option("stream", 6)
+    # adjoint and transpose on the streaming kernel over the transposed copy
+    # (an item's blocks back to back, its several input vectors copied one
+    # after the other); several RHS take the dense kernel over the copy
+    g.set_option("tcopy", 1)
+    for kib in (8, 4):
+        say("transposed tables streaming over the copy,", kib, "KiB")
+        _force_stream_tr(bfgpu, g, kib)
+        assert all(g.tuning()[8 * L + 4] == 9 for L in range(len(g)))
+        for name in ("C", "T"):
+            op = g.adjoint() if name == "C" else g.transpose()
+            got = op @ X[:, 0].copy()
+            assert rel(got, refs[name][:, 0]) < 1e-12, (name, kib)
+            got = op @ np.asfortranarray(X[:, :3])
+            assert rel(got, refs[name][:, :3]) < 1e-12, (name, kib, 3)
+        Y0 = crandn(np.random.default_rng(24), n, 1)[:, 0]
+        a, b = 0.5 - 0.25j, -1.5 + 0.5j
+        got = g.adjoint().mul(Y0.copy(), X[:, 0].copy(), a, b)
+        assert rel(got, a * refs["C"][:, 0] + b * Y0) < 1e-12
+    g.set_option("tcopy", 0)  # released: variant 9 of the transposed tables goes back to the lazy kernels
+    assert all(g.tuning()[8 * L + 4] != 9 for L in range(len(g)))
+    assert rel(g.adjoint() @ X[:, 0].copy(), refs["C"][:, 0]) < 1e-12
 
 
 def test_stream_sphere_far_field_matches_oracle(bfgpu, oracle, ctx):
@@ -138,3 +174,11 @@ def test_stream_sphere_far_field_matches_oracle(bfgpu, oracle, ctx):
             assert rel(got.astype(np.complex128), dense.astype(np.complex128)) < bar
         op.autotune(nrhs=1, iters=2)
         _check(op, refs, Xa, bar)
+        # the adjoint / transpose streaming over the transposed copy (the
+        # FP32 last level stays matrix-free and is not copied)
+        op.set_option("tcopy", 1)
+        _force_stream_tr(bfgpu, op, 8, skip_last=acc == np.complex64)
+        for name in ("C", "T"):
+            view = op.adjoint() if name == "C" else op.transpose()
+            got = view @ Xa[:, 0].copy()
+            assert rel(got.astype(np.complex128), refs[name][:, 0]) < bar, name
