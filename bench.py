@@ -125,7 +125,7 @@ def kernel_name(d, op, real):
     if variant == 10:  # matrix-free last level on the matrix cores
         return f"pfree_mfma_kernel<{(nr + 15) // 16}>"
     if variant == 9:  # streaming kernel (its other tasks, if any, run the dense kernel in the same slot)
-        return f"stream_kernel<{real}, {4 if (cs >> 28) & 1 else 8}>"
+        return f"stream_kernel<{real}, {4 if (cs >> 28) & 1 else 8}, {'true' if op == 2 else 'false'}>"
     if variant == 5:
         return f"mfma_rhs_kernel<{outk}, {(nr + 15) // 16}>"
     if variant == 4:
