@@ -271,10 +271,11 @@ struct Engine {
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
     // residual judge: tasks and results
-    DevBuf d_jtask, d_jres;
+    DevBuf d_jq, d_jres;
     int64_t n_comp = 0, n_retry = 0, n_rank0 = 0, n_judged = 0, n_fallback = 0;
     double t_host = 0, t_qr = 0, t_interp = 0;  // seconds: sample prep, fill+QR, interpolation
     double t_judge = 0;                         // seconds: held-out rows, fill, judge (part of t_qr)
+    double t_judge_gpu = 0;                     // of which the judge kernel and its copies
     double t_guess = 0, t_trim = 0;             // seconds: rank guesses, factor-buffer trims
     // host lists of a level's batches, kept across levels: their capacity is
     // reused instead of hundreds of MB being unmapped and faulted in again
@@ -289,10 +290,7 @@ struct Engine {
     hvec<InterpTask> it;
     std::vector<float> gfeat;
     std::vector<int32_t> gpred;
-    std::vector<int64_t> j_at;  // residual judge
-    hvec<JudgeTask> j_task;
-    std::vector<int32_t> j_who;
-    hvec<double> j_res;
+    hvec<double> j_res;  // residual judge
     std::vector<uint8_t> j_fail;
     std::vector<uint8_t> e_dbl;  // per batch: doubled, held-out rows
     std::vector<int32_t> e_jm;
@@ -345,15 +343,15 @@ struct Engine {
 
     void release() {
         for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z,
-                          &d_jtask, &d_jres})
+                          &d_jq, &d_jres})
             *d = DevBuf();
         Engine *e = this;
         auto drop = [](auto &v) { std::remove_reference_t<decltype(v)>().swap(v); };
         for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->hpiv, &e->jq, &e->jpos, &e->gpred,
-                        &e->j_who, &e->e_jm})
+                        &e->e_jm})
             drop(*v);
         for (auto *v : {&e->rofs, &e->cofs, &e->zo, &e->jr, &e->jz, &e->jrf, &e->jzf, &e->jcnt, &e->tofs_run,
-                        &e->it_pos, &e->it_big, &e->sk_run, &e->j_at})
+                        &e->it_pos, &e->it_big, &e->sk_run})
             drop(*v);
         drop(hrows);
         drop(hcols);
@@ -362,7 +360,6 @@ struct Engine {
         drop(qt);
         drop(it);
         drop(gfeat);
-        drop(j_task);
         drop(j_res);
         drop(j_fail);
         drop(e_dbl);
@@ -461,64 +458,61 @@ struct Engine {
         return std::min(o.judge_rows, c.n_obs - c.rows);
     }
 
-    // Residual judge of the compressions `acc` of the current batch that
-    // the pivoted QR accepted and that have held-out tiles (jq[a]: batch
-    // index, rows [jr[a], jr[a + 1]) of the held-out rows, tile at jz[a]):
-    // a compression passes when the largest residual column norm of its
-    // interpolative decomposition on the held-out tile is at most
-    // judge_slack * eps times the tile's largest column norm; one that fails
-    // is redone by plain adaptive RRQR from max(ceil(c_tau ln(1/eps)), 10,
-    // 2 * rows) rows.  The QR tiles, pivots and held-out tiles of the batch
-    // are still on the device.
-    void judge(std::vector<Comp> &cs, const std::vector<int32_t> &pend, const std::vector<int32_t> &acc,
-               const hvec<QrTask> &qt, const std::vector<int32_t> &jq, const std::vector<int64_t> &jr,
-               const std::vector<int64_t> &jz, const std::vector<int32_t> &jpos, std::vector<int32_t> &retry) {
+    // Residual judge of the held-out tiles of the current batch (jq[a]:
+    // batch index of tile a, the tile's fill task at np + a) whose
+    // compression the pivoted QR accepted (dbl[q] == 0; doubled ones are
+    // judged after their retry): a compression passes when the largest
+    // residual column norm of its interpolative decomposition on the
+    // held-out tile is at most judge_slack * eps times the tile's largest
+    // column norm; one that fails is redone by plain adaptive RRQR from
+    // max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.  The QR tiles, pivots,
+    // ranks, held-out tiles and their descriptors are still on the device:
+    // only the batch indices go up and two numbers per tile come back.
+    void judge(std::vector<Comp> &cs, const std::vector<int32_t> &pend, const std::vector<uint8_t> &dbl, size_t np,
+               std::vector<int32_t> &retry) {
         auto t0 = clk::now();
-        (void)jq;
-        // the accepted compressions with held-out tiles, in batch order
-        const int64_t na = int64_t(acc.size());
-        std::vector<int64_t> &at = j_at;
-        prefix_sum(na, at, [&](int64_t i) { return int64_t(jpos[acc[size_t(i)]] >= 0); });
-        const int64_t nj = at[size_t(na)];
+        const int64_t nj = int64_t(jq.size());
         if (nj == 0) return;
-        hvec<JudgeTask> &jt = j_task;
-        jt.resize(static_cast<size_t>(nj));
-        std::vector<int32_t> &who = j_who;
-        who.resize(static_cast<size_t>(nj));
-        parallel_for(na, [&](int64_t i) {
-            const int32_t q = acc[size_t(i)];
-            const int32_t a = jpos[q];
-            if (a < 0) return;
-            jt[size_t(at[size_t(i)])] = JudgeTask{jz[a], qt[q].zofs, int32_t(jr[a + 1] - jr[a]), qt[q].m, qt[q].n,
-                                                  cs[pend[q]].rank, qt[q].pofs, 0};
-            who[size_t(at[size_t(i)])] = q;
-        }, 4096);
-        upload(d_jtask, jt, ctx.stream);
+        upload(d_jq, jq, ctx.stream);
         need(d_jres, sizeof(double) * 2 * size_t(nj));
-        launch_judge(ctx.stream, prec, d_Z.p, d_jtask.as<JudgeTask>(), int32_t(nj), d_piv.as<int32_t>(),
-                     d_jres.as<double>());
+        launch_judge(ctx.stream, prec, d_Z.p, d_jq.as<int32_t>(), d_qr.as<QrTask>(), d_fill.as<FillTask>() + np,
+                     d_rank.as<int32_t>(), int32_t(nj), d_piv.as<int32_t>(), d_jres.as<double>());
         hvec<double> &res = j_res;
         res.resize(2 * static_cast<size_t>(nj));
         BFG_HIP(hipMemcpyAsync(res.data(), d_jres.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, ctx.stream));
         BFG_HIP(hipStreamSynchronize(ctx.stream));
+        auto t1 = clk::now();
         const double slack = o.judge_slack > 0.0 ? o.judge_slack : 4.0;
-        std::vector<uint8_t> &fail = j_fail;
-        fail.assign(static_cast<size_t>(nj), 0);
+        // per tile: 0 accepted, 1 rejected, 2 not judged (QR doubled the rows)
+        std::vector<uint8_t> &verdict = j_fail;
+        verdict.resize(static_cast<size_t>(nj));
         parallel_for(nj, [&](int64_t a) {
-            Comp &c = cs[pend[who[size_t(a)]]];
-            fail[size_t(a)] = !(res[2 * a] <= slack * c.eps * res[2 * a + 1]);
-            c.judge = fail[size_t(a)] ? 2 : 1;
+            const int32_t q = jq[size_t(a)];
+            if (dbl[size_t(q)]) {
+                verdict[size_t(a)] = 2;
+                return;
+            }
+            Comp &c = cs[pend[q]];
+            const bool fail = !(res[2 * a] <= slack * c.eps * res[2 * a + 1]);
+            verdict[size_t(a)] = fail ? 1 : 0;
+            c.judge = fail ? 2 : 1;
         }, 4096);
-        n_judged += nj;
         // rejected ones (about one in a hundred), in batch order
+        int64_t skipped = 0;
         for (int64_t a = 0; a < nj; ++a) {
-            if (!fail[size_t(a)]) continue;
-            Comp &c = cs[pend[who[size_t(a)]]];
+            if (verdict[size_t(a)] == 0) continue;
+            if (verdict[size_t(a)] == 2) {
+                ++skipped;
+                continue;
+            }
+            Comp &c = cs[pend[jq[size_t(a)]]];
             ++n_fallback;
             const int32_t plain = std::max(int32_t(std::ceil(o.c_tau * std::log(1.0 / c.eps))), 10);
             c.rows = std::min(std::max(plain, 2 * c.rows), c.n_obs);
-            retry.push_back(pend[who[size_t(a)]]);
+            retry.push_back(pend[jq[size_t(a)]]);
         }
+        n_judged += nj - skipped;
+        t_judge_gpu += secs(t0, t1);
         t_judge += secs(t0, clk::now());
     }
 
@@ -693,6 +687,8 @@ struct Engine {
                     BFG_HIP(hipStreamSynchronize(ctx.stream));
                     for (size_t q = 0; q < where.size(); ++q) hrank[where[q]] = r[q];
                 }
+                // the judge reads the ranks in batch order
+                BFG_HIP(hipMemcpyAsync(d_rank.p, hrank.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx.stream));
             }
             hpiv.resize(std::max(pofs, 1));
             BFG_HIP(hipMemcpyAsync(hpiv.data(), d_piv.p, sizeof(int32_t) * size_t(pofs), hipMemcpyDeviceToHost, ctx.stream));
@@ -729,7 +725,7 @@ struct Engine {
             // sized: rejected ones are redone by plain adaptive RRQR in a
             // later batch
             const double tj0 = t_judge;
-            if (!jq.empty()) judge(cs, pend, acc, qt, jq, jr, jz, jpos, retry);
+            if (!jq.empty()) judge(cs, pend, dbl, np, retry);
             t_qr += t_judge - tj0;  // reported with fill + QR
             // interpolation matrices of the accepted compressions, in batch
             // order, at running offsets of the level's factor buffer
@@ -1236,10 +1232,11 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
                      "[bfgpu] factorize %.3fs: setup %.3fs, plan %.3fs, guess %.3fs, sample-prep %.3fs, "
                      "fill+qr %.3fs, interp %.3fs, trim %.3fs, emit %.3fs, drop %.3fs, free %.3fs, pack %.3fs "
                      "(%lld compressions, %lld retries, "
-                     "%lld of rank 0, judge %.3fs: %lld judged, %lld fallbacks, %d threads)\n",
+                     "%lld of rank 0, judge %.3fs (kernel and copies %.3fs): %lld judged, %lld fallbacks, "
+                     "%d threads)\n",
                      bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_guess, E.t_host, E.t_qr, E.t_interp, E.t_trim,
                      t_emit, t_drop, t_free, secs(tpk, t1),
-                     (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, E.t_judge,
+                     (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, E.t_judge, E.t_judge_gpu,
                      (long long)E.n_judged, (long long)E.n_fallback, host_threads());
 }
 

@@ -468,22 +468,19 @@ constexpr int32_t INTERP_SMALL_RANK = 16;
 void launch_interp_small(hipStream_t s, int prec, const void *Z, const InterpTask *tasks, int32_t n_tasks,
                          const int32_t *piv, void *T);
 
-// Residual judge of one compression (kernels.hip): R of its pivoted QR in
-// the tile workspace at rofs (leading dimension ldr, columns in pivot
-// order, r steps taken), its mc held-out rows at cofs (mc x n, column-major,
-// columns in the original order), pivots at pofs.
-struct JudgeTask {
-    int64_t cofs, rofs;
-    int32_t mc, ldr, n, r, pofs, pad;
-};
+// Residual judge (kernels.hip) of held-out tiles t = 0 .. n_tasks-1 of one
+// batch: compression q = jq[t] has R of its pivoted QR at qr[q].zofs
+// (leading dimension qr[q].m, columns in pivot order, rank[q] steps taken,
+// pivots at qr[q].pofs) and its held[t].m held-out rows at held[t].zofs
+// (held[t].m x qr[q].n, column-major, columns in the original order).
 // Most columns of a judged compression (the judge kernel keeps four
 // 64-lane chunks of a held-out row in registers).
 constexpr int32_t JUDGE_MAX_COLS = 256;
 // out[2 t], out[2 t + 1] = (e_max, z_max): the largest column norm of the
 // interpolative decomposition's residual on the held-out rows, and of the
 // held-out tile itself.
-void launch_judge(hipStream_t s, int prec, const void *Z, const JudgeTask *tasks, int32_t n_tasks,
-                  const int32_t *piv, double *out);
+void launch_judge(hipStream_t s, int prec, const void *Z, const int32_t *jq, const QrTask *qr,
+                  const FillTask *held, const int32_t *rank, int32_t n_tasks, const int32_t *piv, double *out);
 
 void launch_stage(hipStream_t s, int prec, int op, const StageTable &t, const void *factors,
                   const void *x, int64_t ldx, void *y, int64_t ldy, void *slots,
@@ -506,6 +503,10 @@ void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *fact
 // table's other tasks are not launched here.
 void launch_stream(hipStream_t s, int prec, const StageTable &t, const void *factors, void *slots,
                    int64_t in_shift, int n_cu, bool conj = false);
+// Device tasks (MvTask and MvTaskC) of a table from its uploaded items and
+// contributions.
+void launch_build_tasks(hipStream_t s, const MvItem *items, int32_t n_items, const MvContrib *con, int32_t n_con,
+                        MvTask *tasks, MvTaskC *tc);
 // Block transposition: for block q, dst[dofs + i n + k] = src[sofs + k m + i].
 struct TBlock {
     int64_t sofs, dofs;

@@ -608,17 +608,20 @@ void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tas
 // four tasks per workgroup, whole waves leave together.
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Z, const JudgeTask *__restrict__ tasks,
-                                                    int32_t n_tasks, const int32_t *__restrict__ piv_all,
-                                                    double *__restrict__ out) {
+__global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Z, const int32_t *__restrict__ jq,
+                                                    const QrTask *__restrict__ qr, const FillTask *__restrict__ held,
+                                                    const int32_t *__restrict__ rank, int32_t n_tasks,
+                                                    const int32_t *__restrict__ piv_all, double *__restrict__ out) {
     const int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     if (t >= n_tasks) return;
     const int lane = threadIdx.x & 63;
-    const JudgeTask tk = tasks[t];
-    const C<R> *Rm = Z + tk.rofs;
-    const C<R> *Zc = Z + tk.cofs;
-    const int32_t *piv = piv_all + tk.pofs;
-    const int mc = tk.mc, ldr = tk.ldr, n = tk.n, r = tk.r;
+    const int32_t q = jq[t];
+    const QrTask qt = qr[q];
+    const FillTask ht = held[t];
+    const C<R> *Rm = Z + qt.zofs;
+    const C<R> *Zc = Z + ht.zofs;
+    const int32_t *piv = piv_all + qt.pofs;
+    const int mc = ht.m, ldr = qt.m, n = qt.n, r = rank[q];
     if (n <= 64) {
         // one column per lane; eight held-out rows at a time, so each
         // entry of R is loaded once per eight rows
@@ -726,17 +729,17 @@ __global__ __launch_bounds__(256) void judge_kernel(const C<R> *__restrict__ Z, 
     }
 }
 
-void launch_judge(hipStream_t s, int prec, const void *Z, const JudgeTask *tasks, int32_t n_tasks,
-                  const int32_t *piv, double *out) {
+void launch_judge(hipStream_t s, int prec, const void *Z, const int32_t *jq, const QrTask *qr,
+                  const FillTask *held, const int32_t *rank, int32_t n_tasks, const int32_t *piv, double *out) {
     if (n_tasks <= 0) return;
     const int64_t nb = (int64_t(n_tasks) + 3) / 4;
     BFG_REQUIRE(nb < (int64_t(1) << 31), "judge: too many tasks");
     if (prec == BFGPU_C128)
-        hipLaunchKernelGGL(judge_kernel<double>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<double> *)Z, tasks,
-                           n_tasks, piv, out);
+        hipLaunchKernelGGL(judge_kernel<double>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<double> *)Z, jq, qr,
+                           held, rank, n_tasks, piv, out);
     else
-        hipLaunchKernelGGL(judge_kernel<float>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<float> *)Z, tasks,
-                           n_tasks, piv, out);
+        hipLaunchKernelGGL(judge_kernel<float>, dim3(unsigned(nb)), dim3(256), 0, s, (const C<float> *)Z, jq, qr,
+                           held, rank, n_tasks, piv, out);
     BFG_HIP(hipGetLastError());
 }
 
@@ -2847,6 +2850,31 @@ void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *fact
     else
         launch_dense_nr<float>(s, t, factors, slots, slot_len, in_shift, y, ldy, perm_out, nrhs, alpha, beta,
                                fuse_beta, n_cu, tab, conj);
+}
+
+// ---------------------------------------------------------------------------
+// Launch-table tasks from items and contributions: task i is item i with its
+// first contribution inline (the host keeps the same table for its own
+// passes; only items and contributions cross the bus).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void build_tasks_kernel(const MvItem *__restrict__ items, int32_t n_items,
+                                                          const MvContrib *__restrict__ con, int32_t n_con,
+                                                          MvTask *__restrict__ tasks, MvTaskC *__restrict__ tc) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
+        const MvItem it = items[i];
+        MvContrib c0{-1, 0, 0, 0, 0};
+        if (it.c_begin >= 0 && it.c_begin < n_con) c0 = con[it.c_begin];
+        tasks[i] = MvTask{c0.fofs, c0.m, c0.n, c0.in_ofs, it.out_ofs, it.out_len, it.c_begin + 1, it.c_end, -1, 0, -1, 0};
+        tc[i] = MvTaskC{c0.fofs, c0.in_ofs, it.out_ofs, c0.m, c0.n, it.c_begin + 1, it.c_end};
+    }
+}
+
+void launch_build_tasks(hipStream_t s, const MvItem *items, int32_t n_items, const MvContrib *con, int32_t n_con,
+                        MvTask *tasks, MvTaskC *tc) {
+    if (n_items <= 0) return;
+    const unsigned grid = unsigned(std::min<int64_t>((int64_t(n_items) + 255) / 256, 4096));
+    hipLaunchKernelGGL(build_tasks_kernel, dim3(grid), dim3(256), 0, s, items, n_items, con, n_con, tasks, tc);
+    BFG_HIP(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------

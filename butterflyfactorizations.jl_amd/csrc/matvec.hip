@@ -205,8 +205,9 @@ void pack_tables(BF &bf) {
     const int64_t nb = int64_t(bf.bfs.size());
     using clk = std::chrono::steady_clock;
     const bool timing = std::getenv("BFGPU_TIMING") != nullptr;
-    double t_group = 0, t_split = 0, t_finish = 0;
+    double t_group = 0, t_split = 0, t_finish = 0, t_up = 0, t_g_items = 0, t_g_fe = 0, t_g_merge = 0;
     auto tp0 = clk::now();
+    const hipStream_t ps = bf.ctx ? bf.ctx->stream : hipStream_t(nullptr);
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     const BlockIndex ix = index_blocks(bf);
     double t_keys = 0, t_slots = 0, t_pf = 0;
@@ -313,6 +314,7 @@ void pack_tables(BF &bf) {
     // contributions read the input; transposed: the other way round.
     auto group = [&](int32_t L, bool fwd, StageTable &S) {
         const bool node_keyed = fwd ? (L == Lmax) : (L == 0);
+        auto tg0 = clk::now();
         hvec<MvItem> items;
         hvec<MvContrib> con;
         auto describe = [&](int64_t b, int32_t s, const Block &k, int64_t g, int64_t &out_ofs,
@@ -443,6 +445,8 @@ void pack_tables(BF &bf) {
                 }
             });
         }
+        t_g_items += since(tg0);
+        tg0 = clk::now();
         {  // factor entries read, summed in parallel over chunks
             constexpr int64_t CH = int64_t(1) << 16;
             const int64_t nk = int64_t(con.size()), nch = (nk + CH - 1) / CH;
@@ -457,6 +461,8 @@ void pack_tables(BF &bf) {
             for (int64_t f : part) fe += f;
             S.factor_elems = fe;
         }
+        t_g_fe += since(tg0);
+        tg0 = clk::now();
         if (fwd) {
             // merge within each item, then compact
             std::vector<int32_t> kept(items.size() + 1, 0);
@@ -477,6 +483,7 @@ void pack_tables(BF &bf) {
         }
         S.h_items.swap(items);
         S.h_contribs.swap(con);
+        t_g_merge += since(tg0);
     };
     // Split items whose contribution lists are long (observer-leaf sums over
     // many factorizations) into chunks of about CHUNK factor entries: a
@@ -670,17 +677,15 @@ void pack_tables(BF &bf) {
         // default forward variant until the autotuner measures: a
         // wavefront per task for small blocks, a workgroup otherwise
         S.variant = fwd ? ((S.n_items > 0 && S.factor_elems / S.n_items < 4096) ? 1 : 0) : 0;
+        auto tu = clk::now();
         upload_tab(S.items, S.h_items);
         upload_tab(S.contribs, S.h_contribs);
-        upload_tab(S.tasks, S.h_tasks);
-        {
-            hvec<MvTaskC> tc(S.h_tasks.size());
-            parallel_for(int64_t(tc.size()), [&](int64_t i) {
-                const MvTask &t = S.h_tasks[i];
-                tc[i] = MvTaskC{t.fofs, t.in_ofs, t.out_ofs, t.m, t.n, t.c_begin, t.c_end};
-            }, 4096);
-            upload_tab(S.tasks_c, tc);
-        }
+        // the tasks (as the host's h_tasks) are derived on the device
+        S.tasks.alloc(sizeof(MvTask) * std::max<size_t>(S.h_items.size(), 1));
+        S.tasks_c.alloc(sizeof(MvTaskC) * std::max<size_t>(S.h_items.size(), 1));
+        launch_build_tasks(ps, S.items.as<MvItem>(), S.n_items, S.contribs.as<MvContrib>(), S.n_contrib,
+                           S.tasks.as<MvTask>(), S.tasks_c.as<MvTaskC>());
+        t_up += since(tu);
         if (S.pre) finish(*S.pre, fwd);
     };
 
@@ -838,14 +843,16 @@ void pack_tables(BF &bf) {
     const double t_valid = since(tv);
     upload_tab(bf.perm_test, test.perm);
     upload_tab(bf.perm_trial, trial.perm);
+    BFG_HIP(hipStreamSynchronize(ps));  // device-built tasks complete
     bf.sweep = SweepTables{};
     if (bf.use_sweep) build_sweep(bf);
     if (timing)
-        std::fprintf(stderr, "[bfgpu] pack %.3fs: index+keys %.3fs, slots %.3fs, group %.3fs, split %.3fs, "
-                             "finish+upload %.3fs, matrix-free tables %.3fs (check %.3fs, columns %.3fs, upload %.3fs), "
-                             "validate %.3fs\n",
-                     since(tp0), t_keys, t_slots, t_group, t_split, t_finish, t_pf, t_pf_check, t_pf_cols, t_pf_up,
-                     t_valid);
+        std::fprintf(stderr, "[bfgpu] pack %.3fs: index+keys %.3fs, slots %.3fs, group %.3fs (items %.3fs, entries %.3fs, "
+                             "merge %.3fs), split %.3fs, "
+                             "finish+upload %.3fs (upload %.3fs), matrix-free tables %.3fs (check %.3fs, columns %.3fs, "
+                             "upload %.3fs), validate %.3fs\n",
+                     since(tp0), t_keys, t_slots, t_group, t_g_items, t_g_fe, t_g_merge, t_split, t_finish, t_up, t_pf, t_pf_check, t_pf_cols,
+                     t_pf_up, t_valid);
 }
 
 // Streaming-kernel tables of a launch table (variant 9): the forward tables

@@ -214,12 +214,159 @@ void mlp_upload_norm(bfgpu_mlp &m) {
     BFG_HIP(hipMemcpy(m.norm.p, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
 }
 
+// Fused inference for the shape the rank predictor ships in: n_in <= 8
+// inputs, hidden width 128, four layers (two hidden-to-hidden).  FP32 on
+// the matrix cores (v_mfma_f32_16x16x4f32) with the transposed product
+//     a_{l+1}^T (128 x 16 samples) = relu(W_l a_l^T + b_l),
+// 16 samples per operand, two sample groups per wavefront.  The 16 x 16
+// output block of a layer leaves lane (sample s = lane % 16, g = lane / 16)
+// holding outputs 16 ob + 4 g + r (register r), which is the B operand of
+// the next layer's step (ib = ob, r) for the input order
+// k = 16 ib + 4 g + r -- activations never leave registers; the weights of
+// the two hidden-to-hidden layers sit in LDS (128 KiB) in that order, one
+// 16-byte read per lane feeding four steps.  Only the summation order
+// differs from the layer-by-layer GEMMs.
+constexpr int MI_H = 128, MI_NB = MI_H / 16, MI_S = 2, MI_THREADS = 512;
+typedef float mi_f4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(MI_THREADS) void mlp_infer_kernel(const float *__restrict__ x, int64_t n, int n_in,
+                                                               const float *__restrict__ norm,
+                                                               const float *__restrict__ P, int64_t w0,
+                                                               int64_t b0, int64_t w1, int64_t b1, int64_t w2,
+                                                               int64_t b2, int64_t w3, int64_t b3,
+                                                               float *__restrict__ y) {
+    __shared__ mi_f4 Wm[2][MI_NB * MI_NB * 64];
+    __shared__ float W0s[MI_NB * 2 * 64];
+    __shared__ float bs[3][MI_H];
+    __shared__ float w3s[MI_H];
+    for (int e = threadIdx.x; e < MI_NB * MI_NB * 64; e += MI_THREADS) {
+        const int ln = e & 63, ib = (e >> 6) % MI_NB, ob = (e >> 6) / MI_NB;
+        const int64_t row = int64_t(ob * 16 + (ln & 15)) * MI_H + ib * 16 + 4 * (ln >> 4);
+        Wm[0][e] = mi_f4{P[w1 + row], P[w1 + row + 1], P[w1 + row + 2], P[w1 + row + 3]};
+        Wm[1][e] = mi_f4{P[w2 + row], P[w2 + row + 1], P[w2 + row + 2], P[w2 + row + 3]};
+    }
+    for (int e = threadIdx.x; e < MI_NB * 2 * 64; e += MI_THREADS) {
+        const int ln = e & 63, t = (e >> 6) & 1, ob = e >> 7;
+        const int k = 4 * t + (ln >> 4);
+        W0s[e] = k < n_in ? P[w0 + int64_t(ob * 16 + (ln & 15)) * n_in + k] : 0.f;
+    }
+    for (int e = threadIdx.x; e < MI_H; e += MI_THREADS) {
+        bs[0][e] = P[b0 + e];
+        bs[1][e] = P[b1 + e];
+        bs[2][e] = P[b2 + e];
+        w3s[e] = P[w3 + e];
+    }
+    const float bias3 = P[b3];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, s = lane & 15, g = lane >> 4;
+    const int64_t nw = int64_t(gridDim.x) * (MI_THREADS / 64);
+    const int64_t ntile = (n + 16 * MI_S - 1) / (16 * MI_S);
+    for (int64_t tile = int64_t(blockIdx.x) * (MI_THREADS / 64) + (threadIdx.x >> 6); tile < ntile; tile += nw) {
+        mi_f4 act[MI_S][MI_NB];
+        float xin[MI_S][2];
+#pragma unroll
+        for (int q = 0; q < MI_S; ++q) {
+            const int64_t smp = tile * (16 * MI_S) + q * 16 + s;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int k = 4 * t + g;
+                xin[q][t] = (smp < n && k < n_in) ? (x[smp * n_in + k] - norm[k]) * norm[n_in + k] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int ob = 0; ob < MI_NB; ++ob) {
+            const float wa = W0s[(ob * 2 + 0) * 64 + lane], wb = W0s[(ob * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int q = 0; q < MI_S; ++q) {
+                mi_f4 c = {0.f, 0.f, 0.f, 0.f};
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(wa, xin[q][0], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f32_16x16x4f32(wb, xin[q][1], c, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) act[q][ob][r] = fmaxf(c[r] + bs[0][ob * 16 + 4 * g + r], 0.f);
+            }
+        }
+#pragma unroll
+        for (int L = 0; L < 2; ++L) {
+            mi_f4 nxt[MI_S][MI_NB];
+#pragma unroll
+            for (int q = 0; q < MI_S; ++q)
+#pragma unroll
+                for (int ob = 0; ob < MI_NB; ++ob) nxt[q][ob] = mi_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ib = 0; ib < MI_NB; ++ib) {
+                mi_f4 w[MI_NB];
+#pragma unroll
+                for (int ob = 0; ob < MI_NB; ++ob) w[ob] = Wm[L][(ob * MI_NB + ib) * 64 + lane];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int ob = 0; ob < MI_NB; ++ob)
+#pragma unroll
+                        for (int q = 0; q < MI_S; ++q)
+                            nxt[q][ob] = __builtin_amdgcn_mfma_f32_16x16x4f32(w[ob][r], act[q][ib][r], nxt[q][ob],
+                                                                              0, 0, 0);
+                // keep the next block's weight reads from being hoisted
+                // (registers: activations, accumulators, one block of weights)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int q = 0; q < MI_S; ++q)
+#pragma unroll
+                for (int ob = 0; ob < MI_NB; ++ob)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        act[q][ob][r] = fmaxf(nxt[q][ob][r] + bs[L + 1][ob * 16 + 4 * g + r], 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < MI_S; ++q) {
+            float p = 0.f;
+#pragma unroll
+            for (int ob = 0; ob < MI_NB; ++ob)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) p = fmaf(w3s[ob * 16 + 4 * g + r], act[q][ob][r], p);
+            p += __shfl_xor(p, 16, 64);
+            p += __shfl_xor(p, 32, 64);
+            const int64_t smp = tile * (16 * MI_S) + q * 16 + s;
+            if (g == 0 && smp < n) y[smp] = p + bias3;
+        }
+    }
+}
+
+static bool fused_infer_ok(const bfgpu_mlp &m) {
+    return m.n_layers == 4 && m.hidden == MI_H && m.n_in >= 1 && m.n_in <= 8;
+}
+
 // Forward pass over n samples in chunks of at most INFER_CHUNK, so that the
 // activation workspace stays bounded (n can be every compression of a
 // level); raw network outputs into y (host).
 constexpr int64_t INFER_CHUNK = int64_t(1) << 20;
 static void forward_chunked(bfgpu_mlp &m, int64_t n, const float *features, float *y) {
     hipStream_t s = m.ctx->stream;
+    if (fused_infer_ok(m) && !std::getenv("BFGPU_MLP_GEMM")) {
+        // fused kernel: no activation workspace, chunks of 4M samples
+        const int64_t ch = std::min(n, int64_t(4) * INFER_CHUNK);
+        DevBuf dx, dy;
+        dx.alloc(sizeof(float) * size_t(ch) * m.n_in);
+        dy.alloc(sizeof(float) * size_t(ch));
+        int dev = 0, n_cu = 0;
+        BFG_HIP(hipGetDevice(&dev));
+        BFG_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        for (int64_t i0 = 0; i0 < n; i0 += ch) {
+            const int64_t nc = std::min(ch, n - i0);
+            BFG_HIP(hipMemcpyAsync(dx.p, features + i0 * m.n_in, sizeof(float) * size_t(nc) * m.n_in,
+                                   hipMemcpyHostToDevice, s));
+            const int64_t tiles = (nc + 16 * MI_S - 1) / (16 * MI_S);
+            const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_cu, (tiles + 7) / 8));
+            hipLaunchKernelGGL(mlp_infer_kernel, dim3(unsigned(grid)), dim3(MI_THREADS), 0, s, dx.as<float>(), nc,
+                               int(m.n_in), m.norm.as<float>(), m.params.as<float>(), m.w_ofs[0], m.b_ofs[0],
+                               m.w_ofs[1], m.b_ofs[1], m.w_ofs[2], m.b_ofs[2], m.w_ofs[3], m.b_ofs[3],
+                               dy.as<float>());
+            BFG_HIP(hipGetLastError());
+            BFG_HIP(hipMemcpyAsync(y + i0, dy.p, sizeof(float) * size_t(nc), hipMemcpyDeviceToHost, s));
+            BFG_HIP(hipStreamSynchronize(s));
+        }
+        return;
+    }
     DevBuf dx;
     const int64_t ch = std::min(n, INFER_CHUNK);
     dx.alloc(sizeof(float) * size_t(ch) * m.n_in);
