@@ -270,6 +270,7 @@ struct Engine {
     size_t es;
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
+    DevBuf d_test_perm, d_stask, d_big;  // row samples drawn on the device
     // residual judge: tasks and results
     DevBuf d_jq, d_jres;
     int64_t n_comp = 0, n_retry = 0, n_rank0 = 0, n_judged = 0, n_fallback = 0;
@@ -281,7 +282,9 @@ struct Engine {
     // reused instead of hundreds of MB being unmapped and faulted in again
     // per level (unmapping stalls every worker thread)
     std::vector<int32_t> pend, later, retry, acc;
-    hvec<int32_t> hrows, hcols;
+    hvec<int32_t> hrows;  // host-drawn samples (prefixes too long for the device)
+    hvec<SampleTask> stask;
+    std::vector<int64_t> bigofs;
     std::vector<int32_t> hrank, hpiv, jq, jpos;
     hvec<FillTask> ft;
     hvec<int64_t> fe;
@@ -300,6 +303,7 @@ struct Engine {
           es(op.precision == BFGPU_C128 ? 16 : 8) {
         upload(d_test_pts, test.points, ctx.stream);
         upload(d_trial_pts, trial.points, ctx.stream);
+        upload(d_test_perm, test.perm, ctx.stream);
     }
 
     // Kernel-matrix entries of the listed blocks: the plug-in's fill when
@@ -343,6 +347,7 @@ struct Engine {
 
     void release() {
         for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z,
+                          &d_test_perm, &d_stask, &d_big,
                           &d_jq, &d_jres})
             *d = DevBuf();
         Engine *e = this;
@@ -354,7 +359,8 @@ struct Engine {
                         &e->it_pos, &e->it_big, &e->sk_run})
             drop(*v);
         drop(hrows);
-        drop(hcols);
+        drop(stask);
+        drop(bigofs);
         drop(ft);
         drop(fe);
         drop(qt);
@@ -398,8 +404,11 @@ struct Engine {
         d.reserve(n);
     }
     template <class T, class A> void upload(DevBuf &d, const std::vector<T, A> &h, hipStream_t s) {
-        need(d, sizeof(T) * std::max<size_t>(h.size(), 1));
-        if (!h.empty()) BFG_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+        upload_n(d, h.data(), h.size(), s);
+    }
+    template <class T> void upload_n(DevBuf &d, const T *h, size_t n, hipStream_t s) {
+        need(d, sizeof(T) * std::max<size_t>(n, 1));
+        if (n) BFG_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, s));
     }
 
     void ensure_pool(DevBuf &pool, int64_t &cap_elems, int64_t need) {
@@ -520,8 +529,19 @@ struct Engine {
     // (adaptive doubling of the rows that saturated) -> interpolation
     // matrices written into `pool` (the level's factor buffer), skeletons
     // appended to `skp` (the level's skeleton pool, Comp::skel_ofs).
-    void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used, std::vector<int32_t> &skp) {
+    // Every compression's columns lie in colpool[0, colpool_len) (the
+    // trial tree's values at Q, the level's column lists at R): uploaded once
+    // per level, the fill tasks index it.
+    void run(std::vector<Comp> &cs, DevBuf &pool, int64_t &pool_cap, int64_t &pool_used, std::vector<int32_t> &skp,
+             const int32_t *colpool, int64_t colpool_len) {
         const auto tg = clk::now();
+        BFG_REQUIRE(colpool_len >= 0 && colpool_len < (int64_t(1) << 31), "factorize: column lists exceed 2^31 entries");
+        parallel_for(int64_t(cs.size()), [&](int64_t i) {
+            const Comp &c = cs[size_t(i)];
+            BFG_REQUIRE(c.n_cols == 0 || (c.cols >= colpool && c.cols + c.n_cols <= colpool + colpool_len),
+                        "factorize: compression columns outside the level's column lists");
+        }, 1 << 14);
+        upload_n(d_cols, colpool, size_t(colpool_len), ctx.stream);
         guesses(cs);
         pend.clear();
         for (size_t i = 0; i < cs.size(); ++i) {
@@ -588,7 +608,8 @@ struct Engine {
             parallel_for(int64_t(np), [&](int64_t q) {
                 const Comp &c = cs[pend[q]];
                 const int32_t n = c.n_cols;
-                ft[q] = FillTask{zo[q], c.rows, n, int32_t(rofs[q]), int32_t(cofs[q])};
+                const int32_t cp = c.n_cols ? int32_t(c.cols - colpool) : 0;  // columns in the level's lists
+                ft[q] = FillTask{zo[q], c.rows, n, int32_t(rofs[q]), cp};
                 fe[q] = zo[q + 1];
                 qt[q] = QrTask{zo[q], c.rows, n, int32_t(cofs[q]), 0};
                 int32_t v = amax_n.load(std::memory_order_relaxed);
@@ -606,7 +627,7 @@ struct Engine {
                 jq[a] = int32_t(q);
                 jr[a] = jrf[q];
                 jz[a] = zmain + jzf[q];
-                ft[np + a] = FillTask{jz[a], jm[q], n, int32_t(rofs[np] + jrf[q]), int32_t(cofs[q])};
+                ft[np + a] = FillTask{jz[a], jm[q], n, int32_t(rofs[np] + jrf[q]), cp};
                 fe[np + a] = jz[a] + int64_t(jm[q]) * n;
             }, 4096);
             jr[nj] = jrf[np];
@@ -614,32 +635,36 @@ struct Engine {
             BFG_REQUIRE(rofs[np] + jr[nj] < (int64_t(1) << 31) && cofs[np] < (int64_t(1) << 31),
                         "factorize: level too large for 32-bit index lists");
             const int32_t pofs = int32_t(cofs[np]);
-            hrows.resize(size_t(rofs[np] + jr[nj]));
-            hcols.resize(size_t(cofs[np]));
+            // row samples: drawn on the device, except prefixes longer than
+            // the device's swap map holds (drawn here, copied there)
+            stask.resize(np);
+            std::vector<int64_t> &bo = bigofs;
+            prefix_sum(int64_t(np), bo, [&](int64_t q) {
+                const int64_t t = int64_t(cs[pend[q]].rows) + jm[q];
+                return t > SAMPLE_DEVICE_MAX ? t : int64_t(0);
+            });
+            BFG_REQUIRE(bo[np] < (int64_t(1) << 31), "factorize: host-drawn samples exceed 2^31 entries");
+            hrows.resize(size_t(std::max<int64_t>(bo[np], 1)));
             parallel_for(int64_t(np), [&](int64_t q) {
                 const Comp &c = cs[pend[q]];
-                int32_t *r = hrows.data() + rofs[q];
-                if (jpos[q] < 0) {
-                    sample_prefix(c.key, c.n_obs, c.rows, r);
-                    for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[r[i]];
-                } else {
-                    const int32_t mc = jm[q];
-                    thread_local std::vector<int32_t> pre;
-                    pre.resize(size_t(c.rows) + size_t(mc));
-                    sample_prefix(c.key, c.n_obs, c.rows + mc, pre.data());
-                    for (int32_t i = 0; i < c.rows; ++i) r[i] = c.obs[pre[size_t(i)]];
-                    int32_t *h = hrows.data() + rofs[np] + jrf[q];
-                    for (int32_t i = 0; i < mc; ++i) h[i] = c.obs[pre[size_t(c.rows + i)]];
-                }
-                std::copy(c.cols, c.cols + c.n_cols, hcols.begin() + cofs[q]);
-            });
+                const bool host = bo[q + 1] > bo[q];
+                stask[q] = SampleTask{c.key, test.val_begin[c.O], c.n_obs, jpos[q], host ? int32_t(bo[q]) : -1};
+                if (!host) return;
+                int32_t *r = hrows.data() + bo[q];
+                const int32_t t = int32_t(bo[q + 1] - bo[q]);
+                sample_prefix(c.key, c.n_obs, t, r);
+                for (int32_t i = 0; i < t; ++i) r[i] = c.obs[r[i]];
+            }, 1024);
             auto tq = clk::now();
             t_host += secs(th, tq);
-            upload(d_rows, hrows, ctx.stream);
-            upload(d_cols, hcols, ctx.stream);
             upload(d_fill, ft, ctx.stream);
             upload(d_fill_end, fe, ctx.stream);
             upload(d_qr, qt, ctx.stream);
+            upload(d_stask, stask, ctx.stream);
+            if (bo[np] > 0) upload_n(d_big, hrows.data(), size_t(bo[np]), ctx.stream);
+            need(d_rows, sizeof(int32_t) * size_t(std::max<int64_t>(rofs[np] + jr[nj], 1)));
+            launch_sample_rows(ctx.stream, d_stask.as<SampleTask>(), d_fill.as<FillTask>(), int32_t(np),
+                               d_test_perm.as<int32_t>(), d_big.as<int32_t>(), d_rows.as<int32_t>());
             need(d_Z, size_t(zofs) * es);
             need(d_piv, sizeof(int32_t) * size_t(std::max(pofs, 1)));
             need(d_rank, sizeof(int32_t) * np);
@@ -946,7 +971,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
         skp.clear();
-        E.run(cs, bf.factors[0], cap, used, skp);
+        E.run(cs, bf.factors[0], cap, used, skp, trial.perm.data(), int64_t(trial.perm.size()));
         E.trim_pool(bf.factors[0], cap, used);
         bf.factor_elems[0] = used;
         auto te = clk::now();
@@ -1079,7 +1104,7 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
         t_plan += secs(tp, clk::now());
         int64_t cap = 0, used = 0;
         skp.clear();
-        E.run(cs, bf.factors[l], cap, used, skp);
+        E.run(cs, bf.factors[l], cap, used, skp, up.data(), int64_t(up.size()));
         E.trim_pool(bf.factors[l], cap, used);
         bf.factor_elems[l] = used;
         // emit blocks in action order; K updates in the same order, applied

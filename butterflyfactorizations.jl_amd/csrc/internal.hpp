@@ -443,6 +443,21 @@ void launch_fill(hipStream_t s, int prec, const double *obs_pts, const double *s
                  const int64_t *task_end, int32_t n_tasks, int64_t total, void *Z, double k,
                  double self_re = 0.0, double self_im = 0.0);
 
+// Row sample of one compression, drawn on the device (kernels.hip): the
+// same sparse Fisher-Yates prefix over the SplitMix64 counter stream as the
+// host's sample_prefix, of length fill[q].m (+ the held-out tile's rows),
+// mapped through perm[obs0 ..] and written at the fill tasks' row offsets.
+struct SampleTask {
+    uint64_t key;           // the compression's sample stream
+    int32_t obs0, n_obs;    // its observers: perm[obs0 .. obs0 + n_obs)
+    int32_t held;           // its held-out tile among the batch's (-1: none)
+    int32_t big;            // offset of host-drawn rows in `big` (-1: drawn on the device)
+};
+// Longest prefix drawn on the device (swap-map entries: four per lane).
+constexpr int32_t SAMPLE_DEVICE_MAX = 256;
+void launch_sample_rows(hipStream_t s, const SampleTask *tasks, const FillTask *fill, int32_t n_tasks,
+                        const int32_t *perm, const int32_t *big, int32_t *rows);
+
 struct QrTask {
     int64_t zofs;       // Z block (m x n, ld = m)
     int32_t m, n;

@@ -595,6 +595,99 @@ void launch_interp(hipStream_t s, int prec, const void *Z, const InterpTask *tas
 }
 
 // ---------------------------------------------------------------------------
+// Row samples on the device: one wavefront per compression.  Step i of the
+// sparse Fisher-Yates prefix draws j = i + floor(hi32(u) (n - i) / 2^32),
+// u = SplitMix64(key + (i + 1) 0x9E3779B97F4A7C15), takes a_j (the value
+// at position j, j if untouched) and moves a_i to position j; position i is
+// never read again (later steps read positions > i), so only moved values
+// are kept: a swap map of at most one entry per step, entry e in lane
+// e % 64, register e / 64, found by ballot.  Lane i % 64 keeps a_j of step
+// i, and the sample is written at the end in one pass.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t dmix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void sample_rows_kernel(const SampleTask *__restrict__ tasks,
+                                                          const FillTask *__restrict__ fill, int32_t n_tasks,
+                                                          const int32_t *__restrict__ perm,
+                                                          const int32_t *__restrict__ big, int32_t *__restrict__ rows) {
+    const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (q >= n_tasks) return;
+    const int lane = threadIdx.x & 63;
+    const SampleTask t = tasks[q];
+    const FillTask f = fill[q];
+    const int32_t m = f.m;
+    int32_t mc = 0, hofs = 0;
+    if (t.held >= 0) {
+        const FillTask h = fill[n_tasks + t.held];
+        mc = h.m;
+        hofs = h.rofs;
+    }
+    const int32_t total = m + mc;
+    auto dst = [&](int32_t i) { return i < m ? int64_t(f.rofs) + i : int64_t(hofs) + (i - m); };
+    if (t.big >= 0) {
+        for (int32_t i = lane; i < total; i += 64) rows[dst(i)] = big[int64_t(t.big) + i];
+        return;
+    }
+    constexpr int NS = SAMPLE_DEVICE_MAX / 64;
+    int32_t mk[NS], mv[NS], res[NS];
+#pragma unroll
+    for (int c = 0; c < NS; ++c) {
+        mk[c] = -1;
+        mv[c] = 0;
+        res[c] = 0;
+    }
+    int32_t cnt = 0;
+    for (int32_t i = 0; i < total; ++i) {
+        const uint64_t u = dmix64(t.key + uint64_t(i + 1) * 0x9E3779B97F4A7C15ULL);
+        const uint32_t span = uint32_t(t.n_obs - i);
+        const int32_t j = i + int32_t(((u >> 32) * uint64_t(span)) >> 32);
+        int32_t ai = i, aj = j, js = -1, jl = 0;
+#pragma unroll
+        for (int c = 0; c < NS; ++c) {
+            if (64 * c >= cnt) break;
+            const uint64_t bi = __ballot(mk[c] == i);
+            if (bi) ai = __builtin_amdgcn_readlane(mv[c], int(__builtin_ctzll(bi)));
+            const uint64_t bj = __ballot(mk[c] == j);
+            if (bj) {
+                jl = int(__builtin_ctzll(bj));
+                aj = __builtin_amdgcn_readlane(mv[c], jl);
+                js = c;
+            }
+        }
+        if (j != i) {
+            const int e = js >= 0 ? 64 * js + jl : cnt++;
+#pragma unroll
+            for (int c = 0; c < NS; ++c)
+                if (c == (e >> 6) && lane == (e & 63)) {
+                    mk[c] = j;
+                    mv[c] = ai;
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < NS; ++c)
+            if (c == (i >> 6) && lane == (i & 63)) res[c] = aj;
+    }
+#pragma unroll
+    for (int c = 0; c < NS; ++c) {
+        const int32_t i = lane + 64 * c;
+        if (i < total) rows[dst(i)] = perm[int64_t(t.obs0) + res[c]];
+    }
+}
+
+void launch_sample_rows(hipStream_t s, const SampleTask *tasks, const FillTask *fill, int32_t n_tasks,
+                        const int32_t *perm, const int32_t *big, int32_t *rows) {
+    if (n_tasks <= 0) return;
+    const int64_t nb = (int64_t(n_tasks) + 3) / 4;
+    BFG_REQUIRE(nb < (int64_t(1) << 31), "sample: too many tasks");
+    hipLaunchKernelGGL(sample_rows_kernel, dim3(unsigned(nb)), dim3(256), 0, s, tasks, fill, n_tasks, perm, big, rows);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Residual judge of compressions sized by a rank prediction.  For each
 // held-out row z (1 x n) the row elimination
 //     s_k = z[piv[k]],  then for q < r:  w_q = s_q / R[q, q],

@@ -72,6 +72,32 @@ struct KeyMap {
     }
 };
 
+// Map from a factorization's local slot offsets [0, n) to ordinals: a
+// directly indexed array with generation stamps (no hashing, no clearing).
+struct SlotMap {
+    std::vector<uint32_t> g;
+    std::vector<int32_t> v;
+    uint32_t gen = 0;
+    void reset(size_t n) {
+        if (g.size() < n) {
+            g.assign(n, 0u);
+            v.assign(n, 0);
+            gen = 0;
+        }
+        if (++gen == 0) {
+            std::fill(g.begin(), g.end(), 0u);
+            gen = 1;
+        }
+    }
+    // the ordinal stored for `key`, inserting `val` when absent
+    std::pair<int32_t, bool> emplace(uint32_t key, int32_t val) {
+        if (g[key] == gen) return {v[key], false};
+        g[key] = gen;
+        v[key] = val;
+        return {val, true};
+    }
+};
+
 template <class T, class A> void upload_tab(DevBuf &d, const std::vector<T, A> &h) {
     d.alloc(sizeof(T) * std::max<size_t>(h.size(), 1));
     if (!h.empty()) BFG_HIP(hipMemcpy(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
@@ -205,6 +231,7 @@ void pack_tables(BF &bf) {
     const int64_t nb = int64_t(bf.bfs.size());
     using clk = std::chrono::steady_clock;
     const bool timing = std::getenv("BFGPU_TIMING") != nullptr;
+    const bool timing_detail = timing && std::atoi(std::getenv("BFGPU_TIMING")) >= 2;
     double t_group = 0, t_split = 0, t_finish = 0, t_up = 0, t_g_items = 0, t_g_fe = 0, t_g_merge = 0;
     auto tp0 = clk::now();
     const hipStream_t ps = bf.ctx ? bf.ctx->stream : hipStream_t(nullptr);
@@ -212,7 +239,16 @@ void pack_tables(BF &bf) {
     const BlockIndex ix = index_blocks(bf);
     double t_keys = 0, t_slots = 0, t_pf = 0;
     // block keys and roots index the trees' node arrays below
+    // The blocks of one factorization and stage sit in their own heap array
+    // behind two pointers: loops over factorizations request every stage's
+    // blocks of factorization b + 4 (its stage array of b + 8) ahead of use.
+    auto prefetch_all = [&](int64_t b) {
+        if (b + 8 < nb) __builtin_prefetch(bf.bfs[size_t(b + 8)].st.data());
+        if (b + 4 < nb)
+            for (const StageBlocks &st : bf.bfs[size_t(b + 4)].st) __builtin_prefetch(st.b.data());
+    };
     parallel_for(nb, [&](int64_t b) {
+        prefetch_all(b);
         const OneBF &ob = bf.bfs[b];
         BFG_REQUIRE(ob.no >= 0 && ob.no < test.n_nodes && ob.ns >= 0 && ob.ns < trial.n_nodes,
                     "pack: root node outside the trees");
@@ -229,6 +265,7 @@ void pack_tables(BF &bf) {
     std::vector<int32_t> sl_out(size_t(ix.total), -1), sl_in(size_t(ix.total), -1);
     std::vector<int64_t> slot_base(nb + 1, 0);
     parallel_for(nb, [&](int64_t b) {
+        prefetch_all(b);
         thread_local KeyMap len, sm, ids;
         OneBF &ob = bf.bfs[b];
         const int32_t ns = int32_t(ob.st.size());
@@ -309,12 +346,21 @@ void pack_tables(BF &bf) {
         if (L == Lmax) return ns - 1;
         return L < ns - 1 ? L : -1;
     };
+    // the blocks of one stage only
+    auto prefetch_blocks = [&](int64_t b, int32_t L) {
+        if (b + 8 < nb) __builtin_prefetch(bf.bfs[size_t(b + 8)].st.data());
+        if (b + 4 < nb) {
+            const int32_t s = stage_at(b + 4, L);
+            if (s >= 0) __builtin_prefetch(bf.bfs[size_t(b + 4)].st[size_t(s)].b.data());
+        }
+    };
     hvec<int32_t> litem(size_t(std::max<int64_t>(ix.total, 1)));  // written before it is read
     // Group the blocks of launch L into items.  fwd: keyed by output,
     // contributions read the input; transposed: the other way round.
     auto group = [&](int32_t L, bool fwd, StageTable &S) {
         const bool node_keyed = fwd ? (L == Lmax) : (L == 0);
-        auto tg0 = clk::now();
+        const auto tg_start = clk::now();
+        auto tg0 = tg_start;
         hvec<MvItem> items;
         hvec<MvContrib> con;
         auto describe = [&](int64_t b, int32_t s, const Block &k, int64_t g, int64_t &out_ofs,
@@ -338,6 +384,7 @@ void pack_tables(BF &bf) {
             parallel_for(nr, [&](int64_t r) {
                 int64_t *c = &pos[size_t(r * NV)];
                 for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b) {
+                    prefetch_blocks(b, L);
                     const int32_t s = stage_at(b, L);
                     if (s < 0) continue;
                     for (const Block &k : bf.bfs[b].st[s].b) ++c[fwd ? k.obs_out : k.src_in];
@@ -362,6 +409,7 @@ void pack_tables(BF &bf) {
             parallel_for(nr, [&](int64_t r) {
                 int64_t *c = &pos[size_t(r * NV)];
                 for (int64_t b = rbeg(r); b < rbeg(r + 1); ++b) {
+                    prefetch_blocks(b, L);
                     const int32_t s = stage_at(b, L);
                     if (s < 0) continue;
                     const auto &blk = bf.bfs[b].st[s].b;
@@ -385,12 +433,13 @@ void pack_tables(BF &bf) {
             // items: vectors of one factorization, first-use order within it
             std::vector<int64_t> nit(nb + 1, 0), nct(nb + 1, 0);
             parallel_for(nb, [&](int64_t b) {
+                prefetch_blocks(b, L);
                 const int32_t s = stage_at(b, L);
                 if (s < 0) return;
-                thread_local KeyMap km;
+                thread_local SlotMap km;
                 const auto &blk = bf.bfs[b].st[s].b;
                 const int64_t g0 = ix.first[ix.sf[b] + s];
-                km.reset(blk.size());
+                km.reset(size_t(slot_base[b + 1] - slot_base[b]));
                 int32_t n = 0, kept = 0;
                 for (size_t q = 0; q < blk.size(); ++q) {
                     const int64_t g = g0 + int64_t(q);
@@ -401,9 +450,9 @@ void pack_tables(BF &bf) {
                         litem[g] = -1;
                         continue;
                     }
-                    auto r = km.emplace(uint64_t(uint32_t(fwd ? sl_out[g] : sl_in[g])), n);
+                    auto r = km.emplace(uint32_t(fwd ? sl_out[g] : sl_in[g]), n);
                     n += r.second;
-                    litem[g] = *r.first;
+                    litem[g] = r.first;
                     ++kept;
                 }
                 nit[b + 1] = n;
@@ -417,6 +466,7 @@ void pack_tables(BF &bf) {
             items.resize(size_t(nit[nb]));
             con.resize(size_t(nct[nb]));
             parallel_for(nb, [&](int64_t b) {
+                prefetch_blocks(b, L);
                 const int32_t s = stage_at(b, L);
                 if (s < 0) return;
                 thread_local std::vector<int32_t> cur;
@@ -484,6 +534,10 @@ void pack_tables(BF &bf) {
         S.h_items.swap(items);
         S.h_contribs.swap(con);
         t_g_merge += since(tg0);
+        if (timing_detail)
+            std::fprintf(stderr, "[bfgpu] group L=%d %s%s: %zu items, %zu contributions, %.3fs\n", L,
+                         fwd ? "forward" : "transposed", node_keyed ? " (leaf-keyed)" : "", S.h_items.size(),
+                         S.h_contribs.size(), since(tg_start));
     };
     // Split items whose contribution lists are long (observer-leaf sums over
     // many factorizations) into chunks of about CHUNK factor entries: a
