@@ -17,6 +17,10 @@
 #include <map>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
+
+#include <sys/mman.h>
+#include <sys/resource.h>
 #include <type_traits>
 #include <malloc.h>
 #include <climits>
@@ -46,6 +50,75 @@ static inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
+}
+
+namespace {
+struct BigPool {
+    std::mutex mu;
+    std::multimap<size_t, void *> free_blocks;    // mapped size -> block
+    std::unordered_map<void *, size_t> mapped;    // block -> mapped size
+    size_t cached = 0;
+    const size_t cap = [] {
+        const char *e = std::getenv("BFGPU_HEAP_KEEP");
+        return (e && std::atoi(e) == 0) ? size_t(0) : size_t(32) << 30;
+    }();
+};
+BigPool &big_pool() {
+    static BigPool *p = new BigPool;  // outlives static destructors of arrays that use it
+    return *p;
+}
+constexpr size_t HUGE_PAGE = size_t(2) << 20;
+}  // namespace
+
+void *host_big_alloc(size_t bytes) {
+    const size_t rb = (bytes + HUGE_PAGE - 1) & ~(HUGE_PAGE - 1);
+    BigPool &P = big_pool();
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        // a kept block of at least the size, at most a quarter larger
+        auto it = P.free_blocks.lower_bound(rb);
+        if (it != P.free_blocks.end() && it->first <= rb + rb / 4) {
+            void *p = it->second;
+            P.cached -= it->first;
+            P.free_blocks.erase(it);
+            return p;
+        }
+    }
+    void *raw = mmap(nullptr, rb + HUGE_PAGE, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (raw == MAP_FAILED) throw std::bad_alloc();
+    const uintptr_t a = (uintptr_t(raw) + HUGE_PAGE - 1) & ~uintptr_t(HUGE_PAGE - 1);
+    if (a > uintptr_t(raw)) munmap(raw, a - uintptr_t(raw));
+    const uintptr_t end = uintptr_t(raw) + rb + HUGE_PAGE;
+    if (end > a + rb) munmap(reinterpret_cast<void *>(a + rb), end - (a + rb));
+    void *p = reinterpret_cast<void *>(a);
+    static const bool thp = [] {
+        const char *e = std::getenv("BFGPU_HOST_THP");
+        return !(e && std::atoi(e) == 0);
+    }();
+    if (thp) madvise(p, rb, MADV_HUGEPAGE);
+    std::lock_guard<std::mutex> g(P.mu);
+    P.mapped[p] = rb;
+    return p;
+}
+
+void host_big_free(void *p, size_t bytes) {
+    (void)bytes;
+    if (!p) return;
+    BigPool &P = big_pool();
+    size_t rb = 0;
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        auto it = P.mapped.find(p);
+        if (it == P.mapped.end()) return;  // not ours: cannot happen for arrays of this allocator
+        rb = it->second;
+        if (P.cached + rb <= P.cap) {
+            P.free_blocks.emplace(rb, p);
+            P.cached += rb;
+            return;
+        }
+        P.mapped.erase(it);
+    }
+    munmap(p, rb);
 }
 
 uint64_t sample_key(uint64_t seed, int32_t no, int32_t ns, int32_t stage, int32_t s, int32_t o) {
@@ -281,22 +354,22 @@ struct Engine {
     // host lists of a level's batches, kept across levels: their capacity is
     // reused instead of hundreds of MB being unmapped and faulted in again
     // per level (unmapping stalls every worker thread)
-    std::vector<int32_t> pend, later, retry, acc;
+    hvec<int32_t> pend, later, retry, acc;
     hvec<int32_t> hrows;  // host-drawn samples (prefixes too long for the device)
     hvec<SampleTask> stask;
-    std::vector<int64_t> bigofs;
-    std::vector<int32_t> hrank, hpiv, jq, jpos;
+    hvec<int64_t> bigofs;
+    hvec<int32_t> hrank, hpiv, jq, jpos;
     hvec<FillTask> ft;
     hvec<int64_t> fe;
-    std::vector<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big, sk_run;
+    hvec<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big, sk_run;
     hvec<QrTask> qt;
     hvec<InterpTask> it;
     std::vector<float> gfeat;
     std::vector<int32_t> gpred;
     hvec<double> j_res;  // residual judge
     std::vector<uint8_t> j_fail;
-    std::vector<uint8_t> e_dbl;  // per batch: doubled, held-out rows
-    std::vector<int32_t> e_jm;
+    hvec<uint8_t> e_dbl;  // per batch: doubled, held-out rows
+    hvec<int32_t> e_jm;
 
     Engine(Ctx &c, const Tree &te, const Tree &tr, const bfgpu_opts &op)
         : ctx(c), test(te), trial(tr), o(op), prec(op.precision),
@@ -352,9 +425,9 @@ struct Engine {
             *d = DevBuf();
         Engine *e = this;
         auto drop = [](auto &v) { std::remove_reference_t<decltype(v)>().swap(v); };
-        for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->hpiv, &e->jq, &e->jpos, &e->gpred,
-                        &e->e_jm})
+        for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->hpiv, &e->jq, &e->jpos, &e->e_jm})
             drop(*v);
+        drop(gpred);
         for (auto *v : {&e->rofs, &e->cofs, &e->zo, &e->jr, &e->jz, &e->jrf, &e->jzf, &e->jcnt, &e->tofs_run,
                         &e->it_pos, &e->it_big, &e->sk_run})
             drop(*v);
@@ -373,7 +446,7 @@ struct Engine {
 
     // out[0] = 0, out[i + 1] = out[i] + v(i): chunk sums in parallel, a
     // serial pass over the chunk totals, then the chunks offset in parallel
-    template <class F> static void prefix_sum(int64_t n, std::vector<int64_t> &out, F v) {
+    template <class V, class F> static void prefix_sum(int64_t n, V &out, F v) {
         out.resize(size_t(n) + 1);
         out[0] = 0;
         constexpr int64_t CH = int64_t(1) << 15;
@@ -477,8 +550,8 @@ struct Engine {
     // max(ceil(c_tau ln(1/eps)), 10, 2 * rows) rows.  The QR tiles, pivots,
     // ranks, held-out tiles and their descriptors are still on the device:
     // only the batch indices go up and two numbers per tile come back.
-    void judge(std::vector<Comp> &cs, const std::vector<int32_t> &pend, const std::vector<uint8_t> &dbl, size_t np,
-               std::vector<int32_t> &retry) {
+    void judge(std::vector<Comp> &cs, const hvec<int32_t> &pend, const hvec<uint8_t> &dbl, size_t np,
+               hvec<int32_t> &retry) {
         auto t0 = clk::now();
         const int64_t nj = int64_t(jq.size());
         if (nj == 0) return;
@@ -583,7 +656,7 @@ struct Engine {
             const size_t np = pend.size();
             // sizes of every tile, then running offsets (parallel prefix
             // sums): main tiles, then the judge's held-out tiles
-            std::vector<int32_t> &jm = e_jm;
+            hvec<int32_t> &jm = e_jm;
             jm.resize(np);
             parallel_for(int64_t(np), [&](int64_t q) { jm[q] = judge_m(cs[pend[q]]); }, 4096);
             prefix_sum(int64_t(np), rofs, [&](int64_t q) { return int64_t(cs[pend[q]].rows); });
@@ -638,7 +711,7 @@ struct Engine {
             // row samples: drawn on the device, except prefixes longer than
             // the device's swap map holds (drawn here, copied there)
             stask.resize(np);
-            std::vector<int64_t> &bo = bigofs;
+            hvec<int64_t> &bo = bigofs;
             prefix_sum(int64_t(np), bo, [&](int64_t q) {
                 const int64_t t = int64_t(cs[pend[q]].rows) + jm[q];
                 return t > SAMPLE_DEVICE_MAX ? t : int64_t(0);
@@ -722,7 +795,7 @@ struct Engine {
             t_qr += secs(tq, ti);
             // adaptive doubling (compressors.jl:134) decided per compression
             // in parallel; the retried ones in batch order
-            std::vector<uint8_t> &dbl = e_dbl;
+            hvec<uint8_t> &dbl = e_dbl;
             dbl.assign(np, 0);
             parallel_for(int64_t(np), [&](int64_t q) {
                 Comp &c = cs[pend[q]];
@@ -870,6 +943,12 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
     }();
     (void)heap_keep;
     auto t0 = clk::now();
+    auto minflt = [] {
+        rusage u{};
+        getrusage(RUSAGE_SELF, &u);
+        return int64_t(u.ru_minflt);
+    };
+    const int64_t flt0 = minflt();
     bf.pf_k = o.kernel ? 0.0 : o.k;  // P blocks of the point kernel can be recomputed
     Ctx &ctx = *bf.ctx;
     const Tree &test = *bf.test, &trial = *bf.trial;
@@ -1258,11 +1337,11 @@ void factorize_all(BF &bf, const bfgpu_opts &o, const std::vector<std::pair<int3
                      "fill+qr %.3fs, interp %.3fs, trim %.3fs, emit %.3fs, drop %.3fs, free %.3fs, pack %.3fs "
                      "(%lld compressions, %lld retries, "
                      "%lld of rank 0, judge %.3fs (kernel and copies %.3fs): %lld judged, %lld fallbacks, "
-                     "%d threads)\n",
+                     "%d threads; %lld page faults)\n",
                      bf.t_factor_ms * 1e-3, t_setup, t_plan, E.t_guess, E.t_host, E.t_qr, E.t_interp, E.t_trim,
                      t_emit, t_drop, t_free, secs(tpk, t1),
                      (long long)E.n_comp, (long long)E.n_retry, (long long)E.n_rank0, E.t_judge, E.t_judge_gpu,
-                     (long long)E.n_judged, (long long)E.n_fallback, host_threads());
+                     (long long)E.n_judged, (long long)E.n_fallback, host_threads(), (long long)(minflt() - flt0));
 }
 
 }  // namespace bfg

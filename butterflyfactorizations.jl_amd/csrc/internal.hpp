@@ -119,13 +119,31 @@ struct OneBF {
 
 // Work item of one stage launch: an output target with a list of
 // contributions (factor blocks) summed into it.
+// Host memory for large plain arrays (HOST_BIG_MIN bytes and more): 2 MiB
+// aligned anonymous mappings advised for transparent huge pages, and freed
+// blocks kept for reuse (up to a cap), so that the temporaries of one level,
+// launch table or call do not fault fresh pages in again for the next.
+// BFGPU_HEAP_KEEP=0 returns every block to the system when it is freed.
+constexpr size_t HOST_BIG_MIN = size_t(4) << 20;
+void *host_big_alloc(size_t bytes);
+void host_big_free(void *p, size_t bytes);
+
 // Host arrays of plain records whose elements are all written after
 // allocation: resize() leaves new elements uninitialized (no serial
-// zero-fill of hundreds of MB before a parallel fill).
+// zero-fill of hundreds of MB before a parallel fill); large ones come from
+// host_big_alloc.
 template <class T> struct NoInitAlloc : std::allocator<T> {
     template <class U> struct rebind { using other = NoInitAlloc<U>; };
     NoInitAlloc() = default;
     template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+    T *allocate(size_t n) {
+        if (n * sizeof(T) >= HOST_BIG_MIN) return static_cast<T *>(host_big_alloc(n * sizeof(T)));
+        return std::allocator<T>::allocate(n);
+    }
+    void deallocate(T *p, size_t n) {
+        if (n * sizeof(T) >= HOST_BIG_MIN) host_big_free(p, n * sizeof(T));
+        else std::allocator<T>::deallocate(p, n);
+    }
     template <class U> void construct(U *p) noexcept { ::new (static_cast<void *>(p)) U; }
     template <class U, class... A> void construct(U *p, A &&...a) {
         ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);

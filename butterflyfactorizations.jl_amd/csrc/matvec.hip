@@ -20,6 +20,8 @@
 #include <string>
 #include <unordered_map>
 
+#include <sys/resource.h>
+
 #include "internal.hpp"
 
 namespace bfg {
@@ -235,6 +237,12 @@ void pack_tables(BF &bf) {
     double t_group = 0, t_split = 0, t_finish = 0, t_up = 0, t_g_items = 0, t_g_fe = 0, t_g_merge = 0;
     auto tp0 = clk::now();
     const hipStream_t ps = bf.ctx ? bf.ctx->stream : hipStream_t(nullptr);
+    auto minflt = [] {
+        rusage u{};
+        getrusage(RUSAGE_SELF, &u);
+        return int64_t(u.ru_minflt);
+    };
+    const int64_t flt0 = minflt();
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     const BlockIndex ix = index_blocks(bf);
     double t_keys = 0, t_slots = 0, t_pf = 0;
@@ -262,7 +270,11 @@ void pack_tables(BF &bf) {
     t_keys = since(tp0);
     auto tsl = clk::now();
     // ---- slot offsets (per factorization, local), reference slot ids ----
-    std::vector<int32_t> sl_out(size_t(ix.total), -1), sl_in(size_t(ix.total), -1);
+    hvec<int32_t> sl_out(static_cast<size_t>(ix.total)), sl_in(static_cast<size_t>(ix.total));
+    parallel_for(ix.total, [&](int64_t g) {
+        sl_out[size_t(g)] = -1;
+        sl_in[size_t(g)] = -1;
+    }, 1 << 16);
     std::vector<int64_t> slot_base(nb + 1, 0);
     parallel_for(nb, [&](int64_t b) {
         prefetch_all(b);
@@ -904,9 +916,9 @@ void pack_tables(BF &bf) {
         std::fprintf(stderr, "[bfgpu] pack %.3fs: index+keys %.3fs, slots %.3fs, group %.3fs (items %.3fs, entries %.3fs, "
                              "merge %.3fs), split %.3fs, "
                              "finish+upload %.3fs (upload %.3fs), matrix-free tables %.3fs (check %.3fs, columns %.3fs, "
-                             "upload %.3fs), validate %.3fs\n",
+                             "upload %.3fs), validate %.3fs; %lld page faults\n",
                      since(tp0), t_keys, t_slots, t_group, t_g_items, t_g_fe, t_g_merge, t_split, t_finish, t_up, t_pf, t_pf_check, t_pf_cols,
-                     t_pf_up, t_valid);
+                     t_pf_up, t_valid, (long long)(minflt() - flt0));
 }
 
 // Streaming-kernel tables of a launch table (variant 9): the forward tables
