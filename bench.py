@@ -122,12 +122,12 @@ def kernel_name(d, op, real):
     ink = 0 if (table == 1 or level > 0) else 1
     if variant == 6:
         return "pfree_kernel"
-    if variant == 10:  # matrix-free last level on the matrix cores
-        return f"pfree_mfma_kernel<{(nr + 15) // 16}>"
+    if variant == 10:  # matrix-free last level on the matrix cores (cs 1: FP32 emulated on bf16, bf16x3)
+        return f"pfree_mfma{'3' if cs & 1 else ''}_kernel<{(nr + 15) // 16}>"
     if variant == 9:  # streaming kernel (its other tasks, if any, run the dense kernel in the same slot)
         return f"stream_kernel<{real}, {4 if (cs >> 28) & 1 else 8}, {'true' if op == 2 else 'false'}>"
-    if variant == 5:
-        return f"mfma_rhs_kernel<{outk}, {(nr + 15) // 16}>"
+    if variant == 5:  # one launch per tile height present (RB row blocks of 16, up to cs rows)
+        return f"mfma_rhs_kernel<{outk}, {(nr + 15) // 16}, RB<={max(1, cs // 16)}>"
     if variant == 4:
         return f"rhs_kernel<{real}, {outk}, 8>"
     if variant == 3 and op != 0:

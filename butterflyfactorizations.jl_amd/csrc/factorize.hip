@@ -29,8 +29,7 @@
 
 namespace bfg {
 
-int32_t mlp_predict_ranks(const bfgpu_mlp *m, const std::vector<float> &feat, int64_t n,
-                          std::vector<int32_t> &out);
+int32_t mlp_predict_ranks(const bfgpu_mlp *m, const float *feat, int64_t n, int32_t *out);
 
 int host_threads() {
     static int n = [] {
@@ -364,8 +363,9 @@ struct Engine {
     hvec<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big, sk_run;
     hvec<QrTask> qt;
     hvec<InterpTask> it;
-    std::vector<float> gfeat;
-    std::vector<int32_t> gpred;
+    hvec<float> gfeat;
+    hvec<int32_t> gpred;
+    hvec<int64_t> e_pos;  // per compression of a level: its place among the pending ones
     hvec<double> j_res;  // residual judge
     std::vector<uint8_t> j_fail;
     hvec<uint8_t> e_dbl;  // per batch: doubled, held-out rows
@@ -428,6 +428,7 @@ struct Engine {
         for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->hpiv, &e->jq, &e->jpos, &e->e_jm})
             drop(*v);
         drop(gpred);
+        drop(e_pos);
         for (auto *v : {&e->rofs, &e->cofs, &e->zo, &e->jr, &e->jz, &e->jrf, &e->jzf, &e->jcnt, &e->tofs_run,
                         &e->it_pos, &e->it_big, &e->sk_run})
             drop(*v);
@@ -499,13 +500,14 @@ struct Engine {
     void guesses(std::vector<Comp> &cs) {
         if (o.rank_mode == BFGPU_RANK_LEARNED) {
             BFG_REQUIRE(o.mlp != nullptr, "factorize: learned ranks need an MLP");
-            std::vector<float> &f = gfeat;
+            hvec<float> &f = gfeat;
             f.resize(6 * cs.size());
             parallel_for(int64_t(cs.size()), [&](int64_t i) {
                 for (int q = 0; q < 6; ++q) f[6 * size_t(i) + q] = cs[size_t(i)].feat[q];
             }, 1 << 14);
-            std::vector<int32_t> &pr = gpred;
-            mlp_predict_ranks(o.mlp, f, int64_t(cs.size()), pr);
+            hvec<int32_t> &pr = gpred;
+            pr.resize(cs.size());
+            mlp_predict_ranks(o.mlp, f.data(), int64_t(cs.size()), pr.data());
             parallel_for(int64_t(cs.size()), [&](int64_t i) {
                 cs[size_t(i)].guess = std::max(pr[size_t(i)], 1) + o.oversample;
             }, 1 << 14);
@@ -616,19 +618,24 @@ struct Engine {
         }, 1 << 14);
         upload_n(d_cols, colpool, size_t(colpool_len), ctx.stream);
         guesses(cs);
-        pend.clear();
-        for (size_t i = 0; i < cs.size(); ++i) {
-            Comp &c = cs[i];
-            ++n_comp;
-            if (c.n_obs == 0 || c.n_cols == 0) {
+        // the compressions with observers and columns, in level order (the
+        // others have rank 0 and no factor entries)
+        const int64_t ncs = int64_t(cs.size());
+        auto live = [&](int64_t i) { return cs[size_t(i)].n_obs > 0 && cs[size_t(i)].n_cols > 0; };
+        prefix_sum(ncs, e_pos, [&](int64_t i) { return int64_t(live(i)); });
+        pend.resize(size_t(e_pos[size_t(ncs)]));
+        parallel_for(ncs, [&](int64_t i) {
+            Comp &c = cs[size_t(i)];
+            if (!live(i)) {
                 c.rank = 0;
                 c.rows = 0;
                 c.tofs = pool_used;
-                continue;
+                return;
             }
             c.rows = std::min(std::max(c.guess, 10), c.n_obs);
-            pend.push_back(int32_t(i));
-        }
+            pend[size_t(e_pos[size_t(i)])] = int32_t(i);
+        }, 1 << 14);
+        n_comp += ncs;
         t_guess += secs(tg, clk::now());
         // kernel-matrix tiles of one batch: at most 32 GiB or 30 % of the
         // free device memory (besides what the tile buffer already holds)

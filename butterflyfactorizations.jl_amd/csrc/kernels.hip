@@ -1830,20 +1830,23 @@ __global__ __launch_bounds__(256) void denset_kernel(
 // observer leaf and the skeleton points of its P columns, so the level is
 // computed from the points instead of read: one workgroup per observer
 // leaf, the four wavefronts on quarters of its flat column list, a lane per
-// column, rows in chunks of 16 (8 for a leaf's last few rows) accumulated
-// per lane (row data are uniform loads).  Coordinates are local to the leaf centre c: rows hold
-// u = x - c and |u|^2, columns d = y - c and the phase of |d| in
-// revolutions (from FP64), so the distance enters as
+// column, rows in chunks of 16 (8 or 4 for a leaf's last few rows)
+// accumulated per lane (row data are uniform loads).  Coordinates are local
+// to the leaf centre c: rows hold u = x - c and |u|^2, columns d = y - c and
+// the phase of |d| in revolutions (from FP64), so the distance enters as
 //   r = |d| + delta,  delta = (|u|^2 - 2 d.u) / (r + |d|),
 // and the phase as frac(phase(|d|) + k delta / (2 pi)): its FP32 error is
 // relative to the leaf's size, not to the distance, and does not grow with
-// the electrical size of the geometry.  1 / (4 pi r) is the reciprocal
-// square root of r^2 (1 / (4 pi) folded into the column's input value);
+// the electrical size of the geometry.  r^2 = |d|^2 + (|u|^2 - 2 d.u) reuses
+// the numerator (one add per entry instead of three differences and a
+// square norm); 1 / (4 pi r) is the reciprocal square root of r^2
+// (1 / (4 pi) folded into the column's input value); the entry is VALU
+// bound (about 14 full-rate and 4 transcendental instructions);
 // sine and cosine are the hardware ones on revolutions.  The 16 row sums
 // leave the 64 lanes by a halving exchange and the wavefronts meet in LDS,
 // in fixed order.
 // ---------------------------------------------------------------------------
-// One chunk of NR rows (16, or 8 for a leaf's last few rows) of an item.
+// One chunk of NR rows (16; 8 or 4 for a leaf's last few rows) of an item.
 template <int NR>
 __device__ __forceinline__ void pfree_rows(const int4 it, int r0, int cb, int ce, int lane, int w,
                                            const float4 *__restrict__ cdat, const int32_t *__restrict__ colin,
@@ -1852,7 +1855,7 @@ __device__ __forceinline__ void pfree_rows(const int4 it, int r0, int cb, int ce
                                            const int32_t *__restrict__ perm_out, C<float> alpha, C<float> beta,
                                            int fuse_beta) {
     constexpr float inv4pi = 0.07957747154594767f;
-    constexpr int NS = NR == 16 ? 4 : 3;  // exchange steps: 2^NS = NR rows
+    constexpr int NS = NR == 16 ? 4 : NR == 8 ? 3 : 2;  // exchange steps: 2^NS = NR rows
     const int nrow = min(NR, it.y - r0);
     float ux[NR], uy[NR], uz[NR], uu[NR];
 #pragma unroll
@@ -1878,10 +1881,9 @@ __device__ __forceinline__ void pfree_rows(const int4 it, int r0, int cb, int ce
         const float mx = -2.f * d.x, my = -2.f * d.y, mz = -2.f * d.z;
 #pragma unroll
         for (int rr = 0; rr < NR; ++rr) {
-            const float ex = d.x - ux[rr], ey = d.y - uy[rr], ez = d.z - uz[rr];
-            const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-            const float rs = __builtin_amdgcn_rsqf(r2);
             const float num = fmaf(mx, ux[rr], fmaf(my, uy[rr], fmaf(mz, uz[rr], uu[rr])));
+            const float r2 = D2 + num;  // |d - u|^2 = |d|^2 - 2 d.u + |u|^2
+            const float rs = __builtin_amdgcn_rsqf(r2);
             const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
             const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
             const float cs = __builtin_amdgcn_cosf(rev), sn = __builtin_amdgcn_sinf(rev);
@@ -1934,11 +1936,15 @@ __global__ __launch_bounds__(256) void pfree_kernel(const int4 *__restrict__ ite
     const int ncol = it.w - it.z;
     const int q = (((ncol + 3) >> 2) + 63) & ~63;
     const int cb = it.z + w * q, ce = min(it.w, cb + q);
-    for (int r0 = 0; r0 < it.y;) {  // chunks of 16 rows; at most 8 left: one chunk of 8
+    for (int r0 = 0; r0 < it.y;) {  // chunks of 16 rows; at most 8 / 4 left: one chunk of 8 / 4
         if (it.y - r0 > 8) {
             pfree_rows<16>(it, r0, cb, ce, lane, w, cdat, colin, rdat, slots, kr, red, y, perm_out, alpha, beta,
                            fuse_beta);
             r0 += 16;
+        } else if (it.y - r0 <= 4) {
+            pfree_rows<4>(it, r0, cb, ce, lane, w, cdat, colin, rdat, slots, kr, red, y, perm_out, alpha, beta,
+                          fuse_beta);
+            r0 += 4;
         } else {
             pfree_rows<8>(it, r0, cb, ce, lane, w, cdat, colin, rdat, slots, kr, red, y, perm_out, alpha, beta,
                           fuse_beta);
@@ -2001,10 +2007,9 @@ __global__ __launch_bounds__(256) void pfree_t_kernel(const int4 *__restrict__ i
             for (int r = 0; r < nr; ++r) {
                 const float4 u = su[r];
                 const C<float> xv = sx[r];
-                const float ex = d.x - u.x, ey = d.y - u.y, ez = d.z - u.z;
-                const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                const float rs = __builtin_amdgcn_rsqf(r2);
                 const float num = fmaf(mx, u.x, fmaf(my, u.y, fmaf(mz, u.z, u.w)));
+                const float r2 = D2 + num;  // |d - u|^2 = |d|^2 - 2 d.u + |u|^2
+                const float rs = __builtin_amdgcn_rsqf(r2);
                 const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
                 const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
                 const float cs = __builtin_amdgcn_cosf(rev), sn = sgn * __builtin_amdgcn_sinf(rev);
@@ -2457,10 +2462,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 4
 #pragma unroll
             for (int rb = 0; rb < 4; ++rb) {
                 if (rb >= nrb) break;
-                const float ex = d.x - ux[rb], ey = d.y - uy[rb], ez = d.z - uz[rb];
-                const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                const float rs = __builtin_amdgcn_rsqf(r2);
                 const float num = fmaf(mx, ux[rb], fmaf(my, uy[rb], fmaf(mz, uz[rb], uu[rb])));
+                const float r2 = D2 + num;  // |d - u|^2 = |d|^2 - 2 d.u + |u|^2
+                const float rs = __builtin_amdgcn_rsqf(r2);
                 const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
                 const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
                 const float amp = cok && rok[rb] ? rs * inv4pi : 0.f;
@@ -2599,11 +2603,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 3
                     const float4 d = dd[u];
                     const float D2 = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
                     const float D = D2 * __builtin_amdgcn_rsqf(D2);
-                    const float ex = d.x - ux[rb], ey = d.y - uy[rb], ez = d.z - uz[rb];
-                    const float r2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-                    const float rs = __builtin_amdgcn_rsqf(r2);
                     const float num =
                         fmaf(-2.f * d.x, ux[rb], fmaf(-2.f * d.y, uy[rb], fmaf(-2.f * d.z, uz[rb], uu[rb])));
+                    const float r2 = D2 + num;  // |d - u|^2 = |d|^2 - 2 d.u + |u|^2
+                    const float rs = __builtin_amdgcn_rsqf(r2);
                     const float delta = num * __builtin_amdgcn_rcpf(fmaf(r2, rs, D));
                     const float rev = __builtin_amdgcn_fractf(fmaf(delta, kr, d.w));
                     const float amp = okc[u] && rok[rb] ? rs * inv4pi : 0.f;

@@ -1568,7 +1568,7 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
                 if (mfma && L == nL - 1 && bf.pfree_ok && bf.use_pfree && bf.pf_order_built) {
                     // last level's entries from the points, on the matrix cores
                     launch_pfree_mfma(s, bf, slots, ns, yc, ldy, nc, alpha, b_eff, fuse, bf.ctx->n_cu);
-                    if (tl) tl->launched(s, {L, 0, 10, OUT_SCATTER, 0, nc});
+                    if (tl) tl->launched(s, {L, 0, 10, OUT_SCATTER, bf.mfma_emul ? 1 : 0, nc});
                     continue;
                 }
                 const StageTable *pre = t.pre.get();
@@ -1582,7 +1582,7 @@ static void stage_launches(BF &bf, int op, int64_t nrhs, const void *x, int64_t 
                         launch_rhs(s, bf.precision, *u, bf.factors[L].p, slots, ns, shift, yc, ldy, pt, nc,
                                    alpha, b_eff, fuse, bf.ctx->n_cu);
                     }
-                    if (tl) tl->launched(s, {L, u == &t ? 0 : 1, mfma ? 5 : 4, u->out_kind, 0, nc});
+                    if (tl) tl->launched(s, {L, u == &t ? 0 : 1, mfma ? 5 : 4, u->out_kind, mfma ? bf.mfma_rows : 0, nc});
                 }
             }
         }

@@ -381,13 +381,11 @@ static void forward_chunked(bfgpu_mlp &m, int64_t n, const float *features, floa
     }
 }
 
-int32_t mlp_predict_ranks(const bfgpu_mlp *mc, const std::vector<float> &feat, int64_t n,
-                          std::vector<int32_t> &out) {
+int32_t mlp_predict_ranks(const bfgpu_mlp *mc, const float *feat, int64_t n, int32_t *out) {
     bfgpu_mlp &m = const_cast<bfgpu_mlp &>(*mc);
-    out.resize(size_t(n));
     if (n == 0) return 0;
-    std::vector<float> y(static_cast<size_t>(n));
-    forward_chunked(m, n, feat.data(), y.data());
+    hvec<float> y(static_cast<size_t>(n));
+    forward_chunked(m, n, feat, y.data());
     parallel_for(n, [&](int64_t i) {
         const float rk = y[size_t(i)] * m.out_scale + m.out_shift;
         out[size_t(i)] = std::max(1, int32_t(std::ceil(rk)));

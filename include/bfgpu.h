@@ -377,8 +377,10 @@ int bfgpu_matvec_timed(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, in
  * n_launch receives the count, the first `cap` launches are written.
  * desc (6 int32 per launch): level, table (0 level table, 1 partial sums,
  * 2 input permutation, 3 fused sweep), kernel variant, output kind (0
- * slot, 1 rows of y), packed launch shape (as BFGPU_OPT_TILE), right-hand
- * sides per lane. */
+ * slot, 1 rows of y), packed launch shape (as BFGPU_OPT_TILE; matrix-core
+ * tables, variant 5: the most rows of their tiles; the matrix-free last
+ * level on the matrix cores, variant 10: 1 when FP32 is emulated on bf16),
+ * right-hand sides per lane. */
 int bfgpu_matvec_launch_times(bfgpu_bf *bf, int32_t op, int64_t nrhs, const void *x, int64_t ldx,
                               void *y, int64_t ldy, int32_t iters, int32_t cap, int32_t *n_launch,
                               double *launch_ms, int32_t *desc);
