@@ -2168,6 +2168,27 @@ typedef float bfg_v4f __attribute__((ext_vector_type(4)));
 constexpr int mfma_waves(int NT, int RB) {
     return RB == 1 ? (NT <= 2 ? 6 : NT == 3 ? 5 : 4) : (NT * RB <= 3 ? 4 : 2);
 }
+// Right-hand side of group g in lane column li of the matrix-core kernel:
+// groups pair up, lane li holding RHS 32 h + 2 li and 32 h + 2 li + 1 of pair
+// h (one 16-byte load or store per pair, 256 contiguous bytes per 16
+// lanes; skeleton-vector rows are 16-byte aligned, ns a multiple of 16
+// entries); an odd last group holds RHS 32 h + li.
+template <int NT>
+__device__ __forceinline__ int rhs_col(int g, int li) {
+    return (g < (NT & ~1)) ? 32 * (g >> 1) + 2 * li + (g & 1) : 32 * (NT >> 1) + li;
+}
+// The lane's NT right-hand sides of one entry row p (zero when !ok).
+template <int NT>
+__device__ __forceinline__ void rhs_load(C<float> *b, const C<float> *p, int li, bool ok) {
+#pragma unroll
+    for (int h = 0; h < NT / 2; ++h) {
+        const float4 v = ok ? *reinterpret_cast<const float4 *>(p + 32 * h + 2 * li) : float4{0.f, 0.f, 0.f, 0.f};
+        b[2 * h] = C<float>{v.x, v.y};
+        b[2 * h + 1] = C<float>{v.z, v.w};
+    }
+    if (NT & 1) b[NT - 1] = ok ? p[32 * (NT >> 1) + li] : C<float>{0.f, 0.f};
+}
+
 template <int OUTK, int NT, int RB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(NT, RB)))) void mfma_rhs_kernel(
     const int4 *__restrict__ tiles, int32_t n_tiles, const MvTaskC *__restrict__ mtasks,
@@ -2257,7 +2278,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
                         if (r < tl.w) {
 #pragma unroll
                             for (int g = 0; g < NT; ++g) {
-                                const C<float> xv = in[int64_t(ci + tl.z + r) * ns + 16 * g + li];
+                                const C<float> xv = in[int64_t(ci + tl.z + r) * ns + rhs_col<NT>(g, li)];
                                 accr[rb][g][v] += xv.x;
                                 acci[rb][g][v] += xv.y;
                             }
@@ -2271,23 +2292,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
                     ap[rb] = F + (tl.y == 1 ? cf + tl.z + 16 * rb + li : rbase[rb]);
                     ald[rb] = tl.y == 1 ? cm : rld[rb];
                 }
-                const C<float> *xin = in + int64_t(ci) * ns + li;
+                const C<float> *xin = in + int64_t(ci) * ns;
                 // step k0 multiplies entries k0..k0+3 while k0+4..k0+7 load
                 C<float> a[RB], b[NT];
 #pragma unroll
                 for (int rb = 0; rb < RB; ++rb)
                     a[rb] = (rok[rb] && lk < cn) ? ap[rb][int64_t(lk) * ald[rb]] : C<float>{0.f, 0.f};
-#pragma unroll
-                for (int g = 0; g < NT; ++g) b[g] = lk < cn ? xin[int64_t(lk) * ns + 16 * g] : C<float>{0.f, 0.f};
+                rhs_load<NT>(b, xin + int64_t(lk) * ns, li, lk < cn);
                 for (int k0 = 0; k0 < cn; k0 += 4) {
                     const int kn = k0 + 4 + lk;
                     C<float> an[RB], bn[NT];
 #pragma unroll
                     for (int rb = 0; rb < RB; ++rb)
                         an[rb] = (rok[rb] && kn < cn) ? ap[rb][int64_t(kn) * ald[rb]] : C<float>{0.f, 0.f};
-#pragma unroll
-                    for (int g = 0; g < NT; ++g)
-                        bn[g] = kn < cn ? xin[int64_t(kn) * ns + 16 * g] : C<float>{0.f, 0.f};
+                    rhs_load<NT>(bn, xin + int64_t(kn) * ns, li, kn < cn);
 #pragma unroll
                     for (int rb = 0; rb < RB; ++rb) {
                         const float nai = -a[rb].y;
@@ -2315,7 +2333,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
             cn = nn;
             ci = ni;
         }
-        // lane li + 16 lk holds rows 16 rb + 4 lk + v of column li (+ 16 g)
+        // lane li + 16 lk holds rows 16 rb + 4 lk + v of right-hand sides
+        // rhs_col(g, li)
 #pragma unroll
         for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
@@ -2323,9 +2342,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mfma_waves(
                 const int r = 4 * lk + v;
                 const int orow = __shfl(rout[rb], r, 64);
                 if (__shfl(int(rok[rb]), r, 64) == 0) continue;
+                if (OUTK == OUT_SLOT) {  // pairs of RHS: one 16-byte store each
+#pragma unroll
+                    for (int h = 0; h < NT / 2; ++h) {
+                        const int j = 32 * h + 2 * li;
+                        if (j + 1 < nc) {
+                            *reinterpret_cast<float4 *>(slots + int64_t(orow) * ns + j) =
+                                float4{accr[rb][2 * h][v], acci[rb][2 * h][v], accr[rb][2 * h + 1][v],
+                                       acci[rb][2 * h + 1][v]};
+                        } else if (j < nc) {
+                            slots[int64_t(orow) * ns + j] = C<float>{accr[rb][2 * h][v], acci[rb][2 * h][v]};
+                        }
+                    }
+                    if (NT & 1) {
+                        const int j = 32 * (NT >> 1) + li;
+                        if (j < nc) slots[int64_t(orow) * ns + j] = C<float>{accr[rb][NT - 1][v], acci[rb][NT - 1][v]};
+                    }
+                    continue;
+                }
 #pragma unroll
                 for (int g = 0; g < NT; ++g) {
-                    const int j = 16 * g + li;
+                    const int j = rhs_col<NT>(g, li);
                     if (j >= nc) continue;
                     const C<float> val = C<float>{accr[rb][g][v], acci[rb][g][v]};
                     if (OUTK == OUT_SLOT) {
