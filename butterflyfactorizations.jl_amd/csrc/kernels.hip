@@ -2149,7 +2149,9 @@ __global__ __launch_bounds__(256) void rhs_kernel(
 // blocks fill the rows of the product; a tile holds RB row blocks of 16
 // (RB = 1..4, up to 64 rows), which share every input load.  Per step of 4 input entries the
 // lanes load A[i][k] (lane i + 16 k: row i of the tile, entry k) and, per
-// group of 16 right-hand sides, B[k][j] (lane j + 16 k), and issue
+// group g of 16 right-hand sides, B[k][j] (lane j + 16 k; column j of group
+// g is right-hand side rhs_col(g, j): groups pair up so that a lane's two
+// right-hand sides are adjacent, one 16-byte load), and issue
 // v_mfma_f32_16x16x4_f32 four times per group (re*re, -im*im into the real
 // part, re*im, im*re into the imaginary part); lane j + 16 q then holds
 // rows 4q..4q+3 of column j.  Padding rows and entries load zeros; lanes of
@@ -2478,9 +2480,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 4
             ok = c < it.w;
             const int cc = min(c, it.w - 1);
             d = cdat[cc];
-            const int64_t e = int64_t(colin[cc]) * ns + li;
-#pragma unroll
-            for (int g = 0; g < NT; ++g) b[g] = slots[e + 16 * g];
+            rhs_load<NT>(b, slots + int64_t(colin[cc]) * ns, li, true);
         };
         float4 d;
         C<float> b[NT];
@@ -2535,7 +2535,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 4
                 const int64_t yr = perm_out[it.x + row];
 #pragma unroll
                 for (int g = 0; g < NT; ++g) {
-                    const int j = 16 * g + li;
+                    const int j = rhs_col<NT>(g, li);
                     if (j >= nc) continue;
                     C<float> *yp = y + int64_t(j) * ldy + yr;
                     const C<float> old = *yp;
@@ -2616,10 +2616,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 3
                 okc[u] = c < it.w;
                 const int cc = min(c, it.w - 1);
                 dd[u] = cdat[cc];
-                const int64_t e = int64_t(colin[cc]) * ns + li;
+                C<float> vv[NT];
+                rhs_load<NT>(vv, slots + int64_t(colin[cc]) * ns, li, true);
 #pragma unroll
                 for (int g = 0; g < NT; ++g) {
-                    const C<float> v = slots[e + 16 * g];
+                    const C<float> v = vv[g];
                     short h, lo;
                     bf16_split(v.x, h, lo);
                     brh[g][u] = h;
@@ -2685,7 +2686,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NT <= 2 ? 3
                 const int64_t yr = perm_out[it.x + row];
 #pragma unroll
                 for (int g = 0; g < NT; ++g) {
-                    const int j = 16 * g + li;
+                    const int j = rhs_col<NT>(g, li);
                     if (j >= nc) continue;
                     C<float> *yp = y + int64_t(j) * ldy + yr;
                     const C<float> old = *yp;
