@@ -1467,6 +1467,8 @@ __global__ __launch_bounds__(256) void dense_kernel(
 // odd-height block reads one entry past it (the region has a one-word pad)
 // and is dropped.
 // ---------------------------------------------------------------------------
+typedef unsigned int bfg_u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename R>
 __device__ __forceinline__ C<R> lds_c(const C<R> *p) {  // one entry, 8-byte aligned
     using V = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
@@ -1551,7 +1553,12 @@ __global__ __launch_bounds__(128) void stream_kernel(const StreamChunk *__restri
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int wi = q * 64 + lane;
-            if (wi < r.nw) V[q] = FW[r.f0w + wi];
+            // factor words are read once: non-temporal, so that they do not
+            // evict the input vectors other chunks read again
+            if (wi < r.nw) {
+                const bfg_u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const bfg_u32x4 *>(FW) + r.f0w + wi);
+                V[q] = make_uint4(v.x, v.y, v.z, v.w);
+            }
         }
         int before = 0;  // input vectors starting in earlier rounds
 #pragma unroll
