@@ -16,6 +16,7 @@
 #include <exception>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <thread>
 #include <unordered_map>
 
@@ -343,6 +344,7 @@ struct Engine {
     DevBuf d_test_pts, d_trial_pts;
     DevBuf d_rows, d_cols, d_fill, d_fill_end, d_qr, d_interp, d_piv, d_rank, d_Z;
     DevBuf d_test_perm, d_stask, d_big;  // row samples drawn on the device
+    DevBuf d_skofs, d_skel;               // skeletons gathered on the device
     // residual judge: tasks and results
     DevBuf d_jq, d_jres;
     int64_t n_comp = 0, n_retry = 0, n_rank0 = 0, n_judged = 0, n_fallback = 0;
@@ -357,7 +359,7 @@ struct Engine {
     hvec<int32_t> hrows;  // host-drawn samples (prefixes too long for the device)
     hvec<SampleTask> stask;
     hvec<int64_t> bigofs;
-    hvec<int32_t> hrank, hpiv, jq, jpos;
+    hvec<int32_t> hrank, jq, jpos;
     hvec<FillTask> ft;
     hvec<int64_t> fe;
     hvec<int64_t> rofs, cofs, zo, jr, jz, jrf, jzf, jcnt, tofs_run, it_pos, it_big, sk_run;
@@ -368,6 +370,7 @@ struct Engine {
     hvec<int64_t> e_pos;  // per compression of a level: its place among the pending ones
     hvec<double> j_res;  // residual judge
     std::vector<uint8_t> j_fail;
+    hvec<int32_t> e_skofs;  // per batch: skeleton offsets
     hvec<uint8_t> e_dbl;  // per batch: doubled, held-out rows
     hvec<int32_t> e_jm;
 
@@ -420,12 +423,12 @@ struct Engine {
 
     void release() {
         for (DevBuf *d : {&d_rows, &d_cols, &d_fill, &d_fill_end, &d_qr, &d_interp, &d_piv, &d_rank, &d_Z,
-                          &d_test_perm, &d_stask, &d_big,
+                          &d_test_perm, &d_stask, &d_big, &d_skofs, &d_skel,
                           &d_jq, &d_jres})
             *d = DevBuf();
         Engine *e = this;
         auto drop = [](auto &v) { std::remove_reference_t<decltype(v)>().swap(v); };
-        for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->hpiv, &e->jq, &e->jpos, &e->e_jm})
+        for (auto *v : {&e->pend, &e->later, &e->retry, &e->acc, &e->hrank, &e->jq, &e->jpos, &e->e_jm, &e->e_skofs})
             drop(*v);
         drop(gpred);
         drop(e_pos);
@@ -795,9 +798,7 @@ struct Engine {
                 // the judge reads the ranks in batch order
                 BFG_HIP(hipMemcpyAsync(d_rank.p, hrank.data(), sizeof(int32_t) * np, hipMemcpyHostToDevice, ctx.stream));
             }
-            hpiv.resize(std::max(pofs, 1));
-            BFG_HIP(hipMemcpyAsync(hpiv.data(), d_piv.p, sizeof(int32_t) * size_t(pofs), hipMemcpyDeviceToHost, ctx.stream));
-            BFG_HIP(hipStreamSynchronize(ctx.stream));
+            BFG_HIP(hipStreamSynchronize(ctx.stream));  // ranks on the host
             auto ti = clk::now();
             t_qr += secs(tq, ti);
             // adaptive doubling (compressors.jl:134) decided per compression
@@ -869,15 +870,26 @@ struct Engine {
             // rows take theirs from the later batch
             auto done = [&](int64_t q) { return cs[pend[q]].rows == ft[q].m; };
             prefix_sum(int64_t(np), sk_run, [&](int64_t q) { return done(q) ? int64_t(cs[pend[q]].rank) : int64_t(0); });
+            // gathered on the device (column ids of the pivot prefixes), only
+            // they come back
             const int64_t sk0 = int64_t(skp.size());
+            BFG_REQUIRE(sk_run[np] < (int64_t(1) << 31), "factorize: skeletons of one batch exceed 2^31 entries");
             skp.resize(size_t(sk0 + sk_run[np]));
-            parallel_for(int64_t(np), [&](int64_t q) {
-                if (!done(q)) return;
-                Comp &c = cs[pend[q]];
-                c.skel_ofs = sk0 + sk_run[q];
-                int32_t *o = skp.data() + c.skel_ofs;
-                for (int32_t i = 0; i < c.rank; ++i) o[i] = c.cols[hpiv[qt[q].pofs + i]];
-            }, 4096);
+            hvec<int32_t> &so = e_skofs;
+            so.resize(np + 1);
+            parallel_for(int64_t(np) + 1, [&](int64_t q) {
+                so[size_t(q)] = int32_t(sk_run[size_t(q)]);
+                if (q < int64_t(np) && done(q)) cs[pend[q]].skel_ofs = sk0 + sk_run[size_t(q)];
+            }, 1 << 14);
+            if (sk_run[np] > 0) {
+                upload(d_skofs, so, ctx.stream);
+                need(d_skel, sizeof(int32_t) * size_t(sk_run[np]));
+                launch_skeleton_gather(ctx.stream, d_qr.as<QrTask>(), d_fill.as<FillTask>(), d_cols.as<int32_t>(),
+                                       d_piv.as<int32_t>(), d_skofs.as<int32_t>(), int32_t(np), d_skel.as<int32_t>());
+                BFG_HIP(hipMemcpyAsync(skp.data() + sk0, d_skel.p, sizeof(int32_t) * size_t(sk_run[np]),
+                                       hipMemcpyDeviceToHost, ctx.stream));
+                BFG_HIP(hipStreamSynchronize(ctx.stream));
+            }
             if (!it.empty()) {
                 ensure_pool(pool, pool_cap, pool_used + FACTOR_PAD);
                 upload(d_interp, it, ctx.stream);

@@ -482,6 +482,11 @@ struct QrTask {
     int32_t pofs;       // pivot array offset (n entries)
     int32_t pad;
 };
+// Skeletons of a batch (kernels.hip): for compression q, its first
+// skofs[q + 1] - skofs[q] pivots mapped to column ids,
+// out[skofs[q] + i] = colpool[fill[q].cofs + piv[qr[q].pofs + i]].
+void launch_skeleton_gather(hipStream_t s, const QrTask *qr, const FillTask *fill, const int32_t *colpool,
+                            const int32_t *piv, const int32_t *skofs, int32_t n, int32_t *out);
 // Blocks the wavefront QR kernel takes (the rest: a workgroup per block).
 bool cpqr_wave_ok(int prec, int32_t m, int32_t n);
 void launch_cpqr(hipStream_t s, int prec, void *Z, const QrTask *tasks, int32_t n_tasks,

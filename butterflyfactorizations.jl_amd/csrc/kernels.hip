@@ -2994,6 +2994,32 @@ void launch_dense(hipStream_t s, int prec, const StageTable &t, const void *fact
 }
 
 // ---------------------------------------------------------------------------
+// Skeletons of a batch: a thread per compression maps its pivot prefix to
+// column ids.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void skeleton_gather_kernel(const QrTask *__restrict__ qr,
+                                                              const FillTask *__restrict__ fill,
+                                                              const int32_t *__restrict__ colpool,
+                                                              const int32_t *__restrict__ piv,
+                                                              const int32_t *__restrict__ skofs, int32_t n,
+                                                              int32_t *__restrict__ out) {
+    for (int32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int32_t o = skofs[q], cnt = skofs[q + 1] - o;
+        const int32_t *p = piv + qr[q].pofs;
+        const int32_t *cols = colpool + fill[q].cofs;
+        for (int32_t i = 0; i < cnt; ++i) out[int64_t(o) + i] = cols[p[i]];
+    }
+}
+
+void launch_skeleton_gather(hipStream_t s, const QrTask *qr, const FillTask *fill, const int32_t *colpool,
+                            const int32_t *piv, const int32_t *skofs, int32_t n, int32_t *out) {
+    if (n <= 0) return;
+    const unsigned grid = unsigned(std::min<int64_t>((int64_t(n) + 255) / 256, 8192));
+    hipLaunchKernelGGL(skeleton_gather_kernel, dim3(grid), dim3(256), 0, s, qr, fill, colpool, piv, skofs, n, out);
+    BFG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Launch-table tasks from items and contributions: task i is item i with its
 // first contribution inline (the host keeps the same table for its own
 // passes; only items and contributions cross the bus).
