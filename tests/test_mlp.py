@@ -32,6 +32,46 @@ def test_oracle_gradient_matches_finite_differences():
         assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(g[i]))
 
 
+def test_oracle_matches_torch_linear_relu_autograd():
+    """The oracle's layer order, ReLU placement, standardisation and loss
+    against an independent torch model: nn.Linear layers loaded from the
+    flat parameter vector (W_l row-major out x in, then b_l), ReLU between
+    layers, MSE against (t - shift) / scale, gradients by autograd (FP64)."""
+    torch = pytest.importorskip("torch")
+    d, p, X, t, mu, sigma = _setup(seed=3, n=257, hidden=12, layers=4)
+    shift, scale = 20.0, 10.0
+    layers, o = [], 0
+    for l in range(len(d) - 1):
+        lin = torch.nn.Linear(d[l], d[l + 1]).double()
+        with torch.no_grad():
+            lin.weight.copy_(torch.from_numpy(p[o:o + d[l] * d[l + 1]].reshape(d[l + 1], d[l])))
+            o += d[l] * d[l + 1]
+            lin.bias.copy_(torch.from_numpy(p[o:o + d[l + 1]]))
+            o += d[l + 1]
+        layers.append(lin)
+    assert o == len(p)
+    mods = []
+    for l, lin in enumerate(layers):
+        mods.append(lin)
+        if l < len(layers) - 1:
+            mods.append(torch.nn.ReLU())
+    net = torch.nn.Sequential(*mods)
+    xt = (torch.from_numpy(X) - torch.from_numpy(mu)) / torch.from_numpy(sigma)
+    yt = net(xt)[:, 0]
+    target = (torch.from_numpy(t) - shift) / scale
+    loss_t = torch.mean((yt - target) ** 2)
+    loss_t.backward()
+    g_t = torch.cat([torch.cat([lin.weight.grad.reshape(-1), lin.bias.grad]) for lin in layers]).numpy()
+    y_o, _ = M.forward(p, d, X, mu, sigma)
+    loss_o, g_o = M.loss_grad(p, d, X, t, mu, sigma, shift, scale)
+    np.testing.assert_allclose(y_o, yt.detach().numpy(), rtol=1e-12, atol=1e-12)
+    assert abs(loss_o - loss_t.item()) < 1e-12 * max(1.0, loss_o)
+    np.testing.assert_allclose(g_o, g_t, rtol=1e-10, atol=1e-12)
+    # a plausible slip fails: ReLU after the last layer as well
+    bad = torch.nn.Sequential(*(mods + [torch.nn.ReLU()]))
+    assert not np.allclose(y_o, bad(xt)[:, 0].detach().numpy(), rtol=1e-6, atol=1e-9)
+
+
 @pytest.mark.gpu
 def test_gpu_mlp_forward_backward_match_oracle(bfgpu):
     print("[case] mlp forward/backward vs oracle", flush=True)
