@@ -719,12 +719,17 @@ struct Engine {
                         "factorize: level too large for 32-bit index lists");
             const int32_t pofs = int32_t(cofs[np]);
             // row samples: drawn on the device, except prefixes longer than
-            // the device's swap map holds (drawn here, copied there)
+            // the device's swap map holds (drawn here, copied there);
+            // BFGPU_SAMPLE_DEVICE_MAX (read per batch) lowers the limit, so
+            // that tests can send every prefix through the host's draw
+            int64_t dev_max = SAMPLE_DEVICE_MAX;
+            if (const char *e = std::getenv("BFGPU_SAMPLE_DEVICE_MAX"))
+                dev_max = std::max<int64_t>(0, std::min<int64_t>(SAMPLE_DEVICE_MAX, std::atoll(e)));
             stask.resize(np);
             hvec<int64_t> &bo = bigofs;
             prefix_sum(int64_t(np), bo, [&](int64_t q) {
                 const int64_t t = int64_t(cs[pend[q]].rows) + jm[q];
-                return t > SAMPLE_DEVICE_MAX ? t : int64_t(0);
+                return t > dev_max ? t : int64_t(0);
             });
             BFG_REQUIRE(bo[np] < (int64_t(1) << 31), "factorize: host-drawn samples exceed 2^31 entries");
             hrows.resize(size_t(std::max<int64_t>(bo[np], 1)));

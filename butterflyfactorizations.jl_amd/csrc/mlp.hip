@@ -342,7 +342,8 @@ static bool fused_infer_ok(const bfgpu_mlp &m) {
 constexpr int64_t INFER_CHUNK = int64_t(1) << 20;
 static void forward_chunked(bfgpu_mlp &m, int64_t n, const float *features, float *y) {
     hipStream_t s = m.ctx->stream;
-    if (fused_infer_ok(m) && !std::getenv("BFGPU_MLP_GEMM")) {
+    const char *gemm_env = std::getenv("BFGPU_MLP_GEMM");  // 1: layer-by-layer GEMMs (for comparisons)
+    if (fused_infer_ok(m) && !(gemm_env && std::atoi(gemm_env) != 0)) {
         // fused kernel: no activation workspace, chunks of 4M samples
         const int64_t ch = std::min(n, int64_t(4) * INFER_CHUNK);
         DevBuf dx, dy;

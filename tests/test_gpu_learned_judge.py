@@ -96,6 +96,23 @@ def test_learned_fp64_records_match_oracle(bfgpu, oracle):
     assert e < 10 * TOL
 
 
+def test_learned_host_drawn_samples_match_oracle(bfgpu, oracle, monkeypatch):
+    """The judged compressions' samples (sampled rows, then the held-out
+    rows of the same prefix) drawn on the host for every prefix longer than
+    16 entries: records, verdicts and factors still equal the oracle's."""
+    from bfgpu import learned
+    net = learned.load(MODEL)
+    n = 4096
+    pts = fibonacci_sphere(n, 8.0 * np.sqrt(n / 262144))
+    monkeypatch.setenv("BFGPU_SAMPLE_DEVICE_MAX", "16")
+    say("learned fp64 sphere", n, "host-drawn samples above 16 entries")
+    op, B = _far(bfgpu, pts, bfgpu.LearnedRankEstimator(net), np.complex128)
+    monkeypatch.delenv("BFGPU_SAMPLE_DEVICE_MAX")
+    assert op.info().n_judged > 0
+    nc, nf = _compare_sample(op, B, oracle, 120, seed=3)
+    say("records compared", nc, "fallbacks among them", nf)
+
+
 def test_forced_underprediction_falls_back_to_plain_rrqr(bfgpu, oracle):
     from bfgpu import learned
     net = bfgpu.RankPredictor(seed=3)

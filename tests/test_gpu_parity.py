@@ -114,6 +114,23 @@ def test_gpu_factorization_fp64_matches_oracle(bfgpu, oracle, ctx, n, tol):
     assert rel(y, A @ x) < 10 * tol
 
 
+def test_host_drawn_row_samples_match_oracle(bfgpu, oracle, ctx, monkeypatch):
+    """Row samples longer than the device's swap map are drawn on the host
+    and copied to the device: with the limit lowered to 0 (every prefix)
+    and to 12 (longer ones only), the records, skeletons and factors equal
+    the oracle's as with samples drawn on the GPU."""
+    n = 2048
+    obs, src = two_squares(n, seed=5)
+    bt = trees(bfgpu, obs, src)
+    obf = oracle.assemble(bt.testcluster, bt.trialcluster, 0, 0, K, 1e-4)
+    for lim in ("0", "12"):
+        say("factorize fp64 two squares", n, "host-drawn samples above", lim)
+        monkeypatch.setenv("BFGPU_SAMPLE_DEVICE_MAX", lim)
+        g = bfgpu.assemble_BF(bfgpu.HelmholtzKernelMatrix(K), bt, 0, 0, K, 1e-4, ctx=ctx)
+        _compare_factorizations(g.export(), obf.export())
+    monkeypatch.delenv("BFGPU_SAMPLE_DEVICE_MAX")
+
+
 @pytest.mark.parametrize("case", ["deep_obs", "shallow_obs", "identity"])
 def test_gpu_factorization_unbalanced(bfgpu, oracle, ctx, case):
     say("factorize fp64 unbalanced", case)
