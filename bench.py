@@ -253,7 +253,7 @@ def main():
     if rank == "learned":
         from bfgpu import learned
         est = bfgpu.LearnedRankEstimator(learned.load(MODEL, ctx=ctx))
-    # the two forms of the north-star comparison, factorized in turn (the
+    # the two forms compared (FP32 learned or plain vs FP64 plain), factorized in turn (the
     # first large factorization of a process also pays for first-touch host
     # memory, and the host-side emission time alternates between
     # factorizations of one process, so each form runs `reps` times, the
@@ -440,7 +440,7 @@ def main():
         "fp32_adjoint_ms": round(ms32_adj, 4), "fp32_32rhs_ms": round(ms32_32, 4),
         "factorize_s": {"fp32": round(t_f32, 2), "fp64": None if t_f64 is None else round(t_f64, 2),
                         "autotune_fp32": round(t_tune, 2)},
-        "north_star": {
+        "fp32_vs_fp64_plain": {
             "forms": {f: {"factorize": best[f], "runs_s": [r["s"] for r in runs[f]]} for f in runs},
             "fp32_form": f"fp32_{rank}",
             "factorize_speedup_vs_fp64_plain": None if t_f64 is None else round(t_f64 / t_f32, 3),
