@@ -16,7 +16,7 @@ both factorization times with their parts, and both errors against dense
 FP64 far-field rows are reported beside it.  `--workload squares` selects
 the single-block two-squares case (65,536 points per square, plain ranks).
 
-Prints one JSON line.  --gpus N > 1 is rejected: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 is rejected: the library runs on one GPU.
 """
 import argparse
 import json
@@ -232,7 +232,7 @@ def main():
                     help="also run a plain 1 GiB device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the library runs on one GPU only (asked for --gpus {args.gpus}); "
               "the sharding of the far field over GPUs is described but not built", file=sys.stderr)
         sys.exit(2)
 

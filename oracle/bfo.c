@@ -2,7 +2,7 @@
  * bfo.c -- CPU FP64 oracle for the butterfly far-field path.
  *
  * TEST INFRASTRUCTURE ONLY: the tests and the benchmark's CPU-baseline leg
- * use this file as the checker.  The product library never links or calls it.
+ * use this file as the reference the GPU results are compared with.  The product library never links or calls it.
  *
  * Each function cites the reference ButterflyFactorizations.jl file:line it
  * restates.  Parts the reference delegates to third-party code are restated

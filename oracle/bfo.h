@@ -5,7 +5,7 @@
  * ButterflyFactorizations.jl algorithm (kernel fill, rank estimation, partial
  * pivoted QR compression, the level-by-level factorization constructor, the
  * flat-workspace matvec and the adjoint/transpose), used by the tests and by
- * the CPU-baseline leg of the benchmark as the checker.  Nothing in the
+ * the CPU-baseline leg of the benchmark, as the reference results.  Nothing in the
  * product library links, loads or calls it.
  *
  * Conventions: indices are 0-based int32; complex numbers are interleaved

@@ -4,7 +4,7 @@ BEAST plug-in computes for far interactions (ext/BFBEAST/kernelmatrix.jl:
 numerical quadrature DoubleNumQStrat(2, 3), read here as triangle rules of
 polynomial degree 2 on test and 3 on trial triangles).
 
-TEST INFRASTRUCTURE ONLY (the checker of the GPU plug-in); the product never
+TEST INFRASTRUCTURE ONLY (the reference results for the GPU plug-in); the product never
 imports it.  BEAST itself is absent, so its exact rule tables cannot be
 compared: the rules below are the published Strang-Fix / Dunavant (1985)
 ones, pinned by exactness on monomials; the entries are pinned by symmetry,

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU FP64 oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
-by the CPU-baseline leg of bench.py, always as the checker.  The product
+by the CPU-baseline leg of bench.py, always as the reference results.  The product
 package never imports it.
 """
 import ctypes as C

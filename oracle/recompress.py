@@ -2,7 +2,7 @@
 (src/Butterflyalgebra/algrecomp.jl:19-148), over factorizations in the
 exported reference layout.
 
-TEST INFRASTRUCTURE ONLY (the checker of the GPU recompression); the
+TEST INFRASTRUCTURE ONLY (the reference results for the GPU recompression); the
 product never imports it.
 
 A factorization is a dict as ``export()`` returns it: ``table`` rows
